@@ -1,0 +1,131 @@
+// Shared helpers for the gfx950 kernels of libsmx_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+
+#include "../../include/smx_hip.h"
+
+namespace smx {
+
+constexpr int kWave = 64;
+
+// Last error message of the calling host thread (read with smx_last_error()).
+void set_error(const char* fmt, ...);
+
+#define SMX_HIP_CHECK(expr)                                                   \
+  do {                                                                        \
+    hipError_t _e = (expr);                                                   \
+    if (_e != hipSuccess) {                                                   \
+      ::smx::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), \
+                       __FILE__, __LINE__);                                   \
+      return SMX_ERR_HIP;                                                     \
+    }                                                                         \
+  } while (0)
+
+#define SMX_LAUNCH_CHECK()                                                    \
+  do {                                                                        \
+    hipError_t _e = hipGetLastError();                                        \
+    if (_e != hipSuccess) {                                                   \
+      ::smx::set_error("kernel launch failed: %s (%s:%d)",                    \
+                       hipGetErrorString(_e), __FILE__, __LINE__);            \
+      return SMX_ERR_HIP;                                                     \
+    }                                                                         \
+  } while (0)
+
+#define SMX_REQUIRE(cond, code, ...)                                          \
+  do {                                                                        \
+    if (!(cond)) {                                                            \
+      ::smx::set_error(__VA_ARGS__);                                          \
+      return (code);                                                          \
+    }                                                                         \
+  } while (0)
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+__device__ __forceinline__ float wave_sum_f(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// Deterministic block sum of NV doubles per thread; result valid in thread 0.
+// `scratch` must hold NV * (blockDim.x / 64) doubles.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* scratch) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wid = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) v[q] = wave_sum(v[q]);
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) scratch[q * nw + wid] = v[q];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      double s = 0.0;
+      for (int w = 0; w < nw; ++w) s += scratch[q * nw + w];
+      v[q] = s;
+    }
+  }
+  __syncthreads();
+}
+
+// Grid-wide deterministic reduction ("last block" pattern).  `red` layout:
+// red[0] reinterpreted as the arrival counter (must be 0 on entry; the last
+// block resets it), partials of quantity q of block b at red[8 + q*G + b].
+// Every block calls this once with its per-thread values; it returns true in
+// the block that arrives last, whose thread 0 then holds the grid totals in
+// `v`.  Totals are summed in a fixed order independent of arrival order.
+template <int NV>
+__device__ __forceinline__ bool grid_sum_last(double (&v)[NV], double* red, double* scratch) {
+  __shared__ bool s_last;
+  block_sum<NV>(v, scratch);
+  const int G = gridDim.x;
+  double* part = red + 8;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) part[q * G + blockIdx.x] = v[q];
+    __threadfence();
+    const unsigned t = atomicAdd(reinterpret_cast<unsigned*>(red), 1u);
+    s_last = (t == (unsigned)(G - 1));
+  }
+  __syncthreads();
+  if (!s_last) return false;
+  __threadfence();
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < G; b += blockDim.x) {
+      s += __hip_atomic_load(part + q * G + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    v[q] = s;
+  }
+  block_sum<NV>(v, scratch);
+  if (threadIdx.x == 0) {
+    __hip_atomic_store(reinterpret_cast<unsigned*>(red), 0u, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+  return true;
+}
+
+// Doubles of reduction workspace needed for NV quantities over G blocks.
+inline int64_t red_ws_doubles(int64_t G, int NV) { return 8 + (int64_t)NV * G; }
+
+inline int grid_for(int64_t n, int block) {
+  int64_t g = (n + block - 1) / block;
+  return (int)(g < 1 ? 1 : g);
+}
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+}  // namespace smx

@@ -1,0 +1,355 @@
+"""Fredkin-Koehler (FEM/BEM) demagnetizing field with the Lindholm double layer.
+
+Drop-in counterpart of ``DemagFieldLindholmGPU``
+(gpu/fields/Demag_Lindholm_GPU.py:82-724): same constructor keywords, same
+``compute_vec(m_vec, out_vec)`` / ``Energy_lumped_gpu`` / memory-report surface,
+with torch device tensors in place of PETSc CUDA Vecs.  One evaluation is
+
+    b   = Div m,  b -= mean(b)                   (Div SpMV + fixed-order sum)
+    u1  : A u1 = b  (Jacobi CG, Neumann)          (one kernel per iteration)
+    g|G = B (u1|G - mean(u1))                    (dense GEMV or H-matrix)
+    u2  : A_bc u2 = lift(g)                        (Jacobi CG, Dirichlet)
+    H   = G (u1 - mean(u1) + u2) / V                (node-graph gradient)
+
+exactly the order of _solve_u1_gpu / _apply_B_gpu / _solve_u2_gpu / compute_vec
+(:569-670).  The mean removal is the nullspace projection of the single
+connected component (ns_u1.remove).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+
+import numpy as np
+import scipy.sparse as sp
+from scipy.sparse.csgraph import connected_components
+import torch
+
+from . import _capi
+from .boundary import BoundaryGeometry, build_boundary_geometry
+from .device import DeviceSell, default_device, reduce_workspace, to_dev
+from .fem import FEMSystem, assemble_fem, compact_csr, to_sell
+from .mesh import TetMesh
+
+MU0 = 4.0 * math.pi * 1e-7
+
+_B_DTYPES = {
+    "float64": torch.float64, "fp64": torch.float64, "f64": torch.float64,
+    "float32": torch.float32, "fp32": torch.float32, "f32": torch.float32,
+    "bfloat16": torch.bfloat16, "bf16": torch.bfloat16,
+}
+
+
+def _torch_dtype(B_dtype):
+    if isinstance(B_dtype, torch.dtype):
+        return B_dtype
+    if isinstance(B_dtype, str):
+        key = B_dtype.lower()
+    else:
+        key = np.dtype(B_dtype).name
+    if key not in _B_DTYPES:
+        raise TypeError(f"unsupported B_dtype {B_dtype!r}")
+    return _B_DTYPES[key]
+
+
+class OperatorSet:
+    """Device copies of the FEM operators one mesh needs (shared between the
+    demag field and the effective field)."""
+
+    def __init__(self, mesh: TetMesh, device=None, fem: FEMSystem | None = None):
+        self.device = device or default_device()
+        self.mesh = mesh
+        self.fem = fem or assemble_fem(mesh.X, mesh.tets)
+        f = self.fem
+        g = f.graph
+        self.N = f.N
+        self.graph_sell = to_sell(g.indptr, g.indices, np.zeros(g.nnz), f.N)
+        self.graph = DeviceSell.upload(self.graph_sell, self.device)
+        self._sell_pos = None
+        self.inv_vol = to_dev(1.0 / f.volume, self.device)
+        self.vol = to_dev(f.volume, self.device)
+
+    def graph_values(self, vals: np.ndarray, k: int) -> DeviceSell:
+        """Upload graph-ordered values (nnz, *k) in the node-graph SELL layout."""
+        from .device import sell_values
+        sv = sell_values(self.fem.graph.indptr, vals, self.N)
+        return self.graph.with_vals(sv, k)
+
+    def compact(self, vals: np.ndarray) -> DeviceSell:
+        g = self.fem.graph
+        ip, ix, vv = compact_csr(g.indptr, g.indices, vals)
+        return DeviceSell.upload(to_sell(ip, ix, vv, self.N), self.device)
+
+
+class DemagFieldLindholmGPU:
+    """FEM/BEM demag field on one MI355X (see module docstring)."""
+
+    def __init__(self, mesh, V=None, V1=None, Ms=8e5, VolN=None, serial_mesh_path=None,
+                 cache_dir=None, cache_prefix="Bcache_lindholm_ownedrows", cache_tag=None,
+                 map_tol=1e-10, ksp_rtol_u1=1e-6, ksp_rtol_u2=1e-6, incident_policy="skip",
+                 volume_scale=1e27, B_dtype=np.float64, boundary_backend="dense",
+                 dense_gpu_max_bytes=512 * 1024 ** 2, hmatrix_cache_path=None,
+                 hmatrix_cache_dir="./hmatrix_cache", hmatrix_epsilon=1e-5, hmatrix_eta=2.0,
+                 hmatrix_leaf_size=64, hmatrix_compressor="fullaca", hmatrix_max_rank=None,
+                 hmatrix_max_temporary_block_bytes=256 * 1024 ** 2, hmatrix_force_rebuild=False,
+                 hmatrix_cache_compressed=False, hmatrix_build_backend="gpu",
+                 hmatrix_build_device_id=0, hmatrix_build_threads_per_block=128,
+                 hmatrix_gpu_entry_kernel="auto", hmatrix_gpu_fast_math=False,
+                 hmatrix_gpu_index_cache_max_bytes=128 * 1024 ** 2,
+                 hmatrix_gpu_memory_fraction=0.75, hmatrix_gpu_fallback_to_cpu=True,
+                 hmatrix_gpu_aca_residual_check_interval=1, hmatrix_gpu_rsvd_initial_rank=16,
+                 hmatrix_gpu_rsvd_oversampling=8, hmatrix_gpu_rsvd_power_iterations=1,
+                 hmatrix_gpu_progress_interval_seconds=5.0, hmatrix_threads_per_block=128,
+                 hmatrix_use_fp32=True, hmatrix_direct_gpu_pack=False,
+                 hmatrix_release_build_memory=True, hmatrix_verbose=False,
+                 *, ksp_atol=1e-12, ksp_max_it_u1=2000, ksp_max_it_u2=12000,
+                 warm_start=False, device=None, operators: OperatorSet | None = None,
+                 geometry: BoundaryGeometry | None = None, verbose=False):
+        if not isinstance(mesh, TetMesh):
+            raise TypeError("mesh must be a smx.mesh.TetMesh")
+        self.mesh = mesh
+        self.Ms = float(Ms)
+        self.mu0 = MU0
+        self.volume_scale = float(volume_scale)
+        self.ops = operators or OperatorSet(mesh, device)
+        self.device = self.ops.device
+        dev = self.device
+        fem = self.ops.fem
+        self.N = fem.N
+        self.local_dofs = self.N
+        self.local_size = 3 * self.N
+        self.last_its_u1 = 1
+        self.last_its_u2 = 1
+        self.ksp_rtol_u1 = float(ksp_rtol_u1)
+        self.ksp_rtol_u2 = float(ksp_rtol_u2)
+        self.ksp_atol = float(ksp_atol)
+        self.ksp_max_it_u1 = int(ksp_max_it_u1)
+        self.ksp_max_it_u2 = int(ksp_max_it_u2)
+        self.warm_start = bool(warm_start)
+        self.verbose = bool(verbose)
+        self.converged = True
+
+        if VolN is not None:
+            vol = np.asarray(VolN, dtype=np.float64).reshape(-1, 3)[:, 0]
+            self.vol_scalar_gpu = to_dev(vol, dev)
+        else:
+            self.vol_scalar_gpu = self.ops.vol
+        self.inv_vol_scalar_gpu = 1.0 / self.vol_scalar_gpu
+
+        # single connected component: nullspace = constants (ns_u1)
+        g = fem.graph
+        r = np.repeat(np.arange(self.N), np.diff(g.indptr))
+        adj = sp.csr_matrix((np.ones(g.nnz, dtype=np.int8), (r, g.indices)), shape=(self.N, self.N))
+        ncomp, _ = connected_components(adj, directed=False)
+        if ncomp != 1:
+            raise NotImplementedError(
+                f"mesh has {ncomp} connected components; the device path projects one constant nullspace")
+
+        # ---------------- FEM operators ----------------
+        self.Div = self.ops.graph_values(self.Ms * fem.div, 3)
+        self.G = self.ops.graph_values(fem.grad, 3)
+        self.A_u1 = self.ops.compact(fem.stiff)
+        self.A_u2 = self.ops.compact(fem.stiffness_bc())
+        d1 = self._diag(fem.stiff)
+        d2 = self._diag(fem.stiffness_bc())
+        self.dinv_u1 = to_dev(1.0 / d1, dev)
+        self.dinv_u2 = to_dev(1.0 / d2, dev)
+
+        # ---------------- boundary operator ----------------
+        self.opB = geometry or build_boundary_geometry(mesh.X, mesh.tets, incident_policy)
+        geo = self.opB
+        self.Nb = geo.Nb
+        self.bverts_gpu = to_dev(geo.bverts.astype(np.int32), dev)
+        self.bpos_gpu = to_dev(geo.bpos.astype(np.int32), dev)
+        self.diag_jump_gpu = to_dev(geo.diag_jump, dev)
+        self.B_dtype = _torch_dtype(B_dtype)
+
+        requested = str(boundary_backend).strip().lower()
+        if requested not in {"dense", "hmatrix", "auto"}:
+            raise ValueError("boundary_backend must be 'dense', 'hmatrix', or 'auto'.")
+        dense_bytes = self.Nb * self.Nb * torch.tensor([], dtype=self.B_dtype).element_size()
+        if requested == "auto":
+            requested = "dense" if dense_bytes <= int(dense_gpu_max_bytes) else "hmatrix"
+        self.boundary_backend = requested
+        self.B_dense_gpu = None
+        self.B_hmatrix_gpu = None
+        self.B_hmatrix_cpu = None
+        self.hmatrix_cache_path = None
+        t0 = time.perf_counter()
+        if requested == "dense":
+            self.B_dense_gpu, self.B_ld = assemble_dense_B(geo, dev, self.B_dtype)
+        else:
+            from .hmatrix import build_device_hmatrix
+            self.B_hmatrix_cpu, self.B_hmatrix_gpu = build_device_hmatrix(
+                geo, dev, epsilon=float(hmatrix_epsilon), eta=float(hmatrix_eta),
+                leaf_size=int(hmatrix_leaf_size), max_rank=hmatrix_max_rank,
+                max_temporary_block_bytes=int(hmatrix_max_temporary_block_bytes),
+                use_fp32=bool(hmatrix_use_fp32), cache_path=hmatrix_cache_path,
+                force_rebuild=bool(hmatrix_force_rebuild), verbose=bool(hmatrix_verbose))
+        torch.cuda.synchronize(dev)
+        self.setup_boundary_seconds = time.perf_counter() - t0
+
+        # ---------------- per-evaluation buffers ----------------
+        N = self.N
+        z = lambda n: torch.zeros(n, dtype=torch.float64, device=dev)
+        L = _capi.lib()
+        self.b_u1 = z(N)
+        self.u1 = z(N)
+        self.u2 = z(N)
+        self.xb = z(self.Nb)
+        self.y_rows = z(self.Nb)
+        self.red = reduce_workspace(3 * N + 64, dev)
+        self.b_sum = z(1)
+        self.ws1 = z(int(L.smx_pcg_ws_size(N)))
+        self.ws2 = z(int(L.smx_pcg_ws_size(N)))
+        self._have_u = False
+        self._energy_out = z(1)
+
+    # ------------------------------------------------------------------
+    def _diag(self, vals):
+        g = self.ops.fem.graph
+        r = np.repeat(np.arange(self.N), np.diff(g.indptr))
+        d = np.zeros(self.N)
+        m = r == g.indices
+        d[r[m]] = vals[m]
+        return d
+
+    @property
+    def u1_sum_ptr(self) -> int:
+        """Device address of sum(u1) left by the u1 solve (work[8])."""
+        return self.ws1.data_ptr() + 8 * 8
+
+    def _pcg(self, mode, A: DeviceSell, dinv, x, ws, rtol, max_it, b=None, b_sum=None,
+             gb=None, expected=0, warm=False):
+        L = _capi.lib()
+        its = C.c_int32(0)
+        rn = C.c_double(0.0)
+        sums = (C.c_double * 1)()
+        raw = self.A_u1
+        rc = L.smx_pcg_jacobi(self.N, _capi.ptr(A.slice_ptr), _capi.ptr(A.cols), _capi.ptr(A.vals),
+                              _capi.ptr(dinv), mode, _capi.ptr(b), _capi.ptr(b_sum), 1,
+                              _capi.ptr(raw.slice_ptr), _capi.ptr(raw.cols), _capi.ptr(raw.vals),
+                              _capi.ptr(self.bpos_gpu), _capi.ptr(gb), _capi.ptr(x), int(warm),
+                              float(rtol), float(self.ksp_atol), int(max_it), 16, int(expected),
+                              _capi.ptr(ws), C.byref(its), C.byref(rn), sums,
+                              _capi.stream_ptr())
+        if rc == _capi.SMX_ERR_NOT_CONVERGED:
+            # The reference never checks the KSP converged reason; keep going.
+            self.converged = False
+        else:
+            _capi.check(rc, "smx_pcg_jacobi")
+        return int(its.value)
+
+    def boundary_matvec(self, src: torch.Tensor, y: torch.Tensor, gather=None, shift_ptr=None,
+                        shift_scale=0.0):
+        """y = B x with x = src[gather] - (*shift)*shift_scale (dense or H-matrix)."""
+        L = _capi.lib()
+        if self.boundary_backend == "dense":
+            _capi.check(L.smx_dense_bmatvec(_capi.ptr(self.B_dense_gpu),
+                                            _capi.DTYPE_CODE[self.B_dense_gpu.dtype], self.Nb,
+                                            self.B_ld, _capi.ptr(src), _capi.ptr(gather),
+                                            shift_ptr, float(shift_scale), _capi.ptr(self.xb),
+                                            _capi.ptr(y), _capi.stream_ptr()), "smx_dense_bmatvec")
+        else:
+            self.B_hmatrix_gpu.matvec_from(src, y, gather=gather, shift_ptr=shift_ptr,
+                                           shift_scale=shift_scale)
+
+    def solve_potential(self, m_vec: torch.Tensor):
+        """u1, u2 for magnetization m (3N); leaves sum(u1) at u1_sum_ptr."""
+        _capi.require_device(m_vec, "m_vec", numel=self.local_size)
+        warm = self.warm_start and self._have_u
+        self.Div.spmv(m_vec, self.b_u1, kind=1, sum_out=self.b_sum, red=self.red)
+        self.last_its_u1 = self._pcg(0, self.A_u1, self.dinv_u1, self.u1, self.ws1,
+                                     self.ksp_rtol_u1, self.ksp_max_it_u1, b=self.b_u1,
+                                     b_sum=self.b_sum, expected=self.last_its_u1 if warm else 0,
+                                     warm=warm)
+        self.boundary_matvec(self.u1, self.y_rows, gather=self.bverts_gpu,
+                             shift_ptr=self.u1_sum_ptr, shift_scale=1.0 / self.N)
+        self.last_its_u2 = self._pcg(1, self.A_u2, self.dinv_u2, self.u2, self.ws2,
+                                     self.ksp_rtol_u2, self.ksp_max_it_u2, gb=self.y_rows,
+                                     expected=self.last_its_u2 if warm else 0, warm=warm)
+        self._have_u = True
+
+    def field_args(self, m_vec, out_h_demag=None) -> _capi.FieldArgs:
+        a = _capi.FieldArgs()
+        a.n = self.N
+        a.slice_ptr = _capi.ptr(self.G.slice_ptr)
+        a.cols = _capi.ptr(self.G.cols)
+        a.grad_vals = _capi.ptr(self.G.vals)
+        a.u1 = _capi.ptr(self.u1)
+        a.u2 = _capi.ptr(self.u2)
+        a.u1_sum = self.u1_sum_ptr
+        a.inv_vol = _capi.ptr(self.inv_vol_scalar_gpu)
+        a.m = _capi.ptr(m_vec)
+        a.h_demag = _capi.ptr(out_h_demag)
+        return a
+
+    def compute_vec(self, m_vec: torch.Tensor, out_vec: torch.Tensor):
+        """H_demag(m) into out_vec (Demag_Lindholm_GPU.py:643-670)."""
+        _capi.require_device(out_vec, "out_vec", numel=self.local_size)
+        self.solve_potential(m_vec)
+        a = self.field_args(m_vec, out_vec)
+        _capi.check(_capi.lib().smx_field_llg_rhs(C.byref(a), _capi.stream_ptr()), "smx_field_llg_rhs")
+
+    def Energy_lumped_gpu(self, m_vec: torch.Tensor, h_vec: torch.Tensor) -> float:
+        """E = -1/2 mu0 Ms sum_i V_i m_i.H_i / volume_scale (:706-724)."""
+        L = _capi.lib()
+        _capi.check(L.smx_dot3_weighted(self.N, _capi.ptr(m_vec), _capi.ptr(h_vec),
+                                        _capi.ptr(self.vol_scalar_gpu), _capi.ptr(self._energy_out),
+                                        _capi.ptr(self.red), _capi.stream_ptr()), "smx_dot3_weighted")
+        return -0.5 * self.mu0 * self.Ms * float(self._energy_out.item()) / self.volume_scale
+
+    def boundary_memory_report(self) -> dict:
+        if self.boundary_backend == "dense":
+            nbytes = int(self.B_dense_gpu.numel() * self.B_dense_gpu.element_size())
+            return {"backend": "dense", "Nb": self.Nb, "dtype": str(self.B_dense_gpu.dtype),
+                    "accounted_gpu_bytes": nbytes, "accounted_gpu_mib": nbytes / 1024 ** 2}
+        rep = self.B_hmatrix_gpu.memory_report()
+        rep["Nb"] = self.Nb
+        rep["cache_path"] = self.hmatrix_cache_path
+        return rep
+
+    def print_boundary_memory_report(self):
+        rep = self.boundary_memory_report()
+        print(f"[Demag Lindholm GPU] backend={rep['backend']}, Nb={rep['Nb']}, "
+              f"GPU memory={rep.get('accounted_gpu_mib', 0.0):.3f} MiB", flush=True)
+
+
+def assemble_dense_B(geo: BoundaryGeometry, device, dtype=torch.float64):
+    """Dense Lindholm B on the device (diag = omega/4pi - 1), row-major with a
+    leading dimension padded to 8 entries (16-byte rows for every dtype)."""
+    L = _capi.lib()
+    Nb, Nt = geo.Nb, geo.Nt
+    ld = (Nb + 7) // 8 * 8
+    B64 = torch.zeros((Nb, ld), dtype=torch.float64, device=device)
+    Xb = to_dev(geo.Xb, device)
+    geom = to_dev(geo.tri_table(), device)
+    lid = to_dev(geo.tri_lid.astype(np.int32), device)
+    rows = torch.arange(Nb, dtype=torch.int32, device=device)
+    inc = geo.incidence()
+    st = _capi.stream_ptr()
+    T = _capi.LINDHOLM_TILE_COLS
+    keep = []
+    for c0 in range(0, Nb, T):
+        cols = np.arange(c0, min(c0 + T, Nb))
+        rel, rel_cols = geo.relevant_triangles(cols, inc)
+        rel_d = to_dev(rel, device)
+        rc_d = to_dev(rel_cols.reshape(-1), device)
+        keep.append((rel_d, rc_d))
+        _capi.check(L.smx_lindholm_fill(_capi.ptr(Xb), _capi.ptr(rows), Nb, _capi.ptr(geom), Nt,
+                                        _capi.ptr(lid), _capi.ptr(rel_d), _capi.ptr(rc_d),
+                                        int(rel.size), int(cols.size), B64.data_ptr() + 8 * c0,
+                                        ld, 1, int(geo.incident_policy), st), "smx_lindholm_fill")
+    diag = to_dev(geo.diag_jump, device)
+    if dtype == torch.float64:
+        _capi.check(L.smx_dense_finish(B64.data_ptr(), Nb, ld, _capi.ptr(diag), None, 0, st),
+                    "smx_dense_finish")
+        torch.cuda.synchronize(device)
+        return B64, ld
+    lo = torch.zeros((Nb, ld), dtype=dtype, device=device)
+    _capi.check(L.smx_dense_finish(B64.data_ptr(), Nb, ld, _capi.ptr(diag), lo.data_ptr(),
+                                   _capi.DTYPE_CODE[dtype], st), "smx_dense_finish")
+    torch.cuda.synchronize(device)
+    del B64
+    return lo, ld

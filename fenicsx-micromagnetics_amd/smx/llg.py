@@ -1,0 +1,402 @@
+"""Effective field, LLG right-hand side / Jacobian-vector product and the
+explicit Dormand-Prince driver.
+
+``EffectiveField`` keeps the call surface of the reference's GPU class
+(gpu/solvers/llg_module_GPU.py:387-1408): ``llg_rhs_vec``, ``compute_H_eff_vec``,
+``apply_linear_field_vec``, ``update_jac_state``, ``jac_vec_times_vec``,
+``rhs_function``, ``set_m_from_cpu``, ``set_uniform_field`` and the lumped
+energies.  The hot call ``llg_rhs_vec`` is: demag potentials (two CG solves and
+the boundary matvec) followed by ONE fused kernel that gathers m and the
+potential over the node graph and writes H_eff and the RHS.
+
+``LLG_GPU`` provides the explicit RK harness (relax_explicit with the PETSc
+TS rk/5dp adaptive settings of :1724-1787, plus a fixed-step mode).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+
+import numpy as np
+import torch
+
+from . import _capi
+from .demag import DemagFieldLindholmGPU, OperatorSet
+from .device import default_device, reduce_workspace, to_dev
+from .fem import dmi_bulk_values, dmi_interfacial_values, exchange_values
+from .mesh import TetMesh
+
+MU0 = 4.0 * math.pi * 1e-7
+
+
+class EffectiveField:
+    def __init__(self, mesh: TetMesh, Ms, Aex, Ku=0.0, n_ani_vec=None, D_bulk=0.0, D_int=0.0,
+                 n0_int_vec=None, Kc1=0.0, u1_cub=None, u2_cub=None, gamma=2.211e5, alpha=0.5,
+                 do_precess=1, use_demag=False, demag_method="lindholm", demag_kwargs=None,
+                 H0_static=None, H_time_func=None, *, device=None,
+                 operators: OperatorSet | None = None):
+        if abs(float(Kc1)) > 0.0:
+            raise NotImplementedError("cubic anisotropy is not part of this path")
+        self.mesh = mesh
+        self.device = device or default_device()
+        self.ops = operators or OperatorSet(mesh, self.device)
+        fem = self.ops.fem
+        self.N = fem.N
+        self.local_dofs = self.N
+        self.local_size = 3 * self.N
+        self.Ms = float(Ms)
+        self.gamma = float(gamma)
+        self.alpha = float(alpha)
+        self.A = float(Aex)
+        self.Ku = float(Ku)
+        self.D_bulk = float(D_bulk)
+        self.D_int = float(D_int)
+        self.do_precess = float(do_precess)
+        self.use_demag = bool(use_demag)
+        self.H_time_func = H_time_func
+        self.current_time = 0.0
+        self.mu0 = MU0
+        self.prefactorEQ = -self.gamma / (1.0 + self.alpha ** 2)
+        self.Stab = self.Ms * self.gamma / (1.0 + self.alpha ** 2) * 0.5
+        self.volume_scale_energy = 1e-27
+        dev = self.device
+        X, tets = mesh.X, mesh.tets
+
+        # exchange + DMI share the node-graph pattern (K_lin, :598-616)
+        self.ex = None
+        self.dmi = None
+        if abs(self.A) > 0.0:
+            self.ex = self.ops.graph_values(exchange_values(fem, self.A, self.Ms), 1)
+        dmi = None
+        if abs(self.D_int) > 0.0:
+            n0 = np.asarray(n0_int_vec if n0_int_vec is not None else (0.0, 0.0, 1.0), dtype=np.float64)
+            dmi = dmi_interfacial_values(X, tets, fem, self.D_int, self.Ms, n0)
+        if abs(self.D_bulk) > 0.0:
+            db = dmi_bulk_values(X, tets, fem, self.D_bulk, self.Ms)
+            dmi = db if dmi is None else dmi + db
+        if dmi is not None:
+            self.dmi = self.ops.graph_values(dmi.reshape(-1, 9), 9)
+
+        # uniaxial anisotropy: nodal axis normalized per node (Anisotropy_GPU.py:123-130)
+        self.ani_pref = 0.0
+        self.ani_axis_gpu = None
+        self.ani_axis_u = (0.0, 0.0, 0.0)
+        if abs(self.Ku) > 0.0:
+            if n_ani_vec is None:
+                raise ValueError("n_ani_vec is required when Ku != 0")
+            na = np.asarray(n_ani_vec, dtype=np.float64).reshape(-1, 3)
+            na = na / np.maximum(np.linalg.norm(na, axis=1), 1e-300)[:, None]
+            self.ani_pref = 2.0 * self.Ku / (MU0 * self.Ms)
+            if na.shape[0] == 1:
+                self.ani_axis_u = tuple(float(v) for v in na[0])
+            else:
+                if na.shape[0] != self.N:
+                    raise ValueError("n_ani_vec must hold one axis or one axis per node")
+                self.ani_axis_gpu = to_dev(na.ravel(), dev)
+
+        if not (self.ex or self.dmi or self.ani_pref or self.use_demag):
+            raise ValueError("There are no active GPU terms. You must add exchange, anisotropy, DMI, or demag.")
+
+        z = lambda n: torch.zeros(n, dtype=torch.float64, device=dev)
+        self.m_gpu = z(self.local_size)
+        self.dmdt_gpu = z(self.local_size)
+        self.H_eff_gpu = z(self.local_size)
+        self.H_demag_gpu = z(self.local_size) if self.use_demag else None
+        self.Hm_gpu = z(self.local_size)
+        self.Hv_gpu = z(self.local_size)
+        self.M_state_gpu = z(self.local_size)
+        self.H0_gpu = None
+        self.H0_u = (0.0, 0.0, 0.0)
+        if H0_static is not None:
+            h0 = np.asarray(H0_static, dtype=np.float64).ravel()
+            if h0.size == 3:
+                self.H0_u = tuple(float(v) for v in h0)
+            else:
+                self.H0_gpu = to_dev(h0, dev)
+        self.Ht_gpu = None
+        self.red = reduce_workspace(3 * self.N + 64, dev)
+        self._scalar = z(1)
+        self.LLGSteps = 0
+        self.JacSteps = 0
+
+        self.demag_field = None
+        if self.use_demag:
+            method = str(demag_method).lower()
+            if method not in ("lindholm", "lindholm_gpu"):
+                raise NotImplementedError("Unsupported demag_method. Use 'lindholm' or 'lindholm_gpu'.")
+            kw = dict(demag_kwargs or {})
+            self.demag_field = DemagFieldLindholmGPU(mesh, None, None, self.Ms, None,
+                                                     operators=self.ops, **kw)
+
+    # ------------------------------------------------------------------
+    def _args(self, m_vec, h_eff=None, rhs=None, with_demag=True, with_h0=True):
+        a = _capi.FieldArgs()
+        a.n = self.N
+        a.slice_ptr = _capi.ptr(self.ops.graph.slice_ptr)
+        a.cols = _capi.ptr(self.ops.graph.cols)
+        a.ex_vals = _capi.ptr(self.ex.vals) if self.ex is not None else None
+        a.dmi_vals = _capi.ptr(self.dmi.vals) if self.dmi is not None else None
+        if with_demag and self.demag_field is not None:
+            d = self.demag_field
+            a.grad_vals = _capi.ptr(d.G.vals)
+            a.u1 = _capi.ptr(d.u1)
+            a.u2 = _capi.ptr(d.u2)
+            a.u1_sum = d.u1_sum_ptr
+            a.inv_vol = _capi.ptr(d.inv_vol_scalar_gpu)
+            a.h_demag = _capi.ptr(self.H_demag_gpu)
+        a.m = _capi.ptr(m_vec)
+        a.ani_axis = _capi.ptr(self.ani_axis_gpu)
+        a.ani_axis_u = (C.c_double * 3)(*self.ani_axis_u)
+        a.ani_pref = self.ani_pref
+        if with_h0:
+            h0 = self._h_ext_ptr()
+            a.h0 = h0
+            a.h0_u = (C.c_double * 3)(*self._h_ext_uniform())
+        a.h_eff = _capi.ptr(h_eff)
+        a.rhs = _capi.ptr(rhs)
+        a.P = self.prefactorEQ
+        a.alpha = self.alpha
+        a.do_precess = self.do_precess
+        a.stab = self.Stab
+        return a
+
+    def _h_ext_uniform(self):
+        if self.H_time_func is None:
+            return self.H0_u
+        out = np.asarray(self.H_time_func(float(self.current_time)), dtype=np.float64).ravel()
+        if out.size == 3 and self.H0_gpu is None and self.Ht_gpu is None:
+            return tuple(self.H0_u[i] + float(out[i]) for i in range(3))
+        return (0.0, 0.0, 0.0)
+
+    def _h_ext_ptr(self):
+        if self.H_time_func is None:
+            return _capi.ptr(self.H0_gpu)
+        out = np.asarray(self.H_time_func(float(self.current_time)), dtype=np.float64).ravel()
+        if out.size == 3 and self.H0_gpu is None:
+            return None
+        ht = np.broadcast_to(out.reshape(-1, 3) if out.size != 3 else out, (self.N, 3)).ravel()
+        total = torch.from_numpy(np.ascontiguousarray(ht)).to(self.device)
+        total += self.H0_gpu if self.H0_gpu is not None else torch.tensor(self.H0_u * self.N, dtype=torch.float64, device=self.device)
+        self.Ht_gpu = total
+        return _capi.ptr(total)
+
+    def _launch(self, a):
+        _capi.check(_capi.lib().smx_field_llg_rhs(C.byref(a), _capi.stream_ptr()), "smx_field_llg_rhs")
+
+    # ------------------------------------------------------------------
+    def apply_linear_field_vec(self, x_vec, out_vec):
+        """out = K_lin x + H_ani(x) (:962-970)."""
+        self._launch(self._args(x_vec, h_eff=out_vec, with_demag=False, with_h0=False))
+
+    def compute_H_eff_vec(self, m_vec, out_vec):
+        """Full H_eff(m) (:972-1006)."""
+        if self.demag_field is not None:
+            self.demag_field.solve_potential(m_vec)
+        self._launch(self._args(m_vec, h_eff=out_vec))
+
+    def llg_rhs_vec(self, m_vec, out_vec):
+        """dm/dt = LLG(m, H_eff(m)) into out_vec (:1261-1285)."""
+        self.LLGSteps += 1
+        _capi.require_device(m_vec, "m_vec", numel=self.local_size)
+        _capi.require_device(out_vec, "out_vec", numel=self.local_size)
+        if self.demag_field is not None:
+            self.demag_field.solve_potential(m_vec)
+        self._launch(self._args(m_vec, h_eff=self.H_eff_gpu, rhs=out_vec))
+
+    def compute_H_jac_vec(self, m_vec, out_vec):
+        """Field for the Jacobian: everything except demag (:1232-1257)."""
+        self._launch(self._args(m_vec, h_eff=out_vec, with_demag=False))
+
+    def update_jac_state(self, m_vec):
+        self.M_state_gpu.copy_(m_vec)
+        self.compute_H_jac_vec(m_vec, self.Hm_gpu)
+
+    def jac_vec_times_vec(self, v_vec, out_vec):
+        """Jv with dH/dm excluding demag and external fields (:1293-1323)."""
+        self.JacSteps += 1
+        a = self._args(self.M_state_gpu, with_demag=False, with_h0=False)
+        _capi.check(_capi.lib().smx_llg_jvp(C.byref(a), _capi.ptr(self.Hm_gpu), _capi.ptr(v_vec),
+                                            _capi.ptr(out_vec), self.gamma, _capi.stream_ptr()),
+                    "smx_llg_jvp")
+
+    def rhs_function(self, ts, t, y, f):
+        self.current_time = float(t)
+        self.llg_rhs_vec(y, f)
+
+    # ------------------------------------------------------------------
+    def set_m_from_cpu(self, m0_array):
+        self.m_gpu.copy_(torch.as_tensor(np.asarray(m0_array, dtype=np.float64).ravel()))
+
+    def set_uniform_field(self, Hx, Hy, Hz):
+        self.H0_gpu = None
+        self.H0_u = (float(Hx), float(Hy), float(Hz))
+
+    def _dot_weighted(self, a, b) -> float:
+        _capi.check(_capi.lib().smx_dot3_weighted(self.N, _capi.ptr(a), _capi.ptr(b),
+                                                  _capi.ptr(self.ops.vol), _capi.ptr(self._scalar),
+                                                  _capi.ptr(self.red), _capi.stream_ptr()),
+                    "smx_dot3_weighted")
+        return float(self._scalar.item())
+
+    def _energy_from_field_lumped_gpu(self, m_vec, H_vec, factor):
+        """factor * mu0 Ms sum V_i m_i.H_i * 1e-27 (:1014-1051)."""
+        return factor * self.mu0 * self.Ms * self._dot_weighted(m_vec, H_vec) * self.volume_scale_energy
+
+    def compute_Energy_terms(self, m_vec=None) -> dict:
+        m = self.m_gpu if m_vec is None else m_vec
+        out = {}
+        h = torch.empty_like(m)
+        if self.ex is not None or self.dmi is not None:
+            a = self._args(m, h_eff=h, with_demag=False, with_h0=False)
+            a.ani_pref = 0.0
+            self._launch(a)
+            out["E_lin"] = self._energy_from_field_lumped_gpu(m, h, -0.5)
+        if self.ani_pref:
+            n = self.ani_axis_gpu.view(-1, 3) if self.ani_axis_gpu is not None else torch.tensor(self.ani_axis_u, dtype=torch.float64, device=self.device)
+            mdn = (m.view(-1, 3) * n).sum(dim=1)
+            out["E_ani"] = float(-self.Ku * (self.ops.vol * mdn * mdn).sum().item() * 1e-27)
+        if self.demag_field is not None:
+            self.demag_field.compute_vec(m, h)
+            out["E_demag"] = self.demag_field.Energy_lumped_gpu(m, h)
+        out["E_total"] = sum(out.values())
+        return out
+
+
+class LLG_GPU:
+    """Explicit Runge-Kutta LLG driver (relax_explicit, llg_module_GPU.py:1724-1971)."""
+
+    def __init__(self, mesh: TetMesh, Ms, Aex, Ku=0.0, n_ani_vec=None, D_bulk=0.0, D_int=0.0,
+                 n0_int_vec=None, gamma=2.211e5, alpha=0.5, do_precess=1, H0_static=None,
+                 device=None):
+        self.mesh = mesh
+        self._kw = dict(Ms=Ms, Aex=Aex, Ku=Ku, n_ani_vec=n_ani_vec, D_bulk=D_bulk, D_int=D_int,
+                        n0_int_vec=n0_int_vec, gamma=gamma, alpha=alpha, do_precess=do_precess,
+                        H0_static=H0_static)
+        self.device = device or default_device()
+        self.use_demag = False
+        self.demag_kwargs = {}
+        self.hef = None
+
+    def add_demag(self, method="lindholm", **kwargs):
+        self.use_demag = True
+        self.demag_method = method
+        self.demag_kwargs = kwargs
+
+    def _build_effective_field(self):
+        self.hef = EffectiveField(self.mesh, use_demag=self.use_demag,
+                                  demag_method=getattr(self, "demag_method", "lindholm"),
+                                  demag_kwargs=self.demag_kwargs, device=self.device, **self._kw)
+        return self.hef
+
+    def set_m(self, m0_array):
+        if self.hef is None:
+            self._build_effective_field()
+        self.hef.set_m_from_cpu(m0_array)
+
+    def relax_explicit(self, t_final, dt_init=1e-14, ts_rtol=1e-5, ts_atol=1e-5, m0_array=None,
+                       max_steps=50_000_000, fixed_dt=False, dt_min=1e-18, dt_max=1e-10):
+        if self.hef is None:
+            self._build_effective_field()
+        if m0_array is not None:
+            self.hef.set_m_from_cpu(m0_array)
+        stepper = DormandPrince(self.hef, rtol=ts_rtol, atol=ts_atol, dt_min=dt_min, dt_max=dt_max)
+        t0 = time.perf_counter()
+        stats = stepper.integrate(self.hef.m_gpu, 0.0, float(t_final), float(dt_init),
+                                  fixed_dt=fixed_dt, max_steps=max_steps)
+        stats["elapsed_s"] = time.perf_counter() - t0
+        return stats
+
+
+class DormandPrince:
+    """Dormand-Prince 5(4) with FSAL and the PETSc "basic" adaptor settings of
+    the reference (clip 0.1..3, safety 0.9, reject safety 0.1, dt in [1e-18, 1e-10])."""
+
+    A = [[], [1 / 5], [3 / 40, 9 / 40], [44 / 45, -56 / 15, 32 / 9],
+         [19372 / 6561, -25360 / 2187, 64448 / 6561, -212 / 729],
+         [9017 / 3168, -355 / 33, 46732 / 5247, 49 / 176, -5103 / 18656],
+         [35 / 384, 0.0, 500 / 1113, 125 / 192, -2187 / 6784, 11 / 84]]
+    Cn = [0.0, 1 / 5, 3 / 10, 4 / 5, 8 / 9, 1.0, 1.0]
+    B5 = [35 / 384, 0.0, 500 / 1113, 125 / 192, -2187 / 6784, 11 / 84, 0.0]
+    B4 = [5179 / 57600, 0.0, 7571 / 16695, 393 / 640, -92097 / 339200, 187 / 2100, 1 / 40]
+
+    def __init__(self, hef: EffectiveField, rtol=1e-5, atol=1e-5, dt_min=1e-18, dt_max=1e-10,
+                 safety=0.9, clip=(0.1, 3.0), reject_safety=0.1):
+        self.hef = hef
+        self.rtol, self.atol = float(rtol), float(atol)
+        self.dt_min, self.dt_max = float(dt_min), float(dt_max)
+        self.safety, self.clip, self.reject_safety = safety, clip, reject_safety
+        n = hef.local_size
+        dev = hef.device
+        self.k = [torch.zeros(n, dtype=torch.float64, device=dev) for _ in range(7)]
+        self.ys = torch.zeros(n, dtype=torch.float64, device=dev)
+        self.ynew = torch.zeros(n, dtype=torch.float64, device=dev)
+        self.err = torch.zeros(1, dtype=torch.float64, device=dev)
+        self.red = reduce_workspace(n, dev)
+        self.n = n
+        self.rhs_evals = 0
+        self._fsal = False
+        self._Ld = _capi.lib()
+
+    def _combine(self, y, dt, ns, coeffs, out, ecoef=None):
+        ptrs = (C.c_void_p * 7)(*[self.k[s].data_ptr() for s in range(ns)] + [None] * (7 - ns))
+        c = (C.c_double * 7)(*(list(coeffs) + [0.0] * (7 - len(coeffs))))
+        e = (C.c_double * 7)(*(list(ecoef) + [0.0] * (7 - len(ecoef)))) if ecoef is not None else None
+        _capi.check(self._Ld.smx_rk_combine(self.n, y.data_ptr(), float(dt), ns,
+                                            C.cast(ptrs, C.c_void_p), C.cast(c, C.c_void_p),
+                                            C.cast(e, C.c_void_p) if e is not None else None,
+                                            self.atol, self.rtol, out.data_ptr(),
+                                            self.err.data_ptr() if ecoef is not None else None,
+                                            self.red.data_ptr(), _capi.stream_ptr()),
+                    "smx_rk_combine")
+
+    def _f(self, t, y, out):
+        self.hef.current_time = t
+        self.hef.llg_rhs_vec(y, out)
+        self.rhs_evals += 1
+
+    def step(self, y, t, dt, want_err=True):
+        """One DP5 step from (t, y); y5 in self.ynew; returns the error norm."""
+        if not self._fsal:
+            self._f(t, y, self.k[0])
+        for s in range(1, 7):
+            self._combine(y, dt, s, self.A[s][:s], self.ys)
+            self._f(t + self.Cn[s] * dt, self.ys, self.k[s])
+        # stage 7 input equals y5 (FSAL); the error uses all 7 stages
+        ecoef = [b5 - b4 for b5, b4 in zip(self.B5, self.B4)]
+        self._combine(y, dt, 7, self.B5, self.ynew, ecoef if want_err else None)
+        return float(self.err.item()) if want_err else 0.0
+
+    def integrate(self, y, t0, t1, dt, fixed_dt=False, max_steps=10 ** 9):
+        t = t0
+        steps = rejected = 0
+        self._fsal = False
+        while t < t1 * (1 - 1e-15) and steps < max_steps:
+            dt = min(dt, t1 - t) if not fixed_dt else dt
+            err = self.step(y, t, dt, want_err=not fixed_dt)
+            if fixed_dt or err <= 1.0:
+                y.copy_(self.ynew)
+                self.k[0], self.k[6] = self.k[6], self.k[0]
+                self._fsal = True
+                t += dt
+                steps += 1
+                if not fixed_dt:
+                    fac = self.safety * (1.0 / max(err, 1e-16)) ** 0.2
+                    dt = min(max(dt * min(max(fac, self.clip[0]), self.clip[1]), self.dt_min), self.dt_max)
+            else:
+                rejected += 1
+                self._fsal = True
+                fac = self.safety * (1.0 / err) ** 0.2
+                dt = max(dt * min(max(fac, self.clip[0]), 1.0), self.dt_min)
+        return {"t": t, "steps": steps, "rejected": rejected, "rhs_evals": self.rhs_evals, "dt": dt}
+
+    def run_steps(self, y, t, dt, nsteps):
+        """nsteps fixed-size steps (no error estimate); returns the final time."""
+        self._fsal = False
+        for _ in range(int(nsteps)):
+            self.step(y, t, dt, want_err=False)
+            y.copy_(self.ynew)
+            self.k[0], self.k[6] = self.k[6], self.k[0]
+            self._fsal = True
+            t += dt
+        return t

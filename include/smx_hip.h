@@ -1,0 +1,205 @@
+/*
+ * libsmx_hip: C ABI of the MI355X (gfx950) demag + LLG hot path.
+ *
+ * Conventions
+ *   - Every pointer argument is a device pointer unless stated otherwise.
+ *   - Vector fields are interleaved per node: [x0, y0, z0, x1, ...].
+ *   - "SELL-64" sparse storage: rows are grouped in slices of 64 (one
+ *     wavefront).  Slice s holds width_s = (slice_ptr[s+1]-slice_ptr[s])/64
+ *     entries per row, stored column-major inside the slice: entry j of row
+ *     64*s+l sits at position slice_ptr[s] + 64*j + l of `cols` and, with k
+ *     value components per entry, at k*(that position) of the value array.
+ *     Padding entries carry value 0 and a valid column index.
+ *   - Functions are asynchronous on `stream` (a hipStream_t passed as void*),
+ *     allocate no device memory (workspaces are caller-provided) and return a
+ *     smx_status.  smx_last_error() describes the last failure of the thread.
+ *   - Reductions are deterministic: fixed-order, no floating-point atomics.
+ *
+ * Reference interfaces replaced (paths relative to src/spinfemx of the
+ * reference) are named at each entry point.
+ */
+#ifndef SMX_HIP_H
+#define SMX_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  SMX_OK = 0,
+  SMX_ERR_SHAPE = 1,          /* ValueError in the reference               */
+  SMX_ERR_VALUE = 2,          /* ValueError                                */
+  SMX_ERR_NOT_CONVERGED = 3,  /* KSP did not converge (never checked there) */
+  SMX_ERR_OOM = 4,            /* workspace too small / MemoryError          */
+  SMX_ERR_HIP = 5,            /* RuntimeError from the device runtime       */
+  SMX_ERR_TYPE = 6            /* TypeError (unsupported dtype)              */
+} smx_status;
+
+typedef enum { SMX_F64 = 0, SMX_F32 = 1, SMX_BF16 = 2 } smx_dtype;
+
+/* Number of doubles per triangle in the geometry table (structure of arrays:
+ * field f of triangle t at geom[f * ntri_total + t]).  Fields:
+ *   0-8   p0, p1, p2 (xyz each)       9-11  unit normal n
+ *   12-14 s_edge (|p1-p0|, |p2-p1|, |p0-p2|) + 1e-30
+ *   15-23 eta_i = n x xi_i (row i)    24-32 gamma_ij = xi_{i+1}.xi_j
+ *   33    c = 1 / (8 pi (area + 1e-300))                                   */
+#define SMX_TRI_FIELDS 34
+#define SMX_LINDHOLM_TILE_COLS 1024
+
+int smx_version(void);
+const char* smx_last_error(void);
+
+/* ---------------------------------------------------------------------- *
+ * (a) Lindholm double-layer block / row assembly.
+ * Replaces LindholmEntryProviderGPU.fill_block_device
+ * (gpu/fields/lindholm_hmatrix_gpu/entry_provider_gpu.py:687-750, kernels
+ * :205-347) and the numba assemble_B_rows_owned (gpu/fields/Demag_Lindholm.py:51-116).
+ *
+ * out[r*ld_row + c*ld_col] = sum over relevant triangles i (global index
+ * rel_tri[i]) not incident to target rows[r] (when incident_policy == 0) of the
+ * Lindholm weight of vertex k of that triangle, for c = rel_cols[3*i+k] >= 0.
+ * ncols <= SMX_LINDHOLM_TILE_COLS.  Entries without a contribution are 0.
+ * One wavefront per target row; triangle geometry is staged in LDS.
+ * ---------------------------------------------------------------------- */
+int smx_lindholm_fill(const double* Xb, const int32_t* rows, int64_t nrows,
+                      const double* tri_geom, int64_t ntri_total,
+                      const int32_t* tri_lid, const int32_t* rel_tri,
+                      const int32_t* rel_cols, int64_t nrel, int64_t ncols,
+                      double* out, int64_t ld_row, int64_t ld_col,
+                      int32_t incident_policy, void* stream);
+
+/* Write out[i*ld + i] = diag[i] (the solid-angle jump omega/4pi - 1) and, when
+ * out_lo != NULL, convert the n x n FP64 matrix into dtype `lo_dtype` (FP32 or
+ * BF16, round-to-nearest-even) with leading dimension ld.  Used after
+ * smx_lindholm_fill to finish the dense B (Demag_Lindholm.py:73-75). */
+int smx_dense_finish(double* B, int64_t n, int64_t ld, const double* diag,
+                     void* out_lo, int32_t lo_dtype, void* stream);
+
+/* ---------------------------------------------------------------------- *
+ * (b) Dense boundary matvec y = B x, B row-major n x n (leading dimension ldb)
+ * stored in FP64, FP32 or BF16; x, y FP64; FP64 accumulation.  When
+ * `gather` != NULL the input is x[k] = src[gather[k]] - (*shift) * shift_scale
+ * (the boundary gather xb = u1[bdofs] with the nullspace projection folded in).
+ * Replaces cp.matmul(B_dense_gpu, xb) (gpu/fields/Demag_Lindholm_GPU.py:596-616).
+ * ---------------------------------------------------------------------- */
+int smx_dense_bmatvec(const void* B, int32_t dtype, int64_t n, int64_t ldb,
+                      const double* src, const int32_t* gather,
+                      const double* shift, double shift_scale, double* xbuf,
+                      double* y, void* stream);
+
+/* Transposed dense matvec y = B^T x (adjoint of the boundary operator).
+ * work must hold (ceil(n/256)) * n doubles. */
+int smx_dense_bmatvec_t(const void* B, int32_t dtype, int64_t n, int64_t ldb,
+                        const double* x, double* y, double* work, void* stream);
+
+/* ---------------------------------------------------------------------- *
+ * (c) SELL-64 sparse kernels (replace PETSc AIJCUSPARSE MatMult on Div, A_u2_raw,
+ * G and K_lin: Demag_Lindholm_GPU.py:570, 632, 662; llg_module_GPU.py:963-964).
+ * kind 0: y_i = sum_j a_ij x_j                     (k = 1)
+ * kind 1: y_i = sum_j a_ij . x_j   (x 3-vector)    (k = 3, divergence)
+ * kind 2: y_i = sum_j a_ij x_j     (y 3-vector)    (k = 3, gradient)
+ * kind 3: y_i = sum_j A_ij x_j     (3x3 blocks)    (k = 9)
+ * When sum_out != NULL, *sum_out = sum_i y_i (kind 0/1 only) via the
+ * last-block reduction; partials/counter come from `red` (smx_reduce_ws_size).
+ * ---------------------------------------------------------------------- */
+int smx_sell_spmv(int64_t nrows, const int64_t* slice_ptr, const int32_t* cols,
+                  const double* vals, int32_t kind, const double* x, double* y,
+                  double* sum_out, double* red, void* stream);
+
+/* Doubles needed by a reduction workspace for n rows (partials + counter). */
+int64_t smx_reduce_ws_size(int64_t n);
+
+/* ---------------------------------------------------------------------- *
+ * (c) Jacobi-preconditioned CG (Chronopoulos-Gear form: one fused
+ * SpMV + update + 4-dot kernel per iteration, device-side convergence test,
+ * fixed-order reductions).  Replaces PETSc KSP cg (+gamg) of the two Poisson
+ * solves (Demag_Lindholm_GPU.py:216-226, 271-281, 569-640).
+ *
+ * mode 0 (u1, Neumann): solves A x = b - mean(b)*project; x0 = x if warm else 0.
+ *     On return sums[0] = sum(x) (for the nullspace projection of x).
+ * mode 1 (u2, Dirichlet lifting): A = A_bc; boundary rows take the data
+ *     x_k = gb[bpos[k]] where bpos[k] >= 0; interior rows solve
+ *     sum_j A_raw_kj x_j = 0.  `a_raw_*` is the raw stiffness (same n).
+ * Convergence: ||r||_2 <= max(rtol*||b||_2, atol), b the (lifted) right side.
+ * work: smx_pcg_ws_size(n) doubles (zero-filled once at allocation).
+ * its_out / rnorm_out / sums: host pointers filled on return (the call
+ * synchronizes `stream` every `check_every` iterations).
+ * ---------------------------------------------------------------------- */
+int64_t smx_pcg_ws_size(int64_t n);
+int smx_pcg_jacobi(int64_t n, const int64_t* slice_ptr, const int32_t* cols,
+                   const double* vals, const double* dinv, int32_t mode,
+                   const double* b, const double* b_sum, int32_t project,
+                   const int64_t* a_raw_slice_ptr, const int32_t* a_raw_cols,
+                   const double* a_raw_vals, const int32_t* bpos, const double* gb,
+                   double* x, int32_t warm, double rtol, double atol,
+                   int32_t max_it, int32_t check_every, int32_t expected_its,
+                   double* work, int32_t* its_out, double* rnorm_out,
+                   double* sums_out, void* stream);
+
+/* ---------------------------------------------------------------------- *
+ * (d) Fused effective field + LLG right-hand side.
+ * Replaces apply_linear_field_vec + anisotropy + demag G.mult/V + axpys +
+ * _LLG_RHS_KERNEL (llg_module_GPU.py:100-122, 962-1006, 1261-1285;
+ * Anisotropy_GPU.py:25-38; Demag_Lindholm_GPU.py:660-670).  One thread per
+ * node over the node-graph SELL-64 pattern:
+ *   H = ex*m + dmi*m + pref_ani (m.n) n + [grad*(u1+u2-u1_sum/n)]/V + H0
+ *   rhs = P (p m x H + alpha m x (m x H)) + S (1 - |m|^2) m
+ * Any operator pointer may be NULL (term absent).  Outputs may be NULL.
+ * ---------------------------------------------------------------------- */
+typedef struct {
+  int64_t n;                 /* nodes */
+  const int64_t* slice_ptr;  /* node graph (shared by ex/dmi/grad) */
+  const int32_t* cols;
+  const double* ex_vals;     /* k=1 exchange (prefactor folded), or NULL */
+  const double* dmi_vals;    /* k=9 DMI blocks (prefactor folded), or NULL */
+  const double* grad_vals;   /* k=3 demag gradient G, or NULL */
+  const double* u1;          /* N */
+  const double* u2;          /* N */
+  const double* u1_sum;      /* device scalar: sum(u1) (projected out), or NULL */
+  const double* inv_vol;     /* N: 1/V_i */
+  const double* m;           /* 3N */
+  const double* ani_axis;    /* 3N unit axes, or NULL -> ani_axis_u */
+  double ani_axis_u[3];
+  double ani_pref;           /* 2 Ku/(mu0 Ms); 0 disables */
+  const double* h0;          /* 3N external field, or NULL -> h0_u */
+  double h0_u[3];
+  double* h_eff;             /* 3N out (total field) or NULL */
+  double* h_demag;           /* 3N out (demag part) or NULL */
+  double* rhs;               /* 3N out or NULL */
+  double P;                  /* -gamma/(1+alpha^2) */
+  double alpha;
+  double do_precess;
+  double stab;               /* Ms gamma/(1+alpha^2)/2 */
+} smx_field_args;
+
+int smx_field_llg_rhs(const smx_field_args* a, void* stream);
+
+/* Jacobian-vector product of the LLG RHS with the field derivative
+ * Hv = ex*v + dmi*v + pref_ani (v.n) n computed in-kernel (demag and external
+ * field excluded, as compute_H_jac_vec / apply_jacobian_field_vec do).
+ * Replaces _JAC_VEC_KERNEL + jac_vec_times_vec (llg_module_GPU.py:125-164,
+ * 1293-1323).  `a->m` is the Jacobian state, `hm` its field H_jac(m). */
+int smx_llg_jvp(const smx_field_args* a, const double* hm, const double* v,
+                double* out, double gamma, void* stream);
+
+/* out = sum_i w_i (a_i . b_i) over 3-vectors (w may be NULL -> 1); `red` as in
+ * smx_sell_spmv.  Energies (Demag_Lindholm_GPU.py:706-724,
+ * llg_module_GPU.py:1014-1051). */
+int smx_dot3_weighted(int64_t n, const double* a, const double* b,
+                      const double* w, double* out, double* red, void* stream);
+
+/* Dormand-Prince stage combination: out = y + dt * sum_s c[s] k_s over
+ * nstage (<=7) stage vectors (host array of device pointers), length len.
+ * When err_out != NULL, also err = dt*sum_s e[s] k_s and the weighted RMS
+ * norm sqrt(mean((err/(atol+rtol*max(|y|,|out|)))^2)) into *err_out. */
+int smx_rk_combine(int64_t len, const double* y, double dt, int32_t nstage,
+                   const double* const* k, const double* c, const double* e,
+                   double atol, double rtol, double* out, double* err_out,
+                   double* red, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SMX_HIP_H */

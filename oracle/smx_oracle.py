@@ -1,0 +1,789 @@
+"""FP64 CPU oracle for the Fredkin-Koehler demag + LLG RHS hot path.
+
+TEST INFRASTRUCTURE ONLY.  This module is imported by ``tests/``, by
+``__graft_entry__.smoke()`` (as the checker) and by the ``cpu_baseline`` leg of
+``bench.py``.  The product path (the ``smx`` package and ``libsmx_hip.so``) never
+imports, links or calls it.
+
+It restates, in plain numpy/scipy float64, the reference's algorithm for every
+function on the path.  Each function names the reference file:line it follows
+(paths relative to the reference's ``src/spinfemx``).  Where the reference calls
+DOLFINx/UFL/PETSc, the oracle evaluates the same weak forms with exact P1
+element integrals and uses direct sparse LU (``scipy.sparse.linalg.splu``), as
+the reference's CPU path does below 60 000 dofs
+(cpu/fields/Demag/Demag_Lindholm.py:669-680).
+
+How each part is pinned (see tests/test_oracle_*.py):
+  solid_angle         closed forms (octant = pi/2, face = 2 pi ...), sign
+  lindholm_weights    sum_k w_k = -Omega/4pi; brute-force Gauss quadrature of
+                      (1/4pi) int phi_k (x-y).n/|x-y|^3 dS on far targets;
+                      scale invariance
+  boundary_data       counts of the benchmark meshes; omega_sum in {pi/2, pi, 2pi}
+                      on boxes; outward normals (divergence theorem)
+  dense_B             B.1 = -1 per row
+  fem_operators       A.1 = 0; interior Kuhn rows = h * 7-point stencil;
+                      G(linear) = -grad * V; Div = -Ms G^T; sum V = volume
+  demag_field         patch test (uniform m: H = -Ms m for the u1-part);
+                      uniformly magnetized cube -> <H> ~ -Ms/3 (convergence)
+  exchange/DMI/aniso  K 1 = 0; 7-point Laplacian; continuum formulas on linear m
+  llg_rhs / jac_vec   m.r = S(1-|m|^2)|m|^2; central finite differences
+  tree / partition    permutation + interval invariants, leaf sizes,
+                      admissibility by brute force
+  fullaca_dense       ||A - UV||_F <= eps ||A||_F ; exact rank on rank-r input
+  hmatrix matvec      validate_against_dense protocol (seed 1234, 3 trials)
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+MU0 = 4.0 * math.pi * 1e-7
+TINY = 1e-300
+
+
+# --------------------------------------------------------------------------
+# Lindholm kernel (gpu/fields/Lindholm_kernel.py:10-125)
+# --------------------------------------------------------------------------
+def _dot(a, b):
+    return a[..., 0] * b[..., 0] + a[..., 1] * b[..., 1] + a[..., 2] * b[..., 2]
+
+
+def _det3(a, b, c):
+    return (a[..., 0] * (b[..., 1] * c[..., 2] - b[..., 2] * c[..., 1])
+            - a[..., 1] * (b[..., 0] * c[..., 2] - b[..., 2] * c[..., 0])
+            + a[..., 2] * (b[..., 0] * c[..., 1] - b[..., 1] * c[..., 0]))
+
+
+def solid_angle(x0, p0, p1, p2):
+    """Van Oosterom-Strackee solid angle (Lindholm_kernel.py:30-47)."""
+    r0 = p0 - x0
+    r1 = p1 - x0
+    r2 = p2 - x0
+    n0 = np.sqrt(_dot(r0, r0)) + TINY
+    n1 = np.sqrt(_dot(r1, r1)) + TINY
+    n2 = np.sqrt(_dot(r2, r2)) + TINY
+    det = _det3(r0, r1, r2)
+    denom = n0 * n1 * n2 + _dot(r0, r1) * n2 + _dot(r1, r2) * n0 + _dot(r2, r0) * n1
+    return 2.0 * np.arctan2(det, denom)
+
+
+def _p_log1p(ri, rj, s):
+    """Lindholm_kernel.py:51-61: log((ri+rj+s)/(ri+rj-s)) via log1p."""
+    denom = np.maximum(ri + rj - s, TINY)
+    return np.log1p((2.0 * s) / denom)
+
+
+def lindholm_weights(x0, p0, p1, p2, n_unit, area, s_edge, eta, gamma):
+    """(w0, w1, w2) of the linear double layer (Lindholm_kernel.py:65-125).
+
+    Broadcasting: x0 (...,3); triangle arrays p* (...,3), n_unit (...,3),
+    area (...), s_edge (...,3), eta (...,3,3), gamma (...,3,3).
+    """
+    rho0 = p0 - x0
+    rho1 = p1 - x0
+    rho2 = p2 - x0
+    r0 = np.sqrt(_dot(rho0, rho0)) + TINY
+    r1 = np.sqrt(_dot(rho1, rho1)) + TINY
+    r2 = np.sqrt(_dot(rho2, rho2)) + TINY
+    h = _dot(n_unit, rho0)
+    e0 = _dot(eta[..., 0, :], rho0)
+    e1 = _dot(eta[..., 1, :], rho1)
+    e2 = _dot(eta[..., 2, :], rho2)
+    s0 = s_edge[..., 0]
+    s1 = s_edge[..., 1]
+    s2 = s_edge[..., 2]
+    P0 = _p_log1p(r0, r1, s0)
+    P1 = _p_log1p(r1, r2, s1)
+    P2 = _p_log1p(r2, r0, s2)
+    omega = solid_angle(x0, p0, p1, p2)
+    g = gamma
+    gP0 = g[..., 0, 0] * P0 + g[..., 0, 1] * P1 + g[..., 0, 2] * P2
+    gP1 = g[..., 1, 0] * P0 + g[..., 1, 1] * P1 + g[..., 1, 2] * P2
+    gP2 = g[..., 2, 0] * P0 + g[..., 2, 1] * P1 + g[..., 2, 2] * P2
+    c = 1.0 / (8.0 * math.pi * (area + TINY))
+    # opposite-edge convention (Lindholm_kernel.py:118-123)
+    w0 = s1 * c * (e1 * omega - h * gP0)
+    w1 = s2 * c * (e2 * omega - h * gP1)
+    w2 = s0 * c * (e0 * omega - h * gP2)
+    return w0, w1, w2
+
+
+# --------------------------------------------------------------------------
+# Boundary data (gpu/fields/Demag_Lindholm.py:177-285)
+# --------------------------------------------------------------------------
+_TET_FACES = ((1, 2, 3), (0, 2, 3), (0, 1, 3), (0, 1, 2))
+
+
+def exterior_facets(tets):
+    """Exterior facets as (sorted vertex triple, owning cell), ordered
+    lexicographically by the sorted triple (the facet numbering of this build)."""
+    T = tets.shape[0]
+    faces = []
+    owner = []
+    for f in _TET_FACES:
+        faces.append(np.sort(tets[:, f], axis=1))
+        owner.append(np.arange(T))
+    faces = np.concatenate(faces).astype(np.int64)
+    owner = np.concatenate(owner)
+    uniq, inv, counts = np.unique(faces, axis=0, return_inverse=True, return_counts=True)
+    ext = np.nonzero(counts[inv.ravel()] == 1)[0]
+    order = np.lexsort((faces[ext, 2], faces[ext, 1], faces[ext, 0]))
+    ext = ext[order]
+    return faces[ext], owner[ext]
+
+
+@dataclass
+class BoundaryData:
+    bverts: np.ndarray    # (Nb,) mesh node id of each boundary index
+    Xb: np.ndarray
+    tri_lid: np.ndarray   # (Nt,3) int32
+    tri_pts: np.ndarray   # (Nt,3,3)
+    ntri: np.ndarray      # (Nt,3)
+    area: np.ndarray
+    s_edge: np.ndarray    # (Nt,3)
+    eta: np.ndarray       # (Nt,3,3)
+    gamma: np.ndarray     # (Nt,3,3)
+    omega_sum: np.ndarray  # (Nb,)
+
+    @property
+    def Nb(self):
+        return int(self.Xb.shape[0])
+
+    @property
+    def Nt(self):
+        return int(self.tri_lid.shape[0])
+
+    @property
+    def diag_jump(self):
+        return self.omega_sum / (4.0 * math.pi) - 1.0
+
+
+def boundary_data(X, tets) -> BoundaryData:
+    """Restates build_boundary_data_serial (Demag_Lindholm.py:177-285) one facet
+    at a time: outward orientation by the facet-centroid vs cell-centroid test
+    (:207-214), bverts = unique(tris), edge/eta/gamma precomputation (:232-263)
+    and omega_sum = sum over tets of |Omega| at each boundary vertex (:265-283)."""
+    X = np.asarray(X, dtype=np.float64)
+    faces, owner = exterior_facets(np.asarray(tets))
+    tris = []
+    normals = []
+    for (v0, v1, v2), cell in zip(faces.tolist(), owner.tolist()):
+        p0, p1, p2 = X[v0], X[v1], X[v2]
+        n = np.cross(p1 - p0, p2 - p0)
+        nn = np.linalg.norm(n)
+        if nn == 0.0:
+            continue
+        n = n / nn
+        cc = X[tets[cell]].mean(axis=0)
+        fc = (p0 + p1 + p2) / 3.0
+        if np.dot(fc - cc, n) < 0.0:
+            v1, v2 = v2, v1
+            p1, p2 = X[v1], X[v2]
+            n = np.cross(p1 - p0, p2 - p0)
+            n /= (np.linalg.norm(n) + 1e-30)
+        tris.append((v0, v1, v2))
+        normals.append(n)
+    tris = np.asarray(tris, dtype=np.int64)
+    ntri = np.asarray(normals, dtype=np.float64)
+    bverts = np.unique(tris.ravel())
+    Nb = bverts.size
+    bmap = -np.ones(X.shape[0], dtype=np.int64)
+    bmap[bverts] = np.arange(Nb)
+    tri_lid = bmap[tris].astype(np.int32)
+    tri_pts = X[tris].astype(np.float64)
+    cross = np.cross(tri_pts[:, 1] - tri_pts[:, 0], tri_pts[:, 2] - tri_pts[:, 0])
+    area = 0.5 * np.linalg.norm(cross, axis=1)
+    e = [tri_pts[:, 1] - tri_pts[:, 0], tri_pts[:, 2] - tri_pts[:, 1], tri_pts[:, 0] - tri_pts[:, 2]]
+    s = [np.linalg.norm(ei, axis=1) + 1e-30 for ei in e]
+    s_edge = np.stack(s, axis=1)
+    xi = np.stack([e[i] / s[i][:, None] for i in range(3)], axis=1)
+    eta = np.stack([np.cross(ntri, xi[:, i]) for i in range(3)], axis=1)
+    gamma = np.zeros((xi.shape[0], 3, 3))
+    for i in range(3):
+        for j in range(3):
+            gamma[:, i, j] = np.einsum("ij,ij->i", xi[:, (i + 1) % 3], xi[:, j])
+    omega_sum = np.zeros(Nb)
+    for c in range(tets.shape[0]):
+        vs = tets[c]
+        pts = X[vs]
+        for j in range(4):
+            lid = bmap[int(vs[j])]
+            if lid < 0:
+                continue
+            omega_sum[lid] += abs(float(solid_angle(pts[j], pts[(j + 1) % 4],
+                                                    pts[(j + 2) % 4], pts[(j + 3) % 4])))
+    return BoundaryData(bverts=bverts.astype(np.int64), Xb=X[bverts].copy(), tri_lid=tri_lid,
+                        tri_pts=tri_pts, ntri=ntri, area=area, s_edge=s_edge, eta=eta,
+                        gamma=gamma, omega_sum=omega_sum)
+
+
+# --------------------------------------------------------------------------
+# Dense B and block fill (gpu/fields/Demag_Lindholm.py:51-116;
+# lindholm_hmatrix_gpu/entry_provider.py:41-117, 187-223)
+# --------------------------------------------------------------------------
+def _weights_rows(bd: BoundaryData, rows, tri):
+    x0 = bd.Xb[rows][:, None, :]
+    return lindholm_weights(x0, bd.tri_pts[tri, 0][None], bd.tri_pts[tri, 1][None],
+                            bd.tri_pts[tri, 2][None], bd.ntri[tri][None], bd.area[tri][None],
+                            bd.s_edge[tri][None], bd.eta[tri][None], bd.gamma[tri][None])
+
+
+def dense_B_rows(bd: BoundaryData, rows=None, incident_policy=0, chunk=32):
+    """Rows of the dense Lindholm B (assemble_B_rows_owned, Demag_Lindholm.py:51-116):
+    diagonal omega_sum/4pi - 1, off-diagonal sum of (w0,w1,w2) over triangles not
+    incident to the target (incident_policy 0 = "skip").  Accumulation order per
+    row follows the reference loop (triangle-major, vertex a then b then c)."""
+    Nb, Nt = bd.Nb, bd.Nt
+    rows = np.arange(Nb) if rows is None else np.asarray(rows, dtype=np.int64)
+    out = np.zeros((rows.size, Nb))
+    tri = np.arange(Nt)
+    lid = bd.tri_lid.astype(np.int64)
+    for c0 in range(0, rows.size, chunk):
+        rr = rows[c0:c0 + chunk]
+        w0, w1, w2 = _weights_rows(bd, rr, tri)
+        w = np.stack([w0, w1, w2], axis=2)  # (R, Nt, 3)
+        if incident_policy == 0:
+            inc = (lid[None, :, :] == rr[:, None, None]).any(axis=2)
+            w[inc] = 0.0
+        R = rr.size
+        idx = (np.arange(R)[:, None, None] * Nb + lid[None, :, :]).ravel()
+        blk = np.bincount(idx, weights=w.ravel(), minlength=R * Nb).reshape(R, Nb)
+        out[c0:c0 + R] = blk
+    out[np.arange(rows.size), rows] = bd.omega_sum[rows] / (4.0 * math.pi) - 1.0
+    return out
+
+
+def relevant_triangles(bd: BoundaryData, cols):
+    """_relevant_triangles (entry_provider.py:187-223): triangles incident to at
+    least one block column (sorted, unique) and the local column of each of their
+    vertices (-1 if not in the block)."""
+    cols = np.asarray(cols, dtype=np.int64)
+    in_block = np.zeros(bd.Nb, dtype=bool)
+    in_block[cols] = True
+    rel = np.nonzero(in_block[bd.tri_lid].any(axis=1))[0]
+    col_of = -np.ones(bd.Nb, dtype=np.int64)
+    col_of[cols] = np.arange(cols.size)
+    return rel.astype(np.int32), col_of[bd.tri_lid[rel]].astype(np.int32)
+
+
+def block_B(bd: BoundaryData, rows, cols, incident_policy=0):
+    """Off-diagonal block B[rows, cols] with the diagonal jump excluded
+    (LindholmEntryProviderCPU.fill_block, entry_provider.py:225-247)."""
+    rows = np.asarray(rows, dtype=np.int64)
+    cols = np.asarray(cols, dtype=np.int64)
+    rel, rel_cols = relevant_triangles(bd, cols)
+    out = np.zeros((rows.size, cols.size))
+    if rel.size == 0:
+        return out
+    lid = bd.tri_lid[rel].astype(np.int64)
+    for c0 in range(0, rows.size, 64):
+        rr = rows[c0:c0 + 64]
+        w0, w1, w2 = _weights_rows(bd, rr, rel)
+        w = np.stack([w0, w1, w2], axis=2)
+        if incident_policy == 0:
+            w[(lid[None] == rr[:, None, None]).any(axis=2)] = 0.0
+        valid = rel_cols >= 0
+        R = rr.size
+        idx = (np.arange(R)[:, None, None] * cols.size + np.where(valid, rel_cols, 0)[None])
+        wv = np.where(valid[None], w, 0.0)
+        out[c0:c0 + R] = np.bincount(idx.ravel(), weights=wv.ravel(),
+                                     minlength=R * cols.size).reshape(R, cols.size)
+    return out
+
+
+# --------------------------------------------------------------------------
+# P1 FEM operators (UFL forms of gpu/fields/Demag_Lindholm_GPU.py:188-261, 554-561;
+# Exchange_GPU.py:53-68; DMI_Interfacial_GPU.py:134-168; DMI_Bulk_GPU.py:248-264;
+# nodal volume llg_module_GPU.py:469-487)
+# --------------------------------------------------------------------------
+def element_geometry(X, tets):
+    """Per tet: volume and gradients of the 4 barycentric functions (T,4,3)."""
+    P = X[tets]  # (T,4,3)
+    J = np.stack([P[:, 1] - P[:, 0], P[:, 2] - P[:, 0], P[:, 3] - P[:, 0]], axis=2)  # columns
+    det = np.linalg.det(J)
+    vol = np.abs(det) / 6.0
+    Jinv = np.linalg.inv(J)  # rows = grad lambda_1..3
+    grads = np.empty((tets.shape[0], 4, 3))
+    grads[:, 1:] = Jinv
+    grads[:, 0] = -Jinv.sum(axis=1)
+    return vol, grads
+
+
+def _coo(rows, cols, vals, shape):
+    M = sp.coo_matrix((vals.ravel(), (rows.ravel(), cols.ravel())), shape=shape).tocsr()
+    M.sum_duplicates()
+    return M
+
+
+@dataclass
+class FEMOperators:
+    N: int
+    A: sp.csr_matrix          # int grad v . grad u           (N x N)
+    Div: sp.csr_matrix        # Ms int grad v . m             (N x 3N)
+    G: sp.csr_matrix          # int v . (-grad phi)           (3N x N)
+    V: np.ndarray             # lumped nodal volume int phi_i (N)
+    bnodes: np.ndarray        # boundary node ids (sorted)
+    A_bc: sp.csr_matrix       # A with Dirichlet rows/cols -> identity
+    extra: dict = field(default_factory=dict)
+
+
+def fem_operators(X, tets, Ms) -> FEMOperators:
+    X = np.asarray(X, dtype=np.float64)
+    tets = np.asarray(tets, dtype=np.int64)
+    N = X.shape[0]
+    vol, gr = element_geometry(X, tets)
+    T = tets.shape[0]
+    a_idx = np.repeat(tets[:, :, None], 4, axis=2)
+    b_idx = np.repeat(tets[:, None, :], 4, axis=1)
+    Ke = vol[:, None, None] * np.einsum("tak,tbk->tab", gr, gr)
+    A = _coo(a_idx, b_idx, Ke, (N, N))
+    # Div[a, 3b+c] = Ms * int d_c lambda_a * lambda_b = Ms * vol/4 * d_c lambda_a
+    rows = np.repeat(tets[:, :, None, None], 4, axis=2).repeat(3, axis=3)
+    cols = 3 * tets[:, None, :, None] + np.arange(3)[None, None, None, :]
+    cols = np.broadcast_to(cols, (T, 4, 4, 3))
+    vals = Ms * (vol / 4.0)[:, None, None, None] * np.broadcast_to(gr[:, :, None, :], (T, 4, 4, 3))
+    Div = _coo(rows, cols, vals, (N, 3 * N))
+    # G[3a+c, b] = - int lambda_a d_c lambda_b = -vol/4 * d_c lambda_b
+    grows = 3 * tets[:, :, None, None] + np.arange(3)[None, None, None, :]
+    grows = np.broadcast_to(grows, (T, 4, 4, 3))
+    gcols = np.broadcast_to(tets[:, None, :, None], (T, 4, 4, 3))
+    gvals = -(vol / 4.0)[:, None, None, None] * np.broadcast_to(gr[:, None, :, :], (T, 4, 4, 3))
+    G = _coo(grows, gcols, gvals, (3 * N, N))
+    V = np.bincount(tets.ravel(), weights=np.repeat(vol / 4.0, 4), minlength=N)
+    faces, _ = exterior_facets(tets)
+    bnodes = np.unique(faces.ravel())
+    A_bc = dirichlet_identity(A, bnodes)
+    return FEMOperators(N=N, A=A, Div=Div, G=G, V=V, bnodes=bnodes, A_bc=A_bc)
+
+
+def dirichlet_identity(A, bnodes):
+    """DOLFINx assemble_matrix(a, bcs): zero BC rows and columns, unit diagonal."""
+    N = A.shape[0]
+    keep = np.ones(N)
+    keep[bnodes] = 0.0
+    D = sp.diags(keep)
+    Abc = (D @ A @ D).tocsr()
+    I = sp.diags(1.0 - keep)
+    Abc = (Abc + I).tocsr()
+    Abc.sum_duplicates()
+    return Abc
+
+
+def exchange_matrix(X, tets, V, A_ex, Ms):
+    """H_ex = diag(-2A/(mu0 Ms)/V_i/1e-18) * (int grad u : grad v) m (Exchange_GPU.py:53-68)."""
+    fem = fem_operators(X, tets, 1.0)
+    K = sp.kron(fem.A, sp.identity(3), format="csr")
+    pref = np.repeat(-2.0 * A_ex / (MU0 * Ms) / V / 1e-18, 3)
+    return (sp.diags(pref) @ K).tocsr()
+
+
+def _tri_mass3(area):
+    """int_T lambda_a lambda_b lambda_e dA for a,b,e in 0..2 (exact)."""
+    M = np.empty((area.size, 3, 3, 3))
+    for a in range(3):
+        for b in range(3):
+            for e in range(3):
+                mult = np.bincount([a, b, e], minlength=3)
+                num = math.prod(math.factorial(int(k)) for k in mult)
+                M[:, a, b, e] = 2.0 * area * num / math.factorial(5)
+    return M
+
+
+def _boundary_tris_outward(X, tets):
+    faces, owner = exterior_facets(tets)
+    P = X[faces]
+    n = np.cross(P[:, 1] - P[:, 0], P[:, 2] - P[:, 0])
+    nn = np.linalg.norm(n, axis=1)
+    n = n / nn[:, None]
+    cc = X[tets[owner]].mean(axis=1)
+    fc = P.mean(axis=1)
+    flip = np.einsum("ij,ij->i", fc - cc, n) < 0
+    n[flip] *= -1.0
+    return faces, n, 0.5 * nn
+
+
+_EPS = np.zeros((3, 3, 3))
+_EPS[0, 1, 2] = _EPS[1, 2, 0] = _EPS[2, 0, 1] = 1.0
+_EPS[0, 2, 1] = _EPS[2, 1, 0] = _EPS[1, 0, 2] = -1.0
+
+
+def dmi_interfacial_matrix(X, tets, V, D, Ms, n_nodes):
+    """Interfacial DMI (DMI_Interfacial_GPU.py:134-168):
+    K = int (n div u - grad(n.u)) . v dx + 1/2 int ((n x nu) x u) . v ds with the
+    P1 interface normal n, scaled by -2D/(mu0 Ms)/1e-9/V_i."""
+    X = np.asarray(X, dtype=np.float64)
+    tets = np.asarray(tets, dtype=np.int64)
+    N = X.shape[0]
+    n_nodes = np.broadcast_to(np.asarray(n_nodes, dtype=np.float64).reshape(-1, 3), (N, 3))
+    vol, gr = element_geometry(X, tets)
+    T = tets.shape[0]
+    ne = n_nodes[tets]  # (T,4,3)
+    mass = (np.ones((4, 4)) + np.eye(4)) / 20.0  # int lambda_a lambda_b / vol
+    # int lambda_a n_c = vol * sum_e mass[a,e] n_{e,c}
+    ln = vol[:, None, None] * np.einsum("ae,tec->tac", mass, ne)          # (T,4,3) [a,c]
+    dn = np.einsum("tek,tec->tkc", gr, ne)                                # d_k n_c  (T,3,3)
+    # entry[a,b,c,d] = int lambda_a [ d_d lam_b n_c - lam_b d_c n_d - n_d d_c lam_b ]
+    t1 = np.einsum("tac,tbd->tabcd", ln, gr)
+    t2 = np.einsum("ab,t,tcd->tabcd", mass, vol, dn)
+    t3 = np.einsum("tad,tbc->tabcd", ln, gr)
+    Ke = t1 - t2 - t3
+    faces, nu, farea = _boundary_tris_outward(X, tets)
+    M3 = _tri_mass3(farea)  # (F,3,3,3)
+    nf = n_nodes[faces]     # (F,3,3)
+    # ((n x nu) x e_d)_c = n_c? use vector identity (a x b) x e_d with a = n x nu
+    # coefficient: int lam_a lam_b ((n x nu) x e_d)_c = sum_e M3[a,b,e] ((n_e x nu) x e_d)_c
+    a_vec = np.cross(nf, nu[:, None, :])  # (F,3(e),3)
+    # (a x e_d)_c = eps_{c i j} a_i delta_jd = eps_{c i d} a_i
+    axd = np.einsum("cid,fei->fedc", _EPS, a_vec)  # (F,e,d,c)
+    Kb = 0.5 * np.einsum("fabe,fedc->fabcd", M3, axd)
+    rows = 3 * tets[:, :, None, None, None] + np.arange(3)[None, None, None, :, None]
+    cols = 3 * tets[:, None, :, None, None] + np.arange(3)[None, None, None, None, :]
+    rows = np.broadcast_to(rows, Ke.shape)
+    cols = np.broadcast_to(cols, Ke.shape)
+    brows = np.broadcast_to(3 * faces[:, :, None, None, None] + np.arange(3)[None, None, None, :, None], Kb.shape)
+    bcols = np.broadcast_to(3 * faces[:, None, :, None, None] + np.arange(3)[None, None, None, None, :], Kb.shape)
+    K = _coo(np.concatenate([rows.ravel(), brows.ravel()]),
+             np.concatenate([cols.ravel(), bcols.ravel()]),
+             np.concatenate([Ke.ravel(), Kb.ravel()]), (3 * N, 3 * N))
+    pref = np.repeat(-2.0 * D / (MU0 * Ms) / 1e-9 / V, 3)
+    return (sp.diags(pref) @ K).tocsr()
+
+
+def dmi_bulk_matrix(X, tets, V, D, Ms):
+    """Bulk DMI (DMI_Bulk_GPU.py:248-264): K = -int curl(u).v dx + 1/2 int (nu x u).v ds,
+    scaled by 2D/(mu0 Ms)*1e9/V_i."""
+    X = np.asarray(X, dtype=np.float64)
+    tets = np.asarray(tets, dtype=np.int64)
+    N = X.shape[0]
+    vol, gr = element_geometry(X, tets)
+    # curl(lam_b e_d)_c = eps_{c i d} d_i lam_b ; int lam_a = vol/4
+    Ke = -np.einsum("t,cid,tbi->tbcd", vol / 4.0, _EPS, gr)  # independent of a
+    Ke = np.broadcast_to(Ke[:, None], (tets.shape[0], 4, 4, 3, 3))
+    faces, nu, farea = _boundary_tris_outward(X, tets)
+    mass = (np.ones((3, 3)) + np.eye(3)) / 12.0
+    # (nu x e_d)_c = eps_{c i d} nu_i
+    nxd = np.einsum("cid,fi->fcd", _EPS, nu)
+    Kb = 0.5 * np.einsum("f,ab,fcd->fabcd", farea, mass, nxd)
+    rows = np.broadcast_to(3 * tets[:, :, None, None, None] + np.arange(3)[None, None, None, :, None], Ke.shape)
+    cols = np.broadcast_to(3 * tets[:, None, :, None, None] + np.arange(3)[None, None, None, None, :], Ke.shape)
+    brows = np.broadcast_to(3 * faces[:, :, None, None, None] + np.arange(3)[None, None, None, :, None], Kb.shape)
+    bcols = np.broadcast_to(3 * faces[:, None, :, None, None] + np.arange(3)[None, None, None, None, :], Kb.shape)
+    K = _coo(np.concatenate([rows.ravel(), brows.ravel()]),
+             np.concatenate([cols.ravel(), bcols.ravel()]),
+             np.concatenate([np.ascontiguousarray(Ke).ravel(), Kb.ravel()]), (3 * N, 3 * N))
+    pref = np.repeat(2.0 * D / (MU0 * Ms) * 1e9 / V, 3)
+    return (sp.diags(pref) @ K).tocsr()
+
+
+def anisotropy_field(m, n_ani, Ku, Ms):
+    """H = 2Ku/(mu0 Ms) (m.n) n with n normalized per node (Anisotropy_GPU.py:25-38, 128-136)."""
+    n = np.asarray(n_ani, dtype=np.float64).reshape(-1, 3)
+    n = n / np.maximum(np.linalg.norm(n, axis=1), 1e-300)[:, None]
+    m3 = m.reshape(-1, 3)
+    n = np.broadcast_to(n, m3.shape)
+    pref = 2.0 * Ku / (MU0 * Ms)
+    return (pref * np.einsum("ij,ij->i", m3, n)[:, None] * n).ravel()
+
+
+# --------------------------------------------------------------------------
+# Demag field: u1 solve -> B u1|G -> u2 solve -> H = G(u1+u2)/V
+# (gpu/fields/Demag_Lindholm_GPU.py:569-670; cpu/fields/Demag/Demag_Lindholm.py:785-839)
+# --------------------------------------------------------------------------
+class DemagOracle:
+    def __init__(self, X, tets, Ms, B=None, bd: BoundaryData | None = None):
+        self.X = np.asarray(X, dtype=np.float64)
+        self.tets = np.asarray(tets)
+        self.Ms = float(Ms)
+        self.fem = fem_operators(self.X, self.tets, self.Ms)
+        self.bd = bd if bd is not None else boundary_data(self.X, self.tets)
+        self.B = dense_B_rows(self.bd) if B is None else B
+        N = self.fem.N
+        # u1: singular Neumann system; pin node 0 for the factorization and
+        # project the constant afterwards (the nullspace removal of :572-582).
+        A = self.fem.A.tolil()
+        A[0, :] = 0.0
+        A[0, 0] = 1.0
+        self._lu1 = spla.splu(A.tocsc())
+        self._lu2 = spla.splu(self.fem.A_bc.tocsc())
+        self.N = N
+
+    def solve_u1(self, m):
+        b = self.fem.Div @ m
+        b = b - b.mean()
+        bb = b.copy()
+        bb[0] = 0.0
+        u1 = self._lu1.solve(bb)
+        return u1 - u1.mean()
+
+    def boundary_rhs(self, u1):
+        xb = u1[self.bd.bverts]
+        y = self.B @ xb
+        g = np.zeros(self.N)
+        g[self.bd.bverts] = y
+        return g
+
+    def solve_u2(self, g):
+        b = -(self.fem.A @ g)
+        bn = self.fem.bnodes
+        b[bn] = g[bn]
+        return self._lu2.solve(b)
+
+    def compute(self, m, return_parts=False):
+        m = np.asarray(m, dtype=np.float64)
+        u1 = self.solve_u1(m)
+        g = self.boundary_rhs(u1)
+        u2 = self.solve_u2(g)
+        H = (self.fem.G @ (u1 + u2)) / np.repeat(self.fem.V, 3)
+        if return_parts:
+            return H, dict(u1=u1, u2=u2, g=g)
+        return H
+
+    def energy(self, m, H):
+        """Energy_lumped_gpu (Demag_Lindholm_GPU.py:706-724)."""
+        dens = np.einsum("ij,ij->i", m.reshape(-1, 3), H.reshape(-1, 3))
+        return -0.5 * MU0 * self.Ms * float(np.sum(dens * self.fem.V)) / 1e27
+
+
+# --------------------------------------------------------------------------
+# LLG RHS and Jacobian-vector product (gpu/solvers/llg_module_GPU.py:100-164)
+# --------------------------------------------------------------------------
+def llg_rhs(m, H, gamma, alpha, Ms, do_precess=1.0):
+    """r = P (p m x H + alpha m x (m x H)) + S (1-|m|^2) m with P = -gamma/(1+alpha^2),
+    S = Ms gamma/(1+alpha^2)/2 (llg_module_GPU.py:100-122, 430-431)."""
+    m3 = m.reshape(-1, 3)
+    h3 = H.reshape(-1, 3)
+    P = -gamma / (1.0 + alpha * alpha)
+    S = Ms * gamma / (1.0 + alpha * alpha) * 0.5
+    mxh = np.cross(m3, h3)
+    mxmxh = np.cross(m3, mxh)
+    n2 = np.einsum("ij,ij->i", m3, m3)
+    r = P * (do_precess * mxh + alpha * mxmxh) + (S * (1.0 - n2))[:, None] * m3
+    return r.ravel()
+
+
+def jac_vec(m, H, v, Hv, gamma, alpha, Ms, do_precess=1.0):
+    """Jv of the LLG RHS for fixed-field derivative Hv (llg_module_GPU.py:125-164)."""
+    m3, h3, v3, hv3 = (a.reshape(-1, 3) for a in (m, H, v, Hv))
+    S = Ms * gamma / (1.0 + alpha * alpha) * 0.5
+    coef = -gamma / (1.0 + alpha * alpha)
+    dot = lambda a, b: np.einsum("ij,ij->i", a, b)[:, None]
+    mdHm, mdHv, vdHm, mdv, mdmm = dot(m3, h3), dot(m3, hv3), dot(v3, h3), dot(m3, v3), dot(m3, m3)
+    damp = v3 * mdHm - 2.0 * h3 * mdv + m3 * (vdHm + mdHv) - hv3 * mdmm
+    prec = np.cross(v3, h3) + np.cross(m3, hv3)
+    out = coef * (do_precess * prec + alpha * damp) + S * (1.0 - mdmm) * v3 - 2.0 * S * m3 * mdv
+    return out.ravel()
+
+
+# --------------------------------------------------------------------------
+# Dormand-Prince 5(4) (the PETSc TS "rk"/"5dp" scheme of llg_module_GPU.py:1724-1787)
+# --------------------------------------------------------------------------
+DP_A = [
+    [],
+    [1 / 5],
+    [3 / 40, 9 / 40],
+    [44 / 45, -56 / 15, 32 / 9],
+    [19372 / 6561, -25360 / 2187, 64448 / 6561, -212 / 729],
+    [9017 / 3168, -355 / 33, 46732 / 5247, 49 / 176, -5103 / 18656],
+    [35 / 384, 0.0, 500 / 1113, 125 / 192, -2187 / 6784, 11 / 84],
+]
+DP_C = [0.0, 1 / 5, 3 / 10, 4 / 5, 8 / 9, 1.0, 1.0]
+DP_B5 = [35 / 384, 0.0, 500 / 1113, 125 / 192, -2187 / 6784, 11 / 84, 0.0]
+DP_B4 = [5179 / 57600, 0.0, 7571 / 16695, 393 / 640, -92097 / 339200, 187 / 2100, 1 / 40]
+
+
+def dp5_step(f, t, y, dt, k1=None):
+    """One Dormand-Prince step; returns (y5, err_vec, k7) with FSAL k7 = f(t+dt, y5)."""
+    k = [k1 if k1 is not None else f(t, y)]
+    for s in range(1, 7):
+        ys = y + dt * sum(DP_A[s][j] * k[j] for j in range(s) if DP_A[s][j] != 0.0)
+        k.append(f(t + DP_C[s] * dt, ys))
+    y5 = y + dt * sum(DP_B5[j] * k[j] for j in range(7) if DP_B5[j] != 0.0)
+    err = dt * sum((DP_B5[j] - DP_B4[j]) * k[j] for j in range(7))
+    return y5, err, k[6]
+
+
+# --------------------------------------------------------------------------
+# Cluster tree, admissibility, block partition, full-pivot ACA, H-matvec
+# (gpu/fields/lindholm_hmatrix_gpu/tree.py:10-118, builder.py:256-376,
+#  compressors.py:35-99, storage.py:284-299, 435-499)
+# --------------------------------------------------------------------------
+@dataclass
+class Cluster:
+    start: int
+    stop: int
+    bmin: np.ndarray
+    bmax: np.ndarray
+    left: "Cluster | None" = None
+    right: "Cluster | None" = None
+
+    @property
+    def size(self):
+        return self.stop - self.start
+
+    @property
+    def is_leaf(self):
+        return self.left is None and self.right is None
+
+    @property
+    def diameter(self):
+        return float(np.linalg.norm(self.bmax - self.bmin))
+
+
+def cluster_tree(points, leaf_size=64):
+    """Recursive bisection (tree.py:44-99): split axis = first argmax of the
+    bbox extent, stable mergesort of that coordinate, left = first n//2."""
+    points = np.ascontiguousarray(points, dtype=np.float64)
+    order = []
+
+    def rec(idx):
+        lp = points[idx]
+        bmin, bmax = lp.min(axis=0), lp.max(axis=0)
+        if idx.size <= leaf_size:
+            s = len(order)
+            order.extend(idx.tolist())
+            return Cluster(s, len(order), bmin, bmax)
+        ax = int(np.argmax(bmax - bmin))
+        srt = idx[np.argsort(lp[:, ax], kind="mergesort")]
+        mid = srt.size // 2
+        if mid == 0 or mid == srt.size:
+            mid = max(1, srt.size // 2)
+        L = rec(srt[:mid])
+        R = rec(srt[mid:])
+        return Cluster(L.start, R.stop, bmin, bmax, L, R)
+
+    root = rec(np.arange(points.shape[0], dtype=np.int32))
+    return np.asarray(order, dtype=np.int32), root
+
+
+def admissible(a: Cluster, b: Cluster, eta):
+    """tree.py:101-118."""
+    gap = np.maximum(np.maximum(a.bmin - b.bmax, b.bmin - a.bmax), 0.0)
+    d = float(np.linalg.norm(gap))
+    if d <= 0.0:
+        return False
+    return max(a.diameter, b.diameter) <= float(eta) * d
+
+
+def fullaca_dense(A, rtol=1e-6, max_rank=None):
+    """Full-pivot ACA on a dense block (compressors.py:35-99)."""
+    A = np.asarray(A, dtype=np.float64)
+    m, n = A.shape
+    max_rank = min(m, n) if max_rank is None else min(max(int(max_rank), 0), m, n)
+    normA = float(np.linalg.norm(A))
+    if normA == 0.0 or max_rank == 0:
+        return np.zeros((m, 0)), np.zeros((0, n)), 0.0
+    R = A.copy()
+    U, V = [], []
+    rel = 1.0
+    for _ in range(max_rank):
+        i, j = np.unravel_index(int(np.argmax(np.abs(R))), R.shape)
+        piv = R[i, j]
+        if abs(piv) <= np.finfo(np.float64).eps * normA:
+            break
+        u = R[:, j].copy()
+        v = R[i, :].copy() / piv
+        U.append(u)
+        V.append(v)
+        R -= np.outer(u, v)
+        rel = float(np.linalg.norm(R) / normA)
+        if rel <= rtol:
+            break
+    if not U:
+        return np.zeros((m, 0)), np.zeros((0, n)), rel
+    return np.column_stack(U), np.vstack(V), rel
+
+
+@dataclass
+class HMatrixOracle:
+    perm: np.ndarray
+    diag_jump: np.ndarray
+    near: sp.csr_matrix
+    far: list          # (r0, r1, c0, c1, U, V, rel)
+    stats: dict
+
+    def matvec(self, x):
+        """storage.py:284-299."""
+        xp = x[self.perm]
+        yp = self.diag_jump[self.perm] * xp + self.near @ xp
+        for r0, r1, c0, c1, U, V, _ in self.far:
+            yp[r0:r1] += U @ (V @ xp[c0:c1])
+        y = np.empty_like(x)
+        y[self.perm] = yp
+        return y
+
+
+def build_hmatrix(bd: BoundaryData, eps=1e-6, eta=2.0, leaf_size=64,
+                  max_temporary_block_bytes=256 * 1024 ** 2, storage_factor=0.95,
+                  max_rank=None, fill=None):
+    """HMatrixBuilder._build_block / _subdivide (builder.py:256-376) with
+    full-pivot ACA.  ``fill(rows, cols)`` defaults to block_B."""
+    fill = fill or (lambda r, c: block_B(bd, r, c))
+    perm, root = cluster_tree(bd.Xb, leaf_size)
+    nr, nc, nd = [], [], []
+    far = []
+    stats = dict(dense_blocks=0, low_rank_blocks=0, rejected_low_rank_blocks=0,
+                 forced_subdivisions_memory=0, compression_tolerance_subdivisions=0)
+    margin = 1.0 + 64.0 * np.finfo(np.float64).eps
+
+    def add_dense(r, c, A=None):
+        if A is None:
+            A = fill(perm[r.start:r.stop], perm[c.start:c.stop])
+        i, j = np.nonzero(A)
+        if i.size:
+            nr.append(i + r.start)
+            nc.append(j + c.start)
+            nd.append(A[i, j])
+        stats["dense_blocks"] += 1
+
+    def subdivide(r, c):
+        rs, cs = not r.is_leaf, not c.is_leaf
+        if not rs and not cs:
+            add_dense(r, c)
+            return
+        if rs and (not cs or r.diameter >= c.diameter):
+            build(r.left, c)
+            build(r.right, c)
+        else:
+            build(r, c.left)
+            build(r, c.right)
+
+    def build(r, c):
+        adm = admissible(r, c, eta)
+        if adm and r.size * c.size * 8 > max_temporary_block_bytes:
+            stats["forced_subdivisions_memory"] += 1
+            subdivide(r, c)
+            return
+        if adm:
+            A = fill(perm[r.start:r.stop], perm[c.start:c.stop])
+            U, V, rel = fullaca_dense(A, eps, max_rank)
+            rank = U.shape[1]
+            ok = np.isfinite(rel) and rel <= eps * margin
+            if rank > 0 and ok and U.size + V.size < storage_factor * A.size:
+                far.append((r.start, r.stop, c.start, c.stop, U, V, rel))
+                stats["low_rank_blocks"] += 1
+                return
+            if not ok and (not r.is_leaf or not c.is_leaf):
+                stats["compression_tolerance_subdivisions"] += 1
+                subdivide(r, c)
+                return
+            add_dense(r, c, A)
+            stats["rejected_low_rank_blocks"] += 1
+            return
+        if r.is_leaf and c.is_leaf:
+            add_dense(r, c)
+            return
+        subdivide(r, c)
+
+    build(root, root)
+    Nb = bd.Nb
+    if nr:
+        near = sp.coo_matrix((np.concatenate(nd), (np.concatenate(nr), np.concatenate(nc))),
+                             shape=(Nb, Nb)).tocsr()
+        near.sum_duplicates()
+    else:
+        near = sp.csr_matrix((Nb, Nb))
+    return HMatrixOracle(perm=perm, diag_jump=bd.diag_jump, near=near, far=far, stats=stats)
