@@ -72,10 +72,12 @@ def test_block_fill_matches_oracle(c1):
     rel, rel_cols = geo.relevant_triangles(cols)
     out = torch.full((rows.size, cols.size), 7.0, dtype=torch.float64, device=dev)
     L = _capi.lib()
-    _capi.check(L.smx_lindholm_fill(_capi.ptr(to_dev(geo.Xb, dev)), _capi.ptr(to_dev(rows.astype(np.int32), dev)),
-                                    rows.size, _capi.ptr(to_dev(geo.tri_table(), dev)), geo.Nt,
-                                    _capi.ptr(to_dev(geo.tri_lid, dev)), _capi.ptr(to_dev(rel, dev)),
-                                    _capi.ptr(to_dev(rel_cols.reshape(-1), dev)), rel.size, cols.size,
+    # keep every device argument alive until the kernel has finished
+    Xb_d, rows_d = to_dev(geo.Xb, dev), to_dev(rows.astype(np.int32), dev)
+    geom_d, lid_d = to_dev(geo.tri_table(), dev), to_dev(geo.tri_lid, dev)
+    rel_d, rc_d = to_dev(rel, dev), to_dev(rel_cols.reshape(-1), dev)
+    _capi.check(L.smx_lindholm_fill(_capi.ptr(Xb_d), _capi.ptr(rows_d), rows.size, _capi.ptr(geom_d), geo.Nt,
+                                    _capi.ptr(lid_d), _capi.ptr(rel_d), _capi.ptr(rc_d), rel.size, cols.size,
                                     out.data_ptr(), cols.size, 1, 0, _capi.stream_ptr()))
     torch.cuda.synchronize()
     got = out.cpu().numpy()
