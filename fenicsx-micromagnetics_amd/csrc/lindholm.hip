@@ -28,23 +28,27 @@ __device__ __forceinline__ double p_log1p(double ri, double rj, double s) {
   return log1p((2.0 * s) / denom);
 }
 
-__global__ __launch_bounds__(kThreads) void lindholm_fill_kernel(
-    const double* __restrict__ Xb, const int32_t* __restrict__ rows, int64_t nrows,
-    const double* __restrict__ geom, int64_t ntot, const int32_t* __restrict__ tri_lid,
+struct FillSmem {
+  double acc[kWaves][kTile];
+  double g[SMX_TRI_FIELDS][kChunk];
+  int32_t lcol[3][kChunk];
+  int32_t lgid[3][kChunk];
+};
+
+// Rows [row0, row0+4) of one fill task; every thread of the workgroup calls
+// this with the same task (it contains workgroup barriers).
+__device__ __forceinline__ void fill_rows(
+    FillSmem& sm, const double* __restrict__ Xb, const int32_t* __restrict__ rows, int64_t nrows,
+    int64_t row0, const double* __restrict__ geom, int64_t ntot, const int32_t* __restrict__ tri_lid,
     const int32_t* __restrict__ rel_tri, const int32_t* __restrict__ rel_cols, int64_t nrel,
     int32_t ncols, double* __restrict__ out, int64_t ld_row, int64_t ld_col,
     int32_t incident_policy) {
-  __shared__ double acc[kWaves][kTile];
-  __shared__ double g[SMX_TRI_FIELDS][kChunk];
-  __shared__ int32_t lcol[3][kChunk];
-  __shared__ int32_t lgid[3][kChunk];
-
   const int wid = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  const int64_t r = (int64_t)blockIdx.x * kWaves + wid;
+  const int64_t r = row0 + wid;
   const bool active = r < nrows;
 
-  for (int c = lane; c < ncols; c += smx::kWave) acc[wid][c] = 0.0;
+  for (int c = lane; c < ncols; c += smx::kWave) sm.acc[wid][c] = 0.0;
 
   int32_t grow = -1;
   double x0 = 0.0, x1 = 0.0, x2 = 0.0;
@@ -66,7 +70,7 @@ __global__ __launch_bounds__(kThreads) void lindholm_fill_kernel(
         const int64_t gt = rel_tri ? (int64_t)rel_tri[base + t] : (base + t);
         v = geom[(int64_t)f * ntot + gt];
       }
-      g[f][t] = v;
+      sm.g[f][t] = v;
     }
     for (int idx = threadIdx.x; idx < 3 * kChunk; idx += kThreads) {
       const int k = idx / kChunk;
@@ -77,15 +81,16 @@ __global__ __launch_bounds__(kThreads) void lindholm_fill_kernel(
         c = rel_cols[3 * (base + t) + k];
         gid = tri_lid[3 * gt + k];
       }
-      lcol[k][t] = c;
-      lgid[k][t] = gid;
+      sm.lcol[k][t] = c;
+      sm.lgid[k][t] = gid;
     }
     __syncthreads();
 
     if (active && lane < nt) {
       const int t = lane;
+      const double (&g)[SMX_TRI_FIELDS][kChunk] = sm.g;
       const bool skip = (incident_policy == 0) &&
-                        (grow == lgid[0][t] || grow == lgid[1][t] || grow == lgid[2][t]);
+                        (grow == sm.lgid[0][t] || grow == sm.lgid[1][t] || grow == sm.lgid[2][t]);
       if (!skip) {
         const double r0x = g[0][t] - x0, r0y = g[1][t] - x1, r0z = g[2][t] - x2;
         const double r1x = g[3][t] - x0, r1y = g[4][t] - x1, r1z = g[5][t] - x2;
@@ -116,18 +121,42 @@ __global__ __launch_bounds__(kThreads) void lindholm_fill_kernel(
         const double w0 = s1 * c * (e1 * omega - h * gP0);
         const double w1 = s2 * c * (e2 * omega - h * gP1);
         const double w2 = s0 * c * (e0 * omega - h * gP2);
-        const int32_t c0 = lcol[0][t], c1 = lcol[1][t], c2 = lcol[2][t];
-        if (c0 >= 0) atomicAdd(&acc[wid][c0], w0);
-        if (c1 >= 0) atomicAdd(&acc[wid][c1], w1);
-        if (c2 >= 0) atomicAdd(&acc[wid][c2], w2);
+        const int32_t c0 = sm.lcol[0][t], c1 = sm.lcol[1][t], c2 = sm.lcol[2][t];
+        if (c0 >= 0) atomicAdd(&sm.acc[wid][c0], w0);
+        if (c1 >= 0) atomicAdd(&sm.acc[wid][c1], w1);
+        if (c2 >= 0) atomicAdd(&sm.acc[wid][c2], w2);
       }
     }
   }
   __syncthreads();
   if (active) {
     double* orow = out + r * ld_row;
-    for (int c = lane; c < ncols; c += smx::kWave) orow[(int64_t)c * ld_col] = acc[wid][c];
+    for (int c = lane; c < ncols; c += smx::kWave) orow[(int64_t)c * ld_col] = sm.acc[wid][c];
   }
+}
+
+__global__ __launch_bounds__(kThreads) void lindholm_fill_kernel(
+    const double* __restrict__ Xb, const int32_t* __restrict__ rows, int64_t nrows,
+    const double* __restrict__ geom, int64_t ntot, const int32_t* __restrict__ tri_lid,
+    const int32_t* __restrict__ rel_tri, const int32_t* __restrict__ rel_cols, int64_t nrel,
+    int32_t ncols, double* __restrict__ out, int64_t ld_row, int64_t ld_col,
+    int32_t incident_policy) {
+  __shared__ FillSmem sm;
+  fill_rows(sm, Xb, rows, nrows, (int64_t)blockIdx.x * kWaves, geom, ntot, tri_lid, rel_tri,
+            rel_cols, nrel, ncols, out, ld_row, ld_col, incident_policy);
+}
+
+__global__ __launch_bounds__(kThreads) void lindholm_fill_tasks_kernel(
+    const double* __restrict__ Xb, const double* __restrict__ geom, int64_t ntot,
+    const int32_t* __restrict__ tri_lid, const int32_t* __restrict__ rows,
+    const int32_t* __restrict__ rel_tri, const int32_t* __restrict__ rel_cols,
+    const smx_fill_task* __restrict__ tasks, const int32_t* __restrict__ wg_task,
+    const int32_t* __restrict__ wg_row0, double* __restrict__ out, int32_t incident_policy) {
+  __shared__ FillSmem sm;
+  const smx_fill_task T = tasks[wg_task[blockIdx.x]];
+  fill_rows(sm, Xb, rows + T.rows_off, T.nrows, wg_row0[blockIdx.x], geom, ntot, tri_lid,
+            rel_tri + T.rel_off, rel_cols + 3 * T.rel_off, T.nrel, T.ncols, out + T.out_off,
+            T.ld_row, T.ld_col, incident_policy);
 }
 
 __device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
@@ -181,6 +210,26 @@ extern "C" int smx_lindholm_fill(const double* Xb, const int32_t* rows, int64_t 
   hipLaunchKernelGGL(lindholm_fill_kernel, dim3(grid), dim3(kThreads), 0, smx::as_stream(stream),
                      Xb, rows, nrows, tri_geom, ntri_total, tri_lid, rel_tri, rel_cols, nrel,
                      (int32_t)ncols, out, ld_row, ld_col, incident_policy);
+  SMX_LAUNCH_CHECK();
+  return SMX_OK;
+}
+
+extern "C" int smx_lindholm_fill_batched(const double* Xb, const double* tri_geom,
+                                         int64_t ntri_total, const int32_t* tri_lid,
+                                         const int32_t* rows, const int32_t* rel_tri,
+                                         const int32_t* rel_cols, const smx_fill_task* tasks,
+                                         const int32_t* wg_task, const int32_t* wg_row0,
+                                         int64_t nwg, double* out, int32_t incident_policy,
+                                         void* stream) {
+  SMX_REQUIRE(nwg >= 0, SMX_ERR_SHAPE, "negative workgroup count");
+  if (nwg == 0) return SMX_OK;
+  SMX_REQUIRE(nwg <= 0x7fffffffLL, SMX_ERR_SHAPE, "too many workgroups");
+  SMX_REQUIRE(Xb && tri_geom && tri_lid && rows && rel_tri && rel_cols && tasks && wg_task &&
+                  wg_row0 && out,
+              SMX_ERR_VALUE, "null pointer argument");
+  hipLaunchKernelGGL(lindholm_fill_tasks_kernel, dim3((unsigned)nwg), dim3(kThreads), 0,
+                     smx::as_stream(stream), Xb, tri_geom, ntri_total, tri_lid, rows, rel_tri,
+                     rel_cols, tasks, wg_task, wg_row0, out, incident_policy);
   SMX_LAUNCH_CHECK();
   return SMX_OK;
 }

@@ -58,11 +58,27 @@ class FieldArgs(C.Structure):
     ]
 
 
+class HMatrixArgs(C.Structure):
+    _fields_ = [
+        ("nb", I64), ("gidx", P), ("perm", P), ("diag_p", P),
+        ("near_ptr", P), ("near_idx", P), ("near_val", P), ("near_dtype", I32), ("factor_dtype", I32),
+        ("nfar", I64), ("far_c0", P), ("far_n", P), ("far_m", P), ("far_rank", P),
+        ("far_u_off", P), ("far_v_off", P), ("far_z_off", P), ("U", P), ("V", P),
+        ("nleaf", I64), ("leaf_start", P), ("leaf_stop", P), ("leaf_fe_ptr", P),
+        ("leaf_fe_block", P), ("leaf_fe_row", P), ("xp", P), ("z", P),
+    ]
+
+
+byref = C.byref
+
 _SIGS = {
     "smx_version": ([], C.c_int),
     "smx_last_error": ([], C.c_char_p),
     "smx_lindholm_fill": ([P, P, I64, P, I64, P, P, P, I64, I64, P, I64, I64, I32, P], C.c_int),
+    "smx_lindholm_fill_batched": ([P, P, I64, P, P, P, P, P, P, P, I64, P, I32, P], C.c_int),
     "smx_dense_finish": ([P, I64, I64, P, P, I32, P], C.c_int),
+    "smx_aca_batched": ([P, P, I64, F64, P, P, P, P, P, P], C.c_int),
+    "smx_hmat_matvec": ([C.POINTER(HMatrixArgs), P, P, F64, P, P], C.c_int),
     "smx_dense_bmatvec": ([P, I32, I64, I64, P, P, P, F64, P, P, P], C.c_int),
     "smx_dense_bmatvec_t": ([P, I32, I64, I64, P, P, P, P], C.c_int),
     "smx_sell_spmv": ([I64, P, P, P, I32, P, P, P, P, P], C.c_int),

@@ -70,6 +70,31 @@ int smx_lindholm_fill(const double* Xb, const int32_t* rows, int64_t nrows,
                       double* out, int64_t ld_row, int64_t ld_col,
                       int32_t incident_policy, void* stream);
 
+/* Batched fill: one task per (block, column tile).  Task t fills rows
+ * rows[rows_off .. rows_off+nrows) against the relevant triangles
+ * rel_tri[rel_off .. rel_off+nrel) (local columns rel_cols[3*(rel_off+i)+k]),
+ * writing out[out_off + r*ld_row + c*ld_col].  Workgroup g handles rows
+ * wg_row0[g] .. +3 of task wg_task[g].  Replaces the per-block launches of
+ * HMatrixBuilderGPU._build_block (lindholm_hmatrix_gpu/builder.py:683-781). */
+typedef struct {
+  int64_t out_off;
+  int64_t ld_row;
+  int64_t ld_col;
+  int64_t rows_off;
+  int64_t rel_off;
+  int32_t nrows;
+  int32_t nrel;
+  int32_t ncols;
+  int32_t pad;
+} smx_fill_task;
+
+int smx_lindholm_fill_batched(const double* Xb, const double* tri_geom, int64_t ntri_total,
+                              const int32_t* tri_lid, const int32_t* rows,
+                              const int32_t* rel_tri, const int32_t* rel_cols,
+                              const smx_fill_task* tasks, const int32_t* wg_task,
+                              const int32_t* wg_row0, int64_t nwg, double* out,
+                              int32_t incident_policy, void* stream);
+
 /* Write out[i*ld + i] = diag[i] (the solid-angle jump omega/4pi - 1) and, when
  * out_lo != NULL, convert the n x n FP64 matrix into dtype `lo_dtype` (FP32 or
  * BF16, round-to-nearest-even) with leading dimension ld.  Used after
@@ -93,6 +118,76 @@ int smx_dense_bmatvec(const void* B, int32_t dtype, int64_t n, int64_t ldb,
  * work must hold (ceil(n/256)) * n doubles. */
 int smx_dense_bmatvec_t(const void* B, int32_t dtype, int64_t n, int64_t ldb,
                         const double* x, double* y, double* work, void* stream);
+
+/* ---------------------------------------------------------------------- *
+ * (a/b) Full-pivot ACA of many dense blocks, one workgroup per block, no host
+ * synchronization.  Replaces fullaca_dense_gpu + rank1_residual_update
+ * (lindholm_hmatrix_gpu/compressors.py:251-372) with the algorithm of
+ * fullaca_dense (compressors.py:35-99): pivot = first argmax |R| in row-major
+ * order; stop if |pivot| <= eps_mach*||A||_F; u = R[:,j], v = R[i,:]/pivot;
+ * R -= u v^T (unfused multiply and subtract, as numpy does); stop when
+ * ||R||_F/||A||_F <= rtol.  The block (row-major, leading dimension n) is
+ * overwritten by the residual.  U is column-major (m x rcap), V row-major
+ * (rcap x n).  status: 0 converged, 1 pivot break, 2 reached rcap, 3 zero block.
+ * ---------------------------------------------------------------------- */
+typedef struct {
+  int64_t a_off;
+  int64_t u_off;
+  int64_t v_off;
+  int32_t m;
+  int32_t n;
+  int32_t rcap;
+  int32_t pad;
+} smx_aca_task;
+
+int smx_aca_batched(double* A, const smx_aca_task* tasks, int64_t ntasks, double rtol,
+                    double* U, double* V, int32_t* rank_out, double* resid_out,
+                    int32_t* status_out, void* stream);
+
+/* ---------------------------------------------------------------------- *
+ * (b) H-matrix boundary matvec y = B x (replaces HMatrixGPUPackedFused.matvec,
+ * lindholm_hmatrix_gpu/backend_fused.py:496-512, with the semantics of
+ * HMatrixCPUData.matvec_cpu, storage.py:284-299):
+ *   xp = src[gidx] - shift           (perm gather, optional boundary gather)
+ *   z_b = V_b xp[J_b]                 (one wavefront per far block)
+ *   yp[L] = d_p xp[L] + Near xp |L + sum_b U_b[L,:] z_b   (one workgroup per leaf)
+ *   y[perm] = yp                      (scatter in the same kernel)
+ * No atomics: every output row is produced by the workgroup of its leaf, in a
+ * fixed order.  Factors in FP32 or FP64 (factor_dtype), near values FP64/FP32,
+ * FP64 accumulation.
+ * ---------------------------------------------------------------------- */
+typedef struct {
+  int64_t nb;
+  const int32_t* gidx;
+  const int32_t* perm;
+  const double* diag_p;
+  const int64_t* near_ptr;
+  const int32_t* near_idx;
+  const void* near_val;
+  int32_t near_dtype;
+  int32_t factor_dtype;
+  int64_t nfar;
+  const int32_t* far_c0;
+  const int32_t* far_n;
+  const int32_t* far_m;
+  const int32_t* far_rank;
+  const int64_t* far_u_off;
+  const int64_t* far_v_off;
+  const int64_t* far_z_off;
+  const void* U;
+  const void* V;
+  int64_t nleaf;
+  const int32_t* leaf_start;
+  const int32_t* leaf_stop;
+  const int64_t* leaf_fe_ptr;
+  const int32_t* leaf_fe_block;
+  const int32_t* leaf_fe_row;
+  double* xp;  /* workspace nb */
+  double* z;   /* workspace sum of ranks */
+} smx_hmatrix;
+
+int smx_hmat_matvec(const smx_hmatrix* H, const double* src, const double* shift,
+                    double shift_scale, double* y, void* stream);
 
 /* ---------------------------------------------------------------------- *
  * (c) SELL-64 sparse kernels (replace PETSc AIJCUSPARSE MatMult on Div, A_u2_raw,

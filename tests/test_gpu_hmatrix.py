@@ -1,0 +1,131 @@
+"""H-matrix on the device: batched fill + ACA build and the packed matvec.
+
+Partition: compared with the oracle's depth-first build (ACA acceptance flips
+are counted and must be zero on these meshes).  Matvec: the
+validate_against_dense protocol (seed 1234, 3 trials) against the dense FP64
+Lindholm B, bounded by 10 eps (FP64 factors) and 10 eps + 1e-6 (FP32 factors).
+"""
+import numpy as np
+import pytest
+
+import smx_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c1():
+    from smx import mesh as M
+    from smx.boundary import build_boundary_geometry
+    m = M.cube_c1()
+    bd = O.boundary_data(m.X, m.tets)
+    return m, build_boundary_geometry(m.X, m.tets), bd, O.dense_B_rows(bd)
+
+
+@pytest.mark.parametrize("leaf,eps", [(32, 1e-6), (16, 1e-4)])
+def test_device_build_matches_oracle(c1, leaf, eps):
+    import torch
+    from smx.hmatrix import HMatrixBuildConfig, build_hmatrix_device
+    m, geo, bd, B = c1
+    data = build_hmatrix_device(geo, HMatrixBuildConfig(epsilon=eps, leaf_size=leaf), torch.device("cuda"))
+    ref = O.build_hmatrix(bd, eps=eps, eta=2.0, leaf_size=leaf)
+    got = {(b.row_start, b.col_start): b for b in data.far_blocks}
+    exp = {(f[0], f[2]): f for f in ref.far}
+    flips = set(got) ^ set(exp)
+    print(f"[hmatrix build leaf={leaf} eps={eps}] far {len(got)} vs oracle {len(exp)}, flips {len(flips)}, "
+          f"build {data.metadata['build_time_seconds']:.2f}s", flush=True)
+    assert not flips
+    rank_diff = sum(got[k].rank != exp[k][4].shape[1] for k in got)
+    assert rank_diff == 0
+    assert np.array_equal(data.near_csr.indices, ref.near.indices)
+    assert np.abs(data.near_csr.data - ref.near.data).max() <= 1e-13 * np.abs(ref.near.data).max()
+    for k, b in got.items():
+        e = exp[k]
+        blk = B[np.ix_(data.perm[b.row_start:b.row_stop], data.perm[b.col_start:b.col_stop])]
+        assert np.linalg.norm(b.U @ b.V - blk) <= eps * (1 + 1e-9) * np.linalg.norm(blk) + 1e-300
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_device_matvec_validate_against_dense(c1, fp32):
+    import torch
+    from smx.hmatrix import HMatrixBuildConfig, HMatrixGPUPackedFused, build_hmatrix_device
+    m, geo, bd, B = c1
+    eps = 1e-6
+    data = build_hmatrix_device(geo, HMatrixBuildConfig(epsilon=eps, leaf_size=32), torch.device("cuda"))
+    H = HMatrixGPUPackedFused(data, use_fp32=fp32, boundary_gather=geo.bverts)
+    rng = np.random.default_rng(1234)
+    worst = 0.0
+    for _ in range(3):
+        x = rng.standard_normal(geo.Nb)
+        y = H.matvec_numpy(x)
+        worst = max(worst, np.linalg.norm(y - B @ x) / np.linalg.norm(B @ x))
+    print(f"[hmatrix matvec fp32={fp32}] rel err vs dense {worst:.3e}", flush=True)
+    assert worst <= 10 * eps + (1e-6 if fp32 else 0.0)
+    # boundary gather + shift path used by the demag field
+    N = m.num_nodes
+    u = torch.from_numpy(rng.standard_normal(N)).cuda()
+    s = torch.tensor([u.sum().item()], dtype=torch.float64, device="cuda")
+    y = torch.empty(geo.Nb, dtype=torch.float64, device="cuda")
+    H.matvec_from(u, y, gather=True, shift_ptr=s.data_ptr(), shift_scale=1.0 / N)
+    torch.cuda.synchronize()
+    xb = u.cpu().numpy()[geo.bverts] - u.cpu().numpy().mean()
+    assert np.linalg.norm(y.cpu().numpy() - B @ xb) / np.linalg.norm(B @ xb) <= 10 * eps + 1e-6
+
+
+def test_demag_hmatrix_vs_dense(c1):
+    import torch
+    from smx.demag import DemagFieldLindholmGPU
+    m, geo, bd, B = c1
+    kw = dict(ksp_rtol_u1=1e-13, ksp_rtol_u2=1e-13, ksp_atol=0.0, ksp_max_it_u1=5000, ksp_max_it_u2=5000)
+    dd = DemagFieldLindholmGPU(m, None, None, 8e5, **kw)
+    dh = DemagFieldLindholmGPU(m, None, None, 8e5, boundary_backend="hmatrix", hmatrix_epsilon=1e-6,
+                               hmatrix_leaf_size=32, **kw)
+    rng = np.random.default_rng(0)
+    g = rng.standard_normal((m.num_nodes, 3))
+    mv = torch.from_numpy((g / np.linalg.norm(g, axis=1)[:, None]).ravel()).cuda()
+    Hd = torch.empty_like(mv)
+    Hh = torch.empty_like(mv)
+    dd.compute_vec(mv, Hd)
+    dh.compute_vec(mv, Hh)
+    torch.cuda.synchronize()
+    e = float(torch.linalg.norm(Hh - Hd) / torch.linalg.norm(Hd))
+    print(f"[demag hmatrix vs dense c1] {e:.3e}", flush=True)
+    assert e <= 1e-5
+    rep = dh.boundary_memory_report()
+    assert rep["backend"] == "hmatrix" and rep["far_blocks"] > 0
+
+
+def test_c2_build_and_matvec():
+    """150 nm cube (Nb = 5402): leaf 64, eta 2, eps 1e-6; dense FP64 B from the
+    device as the reference operator."""
+    import time
+    import torch
+    from smx import mesh as M
+    from smx.boundary import build_boundary_geometry
+    from smx.demag import assemble_dense_B
+    from smx.hmatrix import HMatrixBuildConfig, HMatrixGPUPackedFused, build_hmatrix_device
+    m = M.cube_c2()
+    geo = build_boundary_geometry(m.X, m.tets)
+    dev = torch.device("cuda")
+    t0 = time.perf_counter()
+    data = build_hmatrix_device(geo, HMatrixBuildConfig(epsilon=1e-6, leaf_size=64), dev)
+    t_build = time.perf_counter() - t0
+    Bd, ld = assemble_dense_B(geo, dev)
+    Bh = Bd[:, :geo.Nb]
+    md = data.metadata
+    print(f"[c2 hmatrix] build {t_build:.2f}s far={md['low_rank_blocks']} dense={md['dense_blocks']} "
+          f"near_nnz={md['near_nnz']} far_entries={md['far_storage_entries']} "
+          f"avg_rank={md['average_far_rank']:.1f} compression={md['compression_ratio_entries']:.2f}", flush=True)
+    for fp32 in (False, True):
+        H = HMatrixGPUPackedFused(data, use_fp32=fp32)
+        rng = np.random.default_rng(1234)
+        worst = 0.0
+        for _ in range(3):
+            x = torch.from_numpy(rng.standard_normal(geo.Nb)).cuda()
+            y = H.matvec(x)
+            ref = Bh @ x
+            worst = max(worst, float(torch.linalg.norm(y - ref) / torch.linalg.norm(ref)))
+        bm = H.benchmark(repeats=20)
+        print(f"[c2 hmatrix fp32={fp32}] rel err {worst:.3e}, {bm['average_seconds']*1e6:.1f} us/matvec, "
+              f"{H.algorithmic_bytes()/bm['average_seconds']/1e9:.0f} GB/s", flush=True)
+        assert worst <= 1e-5
