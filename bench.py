@@ -1,28 +1,35 @@
 """Benchmark: demag + LLG RHS evaluations per second on one MI355X.
 
-Workload (default): the dense-BEM cube, 150 nm per side, 30^3 hexes x 6 Kuhn
-tets (29 791 nodes, 5 402 boundary nodes, dense B 5402^2), Permalloy
-(Ms 8e5 A/m, A 1.3e-11 J/m, alpha 0.5), m = vortex + N(0, 0.05) noise,
-renormalized (seeded).  One step = one full LLG RHS evaluation
-(EffectiveField.llg_rhs_vec: Div SpMV, u1 CG, boundary matvec, u2 CG, fused
-field + RHS kernel) at a magnetization taken from a pool of distinct states,
-with the reference's solver settings (CG rtol 1e-6, atol 1e-12, zero initial
-guess).  Inputs are resident in HBM before the timed region.
+Default workload ("film"): the ACA H-matrix thin film, 500 x 500 x 10 nm at
+2.5 nm spacing (200 x 200 x 4 hexes x 6 Kuhn tets: 202 005 nodes, 83 202
+boundary nodes, 166 400 boundary triangles), cluster tree leaf 64, eta 2, ACA
+tolerance 1e-6; Ms 8e5 A/m, A 1.3e-11 J/m, uniaxial Ku 5e5 J/m^3 along z,
+interfacial DMI 3e-3 J/m^2 (n0 = z), alpha 0.5; m = seeded random unit
+vectors.  "--config cube" runs the dense-BEM cube (150 nm, 30^3 hexes,
+N 29 791, Nb 5 402, vortex + N(0, 0.05) noise).
 
-Precision modes: "fp64" (B in FP64) and "mixed" (B stored in FP32, FP64
-accumulation; everything else FP64).  `value` is the mixed-mode rate; the FP64
-rate and the relative RHS error of mixed vs FP64 are reported beside it.
+One step = one LLG RHS evaluation (EffectiveField.llg_rhs_vec: Div SpMV, u1
+CG, boundary matvec, u2 CG, fused field + RHS kernel) at a magnetization
+taken from a pool of distinct states, with the reference's solver settings
+(CG rtol 1e-6, atol 1e-12, zero initial guess).  Inputs are resident in HBM
+before the timed region.  Precision modes: "fp64" (FP64 boundary operator)
+and "mixed" (FP32 H-matrix factors / FP32 dense B, FP64 accumulation);
+`value` is the mixed-mode rate, the FP64 rate and the relative RHS error of
+mixed vs FP64 are reported beside it.
 
-The JSON line carries `roofline` for the boundary matvec kernel (algorithmic
-bytes Nb^2*s_B + 16 Nb per launch over its event-timed duration, B flushed
-from the 256 MB Infinity Cache between launches) and `cpu_baseline` (the
-oracle's FP64 numpy/scipy path, LU Poisson solves, timed on the host).
+`roofline` describes the dominant kernel of a step (the CG iteration kernel):
+algorithmic bytes per launch = ELL matrix nonzeros x 12 B + 88 B per row
+(reads of r, w, s, p, x, D^-1 and writes of r, w, s, p, x), divided by its
+average duration measured with HIP events on the launch stream.
+`matvec_roofline` gives the same for the boundary matvec (HBM-resident: the
+Infinity Cache is flushed before every timed launch).  `cpu_baseline` times
+the FP64 oracle path on the host on a bounded sample.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
-import math
 import os
 import sys
 import time
@@ -32,6 +39,11 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "fenicsx-micromagnetics_amd"))
 
 HBM_PEAK_GBS = 8000.0
+GAMMA = 2.211e5
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def vortex_m(X, seed=0, center=(75.0, 75.0), rc=10.0, noise=0.05):
@@ -46,76 +58,217 @@ def vortex_m(X, seed=0, center=(75.0, 75.0), rc=10.0, noise=0.05):
     return m.ravel()
 
 
-def time_matvec(d, reps=20, flush_mb=512):
-    """Average duration of the boundary GEMV kernel (HIP events on the launch
-    stream), with a cache flush (write of flush_mb) before every launch."""
+def random_m(n, seed=0):
+    import numpy as np
+    g = np.random.default_rng(seed).standard_normal((n, 3))
+    return (g / np.linalg.norm(g, axis=1)[:, None]).ravel()
+
+
+def event_time(fn, reps, flush=None):
+    """Average seconds of fn() measured by HIP events on the current stream."""
     import torch
-    from smx import _capi
-    L = _capi.lib()
     st = torch.cuda.current_stream()
-    flush = torch.empty(flush_mb * 1024 * 1024 // 4, dtype=torch.float32, device="cuda")
-    x = torch.randn(d.Nb, dtype=torch.float64, device="cuda")
-    y = torch.empty_like(x)
-    B = d.B_dense_gpu
-    code = _capi.DTYPE_CODE[B.dtype]
     ts = []
-    for _ in range(reps + 3):
-        flush.fill_(1.0)
+    for _ in range(reps):
+        if flush is not None:
+            flush.sum()  # read-only sweep: evicts the Infinity Cache without leaving dirty lines
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record(st)
-        _capi.check(L.smx_dense_bmatvec(B.data_ptr(), code, d.Nb, d.B_ld, x.data_ptr(), None, None, 0.0,
-                                        None, y.data_ptr(), st.cuda_stream))
+        fn()
         e1.record(st)
         e1.synchronize()
         ts.append(e0.elapsed_time(e1) * 1e-3)
-    ts = ts[3:]
-    warm = []
-    for _ in range(reps):
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record(st)
-        _capi.check(L.smx_dense_bmatvec(B.data_ptr(), code, d.Nb, d.B_ld, x.data_ptr(), None, None, 0.0,
-                                        None, y.data_ptr(), st.cuda_stream))
-        e1.record(st)
-        e1.synchronize()
-        warm.append(e0.elapsed_time(e1) * 1e-3)
+    ts = sorted(ts)[: max(1, len(ts) - len(ts) // 5)]
+    return sum(ts) / len(ts)
+
+
+def cg_iteration_timing(d, K=200):
+    """Average duration of one CG iteration launch (u1 operator), events on the
+    launch stream: (solve with K iterations - solve with 0) / K."""
+    import torch
+    from smx import _capi
+    L = _capi.lib()
+    A = d.A_u1
+    x = torch.zeros_like(d.u1)
+    its = C.c_int32(0)
+    rn = C.c_double(0.0)
+    sums = (C.c_double * 1)()
+
+    def solve(k):
+        L.smx_pcg_jacobi(d.N, A.width, A.ld, A.cols.data_ptr(), A.vals.data_ptr(), d.dinv_u1.data_ptr(), 0,
+                         d.b_u1.data_ptr(), d.b_part.data_ptr(), d.b_part.numel(), 1, A.width, A.ld,
+                         A.cols.data_ptr(), A.vals.data_ptr(), None, None, x.data_ptr(), 0, 0.0, 0.0, k, 10 ** 6,
+                         k, d.ws1.data_ptr(), C.byref(its), C.byref(rn), sums, _capi.stream_ptr())
+
+    t_k = event_time(lambda: solve(K), 5)
+    t_0 = event_time(lambda: solve(0), 5)
+    per = (t_k - t_0) / K
+    nbytes = A.nnz * 12 + 88 * d.N
+    return per, nbytes
+
+
+def matvec_timing(d, flush_mb=512):
+    import torch
+    flush = torch.ones(flush_mb * 1024 * 1024 // 4, dtype=torch.float32, device="cuda")
+    x = torch.randn(d.Nb, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    if d.boundary_backend == "dense":
+        from smx import _capi
+        L = _capi.lib()
+        B = d.B_dense_gpu
+        code = _capi.DTYPE_CODE[B.dtype]
+        fn = lambda: _capi.check(L.smx_dense_bmatvec(B.data_ptr(), code, d.Nb, d.B_ld, x.data_ptr(), None, None,
+                                                     0.0, None, y.data_ptr(), _capi.stream_ptr()))
+        nbytes = d.Nb * d.Nb * B.element_size() + 16 * d.Nb
+        name = f"gemv_rows_kernel<{str(B.dtype).split('.')[-1]}> (dense boundary matvec)"
+    else:
+        H = d.B_hmatrix_gpu
+        fn = lambda: H.matvec(x, y)
+        nbytes = H.algorithmic_bytes()
+        name = "hm_gather + hm_far_z_kernel + hm_leaf_kernel (H-matrix boundary matvec)"
+    cold = event_time(fn, 12, flush)
+    warm = event_time(fn, 12)
     del flush
-    return sum(ts) / len(ts), sum(warm) / len(warm)
+    return cold, warm, nbytes, name
 
 
-def cpu_baseline(mesh, B, m_pool, Ms, A_ex, alpha, budget_s=20.0):
-    """FP64 oracle RHS evaluations on the host (numpy/scipy, LU solves)."""
+def build_case(config, dev):
     import numpy as np
+    import torch
+    from smx import mesh as M
+    from smx.demag import OperatorSet
+    from smx.llg import EffectiveField
+    common = dict(ksp_rtol_u1=1e-6, ksp_rtol_u2=1e-6, ksp_atol=1e-12)
+    t0 = time.perf_counter()
+    if config == "cube":
+        mesh = M.cube_c2()
+        mat = dict(Ms=8e5, Aex=1.3e-11, alpha=0.5)
+        ops = OperatorSet(mesh, dev)
+        h64 = EffectiveField(mesh, use_demag=True, operators=ops, demag_kwargs=dict(B_dtype="float64", **common),
+                             **mat)
+        hmx = EffectiveField(mesh, use_demag=True, operators=ops,
+                             demag_kwargs=dict(B_dtype="float32", geometry=h64.demag_field.opB, **common), **mat)
+        pool = [torch.from_numpy(vortex_m(mesh.X, seed=s)).to(dev) for s in range(8)]
+        work = "cube150nm_30x30x30_kuhn (N=29791, Nb=5402, dense B)"
+        extra = {}
+    else:
+        from smx.boundary import build_boundary_geometry
+        from smx.hmatrix import HMatrixBuildConfig, build_hmatrix_device
+        mesh = M.film_c3()
+        mat = dict(Ms=8e5, Aex=1.3e-11, alpha=0.5, Ku=5e5, n_ani_vec=(0.0, 0.0, 1.0), D_int=3e-3,
+                   n0_int_vec=(0.0, 0.0, 1.0))
+        ops = OperatorSet(mesh, dev)
+        log(f"film mesh + FEM {time.perf_counter() - t0:.1f}s")
+        geo = build_boundary_geometry(mesh.X, mesh.tets)
+        tb = time.perf_counter()
+        data = build_hmatrix_device(geo, HMatrixBuildConfig(epsilon=1e-6, eta=2.0, leaf_size=64), dev)
+        t_build = time.perf_counter() - tb
+        log(f"H-matrix build {t_build:.1f}s far={len(data.far_blocks)} near_nnz={data.near_csr.nnz}")
+        hk = dict(boundary_backend="hmatrix", geometry=geo, hmatrix_prebuilt=data, **common)
+        h64 = EffectiveField(mesh, use_demag=True, operators=ops, demag_kwargs=dict(hmatrix_use_fp32=False, **hk),
+                             **mat)
+        hmx = EffectiveField(mesh, use_demag=True, operators=ops, demag_kwargs=dict(hmatrix_use_fp32=True, **hk),
+                             **mat)
+        pool = [torch.from_numpy(random_m(mesh.num_nodes, seed=s)).to(dev) for s in range(4)]
+        work = "film500x500x10nm_200x200x4_kuhn (N=202005, Nb=83202, H-matrix leaf 64, eta 2, eps 1e-6)"
+        md = data.metadata
+        extra = {"hmatrix_build_s": round(t_build, 2), "far_blocks": len(data.far_blocks),
+                 "near_nnz": int(md["near_nnz"]), "far_storage_entries": int(md["far_storage_entries"]),
+                 "average_far_rank": round(float(md["average_far_rank"]), 2),
+                 "lindholm_pair_evaluations": int(md["lindholm_pair_evaluations"])}
+        extra["_data"] = data
+    torch.cuda.synchronize()
+    return mesh, mat, h64, hmx, pool, work, extra, time.perf_counter() - t0
+
+
+def cpu_baseline(config, mesh, mat, pool, extra, h64, budget_s):
+    """FP64 oracle RHS evaluations on the host (bounded sample)."""
+    import numpy as np
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
     sys.path.insert(0, str(ROOT / "oracle"))
     import smx_oracle as O
     t0 = time.perf_counter()
-    bd = O.boundary_data(mesh.X, mesh.tets)
-    dem = O.DemagOracle(mesh.X, mesh.tets, Ms, B=B, bd=bd)
-    fem = O.fem_operators(mesh.X, mesh.tets, 1.0)
+    Ms, A_ex, alpha = mat["Ms"], mat["Aex"], mat["alpha"]
+    fem = O.fem_operators(mesh.X, mesh.tets, Ms)
     Kex = O.exchange_matrix(mesh.X, mesh.tets, fem.V, A_ex, Ms)
+    Klin = Kex
+    if mat.get("D_int"):
+        Klin = Klin + O.dmi_interfacial_matrix(mesh.X, mesh.tets, fem.V, mat["D_int"], Ms, mat["n0_int_vec"])
+    geo = h64.demag_field.opB
+    if config == "cube":
+        B = h64.demag_field.B_dense_gpu.cpu().numpy()[:, :geo.Nb]
+        bmat = lambda x: B @ x
+        lu1 = spla.splu(_pinned(fem.A).tocsc())
+        lu2 = spla.splu(fem.A_bc.tocsc())
+        solve1 = lambda b: _pin_solve(lu1, b)
+        solve2 = lambda b: lu2.solve(b)
+        how = "scipy splu Poisson solves (N < 60000, as the reference CPU path), numpy dense B matvec"
+    else:
+        data = extra["_data"]
+        H = O.HMatrixOracle(perm=data.perm, diag_jump=data.diag_jump, near=data.near_csr,
+                            far=[(b.row_start, b.row_stop, b.col_start, b.col_stop, b.U, b.V, 0.0)
+                                 for b in data.far_blocks], stats={})
+        bmat = H.matvec
+        d1 = 1.0 / fem.A.diagonal()
+        d2 = 1.0 / fem.A_bc.diagonal()
+        M1 = spla.LinearOperator(fem.A.shape, matvec=lambda v: d1 * v)
+        M2 = spla.LinearOperator(fem.A.shape, matvec=lambda v: d2 * v)
+        solve1 = lambda b: spla.cg(fem.A, b, rtol=1e-6, atol=1e-12, maxiter=2000, M=M1)[0]
+        solve2 = lambda b: spla.cg(fem.A_bc, b, rtol=1e-6, atol=1e-12, maxiter=12000, M=M2)[0]
+        how = ("scipy CG + Jacobi Poisson solves (rtol 1e-6; the reference uses GAMG-CG above 60000 dofs, "
+               "absent here), H-matrix matvec with the same factors in numpy FP64")
     setup = time.perf_counter() - t0
+    Vn = np.repeat(fem.V, 3)
+
+    def rhs(mv):
+        b = fem.Div @ mv
+        b -= b.mean()
+        u1 = solve1(b)
+        u1 -= u1.mean()
+        g = np.zeros(fem.N)
+        g[geo.bverts] = bmat(u1[geo.bverts])
+        bb = -(fem.A @ g)
+        bb[fem.bnodes] = g[fem.bnodes]
+        u2 = solve2(bb)
+        H = Klin @ mv + (fem.G @ (u1 + u2)) / Vn
+        if mat.get("Ku"):
+            H = H + O.anisotropy_field(mv, mat["n_ani_vec"], mat["Ku"], Ms)
+        return O.llg_rhs(mv, H, GAMMA, alpha, Ms)
+
     n = 0
     t0 = time.perf_counter()
     while True:
-        mv = m_pool[n % len(m_pool)]
-        H = Kex @ mv + dem.compute(mv)
-        O.llg_rhs(mv, H, 2.211e5, alpha, Ms)
+        rhs(pool[n % len(pool)])
         n += 1
         el = time.perf_counter() - t0
         if el >= budget_s or n >= 200:
             break
     threads = os.environ.get("OMP_NUM_THREADS") or os.environ.get("OPENBLAS_NUM_THREADS") or str(os.cpu_count())
     return {"value": n / el, "unit": "evals/s", "cores": int(threads), "kind": "port",
-            "sample": f"{n} FP64 RHS evals of the 150 nm cube (oracle: scipy splu Poisson solves, numpy B matvec; "
-                      f"setup {setup:.1f}s excluded)"}
+            "sample": f"{n} FP64 RHS evals of the same workload in {el:.1f}s ({how}; setup {setup:.1f}s excluded)"}
+
+
+def _pinned(A):
+    A = A.tolil()
+    A[0, :] = 0.0
+    A[0, 0] = 1.0
+    return A.tocsr()
+
+
+def _pin_solve(lu, b):
+    bb = b.copy()
+    bb[0] = 0.0
+    return lu.solve(bb)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", choices=["film", "cube"], default="film")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     args = ap.parse_args()
@@ -124,23 +277,9 @@ def main():
         sys.exit(2)
     import numpy as np
     import torch
-    from smx import mesh as M
-    from smx.llg import EffectiveField
-    from smx.demag import OperatorSet
-
     dev = torch.device("cuda", 0)
-    Ms, A_ex, alpha = 8e5, 1.3e-11, 0.5
-    mesh = M.cube_c2()
-    t0 = time.perf_counter()
-    ops = OperatorSet(mesh, dev)
-    common = dict(ksp_rtol_u1=1e-6, ksp_rtol_u2=1e-6, ksp_atol=1e-12)
-    h64 = EffectiveField(mesh, Ms, A_ex, alpha=alpha, use_demag=True, operators=ops,
-                         demag_kwargs=dict(B_dtype="float64", **common))
-    hmx = EffectiveField(mesh, Ms, A_ex, alpha=alpha, use_demag=True, operators=ops,
-                         demag_kwargs=dict(B_dtype="float32", geometry=h64.demag_field.opB, **common))
-    torch.cuda.synchronize()
-    setup_s = time.perf_counter() - t0
-    pool = [torch.from_numpy(vortex_m(mesh.X, seed=s)).to(dev) for s in range(8)]
+    mesh, mat, h64, hmx, pool, work, extra, setup_s = build_case(args.config, dev)
+    log(f"setup {setup_s:.1f}s")
     out = torch.empty(3 * mesh.num_nodes, dtype=torch.float64, device=dev)
 
     def run(hef, steps, warmup):
@@ -155,24 +294,27 @@ def main():
         torch.cuda.synchronize()
         return time.perf_counter() - t0, its
 
-    t64, its64 = run(h64, args.steps, args.warmup)
+    t64, _ = run(h64, args.steps, args.warmup)
+    log(f"fp64 {args.steps / t64:.2f} evals/s")
     tmx, itsmx = run(hmx, args.steps, args.warmup)
-    # accuracy of mixed vs fp64 at the same m
+    log(f"mixed {args.steps / tmx:.2f} evals/s")
     r64 = torch.empty_like(out)
     rmx = torch.empty_like(out)
-    errs = []
-    for mv in pool[:4]:
+    errs, herr = [], []
+    for mv in pool[:2]:
         h64.llg_rhs_vec(mv, r64)
         hmx.llg_rhs_vec(mv, rmx)
         errs.append(float(torch.linalg.norm(rmx - r64) / torch.linalg.norm(r64)))
-    mv_avg64, mv_warm64 = time_matvec(h64.demag_field)
-    mv_avgmx, mv_warmmx = time_matvec(hmx.demag_field)
-    Nb = h64.demag_field.Nb
-    bytes64 = Nb * Nb * 8 + 16 * Nb
-    bytes32 = Nb * Nb * 4 + 16 * Nb
-    ach = bytes32 / mv_avgmx / 1e9
+        herr.append(float(torch.linalg.norm(hmx.H_demag_gpu - h64.H_demag_gpu) / torch.linalg.norm(h64.H_demag_gpu)))
+    cg_t, cg_bytes = cg_iteration_timing(hmx.demag_field)
+    mv_cold, mv_warm, mv_bytes, mv_name = matvec_timing(hmx.demag_field)
+    mv64_cold, _, mv64_bytes, _ = matvec_timing(h64.demag_field)
+    ach = cg_bytes / cg_t / 1e9
+    mv_ach = mv_bytes / mv_cold / 1e9
+    cfg = {"workload": work, "cg_rtol": 1e-6, "initial_guess": "zero", "setup_s": round(setup_s, 2)}
+    cfg.update({k: v for k, v in extra.items() if not k.startswith("_")})
     res = {
-        "metric": "demag+LLG RHS evals/s (mixed: FP32 boundary operator, FP64 elsewhere)",
+        "metric": "demag+LLG RHS evals/s (mixed precision boundary operator vs FP64)",
         "value": args.steps / tmx,
         "unit": "evals/s",
         "n_gpus": 1,
@@ -182,22 +324,25 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 (B f32)",
-        "data": "synthetic (seeded vortex + noise)",
-        "config": {"workload": "cube150nm_30^3_kuhn (N=29791, Nb=5402, dense B)",
-                   "cg_rtol": 1e-6, "initial_guess": "zero", "setup_s": round(setup_s, 2)},
+        "dtype": "f64 (boundary operator f32)",
+        "data": "synthetic (seeded)",
+        "config": cfg,
         "fp64": {"value": args.steps / t64, "ms_per_step": 1e3 * t64 / args.steps},
-        "mixed_vs_fp64_rel_err": max(errs),
+        "mixed_over_fp64": t64 / tmx,
+        "mixed_vs_fp64_rel_err_rhs": max(errs),
+        "mixed_vs_fp64_rel_err_hdemag": max(herr),
         "cg_iterations_u1_u2": [float(np.mean([a for a, _ in itsmx])), float(np.mean([b for _, b in itsmx]))],
-        "roofline": {"bound": "hbm", "kernel": "gemv_rows_kernel<float,4> (boundary matvec, FP32 B)",
+        "roofline": {"bound": "hbm", "kernel": "cg_iter_kernel (Jacobi CG iteration, u1 operator)",
                      "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                     "traffic": None, "algorithmic_bytes": bytes32, "avg_us_flushed": mv_avgmx * 1e6,
-                     "avg_us_cache_warm": mv_warmmx * 1e6,
-                     "fp64_achieved": bytes64 / mv_avg64 / 1e9},
+                     "traffic": None, "algorithmic_bytes": int(cg_bytes), "avg_us": cg_t * 1e6},
+        "matvec_roofline": {"bound": "hbm", "kernel": mv_name, "achieved": mv_ach, "peak": HBM_PEAK_GBS,
+                            "unit": "GB/s", "frac": mv_ach / HBM_PEAK_GBS, "algorithmic_bytes": int(mv_bytes),
+                            "avg_us_flushed": mv_cold * 1e6, "avg_us_cache_warm": mv_warm * 1e6,
+                            "fp64_factors_GBs": mv64_bytes / mv64_cold / 1e9,
+                            "fp64_factors_avg_us_flushed": mv64_cold * 1e6},
     }
     if not args.no_cpu:
-        Bh = h64.demag_field.B_dense_gpu.cpu().numpy()[:, :Nb]
-        res["cpu_baseline"] = cpu_baseline(mesh, Bh, [p.cpu().numpy() for p in pool[:4]], Ms, A_ex, alpha,
+        res["cpu_baseline"] = cpu_baseline(args.config, mesh, mat, [p.cpu().numpy() for p in pool[:2]], extra, h64,
                                            args.cpu_budget)
     print(json.dumps(res), flush=True)
 

@@ -1,8 +1,9 @@
 // Dense boundary matvec y = B x on gfx950 (HBM-bound streaming GEMV).
 //
-// B is row-major with FP64, FP32 or BF16 storage.  Each wavefront owns R rows
-// and streams them with 16-byte loads per lane (one 1 KiB request per row per
-// step), so one x fragment read from L1/L2 is reused R times.  Products are
+// B is row-major with FP64, FP32 or BF16 storage.  A workgroup owns R rows;
+// its wavefronts split the columns and stream them with 16-byte loads per lane
+// (1 KiB per row per wavefront step, two steps in flight), so one x fragment
+// read from L1/L2 is reused R times.  Products are
 // converted to FP64 and accumulated in FP64, so FP32/BF16 storage only adds
 // the storage rounding of B.  The final lane reduction is a butterfly on DPP /
 // ds_swizzle shuffles.
@@ -70,26 +71,56 @@ __global__ void gather_shift_kernel(int64_t n, const double* __restrict__ src,
   xout[k] = (gather ? src[gather[k]] : src[k]) - s;
 }
 
+// R rows per workgroup; the kWavesPerBlock wavefronts split the columns into
+// contiguous quarters, so each workgroup streams R*n elements with every lane
+// issuing R independent 16-byte loads per step (plus one shared x fragment),
+// and n/R workgroups keep all CUs busy even for a few thousand rows.
 template <typename T, int R>
 __global__ __launch_bounds__(kWavesPerBlock * 64) void gemv_rows_kernel(
     const T* __restrict__ B, int64_t n, int64_t ldb, const double* __restrict__ x,
     double* __restrict__ y) {
   constexpr int V = Vec16<T>::N;
+  __shared__ double part[kWavesPerBlock][R];
   const int wid = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  const int64_t row0 = ((int64_t)blockIdx.x * kWavesPerBlock + wid) * R;
-  if (row0 >= n) return;
+  const int64_t row0 = (int64_t)blockIdx.x * R;
+  const int64_t nvec = n / V;  // full 16-byte vectors per row
+  const int64_t per = (nvec + kWavesPerBlock - 1) / kWavesPerBlock;
+  const int64_t v0 = wid * per;
+  const int64_t v1 = (v0 + per < nvec) ? v0 + per : nvec;
   double acc[R];
 #pragma unroll
   for (int q = 0; q < R; ++q) acc[q] = 0.0;
-  const int64_t nvec = n / V;  // full 16-byte vectors per row
   const T* rowp[R];
 #pragma unroll
   for (int q = 0; q < R; ++q) {
     const int64_t rr = (row0 + q < n) ? row0 + q : n - 1;
     rowp[q] = B + rr * ldb;
   }
-  for (int64_t v = lane; v < nvec; v += 64) {
+  int64_t v = v0 + lane;
+  for (; v + 64 < v1; v += 128) {
+    Raw16 raw[2][R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      raw[0][q] = *reinterpret_cast<const Raw16*>(rowp[q] + v * V);
+      raw[1][q] = *reinterpret_cast<const Raw16*>(rowp[q] + (v + 64) * V);
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      double xv[V];
+      const double* xp = x + (v + 64 * h) * V;
+#pragma unroll
+      for (int e = 0; e < V; ++e) xv[e] = xp[e];
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        double bv[V];
+        unpack16<T>(raw[h][q], bv);
+#pragma unroll
+        for (int e = 0; e < V; ++e) acc[q] = fma(bv[e], xv[e], acc[q]);
+      }
+    }
+  }
+  for (; v < v1; v += 64) {
     double xv[V];
     const double* xp = x + v * V;
 #pragma unroll
@@ -103,15 +134,24 @@ __global__ __launch_bounds__(kWavesPerBlock * 64) void gemv_rows_kernel(
       for (int e = 0; e < V; ++e) acc[q] = fma(bv[e], xv[e], acc[q]);
     }
   }
-  for (int64_t j = nvec * V + lane; j < n; j += 64) {
-    const double xj = x[j];
+  if (wid == kWavesPerBlock - 1) {
+    for (int64_t j = nvec * V + lane; j < n; j += 64) {
+      const double xj = x[j];
 #pragma unroll
-    for (int q = 0; q < R; ++q) acc[q] = fma(to_f64(rowp[q][j]), xj, acc[q]);
+      for (int q = 0; q < R; ++q) acc[q] = fma(to_f64(rowp[q][j]), xj, acc[q]);
+    }
   }
 #pragma unroll
   for (int q = 0; q < R; ++q) {
     const double s = smx::wave_sum(acc[q]);
-    if (lane == 0 && row0 + q < n) y[row0 + q] = s;
+    if (lane == 0) part[wid][q] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < R && row0 + threadIdx.x < n) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < kWavesPerBlock; ++w) s += part[w][threadIdx.x];
+    y[row0 + threadIdx.x] = s;
   }
 }
 
@@ -147,8 +187,7 @@ int launch_gemv(const void* B, int64_t n, int64_t ldb, const double* x, double* 
   SMX_REQUIRE(ldb % V == 0, SMX_ERR_SHAPE, "ldb must be a multiple of %d for 16-byte rows", V);
   SMX_REQUIRE((reinterpret_cast<uintptr_t>(B) & 15) == 0, SMX_ERR_VALUE,
               "B must be 16-byte aligned");
-  const int64_t rows_per_block = (int64_t)kWavesPerBlock * R;
-  const int grid = smx::grid_for(n, (int)rows_per_block);
+  const int grid = smx::grid_for(n, R);
   hipLaunchKernelGGL((gemv_rows_kernel<T, R>), dim3(grid), dim3(kWavesPerBlock * 64), 0, st,
                      reinterpret_cast<const T*>(B), n, ldb, x, y);
   SMX_LAUNCH_CHECK();
@@ -177,7 +216,7 @@ extern "C" int smx_dense_bmatvec(const void* B, int32_t dtype, int64_t n, int64_
     case SMX_F64:
       return launch_gemv<double, 2>(B, n, ldb, x, y, st);
     case SMX_F32:
-      return launch_gemv<float, 4>(B, n, ldb, x, y, st);
+      return launch_gemv<float, 2>(B, n, ldb, x, y, st);
     case SMX_BF16:
       return launch_gemv<uint16_t, 4>(B, n, ldb, x, y, st);
     default:

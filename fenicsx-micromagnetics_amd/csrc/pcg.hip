@@ -7,77 +7,125 @@
 //   g = (r,u), d = (w,u), rr = (r,r);  b = g/g_old;  a = g/(d - b g/a_old)
 // The SpMV needs u_{i+1} at neighbour rows, which each thread rebuilds from
 // the previous iterate (r, w, s are double-buffered), so no grid-wide barrier
-// is needed inside the iteration.  The four sums (g, d, rr, sum x) are formed
-// by the "last block" reduction in a fixed order, and that block also updates
-// the scalars and the convergence flag in device memory: the next launch
-// reads them, and once the flag is set every later launch is a no-op.  The
-// host only synchronizes every `check_every` iterations.
+// is needed inside an iteration.
+//
+// Reductions without atomics or device-scope fences: every workgroup writes
+// its four partial sums; the NEXT launch's workgroups each re-reduce all of
+// them in the same fixed order (the kernel boundary is the only
+// synchronization), so every workgroup derives identical scalars.  A
+// device-scope release per workgroup (an L2 write-back on a multi-XCD part)
+// costs more than re-reading a few KB of partials from the Infinity Cache.
+// Scalars carried between launches are double-buffered by iteration parity;
+// workgroup 0 also publishes a status record that the host polls every
+// `check_every` launches.  Once converged every later launch returns at once.
+//
+// Matrices are ELL: column j of row i at cols[j*ld + i] (width W, padding
+// entries have value 0 and a valid column), so a wavefront's index and value
+// loads are contiguous and the row start needs no extra dependent load.
 #include "smx_common.h"
 
 namespace {
 
-constexpr int kBlock = 256;
-
 enum {
-  ST_ALPHA = 0,
-  ST_BETA = 1,
-  ST_GAMMA = 2,
-  ST_RR = 3,
-  ST_TOL2 = 4,
-  ST_DONE = 5,
-  ST_ITERS = 6,
-  ST_BB = 7,
-  ST_SUMX = 8,
-  ST_FAIL = 9,
-  ST_SIZE = 32
+  // status record (written by workgroup 0, read by the host)
+  SS_DONE = 0,
+  SS_ITERS = 1,
+  SS_RR = 2,
+  SS_TOL2 = 3,
+  SS_BB = 4,
+  SS_FAIL = 5,
+  SS_SUMX = 6,
+  SS_SIZE = 8,
+  // scalars carried to the next launch: [gamma, alpha]
+  ST_OFF = 8,
+  ST_SIZE = 8,
+  HDR = 32
 };
 
 struct CGPtrs {
-  double* st;
-  double* red;
+  double* ss;        // status record
+  double* st[2];     // parity-buffered scalars
+  double* part[2];   // parity-buffered partials, 4 x G
+  double* pbb;       // G partials of ||b||^2
   double* r[2];
   double* w[2];
   double* s[2];
   double* p;
+  int G;
 };
 
-__device__ __forceinline__ void row_span(int64_t row, const int64_t* sp, int64_t& start,
-                                         int64_t& w, int& lane) {
-  const int64_t s = row >> 6;
-  lane = (int)(row & 63);
-  start = sp[s];
-  w = (sp[s + 1] - start) >> 6;
+template <int BS, int NQ>
+__device__ __forceinline__ void block_total(double (&v)[NQ], double* scr) {
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  constexpr int nw = BS / 64;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) v[q] = smx::wave_sum(v[q]);
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) scr[q * nw + wid] = v[q];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    double s = 0.0;
+    for (int w = 0; w < nw; ++w) s += scr[q * nw + w];
+    v[q] = s;
+  }
 }
 
-__global__ void cg_reset_kernel(double* st) {
-  st[ST_DONE] = 0.0;
-  st[ST_ITERS] = 0.0;
-  st[ST_FAIL] = 0.0;
+// Sum of NQ x G partials (quantity q of block b at part[q*G + b]), identical
+// in every workgroup.
+template <int BS, int NQ>
+__device__ __forceinline__ void reduce_partials(const double* __restrict__ part, int G,
+                                                double (&t)[NQ], double* scr) {
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) t[q] = 0.0;
+  for (int b = threadIdx.x; b < G; b += BS) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) t[q] += part[q * G + b];
+  }
+  block_total<BS, NQ>(t, scr);
 }
 
-// r0 = b - shift - A x0   (mode 0) ;  Dirichlet lifting (mode 1)
-__global__ __launch_bounds__(kBlock) void cg_init_a_kernel(
-    int64_t n, int32_t mode, const int64_t* __restrict__ sp, const int32_t* __restrict__ cols,
+template <int BS, int NQ>
+__device__ __forceinline__ void publish_partials(double (&v)[NQ], double* __restrict__ part, int G,
+                                                 double* scr) {
+  block_total<BS, NQ>(v, scr);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) part[q * G + blockIdx.x] = v[q];
+  }
+}
+
+template <int BS, int WMAX>
+__global__ __launch_bounds__(BS) void cg_init_a_kernel(
+    int64_t n, int32_t mode, int32_t W, int64_t ld, const int32_t* __restrict__ cols,
     const double* __restrict__ vals, const double* __restrict__ b,
-    const double* __restrict__ b_sum, int32_t project, const int32_t* __restrict__ bpos,
-    const double* __restrict__ gb, double* __restrict__ x, int32_t warm, CGPtrs P, double rtol,
-    double atol) {
-  __shared__ double scratch[kBlock / 64];
-  const int64_t row = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  double bb = 0.0;
+    const double* __restrict__ b_part, int64_t nb_part, int32_t project,
+    const int32_t* __restrict__ bpos, const double* __restrict__ gb, double* __restrict__ x,
+    int32_t warm, CGPtrs P) {
+  __shared__ double scr[BS / 64 * 1];
+  const int64_t row = (int64_t)blockIdx.x * BS + threadIdx.x;
+  double shift = 0.0;
+  if (mode == 0 && project && b_part) {
+    // sum of the right-hand side partials (the divergence kernel's blocks)
+    double t[1] = {0.0};
+    for (int64_t k = threadIdx.x; k < nb_part; k += BS) t[0] += b_part[k];
+    block_total<BS, 1>(t, scr);
+    shift = t[0] / (double)n;
+  }
+  double v[1] = {0.0};
   if (row < n) {
-    int64_t start, w;
-    int lane;
-    row_span(row, sp, start, w, lane);
     double r0, bk;
     if (mode == 0) {
-      bk = b[row] - ((project && b_sum) ? (*b_sum) / (double)n : 0.0);
+      bk = b[row] - shift;
       double ax = 0.0;
       if (warm) {
-        for (int64_t j = 0; j < w; ++j) {
-          const int64_t e = start + j * 64 + lane;
-          ax = fma(vals[e], x[cols[e]], ax);
-        }
+#pragma unroll
+        for (int j = 0; j < WMAX; ++j)
+          if (j < W) ax = fma(vals[j * ld + row], x[cols[j * ld + row]], ax);
       } else {
         x[row] = 0.0;
       }
@@ -90,17 +138,19 @@ __global__ __launch_bounds__(kBlock) void cg_init_a_kernel(
         r0 = 0.0;
       } else {
         double sx = 0.0, sg = 0.0;
-        for (int64_t j = 0; j < w; ++j) {
-          const int64_t e = start + j * 64 + lane;
-          const int64_t c = cols[e];
-          const double a = vals[e];
-          const int32_t cp = bpos[c];
-          if (cp >= 0) {
-            const double gv = gb[cp];
-            sx = fma(a, gv, sx);
-            sg = fma(a, gv, sg);
-          } else if (warm) {
-            sx = fma(a, x[c], sx);
+#pragma unroll
+        for (int j = 0; j < WMAX; ++j) {
+          if (j < W) {
+            const int32_t c = cols[j * ld + row];
+            const double a = vals[j * ld + row];
+            const int32_t cp = bpos[c];
+            if (cp >= 0) {
+              const double gv = gb[cp];
+              sx = fma(a, gv, sx);
+              sg = fma(a, gv, sg);
+            } else if (warm) {
+              sx = fma(a, x[c], sx);
+            }
           }
         }
         if (!warm) x[row] = 0.0;
@@ -111,38 +161,41 @@ __global__ __launch_bounds__(kBlock) void cg_init_a_kernel(
     P.r[0][row] = r0;
     P.s[0][row] = 0.0;
     P.p[row] = 0.0;
-    bb = bk * bk;
+    v[0] = bk * bk;
   }
-  double v[1] = {bb};
-  if (smx::grid_sum_last<1>(v, P.red, scratch) && threadIdx.x == 0) {
-    P.st[ST_BB] = v[0];
-    const double t1 = rtol * rtol * v[0];
-    const double t2 = atol * atol;
-    P.st[ST_TOL2] = t1 > t2 ? t1 : t2;
-  }
+  publish_partials<BS, 1>(v, P.pbb, P.G, scr);
 }
 
-// w0 = A D^-1 r0 and the first scalars
-__global__ __launch_bounds__(kBlock) void cg_init_b_kernel(int64_t n,
-                                                           const int64_t* __restrict__ sp,
-                                                           const int32_t* __restrict__ cols,
-                                                           const double* __restrict__ vals,
-                                                           const double* __restrict__ dinv,
-                                                           const double* __restrict__ x,
-                                                           CGPtrs P) {
-  __shared__ double scratch[4 * kBlock / 64];
-  const int64_t row = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+// w0 = A D^-1 r0; partials of (g0, d0, rr0, sum x0)
+template <int BS, int WMAX>
+__global__ __launch_bounds__(BS) void cg_init_b_kernel(int64_t n, int32_t W, int64_t ld,
+                                                       const int32_t* __restrict__ cols,
+                                                       const double* __restrict__ vals,
+                                                       const double* __restrict__ dinv,
+                                                       const double* __restrict__ x, CGPtrs P,
+                                                       double rtol, double atol) {
+  __shared__ double scr[BS / 64 * 4];
+  double t[1];
+  reduce_partials<BS, 1>(P.pbb, P.G, t, scr);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const double t1 = rtol * rtol * t[0], t2 = atol * atol;
+    P.ss[SS_BB] = t[0];
+    P.ss[SS_TOL2] = t1 > t2 ? t1 : t2;
+    P.ss[SS_DONE] = 0.0;
+    P.ss[SS_FAIL] = 0.0;
+    P.ss[SS_ITERS] = 0.0;
+  }
+  const int64_t row = (int64_t)blockIdx.x * BS + threadIdx.x;
   double v[4] = {0.0, 0.0, 0.0, 0.0};
   if (row < n) {
-    int64_t start, w;
-    int lane;
-    row_span(row, sp, start, w, lane);
     const double* r = P.r[0];
     double acc = 0.0;
-    for (int64_t j = 0; j < w; ++j) {
-      const int64_t e = start + j * 64 + lane;
-      const int64_t c = cols[e];
-      acc = fma(vals[e], r[c] * dinv[c], acc);
+#pragma unroll
+    for (int j = 0; j < WMAX; ++j) {
+      if (j < W) {
+        const int32_t c = cols[j * ld + row];
+        acc = fma(vals[j * ld + row], r[c] * dinv[c], acc);
+      }
     }
     P.w[0][row] = acc;
     const double rk = r[row];
@@ -152,55 +205,83 @@ __global__ __launch_bounds__(kBlock) void cg_init_b_kernel(int64_t n,
     v[2] = rk * rk;
     v[3] = x[row];
   }
-  if (smx::grid_sum_last<4>(v, P.red, scratch) && threadIdx.x == 0) {
-    const double g = v[0], d = v[1];
-    P.st[ST_ALPHA] = (d != 0.0) ? g / d : 0.0;
-    P.st[ST_BETA] = 0.0;
-    P.st[ST_GAMMA] = g;
-    P.st[ST_RR] = v[2];
-    P.st[ST_SUMX] = v[3];
-    if (v[2] <= P.st[ST_TOL2] || g == 0.0) P.st[ST_DONE] = 1.0;
-    if (!(d > 0.0) && !(v[2] <= P.st[ST_TOL2])) {
-      P.st[ST_DONE] = 1.0;
-      P.st[ST_FAIL] = 1.0;
-    }
-  }
+  publish_partials<BS, 4>(v, P.part[0], P.G, scr);
 }
 
-__global__ __launch_bounds__(kBlock) void cg_iter_kernel(int64_t n,
-                                                         const int64_t* __restrict__ sp,
-                                                         const int32_t* __restrict__ cols,
-                                                         const double* __restrict__ vals,
-                                                         const double* __restrict__ dinv,
-                                                         double* __restrict__ x, CGPtrs P,
-                                                         int parity) {
-  __shared__ double scratch[4 * kBlock / 64];
-  if (P.st[ST_DONE] != 0.0) return;  // uniform over the grid
-  const double alpha = P.st[ST_ALPHA];
-  const double beta = P.st[ST_BETA];
-  const int q = parity & 1;
+template <int BS, int WMAX>
+__global__ __launch_bounds__(BS) void cg_iter_kernel(int64_t n, int32_t W, int64_t ld,
+                                                     const int32_t* __restrict__ cols,
+                                                     const double* __restrict__ vals,
+                                                     const double* __restrict__ dinv,
+                                                     double* __restrict__ x, CGPtrs P, int32_t it,
+                                                     int32_t max_it) {
+  __shared__ double scr[BS / 64 * 4];
+  if (P.ss[SS_DONE] != 0.0) return;  // uniform: written by an earlier launch
+  const int q = it & 1;
   const double* __restrict__ rq = P.r[q];
   const double* __restrict__ wq = P.w[q];
   const double* __restrict__ sq = P.s[q];
-  const int64_t row = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t row = (int64_t)blockIdx.x * BS + threadIdx.x;
+  // own-row and index loads do not depend on the scalars: issue them first
+  int32_t c[WMAX];
+  double av[WMAX];
+  double dk = 0.0, r0 = 0.0, w0 = 0.0, s0 = 0.0, p0 = 0.0, x0 = 0.0;
+  if (row < n) {
+#pragma unroll
+    for (int j = 0; j < WMAX; ++j) {
+      c[j] = j < W ? cols[j * ld + row] : 0;
+      av[j] = j < W ? vals[j * ld + row] : 0.0;
+    }
+    dk = dinv[row];
+    r0 = rq[row];
+    w0 = wq[row];
+    s0 = sq[row];
+    p0 = P.p[row];
+    x0 = x[row];
+  }
+  double t[4];
+  reduce_partials<BS, 4>(P.part[q], P.G, t, scr);
+  const double gam = t[0], del = t[1], rr = t[2];
+  const double tol2 = P.ss[SS_TOL2];
+  double alpha, beta;
+  if (it == 0) {
+    beta = 0.0;
+    alpha = gam / del;
+  } else {
+    const double g_old = P.st[q][0], a_old = P.st[q][1];
+    beta = gam / g_old;
+    alpha = gam / (del - beta * gam / a_old);
+  }
+  const bool conv = rr <= tol2;
+  const bool fail = !conv && !(gam > 0.0 && isfinite(alpha));
+  const bool stop = conv || fail || it >= max_it;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    P.ss[SS_ITERS] = (double)it;
+    P.ss[SS_RR] = rr;
+    P.ss[SS_SUMX] = t[3];
+    if (stop) {
+      P.ss[SS_DONE] = 1.0;
+      P.ss[SS_FAIL] = fail ? 1.0 : (conv ? 0.0 : 2.0);
+    } else {
+      P.st[1 - q][0] = gam;
+      P.st[1 - q][1] = alpha;
+    }
+  }
+  if (stop) return;
   double v[4] = {0.0, 0.0, 0.0, 0.0};
   if (row < n) {
-    int64_t start, w;
-    int lane;
-    row_span(row, sp, start, w, lane);
-    const double dk = dinv[row];
-    const double s_n = wq[row] + beta * sq[row];
-    const double p_n = rq[row] * dk + beta * P.p[row];
-    const double x_n = x[row] + alpha * p_n;
-    const double r_n = rq[row] - alpha * s_n;
-    const double u_n = r_n * dk;
     double acc = 0.0;
-    for (int64_t j = 0; j < w; ++j) {
-      const int64_t e = start + j * 64 + lane;
-      const int64_t c = cols[e];
-      const double rc = rq[c] - alpha * (wq[c] + beta * sq[c]);
-      acc = fma(vals[e], rc * dinv[c], acc);
+#pragma unroll
+    for (int j = 0; j < WMAX; ++j) {
+      const int32_t cc = c[j];
+      const double rc = (rq[cc] - alpha * (wq[cc] + beta * sq[cc])) * dinv[cc];
+      acc = fma(av[j], rc, acc);
     }
+    const double s_n = w0 + beta * s0;
+    const double p_n = r0 * dk + beta * p0;
+    const double x_n = x0 + alpha * p_n;
+    const double r_n = r0 - alpha * s_n;
+    const double u_n = r_n * dk;
     P.s[1 - q][row] = s_n;
     P.r[1 - q][row] = r_n;
     P.w[1 - q][row] = acc;
@@ -211,124 +292,142 @@ __global__ __launch_bounds__(kBlock) void cg_iter_kernel(int64_t n,
     v[2] = r_n * r_n;
     v[3] = x_n;
   }
-  if (smx::grid_sum_last<4>(v, P.red, scratch) && threadIdx.x == 0) {
-    const double g_new = v[0], d_new = v[1];
-    const double g_old = P.st[ST_GAMMA];
-    const double b_new = g_new / g_old;
-    const double den = d_new - b_new * g_new / alpha;
-    const double a_new = g_new / den;
-    P.st[ST_ALPHA] = a_new;
-    P.st[ST_BETA] = b_new;
-    P.st[ST_GAMMA] = g_new;
-    P.st[ST_RR] = v[2];
-    P.st[ST_SUMX] = v[3];
-    P.st[ST_ITERS] = P.st[ST_ITERS] + 1.0;
-    if (v[2] <= P.st[ST_TOL2]) {
-      P.st[ST_DONE] = 1.0;
-    } else if (!(g_new > 0.0) || !(den != 0.0) || !isfinite(a_new)) {
-      P.st[ST_DONE] = 1.0;
-      P.st[ST_FAIL] = 1.0;
-    }
-  }
+  publish_partials<BS, 4>(v, P.part[1 - q], P.G, scr);
 }
 
-int64_t pcg_grid(int64_t n) {
-  const int64_t nslice = (n + 63) / 64;
-  return (nslice * 64 + kBlock - 1) / kBlock;
+int block_size_for(int64_t n) { return n <= 65536 ? 256 : 512; }
+
+int64_t grid_for_n(int64_t n) {
+  const int bs = block_size_for(n);
+  return (n + bs - 1) / bs;
+}
+
+struct Layout {
+  int64_t G, nvec;
+};
+
+Layout layout(int64_t n) {
+  Layout L;
+  L.G = grid_for_n(n);
+  L.nvec = (n + 63) / 64 * 64;
+  return L;
+}
+
+template <int BS, int WMAX>
+int run_pcg(int64_t n, int32_t W, int64_t ld, const int32_t* cols, const double* vals,
+            const double* dinv, int32_t mode, const double* b, const double* b_part,
+            int64_t nb_part, int32_t project, int32_t raw_W, int64_t raw_ld,
+            const int32_t* raw_cols, const double* raw_vals, const int32_t* bpos, const double* gb,
+            double* x, int32_t warm, double rtol, double atol, int32_t max_it, int32_t check_every,
+            int32_t expected_its, CGPtrs P, hipStream_t st, double* hst) {
+  const int G = P.G;
+  if (mode == 0) {
+    hipLaunchKernelGGL((cg_init_a_kernel<BS, WMAX>), dim3(G), dim3(BS), 0, st, n, mode, W, ld,
+                       cols, vals, b, b_part, nb_part, project, bpos, gb, x, warm, P);
+  } else {
+    hipLaunchKernelGGL((cg_init_a_kernel<BS, WMAX>), dim3(G), dim3(BS), 0, st, n, mode, raw_W,
+                       raw_ld, raw_cols, raw_vals, b, b_part, nb_part, project, bpos, gb, x, warm,
+                       P);
+  }
+  SMX_LAUNCH_CHECK();
+  hipLaunchKernelGGL((cg_init_b_kernel<BS, WMAX>), dim3(G), dim3(BS), 0, st, n, W, ld, cols, vals,
+                     dinv, x, P, rtol, atol);
+  SMX_LAUNCH_CHECK();
+  if (check_every < 1) check_every = 16;
+  int32_t batch = expected_its > 0 ? expected_its + 1 : check_every;
+  int32_t launched = 0;  // iterations launched: it = 0 .. launched-1
+  bool done = false;
+  while (!done && launched <= max_it) {
+    const int32_t nb = batch < (max_it + 1 - launched) ? batch : (max_it + 1 - launched);
+    for (int32_t k = 0; k < nb; ++k) {
+      hipLaunchKernelGGL((cg_iter_kernel<BS, WMAX>), dim3(G), dim3(BS), 0, st, n, W, ld, cols,
+                         vals, dinv, x, P, launched + k, max_it);
+    }
+    SMX_LAUNCH_CHECK();
+    launched += nb;
+    SMX_HIP_CHECK(hipMemcpyAsync(hst, P.ss, sizeof(double) * SS_SIZE, hipMemcpyDeviceToHost, st));
+    SMX_HIP_CHECK(hipStreamSynchronize(st));
+    done = hst[SS_DONE] != 0.0;
+    batch = check_every;
+  }
+  return SMX_OK;
 }
 
 }  // namespace
 
 extern "C" int64_t smx_pcg_ws_size(int64_t n) {
-  const int64_t G = pcg_grid(n) + 1;
-  const int64_t red = smx::red_ws_doubles(G, 4);
-  const int64_t nvec = (n + 31) / 32 * 32;
-  return ST_SIZE + (red + 31) / 32 * 32 + 7 * nvec;
+  const Layout L = layout(n);
+  return HDR + 9 * L.G + 7 * L.nvec + 64;
 }
 
-extern "C" int smx_pcg_jacobi(int64_t n, const int64_t* slice_ptr, const int32_t* cols,
+extern "C" int smx_pcg_jacobi(int64_t n, int32_t width, int64_t ld, const int32_t* cols,
                               const double* vals, const double* dinv, int32_t mode,
-                              const double* b, const double* b_sum, int32_t project,
-                              const int64_t* a_raw_slice_ptr, const int32_t* a_raw_cols,
-                              const double* a_raw_vals, const int32_t* bpos, const double* gb,
-                              double* x, int32_t warm, double rtol, double atol,
-                              int32_t max_it, int32_t check_every, int32_t expected_its,
-                              double* work, int32_t* its_out, double* rnorm_out,
-                              double* sums_out, void* stream) {
+                              const double* b, const double* b_part, int64_t nb_part,
+                              int32_t project, int32_t raw_width, int64_t raw_ld,
+                              const int32_t* raw_cols, const double* raw_vals,
+                              const int32_t* bpos, const double* gb, double* x, int32_t warm,
+                              double rtol, double atol, int32_t max_it, int32_t check_every,
+                              int32_t expected_its, double* work, int32_t* its_out,
+                              double* rnorm_out, double* sums_out, void* stream) {
   SMX_REQUIRE(n >= 0, SMX_ERR_SHAPE, "negative size");
   SMX_REQUIRE(mode == 0 || mode == 1, SMX_ERR_VALUE, "mode must be 0 or 1");
   SMX_REQUIRE(max_it >= 0, SMX_ERR_VALUE, "max_it must be >= 0");
+  SMX_REQUIRE(width >= 1 && width <= 32 && ld >= n, SMX_ERR_SHAPE,
+              "ELL width must be in [1, 32] and ld >= n (width %d, ld %lld)", width, (long long)ld);
+  SMX_REQUIRE(mode == 0 || (raw_width >= 1 && raw_width <= 32 && raw_ld >= n), SMX_ERR_SHAPE,
+              "raw operator width must be in [1, 32]");
   if (its_out) *its_out = 0;
   if (rnorm_out) *rnorm_out = 0.0;
   if (n == 0) return SMX_OK;
-  SMX_REQUIRE(slice_ptr && cols && vals && dinv && x && work, SMX_ERR_VALUE,
-              "null pointer argument");
+  SMX_REQUIRE(cols && vals && dinv && x && work, SMX_ERR_VALUE, "null pointer argument");
   SMX_REQUIRE(mode == 1 || b, SMX_ERR_VALUE, "mode 0 needs b");
-  SMX_REQUIRE(mode == 0 || (a_raw_slice_ptr && a_raw_cols && a_raw_vals && bpos && gb),
-              SMX_ERR_VALUE, "mode 1 needs the raw operator, bpos and gb");
+  SMX_REQUIRE(mode == 0 || (raw_cols && raw_vals && bpos && gb), SMX_ERR_VALUE,
+              "mode 1 needs the raw operator, bpos and gb");
   hipStream_t st = smx::as_stream(stream);
-  const int64_t G = pcg_grid(n);
-  const int64_t red = smx::red_ws_doubles(G + 1, 4);
-  const int64_t nvec = (n + 31) / 32 * 32;
+  const Layout Lo = layout(n);
   CGPtrs P;
-  P.st = work;
-  P.red = work + ST_SIZE;
-  double* v0 = work + ST_SIZE + (red + 31) / 32 * 32;
+  P.G = (int)Lo.G;
+  P.ss = work;
+  P.st[0] = work + ST_OFF;
+  P.st[1] = work + ST_OFF + ST_SIZE;
+  P.part[0] = work + HDR;
+  P.part[1] = work + HDR + 4 * Lo.G;
+  P.pbb = work + HDR + 8 * Lo.G;
+  double* v0 = work + HDR + 9 * Lo.G;
+  v0 += (64 - (((uintptr_t)v0 / sizeof(double)) & 63)) & 63;  // 512-byte aligned vectors
   P.r[0] = v0;
-  P.r[1] = v0 + nvec;
-  P.w[0] = v0 + 2 * nvec;
-  P.w[1] = v0 + 3 * nvec;
-  P.s[0] = v0 + 4 * nvec;
-  P.s[1] = v0 + 5 * nvec;
-  P.p = v0 + 6 * nvec;
-
-  hipLaunchKernelGGL(cg_reset_kernel, dim3(1), dim3(1), 0, st, P.st);
-  SMX_LAUNCH_CHECK();
-  if (mode == 0) {
-    hipLaunchKernelGGL(cg_init_a_kernel, dim3(G), dim3(kBlock), 0, st, n, mode, slice_ptr, cols,
-                       vals, b, b_sum, project, bpos, gb, x, warm, P, rtol, atol);
+  P.r[1] = v0 + Lo.nvec;
+  P.w[0] = v0 + 2 * Lo.nvec;
+  P.w[1] = v0 + 3 * Lo.nvec;
+  P.s[0] = v0 + 4 * Lo.nvec;
+  P.s[1] = v0 + 5 * Lo.nvec;
+  P.p = v0 + 6 * Lo.nvec;
+  double hst[SS_SIZE] = {0};
+  const int bs = block_size_for(n);
+  int rc;
+#define SMX_PCG_CALL(BS, WM)                                                                      \
+  run_pcg<BS, WM>(n, width, ld, cols, vals, dinv, mode, b, b_part, nb_part, project, raw_width,  \
+                  raw_ld, raw_cols, raw_vals, bpos, gb, x, warm, rtol, atol, max_it, check_every, \
+                  expected_its, P, st, hst)
+  const int32_t wmax = (mode == 1 && raw_width > width) ? raw_width : width;
+  if (bs == 256) {
+    rc = wmax <= 8 ? SMX_PCG_CALL(256, 8) : wmax <= 16 ? SMX_PCG_CALL(256, 16) : SMX_PCG_CALL(256, 32);
   } else {
-    hipLaunchKernelGGL(cg_init_a_kernel, dim3(G), dim3(kBlock), 0, st, n, mode, a_raw_slice_ptr,
-                       a_raw_cols, a_raw_vals, b, b_sum, project, bpos, gb, x, warm, P, rtol,
-                       atol);
+    rc = wmax <= 8 ? SMX_PCG_CALL(512, 8) : wmax <= 16 ? SMX_PCG_CALL(512, 16) : SMX_PCG_CALL(512, 32);
   }
-  SMX_LAUNCH_CHECK();
-  hipLaunchKernelGGL(cg_init_b_kernel, dim3(G), dim3(kBlock), 0, st, n, slice_ptr, cols, vals,
-                     dinv, x, P);
-  SMX_LAUNCH_CHECK();
-
-  if (check_every < 1) check_every = 16;
-  int32_t batch = expected_its > 0 ? expected_its : check_every;
-  int32_t launched = 0;
-  double hst[ST_SIZE];
-  bool done = false;
-  SMX_HIP_CHECK(hipMemcpyAsync(hst, P.st, sizeof(double) * 10, hipMemcpyDeviceToHost, st));
-  SMX_HIP_CHECK(hipStreamSynchronize(st));
-  done = hst[ST_DONE] != 0.0;
-  while (!done && launched < max_it) {
-    int32_t nb = batch < (max_it - launched) ? batch : (max_it - launched);
-    for (int32_t t = 0; t < nb; ++t) {
-      hipLaunchKernelGGL(cg_iter_kernel, dim3(G), dim3(kBlock), 0, st, n, slice_ptr, cols, vals,
-                         dinv, x, P, (launched + t) & 1);
-    }
-    SMX_LAUNCH_CHECK();
-    launched += nb;
-    SMX_HIP_CHECK(hipMemcpyAsync(hst, P.st, sizeof(double) * 10, hipMemcpyDeviceToHost, st));
-    SMX_HIP_CHECK(hipStreamSynchronize(st));
-    done = hst[ST_DONE] != 0.0;
-    batch = check_every;
-  }
-  if (its_out) *its_out = (int32_t)hst[ST_ITERS];
-  if (rnorm_out) *rnorm_out = sqrt(hst[ST_RR]);
-  if (sums_out) sums_out[0] = hst[ST_SUMX];
-  if (hst[ST_FAIL] != 0.0) {
-    smx::set_error("CG breakdown after %d iterations (rnorm %g)", (int)hst[ST_ITERS],
-                   sqrt(hst[ST_RR]));
+#undef SMX_PCG_CALL
+  if (rc != SMX_OK) return rc;
+  if (its_out) *its_out = (int32_t)hst[SS_ITERS];
+  if (rnorm_out) *rnorm_out = sqrt(hst[SS_RR]);
+  if (sums_out) sums_out[0] = hst[SS_SUMX];
+  if (hst[SS_FAIL] == 1.0) {
+    smx::set_error("CG breakdown after %d iterations (rnorm %g)", (int)hst[SS_ITERS],
+                   sqrt(hst[SS_RR]));
     return SMX_ERR_NOT_CONVERGED;
   }
-  if (!done) {
+  if (hst[SS_FAIL] == 2.0 || hst[SS_DONE] == 0.0) {
     smx::set_error("CG did not converge in %d iterations (rnorm %g, tol %g)", max_it,
-                   sqrt(hst[ST_RR]), sqrt(hst[ST_TOL2]));
+                   sqrt(hst[SS_RR]), sqrt(hst[SS_TOL2]));
     return SMX_ERR_NOT_CONVERGED;
   }
   return SMX_OK;

@@ -13,7 +13,7 @@ template <int KIND>
 __global__ __launch_bounds__(kBlock) void sell_spmv_kernel(
     int64_t nrows, const int64_t* __restrict__ sp, const int32_t* __restrict__ cols,
     const double* __restrict__ vals, const double* __restrict__ x, double* __restrict__ y,
-    double* __restrict__ sum_out, double* __restrict__ red) {
+    double* __restrict__ partials) {
   __shared__ double scratch[kBlock / 64];
   const int64_t row = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int64_t nslice = (nrows + 63) >> 6;
@@ -57,9 +57,10 @@ __global__ __launch_bounds__(kBlock) void sell_spmv_kernel(
       }
     }
   }
-  if (KIND <= 1 && sum_out) {
+  if (KIND <= 1 && partials) {
     double v[1] = {total};
-    if (smx::grid_sum_last<1>(v, red, scratch) && threadIdx.x == 0) *sum_out = v[0];
+    smx::block_sum<1>(v, scratch);
+    if (threadIdx.x == 0) partials[blockIdx.x] = v[0];
   }
 }
 
@@ -69,13 +70,17 @@ extern "C" int64_t smx_reduce_ws_size(int64_t n) {
   return smx::red_ws_doubles((n + 63) / 64 + 1, 8);
 }
 
+extern "C" int64_t smx_sell_spmv_blocks(int64_t nrows) {
+  const int64_t nslice = (nrows + 63) / 64;
+  return (nslice * 64 + kBlock - 1) / kBlock;
+}
+
 extern "C" int smx_sell_spmv(int64_t nrows, const int64_t* slice_ptr, const int32_t* cols,
                              const double* vals, int32_t kind, const double* x, double* y,
-                             double* sum_out, double* red, void* stream) {
+                             double* partials, void* stream) {
   SMX_REQUIRE(nrows >= 0, SMX_ERR_SHAPE, "negative size");
   SMX_REQUIRE(kind >= 0 && kind <= 3, SMX_ERR_VALUE, "bad kind %d", kind);
-  SMX_REQUIRE(!sum_out || (kind <= 1 && red), SMX_ERR_VALUE,
-              "sum_out needs kind 0/1 and a reduction workspace");
+  SMX_REQUIRE(!partials || kind <= 1, SMX_ERR_VALUE, "partial sums need kind 0 or 1");
   if (nrows == 0) return SMX_OK;
   SMX_REQUIRE(slice_ptr && cols && vals && x && y, SMX_ERR_VALUE, "null pointer argument");
   hipStream_t st = smx::as_stream(stream);
@@ -84,19 +89,19 @@ extern "C" int smx_sell_spmv(int64_t nrows, const int64_t* slice_ptr, const int3
   switch (kind) {
     case 0:
       hipLaunchKernelGGL(sell_spmv_kernel<0>, dim3(grid), dim3(kBlock), 0, st, nrows, slice_ptr,
-                         cols, vals, x, y, sum_out, red);
+                         cols, vals, x, y, partials);
       break;
     case 1:
       hipLaunchKernelGGL(sell_spmv_kernel<1>, dim3(grid), dim3(kBlock), 0, st, nrows, slice_ptr,
-                         cols, vals, x, y, sum_out, red);
+                         cols, vals, x, y, partials);
       break;
     case 2:
       hipLaunchKernelGGL(sell_spmv_kernel<2>, dim3(grid), dim3(kBlock), 0, st, nrows, slice_ptr,
-                         cols, vals, x, y, nullptr, nullptr);
+                         cols, vals, x, y, nullptr);
       break;
     default:
       hipLaunchKernelGGL(sell_spmv_kernel<3>, dim3(grid), dim3(kBlock), 0, st, nrows, slice_ptr,
-                         cols, vals, x, y, nullptr, nullptr);
+                         cols, vals, x, y, nullptr);
       break;
   }
   SMX_LAUNCH_CHECK();

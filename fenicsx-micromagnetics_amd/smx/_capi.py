@@ -19,6 +19,7 @@ SMX_OK, SMX_ERR_SHAPE, SMX_ERR_VALUE, SMX_ERR_NOT_CONVERGED, SMX_ERR_OOM, SMX_ER
 SMX_F64, SMX_F32, SMX_BF16 = 0, 1, 2
 TRI_FIELDS = 34
 LINDHOLM_TILE_COLS = 1024
+PCG_SUMX = 6
 
 DTYPE_CODE = {torch.float64: SMX_F64, torch.float32: SMX_F32, torch.bfloat16: SMX_BF16}
 
@@ -81,11 +82,13 @@ _SIGS = {
     "smx_hmat_matvec": ([C.POINTER(HMatrixArgs), P, P, F64, P, P], C.c_int),
     "smx_dense_bmatvec": ([P, I32, I64, I64, P, P, P, F64, P, P, P], C.c_int),
     "smx_dense_bmatvec_t": ([P, I32, I64, I64, P, P, P, P], C.c_int),
-    "smx_sell_spmv": ([I64, P, P, P, I32, P, P, P, P, P], C.c_int),
+    "smx_sell_spmv": ([I64, P, P, P, I32, P, P, P, P], C.c_int),
+    "smx_sell_spmv_blocks": ([I64], I64),
     "smx_reduce_ws_size": ([I64], I64),
     "smx_pcg_ws_size": ([I64], I64),
-    "smx_pcg_jacobi": ([I64, P, P, P, P, I32, P, P, I32, P, P, P, P, P, P, I32, F64, F64, I32, I32,
-                        I32, P, C.POINTER(I32), C.POINTER(F64), C.POINTER(F64), P], C.c_int),
+    "smx_pcg_jacobi": ([I64, I32, I64, P, P, P, I32, P, P, I64, I32, I32, I64, P, P, P, P, P, I32, F64,
+                        F64, I32, I32, I32, P, C.POINTER(I32), C.POINTER(F64), C.POINTER(F64), P],
+                       C.c_int),
     "smx_field_llg_rhs": ([C.POINTER(FieldArgs), P], C.c_int),
     "smx_llg_jvp": ([C.POINTER(FieldArgs), P, P, P, F64, P], C.c_int),
     "smx_dot3_weighted": ([I64, P, P, P, P, P, P], C.c_int),

@@ -28,7 +28,7 @@ import torch
 
 from . import _capi
 from .boundary import BoundaryGeometry, build_boundary_geometry
-from .device import DeviceSell, default_device, reduce_workspace, to_dev
+from .device import DeviceEll, DeviceSell, default_device, reduce_workspace, to_dev
 from .fem import FEMSystem, assemble_fem, compact_csr, to_sell
 from .mesh import TetMesh
 
@@ -76,10 +76,11 @@ class OperatorSet:
         sv = sell_values(self.fem.graph.indptr, vals, self.N)
         return self.graph.with_vals(sv, k)
 
-    def compact(self, vals: np.ndarray) -> DeviceSell:
+    def compact(self, vals: np.ndarray) -> DeviceEll:
+        """Scalar operator with (near-)zero entries dropped, as ELL for the CG kernels."""
         g = self.fem.graph
         ip, ix, vv = compact_csr(g.indptr, g.indices, vals)
-        return DeviceSell.upload(to_sell(ip, ix, vv, self.N), self.device)
+        return DeviceEll.from_csr(ip, ix, vv, self.N, self.device)
 
 
 class DemagFieldLindholmGPU:
@@ -105,7 +106,8 @@ class DemagFieldLindholmGPU:
                  hmatrix_release_build_memory=True, hmatrix_verbose=False,
                  *, ksp_atol=1e-12, ksp_max_it_u1=2000, ksp_max_it_u2=12000,
                  warm_start=False, device=None, operators: OperatorSet | None = None,
-                 geometry: BoundaryGeometry | None = None, verbose=False):
+                 geometry: BoundaryGeometry | None = None, hmatrix_prebuilt=None,
+                 hmatrix_near_fp32=False, verbose=False):
         if not isinstance(mesh, TetMesh):
             raise TypeError("mesh must be a smx.mesh.TetMesh")
         self.mesh = mesh
@@ -179,6 +181,12 @@ class DemagFieldLindholmGPU:
         t0 = time.perf_counter()
         if requested == "dense":
             self.B_dense_gpu, self.B_ld = assemble_dense_B(geo, dev, self.B_dtype)
+        elif hmatrix_prebuilt is not None:
+            from .hmatrix import HMatrixGPUPackedFused
+            self.B_hmatrix_cpu = hmatrix_prebuilt
+            self.B_hmatrix_gpu = HMatrixGPUPackedFused(hmatrix_prebuilt, use_fp32=bool(hmatrix_use_fp32),
+                                                       device=dev, near_fp32=bool(hmatrix_near_fp32),
+                                                       boundary_gather=geo.bverts)
         else:
             from .hmatrix import build_device_hmatrix
             self.B_hmatrix_cpu, self.B_hmatrix_gpu = build_device_hmatrix(
@@ -186,7 +194,8 @@ class DemagFieldLindholmGPU:
                 leaf_size=int(hmatrix_leaf_size), max_rank=hmatrix_max_rank,
                 max_temporary_block_bytes=int(hmatrix_max_temporary_block_bytes),
                 use_fp32=bool(hmatrix_use_fp32), cache_path=hmatrix_cache_path,
-                force_rebuild=bool(hmatrix_force_rebuild), verbose=bool(hmatrix_verbose))
+                force_rebuild=bool(hmatrix_force_rebuild), verbose=bool(hmatrix_verbose),
+                near_fp32=bool(hmatrix_near_fp32))
         torch.cuda.synchronize(dev)
         self.setup_boundary_seconds = time.perf_counter() - t0
 
@@ -200,7 +209,7 @@ class DemagFieldLindholmGPU:
         self.xb = z(self.Nb)
         self.y_rows = z(self.Nb)
         self.red = reduce_workspace(3 * N + 64, dev)
-        self.b_sum = z(1)
+        self.b_part = z(self.Div.blocks())
         self.ws1 = z(int(L.smx_pcg_ws_size(N)))
         self.ws2 = z(int(L.smx_pcg_ws_size(N)))
         self._have_u = False
@@ -217,19 +226,20 @@ class DemagFieldLindholmGPU:
 
     @property
     def u1_sum_ptr(self) -> int:
-        """Device address of sum(u1) left by the u1 solve (work[8])."""
-        return self.ws1.data_ptr() + 8 * 8
+        """Device address of sum(u1) left by the u1 solve."""
+        return self.ws1.data_ptr() + 8 * _capi.PCG_SUMX
 
-    def _pcg(self, mode, A: DeviceSell, dinv, x, ws, rtol, max_it, b=None, b_sum=None,
+    def _pcg(self, mode, A: DeviceEll, dinv, x, ws, rtol, max_it, b=None, b_part=None,
              gb=None, expected=0, warm=False):
         L = _capi.lib()
         its = C.c_int32(0)
         rn = C.c_double(0.0)
         sums = (C.c_double * 1)()
         raw = self.A_u1
-        rc = L.smx_pcg_jacobi(self.N, _capi.ptr(A.slice_ptr), _capi.ptr(A.cols), _capi.ptr(A.vals),
-                              _capi.ptr(dinv), mode, _capi.ptr(b), _capi.ptr(b_sum), 1,
-                              _capi.ptr(raw.slice_ptr), _capi.ptr(raw.cols), _capi.ptr(raw.vals),
+        nbp = 0 if b_part is None else b_part.numel()
+        rc = L.smx_pcg_jacobi(self.N, A.width, A.ld, _capi.ptr(A.cols), _capi.ptr(A.vals),
+                              _capi.ptr(dinv), mode, _capi.ptr(b), _capi.ptr(b_part), nbp, 1,
+                              raw.width, raw.ld, _capi.ptr(raw.cols), _capi.ptr(raw.vals),
                               _capi.ptr(self.bpos_gpu), _capi.ptr(gb), _capi.ptr(x), int(warm),
                               float(rtol), float(self.ksp_atol), int(max_it), 16, int(expected),
                               _capi.ptr(ws), C.byref(its), C.byref(rn), sums,
@@ -259,16 +269,15 @@ class DemagFieldLindholmGPU:
         """u1, u2 for magnetization m (3N); leaves sum(u1) at u1_sum_ptr."""
         _capi.require_device(m_vec, "m_vec", numel=self.local_size)
         warm = self.warm_start and self._have_u
-        self.Div.spmv(m_vec, self.b_u1, kind=1, sum_out=self.b_sum, red=self.red)
+        self.Div.spmv(m_vec, self.b_u1, kind=1, partials=self.b_part)
         self.last_its_u1 = self._pcg(0, self.A_u1, self.dinv_u1, self.u1, self.ws1,
                                      self.ksp_rtol_u1, self.ksp_max_it_u1, b=self.b_u1,
-                                     b_sum=self.b_sum, expected=self.last_its_u1 if warm else 0,
-                                     warm=warm)
+                                     b_part=self.b_part, expected=self.last_its_u1, warm=warm)
         self.boundary_matvec(self.u1, self.y_rows, gather=self.bverts_gpu,
                              shift_ptr=self.u1_sum_ptr, shift_scale=1.0 / self.N)
         self.last_its_u2 = self._pcg(1, self.A_u2, self.dinv_u2, self.u2, self.ws2,
                                      self.ksp_rtol_u2, self.ksp_max_it_u2, gb=self.y_rows,
-                                     expected=self.last_its_u2 if warm else 0, warm=warm)
+                                     expected=self.last_its_u2, warm=warm)
         self._have_u = True
 
     def field_args(self, m_vec, out_h_demag=None) -> _capi.FieldArgs:

@@ -57,12 +57,46 @@ class DeviceSell:
     def nbytes(self) -> int:
         return int(self.slice_ptr.numel() * 8 + self.cols.numel() * 4 + self.vals.numel() * self.vals.element_size())
 
-    def spmv(self, x: torch.Tensor, y: torch.Tensor, kind: int, sum_out=None, red=None, stream=None):
+    def spmv(self, x: torch.Tensor, y: torch.Tensor, kind: int, partials=None, stream=None):
+        """y = A x; with `partials`, block sums of y land in partials[0:blocks()]."""
         L = _capi.lib()
         _capi.check(L.smx_sell_spmv(self.nrows, _capi.ptr(self.slice_ptr), _capi.ptr(self.cols),
                                     _capi.ptr(self.vals), kind, _capi.ptr(x), _capi.ptr(y),
-                                    _capi.ptr(sum_out), _capi.ptr(red), _capi.stream_ptr(stream)),
+                                    _capi.ptr(partials), _capi.stream_ptr(stream)),
                     "smx_sell_spmv")
+
+    def blocks(self) -> int:
+        return int(_capi.lib().smx_sell_spmv_blocks(self.nrows))
+
+
+@dataclass
+class DeviceEll:
+    """Fixed-width ELL matrix: entry j of row i at cols[j*ld + i]."""
+    nrows: int
+    width: int
+    ld: int
+    cols: torch.Tensor
+    vals: torch.Tensor
+    nnz: int
+
+    @classmethod
+    def from_csr(cls, indptr: np.ndarray, indices: np.ndarray, vals: np.ndarray, nrows: int, device,
+                 dtype=torch.float64):
+        lens = np.diff(indptr).astype(np.int64)
+        W = max(int(lens.max()) if lens.size else 1, 1)
+        ld = (nrows + 63) // 64 * 64
+        cols = np.empty((W, ld), dtype=np.int32)
+        cols[:] = np.minimum(np.arange(ld), max(nrows - 1, 0))[None, :]
+        v = np.zeros((W, ld))
+        r = np.repeat(np.arange(nrows, dtype=np.int64), lens)
+        j = np.arange(indices.size, dtype=np.int64) - np.repeat(indptr[:-1], lens)
+        cols[j, r] = indices
+        v[j, r] = vals
+        return cls(nrows=nrows, width=W, ld=ld, cols=to_dev(cols.ravel(), device),
+                   vals=to_dev(v.ravel(), device, dtype), nnz=int(indices.size))
+
+    def nbytes(self) -> int:
+        return int(self.cols.numel() * 4 + self.vals.numel() * self.vals.element_size())
 
 
 def sell_positions(indptr: np.ndarray, nrows: int):

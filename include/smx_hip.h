@@ -196,12 +196,13 @@ int smx_hmat_matvec(const smx_hmatrix* H, const double* src, const double* shift
  * kind 1: y_i = sum_j a_ij . x_j   (x 3-vector)    (k = 3, divergence)
  * kind 2: y_i = sum_j a_ij x_j     (y 3-vector)    (k = 3, gradient)
  * kind 3: y_i = sum_j A_ij x_j     (3x3 blocks)    (k = 9)
- * When sum_out != NULL, *sum_out = sum_i y_i (kind 0/1 only) via the
- * last-block reduction; partials/counter come from `red` (smx_reduce_ws_size).
+ * When partials != NULL (kind 0/1), block b of the launch writes the sum of
+ * its rows of y to partials[b]; smx_sell_spmv_blocks(nrows) blocks are used.
  * ---------------------------------------------------------------------- */
 int smx_sell_spmv(int64_t nrows, const int64_t* slice_ptr, const int32_t* cols,
                   const double* vals, int32_t kind, const double* x, double* y,
-                  double* sum_out, double* red, void* stream);
+                  double* partials, void* stream);
+int64_t smx_sell_spmv_blocks(int64_t nrows);
 
 /* Doubles needed by a reduction workspace for n rows (partials + counter). */
 int64_t smx_reduce_ws_size(int64_t n);
@@ -209,29 +210,34 @@ int64_t smx_reduce_ws_size(int64_t n);
 /* ---------------------------------------------------------------------- *
  * (c) Jacobi-preconditioned CG (Chronopoulos-Gear form: one fused
  * SpMV + update + 4-dot kernel per iteration, device-side convergence test,
- * fixed-order reductions).  Replaces PETSc KSP cg (+gamg) of the two Poisson
- * solves (Demag_Lindholm_GPU.py:216-226, 271-281, 569-640).
+ * fixed-order reductions without atomics).  Replaces PETSc KSP cg (+gamg) of
+ * the two Poisson solves (Demag_Lindholm_GPU.py:216-226, 271-281, 569-640).
  *
- * mode 0 (u1, Neumann): solves A x = b - mean(b)*project; x0 = x if warm else 0.
- *     On return sums[0] = sum(x) (for the nullspace projection of x).
+ * Matrices are ELL: entry j of row i at cols[j*ld + i] / vals[j*ld + i],
+ * width <= 32, padding entries with value 0 and a valid column.
+ * mode 0 (u1, Neumann): solves A x = b - mean(b)*project, where mean(b) is
+ *     (sum of the nb_part device partial sums b_part) / n; x0 = x if warm
+ *     else 0.
  * mode 1 (u2, Dirichlet lifting): A = A_bc; boundary rows take the data
  *     x_k = gb[bpos[k]] where bpos[k] >= 0; interior rows solve
- *     sum_j A_raw_kj x_j = 0.  `a_raw_*` is the raw stiffness (same n).
+ *     sum_j A_raw_kj x_j = 0 (raw stiffness in ELL, raw_width <= 32).
  * Convergence: ||r||_2 <= max(rtol*||b||_2, atol), b the (lifted) right side.
- * work: smx_pcg_ws_size(n) doubles (zero-filled once at allocation).
- * its_out / rnorm_out / sums: host pointers filled on return (the call
+ * work: smx_pcg_ws_size(n) doubles.  On return work[SMX_PCG_SUMX] holds
+ * sum(x) on the device (the nullspace projection of x, read by later kernels).
+ * its_out / rnorm_out / sums_out: host pointers filled on return (the call
  * synchronizes `stream` every `check_every` iterations).
  * ---------------------------------------------------------------------- */
+#define SMX_PCG_SUMX 6
 int64_t smx_pcg_ws_size(int64_t n);
-int smx_pcg_jacobi(int64_t n, const int64_t* slice_ptr, const int32_t* cols,
+int smx_pcg_jacobi(int64_t n, int32_t width, int64_t ld, const int32_t* cols,
                    const double* vals, const double* dinv, int32_t mode,
-                   const double* b, const double* b_sum, int32_t project,
-                   const int64_t* a_raw_slice_ptr, const int32_t* a_raw_cols,
-                   const double* a_raw_vals, const int32_t* bpos, const double* gb,
-                   double* x, int32_t warm, double rtol, double atol,
-                   int32_t max_it, int32_t check_every, int32_t expected_its,
-                   double* work, int32_t* its_out, double* rnorm_out,
-                   double* sums_out, void* stream);
+                   const double* b, const double* b_part, int64_t nb_part,
+                   int32_t project, int32_t raw_width, int64_t raw_ld,
+                   const int32_t* raw_cols, const double* raw_vals,
+                   const int32_t* bpos, const double* gb, double* x, int32_t warm,
+                   double rtol, double atol, int32_t max_it, int32_t check_every,
+                   int32_t expected_its, double* work, int32_t* its_out,
+                   double* rnorm_out, double* sums_out, void* stream);
 
 /* ---------------------------------------------------------------------- *
  * (d) Fused effective field + LLG right-hand side.

@@ -125,7 +125,6 @@ def test_dense_matvec_dtypes(c1, dtype):
 def test_sell_spmv_kinds(c1):
     torch = _torch()
     from smx.demag import OperatorSet
-    from smx.device import reduce_workspace
     m, _, _ = c1
     ops = OperatorSet(m, torch.device("cuda"))
     f = ops.fem
@@ -135,17 +134,17 @@ def test_sell_spmv_kinds(c1):
     x1 = rng.standard_normal(f.N)
     x3 = rng.standard_normal(3 * f.N)
     dev = torch.device("cuda")
-    red = reduce_workspace(3 * f.N, dev)
-    s = torch.zeros(1, dtype=torch.float64, device=dev)
     A = sp.csr_matrix((f.stiff, (r, g.indices)), shape=(f.N, f.N))
     y = torch.zeros(f.N, dtype=torch.float64, device=dev)
-    ops.graph_values(f.stiff, 1).spmv(torch.from_numpy(x1).cuda(), y, 0, sum_out=s, red=red)
+    As = ops.graph_values(f.stiff, 1)
+    s = torch.zeros(As.blocks(), dtype=torch.float64, device=dev)
+    As.spmv(torch.from_numpy(x1).cuda(), y, 0, partials=s)
     torch.cuda.synchronize()
     assert rel_l2(y.cpu().numpy(), A @ x1) <= 1e-14
-    assert abs(s.item() - (A @ x1).sum()) <= 1e-10 * np.abs(A @ x1).sum()
+    assert abs(s.sum().item() - (A @ x1).sum()) <= 1e-10 * np.abs(A @ x1).sum()
     Div = sp.csr_matrix((f.div.reshape(-1), (np.repeat(r, 3), (3 * g.indices[:, None] + np.arange(3)).ravel())),
                         shape=(f.N, 3 * f.N))
-    ops.graph_values(f.div, 3).spmv(torch.from_numpy(x3).cuda(), y, 1, sum_out=s, red=red)
+    ops.graph_values(f.div, 3).spmv(torch.from_numpy(x3).cuda(), y, 1, partials=s)
     torch.cuda.synchronize()
     assert rel_l2(y.cpu().numpy(), Div @ x3) <= 1e-13
     G = sp.csr_matrix((f.grad.reshape(-1), ((3 * r[:, None] + np.arange(3)).ravel(), np.repeat(g.indices, 3))),
