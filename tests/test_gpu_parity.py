@@ -310,3 +310,21 @@ def test_dp5_fixed_steps_c1(c1):
     d = rel_l2(y.cpu().numpy(), mv)
     print(f"[dp5 c1] rel err {e:.3e} (change {d:.3e})", flush=True)
     assert e <= 1e-10 * max(1.0, d / 1e-3)
+
+
+def test_demag_against_golden_fixture():
+    """Device FP64 demag field vs the committed oracle fixture (40 nm cube)."""
+    from pathlib import Path
+    torch = _torch()
+    from smx import mesh
+    from smx.demag import DemagFieldLindholmGPU
+    G = np.load(Path(__file__).parent / "golden" / "cube40_oracle.npz", allow_pickle=False)
+    m = mesh.cube_c1()
+    d = DemagFieldLindholmGPU(m, None, None, 8e5, ksp_rtol_u1=1e-14, ksp_rtol_u2=1e-14, ksp_atol=0.0,
+                              ksp_max_it_u1=5000, ksp_max_it_u2=5000)
+    H = torch.empty(3 * m.num_nodes, dtype=torch.float64, device="cuda")
+    d.compute_vec(torch.from_numpy(G["m"]).cuda(), H)
+    torch.cuda.synchronize()
+    e = rel_l2(H.cpu().numpy(), G["H_demag"])
+    print(f"[golden demag] {e:.3e}", flush=True)
+    assert e <= 1e-10
