@@ -93,6 +93,13 @@ _SIGS = {
     "smx_llg_jvp": ([C.POINTER(FieldArgs), P, P, P, F64, P], C.c_int),
     "smx_dot3_weighted": ([I64, P, P, P, P, P, P], C.c_int),
     "smx_rk_combine": ([I64, P, F64, I32, P, P, P, F64, F64, P, P, P, P], C.c_int),
+    "smx_energy_blocks": ([I64], I64),
+    "smx_energy_cell_forward": ([I64, P, P, P, P, P, P, P, P, P, P], C.c_int),
+    "smx_energy_node_div": ([I64, P, P, P, P, P, P, P], C.c_int),
+    "smx_energy_cell_backward": ([I64, P, P, P, P, P, P, P, P], C.c_int),
+    "smx_energy_node_grad": ([I64, P, P, P, P, P, P, P, P, F64, P, P, P], C.c_int),
+    "smx_vec_glue": ([I32, I64, P, P, P, F64, F64, P, P, F64, P, P], C.c_int),
+    "smx_dot_partials": ([I64, P, P, P, P], C.c_int),
 }
 
 

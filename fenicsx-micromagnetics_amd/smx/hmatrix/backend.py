@@ -27,6 +27,40 @@ from .storage import HMatrixCPUData
 _U_F = {True: torch.float32, False: torch.float64}
 
 
+def _leaf_table(nb, starts_b, stops_b, dev):
+    """Coarsest intervals on which every block's output range is all-or-nothing,
+    cut to <= 64 rows, with the (block, offset) pairs of each interval."""
+    bounds = np.unique(np.concatenate([[0, nb], starts_b, stops_b]))
+    starts, stops = [], []
+    for a, b in zip(bounds[:-1], bounds[1:]):
+        for s in range(int(a), int(b), 64):
+            starts.append(s)
+            stops.append(min(s + 64, int(b)))
+    starts = np.array(starts, dtype=np.int64)
+    stops = np.array(stops, dtype=np.int64)
+    nleaf = starts.size
+    nfar = np.asarray(starts_b).size
+    if nfar:
+        first = np.searchsorted(starts, starts_b)
+        last = np.searchsorted(starts, stops_b)
+        cnt = last - first
+        blk = np.repeat(np.arange(nfar, dtype=np.int64), cnt)
+        leaf = np.repeat(first, cnt) + (np.arange(cnt.sum()) - np.repeat(np.cumsum(cnt) - cnt, cnt))
+        order = np.lexsort((blk, leaf))
+        blk, leaf = blk[order], leaf[order]
+        fe_row = starts[leaf] - np.asarray(starts_b)[blk]
+        fe_ptr = np.zeros(nleaf + 1, dtype=np.int64)
+        np.add.at(fe_ptr, leaf + 1, 1)
+        fe_ptr = np.cumsum(fe_ptr)
+    else:
+        blk = np.zeros(0, np.int64)
+        fe_row = np.zeros(0, np.int64)
+        fe_ptr = np.zeros(nleaf + 1, dtype=np.int64)
+    i32 = lambda a: to_dev(np.asarray(a, dtype=np.int32), dev)
+    i64 = lambda a: to_dev(np.asarray(a, dtype=np.int64), dev)
+    return dict(start=i32(starts), stop=i32(stops), fe_ptr=i64(fe_ptr), fe_block=i32(blk), fe_row=i32(fe_row))
+
+
 class HMatrixGPUPackedFused:
     def __init__(self, data: HMatrixCPUData, threads_per_block: int = 128, use_fp32: bool = True,
                  device=None, near_fp32: bool = False, boundary_gather: np.ndarray | None = None):
@@ -76,40 +110,21 @@ class HMatrixGPUPackedFused:
         i64 = lambda a: to_dev(np.asarray(a, dtype=np.int64), dev)
         self.far = dict(c0=i32(c0), n=i32(n), m=i32(m), rank=i32(rk), u_off=i64(u_off), v_off=i64(v_off),
                         z_off=i64(z_off))
-        # leaf table
-        bounds = np.unique(np.concatenate([[0, nb], r0, r1]))
-        starts, stops = [], []
-        for a, b in zip(bounds[:-1], bounds[1:]):
-            for s in range(int(a), int(b), 64):
-                starts.append(s)
-                stops.append(min(s + 64, int(b)))
-        starts = np.array(starts, dtype=np.int64)
-        stops = np.array(stops, dtype=np.int64)
-        self.nleaf = starts.size
-        if nfar:
-            first = np.searchsorted(starts, r0)
-            last = np.searchsorted(starts, r1)
-            cnt = last - first
-            blk = np.repeat(np.arange(nfar, dtype=np.int64), cnt)
-            leaf = np.repeat(first, cnt) + (np.arange(cnt.sum()) - np.repeat(np.cumsum(cnt) - cnt, cnt))
-            order = np.lexsort((blk, leaf))
-            blk, leaf = blk[order], leaf[order]
-            fe_row = starts[leaf] - r0[blk]
-            fe_ptr = np.zeros(self.nleaf + 1, dtype=np.int64)
-            np.add.at(fe_ptr, leaf + 1, 1)
-            fe_ptr = np.cumsum(fe_ptr)
-        else:
-            blk = np.zeros(0, np.int64)
-            fe_row = np.zeros(0, np.int64)
-            fe_ptr = np.zeros(self.nleaf + 1, dtype=np.int64)
-        self.leaf = dict(start=i32(starts), stop=i32(stops), fe_ptr=i64(fe_ptr), fe_block=i32(blk),
-                         fe_row=i32(fe_row))
+        self._r = (r0, r1, c0, c1, rk, u_off, v_off, z_off)
+        self.leaf = _leaf_table(nb, r0, r1, dev)
+        self.nleaf = int(self.leaf["start"].numel())
         self.xp = torch.zeros(nb, dtype=torch.float64, device=dev)
         self.z = torch.zeros(max(int(rk.sum()), 1), dtype=torch.float64, device=dev)
         self._H = self._struct(self.gidx_plain)
         self._Hb = self._struct(self.gidx_boundary) if self.gidx_boundary is not None else None
 
-    def _struct(self, gidx):
+    def _struct(self, gidx, transpose=False):
+        """Kernel arguments; with transpose=True the same kernels apply B^T:
+        z_b = U_b^T x[I_b] (U's rank vectors are contiguous, as V's are), the
+        outputs J_b take V_b^T z_b, the near field is the transposed CSR and
+        the leaf table runs over column intervals."""
+        if transpose:
+            return self._struct_t(gidx)
         H = _capi.HMatrixArgs()
         p = _capi.ptr
         H.nb = self.nb
@@ -133,7 +148,58 @@ class HMatrixGPUPackedFused:
         H.xp, H.z = p(self.xp), p(self.z)
         return H
 
+    def _struct_t(self, gidx):
+        if not hasattr(self, "_tt"):
+            dev = self.device
+            r0, r1, c0, c1, rk, u_off, v_off, z_off = self._r
+            nt = self.data.near_csr.T.tocsr()
+            nd = self.near_val.dtype
+            i32 = lambda a: to_dev(np.asarray(a, dtype=np.int32), dev)
+            i64 = lambda a: to_dev(np.asarray(a, dtype=np.int64), dev)
+            self._tt = dict(near_ptr=to_dev(nt.indptr.astype(np.int64), dev),
+                            near_idx=to_dev(nt.indices.astype(np.int32), dev),
+                            near_val=to_dev(nt.data, dev, nd),
+                            far=dict(c0=i32(r0), n=i32(r1 - r0), m=i32(c1 - c0), rank=i32(rk), u_off=i64(v_off),
+                                     v_off=i64(u_off), z_off=i64(z_off)),
+                            leaf=_leaf_table(self.nb, c0, c1, dev))
+        tt = self._tt
+        H = _capi.HMatrixArgs()
+        p = _capi.ptr
+        H.nb = self.nb
+        H.gidx = p(gidx)
+        H.perm = p(self.perm)
+        H.diag_p = p(self.diag_p)
+        H.near_ptr = p(tt["near_ptr"]) if self.near_nnz else None
+        H.near_idx = p(tt["near_idx"]) if self.near_nnz else None
+        H.near_val = p(tt["near_val"]) if self.near_nnz else None
+        H.near_dtype = self.near_dtype
+        H.factor_dtype = self.factor_dtype
+        H.nfar = self.nfar
+        f = tt["far"]
+        H.far_c0, H.far_n, H.far_m, H.far_rank = p(f["c0"]), p(f["n"]), p(f["m"]), p(f["rank"])
+        H.far_u_off, H.far_v_off, H.far_z_off = p(f["u_off"]), p(f["v_off"]), p(f["z_off"])
+        H.U, H.V = p(self.V), p(self.U)
+        lf = tt["leaf"]
+        H.nleaf = int(lf["start"].numel())
+        H.leaf_start, H.leaf_stop = p(lf["start"]), p(lf["stop"])
+        H.leaf_fe_ptr, H.leaf_fe_block, H.leaf_fe_row = p(lf["fe_ptr"]), p(lf["fe_block"]), p(lf["fe_row"])
+        H.xp, H.z = p(self.xp), p(self.z)
+        return H
+
     # ------------------------------------------------------------------
+    def rmatvec(self, x_gpu: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        """y = B^T x (original boundary ordering): the adjoint boundary operator."""
+        if x_gpu.shape != (self.nb,):
+            raise ValueError(f"Expected x.shape == {(self.nb,)}, got {tuple(x_gpu.shape)}.")
+        _capi.require_device(x_gpu, "x")
+        if out is None:
+            out = torch.empty_like(x_gpu)
+        if not hasattr(self, "_HT"):
+            self._HT = self._struct(self.gidx_plain, transpose=True)
+        _capi.check(_capi.lib().smx_hmat_matvec(_capi.byref(self._HT), x_gpu.data_ptr(), None, 0.0,
+                                                out.data_ptr(), _capi.stream_ptr()), "smx_hmat_matvec")
+        return out
+
     def matvec(self, x_gpu: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
         """y = B x for x in the original boundary ordering (backend_fused.py:496-512)."""
         if x_gpu.shape != (self.nb,):

@@ -300,6 +300,41 @@ int smx_rk_combine(int64_t len, const double* y, double dt, int32_t nstage,
                    double atol, double rtol, double* out, double* err_out,
                    double* red, void* stream);
 
+/* ---------------------------------------------------------------------- *
+ * (a22) Energy with per-cell Ms and A and its adjoint (no counterpart in the
+ * reference, which has no adjoint; the energies reduce to its lumped
+ * exchange / anisotropy / demag energies, Exchange_GPU.py:85-92,
+ * Anisotropy_GPU.py:250-275, Demag_Lindholm_GPU.py:706-724, for uniform Ms, A).
+ * Cell data: cells (T x 4 int32), ck = vol grad(l_a).grad(l_b) (T x 16),
+ * cgv = vol/4 grad(l_a) (T x 4 x 3).  inc_ptr/inc: node -> 4*cell + local
+ * vertex incidence.  `partials` receive one sum per 256-entry block
+ * (smx_energy_blocks(n) of them).
+ *   cell_forward:  s_c = Ms_c sum_b m_b;  f_{c,a} = 2e-9 A_c sum_b K_ab m_b;
+ *                  e_c = 1e-9 m_c^T K m_c (= dE/dA_c); partials of A_c e_c
+ *   node_div:      b_a = sum_{(c,l) at a} cgv[c,l] . s_c      (Div_Ms m)
+ *   cell_backward: t_c = Ms_c sum_a y_a cgv[c,a]; dMs_c = (sum_a y_a cgv[c,a]).sum_b m_b
+ *   node_grad:     grad_a = sum (f[c,l] + t_c) - 2e-27 Ku V_a (m_a.n) n; partials of E_an
+ *   vec_glue:      op 0 interior restriction, 1 boundary residual x[idx]-y[idx],
+ *                  2 boundary scatter-add, 3 a(x - sx/nrm) + b(y - sy/nrm)
+ * ---------------------------------------------------------------------- */
+int64_t smx_energy_blocks(int64_t n);
+int smx_energy_cell_forward(int64_t T, const int32_t* cells, const double* ck, const double* Ms,
+                            const double* A, const double* m, double* s, double* f, double* e,
+                            double* partials, void* stream);
+int smx_energy_node_div(int64_t N, const int64_t* inc_ptr, const int32_t* inc, const double* cgv,
+                        const double* s, double* b, double* partials, void* stream);
+int smx_energy_cell_backward(int64_t T, const int32_t* cells, const double* cgv, const double* Ms,
+                             const double* y, const double* m, double* t, double* dMs,
+                             void* stream);
+int smx_energy_node_grad(int64_t N, const int64_t* inc_ptr, const int32_t* inc, const double* f,
+                         const double* t, const double* m, const double* vol, const double* axis,
+                         const double* axis_u, double Ku, double* grad, double* partials,
+                         void* stream);
+int smx_vec_glue(int32_t op, int64_t n, const double* x, const double* y, const int32_t* idx,
+                 double a, double b, const double* sx, const double* sy, double nrm, double* out,
+                 void* stream);
+int smx_dot_partials(int64_t n, const double* x, const double* y, double* partials, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -787,3 +787,81 @@ def build_hmatrix(bd: BoundaryData, eps=1e-6, eta=2.0, leaf_size=64,
     else:
         near = sp.csr_matrix((Nb, Nb))
     return HMatrixOracle(perm=perm, diag_jump=bd.diag_jump, near=near, far=far, stats=stats)
+
+
+# --------------------------------------------------------------------------
+# Energy with per-cell Ms and A, and its gradient from the dense FK map
+# (the adjoint of this build is checked against this; the reference has no
+# adjoint).  E_d = 1/2 mu0 1e-27 b^T L b with b = Div_Ms m; for uniform Ms this
+# is the lumped energy of Demag_Lindholm_GPU.py:706-724 (G = -Div^T/Ms).
+# --------------------------------------------------------------------------
+def div_per_cell(X, tets, Ms_cell):
+    """Div_Ms[a, 3b+c] = sum_{cells} Ms_cell * vol/4 * d_c lambda_a (N x 3N)."""
+    X = np.asarray(X, dtype=np.float64)
+    tets = np.asarray(tets, dtype=np.int64)
+    N = X.shape[0]
+    vol, gr = element_geometry(X, tets)
+    T = tets.shape[0]
+    rows = np.broadcast_to(tets[:, :, None, None], (T, 4, 4, 3))
+    cols = np.broadcast_to(3 * tets[:, None, :, None] + np.arange(3)[None, None, None, :], (T, 4, 4, 3))
+    vals = (np.asarray(Ms_cell) * vol / 4.0)[:, None, None, None] * np.broadcast_to(gr[:, :, None, :], (T, 4, 4, 3))
+    return _coo(rows, cols, vals, (N, 3 * N))
+
+
+def fk_linear_map(dem: "DemagOracle"):
+    """Dense L: b -> phi = u1 + u2 of the FK scheme (columns by LU solves)."""
+    N = dem.N
+    Bm = np.eye(N)
+    Bm -= Bm.mean(axis=0, keepdims=True)  # projected right-hand sides
+    Bm[0, :] = 0.0
+    U1 = dem._lu1.solve(Bm)
+    U1 -= U1.mean(axis=0, keepdims=True)
+    G = np.zeros((N, N))
+    G[dem.bd.bverts, :] = dem.B @ U1[dem.bd.bverts, :]
+    Rhs = -(dem.fem.A @ G)
+    bn = dem.fem.bnodes
+    Rhs[bn, :] = G[bn, :]
+    U2 = dem._lu2.solve(Rhs)
+    return U1 + U2
+
+
+def energy_per_cell(X, tets, m, Ms_cell, A_cell, Ku=0.0, n_ani=None, dem=None, L=None):
+    """(E_total, parts, dE/dm, dE/dMs_cell, dE/dA_cell) in J, FP64, dense L."""
+    X = np.asarray(X, dtype=np.float64)
+    tets = np.asarray(tets, dtype=np.int64)
+    N = X.shape[0]
+    vol, gr = element_geometry(X, tets)
+    Ke = vol[:, None, None] * np.einsum("tak,tbk->tab", gr, gr)
+    mc = m.reshape(-1, 3)[tets]                                 # (T,4,3)
+    quad = np.einsum("tab,tac,tbc->t", Ke, mc, mc)
+    parts = {"E_ex": 1e-9 * float(np.sum(A_cell * quad))}
+    gA = 1e-9 * quad
+    fm = 2e-9 * np.asarray(A_cell)[:, None, None] * np.einsum("tab,tbc->tac", Ke, mc)
+    gm = np.zeros((N, 3))
+    np.add.at(gm, tets.ravel(), fm.reshape(-1, 3))
+    V = np.bincount(tets.ravel(), weights=np.repeat(vol / 4.0, 4), minlength=N)
+    if Ku:
+        n = np.asarray(n_ani, dtype=np.float64).reshape(-1, 3)
+        n = n / np.linalg.norm(n, axis=1)[:, None]
+        n = np.broadcast_to(n, (N, 3))
+        mn = np.einsum("ij,ij->i", m.reshape(-1, 3), n)
+        parts["E_an"] = -1e-27 * Ku * float(np.sum(V * mn * mn))
+        gm += (-2e-27 * Ku * V * mn)[:, None] * n
+    else:
+        parts["E_an"] = 0.0
+    gMs = np.zeros(tets.shape[0])
+    if dem is not None:
+        D = div_per_cell(X, tets, Ms_cell)
+        b = D @ m
+        if L is None:
+            L = fk_linear_map(dem)
+        c = 0.5 * MU0 * 1e-27
+        parts["E_d"] = c * float(b @ (L @ b))
+        gb = c * (L @ b + L.T @ b)
+        gm += (D.T @ gb).reshape(-1, 3)
+        # dE/dMs_c = gb^T (D^c m) = sum_a gb_a vol/4 grad(lambda_a) . sum_b m_b
+        gMs = np.einsum("ta,tak,tk->t", gb[tets], (vol / 4.0)[:, None, None] * gr, mc.sum(axis=1))
+    else:
+        parts["E_d"] = 0.0
+    E = parts["E_ex"] + parts["E_an"] + parts["E_d"]
+    return E, parts, gm.ravel(), gMs, gA

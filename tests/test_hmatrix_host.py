@@ -228,3 +228,32 @@ def test_packed_tables_and_matvec(c1):
     x = np.random.default_rng(0).standard_normal(geo.Nb)
     assert np.abs(_emulate_packed_matvec(P, x) - _matvec(data, x)).max() < 1e-12
     assert P.far_storage_entries == data.far_storage_entries
+
+
+def test_packed_transpose_tables(c1):
+    """The transposed view (factors swapped, column leaves, near^T) reproduces
+    B^T x under the same index emulation."""
+    import torch
+    from smx.hmatrix import HMatrixGPUPackedFused
+    _, geo, bd = c1
+    data = build_hmatrix_device(geo, HMatrixBuildConfig(leaf_size=32), ops=HostOps(geo, bd))
+    P = HMatrixGPUPackedFused(data, use_fp32=False, device=torch.device("cpu"))
+    P._struct(P.gidx_plain, transpose=True)
+    tt = P._tt
+
+    class View:
+        pass
+    v = View()
+    v.nb, v.perm, v.diag_p, v.nfar = P.nb, P.perm, P.diag_p, P.nfar
+    v.U, v.V = P.V, P.U
+    v.far, v.leaf = tt["far"], tt["leaf"]
+    v.nleaf = int(tt["leaf"]["start"].numel())
+    v.near_ptr, v.near_idx, v.near_val = tt["near_ptr"], tt["near_idx"], tt["near_val"]
+    x = np.random.default_rng(1).standard_normal(geo.Nb)
+    xp = x[data.perm]
+    yp = data.diag_jump[data.perm] * xp + data.near_csr.T @ xp
+    for b in data.far_blocks:
+        yp[b.col_start:b.col_stop] += b.V.T @ (b.U.T @ xp[b.row_start:b.row_stop])
+    ref = np.empty_like(x)
+    ref[data.perm] = yp
+    assert np.abs(_emulate_packed_matvec(v, x) - ref).max() < 1e-12
