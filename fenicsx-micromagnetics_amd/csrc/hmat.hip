@@ -149,102 +149,148 @@ __global__ void hm_gather_kernel(int64_t nb, const double* __restrict__ src,
   xp[i] = src[gidx[i]] - sh;
 }
 
-__device__ __forceinline__ double ld_factor(const void* p, int64_t i, int32_t dt) {
-  return dt == SMX_F32 ? (double)reinterpret_cast<const float*>(p)[i]
-                       : reinterpret_cast<const double*>(p)[i];
+// Four consecutive factor entries (16-byte aligned) widened to double.
+__device__ __forceinline__ void load4(const float* __restrict__ p, double (&o)[4]) {
+  const float4 v = *reinterpret_cast<const float4*>(p);
+  o[0] = v.x;
+  o[1] = v.y;
+  o[2] = v.z;
+  o[3] = v.w;
+}
+__device__ __forceinline__ void load4(const double* __restrict__ p, double (&o)[4]) {
+  const double2 a = *reinterpret_cast<const double2*>(p);
+  const double2 b = *reinterpret_cast<const double2*>(p + 2);
+  o[0] = a.x;
+  o[1] = a.y;
+  o[2] = b.x;
+  o[3] = b.y;
+}
+__device__ __forceinline__ void load2(const float* __restrict__ p, double& a, double& b) {
+  const float2 v = *reinterpret_cast<const float2*>(p);
+  a = v.x;
+  b = v.y;
+}
+__device__ __forceinline__ void load2(const double* __restrict__ p, double& a, double& b) {
+  const double2 v = *reinterpret_cast<const double2*>(p);
+  a = v.x;
+  b = v.y;
 }
 
+constexpr int kZG = SMX_HMAT_Z_GROUP;
+static_assert(kZG == 8, "the reduce-scatter below is written for 8 ranks per task");
+
+// z task: kc rank rows of one far block against its column slice of xp.
+// Lanes stride the columns four at a time (16-byte factor loads); each lane
+// keeps kZG partial sums, then a butterfly reduce-scatter (xor 32, 16, 8 halve
+// the set, xor 4, 2, 1 finish the sums) leaves rank g's total in lanes whose
+// bits 5..3 spell g.  Fixed order, so the result is deterministic.  The task
+// record holds everything the wavefront needs in one scalar load.
 template <typename T>
-__global__ __launch_bounds__(256) void hm_far_z_kernel(smx_hmatrix H) {
-  const int64_t b = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+__global__ __launch_bounds__(256, 8) void hm_far_z_kernel(smx_hmatrix H) {
+  const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  if (b >= H.nfar) return;
-  const int32_t c0 = H.far_c0[b], n = H.far_n[b], r = H.far_rank[b];
-  const T* __restrict__ V = reinterpret_cast<const T*>(H.V) + H.far_v_off[b];
-  const double* __restrict__ x = H.xp + c0;
-  double* __restrict__ z = H.z + H.far_z_off[b];
-  int k = 0;
-  for (; k + 4 <= r; k += 4) {
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    for (int j = lane; j < n; j += 64) {
-      const double xj = x[j];
-      s0 = fma((double)V[(int64_t)k * n + j], xj, s0);
-      s1 = fma((double)V[(int64_t)(k + 1) * n + j], xj, s1);
-      s2 = fma((double)V[(int64_t)(k + 2) * n + j], xj, s2);
-      s3 = fma((double)V[(int64_t)(k + 3) * n + j], xj, s3);
-    }
-    s0 = smx::wave_sum(s0);
-    s1 = smx::wave_sum(s1);
-    s2 = smx::wave_sum(s2);
-    s3 = smx::wave_sum(s3);
-    if (lane == 0) {
-      z[k] = s0;
-      z[k + 1] = s1;
-      z[k + 2] = s2;
-      z[k + 3] = s3;
+  if (t >= H.nztask) return;  // wave-uniform; the kernel has no barrier
+  const smx_hmat_ztask task = H.ztask[t];
+  const int32_t n = task.n;
+  const int kc = task.kc;
+  const int64_t ld = ((int64_t)n + 3) & ~(int64_t)3;
+  const T* __restrict__ Vt = reinterpret_cast<const T*>(H.V) + task.v_off;
+  const double* __restrict__ x = H.xp + task.c0;
+  double acc[kZG];
+#pragma unroll
+  for (int g = 0; g < kZG; ++g) acc[g] = 0.0;
+  for (int j = 4 * lane; j < n; j += 256) {
+    double xv[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) xv[q] = x[(j + q) < n ? (j + q) : (n - 1)];  // padded V entries are 0
+#pragma unroll
+    for (int g = 0; g < kZG; ++g) {
+      const int gg = g < kc ? g : kc - 1;  // rows past kc re-read row kc-1; their sums are dropped
+      double v[4];
+      load4(Vt + (int64_t)gg * ld + j, v);
+      acc[g] = fma(v[0], xv[0], acc[g]);
+      acc[g] = fma(v[1], xv[1], acc[g]);
+      acc[g] = fma(v[2], xv[2], acc[g]);
+      acc[g] = fma(v[3], xv[3], acc[g]);
     }
   }
-  for (; k < r; ++k) {
-    double s0 = 0.0;
-    for (int j = lane; j < n; j += 64) s0 = fma((double)V[(int64_t)k * n + j], x[j], s0);
-    s0 = smx::wave_sum(s0);
-    if (lane == 0) z[k] = s0;
-  }
+  const bool h5 = lane & 32, h4 = lane & 16, h3 = lane & 8;
+  double a4[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    a4[i] = (h5 ? acc[4 + i] : acc[i]) + __shfl_xor(h5 ? acc[i] : acc[4 + i], 32, 64);
+  double a2[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+    a2[i] = (h4 ? a4[2 + i] : a4[i]) + __shfl_xor(h4 ? a4[i] : a4[2 + i], 16, 64);
+  double a1 = (h3 ? a2[1] : a2[0]) + __shfl_xor(h3 ? a2[0] : a2[1], 8, 64);
+  a1 += __shfl_xor(a1, 4, 64);
+  a1 += __shfl_xor(a1, 2, 64);
+  a1 += __shfl_xor(a1, 1, 64);
+  const int g = (h5 ? 4 : 0) + (h4 ? 2 : 0) + (h3 ? 1 : 0);
+  if ((lane & 7) == 0 && g < kc) H.z[task.z_off + g] = a1;
 }
 
-template <typename T, typename TN>
-__global__ __launch_bounds__(256) void hm_leaf_kernel(smx_hmatrix H, double* __restrict__ y) {
-  __shared__ double nacc[64];
-  __shared__ double facc[4][64];
+// One workgroup per leaf (leaves are stored largest first, so the dispatcher
+// balances the tail): the leaf's z entries are gathered into LDS in column
+// order, then each wavefront produces whole output rows as one dot product
+// (near pairs against xp, then the packed far row against the staged z).
+template <typename T, typename TN, int NW>
+__global__ __launch_bounds__(NW * 64) void hm_leaf_kernel(smx_hmatrix H, double* __restrict__ y) {
+  extern __shared__ double zl[];
   const int64_t L = blockIdx.x;
   const int lane = threadIdx.x & 63;
   const int wid = threadIdx.x >> 6;
   const int32_t l0 = H.leaf_start[L], l1 = H.leaf_stop[L];
-  const int64_t e0 = H.leaf_fe_ptr[L], e1 = H.leaf_fe_ptr[L + 1];
-  const T* __restrict__ U = reinterpret_cast<const T*>(H.U);
+  const int32_t kw = H.leaf_kw[L];
+  {
+    const int64_t g0 = H.leaf_zg_off[L];
+    const int32_t K = (int32_t)(H.leaf_zg_off[L + 1] - g0);
+    const int32_t* __restrict__ zg = H.leaf_zg + g0;
+    for (int k = threadIdx.x; k < kw; k += NW * 64) zl[k] = k < K ? H.z[zg[k]] : 0.0;
+  }
+  __syncthreads();
+  const T* __restrict__ UL = reinterpret_cast<const T*>(H.U) + H.leaf_u_off[L];
   const TN* __restrict__ NV = reinterpret_cast<const TN*>(H.near_val);
-  for (int32_t rb = l0; rb < l1; rb += 64) {
-    const int rows_here = (l1 - rb) < 64 ? (l1 - rb) : 64;
-    for (int i = wid; i < rows_here; i += 4) {
-      const int64_t row = rb + i;
-      double s = 0.0;
-      if (H.near_ptr) {
-        for (int64_t e = H.near_ptr[row] + lane; e < H.near_ptr[row + 1]; e += 64)
-          s = fma((double)NV[e], H.xp[H.near_idx[e]], s);
-      }
-      s = smx::wave_sum(s);
-      if (lane == 0) nacc[i] = s;
-    }
-    double acc = 0.0;
-    if (lane < rows_here) {
-      for (int64_t e = e0 + wid; e < e1; e += 4) {
-        const int32_t b = H.leaf_fe_block[e];
-        const int64_t o = (int64_t)H.leaf_fe_row[e] + (rb - l0) + lane;
-        const int32_t m = H.far_m[b], r = H.far_rank[b];
-        const T* __restrict__ Ub = U + H.far_u_off[b] + o;
-        const double* __restrict__ zb = H.z + H.far_z_off[b];
-        for (int k = 0; k < r; ++k) acc = fma((double)Ub[(int64_t)k * m], zb[k], acc);
+  const double* __restrict__ xp = H.xp;
+  for (int32_t i = wid; i < l1 - l0; i += NW) {
+    const int64_t p = l0 + i;
+    double s = 0.0;
+    if (H.near_ptr) {
+      const int64_t n1 = H.near_ptr[p + 1];
+      for (int64_t e = H.near_ptr[p] + 2 * lane; e < n1; e += 128) {
+        double v0, v1;
+        load2(NV + e, v0, v1);
+        const int2 c = *reinterpret_cast<const int2*>(H.near_idx + e);
+        s = fma(v0, xp[c.x], s);
+        s = fma(v1, xp[c.y], s);
       }
     }
-    facc[wid][lane] = acc;
-    __syncthreads();
-    if (threadIdx.x < rows_here) {
-      const int i = threadIdx.x;
-      const int64_t p = rb + i;
-      const double v = H.diag_p[p] * H.xp[p] + nacc[i] + facc[0][i] + facc[1][i] + facc[2][i] +
-                       facc[3][i];
-      y[H.perm[p]] = v;
+    const T* __restrict__ Ur = UL + (int64_t)i * kw;
+    for (int k = 4 * lane; k < kw; k += 256) {
+      double u[4];
+      load4(Ur + k, u);
+      s = fma(u[0], zl[k], s);
+      s = fma(u[1], zl[k + 1], s);
+      s = fma(u[2], zl[k + 2], s);
+      s = fma(u[3], zl[k + 3], s);
     }
-    __syncthreads();
+    s = smx::wave_sum(s);
+    if (lane == 0) y[H.perm[p]] = H.diag_p[p] * xp[p] + s;
   }
 }
 
+constexpr int kLeafWaves = 8;
+
 template <typename T>
 void launch_leaf(const smx_hmatrix* H, double* y, hipStream_t st) {
+  const size_t lds = sizeof(double) * (size_t)(H->kw_max > 0 ? H->kw_max : 1);
   if (H->near_dtype == SMX_F32) {
-    hipLaunchKernelGGL((hm_leaf_kernel<T, float>), dim3((unsigned)H->nleaf), dim3(256), 0, st, *H, y);
+    hipLaunchKernelGGL((hm_leaf_kernel<T, float, kLeafWaves>), dim3((unsigned)H->nleaf),
+                       dim3(kLeafWaves * 64), lds, st, *H, y);
   } else {
-    hipLaunchKernelGGL((hm_leaf_kernel<T, double>), dim3((unsigned)H->nleaf), dim3(256), 0, st, *H, y);
+    hipLaunchKernelGGL((hm_leaf_kernel<T, double, kLeafWaves>), dim3((unsigned)H->nleaf),
+                       dim3(kLeafWaves * 64), lds, st, *H, y);
   }
 }
 
@@ -267,24 +313,27 @@ extern "C" int smx_aca_batched(double* A, const smx_aca_task* tasks, int64_t nta
 extern "C" int smx_hmat_matvec(const smx_hmatrix* H, const double* src, const double* shift,
                                double shift_scale, double* y, void* stream) {
   SMX_REQUIRE(H, SMX_ERR_VALUE, "null H");
-  SMX_REQUIRE(H->nb >= 0 && H->nfar >= 0 && H->nleaf >= 0, SMX_ERR_SHAPE, "negative size");
+  SMX_REQUIRE(H->nb >= 0 && H->nleaf >= 0 && H->nztask >= 0, SMX_ERR_SHAPE, "negative size");
   if (H->nb == 0) return SMX_OK;
   SMX_REQUIRE(src && y && H->gidx && H->perm && H->diag_p && H->xp && H->leaf_start &&
-                  H->leaf_stop && H->leaf_fe_ptr,
+                  H->leaf_stop && H->leaf_kw && H->leaf_u_off && H->leaf_zg_off && H->U,
               SMX_ERR_VALUE, "null pointer argument");
   SMX_REQUIRE(H->factor_dtype == SMX_F32 || H->factor_dtype == SMX_F64, SMX_ERR_TYPE,
               "factor dtype must be FP32 or FP64");
   SMX_REQUIRE(H->near_dtype == SMX_F32 || H->near_dtype == SMX_F64, SMX_ERR_TYPE,
               "near dtype must be FP32 or FP64");
-  SMX_REQUIRE(H->nfar == 0 || (H->far_c0 && H->far_n && H->far_m && H->far_rank && H->far_u_off &&
-                               H->far_v_off && H->far_z_off && H->U && H->V && H->z),
-              SMX_ERR_VALUE, "far-field arrays missing");
+  SMX_REQUIRE(H->nztask == 0 || (H->ztask && H->V && H->z && H->leaf_zg), SMX_ERR_VALUE,
+              "far-field arrays missing");
+  SMX_REQUIRE(H->kw_max >= 0 && H->kw_max <= SMX_HMAT_KW_MAX && (H->kw_max & 3) == 0, SMX_ERR_SHAPE,
+              "kw_max %d must be a multiple of 4 in [0, %d]", H->kw_max, SMX_HMAT_KW_MAX);
+  SMX_REQUIRE(H->nleaf <= 0x7fffffffLL && H->nztask / 4 < 0x7fffffffLL, SMX_ERR_SHAPE,
+              "too many leaves or z tasks");
   hipStream_t st = smx::as_stream(stream);
   hipLaunchKernelGGL(hm_gather_kernel, dim3(smx::grid_for(H->nb, 256)), dim3(256), 0, st, H->nb,
                      src, H->gidx, shift, shift_scale, H->xp);
   SMX_LAUNCH_CHECK();
-  if (H->nfar > 0) {
-    const int g = smx::grid_for(H->nfar, 4);
+  if (H->nztask > 0) {
+    const unsigned g = (unsigned)((H->nztask + 3) / 4);
     if (H->factor_dtype == SMX_F32)
       hipLaunchKernelGGL(hm_far_z_kernel<float>, dim3(g), dim3(256), 0, st, *H);
     else

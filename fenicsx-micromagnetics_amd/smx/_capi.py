@@ -20,6 +20,8 @@ SMX_F64, SMX_F32, SMX_BF16 = 0, 1, 2
 TRI_FIELDS = 34
 LINDHOLM_TILE_COLS = 1024
 PCG_SUMX = 6
+HMAT_Z_GROUP = 8
+HMAT_KW_MAX = 16384
 
 DTYPE_CODE = {torch.float64: SMX_F64, torch.float32: SMX_F32, torch.bfloat16: SMX_BF16}
 
@@ -63,10 +65,10 @@ class HMatrixArgs(C.Structure):
     _fields_ = [
         ("nb", I64), ("gidx", P), ("perm", P), ("diag_p", P),
         ("near_ptr", P), ("near_idx", P), ("near_val", P), ("near_dtype", I32), ("factor_dtype", I32),
-        ("nfar", I64), ("far_c0", P), ("far_n", P), ("far_m", P), ("far_rank", P),
-        ("far_u_off", P), ("far_v_off", P), ("far_z_off", P), ("U", P), ("V", P),
-        ("nleaf", I64), ("leaf_start", P), ("leaf_stop", P), ("leaf_fe_ptr", P),
-        ("leaf_fe_block", P), ("leaf_fe_row", P), ("xp", P), ("z", P),
+        ("nztask", I64), ("ztask", P), ("V", P),
+        ("nleaf", I64), ("leaf_start", P), ("leaf_stop", P), ("leaf_u_off", P), ("leaf_kw", P),
+        ("leaf_zg_off", P), ("leaf_zg", P), ("kw_max", I32), ("pad", I32),
+        ("U", P), ("xp", P), ("z", P),
     ]
 
 

@@ -1,21 +1,29 @@
 """Packed H-matrix on the device and its matvec (HMatrixGPUPackedFused).
 
 Same constructor / ``matvec(x, out=None)`` / ``benchmark`` / ``memory_report``
-surface as lindholm_hmatrix_gpu/backend_fused.py:359-646.  Device layout:
+surface as lindholm_hmatrix_gpu/backend_fused.py:359-646.  Device layout
+(every factor load of the kernels is 16 bytes and aligned):
 
-* far factors of every block packed into two flat arrays (FP32 by default):
-  U column-major (m x r: one contiguous m-vector per rank, so a wavefront of
-  64 output rows reads 64 consecutive entries), V row-major (r x n);
+* V of every far block row-major (r x n), each row padded to a multiple of 4
+  entries, so ``z_b = V_b x`` streams whole 16-byte vectors; the z stage runs
+  one wavefront per (block, group of 8 ranks) task, each task a 32-byte
+  record with its offsets;
+* leaves: the coarsest row intervals on which every far block is either fully
+  present or absent (cut to <= 64 rows).  Each leaf owns one dense row-major
+  matrix whose columns are the ranks of all far blocks over its rows (in block
+  order), rows padded to a multiple of 4 entries.  The leaf's output row is a
+  single dot product of that row with the concatenated z slices (gathered
+  into LDS through a per-leaf index list), plus its near row, so one
+  workgroup produces each output row without atomics;
 * near field as CSR in permuted coordinates (FP64 values, int32 columns, the
-  exact zeros dropped as storage.py:453-459 does);
-* a leaf table: the coarsest row intervals on which every far block is either
-  fully present or absent (cut to <= 64 rows), each with its list of
-  (far block, row offset) pairs, so one workgroup produces each output row
-  without atomics.
+  exact zeros dropped as storage.py:453-459 does), every row padded to an even
+  entry count so values and columns load in aligned pairs.
+
+The transposed operator (``rmatvec``) is the same layout built from the
+swapped factors (U_b^T as the z-stage matrix, V_b^T as the leaf rows, column
+leaves, near^T), packed on first use.
 """
 from __future__ import annotations
-
-import time
 
 import numpy as np
 import torch
@@ -24,41 +32,145 @@ from .. import _capi
 from ..device import to_dev
 from .storage import HMatrixCPUData
 
-_U_F = {True: torch.float32, False: torch.float64}
+_ZG = _capi.HMAT_Z_GROUP
 
 
-def _leaf_table(nb, starts_b, stops_b, dev):
-    """Coarsest intervals on which every block's output range is all-or-nothing,
-    cut to <= 64 rows, with the (block, offset) pairs of each interval."""
-    bounds = np.unique(np.concatenate([[0, nb], starts_b, stops_b]))
+def _round4(a):
+    return (np.asarray(a, dtype=np.int64) + 3) // 4 * 4
+
+
+def leaf_intervals(nb, starts_b, stops_b):
+    """Coarsest intervals on which every block's row range is all-or-nothing,
+    cut to <= 64 rows."""
+    bounds = np.unique(np.concatenate([[0, nb], np.asarray(starts_b, np.int64), np.asarray(stops_b, np.int64)]))
     starts, stops = [], []
     for a, b in zip(bounds[:-1], bounds[1:]):
         for s in range(int(a), int(b), 64):
             starts.append(s)
             stops.append(min(s + 64, int(b)))
-    starts = np.array(starts, dtype=np.int64)
-    stops = np.array(stops, dtype=np.int64)
-    nleaf = starts.size
-    nfar = np.asarray(starts_b).size
+    return np.array(starts, dtype=np.int64), np.array(stops, dtype=np.int64)
+
+
+ZTASK_DTYPE = np.dtype([("v_off", np.int64), ("z_off", np.int64), ("c0", np.int32), ("n", np.int32),
+                        ("kc", np.int32), ("pad", np.int32)])   # smx_hmat_ztask, 32 bytes
+
+
+def pack_operator(nb, row_start, row_stop, col_start, col_stop, rank, Ufac, Vfac, near_csr, fdt, near_dt):
+    """Host arrays of the packed layout for y = D x + Near x + sum_b U_b V_b x.
+
+    Ufac[b] is (m_b x r_b), Vfac[b] is (r_b x n_b).  Returns a dict of numpy
+    arrays (all offsets in entries) plus the unpadded storage counts.  Work
+    items (z tasks, leaves) are stored largest first: independent items in any
+    order give the same result, and largest-first lets the dispatcher balance
+    the tail."""
+    r0 = np.asarray(row_start, dtype=np.int64)
+    r1 = np.asarray(row_stop, dtype=np.int64)
+    c0 = np.asarray(col_start, dtype=np.int64)
+    c1 = np.asarray(col_stop, dtype=np.int64)
+    rk = np.asarray(rank, dtype=np.int64)
+    nfar = rk.size
+    m, n = r1 - r0, c1 - c0
+    fb = np.dtype(fdt).itemsize
+    nv = np.dtype(near_dt).itemsize
+    P = {}
+    # ---- z stage: V rows padded to 4 entries, tasks of <= _ZG rank rows ----
+    ld = _round4(n)
+    v_off = np.zeros(nfar, dtype=np.int64)
     if nfar:
-        first = np.searchsorted(starts, starts_b)
-        last = np.searchsorted(starts, stops_b)
-        cnt = last - first
-        blk = np.repeat(np.arange(nfar, dtype=np.int64), cnt)
-        leaf = np.repeat(first, cnt) + (np.arange(cnt.sum()) - np.repeat(np.cumsum(cnt) - cnt, cnt))
-        order = np.lexsort((blk, leaf))
-        blk, leaf = blk[order], leaf[order]
-        fe_row = starts[leaf] - np.asarray(starts_b)[blk]
-        fe_ptr = np.zeros(nleaf + 1, dtype=np.int64)
-        np.add.at(fe_ptr, leaf + 1, 1)
-        fe_ptr = np.cumsum(fe_ptr)
-    else:
-        blk = np.zeros(0, np.int64)
-        fe_row = np.zeros(0, np.int64)
-        fe_ptr = np.zeros(nleaf + 1, dtype=np.int64)
-    i32 = lambda a: to_dev(np.asarray(a, dtype=np.int32), dev)
-    i64 = lambda a: to_dev(np.asarray(a, dtype=np.int64), dev)
-    return dict(start=i32(starts), stop=i32(stops), fe_ptr=i64(fe_ptr), fe_block=i32(blk), fe_row=i32(fe_row))
+        v_off[1:] = np.cumsum(rk * ld)[:-1]
+    V = np.zeros(max(int((rk * ld).sum()), 4), dtype=fdt)
+    for b in range(nfar):
+        if rk[b]:
+            V[v_off[b]:v_off[b] + rk[b] * ld[b]].reshape(rk[b], ld[b])[:, :n[b]] = Vfac[b]
+    z_off = np.zeros(nfar, dtype=np.int64)
+    if nfar:
+        z_off[1:] = np.cumsum(rk)[:-1]
+    ng = (rk + _ZG - 1) // _ZG
+    tb = np.repeat(np.arange(nfar, dtype=np.int64), ng)
+    k0 = (np.arange(int(ng.sum()), dtype=np.int64) - np.repeat(np.cumsum(ng) - ng, ng)) * _ZG
+    kc = np.minimum(_ZG, rk[tb] - k0)
+    order = np.argsort(-(n[tb] * kc), kind="stable")
+    tb, k0, kc = tb[order], k0[order], kc[order]
+    zt = np.zeros(max(tb.size, 1), dtype=ZTASK_DTYPE)
+    zt["v_off"][:tb.size] = v_off[tb] + k0 * ld[tb]
+    zt["z_off"][:tb.size] = z_off[tb] + k0
+    zt["c0"][:tb.size] = c0[tb]
+    zt["n"][:tb.size] = n[tb]
+    zt["kc"][:tb.size] = kc
+    P.update(V=V, ztask=zt.view(np.int64), nztask=int(tb.size))
+    # ---- leaves ----
+    starts, stops = leaf_intervals(nb, r0, r1)
+    nleaf = starts.size
+    first = np.searchsorted(starts, r0)
+    last = np.searchsorted(starts, r1)
+    cnt = last - first
+    blk = np.repeat(np.arange(nfar, dtype=np.int64), cnt)
+    leaf = np.repeat(first, cnt) + (np.arange(int(cnt.sum()), dtype=np.int64) - np.repeat(np.cumsum(cnt) - cnt, cnt))
+    o = np.lexsort((blk, leaf))
+    blk, leaf = blk[o], leaf[o]                  # (leaf, block) pairs, blocks increasing in a leaf
+    fe_ptr = np.zeros(nleaf + 1, dtype=np.int64)
+    np.add.at(fe_ptr, leaf + 1, 1)
+    fe_ptr = np.cumsum(fe_ptr)
+    rke = rk[blk]
+    cum = np.cumsum(rke) - rke
+    base = np.zeros(nleaf, dtype=np.int64)
+    nonempty = np.diff(fe_ptr) > 0
+    base[nonempty] = cum[fe_ptr[:-1][nonempty]]
+    kpos = cum - base[leaf]                      # first column of each pair in its leaf matrix
+    K = np.zeros(nleaf, dtype=np.int64)
+    np.add.at(K, leaf, rke)
+    kw = _round4(K)
+    cl = stops - starts
+    near = near_csr.tocsr()
+    near.sort_indices()
+    nptr0 = near.indptr.astype(np.int64)
+    work = cl * kw * fb + (nptr0[stops] - nptr0[starts]) * (nv + 4)
+    lo = np.argsort(-work, kind="stable")        # storage slot -> leaf
+    slot = np.empty(nleaf, dtype=np.int64)
+    slot[lo] = np.arange(nleaf)
+    u_off_s = np.zeros(nleaf, dtype=np.int64)
+    if nleaf:
+        u_off_s[1:] = np.cumsum((cl * kw)[lo])[:-1]
+    u_off = u_off_s[slot]                        # per leaf
+    U = np.zeros(max(int((cl * kw).sum()), 4), dtype=fdt)
+    o_rows = starts[leaf] - r0[blk]
+    for e in range(blk.size):
+        b, L = int(blk[e]), int(leaf[e])
+        c, w, orow = int(cl[L]), int(kw[L]), int(o_rows[e])
+        U[u_off[L]:u_off[L] + c * w].reshape(c, w)[:, kpos[e]:kpos[e] + rke[e]] = Ufac[b][orow:orow + c]
+    # z gather lists: column k of leaf L reads z[z_off[b] + j] for its (b, j)
+    zg_leaf = np.repeat(z_off[blk], rke) + (np.arange(int(rke.sum()), dtype=np.int64)
+                                            - np.repeat(np.cumsum(rke) - rke, rke))
+    gptr = np.zeros(nleaf + 1, dtype=np.int64)
+    gptr[1:] = np.cumsum(K)
+    zg = np.concatenate([zg_leaf[gptr[L]:gptr[L + 1]] for L in lo]) if nleaf else np.zeros(0, np.int64)
+    zg_off = np.zeros(nleaf + 1, dtype=np.int64)
+    zg_off[1:] = np.cumsum(K[lo])
+    P.update(leaf_start=starts[lo].astype(np.int32), leaf_stop=stops[lo].astype(np.int32),
+             leaf_u_off=u_off[lo], leaf_kw=kw[lo].astype(np.int32), leaf_zg_off=zg_off,
+             leaf_zg=(zg if zg.size else np.zeros(1, np.int64)).astype(np.int32), U=U,
+             kw_max=int(kw.max()) if nleaf else 0)
+    # ---- near rows in aligned pairs ----
+    cnt = np.diff(nptr0)
+    pc = cnt + (cnt & 1)
+    ptr = np.zeros(nb + 1, dtype=np.int64)
+    ptr[1:] = np.cumsum(pc)
+    idx = np.repeat(np.arange(nb, dtype=np.int64), pc)   # padding: a valid column with value 0
+    val = np.zeros(max(int(ptr[-1]), 2), dtype=near_dt)
+    rowid = np.repeat(np.arange(nb, dtype=np.int64), cnt)
+    dest = ptr[rowid] + (np.arange(near.nnz, dtype=np.int64) - np.repeat(nptr0[:-1], cnt))
+    idx[dest] = near.indices
+    val[dest] = near.data
+    if idx.size == 0:
+        idx = np.zeros(2, dtype=np.int64)
+    P.update(near_ptr=ptr, near_idx=idx.astype(np.int32), near_val=val)
+    P["far_storage_entries"] = int((m * rk).sum() + (n * rk).sum())
+    P["near_nnz"] = int(near.nnz)
+    P["nz"] = int(rk.sum())
+    if P["kw_max"] > _capi.HMAT_KW_MAX:
+        raise ValueError(f"a leaf carries {P['kw_max']} far ranks; the matvec stages at most "
+                         f"{_capi.HMAT_KW_MAX} in LDS")
+    return P
 
 
 class HMatrixGPUPackedFused:
@@ -77,114 +189,64 @@ class HMatrixGPUPackedFused:
         self.gidx_boundary = (to_dev(np.asarray(boundary_gather, dtype=np.int64)[perm].astype(np.int32), dev)
                               if boundary_gather is not None else None)
         self.diag_p = to_dev(np.asarray(data.diag_jump, dtype=np.float64)[perm], dev)
-        near = data.near_csr.tocsr()
-        self.near_nnz = int(near.nnz)
-        self.near_ptr = to_dev(near.indptr.astype(np.int64), dev)
-        self.near_idx = to_dev(near.indices.astype(np.int32), dev)
-        nd = torch.float32 if near_fp32 else torch.float64
-        self.near_val = to_dev(near.data, dev, nd)
-        self.near_dtype = _capi.DTYPE_CODE[nd]
-        fb = data.far_blocks
-        nfar = len(fb)
-        self.nfar = nfar
-        r0 = np.array([b.row_start for b in fb], dtype=np.int64)
-        r1 = np.array([b.row_stop for b in fb], dtype=np.int64)
-        c0 = np.array([b.col_start for b in fb], dtype=np.int64)
-        c1 = np.array([b.col_stop for b in fb], dtype=np.int64)
-        rk = np.array([b.rank for b in fb], dtype=np.int64)
-        m = r1 - r0
-        n = c1 - c0
-        u_off = np.concatenate([[0], np.cumsum(m * rk)[:-1]]).astype(np.int64) if nfar else np.zeros(0, np.int64)
-        v_off = np.concatenate([[0], np.cumsum(n * rk)[:-1]]).astype(np.int64) if nfar else np.zeros(0, np.int64)
-        z_off = np.concatenate([[0], np.cumsum(rk)[:-1]]).astype(np.int64) if nfar else np.zeros(0, np.int64)
-        fdt = np.float32 if self.use_fp32 else np.float64
-        Uflat = (np.concatenate([np.ascontiguousarray(b.U.T, dtype=fdt).ravel() for b in fb])
-                 if nfar else np.zeros(1, fdt))
-        Vflat = (np.concatenate([np.ascontiguousarray(b.V, dtype=fdt).ravel() for b in fb])
-                 if nfar else np.zeros(1, fdt))
-        self.far_storage_entries = int((m * rk).sum() + (n * rk).sum())
-        self.U = to_dev(Uflat, dev)
-        self.V = to_dev(Vflat, dev)
+        self._fdt = np.float32 if self.use_fp32 else np.float64
+        self._ndt = np.float32 if near_fp32 else np.float64
         self.factor_dtype = _capi.SMX_F32 if self.use_fp32 else _capi.SMX_F64
-        i32 = lambda a: to_dev(np.asarray(a, dtype=np.int32), dev)
-        i64 = lambda a: to_dev(np.asarray(a, dtype=np.int64), dev)
-        self.far = dict(c0=i32(c0), n=i32(n), m=i32(m), rank=i32(rk), u_off=i64(u_off), v_off=i64(v_off),
-                        z_off=i64(z_off))
-        self._r = (r0, r1, c0, c1, rk, u_off, v_off, z_off)
-        self.leaf = _leaf_table(nb, r0, r1, dev)
-        self.nleaf = int(self.leaf["start"].numel())
+        self.near_dtype = _capi.SMX_F32 if near_fp32 else _capi.SMX_F64
+        fb = data.far_blocks
+        self.nfar = len(fb)
+        host = pack_operator(nb, [b.row_start for b in fb], [b.row_stop for b in fb],
+                             [b.col_start for b in fb], [b.col_stop for b in fb], [b.rank for b in fb],
+                             [b.U for b in fb], [b.V for b in fb], data.near_csr, self._fdt, self._ndt)
+        self.far_storage_entries = host["far_storage_entries"]
+        self.near_nnz = host["near_nnz"]
+        self.fwd = self._upload(host)
+        self.nleaf = int(self.fwd["leaf_start"].numel())
+        self.U, self.V = self.fwd["U"], self.fwd["V"]
         self.xp = torch.zeros(nb, dtype=torch.float64, device=dev)
-        self.z = torch.zeros(max(int(rk.sum()), 1), dtype=torch.float64, device=dev)
-        self._H = self._struct(self.gidx_plain)
-        self._Hb = self._struct(self.gidx_boundary) if self.gidx_boundary is not None else None
+        self.z = torch.zeros(max(host["nz"], 1), dtype=torch.float64, device=dev)
+        self._H = self._struct(self.fwd, self.gidx_plain)
+        self._Hb = self._struct(self.fwd, self.gidx_boundary) if self.gidx_boundary is not None else None
 
-    def _struct(self, gidx, transpose=False):
-        """Kernel arguments; with transpose=True the same kernels apply B^T:
-        z_b = U_b^T x[I_b] (U's rank vectors are contiguous, as V's are), the
-        outputs J_b take V_b^T z_b, the near field is the transposed CSR and
-        the leaf table runs over column intervals."""
-        if transpose:
-            return self._struct_t(gidx)
+    def _upload(self, host):
+        return {k: (to_dev(v, self.device) if isinstance(v, np.ndarray) else v) for k, v in host.items()}
+
+    def _struct(self, pk, gidx):
         H = _capi.HMatrixArgs()
         p = _capi.ptr
         H.nb = self.nb
         H.gidx = p(gidx)
         H.perm = p(self.perm)
         H.diag_p = p(self.diag_p)
-        H.near_ptr = p(self.near_ptr) if self.near_nnz else None
-        H.near_idx = p(self.near_idx) if self.near_nnz else None
-        H.near_val = p(self.near_val) if self.near_nnz else None
+        has_near = pk["near_nnz"] > 0
+        H.near_ptr = p(pk["near_ptr"]) if has_near else None
+        H.near_idx = p(pk["near_idx"]) if has_near else None
+        H.near_val = p(pk["near_val"]) if has_near else None
         H.near_dtype = self.near_dtype
         H.factor_dtype = self.factor_dtype
-        H.nfar = self.nfar
-        f = self.far
-        H.far_c0, H.far_n, H.far_m, H.far_rank = p(f["c0"]), p(f["n"]), p(f["m"]), p(f["rank"])
-        H.far_u_off, H.far_v_off, H.far_z_off = p(f["u_off"]), p(f["v_off"]), p(f["z_off"])
-        H.U, H.V = p(self.U), p(self.V)
-        lf = self.leaf
-        H.nleaf = self.nleaf
-        H.leaf_start, H.leaf_stop = p(lf["start"]), p(lf["stop"])
-        H.leaf_fe_ptr, H.leaf_fe_block, H.leaf_fe_row = p(lf["fe_ptr"]), p(lf["fe_block"]), p(lf["fe_row"])
+        H.nztask = int(pk["nztask"])
+        H.ztask = p(pk["ztask"])
+        H.V = p(pk["V"])
+        H.nleaf = int(pk["leaf_start"].numel())
+        H.leaf_start, H.leaf_stop = p(pk["leaf_start"]), p(pk["leaf_stop"])
+        H.leaf_u_off, H.leaf_kw = p(pk["leaf_u_off"]), p(pk["leaf_kw"])
+        H.leaf_zg_off, H.leaf_zg = p(pk["leaf_zg_off"]), p(pk["leaf_zg"])
+        H.kw_max = int(pk["kw_max"])
+        H.U = p(pk["U"])
         H.xp, H.z = p(self.xp), p(self.z)
         return H
 
-    def _struct_t(self, gidx):
-        if not hasattr(self, "_tt"):
-            dev = self.device
-            r0, r1, c0, c1, rk, u_off, v_off, z_off = self._r
-            nt = self.data.near_csr.T.tocsr()
-            nd = self.near_val.dtype
-            i32 = lambda a: to_dev(np.asarray(a, dtype=np.int32), dev)
-            i64 = lambda a: to_dev(np.asarray(a, dtype=np.int64), dev)
-            self._tt = dict(near_ptr=to_dev(nt.indptr.astype(np.int64), dev),
-                            near_idx=to_dev(nt.indices.astype(np.int32), dev),
-                            near_val=to_dev(nt.data, dev, nd),
-                            far=dict(c0=i32(r0), n=i32(r1 - r0), m=i32(c1 - c0), rank=i32(rk), u_off=i64(v_off),
-                                     v_off=i64(u_off), z_off=i64(z_off)),
-                            leaf=_leaf_table(self.nb, c0, c1, dev))
-        tt = self._tt
-        H = _capi.HMatrixArgs()
-        p = _capi.ptr
-        H.nb = self.nb
-        H.gidx = p(gidx)
-        H.perm = p(self.perm)
-        H.diag_p = p(self.diag_p)
-        H.near_ptr = p(tt["near_ptr"]) if self.near_nnz else None
-        H.near_idx = p(tt["near_idx"]) if self.near_nnz else None
-        H.near_val = p(tt["near_val"]) if self.near_nnz else None
-        H.near_dtype = self.near_dtype
-        H.factor_dtype = self.factor_dtype
-        H.nfar = self.nfar
-        f = tt["far"]
-        H.far_c0, H.far_n, H.far_m, H.far_rank = p(f["c0"]), p(f["n"]), p(f["m"]), p(f["rank"])
-        H.far_u_off, H.far_v_off, H.far_z_off = p(f["u_off"]), p(f["v_off"]), p(f["z_off"])
-        H.U, H.V = p(self.V), p(self.U)
-        lf = tt["leaf"]
-        H.nleaf = int(lf["start"].numel())
-        H.leaf_start, H.leaf_stop = p(lf["start"]), p(lf["stop"])
-        H.leaf_fe_ptr, H.leaf_fe_block, H.leaf_fe_row = p(lf["fe_ptr"]), p(lf["fe_block"]), p(lf["fe_row"])
-        H.xp, H.z = p(self.xp), p(self.z)
-        return H
+    def transposed_pack(self):
+        """Packed layout of B^T (built on first use): z stage over U_b^T,
+        leaf rows from V_b^T, leaves over column intervals, near^T."""
+        if not hasattr(self, "_tpk"):
+            fb = self.data.far_blocks
+            host = pack_operator(self.nb, [b.col_start for b in fb], [b.col_stop for b in fb],
+                                 [b.row_start for b in fb], [b.row_stop for b in fb], [b.rank for b in fb],
+                                 [np.asarray(b.V).T for b in fb], [np.asarray(b.U).T for b in fb],
+                                 self.data.near_csr.T.tocsr(), self._fdt, self._ndt)
+            self._tpk = self._upload(host)
+        return self._tpk
 
     # ------------------------------------------------------------------
     def rmatvec(self, x_gpu: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
@@ -195,7 +257,7 @@ class HMatrixGPUPackedFused:
         if out is None:
             out = torch.empty_like(x_gpu)
         if not hasattr(self, "_HT"):
-            self._HT = self._struct(self.gidx_plain, transpose=True)
+            self._HT = self._struct(self.transposed_pack(), self.gidx_plain)
         _capi.check(_capi.lib().smx_hmat_matvec(_capi.byref(self._HT), x_gpu.data_ptr(), None, 0.0,
                                                 out.data_ptr(), _capi.stream_ptr()), "smx_hmat_matvec")
         return out
@@ -225,11 +287,18 @@ class HMatrixGPUPackedFused:
 
     def algorithmic_bytes(self) -> int:
         """Bytes one matvec must move: factors, near CSR (values + columns +
-        row pointers), diagonal, x gather / y scatter."""
+        row pointers), diagonal, x gather / y scatter (padding excluded)."""
         fb = 4 if self.use_fp32 else 8
-        nv = self.near_val.element_size()
+        nv = 4 if self.near_dtype == _capi.SMX_F32 else 8
         return int(self.far_storage_entries * fb + self.near_nnz * (nv + 4) + (self.nb + 1) * 8
                    + self.nb * 8 + 4 * self.nb * 8)
+
+    def stored_bytes(self) -> int:
+        """Bytes the kernels stream per matvec, alignment padding included."""
+        pk = self.fwd
+        size = lambda t: t.numel() * t.element_size()
+        return int(size(pk["U"]) + size(pk["V"]) + size(pk["near_val"]) + size(pk["near_idx"])
+                   + (self.nb + 1) * 8 + self.nb * 8 + 4 * self.nb * 8)
 
     def benchmark(self, x=None, repeats=20, warmup=3):
         if x is None:
@@ -249,8 +318,8 @@ class HMatrixGPUPackedFused:
                 "average_seconds": total / repeats}
 
     def memory_report(self) -> dict:
-        arrays = [self.U, self.V, self.near_ptr, self.near_idx, self.near_val, self.perm, self.diag_p,
-                  self.xp, self.z, *self.far.values(), *self.leaf.values()]
+        arrays = [v for v in self.fwd.values() if isinstance(v, torch.Tensor)]
+        arrays += [self.perm, self.diag_p, self.xp, self.z]
         nbytes = int(sum(a.numel() * a.element_size() for a in arrays))
         return {"backend": "hmatrix", "factor_dtype": "float32" if self.use_fp32 else "float64",
                 "far_blocks": self.nfar, "near_nnz": self.near_nnz,

@@ -147,15 +147,40 @@ int smx_aca_batched(double* A, const smx_aca_task* tasks, int64_t ntasks, double
 /* ---------------------------------------------------------------------- *
  * (b) H-matrix boundary matvec y = B x (replaces HMatrixGPUPackedFused.matvec,
  * lindholm_hmatrix_gpu/backend_fused.py:496-512, with the semantics of
- * HMatrixCPUData.matvec_cpu, storage.py:284-299):
- *   xp = src[gidx] - shift           (perm gather, optional boundary gather)
- *   z_b = V_b xp[J_b]                 (one wavefront per far block)
- *   yp[L] = d_p xp[L] + Near xp |L + sum_b U_b[L,:] z_b   (one workgroup per leaf)
- *   y[perm] = yp                      (scatter in the same kernel)
- * No atomics: every output row is produced by the workgroup of its leaf, in a
- * fixed order.  Factors in FP32 or FP64 (factor_dtype), near values FP64/FP32,
+ * HMatrixCPUData.matvec_cpu, storage.py:284-299).  Three launches:
+ *   xp = src[gidx] - shift                 (perm gather, optional boundary gather)
+ *   z[t.z_off + g] = V_t[g, :] . xp[t.c0 + (0..n)]   for g < t.kc
+ *                                          (one wavefront per z task t)
+ *   yp[p] = d_p xp[p] + Near[p,:] xp + U_L[p - l0, :] . zl_L;  y[perm[p]] = yp[p]
+ *                                          (one workgroup per leaf L)
+ * Layout (every factor load is 16 bytes and aligned):
+ *  - a z task covers kc <= SMX_HMAT_Z_GROUP rank rows of one far block's V:
+ *    row g at V + v_off + g*ld, ld = n rounded up to 4 entries (zero padded),
+ *    v_off a multiple of 4;
+ *  - a leaf L (rows leaf_start..leaf_stop, at most 64) owns a dense row-major
+ *    matrix of its far rows: row i at U + leaf_u_off[L] + i*leaf_kw[L],
+ *    leaf_kw a multiple of 4, zero padded.  Its column k multiplies
+ *    zl_L[k] = z[leaf_zg[leaf_zg_off[L] + k]] for k < K_L =
+ *    leaf_zg_off[L+1] - leaf_zg_off[L] (0 beyond), staged in LDS
+ *    (kw_max <= SMX_HMAT_KW_MAX doubles per workgroup);
+ *  - the near rows are CSR with an even entry count per row (pairs padded
+ *    with value 0 and a valid column), near_ptr[p] even.
+ * No atomics: every z entry and every output row has one producer and a
+ * fixed summation order.  Factors FP32 or FP64, near values FP64 or FP32,
  * FP64 accumulation.
  * ---------------------------------------------------------------------- */
+#define SMX_HMAT_Z_GROUP 8
+#define SMX_HMAT_KW_MAX 16384
+
+typedef struct {
+  int64_t v_off;
+  int64_t z_off;
+  int32_t c0;
+  int32_t n;
+  int32_t kc;
+  int32_t pad;
+} smx_hmat_ztask;
+
 typedef struct {
   int64_t nb;
   const int32_t* gidx;
@@ -166,24 +191,21 @@ typedef struct {
   const void* near_val;
   int32_t near_dtype;
   int32_t factor_dtype;
-  int64_t nfar;
-  const int32_t* far_c0;
-  const int32_t* far_n;
-  const int32_t* far_m;
-  const int32_t* far_rank;
-  const int64_t* far_u_off;
-  const int64_t* far_v_off;
-  const int64_t* far_z_off;
-  const void* U;
+  int64_t nztask;
+  const smx_hmat_ztask* ztask;
   const void* V;
   int64_t nleaf;
   const int32_t* leaf_start;
   const int32_t* leaf_stop;
-  const int64_t* leaf_fe_ptr;
-  const int32_t* leaf_fe_block;
-  const int32_t* leaf_fe_row;
-  double* xp;  /* workspace nb */
-  double* z;   /* workspace sum of ranks */
+  const int64_t* leaf_u_off;
+  const int32_t* leaf_kw;
+  const int64_t* leaf_zg_off; /* nleaf + 1 */
+  const int32_t* leaf_zg;
+  int32_t kw_max;
+  int32_t pad;
+  const void* U;
+  double* xp; /* workspace nb */
+  double* z;  /* workspace: sum of ranks */
 } smx_hmatrix;
 
 int smx_hmat_matvec(const smx_hmatrix* H, const double* src, const double* shift,

@@ -181,79 +181,96 @@ def _matvec(data, x):
     return y
 
 
-def _emulate_packed_matvec(P, x):
+def _emulate_packed(pk, nb, perm, diag_p, x):
     """Index-for-index host emulation of the three matvec kernels on the packed
-    tables (bounds checked), FP64 factors assumed."""
-    t = lambda a: a.cpu().numpy()
-    nb = P.nb
-    perm, dp = t(P.perm), t(P.diag_p)
-    U, V = t(P.U).astype(np.float64), t(P.V).astype(np.float64)
-    f = {k: t(v) for k, v in P.far.items()}
-    lf = {k: t(v) for k, v in P.leaf.items()}
+    arrays of ``pack_operator`` (bounds, alignment and coverage checked)."""
+    from smx.hmatrix.backend import ZTASK_DTYPE
     xp = x[perm]
-    z = np.zeros(max(int(f["rank"].sum()), 1))
-    for b in range(P.nfar):
-        c0, n, r = int(f["c0"][b]), int(f["n"][b]), int(f["rank"][b])
-        vo, zo = int(f["v_off"][b]), int(f["z_off"][b])
-        assert c0 + n <= nb and vo + r * n <= V.size and zo + r <= z.size
-        z[zo:zo + r] = V[vo:vo + r * n].reshape(r, n) @ xp[c0:c0 + n]
+    V = pk["V"].astype(np.float64)
+    U = pk["U"].astype(np.float64)
+    zG = 8
+    z = np.full(max(pk["nz"], 1), np.nan)
+    written = np.zeros(z.size, dtype=int)
+    tasks = pk["ztask"].view(ZTASK_DTYPE)[:pk["nztask"]]
+    for t in tasks:
+        n, kc, c0, vo, zo = int(t["n"]), int(t["kc"]), int(t["c0"]), int(t["v_off"]), int(t["z_off"])
+        assert 1 <= kc <= zG and c0 >= 0 and c0 + n <= nb
+        ld = (n + 3) // 4 * 4
+        assert vo % 4 == 0 and vo + kc * ld <= V.size and zo + kc <= z.size
+        Vt = V[vo:vo + kc * ld].reshape(kc, ld)
+        assert np.all(Vt[:, n:] == 0)
+        z[zo:zo + kc] = Vt[:, :n] @ xp[c0:c0 + n]
+        written[zo:zo + kc] += 1
+    assert np.all(written[:pk["nz"]] == 1)
     y = np.full(nb, np.nan)
-    nptr, nidx, nval = t(P.near_ptr), t(P.near_idx), t(P.near_val)
     covered = np.zeros(nb, dtype=int)
-    for L in range(P.nleaf):
-        l0, l1 = int(lf["start"][L]), int(lf["stop"][L])
-        assert 0 < l1 - l0 <= 64
+    nptr, nidx, nval = pk["near_ptr"], pk["near_idx"], pk["near_val"].astype(np.float64)
+    zg, zgo = pk["leaf_zg"], pk["leaf_zg_off"]
+    for L in range(pk["leaf_start"].size):
+        l0, l1 = int(pk["leaf_start"][L]), int(pk["leaf_stop"][L])
+        c, kw, uo = l1 - l0, int(pk["leaf_kw"][L]), int(pk["leaf_u_off"][L])
+        assert 0 < c <= 64 and kw % 4 == 0 and uo % 4 == 0 and uo + c * kw <= U.size
+        assert kw <= pk["kw_max"]
         covered[l0:l1] += 1
-        acc = np.zeros(l1 - l0)
-        for e in range(int(lf["fe_ptr"][L]), int(lf["fe_ptr"][L + 1])):
-            b = int(lf["fe_block"][e])
-            o = int(lf["fe_row"][e])
-            m, r, uo, zo = int(f["m"][b]), int(f["rank"][b]), int(f["u_off"][b]), int(f["z_off"][b])
-            assert 0 <= o and o + (l1 - l0) <= m and uo + r * m <= U.size
-            Ub = U[uo:uo + r * m].reshape(r, m).T
-            acc += Ub[o:o + (l1 - l0)] @ z[zo:zo + r]
-        for i in range(l0, l1):
-            s = sum(nval[e] * xp[nidx[e]] for e in range(nptr[i], nptr[i + 1]))
-            y[perm[i]] = dp[i] * xp[i] + s + acc[i - l0]
+        K = int(zgo[L + 1] - zgo[L])
+        assert 0 <= kw - K < 4
+        idx = zg[zgo[L]:zgo[L + 1]]
+        assert np.all((idx >= 0) & (idx < pk["nz"]))
+        zl = np.zeros(kw)
+        zl[:K] = z[idx]
+        UL = U[uo:uo + c * kw].reshape(c, kw)
+        assert np.all(UL[:, K:] == 0)
+        for i in range(c):
+            p = l0 + i
+            n0, n1 = int(nptr[p]), int(nptr[p + 1])
+            assert n0 % 2 == 0 and n1 % 2 == 0 and n1 <= nidx.size
+            assert np.all((nidx[n0:n1] >= 0) & (nidx[n0:n1] < nb))
+            s = float(nval[n0:n1] @ xp[nidx[n0:n1]]) + float(UL[i] @ zl)
+            y[perm[p]] = diag_p[p] * xp[p] + s
     assert np.all(covered == 1)
     return y
 
 
+def _pack(data, transpose=False, fdt=np.float64):
+    from smx.hmatrix.backend import pack_operator
+    fb = data.far_blocks
+    if transpose:
+        return pack_operator(data.size, [b.col_start for b in fb], [b.col_stop for b in fb],
+                             [b.row_start for b in fb], [b.row_stop for b in fb], [b.rank for b in fb],
+                             [b.V.T for b in fb], [b.U.T for b in fb], data.near_csr.T.tocsr(), fdt, np.float64)
+    return pack_operator(data.size, [b.row_start for b in fb], [b.row_stop for b in fb],
+                         [b.col_start for b in fb], [b.col_stop for b in fb], [b.rank for b in fb],
+                         [b.U for b in fb], [b.V for b in fb], data.near_csr, fdt, np.float64)
+
+
 def test_packed_tables_and_matvec(c1):
-    import torch
-    from smx.hmatrix import HMatrixGPUPackedFused
     _, geo, bd = c1
     data = build_hmatrix_device(geo, HMatrixBuildConfig(leaf_size=32), ops=HostOps(geo, bd))
-    P = HMatrixGPUPackedFused(data, use_fp32=False, device=torch.device("cpu"), boundary_gather=geo.bverts)
+    pk = _pack(data)
+    perm = np.asarray(data.perm)
     x = np.random.default_rng(0).standard_normal(geo.Nb)
-    assert np.abs(_emulate_packed_matvec(P, x) - _matvec(data, x)).max() < 1e-12
-    assert P.far_storage_entries == data.far_storage_entries
+    ref = _matvec(data, x)
+    assert np.abs(_emulate_packed(pk, data.size, perm, data.diag_jump[perm], x) - ref).max() < 1e-12
+    assert pk["far_storage_entries"] == data.far_storage_entries
+    assert pk["near_nnz"] == data.near_csr.nnz
+    # FP32 factors: same tables, rounding only in the factors
+    pk32 = _pack(data, fdt=np.float32)
+    y32 = _emulate_packed(pk32, data.size, perm, data.diag_jump[perm], x)
+    assert np.abs(y32 - ref).max() < 1e-5 * np.abs(ref).max()
 
 
 def test_packed_transpose_tables(c1):
-    """The transposed view (factors swapped, column leaves, near^T) reproduces
+    """The transposed pack (factors swapped, column leaves, near^T) reproduces
     B^T x under the same index emulation."""
-    import torch
-    from smx.hmatrix import HMatrixGPUPackedFused
     _, geo, bd = c1
     data = build_hmatrix_device(geo, HMatrixBuildConfig(leaf_size=32), ops=HostOps(geo, bd))
-    P = HMatrixGPUPackedFused(data, use_fp32=False, device=torch.device("cpu"))
-    P._struct(P.gidx_plain, transpose=True)
-    tt = P._tt
-
-    class View:
-        pass
-    v = View()
-    v.nb, v.perm, v.diag_p, v.nfar = P.nb, P.perm, P.diag_p, P.nfar
-    v.U, v.V = P.V, P.U
-    v.far, v.leaf = tt["far"], tt["leaf"]
-    v.nleaf = int(tt["leaf"]["start"].numel())
-    v.near_ptr, v.near_idx, v.near_val = tt["near_ptr"], tt["near_idx"], tt["near_val"]
+    pk = _pack(data, transpose=True)
+    perm = np.asarray(data.perm)
     x = np.random.default_rng(1).standard_normal(geo.Nb)
-    xp = x[data.perm]
-    yp = data.diag_jump[data.perm] * xp + data.near_csr.T @ xp
+    xp = x[perm]
+    yp = data.diag_jump[perm] * xp + data.near_csr.T @ xp
     for b in data.far_blocks:
         yp[b.col_start:b.col_stop] += b.V.T @ (b.U.T @ xp[b.row_start:b.row_stop])
     ref = np.empty_like(x)
-    ref[data.perm] = yp
-    assert np.abs(_emulate_packed_matvec(v, x) - ref).max() < 1e-12
+    ref[perm] = yp
+    assert np.abs(_emulate_packed(pk, data.size, perm, data.diag_jump[perm], x) - ref).max() < 1e-12
