@@ -299,11 +299,13 @@ __global__ __launch_bounds__(BS) void cg_iter_kernel(int64_t n, int32_t W, int64
     const double x_n = x0 + alpha * p_n;
     const double r_n = r0 - alpha * s_n;
     const double u_n = r_n * dk;
-    P.s[1 - q][row] = s_n;
-    P.r[1 - q][row] = r_n;
-    P.w[1 - q][row] = acc;
-    P.p[row] = p_n;
-    x[row] = x_n;
+    // streaming stores: the next launch reads these from another XCD anyway,
+    // and lines that bypass L2 need no write-back at the kernel boundary
+    __builtin_nontemporal_store(s_n, &P.s[1 - q][row]);
+    __builtin_nontemporal_store(r_n, &P.r[1 - q][row]);
+    __builtin_nontemporal_store(acc, &P.w[1 - q][row]);
+    __builtin_nontemporal_store(p_n, &P.p[row]);
+    __builtin_nontemporal_store(x_n, &x[row]);
     v[0] = r_n * u_n;
     v[1] = acc * u_n;
     v[2] = r_n * r_n;
