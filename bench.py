@@ -133,6 +133,38 @@ def matvec_timing(d, flush_mb=512):
     return cold, warm, nbytes, name
 
 
+def multi_rhs_timing(d, k=64, flush_mb=512):
+    """Dense boundary product for k right-hand sides on the matrix cores
+    (FP64, FP32 and BF16 storage of the same B), flushed, HIP events."""
+    import numpy as np
+    import torch
+    from smx import _capi
+    from smx.demag import dense_bmatmul
+    L = _capi.lib()
+    n, ld = d.Nb, d.B_ld
+    B64 = d.B_dense_gpu if d.B_dense_gpu.dtype == torch.float64 else None
+    if B64 is None:
+        return None
+    X = torch.from_numpy(np.random.default_rng(0).standard_normal((n, k))).to(B64.device)
+    ref = B64[:, :n] @ X
+    flush = torch.ones(flush_mb * 1024 * 1024 // 4, dtype=torch.float32, device=B64.device)
+    out = {}
+    for dt in (torch.float64, torch.float32, torch.bfloat16):
+        B = B64 if dt == torch.float64 else B64.to(dt)
+        work = torch.empty(int(L.smx_dense_bmatmul_ws_size(n, _capi.DTYPE_CODE[dt])), dtype=torch.uint8,
+                           device=B64.device)
+        Y = dense_bmatmul(B, ld, X, work=work)
+        t = event_time(lambda: dense_bmatmul(B, ld, X, work=work), 10, flush)
+        nbytes = n * n * B.element_size()
+        out[str(dt).split(".")[-1]] = {
+            "us": t * 1e6, "B_GBs": nbytes / t / 1e9, "hbm_frac": nbytes / t / 1e9 / HBM_PEAK_GBS,
+            "TFLOPs": 2.0 * n * n * k / t / 1e12,
+            "rel_err_vs_fp64": float(torch.linalg.norm(Y - ref) / torch.linalg.norm(ref))}
+        del B, work
+    return {"nrhs": k, "kernel": "mm_tile_kernel (v_mfma_f64_16x16x4 / f32_16x16x4 / f32_16x16x32_bf16)",
+            "note": "times include the X staging and split-K reduction launches", **out}
+
+
 def build_case(config, dev):
     import numpy as np
     import torch
@@ -341,6 +373,8 @@ def main():
                             "fp64_factors_GBs": mv64_bytes / mv64_cold / 1e9,
                             "fp64_factors_avg_us_flushed": mv64_cold * 1e6},
     }
+    if h64.demag_field.boundary_backend == "dense":
+        res["multi_rhs_boundary_product"] = multi_rhs_timing(h64.demag_field)
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(args.config, mesh, mat, [p.cpu().numpy() for p in pool[:2]], extra, h64,
                                            args.cpu_budget)

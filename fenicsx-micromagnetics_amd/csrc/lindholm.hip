@@ -159,16 +159,6 @@ __global__ __launch_bounds__(kThreads) void lindholm_fill_tasks_kernel(
             T.ld_row, T.ld_col, incident_policy);
 }
 
-__device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7f800000u) == 0x7f800000u) {  // inf / nan
-    return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
-  }
-  const uint32_t lsb = (u >> 16) & 1u;
-  u += 0x7fffu + lsb;
-  return (uint16_t)(u >> 16);
-}
-
 __global__ void dense_finish_kernel(double* __restrict__ B, int64_t n, int64_t ld,
                                     const double* __restrict__ diag, void* __restrict__ lo,
                                     int32_t lo_dtype, int64_t r0) {
@@ -186,7 +176,7 @@ __global__ void dense_finish_kernel(double* __restrict__ B, int64_t n, int64_t l
     if (lo_dtype == SMX_F32) {
       reinterpret_cast<float*>(lo)[i * ld + j] = (float)v;
     } else {
-      reinterpret_cast<uint16_t*>(lo)[i * ld + j] = f32_to_bf16_rne((float)v);
+      reinterpret_cast<uint16_t*>(lo)[i * ld + j] = smx::f64_to_bf16_bits(v);
     }
   }
 }

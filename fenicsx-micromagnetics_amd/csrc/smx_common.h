@@ -43,6 +43,23 @@ void set_error(const char* fmt, ...);
     }                                                                         \
   } while (0)
 
+// FP64 -> BF16 with a single round-to-nearest-even: round to FP32 with
+// round-to-odd (truncate, then set the last bit when inexact), then round that
+// FP32 to BF16 (Boldo & Melquiond, "Emulation of FMA and correctly rounded
+// sums: proved algorithms using rounding to odd", 2008: with at least two
+// extra bits the second rounding is the correct one).  Returns the bits.
+__device__ __forceinline__ uint16_t f64_to_bf16_bits(double v) {
+  float f = (float)v;
+  if ((double)f != v && f == f && (__float_as_uint(f) & 0x7f800000u) != 0x7f800000u) {
+    if (fabs((double)f) > fabs(v)) f = __uint_as_float(__float_as_uint(f) - 1u);  // toward zero
+    f = __uint_as_float(__float_as_uint(f) | 1u);                                  // odd
+  }
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);

@@ -84,6 +84,8 @@ _SIGS = {
     "smx_hmat_matvec": ([C.POINTER(HMatrixArgs), P, P, F64, P, P], C.c_int),
     "smx_dense_bmatvec": ([P, I32, I64, I64, P, P, P, F64, P, P, P], C.c_int),
     "smx_dense_bmatvec_t": ([P, I32, I64, I64, P, P, P, P], C.c_int),
+    "smx_dense_bmatmul_ws_size": ([I64, I32], I64),
+    "smx_dense_bmatmul": ([P, I32, I64, I64, P, I64, I32, P, I64, P, I64, P], C.c_int),
     "smx_sell_spmv": ([I64, P, P, P, I32, P, P, P, P], C.c_int),
     "smx_sell_spmv_blocks": ([I64], I64),
     "smx_reduce_ws_size": ([I64], I64),

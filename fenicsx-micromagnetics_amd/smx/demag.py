@@ -362,3 +362,36 @@ def assemble_dense_B(geo: BoundaryGeometry, device, dtype=torch.float64):
     torch.cuda.synchronize(device)
     del B64
     return lo, ld
+
+
+def dense_bmatmul(B: torch.Tensor, ld: int, X: torch.Tensor, out: torch.Tensor | None = None,
+                  work: torch.Tensor | None = None) -> torch.Tensor:
+    """Y = B X for a block of right-hand sides on the matrix cores.
+
+    B: dense boundary matrix from ``assemble_dense_B`` (FP64, FP32 or BF16,
+    row-major with leading dimension ``ld``).  X: (n, k) FP64 on the device,
+    any k (processed 64 columns per launch).  Returns Y (n, k) FP64.  The
+    storage precision of B sets the MFMA kind (BF16 / FP32 / FP64)."""
+    n = int(B.shape[0])
+    if X.dim() != 2 or X.shape[0] != n or X.dtype != torch.float64:
+        raise ValueError(f"X must be ({n}, k) float64, got {tuple(X.shape)} {X.dtype}")
+    _capi.require_device(X, "X")
+    L = _capi.lib()
+    code = _capi.DTYPE_CODE[B.dtype]
+    k = int(X.shape[1])
+    Xc = X.t().contiguous()                      # column j contiguous: X[:, j] at j*n
+    Yc = torch.empty((k, n), dtype=torch.float64, device=X.device)
+    need = int(L.smx_dense_bmatmul_ws_size(n, code))
+    if work is None or work.numel() < need:
+        work = torch.empty(max(need, 1), dtype=torch.uint8, device=X.device)
+    st = _capi.stream_ptr()
+    for j0 in range(0, k, 64):
+        nr = min(64, k - j0)
+        _capi.check(L.smx_dense_bmatmul(B.data_ptr(), code, n, ld, Xc.data_ptr() + 8 * j0 * n, n, nr,
+                                        Yc.data_ptr() + 8 * j0 * n, n, work.data_ptr(), work.numel(), st),
+                    "smx_dense_bmatmul")
+    Y = Yc.t()
+    if out is not None:
+        out.copy_(Y)
+        return out
+    return Y.contiguous()

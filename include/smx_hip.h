@@ -114,6 +114,21 @@ int smx_dense_bmatvec(const void* B, int32_t dtype, int64_t n, int64_t ldb,
                       const double* shift, double shift_scale, double* xbuf,
                       double* y, void* stream);
 
+/* Multi-RHS dense product Y = B X on the matrix cores (nrhs <= 64 columns,
+ * column j of X / Y at X + j*ldx / Y + j*ldy, FP64 in and out).  B as in
+ * smx_dense_bmatvec with ldb a multiple of 8 and zero padding.  BF16 B uses
+ * v_mfma_f32_16x16x32_bf16 with X rounded to BF16 and FP32 accumulation;
+ * FP32 B uses v_mfma_f32_16x16x4_f32 (X rounded to FP32); FP64 B uses
+ * v_mfma_f64_16x16x4_f64.  Split-K partials are added in a fixed order in
+ * FP64.  work: smx_dense_bmatmul_ws_size(n, dtype) bytes.
+ * Batched form of the boundary apply (gpu/fields/Demag_Lindholm_GPU.py:596-616)
+ * for many right-hand sides at once (the c2 precision study, FD directions).
+ */
+int64_t smx_dense_bmatmul_ws_size(int64_t n, int32_t dtype);
+int smx_dense_bmatmul(const void* B, int32_t dtype, int64_t n, int64_t ldb,
+                      const double* X, int64_t ldx, int32_t nrhs, double* Y,
+                      int64_t ldy, void* work, int64_t work_bytes, void* stream);
+
 /* Transposed dense matvec y = B^T x (adjoint of the boundary operator).
  * work must hold (ceil(n/256)) * n doubles. */
 int smx_dense_bmatvec_t(const void* B, int32_t dtype, int64_t n, int64_t ldb,
