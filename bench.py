@@ -133,6 +133,63 @@ def matvec_timing(d, flush_mb=512):
     return cold, warm, nbytes, name
 
 
+def measured_traffic(config):
+    """HBM-side bytes per CG-iteration launch from the committed counter passes
+    (FETCH_SIZE and WRITE_SIZE of `bench.py --pmc-probe`, each scaled by the
+    same counter's reading for a 256 MiB device copy), or None."""
+    f = ROOT / "profiles" / "cg_iter_traffic.json"
+    if not f.exists():
+        return None
+    rec = json.loads(f.read_text()).get(config)
+    return None if rec is None else int(rec["traffic_bytes_per_launch"])
+
+
+def pmc_probe(config):
+    """Short program for hardware-counter passes: one 256 MiB device copy (the
+    calibration kernel of known traffic: 256 MiB read + 256 MiB written), then
+    200 CG iterations of the workload's u1 operator (the same ELL matrix and
+    launch shape as an evaluation; no boundary operator is built)."""
+    import numpy as np
+    import torch
+    from smx import _capi
+    from smx import mesh as M
+    from smx.device import DeviceEll, to_dev
+    from smx.fem import assemble_fem, compact_csr
+    dev = torch.device("cuda", 0)
+    src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    mesh = M.film_c3() if config == "film" else M.cube_c2()
+    fem = assemble_fem(mesh.X, mesh.tets)
+    g = fem.graph
+    ip, ix, vv = compact_csr(g.indptr, g.indices, fem.stiff)
+    A = DeviceEll.from_csr(ip, ix, vv, fem.N, dev)
+    r = np.repeat(np.arange(fem.N), np.diff(g.indptr))
+    dg = np.zeros(fem.N)
+    on = r == g.indices
+    dg[r[on]] = fem.stiff[on]
+    dinv = to_dev(1.0 / dg, dev)
+    b = np.random.default_rng(0).standard_normal(fem.N)
+    b -= b.mean()
+    bd = to_dev(b, dev)
+    bpart = to_dev(np.zeros(1), dev)
+    x = torch.zeros(fem.N, dtype=torch.float64, device=dev)
+    L = _capi.lib()
+    ws = torch.zeros(int(L.smx_pcg_ws_size(fem.N)), dtype=torch.float64, device=dev)
+    its, rn, sums = C.c_int32(0), C.c_double(0.0), (C.c_double * 1)()
+    K = 200
+    L.smx_pcg_jacobi(fem.N, A.width, A.ld, A.cols.data_ptr(), A.vals.data_ptr(), dinv.data_ptr(), 0,
+                     bd.data_ptr(), bpart.data_ptr(), 1, 1, A.width, A.ld, A.cols.data_ptr(), A.vals.data_ptr(),
+                     None, None, x.data_ptr(), 0, 0.0, 0.0, K, 10 ** 6, K, ws.data_ptr(), C.byref(its),
+                     C.byref(rn), sums, _capi.stream_ptr())
+    torch.cuda.synchronize()
+    print(json.dumps({"pmc_probe": config, "N": fem.N, "ell_nnz": A.nnz, "cg_iterations": K,
+                      "algorithmic_bytes_per_iteration": A.nnz * 12 + 88 * fem.N,
+                      "calibration_copy_bytes": 2 * src.numel() * 4}), flush=True)
+
+
 def multi_rhs_timing(d, k=64, flush_mb=512):
     """Dense boundary product for k right-hand sides on the matrix cores
     (FP64, FP32 and BF16 storage of the same B), flushed, HIP events."""
@@ -303,7 +360,12 @@ def main():
     ap.add_argument("--config", choices=["film", "cube"], default="film")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="short program for rocprofv3 --pmc passes (calibration copy + CG iterations)")
     args = ap.parse_args()
+    if args.pmc_probe:
+        pmc_probe(args.config)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
@@ -366,7 +428,8 @@ def main():
         "cg_iterations_u1_u2": [float(np.mean([a for a, _ in itsmx])), float(np.mean([b for _, b in itsmx]))],
         "roofline": {"bound": "hbm", "kernel": "cg_iter_kernel (Jacobi CG iteration, u1 operator)",
                      "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                     "traffic": None, "algorithmic_bytes": int(cg_bytes), "avg_us": cg_t * 1e6},
+                     "traffic": measured_traffic(args.config), "algorithmic_bytes": int(cg_bytes),
+                     "avg_us": cg_t * 1e6},
         "matvec_roofline": {"bound": "hbm", "kernel": mv_name, "achieved": mv_ach, "peak": HBM_PEAK_GBS,
                             "unit": "GB/s", "frac": mv_ach / HBM_PEAK_GBS, "algorithmic_bytes": int(mv_bytes),
                             "avg_us_flushed": mv_cold * 1e6, "avg_us_cache_warm": mv_warm * 1e6,
