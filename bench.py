@@ -98,7 +98,8 @@ def cg_iteration_timing(d, K=200):
     def solve(k):
         L.smx_pcg_jacobi(d.N, A.width, A.ld, A.cols.data_ptr(), A.vals.data_ptr(), d.dinv_u1.data_ptr(), 0,
                          d.b_u1.data_ptr(), d.b_part.data_ptr(), d.b_part.numel(), 1, A.width, A.ld,
-                         A.cols.data_ptr(), A.vals.data_ptr(), None, None, x.data_ptr(), 0, 0.0, 0.0, k, 10 ** 6,
+                         A.cols.data_ptr(), A.vals.data_ptr(), None, None, x.data_ptr(), 0,
+                         _capi.ptr(d.cg_perm), 0.0, 0.0, k, 10 ** 6,
                          k, d.ws1.data_ptr(), C.byref(its), C.byref(rn), sums, _capi.stream_ptr())
 
     t_k = event_time(lambda: solve(K), 5)
@@ -153,8 +154,8 @@ def pmc_probe(config):
     import torch
     from smx import _capi
     from smx import mesh as M
-    from smx.device import DeviceEll, to_dev
-    from smx.fem import assemble_fem, compact_csr
+    from smx.device import to_dev
+    from smx.fem import assemble_fem
     dev = torch.device("cuda", 0)
     src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
     dst = torch.empty_like(src)
@@ -163,14 +164,17 @@ def pmc_probe(config):
     torch.cuda.synchronize()
     mesh = M.film_c3() if config == "film" else M.cube_c2()
     fem = assemble_fem(mesh.X, mesh.tets)
+    from smx.demag import OperatorSet
+    ops = OperatorSet(mesh, dev, fem)
+    perm = ops.solver_order()                     # the demag solver's row order
+    A = ops.compact(fem.stiff, perm)
     g = fem.graph
-    ip, ix, vv = compact_csr(g.indptr, g.indices, fem.stiff)
-    A = DeviceEll.from_csr(ip, ix, vv, fem.N, dev)
     r = np.repeat(np.arange(fem.N), np.diff(g.indptr))
     dg = np.zeros(fem.N)
     on = r == g.indices
     dg[r[on]] = fem.stiff[on]
-    dinv = to_dev(1.0 / dg, dev)
+    dinv = to_dev(1.0 / dg[perm], dev)
+    pd = to_dev(perm.astype(np.int32), dev)
     b = np.random.default_rng(0).standard_normal(fem.N)
     b -= b.mean()
     bd = to_dev(b, dev)
@@ -182,7 +186,7 @@ def pmc_probe(config):
     K = 200
     L.smx_pcg_jacobi(fem.N, A.width, A.ld, A.cols.data_ptr(), A.vals.data_ptr(), dinv.data_ptr(), 0,
                      bd.data_ptr(), bpart.data_ptr(), 1, 1, A.width, A.ld, A.cols.data_ptr(), A.vals.data_ptr(),
-                     None, None, x.data_ptr(), 0, 0.0, 0.0, K, 10 ** 6, K, ws.data_ptr(), C.byref(its),
+                     None, None, x.data_ptr(), 0, pd.data_ptr(), 0.0, 0.0, K, 10 ** 6, K, ws.data_ptr(), C.byref(its),
                      C.byref(rn), sums, _capi.stream_ptr())
     torch.cuda.synchronize()
     print(json.dumps({"pmc_probe": config, "N": fem.N, "ell_nnz": A.nnz, "cg_iterations": K,

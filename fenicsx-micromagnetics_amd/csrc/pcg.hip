@@ -121,7 +121,7 @@ __global__ __launch_bounds__(BS) void cg_init_a_kernel(
     const double* __restrict__ vals, const double* __restrict__ b,
     const double* __restrict__ b_part, int64_t nb_part, int32_t project,
     const int32_t* __restrict__ bpos, const double* __restrict__ gb, double* __restrict__ x,
-    int32_t warm, CGPtrs P) {
+    int32_t warm, const int32_t* __restrict__ perm, CGPtrs P) {
   __shared__ double scr[BS / 64 * 1];
   const int64_t row = (int64_t)blockIdx.x * BS + threadIdx.x;
   double shift = 0.0;
@@ -136,7 +136,7 @@ __global__ __launch_bounds__(BS) void cg_init_a_kernel(
   if (row < n) {
     double r0, bk;
     if (mode == 0) {
-      bk = b[row] - shift;
+      bk = b[perm ? perm[row] : row] - shift;
       double ax = 0.0;
       if (warm) {
 #pragma unroll
@@ -180,6 +180,20 @@ __global__ __launch_bounds__(BS) void cg_init_a_kernel(
     v[0] = bk * bk;
   }
   publish_partials<BS, 1>(v, P.pbb, P.G, scr);
+}
+
+// x_p[i] = x[perm[i]] (warm start in solver order)
+__global__ void cg_gather_kernel(int64_t n, const int32_t* __restrict__ perm, const double* __restrict__ x,
+                                 double* __restrict__ xp) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) xp[i] = x[perm[i]];
+}
+
+// x[perm[i]] = x_p[i] (solution back in the caller's order)
+__global__ void cg_scatter_kernel(int64_t n, const int32_t* __restrict__ perm, const double* __restrict__ xp,
+                                  double* __restrict__ x) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[perm[i]] = xp[i];
 }
 
 // w0 = A D^-1 r0; partials of (g0, d0, rr0, sum x0)
@@ -337,16 +351,23 @@ int run_pcg(int64_t n, int32_t W, int64_t ld, const int32_t* cols, const double*
             const double* dinv, int32_t mode, const double* b, const double* b_part,
             int64_t nb_part, int32_t project, int32_t raw_W, int64_t raw_ld,
             const int32_t* raw_cols, const double* raw_vals, const int32_t* bpos, const double* gb,
-            double* x, int32_t warm, double rtol, double atol, int32_t max_it, int32_t check_every,
-            int32_t expected_its, CGPtrs P, hipStream_t st, double* hst) {
+            double* x_user, int32_t warm, const int32_t* perm, double* x_perm, double rtol,
+            double atol, int32_t max_it, int32_t check_every, int32_t expected_its, CGPtrs P,
+            hipStream_t st, double* hst) {
   const int G = P.G;
+  const unsigned gs = (unsigned)((n + 255) / 256);
+  double* x = perm ? x_perm : x_user;  // the kernels work in solver (permuted) row order
+  if (perm && warm) {
+    hipLaunchKernelGGL(cg_gather_kernel, dim3(gs), dim3(256), 0, st, n, perm, x_user, x);
+    SMX_LAUNCH_CHECK();
+  }
   if (mode == 0) {
     hipLaunchKernelGGL((cg_init_a_kernel<BS, WMAX>), dim3(G), dim3(BS), 0, st, n, mode, W, ld,
-                       cols, vals, b, b_part, nb_part, project, bpos, gb, x, warm, P);
+                       cols, vals, b, b_part, nb_part, project, bpos, gb, x, warm, perm, P);
   } else {
     hipLaunchKernelGGL((cg_init_a_kernel<BS, WMAX>), dim3(G), dim3(BS), 0, st, n, mode, raw_W,
                        raw_ld, raw_cols, raw_vals, b, b_part, nb_part, project, bpos, gb, x, warm,
-                       P);
+                       perm, P);
   }
   SMX_LAUNCH_CHECK();
   hipLaunchKernelGGL((cg_init_b_kernel<BS, WMAX>), dim3(G), dim3(BS), 0, st, n, W, ld, cols, vals,
@@ -369,6 +390,10 @@ int run_pcg(int64_t n, int32_t W, int64_t ld, const int32_t* cols, const double*
     done = hst[SS_DONE] != 0.0;
     batch = check_every;
   }
+  if (perm) {
+    hipLaunchKernelGGL(cg_scatter_kernel, dim3(gs), dim3(256), 0, st, n, perm, x, x_user);
+    SMX_LAUNCH_CHECK();
+  }
   return SMX_OK;
 }
 
@@ -376,7 +401,7 @@ int run_pcg(int64_t n, int32_t W, int64_t ld, const int32_t* cols, const double*
 
 extern "C" int64_t smx_pcg_ws_size(int64_t n) {
   const Layout L = layout(n);
-  return HDR + 9 * L.G + 7 * L.nvec + 64;
+  return HDR + 9 * L.G + 8 * L.nvec + 64;
 }
 
 extern "C" int smx_pcg_jacobi(int64_t n, int32_t width, int64_t ld, const int32_t* cols,
@@ -385,7 +410,8 @@ extern "C" int smx_pcg_jacobi(int64_t n, int32_t width, int64_t ld, const int32_
                               int32_t project, int32_t raw_width, int64_t raw_ld,
                               const int32_t* raw_cols, const double* raw_vals,
                               const int32_t* bpos, const double* gb, double* x, int32_t warm,
-                              double rtol, double atol, int32_t max_it, int32_t check_every,
+                              const int32_t* perm, double rtol, double atol, int32_t max_it,
+                              int32_t check_every,
                               int32_t expected_its, double* work, int32_t* its_out,
                               double* rnorm_out, double* sums_out, void* stream) {
   SMX_REQUIRE(n >= 0, SMX_ERR_SHAPE, "negative size");
@@ -421,13 +447,14 @@ extern "C" int smx_pcg_jacobi(int64_t n, int32_t width, int64_t ld, const int32_
   P.s[0] = v0 + 4 * Lo.nvec;
   P.s[1] = v0 + 5 * Lo.nvec;
   P.p = v0 + 6 * Lo.nvec;
+  double* x_perm = v0 + 7 * Lo.nvec;
   double hst[SS_SIZE] = {0};
   const int bs = block_size_for(n);
   int rc;
-#define SMX_PCG_CALL(BS, WM)                                                                      \
-  run_pcg<BS, WM>(n, width, ld, cols, vals, dinv, mode, b, b_part, nb_part, project, raw_width,  \
-                  raw_ld, raw_cols, raw_vals, bpos, gb, x, warm, rtol, atol, max_it, check_every, \
-                  expected_its, P, st, hst)
+#define SMX_PCG_CALL(BS, WM)                                                                     \
+  run_pcg<BS, WM>(n, width, ld, cols, vals, dinv, mode, b, b_part, nb_part, project, raw_width, \
+                  raw_ld, raw_cols, raw_vals, bpos, gb, x, warm, perm, x_perm, rtol, atol,      \
+                  max_it, check_every, expected_its, P, st, hst)
   const int32_t wmax = (mode == 1 && raw_width > width) ? raw_width : width;
   if (bs == 256) {
     rc = wmax <= 8 ? SMX_PCG_CALL(256, 8) : wmax <= 16 ? SMX_PCG_CALL(256, 16) : SMX_PCG_CALL(256, 32);

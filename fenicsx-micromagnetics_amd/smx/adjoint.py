@@ -123,8 +123,9 @@ class MicromagneticEnergy:
         nbp = 0 if b_part is None else b_part.numel()
         rc = self._L().smx_pcg_jacobi(self.N, A.width, A.ld, _capi.ptr(A.cols), _capi.ptr(A.vals), _capi.ptr(dinv),
                                       mode, _capi.ptr(b), _capi.ptr(b_part), nbp, int(project), raw.width, raw.ld,
-                                      _capi.ptr(raw.cols), _capi.ptr(raw.vals), _capi.ptr(d.bpos_gpu), _capi.ptr(gb),
-                                      _capi.ptr(x), 0, float(d.ksp_rtol_u1), 0.0, int(d.ksp_max_it_u1), 32, 0,
+                                      _capi.ptr(raw.cols), _capi.ptr(raw.vals), _capi.ptr(d.bpos_cg), _capi.ptr(gb),
+                                      _capi.ptr(x), 0, _capi.ptr(d.cg_perm), float(d.ksp_rtol_u1), 0.0,
+                                      int(d.ksp_max_it_u1), 32, 0,
                                       _capi.ptr(ws), C.byref(its), C.byref(rn), sums, _capi.stream_ptr())
         if rc != _capi.SMX_ERR_NOT_CONVERGED:
             _capi.check(rc, "smx_pcg_jacobi")

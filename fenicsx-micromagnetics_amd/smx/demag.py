@@ -23,7 +23,7 @@ import time
 
 import numpy as np
 import scipy.sparse as sp
-from scipy.sparse.csgraph import connected_components
+from scipy.sparse.csgraph import connected_components, reverse_cuthill_mckee
 import torch
 
 from . import _capi
@@ -76,11 +76,35 @@ class OperatorSet:
         sv = sell_values(self.fem.graph.indptr, vals, self.N)
         return self.graph.with_vals(sv, k)
 
-    def compact(self, vals: np.ndarray) -> DeviceEll:
-        """Scalar operator with (near-)zero entries dropped, as ELL for the CG kernels."""
+    def compact(self, vals: np.ndarray, perm: np.ndarray | None = None) -> DeviceEll:
+        """Scalar operator with (near-)zero entries dropped, as ELL for the CG
+        kernels; with ``perm`` (solver row i = node perm[i]) it is P A P^T."""
         g = self.fem.graph
         ip, ix, vv = compact_csr(g.indptr, g.indices, vals)
+        if perm is not None:
+            S = sp.csr_matrix((vv, ix, ip), shape=(self.N, self.N))[perm][:, perm].tocsr()
+            S.sort_indices()
+            ip, ix, vv = S.indptr, S.indices, S.data
         return DeviceEll.from_csr(ip, ix, vv, self.N, self.device)
+
+    def solver_order(self) -> np.ndarray:
+        """Row order for the CG solves: the node numbering or its reverse
+        Cuthill-McKee order, whichever has fewer matrix entries crossing the
+        eight contiguous row ranges the CG kernel gives the eight XCDs (those
+        entries are fetched from another XCD's L2 every iteration)."""
+        if not hasattr(self, "_order"):
+            g = self.fem.graph
+            N = self.N
+            r = np.repeat(np.arange(N), np.diff(g.indptr))
+            adj = sp.csr_matrix((np.ones(g.nnz, dtype=np.int8), (r, g.indices)), shape=(N, N))
+            rcm = np.asarray(reverse_cuthill_mckee(adj, symmetric_mode=True), dtype=np.int64)
+            pos = np.empty(N, dtype=np.int64)
+            pos[rcm] = np.arange(N)
+            chunk = (N + 7) // 8
+            cross_id = int(np.count_nonzero(r // chunk != g.indices // chunk))
+            cross_rcm = int(np.count_nonzero(pos[r] // chunk != pos[g.indices] // chunk))
+            self._order = rcm if cross_rcm < cross_id else np.arange(N, dtype=np.int64)
+        return self._order
 
 
 class DemagFieldLindholmGPU:
@@ -107,7 +131,7 @@ class DemagFieldLindholmGPU:
                  *, ksp_atol=1e-12, ksp_max_it_u1=2000, ksp_max_it_u2=12000,
                  warm_start=False, device=None, operators: OperatorSet | None = None,
                  geometry: BoundaryGeometry | None = None, hmatrix_prebuilt=None,
-                 hmatrix_near_fp32=False, verbose=False):
+                 hmatrix_near_fp32=False, cg_reorder=True, verbose=False):
         if not isinstance(mesh, TetMesh):
             raise TypeError("mesh must be a smx.mesh.TetMesh")
         self.mesh = mesh
@@ -151,12 +175,17 @@ class DemagFieldLindholmGPU:
         # ---------------- FEM operators ----------------
         self.Div = self.ops.graph_values(self.Ms * fem.div, 3)
         self.G = self.ops.graph_values(fem.grad, 3)
-        self.A_u1 = self.ops.compact(fem.stiff)
-        self.A_u2 = self.ops.compact(fem.stiffness_bc())
+        # CG operators in the solver's row order (P A P^T); vectors stay in node order
+        perm = self.ops.solver_order() if cg_reorder else None
+        self.cg_perm = to_dev(perm.astype(np.int32), dev) if perm is not None else None
+        ident = np.arange(self.N)
+        sel = perm if perm is not None else ident
+        self.A_u1 = self.ops.compact(fem.stiff, perm)
+        self.A_u2 = self.ops.compact(fem.stiffness_bc(), perm)
         d1 = self._diag(fem.stiff)
         d2 = self._diag(fem.stiffness_bc())
-        self.dinv_u1 = to_dev(1.0 / d1, dev)
-        self.dinv_u2 = to_dev(1.0 / d2, dev)
+        self.dinv_u1 = to_dev(1.0 / d1[sel], dev)
+        self.dinv_u2 = to_dev(1.0 / d2[sel], dev)
 
         # ---------------- boundary operator ----------------
         self.opB = geometry or build_boundary_geometry(mesh.X, mesh.tets, incident_policy)
@@ -164,6 +193,7 @@ class DemagFieldLindholmGPU:
         self.Nb = geo.Nb
         self.bverts_gpu = to_dev(geo.bverts.astype(np.int32), dev)
         self.bpos_gpu = to_dev(geo.bpos.astype(np.int32), dev)
+        self.bpos_cg = to_dev(geo.bpos[sel].astype(np.int32), dev)   # solver row order
         self.diag_jump_gpu = to_dev(geo.diag_jump, dev)
         self.B_dtype = _torch_dtype(B_dtype)
 
@@ -240,7 +270,8 @@ class DemagFieldLindholmGPU:
         rc = L.smx_pcg_jacobi(self.N, A.width, A.ld, _capi.ptr(A.cols), _capi.ptr(A.vals),
                               _capi.ptr(dinv), mode, _capi.ptr(b), _capi.ptr(b_part), nbp, 1,
                               raw.width, raw.ld, _capi.ptr(raw.cols), _capi.ptr(raw.vals),
-                              _capi.ptr(self.bpos_gpu), _capi.ptr(gb), _capi.ptr(x), int(warm),
+                              _capi.ptr(self.bpos_cg), _capi.ptr(gb), _capi.ptr(x), int(warm),
+                              _capi.ptr(self.cg_perm),
                               float(rtol), float(self.ksp_atol), int(max_it), 16, int(expected),
                               _capi.ptr(ws), C.byref(its), C.byref(rn), sums,
                               _capi.stream_ptr())

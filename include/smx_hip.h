@@ -258,6 +258,11 @@ int64_t smx_reduce_ws_size(int64_t n);
  * mode 1 (u2, Dirichlet lifting): A = A_bc; boundary rows take the data
  *     x_k = gb[bpos[k]] where bpos[k] >= 0; interior rows solve
  *     sum_j A_raw_kj x_j = 0 (raw stiffness in ELL, raw_width <= 32).
+ * perm (optional, NULL = identity): solver row i is caller row perm[i].  The
+ *     matrices (cols/vals, raw), dinv and bpos are then given in solver order
+ *     (P A P^T etc.); b, gb and x stay in the caller's order (b is gathered
+ *     on entry, x scattered back on exit).  A bandwidth-reducing order keeps
+ *     the SpMV neighbours of each XCD's contiguous row range in its own L2.
  * Convergence: ||r||_2 <= max(rtol*||b||_2, atol), b the (lifted) right side.
  * work: smx_pcg_ws_size(n) doubles.  On return work[SMX_PCG_SUMX] holds
  * sum(x) on the device (the nullspace projection of x, read by later kernels).
@@ -272,9 +277,9 @@ int smx_pcg_jacobi(int64_t n, int32_t width, int64_t ld, const int32_t* cols,
                    int32_t project, int32_t raw_width, int64_t raw_ld,
                    const int32_t* raw_cols, const double* raw_vals,
                    const int32_t* bpos, const double* gb, double* x, int32_t warm,
-                   double rtol, double atol, int32_t max_it, int32_t check_every,
-                   int32_t expected_its, double* work, int32_t* its_out,
-                   double* rnorm_out, double* sums_out, void* stream);
+                   const int32_t* perm, double rtol, double atol, int32_t max_it,
+                   int32_t check_every, int32_t expected_its, double* work,
+                   int32_t* its_out, double* rnorm_out, double* sums_out, void* stream);
 
 /* ---------------------------------------------------------------------- *
  * (d) Fused effective field + LLG right-hand side.
