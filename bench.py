@@ -134,6 +134,66 @@ def matvec_timing(d, flush_mb=512):
     return cold, warm, nbytes, name
 
 
+def adjoint_through_time(steps=500, dt=1e-13, every=50, n_fd=2):
+    """Gradient of <m_z>(t_end) w.r.t. per-cell Ms after `steps` fixed DP5 steps
+    on the 150 nm cube (alpha 0.02, vortex start, Ms 8e5 A/m in every cell),
+    checkpoint every `every` steps.  FP64 boundary operator (the reference
+    gradient) and FP32 boundary operator (mixed forward); central differences
+    (relative step 1e-6) on `n_fd` random directions of the FP64 model."""
+    import numpy as np
+    import torch
+    from smx import mesh as M
+    from smx.llg_adjoint import LLGAdjointModel, integrate, mean_mz
+    dev = torch.device("cuda", 0)
+    mesh = M.cube_c2()
+    m0 = torch.from_numpy(vortex_m(mesh.X, seed=0)).to(dev)
+    out = {"metric": "adjoint through time: dJ/dMs_cell, J = <m_z>(t_end)", "config": {
+        "workload": "cube150nm_30x30x30_kuhn (N=29791, T=162000 cells, Nb=5402, dense B)", "steps": steps,
+        "dt_s": dt, "alpha": 0.02, "checkpoint_every": every, "Ms_A_per_m": 8e5, "A_J_per_m": 1.3e-11,
+        "cg_rtol": 1e-10}}
+    grads = {}
+    for name, bdt in (("fp64", "float64"), ("fp32_boundary", "float32")):
+        model = LLGAdjointModel(mesh, A_ex=1.3e-11, alpha=0.02, ksp_rtol=1e-10, device=dev,
+                                demag_kwargs=dict(B_dtype=bdt, ksp_max_it_u1=20000, ksp_max_it_u2=20000))
+        Ms = torch.full((model.T,), 8e5, dtype=torch.float64, device=dev, requires_grad=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        with torch.no_grad():
+            Jf = float(mean_mz(model, integrate(model, m0, Ms.detach(), dt, steps, every)))
+        torch.cuda.synchronize()
+        t_fwd = time.perf_counter() - t0
+        model.rhs_evals = model.vjp_evals = 0
+        t0 = time.perf_counter()
+        J = mean_mz(model, integrate(model, m0, Ms, dt, steps, every))
+        (g,) = torch.autograd.grad(J, Ms)
+        torch.cuda.synchronize()
+        t_grad = time.perf_counter() - t0
+        grads[name] = g
+        out[name] = {"J": Jf, "forward_s": t_fwd, "forward_and_gradient_s": t_grad,
+                     "rhs_evals": model.rhs_evals, "rhs_vjps": model.vjp_evals,
+                     "grad_norm": float(torch.linalg.norm(g))}
+        log(f"{name}: J={Jf:.12f} forward {t_fwd:.1f}s gradient {t_grad:.1f}s")
+        if name == "fp64":
+            rng = np.random.default_rng(11)
+            errs = []
+            with torch.no_grad():
+                for _ in range(n_fd):
+                    w = torch.from_numpy(rng.standard_normal(model.T)).to(dev) * 8e5
+                    h = 1e-6
+                    Jp = float(mean_mz(model, integrate(model, m0, Ms.detach() + h * w, dt, steps, every)))
+                    Jm = float(mean_mz(model, integrate(model, m0, Ms.detach() - h * w, dt, steps, every)))
+                    fd = (Jp - Jm) / (2 * h)
+                    an = float(g @ w)
+                    errs.append(abs(fd - an) / abs(an))
+            out["fp64"]["fd_rel_err"] = errs
+            log(f"fp64 central-difference check: {errs}")
+        del model
+        torch.cuda.empty_cache()
+    out["fp32_boundary_vs_fp64_grad_rel_err"] = float(torch.linalg.norm(grads["fp32_boundary"] - grads["fp64"])
+                                                      / torch.linalg.norm(grads["fp64"]))
+    print(json.dumps(out), flush=True)
+
+
 def measured_traffic(config):
     """HBM-side bytes per CG-iteration launch from the committed counter passes
     (FETCH_SIZE and WRITE_SIZE of `bench.py --pmc-probe`, each scaled by the
@@ -361,7 +421,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", choices=["film", "cube"], default="film")
+    ap.add_argument("--config", choices=["film", "cube", "adjoint"], default="film")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--pmc-probe", action="store_true",
@@ -369,6 +429,9 @@ def main():
     args = ap.parse_args()
     if args.pmc_probe:
         pmc_probe(args.config)
+        return
+    if args.config == "adjoint":
+        adjoint_through_time()
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))

@@ -104,6 +104,11 @@ _SIGS = {
     "smx_energy_node_grad": ([I64, P, P, P, P, P, P, P, P, F64, P, P, P], C.c_int),
     "smx_vec_glue": ([I32, I64, P, P, P, F64, F64, P, P, F64, P, P], C.c_int),
     "smx_dot_partials": ([I64, P, P, P, P], C.c_int),
+    "smx_node_rhs": ([I64, P, P, P, F64, F64, F64, P, P], C.c_int),
+    "smx_node_rhs_vjp": ([I64, P, P, P, P, P, P, F64, P, P, F64, F64, F64, P, P, P, P, P, P], C.c_int),
+    "smx_node_to_cell": ([I64, P, P, P, P, I32, P, P], C.c_int),
+    "smx_cell_to_node": ([I64, P, P, P, P, P, P, P], C.c_int),
+    "smx_row_scale": ([I64, P, P, P, P, P], C.c_int),
 }
 
 

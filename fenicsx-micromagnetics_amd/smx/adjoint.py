@@ -176,6 +176,35 @@ class MicromagneticEnergy:
         out["E_total"] = out["E_ex"] + out["E_an"] + out.get("E_d", 0.0)
         return out
 
+    def lt_apply(self, y: torch.Tensor, out: torch.Tensor):
+        """out = L^T y (L: b -> phi of the demag potential): an interior Dirichlet
+        solve for E^T, the transposed boundary operator, and a Neumann solve.
+        Leaves the iteration counts of the two solves in self.its_backward."""
+        L = self._L()
+        st = _capi.stream_ptr()
+        d = self.demag
+        N, Nb = self.N, d.Nb
+        # E^T y: interior Dirichlet solve, then the boundary residual
+        self._glue(0, N, y, idx=d.bpos_gpu, out=self.y)
+        its = [self._solve(0, d.A_u2, d.dinv_u2, self.zv, self.ws3, b=self.y, project=0)]
+        self.stiff_graph.spmv(self.zv, self.Az, 0)
+        self._glue(1, Nb, y, self.Az, idx=d.bverts_gpu, out=self.q)
+        # B^T
+        if d.boundary_backend == "dense":
+            self._check(L.smx_dense_bmatvec_t(_capi.ptr(d.B_dense_gpu), _capi.DTYPE_CODE[d.B_dense_gpu.dtype],
+                                              Nb, d.B_ld, _capi.ptr(self.q), _capi.ptr(self.t),
+                                              _capi.ptr(self.tmp_t), st), "smx_dense_bmatvec_t")
+        else:
+            d.B_hmatrix_gpu.rmatvec(self.q, self.t)
+        # K (y + Gamma^T t), K = Pi A^+ Pi
+        self._glue(2, N, y, self.t, idx=d.bpos_gpu, out=self.w)
+        self._check(L.smx_dot_partials(N, _capi.ptr(self.w), _capi.ptr(self.ones), _capi.ptr(self.pN2), st),
+                    "smx_dot_partials")
+        its.append(self._solve(0, d.A_u1, d.dinv_u1, self.v, self.ws3, b=self.w, b_part=self.pN2, project=1))
+        self._glue(3, N, self.v, self.v, a=0.0, b=1.0, sy=self.ws3.data_ptr() + 8 * _capi.PCG_SUMX, out=out)
+        self.its_backward = its
+        return out
+
     def backward(self, m, Ms_cell, A_cell):
         """(dE/dm, dE/dMs_cell, dE/dA_cell) at the state of the last forward()."""
         L = self._L()
@@ -183,28 +212,10 @@ class MicromagneticEnergy:
         tptr = None
         if self.demag is not None:
             d = self.demag
-            N, Nb = self.N, d.Nb
-            # E^T b: interior Dirichlet solve, then the boundary residual
-            self._glue(0, N, self.b, idx=d.bpos_gpu, out=self.y)
-            its = [self._solve(0, d.A_u2, d.dinv_u2, self.zv, self.ws3, b=self.y, project=0)]
-            self.stiff_graph.spmv(self.zv, self.Az, 0)
-            self._glue(1, Nb, self.b, self.Az, idx=d.bverts_gpu, out=self.q)
-            # B^T
-            if d.boundary_backend == "dense":
-                self._check(L.smx_dense_bmatvec_t(_capi.ptr(d.B_dense_gpu), _capi.DTYPE_CODE[d.B_dense_gpu.dtype],
-                                                  Nb, d.B_ld, _capi.ptr(self.q), _capi.ptr(self.t),
-                                                  _capi.ptr(self.tmp_t), st), "smx_dense_bmatvec_t")
-            else:
-                d.B_hmatrix_gpu.rmatvec(self.q, self.t)
-            # K (b + Gamma^T t)
-            self._glue(2, N, self.b, self.t, idx=d.bpos_gpu, out=self.w)
-            self._check(L.smx_dot_partials(N, _capi.ptr(self.w), _capi.ptr(self.ones), _capi.ptr(self.pN2), st),
-                        "smx_dot_partials")
-            its.append(self._solve(0, d.A_u1, d.dinv_u1, self.v, self.ws3, b=self.w, b_part=self.pN2, project=1))
+            # dE_d/db = c (phi + L^T b)
+            self.lt_apply(self.b, self.gb)
             c = 0.5 * MU0 * 1e-27
-            self._glue(3, N, self.phi, self.v, a=c, b=c, sy=self.ws3.data_ptr() + 8 * _capi.PCG_SUMX,
-                       out=self.gb)
-            self.its_backward = its
+            self._glue(3, self.N, self.phi, self.gb, a=c, b=c, out=self.gb)
             self._check(L.smx_energy_cell_backward(self.T, _capi.ptr(self.cells), _capi.ptr(self.cgv),
                                                    _capi.ptr(Ms_cell), _capi.ptr(self.gb), _capi.ptr(m),
                                                    _capi.ptr(self.tc), _capi.ptr(self.grad_Ms), st),

@@ -377,6 +377,33 @@ int smx_vec_glue(int32_t op, int64_t n, const double* x, const double* y, const 
                  void* stream);
 int smx_dot_partials(int64_t n, const double* x, const double* y, double* partials, void* stream);
 
+/* ---------------------------------------------------------------------- *
+ * (a22) LLG right-hand side with per-cell Ms and its vector-Jacobian product,
+ * for the adjoint through the Dormand-Prince steps (no counterpart in the
+ * reference; the forward is _LLG_RHS_KERNEL, llg_module_GPU.py:100-122, with
+ * a nodal Ms_i = sum_c |c|/4 Ms_c / V_i in the exchange prefactor and in S).
+ *   node_rhs:     r = P (p m x H + alpha m x (m x H)) + S_i (1 - |m|^2) m
+ *   node_rhs_vjp: for the cotangent lam of r: gm (explicit m part), gk = cex gH,
+ *                 gd = gH / V, gnode = gH.(-Hex/Ms_i) + s0 (1-|m|^2)(m.lam)
+ *                 with gH = dr/dH^T lam and Hex = H - Hd - H0
+ *   node_to_cell: gcell_c (+)= |c|/4 sum_{i in c} gnode_i / V_i
+ *   cell_to_node: Ms_i = sum_{c ni i} |c|/4 Ms_c / V_i   (qvol = |c|/4)
+ *   row_scale:    out[e] = vals[e] * scale[rows[e]] (rows < 0: out 0)
+ * ---------------------------------------------------------------------- */
+int smx_node_rhs(int64_t n, const double* m, const double* h, const double* S, double P,
+                 double alpha, double do_precess, double* r, void* stream);
+int smx_node_rhs_vjp(int64_t n, const double* m, const double* h, const double* hd,
+                     const double* h0_u, const double* Ms, const double* S, double s0,
+                     const double* cex, const double* inv_vol, double P, double alpha,
+                     double do_precess, const double* lam, double* gm, double* gk,
+                     double* gd, double* gnode, void* stream);
+int smx_node_to_cell(int64_t T, const int32_t* cells, const double* qvol, const double* inv_vol,
+                     const double* gnode, int32_t accumulate, double* gcell, void* stream);
+int smx_cell_to_node(int64_t N, const int64_t* inc_ptr, const int32_t* inc, const double* qvol,
+                     const double* inv_vol, const double* Ms_cell, double* Ms_node, void* stream);
+int smx_row_scale(int64_t nslots, const double* vals, const int32_t* rows, const double* scale,
+                  double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
