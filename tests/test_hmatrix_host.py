@@ -274,3 +274,32 @@ def test_packed_transpose_tables(c1):
     ref = np.empty_like(x)
     ref[perm] = yp
     assert np.abs(_emulate_packed(pk, data.size, perm, data.diag_jump[perm], x) - ref).max() < 1e-12
+
+
+def test_pack_all_near(c1):
+    """A leaf size of the whole boundary leaves no far block: no z tasks, empty
+    leaf matrices, the matvec is diag + near only."""
+    _, geo, bd = c1
+    data = build_hmatrix_device(geo, HMatrixBuildConfig(leaf_size=4096), ops=HostOps(geo, bd))
+    assert len(data.far_blocks) == 0
+    pk = _pack(data)
+    assert pk["nztask"] == 0 and pk["kw_max"] == 0 and pk["nz"] == 0
+    perm = np.asarray(data.perm)
+    x = np.random.default_rng(3).standard_normal(geo.Nb)
+    assert np.abs(_emulate_packed(pk, data.size, perm, data.diag_jump[perm], x) - _matvec(data, x)).max() < 1e-12
+
+
+def test_pack_fp32_near_values(c1):
+    """FP32 near values: same tables, entries rounded once."""
+    from smx.hmatrix.backend import pack_operator
+    _, geo, bd = c1
+    data = build_hmatrix_device(geo, HMatrixBuildConfig(leaf_size=32), ops=HostOps(geo, bd))
+    fb = data.far_blocks
+    pk = pack_operator(data.size, [b.row_start for b in fb], [b.row_stop for b in fb], [b.col_start for b in fb],
+                       [b.col_stop for b in fb], [b.rank for b in fb], [b.U for b in fb], [b.V for b in fb],
+                       data.near_csr, np.float64, np.float32)
+    assert pk["near_val"].dtype == np.float32
+    nz = pk["near_val"] != 0
+    ref = _pack(data)
+    assert np.array_equal(pk["near_idx"], ref["near_idx"])
+    assert np.array_equal(pk["near_val"][nz], ref["near_val"][nz].astype(np.float32))
