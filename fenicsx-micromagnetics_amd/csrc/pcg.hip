@@ -26,7 +26,12 @@
 // Cost model (film operator, N = 202005, 7 entries per row): a dependent
 // empty launch costs about 2.8 us and the 27 MB an iteration streams add about
 // 6 us, so an iteration is bound by the kernel boundary and the memory system
-// together, not by instruction issue.
+// together, not by instruction issue.  Inside a workgroup the two reductions
+// sit on that dependent chain, so each is one DPP wave total plus ONE LDS-only
+// barrier (two LDS buffers); the shuffle-based two-barrier form cost 1.7 us
+// more per iteration at this size.  Gathering the neighbour values before the
+// first reduction needs 162 VGPRs, which halves the resident workgroups per CU
+// and loses.
 #include "smx_common.h"
 
 namespace {
@@ -69,29 +74,36 @@ __device__ __forceinline__ int xcd_block(int G) {
   return x * q + (x < r ? x : r) + k;
 }
 
-template <int BS, int NQ>
+// Workgroup sum of NQ values per thread.  Wave totals go through DPP lane
+// moves; the waves meet at ONE LDS-only barrier, so each call site needs its
+// own `scr` (NQ * BS/64 doubles): a buffer reused by a later reduction in the
+// same launch would need a second barrier.  ALL: every thread gets the totals
+// (else only thread 0).  The summation order is fixed, so every workgroup that
+// reduces the same values gets bit-identical totals.
+template <int BS, int NQ, bool ALL>
 __device__ __forceinline__ void block_total(double (&v)[NQ], double* scr) {
-  const int lane = threadIdx.x & 63;
-  const int wid = threadIdx.x >> 6;
   constexpr int nw = BS / 64;
+  const int wid = threadIdx.x >> 6;
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) v[q] = smx::wave_sum(v[q]);
-  __syncthreads();
-  if (lane == 0) {
+  for (int q = 0; q < NQ; ++q) v[q] = smx::wave_total(v[q]);
+  if ((threadIdx.x & 63) == 0) {
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) scr[q * nw + wid] = v[q];
+    for (int q = 0; q < NQ; ++q) scr[wid * NQ + q] = v[q];
   }
-  __syncthreads();
+  smx::lds_barrier();
+  if (ALL || threadIdx.x == 0) {
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) {
-    double s = 0.0;
-    for (int w = 0; w < nw; ++w) s += scr[q * nw + w];
-    v[q] = s;
+    for (int q = 0; q < NQ; ++q) {
+      double s = 0.0;
+#pragma unroll
+      for (int w = 0; w < nw; ++w) s += scr[w * NQ + q];
+      v[q] = s;
+    }
   }
 }
 
 // Sum of NQ x G partials (quantity q of block b at part[q*G + b]), identical
-// in every workgroup.
+// in every workgroup and every thread.
 template <int BS, int NQ>
 __device__ __forceinline__ void reduce_partials(const double* __restrict__ part, int G,
                                                 double (&t)[NQ], double* scr) {
@@ -101,14 +113,14 @@ __device__ __forceinline__ void reduce_partials(const double* __restrict__ part,
 #pragma unroll
     for (int q = 0; q < NQ; ++q) t[q] += part[q * G + b];
   }
-  block_total<BS, NQ>(t, scr);
+  block_total<BS, NQ, true>(t, scr);
 }
 
 template <int BS, int NQ>
 __device__ __forceinline__ void publish_partials(double (&v)[NQ], double* __restrict__ part, int G,
                                                  double* scr, int blk = -1) {
   if (blk < 0) blk = blockIdx.x;
-  block_total<BS, NQ>(v, scr);
+  block_total<BS, NQ, false>(v, scr);
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int q = 0; q < NQ; ++q) part[q * G + blk] = v[q];
@@ -123,13 +135,14 @@ __global__ __launch_bounds__(BS) void cg_init_a_kernel(
     const int32_t* __restrict__ bpos, const double* __restrict__ gb, double* __restrict__ x,
     int32_t warm, const int32_t* __restrict__ perm, CGPtrs P) {
   __shared__ double scr[BS / 64 * 1];
+  __shared__ double scr_pub[BS / 64 * 1];
   const int64_t row = (int64_t)blockIdx.x * BS + threadIdx.x;
   double shift = 0.0;
   if (mode == 0 && project && b_part) {
     // sum of the right-hand side partials (the divergence kernel's blocks)
     double t[1] = {0.0};
     for (int64_t k = threadIdx.x; k < nb_part; k += BS) t[0] += b_part[k];
-    block_total<BS, 1>(t, scr);
+    block_total<BS, 1, true>(t, scr);
     shift = t[0] / (double)n;
   }
   double v[1] = {0.0};
@@ -179,7 +192,7 @@ __global__ __launch_bounds__(BS) void cg_init_a_kernel(
     P.p[row] = 0.0;
     v[0] = bk * bk;
   }
-  publish_partials<BS, 1>(v, P.pbb, P.G, scr);
+  publish_partials<BS, 1>(v, P.pbb, P.G, scr_pub);
 }
 
 // x_p[i] = x[perm[i]] (warm start in solver order)
@@ -204,7 +217,8 @@ __global__ __launch_bounds__(BS) void cg_init_b_kernel(int64_t n, int32_t W, int
                                                        const double* __restrict__ dinv,
                                                        const double* __restrict__ x, CGPtrs P,
                                                        double rtol, double atol) {
-  __shared__ double scr[BS / 64 * 4];
+  __shared__ double scr[BS / 64 * 1];
+  __shared__ double scr_pub[BS / 64 * 4];
   double t[1];
   reduce_partials<BS, 1>(P.pbb, P.G, t, scr);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -235,7 +249,7 @@ __global__ __launch_bounds__(BS) void cg_init_b_kernel(int64_t n, int32_t W, int
     v[2] = rk * rk;
     v[3] = x[row];
   }
-  publish_partials<BS, 4>(v, P.part[0], P.G, scr);
+  publish_partials<BS, 4>(v, P.part[0], P.G, scr_pub);
 }
 
 template <int BS, int WMAX>
@@ -246,7 +260,11 @@ __global__ __launch_bounds__(BS) void cg_iter_kernel(int64_t n, int32_t W, int64
                                                      double* __restrict__ x, CGPtrs P, int32_t it,
                                                      int32_t max_it) {
   __shared__ double scr[BS / 64 * 4];
-  if (P.ss[SS_DONE] != 0.0) return;  // uniform: written by an earlier launch
+  __shared__ double scr_pub[BS / 64 * 4];
+  // Written by an earlier launch.  Tested only after the loads below are in
+  // flight: a return at the top would put this load's latency in front of
+  // every other one in every iteration.
+  const bool was_done = P.ss[SS_DONE] != 0.0;
   const int q = it & 1;
   const double* __restrict__ rq = P.r[q];
   const double* __restrict__ wq = P.w[q];
@@ -285,6 +303,7 @@ __global__ __launch_bounds__(BS) void cg_iter_kernel(int64_t n, int32_t W, int64
   }
   const bool conv = rr <= tol2;
   const bool fail = !conv && !(gam > 0.0 && isfinite(alpha));
+  if (was_done) return;  // uniform
   const bool stop = conv || fail || it >= max_it;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     P.ss[SS_ITERS] = (double)it;
@@ -325,7 +344,7 @@ __global__ __launch_bounds__(BS) void cg_iter_kernel(int64_t n, int32_t W, int64
     v[2] = r_n * r_n;
     v[3] = x_n;
   }
-  publish_partials<BS, 4>(v, P.part[1 - q], P.G, scr, blk);
+  publish_partials<BS, 4>(v, P.part[1 - q], P.G, scr_pub, blk);
 }
 
 int block_size_for(int64_t n) { return n <= 65536 ? 256 : 512; }
