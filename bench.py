@@ -90,6 +90,7 @@ def cg_iteration_timing(d, K=200):
     from smx import _capi
     L = _capi.lib()
     A = d.A_u1
+    raw = d.A_raw
     x = torch.zeros_like(d.u1)
     its = C.c_int32(0)
     rn = C.c_double(0.0)
@@ -97,8 +98,8 @@ def cg_iteration_timing(d, K=200):
 
     def solve(k):
         L.smx_pcg_jacobi(d.N, A.width, A.ld, A.cols.data_ptr(), A.vals.data_ptr(), d.dinv_u1.data_ptr(), 0,
-                         d.b_u1.data_ptr(), d.b_part.data_ptr(), d.b_part.numel(), 1, A.width, A.ld,
-                         A.cols.data_ptr(), A.vals.data_ptr(), None, None, x.data_ptr(), 0,
+                         d.b_u1.data_ptr(), d.b_part.data_ptr(), d.b_part.numel(), 1, raw.width, raw.ld,
+                         raw.cols.data_ptr(), raw.vals.data_ptr(), None, None, x.data_ptr(), 0,
                          _capi.ptr(d.cg_perm), 0.0, 0.0, k, 10 ** 6,
                          k, d.ws1.data_ptr(), C.byref(its), C.byref(rn), sums, _capi.stream_ptr())
 
@@ -278,16 +279,17 @@ def pmc_probe(config):
     torch.cuda.synchronize()
     mesh = M.film_c3() if config == "film" else M.cube_c2()
     fem = assemble_fem(mesh.X, mesh.tets)
-    from smx.demag import OperatorSet
+    from smx.demag import OperatorSet, solver_operator
     ops = OperatorSet(mesh, dev, fem)
     perm = ops.solver_order()                     # the demag solver's row order
-    A = ops.compact(fem.stiff, perm)
+    A_plain = ops.compact(fem.stiff, perm)
     g = fem.graph
     r = np.repeat(np.arange(fem.N), np.diff(g.indptr))
     dg = np.zeros(fem.N)
     on = r == g.indices
     dg[r[on]] = fem.stiff[on]
     dinv = to_dev(1.0 / dg[perm], dev)
+    A = solver_operator(A_plain, dinv)            # A D^-1, diagonal in slot 0
     pd = to_dev(perm.astype(np.int32), dev)
     b = np.random.default_rng(0).standard_normal(fem.N)
     b -= b.mean()
@@ -299,7 +301,8 @@ def pmc_probe(config):
     its, rn, sums = C.c_int32(0), C.c_double(0.0), (C.c_double * 1)()
     K = 200
     L.smx_pcg_jacobi(fem.N, A.width, A.ld, A.cols.data_ptr(), A.vals.data_ptr(), dinv.data_ptr(), 0,
-                     bd.data_ptr(), bpart.data_ptr(), 1, 1, A.width, A.ld, A.cols.data_ptr(), A.vals.data_ptr(),
+                     bd.data_ptr(), bpart.data_ptr(), 1, 1, A_plain.width, A_plain.ld, A_plain.cols.data_ptr(),
+                     A_plain.vals.data_ptr(),
                      None, None, x.data_ptr(), 0, pd.data_ptr(), 0.0, 0.0, K, 10 ** 6, K, ws.data_ptr(), C.byref(its),
                      C.byref(rn), sums, _capi.stream_ptr())
     torch.cuda.synchronize()

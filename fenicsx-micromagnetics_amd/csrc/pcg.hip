@@ -17,21 +17,34 @@
 // costs more than re-reading a few KB of partials from the Infinity Cache.
 // Scalars carried between launches are double-buffered by iteration parity;
 // workgroup 0 also publishes a status record that the host polls every
-// `check_every` launches.  Once converged every later launch returns at once.
+// `check_every` launches.  Once converged every later launch returns early.
 //
 // Matrices are ELL: column j of row i at cols[j*ld + i] (width W, padding
 // entries have value 0 and a valid column), so a wavefront's index and value
 // loads are contiguous and the row start needs no extra dependent load.
+// The CG operator is given in SOLVER FORM (smx_pcg_prepare builds it): the
+// diagonal entry of every row sits in slot 0 and every value is a_ij * dinv_j,
+// i.e. the matrix is A D^-1.  Then
+//   w_{i+1} = A D^-1 r_{i+1} = sum_j (a_ij dinv_j) (r_j - a s_j)
+// needs three neighbour values per entry instead of four, and the row's own
+// r, w, s come from the slot-0 gather instead of separate loads.
 //
 // Cost model (film operator, N = 202005, 7 entries per row): a dependent
 // empty launch costs about 2.8 us and the 27 MB an iteration streams add about
 // 6 us, so an iteration is bound by the kernel boundary and the memory system
-// together, not by instruction issue.  Inside a workgroup the two reductions
-// sit on that dependent chain, so each is one DPP wave total plus ONE LDS-only
-// barrier (two LDS buffers); the shuffle-based two-barrier form cost 1.7 us
-// more per iteration at this size.  Gathering the neighbour values before the
-// first reduction needs 162 VGPRs, which halves the resident workgroups per CU
-// and loses.
+// together, not by instruction issue.  What sits on the dependent chain inside
+// a launch is kept short:
+//  - the partial sums are the FIRST loads issued, and every load before the
+//    reduction is unconditional (rows past n read row n-1), so the reduction
+//    waits on the partials only while the streaming loads are in flight (the
+//    hardware retires loads in issue order, and a branch around some loads
+//    makes the compiler wait as if they had not been issued);
+//  - each reduction is one DPP wave total plus ONE LDS-only barrier;
+//  - loads use a uniform base plus a 32-bit offset, one offset register per
+//    neighbour shared by r, w and s;
+//  - a stencil of width 7 is compiled with its exact width.
+// Gathering the neighbour values before the first reduction needs more than
+// 128 VGPRs, which halves the resident workgroups per CU and loses.
 #include "smx_common.h"
 
 namespace {
@@ -43,7 +56,7 @@ enum {
   SS_RR = 2,
   SS_TOL2 = 3,
   SS_BB = 4,
-  SS_FAIL = 5,
+  SS_FAIL = 5,  // 1 breakdown, 2 max_it reached, 3 operator not in solver form
   SS_SUMX = 6,
   SS_SIZE = 8,
   // scalars carried to the next launch: [gamma, alpha]
@@ -56,7 +69,7 @@ struct CGPtrs {
   double* ss;        // status record
   double* st[2];     // parity-buffered scalars
   double* part[2];   // parity-buffered partials, 4 x G
-  double* pbb;       // G partials of ||b||^2
+  double* pbb;       // 2 x G partials: ||b||^2 and rows whose slot 0 is not the diagonal
   double* r[2];
   double* w[2];
   double* s[2];
@@ -72,6 +85,13 @@ __device__ __forceinline__ int xcd_block(int G) {
   const int b = blockIdx.x, x = b & 7, k = b >> 3;
   const int q = G >> 3, r = G & 7;
   return x * q + (x < r ? x : r) + k;
+}
+
+// Element at a byte offset from a uniform base: the SGPR-base + VGPR-offset
+// load form, without a 64-bit address register per load.
+template <typename T>
+__device__ __forceinline__ T ldo(const T* __restrict__ base, uint32_t byte_off) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
 }
 
 // Workgroup sum of NQ values per thread.  Wave totals go through DPP lane
@@ -127,15 +147,54 @@ __device__ __forceinline__ void publish_partials(double (&v)[NQ], double* __rest
   }
 }
 
+// Solver form, one thread per row: the first slot whose column is the row
+// moves to slot 0 (slot 0's entry takes its place) and every value is scaled
+// by dinv of its column.  A row without a diagonal entry is copied unchanged
+// (the solver's layout check then rejects the operator).  out may alias in:
+// every slot is read before it is written, slots 0 and d from saved copies.
+__global__ __launch_bounds__(256) void cg_prepare_kernel(int64_t n, int32_t W, int64_t ld,
+                                                         const int32_t* cols, const double* vals,
+                                                         const double* __restrict__ dinv,
+                                                         int32_t* cols_out, double* vals_out) {
+  const int64_t row = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (row >= n) return;
+  int d = -1;
+  for (int j = 0; j < W; ++j) {
+    if (cols[j * ld + row] == (int32_t)row) {
+      d = j;
+      break;
+    }
+  }
+  if (d < 0) {
+    for (int j = 0; j < W; ++j) {
+      cols_out[j * ld + row] = cols[j * ld + row];
+      vals_out[j * ld + row] = vals[j * ld + row];
+    }
+    return;
+  }
+  const int32_t c0 = cols[row], cd = cols[d * ld + row];
+  const double v0 = vals[row], vd = vals[d * ld + row];
+  for (int j = 0; j < W; ++j) {
+    const int32_t c = j == 0 ? cd : (j == d ? c0 : cols[j * ld + row]);
+    const double v = j == 0 ? vd : (j == d ? v0 : vals[j * ld + row]);
+    cols_out[j * ld + row] = c;
+    vals_out[j * ld + row] = v * dinv[c];
+  }
+}
+
+// r0 = b - A x0 (mode 0) or the lifted residual (mode 1); partials of ||b||^2
+// and of the rows whose slot 0 is not their diagonal (acols: the solver-form
+// operator, checked in both modes).
 template <int BS, int WMAX>
 __global__ __launch_bounds__(BS) void cg_init_a_kernel(
     int64_t n, int32_t mode, int32_t W, int64_t ld, const int32_t* __restrict__ cols,
-    const double* __restrict__ vals, const double* __restrict__ b,
+    const double* __restrict__ vals, const int32_t* __restrict__ acols,
+    const double* __restrict__ dinv, const double* __restrict__ b,
     const double* __restrict__ b_part, int64_t nb_part, int32_t project,
     const int32_t* __restrict__ bpos, const double* __restrict__ gb, double* __restrict__ x,
     int32_t warm, const int32_t* __restrict__ perm, CGPtrs P) {
   __shared__ double scr[BS / 64 * 1];
-  __shared__ double scr_pub[BS / 64 * 1];
+  __shared__ double scr_pub[BS / 64 * 2];
   const int64_t row = (int64_t)blockIdx.x * BS + threadIdx.x;
   double shift = 0.0;
   if (mode == 0 && project && b_part) {
@@ -145,16 +204,21 @@ __global__ __launch_bounds__(BS) void cg_init_a_kernel(
     block_total<BS, 1, true>(t, scr);
     shift = t[0] / (double)n;
   }
-  double v[1] = {0.0};
+  double v[2] = {0.0, 0.0};
   if (row < n) {
     double r0, bk;
     if (mode == 0) {
       bk = b[perm ? perm[row] : row] - shift;
       double ax = 0.0;
       if (warm) {
+        // solver-form values are a_ij dinv_j: A x = sum_j (a_ij dinv_j) (x_j / dinv_j)
 #pragma unroll
-        for (int j = 0; j < WMAX; ++j)
-          if (j < W) ax = fma(vals[j * ld + row], x[cols[j * ld + row]], ax);
+        for (int j = 0; j < WMAX; ++j) {
+          if (j < W) {
+            const int32_t c = cols[j * ld + row];
+            ax = fma(vals[j * ld + row], x[c] / dinv[c], ax);
+          }
+        }
       } else {
         x[row] = 0.0;
       }
@@ -191,8 +255,9 @@ __global__ __launch_bounds__(BS) void cg_init_a_kernel(
     P.s[0][row] = 0.0;
     P.p[row] = 0.0;
     v[0] = bk * bk;
+    v[1] = acols[row] != (int32_t)row ? 1.0 : 0.0;
   }
-  publish_partials<BS, 1>(v, P.pbb, P.G, scr_pub);
+  publish_partials<BS, 2>(v, P.pbb, P.G, scr_pub);
 }
 
 // x_p[i] = x[perm[i]] (warm start in solver order)
@@ -217,17 +282,17 @@ __global__ __launch_bounds__(BS) void cg_init_b_kernel(int64_t n, int32_t W, int
                                                        const double* __restrict__ dinv,
                                                        const double* __restrict__ x, CGPtrs P,
                                                        double rtol, double atol) {
-  __shared__ double scr[BS / 64 * 1];
+  __shared__ double scr[BS / 64 * 2];
   __shared__ double scr_pub[BS / 64 * 4];
-  double t[1];
-  reduce_partials<BS, 1>(P.pbb, P.G, t, scr);
+  double t[2];
+  reduce_partials<BS, 2>(P.pbb, P.G, t, scr);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     const double t1 = rtol * rtol * t[0], t2 = atol * atol;
     P.ss[SS_BB] = t[0];
     P.ss[SS_TOL2] = t1 > t2 ? t1 : t2;
-    P.ss[SS_DONE] = 0.0;
-    P.ss[SS_FAIL] = 0.0;
     P.ss[SS_ITERS] = 0.0;
+    P.ss[SS_DONE] = t[1] > 0.0 ? 1.0 : 0.0;  // a layout error stops before iteration 0
+    P.ss[SS_FAIL] = t[1] > 0.0 ? 3.0 : 0.0;
   }
   const int64_t row = (int64_t)blockIdx.x * BS + threadIdx.x;
   double v[4] = {0.0, 0.0, 0.0, 0.0};
@@ -236,10 +301,7 @@ __global__ __launch_bounds__(BS) void cg_init_b_kernel(int64_t n, int32_t W, int
     double acc = 0.0;
 #pragma unroll
     for (int j = 0; j < WMAX; ++j) {
-      if (j < W) {
-        const int32_t c = cols[j * ld + row];
-        acc = fma(vals[j * ld + row], r[c] * dinv[c], acc);
-      }
+      if (j < W) acc = fma(vals[j * ld + row], r[cols[j * ld + row]], acc);
     }
     P.w[0][row] = acc;
     const double rk = r[row];
@@ -252,7 +314,8 @@ __global__ __launch_bounds__(BS) void cg_init_b_kernel(int64_t n, int32_t W, int
   publish_partials<BS, 4>(v, P.part[0], P.G, scr_pub);
 }
 
-template <int BS, int WMAX>
+// One CG iteration.  EXACT: the ELL width is WMAX (no per-slot predicate).
+template <int BS, int WMAX, bool EXACT>
 __global__ __launch_bounds__(BS) void cg_iter_kernel(int64_t n, int32_t W, int64_t ld,
                                                      const int32_t* __restrict__ cols,
                                                      const double* __restrict__ vals,
@@ -261,35 +324,42 @@ __global__ __launch_bounds__(BS) void cg_iter_kernel(int64_t n, int32_t W, int64
                                                      int32_t max_it) {
   __shared__ double scr[BS / 64 * 4];
   __shared__ double scr_pub[BS / 64 * 4];
-  // Written by an earlier launch.  Tested only after the loads below are in
-  // flight: a return at the top would put this load's latency in front of
-  // every other one in every iteration.
+  // Written by an earlier launch; tested once the loads below are in flight.
   const bool was_done = P.ss[SS_DONE] != 0.0;
   const int q = it & 1;
   const double* __restrict__ rq = P.r[q];
   const double* __restrict__ wq = P.w[q];
   const double* __restrict__ sq = P.s[q];
-  const int blk = xcd_block(P.G);
-  const int64_t row = (int64_t)blk * BS + threadIdx.x;
-  // own-row and index loads do not depend on the scalars: issue them first
-  int32_t c[WMAX];
-  double av[WMAX];
-  double dk = 0.0, r0 = 0.0, w0 = 0.0, s0 = 0.0, p0 = 0.0, x0 = 0.0;
-  if (row < n) {
-#pragma unroll
-    for (int j = 0; j < WMAX; ++j) {
-      c[j] = j < W ? cols[j * ld + row] : 0;
-      av[j] = j < W ? vals[j * ld + row] : 0.0;
-    }
-    dk = dinv[row];
-    r0 = rq[row];
-    w0 = wq[row];
-    s0 = sq[row];
-    p0 = P.p[row];
-    x0 = x[row];
-  }
+  const int G = P.G;
+  const double* __restrict__ part = P.part[q];
+  const int b0 = (int)threadIdx.x < G ? (int)threadIdx.x : 0;
   double t[4];
-  reduce_partials<BS, 4>(P.part[q], P.G, t, scr);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) t[k] = part[k * G + b0];
+  const int blk = xcd_block(G);
+  const int64_t row = (int64_t)blk * BS + threadIdx.x;
+  const int64_t rowc = row < n ? row : n - 1;
+  const int Wk = EXACT ? WMAX : W;
+  const uint32_t o4 = (uint32_t)rowc * 4u, o8 = (uint32_t)rowc * 8u;
+  int32_t c[WMAX];  // byte offsets are formed after the reduction, where they are used
+  double av[WMAX];
+#pragma unroll
+  for (int j = 0; j < WMAX; ++j) {
+    c[j] = j < Wk ? ldo(cols + j * ld, o4) : (int32_t)rowc;
+    av[j] = j < Wk ? ldo(vals + j * ld, o8) : 0.0;
+  }
+  const double dk = ldo(dinv, o8);
+  const double p0 = ldo((const double*)P.p, o8);
+  const double x0 = ldo((const double*)x, o8);
+  if ((int)threadIdx.x >= G) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) t[k] = 0.0;
+  }
+  for (int b = threadIdx.x + BS; b < G; b += BS) {  // only when G > BS
+#pragma unroll
+    for (int k = 0; k < 4; ++k) t[k] += part[k * G + b];
+  }
+  block_total<BS, 4, true>(t, scr);
   const double gam = t[0], del = t[1], rr = t[2];
   const double tol2 = P.ss[SS_TOL2];
   double alpha, beta;
@@ -320,13 +390,18 @@ __global__ __launch_bounds__(BS) void cg_iter_kernel(int64_t n, int32_t W, int64
   if (stop) return;
   double v[4] = {0.0, 0.0, 0.0, 0.0};
   if (row < n) {
-    double acc = 0.0;
+    double rj[WMAX], wj[WMAX], sj[WMAX];
 #pragma unroll
     for (int j = 0; j < WMAX; ++j) {
-      const int32_t cc = c[j];
-      const double rc = (rq[cc] - alpha * (wq[cc] + beta * sq[cc])) * dinv[cc];
-      acc = fma(av[j], rc, acc);
+      const uint32_t oc = (uint32_t)c[j] * 8u;
+      rj[j] = ldo(rq, oc);
+      wj[j] = ldo(wq, oc);
+      sj[j] = ldo(sq, oc);
     }
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < WMAX; ++j) acc = fma(av[j], rj[j] - alpha * (wj[j] + beta * sj[j]), acc);
+    const double r0 = rj[0], w0 = wj[0], s0 = sj[0];  // slot 0 is the row itself
     const double s_n = w0 + beta * s0;
     const double p_n = r0 * dk + beta * p0;
     const double x_n = x0 + alpha * p_n;
@@ -344,7 +419,7 @@ __global__ __launch_bounds__(BS) void cg_iter_kernel(int64_t n, int32_t W, int64
     v[2] = r_n * r_n;
     v[3] = x_n;
   }
-  publish_partials<BS, 4>(v, P.part[1 - q], P.G, scr_pub, blk);
+  publish_partials<BS, 4>(v, P.part[1 - q], G, scr_pub, blk);
 }
 
 int block_size_for(int64_t n) { return n <= 65536 ? 256 : 512; }
@@ -365,7 +440,7 @@ Layout layout(int64_t n) {
   return L;
 }
 
-template <int BS, int WMAX>
+template <int BS, int WMAX, bool EXACT>
 int run_pcg(int64_t n, int32_t W, int64_t ld, const int32_t* cols, const double* vals,
             const double* dinv, int32_t mode, const double* b, const double* b_part,
             int64_t nb_part, int32_t project, int32_t raw_W, int64_t raw_ld,
@@ -382,11 +457,12 @@ int run_pcg(int64_t n, int32_t W, int64_t ld, const int32_t* cols, const double*
   }
   if (mode == 0) {
     hipLaunchKernelGGL((cg_init_a_kernel<BS, WMAX>), dim3(G), dim3(BS), 0, st, n, mode, W, ld,
-                       cols, vals, b, b_part, nb_part, project, bpos, gb, x, warm, perm, P);
+                       cols, vals, cols, dinv, b, b_part, nb_part, project, bpos, gb, x, warm,
+                       perm, P);
   } else {
     hipLaunchKernelGGL((cg_init_a_kernel<BS, WMAX>), dim3(G), dim3(BS), 0, st, n, mode, raw_W,
-                       raw_ld, raw_cols, raw_vals, b, b_part, nb_part, project, bpos, gb, x, warm,
-                       perm, P);
+                       raw_ld, raw_cols, raw_vals, cols, dinv, b, b_part, nb_part, project, bpos,
+                       gb, x, warm, perm, P);
   }
   SMX_LAUNCH_CHECK();
   hipLaunchKernelGGL((cg_init_b_kernel<BS, WMAX>), dim3(G), dim3(BS), 0, st, n, W, ld, cols, vals,
@@ -399,8 +475,8 @@ int run_pcg(int64_t n, int32_t W, int64_t ld, const int32_t* cols, const double*
   while (!done && launched <= max_it) {
     const int32_t nb = batch < (max_it + 1 - launched) ? batch : (max_it + 1 - launched);
     for (int32_t k = 0; k < nb; ++k) {
-      hipLaunchKernelGGL((cg_iter_kernel<BS, WMAX>), dim3(G), dim3(BS), 0, st, n, W, ld, cols,
-                         vals, dinv, x, P, launched + k, max_it);
+      hipLaunchKernelGGL((cg_iter_kernel<BS, WMAX, EXACT>), dim3(G), dim3(BS), 0, st, n, W, ld,
+                         cols, vals, dinv, x, P, launched + k, max_it);
     }
     SMX_LAUNCH_CHECK();
     launched += nb;
@@ -418,9 +494,23 @@ int run_pcg(int64_t n, int32_t W, int64_t ld, const int32_t* cols, const double*
 
 }  // namespace
 
+extern "C" int smx_pcg_prepare(int64_t n, int32_t width, int64_t ld, const int32_t* cols,
+                               const double* vals, const double* dinv, int32_t* cols_out,
+                               double* vals_out, void* stream) {
+  SMX_REQUIRE(n >= 0, SMX_ERR_SHAPE, "negative size");
+  SMX_REQUIRE(width >= 1 && width <= 32 && ld >= n, SMX_ERR_SHAPE,
+              "ELL width must be in [1, 32] and ld >= n (width %d, ld %lld)", width, (long long)ld);
+  if (n == 0) return SMX_OK;
+  SMX_REQUIRE(cols && vals && dinv && cols_out && vals_out, SMX_ERR_VALUE, "null pointer argument");
+  hipLaunchKernelGGL(cg_prepare_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     smx::as_stream(stream), n, width, ld, cols, vals, dinv, cols_out, vals_out);
+  SMX_LAUNCH_CHECK();
+  return SMX_OK;
+}
+
 extern "C" int64_t smx_pcg_ws_size(int64_t n) {
   const Layout L = layout(n);
-  return HDR + 9 * L.G + 8 * L.nvec + 64;
+  return HDR + 10 * L.G + 8 * L.nvec + 64;
 }
 
 extern "C" int smx_pcg_jacobi(int64_t n, int32_t width, int64_t ld, const int32_t* cols,
@@ -440,6 +530,8 @@ extern "C" int smx_pcg_jacobi(int64_t n, int32_t width, int64_t ld, const int32_
               "ELL width must be in [1, 32] and ld >= n (width %d, ld %lld)", width, (long long)ld);
   SMX_REQUIRE(mode == 0 || (raw_width >= 1 && raw_width <= 32 && raw_ld >= n), SMX_ERR_SHAPE,
               "raw operator width must be in [1, 32]");
+  SMX_REQUIRE(n < (int64_t)1 << 28, SMX_ERR_SHAPE,
+              "CG operator too large for 32-bit byte offsets (n %lld)", (long long)n);
   if (its_out) *its_out = 0;
   if (rnorm_out) *rnorm_out = 0.0;
   if (n == 0) return SMX_OK;
@@ -457,7 +549,7 @@ extern "C" int smx_pcg_jacobi(int64_t n, int32_t width, int64_t ld, const int32_
   P.part[0] = work + HDR;
   P.part[1] = work + HDR + 4 * Lo.G;
   P.pbb = work + HDR + 8 * Lo.G;
-  double* v0 = work + HDR + 9 * Lo.G;
+  double* v0 = work + HDR + 10 * Lo.G;
   v0 += (64 - (((uintptr_t)v0 / sizeof(double)) & 63)) & 63;  // 512-byte aligned vectors
   P.r[0] = v0;
   P.r[1] = v0 + Lo.nvec;
@@ -470,21 +562,31 @@ extern "C" int smx_pcg_jacobi(int64_t n, int32_t width, int64_t ld, const int32_
   double hst[SS_SIZE] = {0};
   const int bs = block_size_for(n);
   int rc;
-#define SMX_PCG_CALL(BS, WM)                                                                     \
-  run_pcg<BS, WM>(n, width, ld, cols, vals, dinv, mode, b, b_part, nb_part, project, raw_width, \
-                  raw_ld, raw_cols, raw_vals, bpos, gb, x, warm, perm, x_perm, rtol, atol,      \
-                  max_it, check_every, expected_its, P, st, hst)
+#define SMX_PCG_CALL(BS, WM, EX)                                                                 \
+  run_pcg<BS, WM, EX>(n, width, ld, cols, vals, dinv, mode, b, b_part, nb_part, project,        \
+                      raw_width, raw_ld, raw_cols, raw_vals, bpos, gb, x, warm, perm, x_perm,   \
+                      rtol, atol, max_it, check_every, expected_its, P, st, hst)
   const int32_t wmax = (mode == 1 && raw_width > width) ? raw_width : width;
+  const bool w7 = width == 7 && wmax == 7;  // the tetrahedral Kuhn-grid stencil
   if (bs == 256) {
-    rc = wmax <= 8 ? SMX_PCG_CALL(256, 8) : wmax <= 16 ? SMX_PCG_CALL(256, 16) : SMX_PCG_CALL(256, 32);
+    rc = w7 ? SMX_PCG_CALL(256, 7, true)
+            : wmax <= 8 ? SMX_PCG_CALL(256, 8, false)
+                        : wmax <= 16 ? SMX_PCG_CALL(256, 16, false) : SMX_PCG_CALL(256, 32, false);
   } else {
-    rc = wmax <= 8 ? SMX_PCG_CALL(512, 8) : wmax <= 16 ? SMX_PCG_CALL(512, 16) : SMX_PCG_CALL(512, 32);
+    rc = w7 ? SMX_PCG_CALL(512, 7, true)
+            : wmax <= 8 ? SMX_PCG_CALL(512, 8, false)
+                        : wmax <= 16 ? SMX_PCG_CALL(512, 16, false) : SMX_PCG_CALL(512, 32, false);
   }
 #undef SMX_PCG_CALL
   if (rc != SMX_OK) return rc;
   if (its_out) *its_out = (int32_t)hst[SS_ITERS];
   if (rnorm_out) *rnorm_out = sqrt(hst[SS_RR]);
   if (sums_out) sums_out[0] = hst[SS_SUMX];
+  if (hst[SS_FAIL] == 3.0) {
+    smx::set_error("CG operator is not in solver form: some row does not hold its diagonal in "
+                   "slot 0 (build it with smx_pcg_prepare)");
+    return SMX_ERR_VALUE;
+  }
   if (hst[SS_FAIL] == 1.0) {
     smx::set_error("CG breakdown after %d iterations (rnorm %g)", (int)hst[SS_ITERS],
                    sqrt(hst[SS_RR]));

@@ -90,6 +90,7 @@ _SIGS = {
     "smx_sell_spmv_blocks": ([I64], I64),
     "smx_reduce_ws_size": ([I64], I64),
     "smx_pcg_ws_size": ([I64], I64),
+    "smx_pcg_prepare": ([I64, I32, I64, P, P, P, P, P, P], I32),
     "smx_pcg_jacobi": ([I64, I32, I64, P, P, P, I32, P, P, I64, I32, I32, I64, P, P, P, P, P, I32, P, F64,
                         F64, I32, I32, I32, P, C.POINTER(I32), C.POINTER(F64), C.POINTER(F64), P],
                        C.c_int),

@@ -119,7 +119,7 @@ class MicromagneticEnergy:
         its = C.c_int32(0)
         rn = C.c_double(0.0)
         sums = (C.c_double * 1)()
-        raw = d.A_u1
+        raw = d.A_raw
         nbp = 0 if b_part is None else b_part.numel()
         rc = self._L().smx_pcg_jacobi(self.N, A.width, A.ld, _capi.ptr(A.cols), _capi.ptr(A.vals), _capi.ptr(dinv),
                                       mode, _capi.ptr(b), _capi.ptr(b_part), nbp, int(project), raw.width, raw.ld,

@@ -53,6 +53,17 @@ def _torch_dtype(B_dtype):
     return _B_DTYPES[key]
 
 
+def solver_operator(A: DeviceEll, dinv: torch.Tensor) -> DeviceEll:
+    """The CG operator in solver form (smx_pcg_prepare): the diagonal of every
+    row in slot 0 and every value a_ij * dinv_j, i.e. A D^-1 (a new copy)."""
+    out = DeviceEll(nrows=A.nrows, width=A.width, ld=A.ld, cols=A.cols.clone(), vals=A.vals.clone(),
+                    nnz=A.nnz)                    # rows n..ld-1 (never read) keep A's padding
+    _capi.check(_capi.lib().smx_pcg_prepare(A.nrows, A.width, A.ld, _capi.ptr(A.cols), _capi.ptr(A.vals),
+                                            _capi.ptr(dinv), _capi.ptr(out.cols), _capi.ptr(out.vals),
+                                            _capi.stream_ptr()), "smx_pcg_prepare")
+    return out
+
+
 class OperatorSet:
     """Device copies of the FEM operators one mesh needs (shared between the
     demag field and the effective field)."""
@@ -180,12 +191,15 @@ class DemagFieldLindholmGPU:
         self.cg_perm = to_dev(perm.astype(np.int32), dev) if perm is not None else None
         ident = np.arange(self.N)
         sel = perm if perm is not None else ident
-        self.A_u1 = self.ops.compact(fem.stiff, perm)
-        self.A_u2 = self.ops.compact(fem.stiffness_bc(), perm)
         d1 = self._diag(fem.stiff)
         d2 = self._diag(fem.stiffness_bc())
         self.dinv_u1 = to_dev(1.0 / d1[sel], dev)
         self.dinv_u2 = to_dev(1.0 / d2[sel], dev)
+        # plain P A P^T: the lifting operator b = -A_raw g of the u2 solve
+        self.A_raw = self.ops.compact(fem.stiff, perm)
+        # the CG kernels take A D^-1 with the diagonal in slot 0
+        self.A_u1 = solver_operator(self.A_raw, self.dinv_u1)
+        self.A_u2 = solver_operator(self.ops.compact(fem.stiffness_bc(), perm), self.dinv_u2)
 
         # ---------------- boundary operator ----------------
         self.opB = geometry or build_boundary_geometry(mesh.X, mesh.tets, incident_policy)
@@ -265,7 +279,7 @@ class DemagFieldLindholmGPU:
         its = C.c_int32(0)
         rn = C.c_double(0.0)
         sums = (C.c_double * 1)()
-        raw = self.A_u1
+        raw = self.A_raw
         nbp = 0 if b_part is None else b_part.numel()
         rc = L.smx_pcg_jacobi(self.N, A.width, A.ld, _capi.ptr(A.cols), _capi.ptr(A.vals),
                               _capi.ptr(dinv), mode, _capi.ptr(b), _capi.ptr(b_part), nbp, 1,

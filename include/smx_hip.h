@@ -252,6 +252,11 @@ int64_t smx_reduce_ws_size(int64_t n);
  *
  * Matrices are ELL: entry j of row i at cols[j*ld + i] / vals[j*ld + i],
  * width <= 32, padding entries with value 0 and a valid column.
+ * The operator (cols/vals) is given in SOLVER FORM, built once per matrix by
+ * smx_pcg_prepare from A and dinv = 1/diag(A): the diagonal entry of every row
+ * in slot 0 and every value scaled by dinv of its column (A D^-1).  The call
+ * fails with SMX_ERR_VALUE, before any iteration, when some row's slot 0 is
+ * not its diagonal.  The raw operator of mode 1 stays plain A_raw.
  * mode 0 (u1, Neumann): solves A x = b - mean(b)*project, where mean(b) is
  *     (sum of the nb_part device partial sums b_part) / n; x0 = x if warm
  *     else 0.
@@ -271,6 +276,15 @@ int64_t smx_reduce_ws_size(int64_t n);
  * ---------------------------------------------------------------------- */
 #define SMX_PCG_SUMX 6
 int64_t smx_pcg_ws_size(int64_t n);
+/* Solver form of a CG operator (replaces nothing in the reference: PETSc's
+ * KSP takes A itself; this is the one-time set-up that a PCSetUp would do).
+ * Per row: the first slot whose column is the row moves to slot 0 (slot 0's
+ * entry takes its place) and every value becomes a_ij * dinv[col].  A row
+ * without a diagonal entry is copied unchanged.  cols_out/vals_out may alias
+ * cols/vals (in-place).  Applying it twice scales twice: call it once. */
+int smx_pcg_prepare(int64_t n, int32_t width, int64_t ld, const int32_t* cols,
+                    const double* vals, const double* dinv, int32_t* cols_out,
+                    double* vals_out, void* stream);
 int smx_pcg_jacobi(int64_t n, int32_t width, int64_t ld, const int32_t* cols,
                    const double* vals, const double* dinv, int32_t mode,
                    const double* b, const double* b_part, int64_t nb_part,
