@@ -59,7 +59,7 @@ enum {
   SS_FAIL = 5,  // 1 breakdown, 2 max_it reached, 3 operator not in solver form
   SS_SUMX = 6,
   SS_SIZE = 8,
-  // scalars carried to the next launch: [gamma, alpha]
+  // scalars carried to the next launch: [gamma, alpha, 1/gamma, 1/alpha]
   ST_OFF = 8,
   ST_SIZE = 8,
   HDR = 32
@@ -367,9 +367,10 @@ __global__ __launch_bounds__(BS) void cg_iter_kernel(int64_t n, int32_t W, int64
     beta = 0.0;
     alpha = gam / del;
   } else {
-    const double g_old = P.st[q][0], a_old = P.st[q][1];
-    beta = gam / g_old;
-    alpha = gam / (del - beta * gam / a_old);
+    // reciprocals stored by the previous launch: one division on the chain
+    const double inv_g_old = P.st[q][2], inv_a_old = P.st[q][3];
+    beta = gam * inv_g_old;
+    alpha = gam / (del - beta * gam * inv_a_old);
   }
   const bool conv = rr <= tol2;
   const bool fail = !conv && !(gam > 0.0 && isfinite(alpha));
@@ -385,6 +386,8 @@ __global__ __launch_bounds__(BS) void cg_iter_kernel(int64_t n, int32_t W, int64
     } else {
       P.st[1 - q][0] = gam;
       P.st[1 - q][1] = alpha;
+      P.st[1 - q][2] = 1.0 / gam;
+      P.st[1 - q][3] = 1.0 / alpha;
     }
   }
   if (stop) return;
