@@ -184,10 +184,22 @@ static_assert(kZG == 8, "the reduce-scatter below is written for 8 ranks per tas
 // keeps kZG partial sums, then a butterfly reduce-scatter (xor 32, 16, 8 halve
 // the set, xor 4, 2, 1 finish the sums) leaves rank g's total in lanes whose
 // bits 5..3 spell g.  Fixed order, so the result is deterministic.  The task
-// record holds everything the wavefront needs in one scalar load.
+// record holds everything the wavefront needs in one load.
+//
+// The two factor types take different forms, each the faster one measured on
+// the film's 74 379 tasks (one in five has kc < 8):
+//  - FP32: the task index goes through readfirstlane, so the record is a
+//    scalar load and rows past kc are skipped by a scalar branch: 127 us
+//    against 146 us for the branch-free body that re-reads row kc-1;
+//  - FP64: eight rows of four doubles fill the 64-VGPR budget of 8 waves per
+//    SIMD.  With a scalar record the compiler serializes the row loads
+//    (290-470 us), so the record stays per-lane and the body stays branch-free
+//    with row kc-1 re-read (250 us).
 template <typename T>
 __global__ __launch_bounds__(256, 8) void hm_far_z_kernel(smx_hmatrix H) {
-  const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  constexpr bool kF32 = sizeof(T) == 4;
+  const int wv = (int)threadIdx.x >> 6;
+  const int64_t t = (int64_t)blockIdx.x * 4 + (kF32 ? __builtin_amdgcn_readfirstlane(wv) : wv);
   const int lane = threadIdx.x & 63;
   if (t >= H.nztask) return;  // wave-uniform; the kernel has no barrier
   const smx_hmat_ztask task = H.ztask[t];
@@ -205,7 +217,12 @@ __global__ __launch_bounds__(256, 8) void hm_far_z_kernel(smx_hmatrix H) {
     for (int q = 0; q < 4; ++q) xv[q] = x[(j + q) < n ? (j + q) : (n - 1)];  // padded V entries are 0
 #pragma unroll
     for (int g = 0; g < kZG; ++g) {
-      const int gg = g < kc ? g : kc - 1;  // rows past kc re-read row kc-1; their sums are dropped
+      int gg = g;
+      if constexpr (kF32) {
+        if (g >= kc) break;
+      } else {
+        gg = g < kc ? g : kc - 1;  // re-read row kc-1; the sums of rows past kc are dropped
+      }
       double v[4];
       load4(Vt + (int64_t)gg * ld + j, v);
       acc[g] = fma(v[0], xv[0], acc[g]);
@@ -240,7 +257,9 @@ __global__ __launch_bounds__(NW * 64) void hm_leaf_kernel(smx_hmatrix H, double*
   extern __shared__ double zl[];
   const int64_t L = blockIdx.x;
   const int lane = threadIdx.x & 63;
-  const int wid = threadIdx.x >> 6;
+  // wave-uniform row loop (readfirstlane), so the row's wave total can use
+  // DPP lane moves and readlane: 2 % faster than shuffles on the film (FP32)
+  const int wid = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
   const int32_t l0 = H.leaf_start[L], l1 = H.leaf_stop[L];
   const int32_t kw = H.leaf_kw[L];
   {
@@ -275,7 +294,7 @@ __global__ __launch_bounds__(NW * 64) void hm_leaf_kernel(smx_hmatrix H, double*
       s = fma(u[2], zl[k + 2], s);
       s = fma(u[3], zl[k + 3], s);
     }
-    s = smx::wave_sum(s);
+    s = smx::wave_total(s);
     if (lane == 0) y[H.perm[p]] = H.diag_p[p] * xp[p] + s;
   }
 }
