@@ -42,6 +42,9 @@ HBM_PEAK_GBS = 8000.0
 GAMMA = 2.211e5
 
 
+PC_TYPE = "gamg"
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -83,9 +86,29 @@ def event_time(fn, reps, flush=None):
     return sum(ts) / len(ts)
 
 
+def amg_iteration_bytes(d):
+    """Algorithmic bytes of one AMG-CG iteration of the u1 solve: every
+    operator entry once (4-byte column + value), every vector read or written
+    by a launch once (the restriction's reads of r, w, s are the update's)."""
+    h = d.amg_u1
+    vs = 4 if h.fp32 else 8
+    N = d.N
+    lv = h.host
+    op = lambda M: M.nnz * (4 + vs) + 4 * (M.shape[0] + 1)
+    nb = 80 * N                                        # update: r, w, s, u, p, x read; r, s, p, x written
+    nb += d.A_u1.nnz * 12 + 16 * N                      # w = A u: matrix, u read, w written
+    for l, L in enumerate(lv[:-1]):
+        nb += op(L.R) + 8 * lv[l + 1].n + (8 * L.n if l > 0 else 0)          # restriction
+        nb += op(L.S) + op(L.Q) + 8 * (2 * L.n + lv[l + 1].n)                # up: S b + Q x'
+    c = lv[-1]
+    nb += op(c.S) + 16 * c.n                             # coarse
+    return nb
+
+
 def cg_iteration_timing(d, K=200):
-    """Average duration of one CG iteration launch (u1 operator), events on the
-    launch stream: (solve with K iterations - solve with 0) / K."""
+    """Average duration of one CG iteration (u1 operator), events on the
+    launch stream: (solve with K iterations - solve with 0) / K.  For the AMG
+    preconditioner an iteration is its whole launch sequence."""
     import torch
     from smx import _capi
     L = _capi.lib()
@@ -95,6 +118,20 @@ def cg_iteration_timing(d, K=200):
     its = C.c_int32(0)
     rn = C.c_double(0.0)
     sums = (C.c_double * 1)()
+    if d.pc_type == "gamg":
+        K = min(K, 20)                                  # stays well above round-off for both workloads
+        h = d.amg_u1
+
+        def solve_amg(k):
+            L.smx_pcg_amg(d.N, A.width, A.ld, A.cols.data_ptr(), A.vals.data_ptr(), 0,
+                          d.b_u1.data_ptr(), d.b_part.data_ptr(), d.b_part.numel(), 1, raw.width, raw.ld,
+                          raw.cols.data_ptr(), raw.vals.data_ptr(), None, None, x.data_ptr(), 0,
+                          _capi.ptr(d.cg_perm), C.addressof(h.c_levels), h.nlev, h.val_dtype, 0.0, 0.0, k,
+                          10 ** 6, k, d.ws1.data_ptr(), C.byref(its), C.byref(rn), sums, _capi.stream_ptr())
+
+        t_k = event_time(lambda: solve_amg(K), 5)
+        t_0 = event_time(lambda: solve_amg(0), 5)
+        return (t_k - t_0) / K, amg_iteration_bytes(d)
 
     def solve(k):
         L.smx_pcg_jacobi(d.N, A.width, A.ld, A.cols.data_ptr(), A.vals.data_ptr(), d.dinv_u1.data_ptr(), 0,
@@ -349,7 +386,7 @@ def build_case(config, dev):
     from smx import mesh as M
     from smx.demag import OperatorSet
     from smx.llg import EffectiveField
-    common = dict(ksp_rtol_u1=1e-6, ksp_rtol_u2=1e-6, ksp_atol=1e-12)
+    common = dict(ksp_rtol_u1=1e-6, ksp_rtol_u2=1e-6, ksp_atol=1e-12, ksp_pc_type=PC_TYPE)
     t0 = time.perf_counter()
     if config == "cube":
         mesh = M.cube_c2()
@@ -421,14 +458,18 @@ def cpu_baseline(config, mesh, mat, pool, extra, h64, budget_s):
                             far=[(b.row_start, b.row_stop, b.col_start, b.col_stop, b.U, b.V, 0.0)
                                  for b in data.far_blocks], stats={})
         bmat = H.matvec
-        d1 = 1.0 / fem.A.diagonal()
-        d2 = 1.0 / fem.A_bc.diagonal()
-        M1 = spla.LinearOperator(fem.A.shape, matvec=lambda v: d1 * v)
-        M2 = spla.LinearOperator(fem.A.shape, matvec=lambda v: d2 * v)
-        solve1 = lambda b: spla.cg(fem.A, b, rtol=1e-6, atol=1e-12, maxiter=2000, M=M1)[0]
-        solve2 = lambda b: spla.cg(fem.A_bc, b, rtol=1e-6, atol=1e-12, maxiter=12000, M=M2)[0]
-        how = ("scipy CG + Jacobi Poisson solves (rtol 1e-6; the reference uses GAMG-CG above 60000 dofs, "
-               "absent here), H-matrix matvec with the same factors in numpy FP64")
+        from smx.amg import apply_vcycle, build_levels
+        A1 = sp.csr_matrix(fem.A)
+        A2 = sp.csr_matrix(fem.A_bc)
+        lv1 = build_levels(A1)
+        lv2 = build_levels(A2)
+        M1 = spla.LinearOperator(A1.shape, matvec=lambda v: apply_vcycle(lv1, v))
+        M2 = spla.LinearOperator(A2.shape, matvec=lambda v: apply_vcycle(lv2, v))
+        solve1 = lambda b: spla.cg(A1, b, rtol=1e-6, atol=1e-12, maxiter=2000, M=M1)[0]
+        solve2 = lambda b: spla.cg(A2, b, rtol=1e-6, atol=1e-12, maxiter=12000, M=M2)[0]
+        how = ("scipy CG preconditioned by the same smoothed-aggregation V-cycle on the host (rtol 1e-6; the "
+               "reference CPU path uses GAMG-CG above 60000 dofs), H-matrix matvec with the same factors in "
+               "numpy FP64")
     setup = time.perf_counter() - t0
     Vn = np.repeat(fem.V, 3)
 
@@ -480,10 +521,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", choices=["film", "cube", "adjoint", "disk"], default="film")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--pc", choices=["gamg", "jacobi"], default="gamg",
+                    help="CG preconditioner of both Poisson solves (the reference uses GAMG)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--pmc-probe", action="store_true",
                     help="short program for rocprofv3 --pmc passes (calibration copy + CG iterations)")
     args = ap.parse_args()
+    global PC_TYPE
+    PC_TYPE = args.pc
     if args.pmc_probe:
         pmc_probe(args.config)
         return
@@ -553,9 +598,14 @@ def main():
         "mixed_vs_fp64_rel_err_rhs": max(errs),
         "mixed_vs_fp64_rel_err_hdemag": max(herr),
         "cg_iterations_u1_u2": [float(np.mean([a for a, _ in itsmx])), float(np.mean([b for _, b in itsmx]))],
-        "roofline": {"bound": "hbm", "kernel": "cg_iter_kernel (Jacobi CG iteration, u1 operator)",
+        "pc": hmx.demag_field.pc_type,
+        "amg_levels_u1": hmx.demag_field.amg_u1.describe() if hmx.demag_field.amg_u1 else None,
+        "roofline": {"bound": "hbm", "kernel": ("AMG-CG iteration (u1: update + V-cycle + A u launches)"
+                                                if hmx.demag_field.pc_type == "gamg"
+                                                else "cg_iter_kernel (Jacobi CG iteration, u1 operator)"),
                      "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(args.config), "algorithmic_bytes": int(cg_bytes),
+                     "traffic": (measured_traffic(args.config) if hmx.demag_field.pc_type == "jacobi" else None),
+                     "algorithmic_bytes": int(cg_bytes),
                      "avg_us": cg_t * 1e6},
         "matvec_roofline": {"bound": "hbm", "kernel": mv_name, "achieved": mv_ach, "peak": HBM_PEAK_GBS,
                             "unit": "GB/s", "frac": mv_ach / HBM_PEAK_GBS, "algorithmic_bytes": int(mv_bytes),

@@ -1,0 +1,706 @@
+// Conjugate gradients preconditioned by one smoothed-aggregation AMG V-cycle.
+//
+// The V-cycle (damped-Jacobi pre- and post-smoothing, one sweep each, zero
+// initial guess on every level) is folded into three sparse operators per
+// level, built once on the host:
+//   S = 2w D^-1 - w^2 D^-1 A D^-1        (pre-smooth, residual, post-smooth)
+//   Q = (I - w D^-1 A) P                 (prolongation, then post-smoothing)
+//   R = Q^T                              (pre-smoothing, residual, restriction)
+// so that M^-1 b = S b + Q M_c^-1 (R b), with M_c^-1 the V-cycle of the next
+// level and, on the coarsest level, a dense pseudo-inverse (also stored as a
+// sparse operator with full rows).  One level of the cycle is therefore one
+// launch on the way down (b' = R b) and one on the way up (x = S b + Q x').
+//
+// CG in the Chronopoulos-Gear form (one reduction per iteration):
+//   s = w + b s;  p = u + b p;  x += a p;  r -= a s;  u = M^-1 r;  w = A u
+//   g = (r,u), d = (w,u), rr = (r,r);  b = g/g_old;  a = g/(d - b g/a_old)
+// Launches of one iteration:
+//   update   fine rows: s, p, x, r and partials of (rr, sum x);
+//            extra workgroups: b_1 = R_0 r_new, each entry of r_new rebuilt
+//            from r, w, s (r and s are double-buffered by iteration parity);
+//   down     b_{l+1} = R_l b_l                 (levels 1 .. L-2)
+//   coarse   x_c = Z b_c
+//   up       x_l = S_l b_l + Q_l x_{l+1}       (levels L-2 .. 1)
+//   up0      u = S_0 r_new + Q_0 x_1, partials of (r_new, u)
+//   matvec   w = A u, partials of (w, u)
+// Every workgroup of the next update re-reduces all partial sums in one fixed
+// order (no atomics, no device-scope fences), so every workgroup derives the
+// same scalars.  The update tests the residual of the previous iteration and
+// raises a done flag; every later launch reads it first and returns at once.
+//
+// Level operators are CSR with TPR lanes per row (TPR a power of two): each
+// lane sums every TPR-th entry of the row and the lanes of a row meet in a
+// fixed butterfly, so results do not depend on scheduling.  Values are FP64
+// or FP32 (the FP32 hierarchy halves the bytes of the preconditioner; the
+// outer CG and every vector stay FP64).
+#include "smx_common.h"
+
+namespace {
+
+enum {
+  SS_DONE = 0,
+  SS_ITERS = 1,
+  SS_RR = 2,
+  SS_TOL2 = 3,
+  SS_BB = 4,
+  SS_FAIL = 5,  // 1 breakdown, 2 max_it reached
+  SS_SUMX = 6,
+  SS_SIZE = 8,
+  ST_OFF = 8,   // [gamma, alpha, 1/gamma, 1/alpha] x 2 parities
+  ST_SIZE = 8,
+  HDR = 32
+};
+
+struct AmgPtrs {
+  double* ss;
+  double* st[2];
+  double* pbb;      // Gf partials of ||b||^2 (written once, by the init launch)
+  double* prr[2];   // 2 x Gf partials (rr, sum x), by iteration parity
+  double* pg;       // Gu partials of (r, u)
+  double* pd;       // Gf partials of (w, u)
+  double* r[2];
+  double* s[2];
+  double* w;
+  double* u;
+  double* p;
+  int Gf;           // fine workgroups (one row per thread)
+  int Gu;           // workgroups of the up0 launch
+};
+
+struct CsrOp {
+  const int32_t* ptr;
+  const int32_t* col;
+  const void* val;
+  int64_t rows;
+};
+
+// Workgroup b of G is dispatched to XCD b % 8; renumber so that each XCD
+// handles one contiguous range of blocks (bijective for every G).
+__device__ __forceinline__ int xcd_map(int b, int G) {
+  const int x = b & 7, k = b >> 3;
+  const int q = G >> 3, r = G & 7;
+  return x * q + (x < r ? x : r) + k;
+}
+
+template <int BS, int NQ, bool ALL>
+__device__ __forceinline__ void block_total(double (&v)[NQ], double* scr) {
+  constexpr int nw = BS / 64;
+  const int wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) v[q] = smx::wave_total(v[q]);
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) scr[wid * NQ + q] = v[q];
+  }
+  smx::lds_barrier();
+  if (ALL || threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      double s = 0.0;
+#pragma unroll
+      for (int w = 0; w < nw; ++w) s += scr[w * NQ + q];
+      v[q] = s;
+    }
+  }
+}
+
+template <int BS>
+__device__ __forceinline__ double sum_parts(const double* __restrict__ part, int G) {
+  double t = 0.0;
+  for (int b = threadIdx.x; b < G; b += BS) t += part[b];
+  return t;
+}
+
+template <typename VT>
+__device__ __forceinline__ double ldv(const void* v, int64_t k) {
+  return (double)static_cast<const VT*>(v)[k];
+}
+
+// Row sum of a CSR operator over TPR lanes; every lane of the row ends with
+// the total (fixed butterfly order).
+template <int TPR, typename VT>
+__device__ __forceinline__ double csr_row(const CsrOp& M, const double* __restrict__ v, int64_t row,
+                                          int lane) {
+  double acc = 0.0;
+  const int32_t k0 = M.ptr[row], k1 = M.ptr[row + 1];
+  for (int32_t k = k0 + lane; k < k1; k += TPR) acc = fma(ldv<VT>(M.val, k), v[M.col[k]], acc);
+#pragma unroll
+  for (int off = TPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  return acc;
+}
+
+// r_{i+1} = r_i - a (w_i + b s_{i-1}): one formula for the fine update and the
+// fused restriction, so both see the same residual bit for bit.
+__device__ __forceinline__ double next_s(double w, double b, double s) { return fma(b, s, w); }
+__device__ __forceinline__ double next_r(double r, double a, double sn) { return fma(-a, sn, r); }
+
+// r0 = b - A x0 (mode 0) or the lifted residual (mode 1); s = p = 0.
+// Partials: ||b||^2 (pbb) and (||r0||^2, sum x0) (prr[0]).  Block 0 resets the
+// status record.
+template <int BS, int WMAX>
+__global__ __launch_bounds__(BS) void amg_init_kernel(
+    int64_t n, int32_t mode, int32_t W, int64_t ld, const int32_t* __restrict__ cols,
+    const double* __restrict__ vals, const double* __restrict__ b,
+    const double* __restrict__ b_part, int64_t nb_part, int32_t project,
+    const int32_t* __restrict__ bpos, const double* __restrict__ gb, double* __restrict__ x,
+    int32_t warm, const int32_t* __restrict__ perm, AmgPtrs P) {
+  __shared__ double scr[BS / 64];
+  __shared__ double scr_pub[BS / 64 * 3];
+  const int blk = xcd_map(blockIdx.x, P.Gf);
+  const int64_t row = (int64_t)blk * BS + threadIdx.x;
+  double shift = 0.0;
+  if (mode == 0 && project && b_part) {
+    double t[1] = {0.0};
+    for (int64_t k = threadIdx.x; k < nb_part; k += BS) t[0] += b_part[k];
+    block_total<BS, 1, true>(t, scr);
+    shift = t[0] / (double)n;
+  }
+  double v[3] = {0.0, 0.0, 0.0};
+  if (row < n) {
+    double r0, bk, x0 = 0.0;
+    if (mode == 0) {
+      bk = b[perm ? perm[row] : row] - shift;
+      double ax = 0.0;
+      if (warm) {
+        x0 = x[row];
+#pragma unroll
+        for (int j = 0; j < WMAX; ++j) {
+          if (j < W) ax = fma(vals[j * ld + row], x[cols[j * ld + row]], ax);
+        }
+      }
+      r0 = bk - ax;
+    } else {
+      const int32_t bk_pos = bpos[row];
+      if (bk_pos >= 0) {
+        bk = gb[bk_pos];
+        x0 = bk;
+        r0 = 0.0;
+      } else {
+        double sx = 0.0, sg = 0.0;
+#pragma unroll
+        for (int j = 0; j < WMAX; ++j) {
+          if (j < W) {
+            const int32_t c = cols[j * ld + row];
+            const double a = vals[j * ld + row];
+            const int32_t cp = bpos[c];
+            if (cp >= 0) {
+              const double gv = gb[cp];
+              sx = fma(a, gv, sx);
+              sg = fma(a, gv, sg);
+            } else if (warm) {
+              sx = fma(a, x[c], sx);
+            }
+          }
+        }
+        if (warm) x0 = x[row];
+        r0 = -sx;
+        bk = -sg;
+      }
+    }
+    // Other workgroups may read x[row] as a neighbour of a warm start, so x is
+    // written only where no workgroup reads it: cold starts, and the boundary
+    // rows of mode 1 (their neighbours read gb instead).
+    if (!warm && (mode == 0 || bpos[row] < 0)) x[row] = 0.0;
+    if (mode == 1 && bpos[row] >= 0) x[row] = x0;
+    P.r[0][row] = r0;
+    P.s[0][row] = 0.0;
+    P.p[row] = 0.0;
+    v[0] = bk * bk;
+    v[1] = r0 * r0;
+    v[2] = x0;
+  }
+  block_total<BS, 3, false>(v, scr_pub);
+  if (threadIdx.x == 0) {
+    P.pbb[blk] = v[0];
+    P.prr[0][blk] = v[1];
+    P.prr[0][P.Gf + blk] = v[2];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    P.ss[SS_DONE] = 0.0;
+    P.ss[SS_FAIL] = 0.0;
+    P.ss[SS_ITERS] = 0.0;
+  }
+}
+
+// One level launch: y = M1 v1 (+ M2 v2).  MODE 1 (up0): partials of (v1, y).
+template <int BS, int TPR, typename VT, int MODE>
+__global__ __launch_bounds__(BS) void amg_level_kernel(CsrOp M1, const double* __restrict__ v1,
+                                                       CsrOp M2, const double* __restrict__ v2,
+                                                       double* __restrict__ y,
+                                                       const double* __restrict__ ss,
+                                                       double* __restrict__ part) {
+  __shared__ double scr[BS / 64];
+  if (ss[SS_DONE] != 0.0) return;  // uniform
+  constexpr int RPB = BS / TPR;
+  const int blk = xcd_map(blockIdx.x, gridDim.x);
+  const int64_t row = (int64_t)blk * RPB + threadIdx.x / TPR;
+  const int lane = threadIdx.x % TPR;
+  const int64_t rows = M1.rows;
+  double acc = 0.0;
+  if (row < rows) {
+    acc = csr_row<TPR, VT>(M1, v1, row, lane);
+    if (M2.ptr) acc += csr_row<TPR, VT>(M2, v2, row, lane);
+    if (lane == 0) y[row] = acc;
+  }
+  if (MODE == 1) {
+    double t[1] = {(row < rows && lane == 0) ? v1[row] * acc : 0.0};
+    block_total<BS, 1, false>(t, scr);
+    if (threadIdx.x == 0) part[blockIdx.x] = t[0];
+  }
+}
+
+// Start of iteration `it`: scalars from the partial sums, convergence test,
+// then (blocks < Gf) the fine update and (blocks >= Gf) b_1 = R_0 r_new.
+template <int BS, int TPR, typename VT>
+__global__ __launch_bounds__(BS) void amg_update_kernel(int64_t n, AmgPtrs P, double* __restrict__ x,
+                                                        CsrOp R0, double* __restrict__ b1,
+                                                        int32_t it, int32_t max_it, double rtol,
+                                                        double atol) {
+  __shared__ double scr[BS / 64 * 5];
+  __shared__ double scr_pub[BS / 64 * 2];
+  const bool was_done = P.ss[SS_DONE] != 0.0;
+  const int q = it & 1;
+  const int Gf = P.Gf;
+  double t[5];
+  t[0] = sum_parts<BS>(P.pg, P.Gu);
+  t[1] = sum_parts<BS>(P.pd, Gf);
+  t[2] = sum_parts<BS>(P.prr[q], Gf);
+  t[3] = sum_parts<BS>(P.prr[q] + Gf, Gf);
+  t[4] = sum_parts<BS>(P.pbb, Gf);
+  block_total<BS, 5, true>(t, scr);
+  const double gam = t[0], del = t[1], rr = t[2];
+  const double tol2 = fmax(rtol * rtol * t[4], atol * atol);
+  double alpha, beta;
+  if (it == 0) {
+    beta = 0.0;
+    alpha = gam / del;
+  } else {
+    const double inv_g_old = P.st[q][2], inv_a_old = P.st[q][3];
+    beta = gam * inv_g_old;
+    alpha = gam / (del - beta * gam * inv_a_old);
+  }
+  const bool conv = rr <= tol2;
+  const bool fail = !conv && !(gam > 0.0 && isfinite(alpha));
+  if (was_done) return;  // uniform
+  const bool stop = conv || fail || it >= max_it;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    P.ss[SS_ITERS] = (double)it;
+    P.ss[SS_RR] = rr;
+    P.ss[SS_SUMX] = t[3];
+    P.ss[SS_BB] = t[4];
+    P.ss[SS_TOL2] = tol2;
+    if (stop) {
+      P.ss[SS_DONE] = 1.0;
+      P.ss[SS_FAIL] = fail ? 1.0 : (conv ? 0.0 : 2.0);
+    } else {
+      P.st[1 - q][0] = gam;
+      P.st[1 - q][1] = alpha;
+      P.st[1 - q][2] = 1.0 / gam;
+      P.st[1 - q][3] = 1.0 / alpha;
+    }
+  }
+  if (stop) return;
+  const double* __restrict__ rq = P.r[q];
+  const double* __restrict__ sq = P.s[q];
+  const double* __restrict__ w = P.w;
+  if ((int)blockIdx.x < Gf) {
+    const int blk = xcd_map(blockIdx.x, Gf);
+    const int64_t row = (int64_t)blk * BS + threadIdx.x;
+    double v[2] = {0.0, 0.0};
+    if (row < n) {
+      const double s_n = next_s(w[row], beta, sq[row]);
+      const double p_n = fma(beta, P.p[row], P.u[row]);
+      const double x_n = fma(alpha, p_n, x[row]);
+      const double r_n = next_r(rq[row], alpha, s_n);
+      __builtin_nontemporal_store(s_n, &P.s[1 - q][row]);
+      __builtin_nontemporal_store(r_n, &P.r[1 - q][row]);
+      __builtin_nontemporal_store(p_n, &P.p[row]);
+      __builtin_nontemporal_store(x_n, &x[row]);
+      v[0] = r_n * r_n;
+      v[1] = x_n;
+    }
+    block_total<BS, 2, false>(v, scr_pub);
+    if (threadIdx.x == 0) {
+      P.prr[1 - q][blk] = v[0];
+      P.prr[1 - q][Gf + blk] = v[1];
+    }
+  } else {
+    constexpr int RPB = BS / TPR;
+    const int64_t row = (int64_t)(blockIdx.x - Gf) * RPB + threadIdx.x / TPR;
+    const int lane = threadIdx.x % TPR;
+    if (row < R0.rows) {
+      double acc = 0.0;
+      const int32_t k0 = R0.ptr[row], k1 = R0.ptr[row + 1];
+      for (int32_t k = k0 + lane; k < k1; k += TPR) {
+        const int32_t c = R0.col[k];
+        const double rn = next_r(rq[c], alpha, next_s(w[c], beta, sq[c]));
+        acc = fma(ldv<VT>(R0.val, k), rn, acc);
+      }
+#pragma unroll
+      for (int off = TPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+      if (lane == 0) b1[row] = acc;
+    }
+  }
+}
+
+// w = A u (plain operator, ELL), partials of (w, u).
+template <int BS, int WMAX, bool EXACT>
+__global__ __launch_bounds__(BS) void amg_matvec_kernel(int64_t n, int32_t W, int64_t ld,
+                                                        const int32_t* __restrict__ cols,
+                                                        const double* __restrict__ vals, AmgPtrs P) {
+  __shared__ double scr[BS / 64];
+  if (P.ss[SS_DONE] != 0.0) return;  // uniform
+  const int blk = xcd_map(blockIdx.x, P.Gf);
+  const int64_t row = (int64_t)blk * BS + threadIdx.x;
+  const int Wk = EXACT ? WMAX : W;
+  double t[1] = {0.0};
+  if (row < n) {
+    const double* __restrict__ u = P.u;
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < WMAX; ++j) {
+      if (j < Wk) acc = fma(vals[j * ld + row], u[cols[j * ld + row]], acc);
+    }
+    __builtin_nontemporal_store(acc, &P.w[row]);
+    t[0] = acc * u[row];
+  }
+  block_total<BS, 1, false>(t, scr);
+  if (threadIdx.x == 0) P.pd[blk] = t[0];
+}
+
+__global__ void amg_gather_kernel(int64_t n, const int32_t* __restrict__ perm,
+                                  const double* __restrict__ x, double* __restrict__ xp) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) xp[i] = x[perm[i]];
+}
+
+__global__ void amg_scatter_kernel(int64_t n, const int32_t* __restrict__ perm,
+                                   const double* __restrict__ xp, double* __restrict__ x) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[perm[i]] = xp[i];
+}
+
+int fine_bs(int64_t n) { return n <= 65536 ? 256 : 512; }
+
+CsrOp to_op(const smx_csr_op& o) {
+  CsrOp c;
+  c.ptr = o.ptr;
+  c.col = o.col;
+  c.val = o.val;
+  c.rows = o.rows;
+  return c;
+}
+
+constexpr int kLevelBS = 256;
+
+template <typename VT, int MODE>
+int launch_level(const smx_csr_op& m1, const double* v1, const smx_csr_op* m2, const double* v2,
+                 double* y, const double* ss, double* part, int tpr, hipStream_t st) {
+  const CsrOp a = to_op(m1);
+  CsrOp b = {nullptr, nullptr, nullptr, 0};
+  if (m2 && m2->rows > 0) b = to_op(*m2);
+  if (a.rows <= 0) return SMX_OK;
+  const int64_t rpb = kLevelBS / tpr;
+  const unsigned g = (unsigned)((a.rows + rpb - 1) / rpb);
+#define SMX_AMG_LVL(T)                                                                        \
+  hipLaunchKernelGGL((amg_level_kernel<kLevelBS, T, VT, MODE>), dim3(g), dim3(kLevelBS), 0, st, \
+                     a, v1, b, v2, y, ss, part)
+  switch (tpr) {
+    case 1: SMX_AMG_LVL(1); break;
+    case 2: SMX_AMG_LVL(2); break;
+    case 4: SMX_AMG_LVL(4); break;
+    case 8: SMX_AMG_LVL(8); break;
+    case 16: SMX_AMG_LVL(16); break;
+    case 32: SMX_AMG_LVL(32); break;
+    default: SMX_AMG_LVL(64); break;
+  }
+#undef SMX_AMG_LVL
+  SMX_LAUNCH_CHECK();
+  return SMX_OK;
+}
+
+int64_t up0_blocks(int64_t n, int tpr) {
+  const int64_t rpb = kLevelBS / tpr;
+  return (n + rpb - 1) / rpb;
+}
+
+struct Layout {
+  int64_t Gf, Gu_max, nvec;
+};
+
+Layout layout(int64_t n) {
+  Layout L;
+  L.Gf = (n + fine_bs(n) - 1) / fine_bs(n);
+  L.Gu_max = up0_blocks(n, 64);
+  L.nvec = (n + 63) / 64 * 64;
+  return L;
+}
+
+bool valid_tpr(int t) { return t == 1 || t == 2 || t == 4 || t == 8 || t == 16 || t == 32 || t == 64; }
+
+// The V-cycle below the fine level and the up0 launch (u = M^-1 r_fine).
+// On entry levels[1].b holds R_0 r_fine (or, with one level, nothing).
+template <typename VT>
+int vcycle_tail(const smx_amg_level* lv, int nlev, const double* r_fine, AmgPtrs& P,
+                hipStream_t st) {
+  int rc;
+  for (int l = 1; l + 1 < nlev; ++l) {  // down: b_{l+1} = R_l b_l
+    rc = launch_level<VT, 0>(lv[l].R, lv[l].b, nullptr, nullptr, lv[l + 1].b, P.ss, nullptr,
+                             lv[l].r_tpr, st);
+    if (rc) return rc;
+  }
+  if (nlev > 1) {  // coarse: x_c = Z b_c
+    const smx_amg_level& c = lv[nlev - 1];
+    rc = launch_level<VT, 0>(c.S, c.b, nullptr, nullptr, c.x, P.ss, nullptr, c.s_tpr, st);
+    if (rc) return rc;
+  }
+  for (int l = nlev - 2; l >= 1; --l) {  // up: x_l = S_l b_l + Q_l x_{l+1}
+    rc = launch_level<VT, 0>(lv[l].S, lv[l].b, &lv[l].Q, lv[l + 1].x, lv[l].x, P.ss, nullptr,
+                             lv[l].s_tpr, st);
+    if (rc) return rc;
+  }
+  const smx_csr_op* q0 = nlev > 1 ? &lv[0].Q : nullptr;
+  const double* x1 = nlev > 1 ? lv[1].x : nullptr;
+  return launch_level<VT, 1>(lv[0].S, r_fine, q0, x1, P.u, P.ss, P.pg, lv[0].s_tpr, st);
+}
+
+template <int BS, int WMAX, bool EXACT, typename VT>
+int run_amg(int64_t n, int32_t W, int64_t ld, const int32_t* cols, const double* vals, int32_t mode,
+            const double* b, const double* b_part, int64_t nb_part, int32_t project, int32_t raw_W,
+            int64_t raw_ld, const int32_t* raw_cols, const double* raw_vals, const int32_t* bpos,
+            const double* gb, double* x_user, int32_t warm, const int32_t* perm, double* x_perm,
+            const smx_amg_level* lv, int nlev, double rtol, double atol, int32_t max_it,
+            int32_t check_every, int32_t expected_its, AmgPtrs P, hipStream_t st, double* hst) {
+  const int Gf = P.Gf;
+  const unsigned gs = (unsigned)((n + 255) / 256);
+  double* x = perm ? x_perm : x_user;
+  if (perm && warm) {
+    hipLaunchKernelGGL(amg_gather_kernel, dim3(gs), dim3(256), 0, st, n, perm, x_user, x);
+    SMX_LAUNCH_CHECK();
+  }
+  if (mode == 0) {
+    hipLaunchKernelGGL((amg_init_kernel<BS, WMAX>), dim3(Gf), dim3(BS), 0, st, n, mode, W, ld, cols,
+                       vals, b, b_part, nb_part, project, bpos, gb, x, warm, perm, P);
+  } else {
+    hipLaunchKernelGGL((amg_init_kernel<BS, WMAX>), dim3(Gf), dim3(BS), 0, st, n, mode, raw_W,
+                       raw_ld, raw_cols, raw_vals, b, b_part, nb_part, project, bpos, gb, x, warm,
+                       perm, P);
+  }
+  SMX_LAUNCH_CHECK();
+  int rc;
+  // u0 = M^-1 r0, w0 = A u0
+  if (nlev > 1) {
+    rc = launch_level<VT, 0>(lv[0].R, P.r[0], nullptr, nullptr, lv[1].b, P.ss, nullptr,
+                             lv[0].r_tpr, st);
+    if (rc) return rc;
+  }
+  rc = vcycle_tail<VT>(lv, nlev, P.r[0], P, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL((amg_matvec_kernel<BS, WMAX, EXACT>), dim3(Gf), dim3(BS), 0, st, n, W, ld,
+                     cols, vals, P);
+  SMX_LAUNCH_CHECK();
+  const int64_t n1 = nlev > 1 ? lv[0].R.rows : 0;
+  const int rt = nlev > 1 ? lv[0].r_tpr : 1;
+  const int64_t Gr = nlev > 1 ? (n1 + (BS / rt) - 1) / (BS / rt) : 0;
+  CsrOp R0 = {nullptr, nullptr, nullptr, 0};
+  if (nlev > 1) R0 = to_op(lv[0].R);
+  double* b1 = nlev > 1 ? lv[1].b : nullptr;
+  if (check_every < 1) check_every = 8;
+  int32_t batch = expected_its > 0 ? expected_its + 1 : check_every;
+  int32_t launched = 0;
+  bool done = false;
+  while (!done && launched <= max_it) {
+    const int32_t nb = batch < (max_it + 1 - launched) ? batch : (max_it + 1 - launched);
+    for (int32_t k = 0; k < nb; ++k) {
+      const int32_t it = launched + k;
+      const unsigned grid = (unsigned)(Gf + Gr);
+#define SMX_AMG_UPD(T)                                                                        \
+  hipLaunchKernelGGL((amg_update_kernel<BS, T, VT>), dim3(grid), dim3(BS), 0, st, n, P, x, R0, \
+                     b1, it, max_it, rtol, atol)
+      switch (rt) {
+        case 1: SMX_AMG_UPD(1); break;
+        case 2: SMX_AMG_UPD(2); break;
+        case 4: SMX_AMG_UPD(4); break;
+        case 8: SMX_AMG_UPD(8); break;
+        case 16: SMX_AMG_UPD(16); break;
+        case 32: SMX_AMG_UPD(32); break;
+        default: SMX_AMG_UPD(64); break;
+      }
+#undef SMX_AMG_UPD
+      SMX_LAUNCH_CHECK();
+      rc = vcycle_tail<VT>(lv, nlev, P.r[1 - (it & 1)], P, st);
+      if (rc) return rc;
+      hipLaunchKernelGGL((amg_matvec_kernel<BS, WMAX, EXACT>), dim3(Gf), dim3(BS), 0, st, n, W,
+                         ld, cols, vals, P);
+      SMX_LAUNCH_CHECK();
+    }
+    launched += nb;
+    SMX_HIP_CHECK(hipMemcpyAsync(hst, P.ss, sizeof(double) * SS_SIZE, hipMemcpyDeviceToHost, st));
+    SMX_HIP_CHECK(hipStreamSynchronize(st));
+    done = hst[SS_DONE] != 0.0;
+    batch = check_every;
+  }
+  if (perm) {
+    hipLaunchKernelGGL(amg_scatter_kernel, dim3(gs), dim3(256), 0, st, n, perm, x, x_user);
+    SMX_LAUNCH_CHECK();
+  }
+  return SMX_OK;
+}
+
+__global__ void amg_clear_status_kernel(double* ss) {
+  if (threadIdx.x < SS_SIZE) ss[threadIdx.x] = 0.0;
+}
+
+int check_levels(const smx_amg_level* levels, int32_t nlev, int64_t n) {
+  SMX_REQUIRE(levels && nlev >= 1 && nlev <= 16, SMX_ERR_VALUE, "1 to 16 AMG levels required");
+  // level l has n_l unknowns; S_l is n_l x n_l, Q_l n_l x n_{l+1}, R_l n_{l+1} x n_l;
+  // the last level holds only S (the coarse inverse)
+  SMX_REQUIRE(levels[0].n == n, SMX_ERR_SHAPE, "level 0 has %lld unknowns, the operator %lld",
+              (long long)levels[0].n, (long long)n);
+  for (int l = 0; l < nlev; ++l) {
+    const smx_amg_level& L = levels[l];
+    SMX_REQUIRE(L.n >= 1 && L.S.rows == L.n && L.S.ptr && L.S.col && L.S.val, SMX_ERR_SHAPE,
+                "level %d: S must have n rows", l);
+    SMX_REQUIRE(valid_tpr(L.s_tpr), SMX_ERR_VALUE,
+                "level %d: lanes per row must be 1..64, a power of 2", l);
+    if (l > 0) SMX_REQUIRE(L.b && L.x, SMX_ERR_VALUE, "level %d: b and x vectors required", l);
+    if (l + 1 < nlev) {
+      SMX_REQUIRE(L.Q.rows == L.n && L.R.rows == levels[l + 1].n && L.Q.ptr && L.Q.col &&
+                      L.Q.val && L.R.ptr && L.R.col && L.R.val,
+                  SMX_ERR_SHAPE, "level %d: Q must have n_l rows, R n_{l+1} rows", l);
+      SMX_REQUIRE(valid_tpr(L.r_tpr), SMX_ERR_VALUE,
+                  "level %d: lanes per row must be 1..64, a power of 2", l);
+    }
+  }
+  return SMX_OK;
+}
+
+template <typename VT>
+int apply_once(const smx_amg_level* lv, int nlev, const double* r, double* u, double* work,
+               hipStream_t st) {
+  AmgPtrs P = {};
+  P.ss = work;
+  P.pg = work + HDR;
+  P.u = u;
+  hipLaunchKernelGGL(amg_clear_status_kernel, dim3(1), dim3(64), 0, st, P.ss);
+  SMX_LAUNCH_CHECK();
+  if (nlev > 1) {
+    const int rc = launch_level<VT, 0>(lv[0].R, r, nullptr, nullptr, lv[1].b, P.ss, nullptr,
+                                       lv[0].r_tpr, st);
+    if (rc) return rc;
+  }
+  return vcycle_tail<VT>(lv, nlev, r, P, st);
+}
+
+}  // namespace
+
+extern "C" int smx_amg_apply(const smx_amg_level* levels, int32_t nlev, int32_t val_dtype,
+                             const double* r, double* u, double* work, void* stream) {
+  SMX_REQUIRE(val_dtype == SMX_F64 || val_dtype == SMX_F32, SMX_ERR_TYPE,
+              "AMG operator values must be FP64 or FP32");
+  SMX_REQUIRE(levels, SMX_ERR_VALUE, "null levels");
+  const int rc = check_levels(levels, nlev, levels[0].n);
+  if (rc) return rc;
+  SMX_REQUIRE(r && u && work, SMX_ERR_VALUE, "null pointer argument");
+  hipStream_t st = smx::as_stream(stream);
+  return val_dtype == SMX_F64 ? apply_once<double>(levels, nlev, r, u, work, st)
+                              : apply_once<float>(levels, nlev, r, u, work, st);
+}
+
+extern "C" int64_t smx_pcg_amg_ws_size(int64_t n) {
+  const Layout L = layout(n);
+  return HDR + 6 * L.Gf + L.Gu_max + 8 * L.nvec + 64;
+}
+
+extern "C" int smx_pcg_amg(int64_t n, int32_t width, int64_t ld, const int32_t* cols,
+                           const double* vals, int32_t mode, const double* b,
+                           const double* b_part, int64_t nb_part, int32_t project,
+                           int32_t raw_width, int64_t raw_ld, const int32_t* raw_cols,
+                           const double* raw_vals, const int32_t* bpos, const double* gb,
+                           double* x, int32_t warm, const int32_t* perm,
+                           const smx_amg_level* levels, int32_t nlev, int32_t val_dtype,
+                           double rtol, double atol, int32_t max_it, int32_t check_every,
+                           int32_t expected_its, double* work, int32_t* its_out,
+                           double* rnorm_out, double* sums_out, void* stream) {
+  SMX_REQUIRE(n >= 0, SMX_ERR_SHAPE, "negative size");
+  SMX_REQUIRE(mode == 0 || mode == 1, SMX_ERR_VALUE, "mode must be 0 or 1");
+  SMX_REQUIRE(max_it >= 0, SMX_ERR_VALUE, "max_it must be >= 0");
+  SMX_REQUIRE(width >= 1 && width <= 32 && ld >= n, SMX_ERR_SHAPE,
+              "ELL width must be in [1, 32] and ld >= n (width %d, ld %lld)", width, (long long)ld);
+  SMX_REQUIRE(mode == 0 || (raw_width >= 1 && raw_width <= 32 && raw_ld >= n), SMX_ERR_SHAPE,
+              "raw operator width must be in [1, 32]");
+  SMX_REQUIRE(val_dtype == SMX_F64 || val_dtype == SMX_F32, SMX_ERR_TYPE,
+              "AMG operator values must be FP64 or FP32");
+  if (its_out) *its_out = 0;
+  if (rnorm_out) *rnorm_out = 0.0;
+  if (n == 0) return SMX_OK;
+  SMX_REQUIRE(cols && vals && x && work, SMX_ERR_VALUE, "null pointer argument");
+  SMX_REQUIRE(mode == 1 || b, SMX_ERR_VALUE, "mode 0 needs b");
+  SMX_REQUIRE(mode == 0 || (raw_cols && raw_vals && bpos && gb), SMX_ERR_VALUE,
+              "mode 1 needs the raw operator, bpos and gb");
+  {
+    const int rc = check_levels(levels, nlev, n);
+    if (rc) return rc;
+  }
+  SMX_REQUIRE(n < (int64_t)1 << 30, SMX_ERR_SHAPE, "operator too large (n %lld)", (long long)n);
+  hipStream_t st = smx::as_stream(stream);
+  const Layout Lo = layout(n);
+  AmgPtrs P;
+  P.Gf = (int)Lo.Gf;
+  P.Gu = (int)up0_blocks(n, levels[0].s_tpr);
+  P.ss = work;
+  P.st[0] = work + ST_OFF;
+  P.st[1] = work + ST_OFF + ST_SIZE;
+  double* q = work + HDR;
+  P.pbb = q;
+  P.prr[0] = q + Lo.Gf;
+  P.prr[1] = q + 3 * Lo.Gf;
+  P.pd = q + 5 * Lo.Gf;
+  P.pg = q + 6 * Lo.Gf;
+  double* v0 = q + 6 * Lo.Gf + Lo.Gu_max;
+  v0 += (64 - (((uintptr_t)v0 / sizeof(double)) & 63)) & 63;  // 512-byte aligned vectors
+  P.r[0] = v0;
+  P.r[1] = v0 + Lo.nvec;
+  P.s[0] = v0 + 2 * Lo.nvec;
+  P.s[1] = v0 + 3 * Lo.nvec;
+  P.w = v0 + 4 * Lo.nvec;
+  P.u = v0 + 5 * Lo.nvec;
+  P.p = v0 + 6 * Lo.nvec;
+  double* x_perm = v0 + 7 * Lo.nvec;
+  double hst[SS_SIZE] = {0};
+  const int bs = fine_bs(n);
+  const int32_t wmax = (mode == 1 && raw_width > width) ? raw_width : width;
+  const bool w7 = width == 7 && wmax == 7;
+  int rc;
+#define SMX_AMG_CALL(BS, WM, EX, VT)                                                            \
+  run_amg<BS, WM, EX, VT>(n, width, ld, cols, vals, mode, b, b_part, nb_part, project,          \
+                          raw_width, raw_ld, raw_cols, raw_vals, bpos, gb, x, warm, perm,       \
+                          x_perm, levels, nlev, rtol, atol, max_it, check_every, expected_its, \
+                          P, st, hst)
+#define SMX_AMG_W(BS, VT)                                                                \
+  (w7 ? SMX_AMG_CALL(BS, 7, true, VT)                                                    \
+      : wmax <= 8 ? SMX_AMG_CALL(BS, 8, false, VT)                                       \
+                  : wmax <= 16 ? SMX_AMG_CALL(BS, 16, false, VT) : SMX_AMG_CALL(BS, 32, false, VT))
+  if (val_dtype == SMX_F64) {
+    rc = bs == 256 ? SMX_AMG_W(256, double) : SMX_AMG_W(512, double);
+  } else {
+    rc = bs == 256 ? SMX_AMG_W(256, float) : SMX_AMG_W(512, float);
+  }
+#undef SMX_AMG_W
+#undef SMX_AMG_CALL
+  if (rc != SMX_OK) return rc;
+  if (its_out) *its_out = (int32_t)hst[SS_ITERS];
+  if (rnorm_out) *rnorm_out = sqrt(hst[SS_RR]);
+  if (sums_out) sums_out[0] = hst[SS_SUMX];
+  if (hst[SS_FAIL] == 1.0) {
+    smx::set_error("AMG-CG breakdown after %d iterations (rnorm %g)", (int)hst[SS_ITERS],
+                   sqrt(hst[SS_RR]));
+    return SMX_ERR_NOT_CONVERGED;
+  }
+  if (hst[SS_FAIL] == 2.0 || hst[SS_DONE] == 0.0) {
+    smx::set_error("AMG-CG did not converge in %d iterations (rnorm %g, tol %g)", max_it,
+                   sqrt(hst[SS_RR]), sqrt(hst[SS_TOL2]));
+    return SMX_ERR_NOT_CONVERGED;
+  }
+  return SMX_OK;
+}

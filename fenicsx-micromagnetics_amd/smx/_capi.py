@@ -94,6 +94,10 @@ _SIGS = {
     "smx_pcg_jacobi": ([I64, I32, I64, P, P, P, I32, P, P, I64, I32, I32, I64, P, P, P, P, P, I32, P, F64,
                         F64, I32, I32, I32, P, C.POINTER(I32), C.POINTER(F64), C.POINTER(F64), P],
                        C.c_int),
+    "smx_pcg_amg_ws_size": ([I64], I64),
+    "smx_amg_apply": ([P, I32, I32, P, P, P, P], C.c_int),
+    "smx_pcg_amg": ([I64, I32, I64, P, P, I32, P, P, I64, I32, I32, I64, P, P, P, P, P, I32, P, P, I32, I32,
+                     F64, F64, I32, I32, I32, P, C.POINTER(I32), C.POINTER(F64), C.POINTER(F64), P], C.c_int),
     "smx_field_llg_rhs": ([C.POINTER(FieldArgs), P], C.c_int),
     "smx_llg_jvp": ([C.POINTER(FieldArgs), P, P, P, F64, P], C.c_int),
     "smx_dot3_weighted": ([I64, P, P, P, P, P, P], C.c_int),

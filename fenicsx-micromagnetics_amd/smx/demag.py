@@ -87,16 +87,21 @@ class OperatorSet:
         sv = sell_values(self.fem.graph.indptr, vals, self.N)
         return self.graph.with_vals(sv, k)
 
-    def compact(self, vals: np.ndarray, perm: np.ndarray | None = None) -> DeviceEll:
-        """Scalar operator with (near-)zero entries dropped, as ELL for the CG
-        kernels; with ``perm`` (solver row i = node perm[i]) it is P A P^T."""
+    def compact_host(self, vals: np.ndarray, perm: np.ndarray | None = None) -> sp.csr_matrix:
+        """Scalar operator with (near-)zero entries dropped (host CSR); with
+        ``perm`` (solver row i = node perm[i]) it is P A P^T."""
         g = self.fem.graph
         ip, ix, vv = compact_csr(g.indptr, g.indices, vals)
+        S = sp.csr_matrix((vv, ix, ip), shape=(self.N, self.N))
         if perm is not None:
-            S = sp.csr_matrix((vv, ix, ip), shape=(self.N, self.N))[perm][:, perm].tocsr()
-            S.sort_indices()
-            ip, ix, vv = S.indptr, S.indices, S.data
-        return DeviceEll.from_csr(ip, ix, vv, self.N, self.device)
+            S = S[perm][:, perm].tocsr()
+        S.sort_indices()
+        return S
+
+    def compact(self, vals: np.ndarray, perm: np.ndarray | None = None) -> DeviceEll:
+        """``compact_host`` as ELL for the CG kernels."""
+        S = self.compact_host(vals, perm)
+        return DeviceEll.from_csr(S.indptr, S.indices, S.data, self.N, self.device)
 
     def solver_order(self) -> np.ndarray:
         """Row order for the CG solves: the node numbering or its reverse
@@ -142,7 +147,8 @@ class DemagFieldLindholmGPU:
                  *, ksp_atol=1e-12, ksp_max_it_u1=2000, ksp_max_it_u2=12000,
                  warm_start=False, device=None, operators: OperatorSet | None = None,
                  geometry: BoundaryGeometry | None = None, hmatrix_prebuilt=None,
-                 hmatrix_near_fp32=False, cg_reorder=True, verbose=False):
+                 hmatrix_near_fp32=False, cg_reorder=True, ksp_pc_type="gamg", amg_fp32=None,
+                 amg_max_coarse=1024, amg_theta=0.0, verbose=False):
         if not isinstance(mesh, TetMesh):
             raise TypeError("mesh must be a smx.mesh.TetMesh")
         self.mesh = mesh
@@ -196,10 +202,22 @@ class DemagFieldLindholmGPU:
         self.dinv_u1 = to_dev(1.0 / d1[sel], dev)
         self.dinv_u2 = to_dev(1.0 / d2[sel], dev)
         # plain P A P^T: the lifting operator b = -A_raw g of the u2 solve
-        self.A_raw = self.ops.compact(fem.stiff, perm)
-        # the CG kernels take A D^-1 with the diagonal in slot 0
-        self.A_u1 = solver_operator(self.A_raw, self.dinv_u1)
-        self.A_u2 = solver_operator(self.ops.compact(fem.stiffness_bc(), perm), self.dinv_u2)
+        self.pc_type = str(ksp_pc_type).strip().lower()
+        if self.pc_type not in ("gamg", "jacobi"):
+            raise ValueError("ksp_pc_type must be 'gamg' or 'jacobi'")
+        A1h = self.ops.compact_host(fem.stiff, perm)
+        A2h = self.ops.compact_host(fem.stiffness_bc(), perm)
+        self.A_raw = DeviceEll.from_csr(A1h.indptr, A1h.indices, A1h.data, self.N, dev)
+        A_bc = DeviceEll.from_csr(A2h.indptr, A2h.indices, A2h.data, self.N, dev)
+        if self.pc_type == "jacobi":
+            # the Jacobi CG kernels take A D^-1 with the diagonal in slot 0
+            self.A_u1 = solver_operator(self.A_raw, self.dinv_u1)
+            self.A_u2 = solver_operator(A_bc, self.dinv_u2)
+        else:
+            # the AMG-CG kernels take the plain operators
+            self.A_u1 = self.A_raw
+            self.A_u2 = A_bc
+        self._amg_host = (A1h, A2h, int(amg_max_coarse), float(amg_theta))
 
         # ---------------- boundary operator ----------------
         self.opB = geometry or build_boundary_geometry(mesh.X, mesh.tets, incident_policy)
@@ -243,6 +261,25 @@ class DemagFieldLindholmGPU:
         torch.cuda.synchronize(dev)
         self.setup_boundary_seconds = time.perf_counter() - t0
 
+        # ---------------- AMG hierarchies (PETSc GAMG counterpart) ----------------
+        self.amg_u1 = self.amg_u2 = None
+        if self.pc_type == "gamg":
+            from .amg import AmgHierarchy
+            if amg_fp32 is None:
+                # follow the boundary operator: a preconditioner in FP32 when the
+                # boundary operator is stored below FP64 (the outer CG stays FP64)
+                if self.boundary_backend == "dense":
+                    amg_fp32 = self.B_dtype != torch.float64
+                else:
+                    amg_fp32 = bool(hmatrix_use_fp32)
+            A1h, A2h, mc, th = self._amg_host
+            t1 = time.perf_counter()
+            self.amg_u1 = AmgHierarchy(A1h, dev, fp32=bool(amg_fp32), max_coarse=mc, theta=th)
+            self.amg_u2 = AmgHierarchy(A2h, dev, fp32=bool(amg_fp32), max_coarse=mc, theta=th)
+            self.setup_amg_seconds = time.perf_counter() - t1
+        self.amg_fp32 = bool(amg_fp32) if self.pc_type == "gamg" else False
+        del self._amg_host
+
         # ---------------- per-evaluation buffers ----------------
         N = self.N
         z = lambda n: torch.zeros(n, dtype=torch.float64, device=dev)
@@ -254,8 +291,9 @@ class DemagFieldLindholmGPU:
         self.y_rows = z(self.Nb)
         self.red = reduce_workspace(3 * N + 64, dev)
         self.b_part = z(self.Div.blocks())
-        self.ws1 = z(int(L.smx_pcg_ws_size(N)))
-        self.ws2 = z(int(L.smx_pcg_ws_size(N)))
+        nws = max(int(L.smx_pcg_ws_size(N)), int(L.smx_pcg_amg_ws_size(N)))
+        self.ws1 = z(nws)
+        self.ws2 = z(nws)
         self._have_u = False
         self._energy_out = z(1)
 
@@ -281,6 +319,20 @@ class DemagFieldLindholmGPU:
         sums = (C.c_double * 1)()
         raw = self.A_raw
         nbp = 0 if b_part is None else b_part.numel()
+        amg = (self.amg_u1 if mode == 0 else self.amg_u2) if self.pc_type == "gamg" else None
+        if amg is not None:
+            rc = L.smx_pcg_amg(self.N, A.width, A.ld, _capi.ptr(A.cols), _capi.ptr(A.vals), mode,
+                               _capi.ptr(b), _capi.ptr(b_part), nbp, 1, raw.width, raw.ld,
+                               _capi.ptr(raw.cols), _capi.ptr(raw.vals), _capi.ptr(self.bpos_cg),
+                               _capi.ptr(gb), _capi.ptr(x), int(warm), _capi.ptr(self.cg_perm),
+                               C.addressof(amg.c_levels), amg.nlev, amg.val_dtype, float(rtol),
+                               float(self.ksp_atol), int(max_it), 8, int(expected),
+                               _capi.ptr(ws), C.byref(its), C.byref(rn), sums, _capi.stream_ptr())
+            if rc == _capi.SMX_ERR_NOT_CONVERGED:
+                self.converged = False
+            else:
+                _capi.check(rc, "smx_pcg_amg")
+            return int(its.value)
         rc = L.smx_pcg_jacobi(self.N, A.width, A.ld, _capi.ptr(A.cols), _capi.ptr(A.vals),
                               _capi.ptr(dinv), mode, _capi.ptr(b), _capi.ptr(b_part), nbp, 1,
                               raw.width, raw.ld, _capi.ptr(raw.cols), _capi.ptr(raw.vals),

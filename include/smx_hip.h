@@ -296,6 +296,58 @@ int smx_pcg_jacobi(int64_t n, int32_t width, int64_t ld, const int32_t* cols,
                    int32_t* its_out, double* rnorm_out, double* sums_out, void* stream);
 
 /* ---------------------------------------------------------------------- *
+ * (c') CG preconditioned by one smoothed-aggregation AMG V-cycle.  Replaces
+ * PETSc KSP cg + PC gamg of the two Poisson solves (Demag_Lindholm_GPU.py:
+ * 216-226 for u1, 271-281 for u2; solves at :569-582, :618-640), whose
+ * defaults are one V-cycle per application with Jacobi-type smoothing.
+ *
+ * The hierarchy (built on the host) gives, per level l with n_l unknowns,
+ *   S_l (n_l x n_l)      = 2w D^-1 - w^2 D^-1 A_l D^-1
+ *   Q_l (n_l x n_{l+1})  = (I - w D^-1 A_l) P_l     (P_l smoothed prolongator)
+ *   R_l (n_{l+1} x n_l)  = Q_l^T
+ * and on the last level only S = a dense (pseudo-)inverse of the coarse
+ * operator with full rows.  The preconditioner is M^-1 b = S_0 b + Q_0 M_1^-1
+ * (R_0 b), recursively: one damped-Jacobi pre- and post-smoothing sweep per
+ * level.  CSR operators use int32 offsets and columns; values are FP64 or FP32
+ * (val_dtype, the same for every operator); s_tpr / r_tpr are the lanes per
+ * row (1..64, a power of two) for the S (with Q) and R launches.  Levels >= 1
+ * bring device vectors b and x of n_l doubles.
+ *
+ * Everything else (ELL operator = plain A in solver order, modes, perm,
+ * convergence test, status record, SMX_PCG_SUMX) is as smx_pcg_jacobi.
+ * work: smx_pcg_amg_ws_size(n) doubles.
+ * ---------------------------------------------------------------------- */
+typedef struct {
+  const int32_t* ptr;  /* rows + 1 offsets */
+  const int32_t* col;
+  const void* val;
+  int64_t rows;
+} smx_csr_op;
+
+typedef struct {
+  int64_t n;
+  smx_csr_op S, Q, R;
+  int32_t s_tpr, r_tpr;
+  double* b;
+  double* x;
+} smx_amg_level;
+
+int64_t smx_pcg_amg_ws_size(int64_t n);
+/* u = M^-1 r: one application of the V-cycle above (PETSc PCApply of the
+ * GAMG preconditioner).  r and u hold levels[0].n doubles; work holds at
+ * least smx_pcg_amg_ws_size(levels[0].n) doubles. */
+int smx_amg_apply(const smx_amg_level* levels, int32_t nlev, int32_t val_dtype, const double* r,
+                  double* u, double* work, void* stream);
+int smx_pcg_amg(int64_t n, int32_t width, int64_t ld, const int32_t* cols, const double* vals,
+                int32_t mode, const double* b, const double* b_part, int64_t nb_part,
+                int32_t project, int32_t raw_width, int64_t raw_ld, const int32_t* raw_cols,
+                const double* raw_vals, const int32_t* bpos, const double* gb, double* x,
+                int32_t warm, const int32_t* perm, const smx_amg_level* levels, int32_t nlev,
+                int32_t val_dtype, double rtol, double atol, int32_t max_it,
+                int32_t check_every, int32_t expected_its, double* work, int32_t* its_out,
+                double* rnorm_out, double* sums_out, void* stream);
+
+/* ---------------------------------------------------------------------- *
  * (d) Fused effective field + LLG right-hand side.
  * Replaces apply_linear_field_vec + anisotropy + demag G.mult/V + axpys +
  * _LLG_RHS_KERNEL (llg_module_GPU.py:100-122, 962-1006, 1261-1285;

@@ -172,10 +172,11 @@ def _demag_pair(m, bd, B, **kw):
     return d, ref
 
 
-def test_demag_field_c1_fp64(c1):
+@pytest.mark.parametrize("pc", ["gamg", "jacobi"])
+def test_demag_field_c1_fp64(c1, pc):
     torch = _torch()
     m, bd, B = c1
-    d, ref = _demag_pair(m, bd, B)
+    d, ref = _demag_pair(m, bd, B, ksp_pc_type=pc)
     mv = random_unit(m.num_nodes, 0)
     H = torch.zeros(3 * m.num_nodes, dtype=torch.float64, device="cuda")
     d.compute_vec(torch.from_numpy(mv).cuda(), H)
@@ -185,7 +186,7 @@ def test_demag_field_c1_fp64(c1):
     u1 = d.u1.cpu().numpy()
     e_u1 = rel_l2(u1 - u1.mean(), parts["u1"])
     e_u2 = rel_l2(d.u2.cpu().numpy(), parts["u2"])
-    print(f"[demag c1] rel L2 H {e:.3e} u1 {e_u1:.3e} u2 {e_u2:.3e} its {d.last_its_u1}/{d.last_its_u2}", flush=True)
+    print(f"[demag c1 {pc}] rel L2 H {e:.3e} u1 {e_u1:.3e} u2 {e_u2:.3e} its {d.last_its_u1}/{d.last_its_u2}", flush=True)
     assert e_u1 <= 1e-10 and e_u2 <= 1e-10
     assert e <= 1e-10
     E = d.Energy_lumped_gpu(torch.from_numpy(mv).cuda(), H)
