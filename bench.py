@@ -43,6 +43,7 @@ GAMMA = 2.211e5
 
 
 PC_TYPE = "gamg"
+PC_TYPE_U2 = None
 
 
 def log(msg):
@@ -98,10 +99,13 @@ def amg_iteration_bytes(d):
     nb = 80 * N                                        # update: r, w, s, u, p, x read; r, s, p, x written
     nb += d.A_u1.nnz * 12 + 16 * N                      # w = A u: matrix, u read, w written
     for l, L in enumerate(lv[:-1]):
-        nb += op(L.R) + 8 * lv[l + 1].n + (8 * L.n if l > 0 else 0)          # restriction
+        if l == 0:   # restriction by the carried coarse vectors: b_1 and z_1 read and written
+            nb += op(L.R) + 32 * lv[1].n
+        else:
+            nb += op(L.R) + 8 * lv[l + 1].n + 8 * L.n
         nb += op(L.S) + op(L.Q) + 8 * (2 * L.n + lv[l + 1].n)                # up: S b + Q x'
     c = lv[-1]
-    nb += op(c.S) + 16 * c.n                             # coarse
+    nb += c.S.nnz * vs + 4 * (c.n + 1) + 16 * c.n       # coarse: dense rows, no column indices
     return nb
 
 
@@ -386,7 +390,8 @@ def build_case(config, dev):
     from smx import mesh as M
     from smx.demag import OperatorSet
     from smx.llg import EffectiveField
-    common = dict(ksp_rtol_u1=1e-6, ksp_rtol_u2=1e-6, ksp_atol=1e-12, ksp_pc_type=PC_TYPE)
+    common = dict(ksp_rtol_u1=1e-6, ksp_rtol_u2=1e-6, ksp_atol=1e-12, ksp_pc_type=PC_TYPE,
+                  ksp_pc_type_u2=PC_TYPE_U2)
     t0 = time.perf_counter()
     if config == "cube":
         mesh = M.cube_c2()
@@ -523,12 +528,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pc", choices=["gamg", "jacobi"], default="gamg",
                     help="CG preconditioner of both Poisson solves (the reference uses GAMG)")
+    ap.add_argument("--pc-u2", choices=["gamg", "jacobi"], default=None,
+                    help="preconditioner of the u2 solve when it differs from --pc")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--pmc-probe", action="store_true",
                     help="short program for rocprofv3 --pmc passes (calibration copy + CG iterations)")
     args = ap.parse_args()
-    global PC_TYPE
+    global PC_TYPE, PC_TYPE_U2
     PC_TYPE = args.pc
+    PC_TYPE_U2 = args.pc_u2
     if args.pmc_probe:
         pmc_probe(args.config)
         return
@@ -598,7 +606,7 @@ def main():
         "mixed_vs_fp64_rel_err_rhs": max(errs),
         "mixed_vs_fp64_rel_err_hdemag": max(herr),
         "cg_iterations_u1_u2": [float(np.mean([a for a, _ in itsmx])), float(np.mean([b for _, b in itsmx]))],
-        "pc": hmx.demag_field.pc_type,
+        "pc": [hmx.demag_field.pc_type, hmx.demag_field.pc_type_u2],
         "amg_levels_u1": hmx.demag_field.amg_u1.describe() if hmx.demag_field.amg_u1 else None,
         "roofline": {"bound": "hbm", "kernel": ("AMG-CG iteration (u1: update + V-cycle + A u launches)"
                                                 if hmx.demag_field.pc_type == "gamg"

@@ -16,23 +16,30 @@
 //   g = (r,u), d = (w,u), rr = (r,r);  b = g/g_old;  a = g/(d - b g/a_old)
 // Launches of one iteration:
 //   update   fine rows: s, p, x, r and partials of (rr, sum x);
-//            extra workgroups: b_1 = R_0 r_new, each entry of r_new rebuilt
-//            from r, w, s (r and s are double-buffered by iteration parity);
+//            extra workgroups: b_1 = R_0 r_new by linearity from carried
+//            coarse vectors, b_1 = R_0 r and z_1 = R_0 s, as
+//              z_1 <- R_0 w + b z_1,  b_1 <- b_1 - a z_1
+//            (the same recurrence as s and r), so the fine level is gathered
+//            once per R_0 entry (w) instead of three times (r, w, s);
 //   down     b_{l+1} = R_l b_l                 (levels 1 .. L-2)
 //   coarse   x_c = Z b_c
 //   up       x_l = S_l b_l + Q_l x_{l+1}       (levels L-2 .. 1)
-//   up0      u = S_0 r_new + Q_0 x_1, partials of (r_new, u)
-//   matvec   w = A u, partials of (w, u)
+//   up0      u = S_0 r_new + Q_0 x_1
+//   matvec   w = A u, partials of (w, u) and (r_new, u)
 // Every workgroup of the next update re-reduces all partial sums in one fixed
 // order (no atomics, no device-scope fences), so every workgroup derives the
 // same scalars.  The update tests the residual of the previous iteration and
 // raises a done flag; every later launch reads it first and returns at once.
 //
-// Level operators are CSR with TPR lanes per row (TPR a power of two): each
-// lane sums every TPR-th entry of the row and the lanes of a row meet in a
-// fixed butterfly, so results do not depend on scheduling.  Values are FP64
-// or FP32 (the FP32 hierarchy halves the bytes of the preconditioner; the
-// outer CG and every vector stay FP64).
+// Level operators are sliced ELL with TPR lanes per row (TPR a power of two)
+// and 256/TPR rows per slice, one slice per 256 threads: the slice offset is
+// one scalar load, every row of a slice has the slice's width (padding slots
+// hold value 0 and a valid column), so the column and value loads need no
+// row pointer and every lane of a slice runs the same trip count.  Each lane
+// sums every TPR-th slot of its row and the lanes of a row meet in a fixed
+// butterfly, so results do not depend on scheduling.  Values are FP64 or FP32
+// (the FP32 hierarchy halves the bytes of the preconditioner; the outer CG and
+// every vector stay FP64).
 #include "smx_common.h"
 
 namespace {
@@ -56,23 +63,25 @@ struct AmgPtrs {
   double* st[2];
   double* pbb;      // Gf partials of ||b||^2 (written once, by the init launch)
   double* prr[2];   // 2 x Gf partials (rr, sum x), by iteration parity
-  double* pg;       // Gu partials of (r, u)
+  double* pg;       // Gf partials of (r, u)
   double* pd;       // Gf partials of (w, u)
   double* r[2];
   double* s[2];
   double* w;
   double* u;
   double* p;
+  double* z1;       // R_0 s (coarse, carried)
   int Gf;           // fine workgroups (one row per thread)
-  int Gu;           // workgroups of the up0 launch
 };
 
-struct CsrOp {
-  const int32_t* ptr;
-  const int32_t* col;
+struct SellOp {
+  const int32_t* off;  // slice offsets (nslices + 1)
+  const int32_t* col;  // NULL: dense rows, the column of slot j is j
   const void* val;
   int64_t rows;
 };
+
+constexpr int kSliceThreads = 256;  // TPR * rows per slice
 
 // Workgroup b of G is dispatched to XCD b % 8; renumber so that each XCD
 // handles one contiguous range of blocks (bijective for every G).
@@ -116,16 +125,37 @@ __device__ __forceinline__ double ldv(const void* v, int64_t k) {
   return (double)static_cast<const VT*>(v)[k];
 }
 
-// Row sum of a CSR operator over TPR lanes; every lane of the row ends with
-// the total (fixed butterfly order).
+// Row i (local to a slice at `off` of width W) of a sliced-ELL operator times
+// v over TPR lanes; every lane of the row ends with the total (fixed
+// butterfly order).  Lane l takes slots l, l+TPR, ... in that order, four at a
+// time: the four column and value loads, then the four gathers, are issued
+// before the first multiply (masked slots re-read a valid slot and add an
+// exact zero).  Slots are column-major in the slice for TPR == 1 (lanes of a
+// wavefront read consecutive rows) and row-major otherwise (lanes of a row
+// read consecutive slots).
 template <int TPR, typename VT>
-__device__ __forceinline__ double csr_row(const CsrOp& M, const double* __restrict__ v, int64_t row,
-                                          int lane) {
+__device__ __forceinline__ double sell_row(const SellOp& M, const double* __restrict__ v, int32_t off,
+                                           int32_t W, int i, int lane) {
+  constexpr int U = 4;
+  constexpr int RPS = kSliceThreads / TPR;
   double acc = 0.0;
-  const int32_t k0 = M.ptr[row], k1 = M.ptr[row + 1];
-  for (int32_t k = k0 + lane; k < k1; k += TPR) acc = fma(ldv<VT>(M.val, k), v[M.col[k]], acc);
+  for (int32_t jb = lane; jb < W; jb += U * TPR) {
+    double vv[U], g[U];
 #pragma unroll
-  for (int off = TPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    for (int u = 0; u < U; ++u) {
+      const int32_t j = jb + u * TPR;
+      const bool ok = j < W;
+      const int32_t jj = ok ? j : jb;
+      const int32_t pos = TPR == 1 ? off + jj * RPS + i : off + i * W + jj;
+      const int32_t c = M.col ? M.col[pos] : jj;
+      vv[u] = ok ? ldv<VT>(M.val, pos) : 0.0;
+      g[u] = v[c];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) acc = fma(vv[u], g[u], acc);
+  }
+#pragma unroll
+  for (int o = TPR / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
   return acc;
 }
 
@@ -143,11 +173,12 @@ __global__ __launch_bounds__(BS) void amg_init_kernel(
     const double* __restrict__ vals, const double* __restrict__ b,
     const double* __restrict__ b_part, int64_t nb_part, int32_t project,
     const int32_t* __restrict__ bpos, const double* __restrict__ gb, double* __restrict__ x,
-    int32_t warm, const int32_t* __restrict__ perm, AmgPtrs P) {
+    int32_t warm, const int32_t* __restrict__ perm, AmgPtrs P, int64_t n1) {
   __shared__ double scr[BS / 64];
   __shared__ double scr_pub[BS / 64 * 3];
   const int blk = xcd_map(blockIdx.x, P.Gf);
   const int64_t row = (int64_t)blk * BS + threadIdx.x;
+  if (row < n1) P.z1[row] = 0.0;  // R_0 s_{-1} = 0
   double shift = 0.0;
   if (mode == 0 && project && b_part) {
     double t[1] = {0.0};
@@ -222,38 +253,43 @@ __global__ __launch_bounds__(BS) void amg_init_kernel(
   }
 }
 
-// One level launch: y = M1 v1 (+ M2 v2).  MODE 1 (up0): partials of (v1, y).
-template <int BS, int TPR, typename VT, int MODE>
-__global__ __launch_bounds__(BS) void amg_level_kernel(CsrOp M1, const double* __restrict__ v1,
-                                                       CsrOp M2, const double* __restrict__ v2,
-                                                       double* __restrict__ y,
-                                                       const double* __restrict__ ss,
-                                                       double* __restrict__ part) {
-  __shared__ double scr[BS / 64];
-  if (ss[SS_DONE] != 0.0) return;  // uniform
-  constexpr int RPB = BS / TPR;
-  const int blk = xcd_map(blockIdx.x, gridDim.x);
-  const int64_t row = (int64_t)blk * RPB + threadIdx.x / TPR;
+// One level launch: y = M1 v1 (+ M2 v2), one slice of both operators per
+// workgroup (M1 and M2 have the same rows and slicing).
+template <int TPR, typename VT>
+__global__ __launch_bounds__(kSliceThreads) void amg_level_kernel(SellOp M1,
+                                                                  const double* __restrict__ v1,
+                                                                  SellOp M2,
+                                                                  const double* __restrict__ v2,
+                                                                  double* __restrict__ y,
+                                                                  const double* __restrict__ ss) {
+  constexpr int RPS = kSliceThreads / TPR;
+  // the done flag (written by an earlier launch) is tested once the slice
+  // offsets are in flight
+  const bool was_done = ss[SS_DONE] != 0.0;
+  const int sl = xcd_map(blockIdx.x, gridDim.x);
+  const int32_t a0 = M1.off[sl], a1 = M1.off[sl + 1];
+  int32_t b0 = 0, b1 = 0;
+  if (M2.off) {
+    b0 = M2.off[sl];
+    b1 = M2.off[sl + 1];
+  }
+  if (was_done) return;  // uniform
+  const int i = threadIdx.x / TPR;
   const int lane = threadIdx.x % TPR;
-  const int64_t rows = M1.rows;
-  double acc = 0.0;
-  if (row < rows) {
-    acc = csr_row<TPR, VT>(M1, v1, row, lane);
-    if (M2.ptr) acc += csr_row<TPR, VT>(M2, v2, row, lane);
-    if (lane == 0) y[row] = acc;
-  }
-  if (MODE == 1) {
-    double t[1] = {(row < rows && lane == 0) ? v1[row] * acc : 0.0};
-    block_total<BS, 1, false>(t, scr);
-    if (threadIdx.x == 0) part[blockIdx.x] = t[0];
-  }
+  const int64_t row = (int64_t)sl * RPS + i;
+  double acc = sell_row<TPR, VT>(M1, v1, a0, (a1 - a0) / RPS, i, lane);
+  if (M2.off) acc += sell_row<TPR, VT>(M2, v2, b0, (b1 - b0) / RPS, i, lane);
+  if (lane == 0 && row < M1.rows) y[row] = acc;
 }
 
 // Start of iteration `it`: scalars from the partial sums, convergence test,
-// then (blocks < Gf) the fine update and (blocks >= Gf) b_1 = R_0 r_new.
+// then (blocks < Gf) the fine update and (blocks >= Gf) b_1 = R_0 r_new, BS/256
+// slices of R_0 per workgroup.  Everything that does not need the scalars
+// (the fine vectors, and all of R_0 w) is loaded or computed before the
+// reduction, so the reduction is the only wait on the partial sums.
 template <int BS, int TPR, typename VT>
 __global__ __launch_bounds__(BS) void amg_update_kernel(int64_t n, AmgPtrs P, double* __restrict__ x,
-                                                        CsrOp R0, double* __restrict__ b1,
+                                                        SellOp R0, double* __restrict__ b1,
                                                         int32_t it, int32_t max_it, double rtol,
                                                         double atol) {
   __shared__ double scr[BS / 64 * 5];
@@ -261,12 +297,55 @@ __global__ __launch_bounds__(BS) void amg_update_kernel(int64_t n, AmgPtrs P, do
   const bool was_done = P.ss[SS_DONE] != 0.0;
   const int q = it & 1;
   const int Gf = P.Gf;
+  const bool fine = (int)blockIdx.x < Gf;
+  // partial sums first: the reduction waits only on these
+  const int b0 = (int)threadIdx.x < Gf ? (int)threadIdx.x : 0;
   double t[5];
-  t[0] = sum_parts<BS>(P.pg, P.Gu);
-  t[1] = sum_parts<BS>(P.pd, Gf);
-  t[2] = sum_parts<BS>(P.prr[q], Gf);
-  t[3] = sum_parts<BS>(P.prr[q] + Gf, Gf);
-  t[4] = sum_parts<BS>(P.pbb, Gf);
+  t[0] = P.pg[b0];
+  t[1] = P.pd[b0];
+  t[2] = P.prr[q][b0];
+  t[3] = P.prr[q][Gf + b0];
+  t[4] = P.pbb[b0];
+  const double* __restrict__ rq = P.r[q];
+  const double* __restrict__ sq = P.s[q];
+  const double* __restrict__ w = P.w;
+  // fine role: the row's vectors; restriction role: (R_0 w)_j and b_1, z_1
+  const int blk = fine ? xcd_map(blockIdx.x, Gf) : 0;
+  const int64_t row = fine ? (int64_t)blk * BS + threadIdx.x
+                           : (int64_t)(blockIdx.x - Gf) * (BS / TPR) + threadIdx.x / TPR;
+  double f0 = 0.0, f1 = 0.0, f2 = 0.0, f3 = 0.0, f4 = 0.0, f5 = 0.0;
+  if (fine) {
+    const int64_t rc = row < n ? row : n - 1;
+    f0 = w[rc];
+    f1 = sq[rc];
+    f2 = P.p[rc];
+    f3 = P.u[rc];
+    f4 = x[rc];
+    f5 = rq[rc];
+  } else {
+    constexpr int RPS = kSliceThreads / TPR;
+    const int sl = (int)(row / RPS);
+    const int i = (int)(row % RPS);
+    const int nsl = (int)((R0.rows + RPS - 1) / RPS);
+    if (sl < nsl) {  // uniform per slice (a workgroup may run past the last one)
+      const int32_t o0 = R0.off[sl], o1 = R0.off[sl + 1];
+      f0 = sell_row<TPR, VT>(R0, w, o0, (o1 - o0) / RPS, i, (int)threadIdx.x % TPR);
+    }
+    const int64_t rc = row < R0.rows ? row : R0.rows - 1;
+    f1 = P.z1[rc];
+    f2 = b1[rc];
+  }
+  if ((int)threadIdx.x >= Gf) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) t[k] = 0.0;
+  }
+  for (int b = threadIdx.x + BS; b < Gf; b += BS) {  // only when Gf > BS
+    t[0] += P.pg[b];
+    t[1] += P.pd[b];
+    t[2] += P.prr[q][b];
+    t[3] += P.prr[q][Gf + b];
+    t[4] += P.pbb[b];
+  }
   block_total<BS, 5, true>(t, scr);
   const double gam = t[0], del = t[1], rr = t[2];
   const double tol2 = fmax(rtol * rtol * t[4], atol * atol);
@@ -300,18 +379,13 @@ __global__ __launch_bounds__(BS) void amg_update_kernel(int64_t n, AmgPtrs P, do
     }
   }
   if (stop) return;
-  const double* __restrict__ rq = P.r[q];
-  const double* __restrict__ sq = P.s[q];
-  const double* __restrict__ w = P.w;
-  if ((int)blockIdx.x < Gf) {
-    const int blk = xcd_map(blockIdx.x, Gf);
-    const int64_t row = (int64_t)blk * BS + threadIdx.x;
+  if (fine) {
     double v[2] = {0.0, 0.0};
     if (row < n) {
-      const double s_n = next_s(w[row], beta, sq[row]);
-      const double p_n = fma(beta, P.p[row], P.u[row]);
-      const double x_n = fma(alpha, p_n, x[row]);
-      const double r_n = next_r(rq[row], alpha, s_n);
+      const double s_n = next_s(f0, beta, f1);
+      const double p_n = fma(beta, f2, f3);
+      const double x_n = fma(alpha, p_n, f4);
+      const double r_n = next_r(f5, alpha, s_n);
       __builtin_nontemporal_store(s_n, &P.s[1 - q][row]);
       __builtin_nontemporal_store(r_n, &P.r[1 - q][row]);
       __builtin_nontemporal_store(p_n, &P.p[row]);
@@ -324,48 +398,50 @@ __global__ __launch_bounds__(BS) void amg_update_kernel(int64_t n, AmgPtrs P, do
       P.prr[1 - q][blk] = v[0];
       P.prr[1 - q][Gf + blk] = v[1];
     }
-  } else {
-    constexpr int RPB = BS / TPR;
-    const int64_t row = (int64_t)(blockIdx.x - Gf) * RPB + threadIdx.x / TPR;
-    const int lane = threadIdx.x % TPR;
-    if (row < R0.rows) {
-      double acc = 0.0;
-      const int32_t k0 = R0.ptr[row], k1 = R0.ptr[row + 1];
-      for (int32_t k = k0 + lane; k < k1; k += TPR) {
-        const int32_t c = R0.col[k];
-        const double rn = next_r(rq[c], alpha, next_s(w[c], beta, sq[c]));
-        acc = fma(ldv<VT>(R0.val, k), rn, acc);
-      }
-#pragma unroll
-      for (int off = TPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-      if (lane == 0) b1[row] = acc;
-    }
+  } else if (row < R0.rows && threadIdx.x % TPR == 0) {
+    const double z_n = next_s(f0, beta, f1);
+    b1[row] = next_r(f2, alpha, z_n);
+    P.z1[row] = z_n;
   }
 }
 
-// w = A u (plain operator, ELL), partials of (w, u).
+// w = A u (plain operator, ELL), partials of (w, u) and (r, u), r the
+// residual the preconditioner was applied to.
 template <int BS, int WMAX, bool EXACT>
 __global__ __launch_bounds__(BS) void amg_matvec_kernel(int64_t n, int32_t W, int64_t ld,
                                                         const int32_t* __restrict__ cols,
-                                                        const double* __restrict__ vals, AmgPtrs P) {
-  __shared__ double scr[BS / 64];
-  if (P.ss[SS_DONE] != 0.0) return;  // uniform
+                                                        const double* __restrict__ vals, AmgPtrs P,
+                                                        const double* __restrict__ r) {
+  __shared__ double scr[BS / 64 * 2];
+  const bool was_done = P.ss[SS_DONE] != 0.0;  // tested once the matrix loads are in flight
   const int blk = xcd_map(blockIdx.x, P.Gf);
   const int64_t row = (int64_t)blk * BS + threadIdx.x;
+  const int64_t rowc = row < n ? row : n - 1;
   const int Wk = EXACT ? WMAX : W;
-  double t[1] = {0.0};
+  int32_t c[WMAX];
+  double av[WMAX];
+#pragma unroll
+  for (int j = 0; j < WMAX; ++j) {
+    c[j] = j < Wk ? cols[j * ld + rowc] : (int32_t)rowc;
+    av[j] = j < Wk ? vals[j * ld + rowc] : 0.0;
+  }
+  if (was_done) return;  // uniform
+  double t[2] = {0.0, 0.0};
   if (row < n) {
     const double* __restrict__ u = P.u;
     double acc = 0.0;
 #pragma unroll
-    for (int j = 0; j < WMAX; ++j) {
-      if (j < Wk) acc = fma(vals[j * ld + row], u[cols[j * ld + row]], acc);
-    }
+    for (int j = 0; j < WMAX; ++j) acc = fma(av[j], u[c[j]], acc);
     __builtin_nontemporal_store(acc, &P.w[row]);
-    t[0] = acc * u[row];
+    const double uk = u[row];
+    t[0] = acc * uk;
+    t[1] = r[row] * uk;
   }
-  block_total<BS, 1, false>(t, scr);
-  if (threadIdx.x == 0) P.pd[blk] = t[0];
+  block_total<BS, 2, false>(t, scr);
+  if (threadIdx.x == 0) {
+    P.pd[blk] = t[0];
+    P.pg[blk] = t[1];
+  }
 }
 
 __global__ void amg_gather_kernel(int64_t n, const int32_t* __restrict__ perm,
@@ -382,29 +458,31 @@ __global__ void amg_scatter_kernel(int64_t n, const int32_t* __restrict__ perm,
 
 int fine_bs(int64_t n) { return n <= 65536 ? 256 : 512; }
 
-CsrOp to_op(const smx_csr_op& o) {
-  CsrOp c;
-  c.ptr = o.ptr;
+SellOp to_op(const smx_amg_op& o) {
+  SellOp c;
+  c.off = o.slice_off;
   c.col = o.col;
   c.val = o.val;
   c.rows = o.rows;
   return c;
 }
 
-constexpr int kLevelBS = 256;
+int64_t slices(int64_t rows, int tpr) {
+  const int64_t rps = kSliceThreads / tpr;
+  return (rows + rps - 1) / rps;
+}
 
-template <typename VT, int MODE>
-int launch_level(const smx_csr_op& m1, const double* v1, const smx_csr_op* m2, const double* v2,
-                 double* y, const double* ss, double* part, int tpr, hipStream_t st) {
-  const CsrOp a = to_op(m1);
-  CsrOp b = {nullptr, nullptr, nullptr, 0};
+template <typename VT>
+int launch_level(const smx_amg_op& m1, const double* v1, const smx_amg_op* m2, const double* v2,
+                 double* y, const double* ss, int tpr, hipStream_t st) {
+  const SellOp a = to_op(m1);
+  SellOp b = {nullptr, nullptr, nullptr, 0};
   if (m2 && m2->rows > 0) b = to_op(*m2);
   if (a.rows <= 0) return SMX_OK;
-  const int64_t rpb = kLevelBS / tpr;
-  const unsigned g = (unsigned)((a.rows + rpb - 1) / rpb);
-#define SMX_AMG_LVL(T)                                                                        \
-  hipLaunchKernelGGL((amg_level_kernel<kLevelBS, T, VT, MODE>), dim3(g), dim3(kLevelBS), 0, st, \
-                     a, v1, b, v2, y, ss, part)
+  const unsigned g = (unsigned)slices(a.rows, tpr);
+#define SMX_AMG_LVL(T)                                                                          \
+  hipLaunchKernelGGL((amg_level_kernel<T, VT>), dim3(g), dim3(kSliceThreads), 0, st, a, v1, b, \
+                     v2, y, ss)
   switch (tpr) {
     case 1: SMX_AMG_LVL(1); break;
     case 2: SMX_AMG_LVL(2); break;
@@ -419,19 +497,13 @@ int launch_level(const smx_csr_op& m1, const double* v1, const smx_csr_op* m2, c
   return SMX_OK;
 }
 
-int64_t up0_blocks(int64_t n, int tpr) {
-  const int64_t rpb = kLevelBS / tpr;
-  return (n + rpb - 1) / rpb;
-}
-
 struct Layout {
-  int64_t Gf, Gu_max, nvec;
+  int64_t Gf, nvec;
 };
 
 Layout layout(int64_t n) {
   Layout L;
   L.Gf = (n + fine_bs(n) - 1) / fine_bs(n);
-  L.Gu_max = up0_blocks(n, 64);
   L.nvec = (n + 63) / 64 * 64;
   return L;
 }
@@ -445,23 +517,22 @@ int vcycle_tail(const smx_amg_level* lv, int nlev, const double* r_fine, AmgPtrs
                 hipStream_t st) {
   int rc;
   for (int l = 1; l + 1 < nlev; ++l) {  // down: b_{l+1} = R_l b_l
-    rc = launch_level<VT, 0>(lv[l].R, lv[l].b, nullptr, nullptr, lv[l + 1].b, P.ss, nullptr,
-                             lv[l].r_tpr, st);
+    rc = launch_level<VT>(lv[l].R, lv[l].b, nullptr, nullptr, lv[l + 1].b, P.ss, lv[l].r_tpr, st);
     if (rc) return rc;
   }
   if (nlev > 1) {  // coarse: x_c = Z b_c
     const smx_amg_level& c = lv[nlev - 1];
-    rc = launch_level<VT, 0>(c.S, c.b, nullptr, nullptr, c.x, P.ss, nullptr, c.s_tpr, st);
+    rc = launch_level<VT>(c.S, c.b, nullptr, nullptr, c.x, P.ss, c.s_tpr, st);
     if (rc) return rc;
   }
   for (int l = nlev - 2; l >= 1; --l) {  // up: x_l = S_l b_l + Q_l x_{l+1}
-    rc = launch_level<VT, 0>(lv[l].S, lv[l].b, &lv[l].Q, lv[l + 1].x, lv[l].x, P.ss, nullptr,
-                             lv[l].s_tpr, st);
+    rc = launch_level<VT>(lv[l].S, lv[l].b, &lv[l].Q, lv[l + 1].x, lv[l].x, P.ss, lv[l].s_tpr,
+                          st);
     if (rc) return rc;
   }
-  const smx_csr_op* q0 = nlev > 1 ? &lv[0].Q : nullptr;
+  const smx_amg_op* q0 = nlev > 1 ? &lv[0].Q : nullptr;
   const double* x1 = nlev > 1 ? lv[1].x : nullptr;
-  return launch_level<VT, 1>(lv[0].S, r_fine, q0, x1, P.u, P.ss, P.pg, lv[0].s_tpr, st);
+  return launch_level<VT>(lv[0].S, r_fine, q0, x1, P.u, P.ss, lv[0].s_tpr, st);
 }
 
 template <int BS, int WMAX, bool EXACT, typename VT>
@@ -474,35 +545,34 @@ int run_amg(int64_t n, int32_t W, int64_t ld, const int32_t* cols, const double*
   const int Gf = P.Gf;
   const unsigned gs = (unsigned)((n + 255) / 256);
   double* x = perm ? x_perm : x_user;
+  const int64_t n1 = nlev > 1 ? lv[0].R.rows : 0;
   if (perm && warm) {
     hipLaunchKernelGGL(amg_gather_kernel, dim3(gs), dim3(256), 0, st, n, perm, x_user, x);
     SMX_LAUNCH_CHECK();
   }
   if (mode == 0) {
     hipLaunchKernelGGL((amg_init_kernel<BS, WMAX>), dim3(Gf), dim3(BS), 0, st, n, mode, W, ld, cols,
-                       vals, b, b_part, nb_part, project, bpos, gb, x, warm, perm, P);
+                       vals, b, b_part, nb_part, project, bpos, gb, x, warm, perm, P, n1);
   } else {
     hipLaunchKernelGGL((amg_init_kernel<BS, WMAX>), dim3(Gf), dim3(BS), 0, st, n, mode, raw_W,
                        raw_ld, raw_cols, raw_vals, b, b_part, nb_part, project, bpos, gb, x, warm,
-                       perm, P);
+                       perm, P, n1);
   }
   SMX_LAUNCH_CHECK();
   int rc;
   // u0 = M^-1 r0, w0 = A u0
   if (nlev > 1) {
-    rc = launch_level<VT, 0>(lv[0].R, P.r[0], nullptr, nullptr, lv[1].b, P.ss, nullptr,
-                             lv[0].r_tpr, st);
+    rc = launch_level<VT>(lv[0].R, P.r[0], nullptr, nullptr, lv[1].b, P.ss, lv[0].r_tpr, st);
     if (rc) return rc;
   }
   rc = vcycle_tail<VT>(lv, nlev, P.r[0], P, st);
   if (rc) return rc;
   hipLaunchKernelGGL((amg_matvec_kernel<BS, WMAX, EXACT>), dim3(Gf), dim3(BS), 0, st, n, W, ld,
-                     cols, vals, P);
+                     cols, vals, P, P.r[0]);
   SMX_LAUNCH_CHECK();
-  const int64_t n1 = nlev > 1 ? lv[0].R.rows : 0;
   const int rt = nlev > 1 ? lv[0].r_tpr : 1;
   const int64_t Gr = nlev > 1 ? (n1 + (BS / rt) - 1) / (BS / rt) : 0;
-  CsrOp R0 = {nullptr, nullptr, nullptr, 0};
+  SellOp R0 = {nullptr, nullptr, nullptr, 0};
   if (nlev > 1) R0 = to_op(lv[0].R);
   double* b1 = nlev > 1 ? lv[1].b : nullptr;
   if (check_every < 1) check_every = 8;
@@ -531,7 +601,7 @@ int run_amg(int64_t n, int32_t W, int64_t ld, const int32_t* cols, const double*
       rc = vcycle_tail<VT>(lv, nlev, P.r[1 - (it & 1)], P, st);
       if (rc) return rc;
       hipLaunchKernelGGL((amg_matvec_kernel<BS, WMAX, EXACT>), dim3(Gf), dim3(BS), 0, st, n, W,
-                         ld, cols, vals, P);
+                         ld, cols, vals, P, P.r[1 - (it & 1)]);
       SMX_LAUNCH_CHECK();
     }
     launched += nb;
@@ -559,14 +629,14 @@ int check_levels(const smx_amg_level* levels, int32_t nlev, int64_t n) {
               (long long)levels[0].n, (long long)n);
   for (int l = 0; l < nlev; ++l) {
     const smx_amg_level& L = levels[l];
-    SMX_REQUIRE(L.n >= 1 && L.S.rows == L.n && L.S.ptr && L.S.col && L.S.val, SMX_ERR_SHAPE,
+    SMX_REQUIRE(L.n >= 1 && L.S.rows == L.n && L.S.slice_off && L.S.val, SMX_ERR_SHAPE,
                 "level %d: S must have n rows", l);
     SMX_REQUIRE(valid_tpr(L.s_tpr), SMX_ERR_VALUE,
                 "level %d: lanes per row must be 1..64, a power of 2", l);
     if (l > 0) SMX_REQUIRE(L.b && L.x, SMX_ERR_VALUE, "level %d: b and x vectors required", l);
     if (l + 1 < nlev) {
-      SMX_REQUIRE(L.Q.rows == L.n && L.R.rows == levels[l + 1].n && L.Q.ptr && L.Q.col &&
-                      L.Q.val && L.R.ptr && L.R.col && L.R.val,
+      SMX_REQUIRE(L.Q.rows == L.n && L.R.rows == levels[l + 1].n && L.Q.slice_off && L.Q.col &&
+                      L.Q.val && L.R.slice_off && L.R.col && L.R.val,
                   SMX_ERR_SHAPE, "level %d: Q must have n_l rows, R n_{l+1} rows", l);
       SMX_REQUIRE(valid_tpr(L.r_tpr), SMX_ERR_VALUE,
                   "level %d: lanes per row must be 1..64, a power of 2", l);
@@ -580,13 +650,11 @@ int apply_once(const smx_amg_level* lv, int nlev, const double* r, double* u, do
                hipStream_t st) {
   AmgPtrs P = {};
   P.ss = work;
-  P.pg = work + HDR;
   P.u = u;
   hipLaunchKernelGGL(amg_clear_status_kernel, dim3(1), dim3(64), 0, st, P.ss);
   SMX_LAUNCH_CHECK();
   if (nlev > 1) {
-    const int rc = launch_level<VT, 0>(lv[0].R, r, nullptr, nullptr, lv[1].b, P.ss, nullptr,
-                                       lv[0].r_tpr, st);
+    const int rc = launch_level<VT>(lv[0].R, r, nullptr, nullptr, lv[1].b, P.ss, lv[0].r_tpr, st);
     if (rc) return rc;
   }
   return vcycle_tail<VT>(lv, nlev, r, P, st);
@@ -609,7 +677,7 @@ extern "C" int smx_amg_apply(const smx_amg_level* levels, int32_t nlev, int32_t 
 
 extern "C" int64_t smx_pcg_amg_ws_size(int64_t n) {
   const Layout L = layout(n);
-  return HDR + 6 * L.Gf + L.Gu_max + 8 * L.nvec + 64;
+  return HDR + 7 * L.Gf + 9 * L.nvec + 64;
 }
 
 extern "C" int smx_pcg_amg(int64_t n, int32_t width, int64_t ld, const int32_t* cols,
@@ -647,7 +715,6 @@ extern "C" int smx_pcg_amg(int64_t n, int32_t width, int64_t ld, const int32_t* 
   const Layout Lo = layout(n);
   AmgPtrs P;
   P.Gf = (int)Lo.Gf;
-  P.Gu = (int)up0_blocks(n, levels[0].s_tpr);
   P.ss = work;
   P.st[0] = work + ST_OFF;
   P.st[1] = work + ST_OFF + ST_SIZE;
@@ -657,7 +724,7 @@ extern "C" int smx_pcg_amg(int64_t n, int32_t width, int64_t ld, const int32_t* 
   P.prr[1] = q + 3 * Lo.Gf;
   P.pd = q + 5 * Lo.Gf;
   P.pg = q + 6 * Lo.Gf;
-  double* v0 = q + 6 * Lo.Gf + Lo.Gu_max;
+  double* v0 = q + 7 * Lo.Gf;
   v0 += (64 - (((uintptr_t)v0 / sizeof(double)) & 63)) & 63;  // 512-byte aligned vectors
   P.r[0] = v0;
   P.r[1] = v0 + Lo.nvec;
@@ -667,6 +734,7 @@ extern "C" int smx_pcg_amg(int64_t n, int32_t width, int64_t ld, const int32_t* 
   P.u = v0 + 5 * Lo.nvec;
   P.p = v0 + 6 * Lo.nvec;
   double* x_perm = v0 + 7 * Lo.nvec;
+  P.z1 = v0 + 8 * Lo.nvec;  // n_1 < n
   double hst[SS_SIZE] = {0};
   const int bs = fine_bs(n);
   const int32_t wmax = (mode == 1 && raw_width > width) ? raw_width : width;

@@ -195,21 +195,76 @@ def apply_vcycle(levels: list[HostLevel], b: np.ndarray, l: int = 0) -> np.ndarr
     return L.S @ b + L.Q @ apply_vcycle(levels, L.R @ b, l + 1)
 
 
-def lanes_per_row(*ops) -> int:
-    """Lanes per CSR row for the device launch: about two entries per lane."""
+def lanes_per_row(*ops, per_lane: int = 2) -> int:
+    """Lanes per CSR row for the device launch: about ``per_lane`` entries per
+    lane (a power of two, at most a wavefront)."""
     avg = max((M.nnz / max(M.shape[0], 1)) for M in ops if M is not None)
     t = 1
-    while t < 64 and 2 * t < avg:
+    while t < 64 and per_lane * t < avg:
         t *= 2
     return t
 
 
-class CsrOp(C.Structure):
-    _fields_ = [("ptr", C.c_void_p), ("col", C.c_void_p), ("val", C.c_void_p), ("rows", C.c_int64)]
+SLICE_THREADS = 256
+
+
+def sell_layout(M: sp.csr_matrix, tpr: int, dense: bool = False):
+    """Sliced-ELL arrays of the device operator (smx_amg_op): 256/tpr rows per
+    slice, each slice as wide as its longest row; slot j of local row i at
+    off[s] + j*rps + i (tpr == 1) or off[s] + i*W_s + j.  Padding slots: value
+    0, column 0.  ``dense``: full rows, no column array (slot j is column j).
+    Returns (off int32, col int32 or None, val float64)."""
+    M = sp.csr_matrix(M)
+    M.sort_indices()
+    rps = SLICE_THREADS // tpr
+    n = M.shape[0]
+    nsl = (n + rps - 1) // rps
+    lens = np.diff(M.indptr).astype(np.int64)
+    if dense and (M.nnz != n * M.shape[1] or not np.array_equal(M.indices, np.tile(np.arange(M.shape[1]), n))):
+        raise ValueError("dense operator must store every column of every row in order")
+    lp = np.zeros(nsl * rps, dtype=np.int64)
+    lp[:n] = lens
+    W = lp.reshape(nsl, rps).max(axis=1) if nsl else np.zeros(0, dtype=np.int64)
+    off = np.zeros(nsl + 1, dtype=np.int64)
+    off[1:] = np.cumsum(W * rps)
+    if off[-1] >= 2 ** 31:
+        raise ValueError("operator too large for int32 slot offsets")
+    r = np.repeat(np.arange(n, dtype=np.int64), lens)
+    j = np.arange(M.nnz, dtype=np.int64) - np.repeat(M.indptr[:-1].astype(np.int64), lens)
+    sl, i = r // rps, r % rps
+    pos = off[sl] + (j * rps + i if tpr == 1 else i * W[sl] + j)
+    val = np.zeros(int(off[-1]))
+    val[pos] = M.data
+    col = None
+    if not dense:
+        col = np.zeros(int(off[-1]), dtype=np.int32)
+        col[pos] = M.indices
+    return off.astype(np.int32), col, val
+
+
+def sell_matvec(off, col, val, rows: int, tpr: int, v: np.ndarray) -> np.ndarray:
+    """Host restatement of the device slice loop (layout check)."""
+    rps = SLICE_THREADS // tpr
+    y = np.zeros(rows)
+    for s in range(len(off) - 1):
+        W = (int(off[s + 1]) - int(off[s])) // rps
+        for i in range(rps):
+            row = s * rps + i
+            if row >= rows:
+                break
+            jj = np.arange(W)
+            pos = off[s] + (jj * rps + i if tpr == 1 else i * W + jj)
+            c = jj if col is None else col[pos]
+            y[row] = float(val[pos] @ v[c])
+    return y
+
+
+class AmgOp(C.Structure):
+    _fields_ = [("slice_off", C.c_void_p), ("col", C.c_void_p), ("val", C.c_void_p), ("rows", C.c_int64)]
 
 
 class AmgLevel(C.Structure):
-    _fields_ = [("n", C.c_int64), ("S", CsrOp), ("Q", CsrOp), ("R", CsrOp),
+    _fields_ = [("n", C.c_int64), ("S", AmgOp), ("Q", AmgOp), ("R", AmgOp),
                 ("s_tpr", C.c_int32), ("r_tpr", C.c_int32),
                 ("b", C.c_void_p), ("x", C.c_void_p)]
 
@@ -232,14 +287,16 @@ class AmgHierarchy:
         for l, L in enumerate(self.host):
             c = self.c_levels[l]
             c.n = L.n
-            c.S = self._op(L.S, vdt)
+            # the fine restriction runs inside the CG update launch, whose every
+            # workgroup first re-reduces the partial sums: fewer, fuller
+            # workgroups there.  The fine S and Q (a few entries per row) run
+            # one row per lane.
+            c.s_tpr = 1 if l == 0 and L.Q is not None else lanes_per_row(L.S, L.Q)
+            c.r_tpr = lanes_per_row(L.R, per_lane=8 if l == 0 else 2) if L.Q is not None else 1
+            c.S = self._op(L.S, vdt, c.s_tpr, dense=L.Q is None)
             if L.Q is not None:
-                c.Q = self._op(L.Q, vdt)
-                c.R = self._op(L.R, vdt)
-                c.r_tpr = lanes_per_row(L.R)
-            else:
-                c.r_tpr = 1
-            c.s_tpr = lanes_per_row(L.S, L.Q)
+                c.Q = self._op(L.Q, vdt, c.s_tpr)
+                c.R = self._op(L.R, vdt, c.r_tpr)
             if l > 0:
                 b = torch.zeros(L.n, dtype=torch.float64, device=device)
                 x = torch.zeros(L.n, dtype=torch.float64, device=device)
@@ -248,14 +305,20 @@ class AmgHierarchy:
                 c.x = x.data_ptr()
         self.nbytes = sum(t.numel() * t.element_size() for t in self._keep)
 
-    def _op(self, M: sp.csr_matrix, vdt) -> CsrOp:
+    def _op(self, M: sp.csr_matrix, vdt, tpr: int, dense: bool = False) -> AmgOp:
+        """Upload one operator in the sliced-ELL layout for ``tpr`` lanes per
+        row; ``dense``: full rows without column indices (the coarse inverse)."""
         import torch
-        M = sp.csr_matrix(M)
-        ptr = torch.from_numpy(M.indptr.astype(np.int32)).to(self.device)
-        col = torch.from_numpy(M.indices.astype(np.int32)).to(self.device)
-        val = torch.from_numpy(M.data.astype(np.float64)).to(self.device).to(vdt)
-        self._keep += [ptr, col, val]
-        return CsrOp(ptr.data_ptr(), col.data_ptr(), val.data_ptr(), M.shape[0])
+        off, col, val = sell_layout(M, tpr, dense)
+        offd = torch.from_numpy(off).to(self.device)
+        vald = torch.from_numpy(val).to(self.device).to(vdt)
+        self._keep += [offd, vald]
+        cp = None
+        if col is not None:
+            cold = torch.from_numpy(col).to(self.device)
+            self._keep.append(cold)
+            cp = cold.data_ptr()
+        return AmgOp(offd.data_ptr(), cp, vald.data_ptr(), M.shape[0])
 
     @property
     def nlev(self) -> int:

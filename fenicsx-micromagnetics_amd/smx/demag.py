@@ -147,8 +147,8 @@ class DemagFieldLindholmGPU:
                  *, ksp_atol=1e-12, ksp_max_it_u1=2000, ksp_max_it_u2=12000,
                  warm_start=False, device=None, operators: OperatorSet | None = None,
                  geometry: BoundaryGeometry | None = None, hmatrix_prebuilt=None,
-                 hmatrix_near_fp32=False, cg_reorder=True, ksp_pc_type="gamg", amg_fp32=None,
-                 amg_max_coarse=1024, amg_theta=0.0, verbose=False):
+                 hmatrix_near_fp32=False, cg_reorder=True, ksp_pc_type="gamg", ksp_pc_type_u2=None,
+                 amg_fp32=None, amg_max_coarse=1024, amg_theta=0.0, verbose=False):
         if not isinstance(mesh, TetMesh):
             raise TypeError("mesh must be a smx.mesh.TetMesh")
         self.mesh = mesh
@@ -203,20 +203,17 @@ class DemagFieldLindholmGPU:
         self.dinv_u2 = to_dev(1.0 / d2[sel], dev)
         # plain P A P^T: the lifting operator b = -A_raw g of the u2 solve
         self.pc_type = str(ksp_pc_type).strip().lower()
-        if self.pc_type not in ("gamg", "jacobi"):
-            raise ValueError("ksp_pc_type must be 'gamg' or 'jacobi'")
+        self.pc_type_u2 = self.pc_type if ksp_pc_type_u2 is None else str(ksp_pc_type_u2).strip().lower()
+        if self.pc_type not in ("gamg", "jacobi") or self.pc_type_u2 not in ("gamg", "jacobi"):
+            raise ValueError("ksp_pc_type / ksp_pc_type_u2 must be 'gamg' or 'jacobi'")
         A1h = self.ops.compact_host(fem.stiff, perm)
         A2h = self.ops.compact_host(fem.stiffness_bc(), perm)
         self.A_raw = DeviceEll.from_csr(A1h.indptr, A1h.indices, A1h.data, self.N, dev)
         A_bc = DeviceEll.from_csr(A2h.indptr, A2h.indices, A2h.data, self.N, dev)
-        if self.pc_type == "jacobi":
-            # the Jacobi CG kernels take A D^-1 with the diagonal in slot 0
-            self.A_u1 = solver_operator(self.A_raw, self.dinv_u1)
-            self.A_u2 = solver_operator(A_bc, self.dinv_u2)
-        else:
-            # the AMG-CG kernels take the plain operators
-            self.A_u1 = self.A_raw
-            self.A_u2 = A_bc
+        # the Jacobi CG kernels take A D^-1 with the diagonal in slot 0; the
+        # AMG-CG kernels take the plain operators
+        self.A_u1 = solver_operator(self.A_raw, self.dinv_u1) if self.pc_type == "jacobi" else self.A_raw
+        self.A_u2 = solver_operator(A_bc, self.dinv_u2) if self.pc_type_u2 == "jacobi" else A_bc
         self._amg_host = (A1h, A2h, int(amg_max_coarse), float(amg_theta))
 
         # ---------------- boundary operator ----------------
@@ -263,22 +260,24 @@ class DemagFieldLindholmGPU:
 
         # ---------------- AMG hierarchies (PETSc GAMG counterpart) ----------------
         self.amg_u1 = self.amg_u2 = None
-        if self.pc_type == "gamg":
-            from .amg import AmgHierarchy
-            if amg_fp32 is None:
-                # follow the boundary operator: a preconditioner in FP32 when the
-                # boundary operator is stored below FP64 (the outer CG stays FP64)
-                if self.boundary_backend == "dense":
-                    amg_fp32 = self.B_dtype != torch.float64
-                else:
-                    amg_fp32 = bool(hmatrix_use_fp32)
-            A1h, A2h, mc, th = self._amg_host
-            t1 = time.perf_counter()
-            self.amg_u1 = AmgHierarchy(A1h, dev, fp32=bool(amg_fp32), max_coarse=mc, theta=th)
-            self.amg_u2 = AmgHierarchy(A2h, dev, fp32=bool(amg_fp32), max_coarse=mc, theta=th)
-            self.setup_amg_seconds = time.perf_counter() - t1
-        self.amg_fp32 = bool(amg_fp32) if self.pc_type == "gamg" else False
+        if amg_fp32 is None:
+            # follow the boundary operator: a preconditioner in FP32 when the
+            # boundary operator is stored below FP64 (the outer CG stays FP64)
+            if self.boundary_backend == "dense":
+                amg_fp32 = self.B_dtype != torch.float64
+            else:
+                amg_fp32 = bool(hmatrix_use_fp32)
+        self.amg_fp32 = bool(amg_fp32)
+        A1h, A2h, mc, th = self._amg_host
         del self._amg_host
+        t1 = time.perf_counter()
+        if self.pc_type == "gamg" or self.pc_type_u2 == "gamg":
+            from .amg import AmgHierarchy
+            if self.pc_type == "gamg":
+                self.amg_u1 = AmgHierarchy(A1h, dev, fp32=self.amg_fp32, max_coarse=mc, theta=th)
+            if self.pc_type_u2 == "gamg":
+                self.amg_u2 = AmgHierarchy(A2h, dev, fp32=self.amg_fp32, max_coarse=mc, theta=th)
+        self.setup_amg_seconds = time.perf_counter() - t1
 
         # ---------------- per-evaluation buffers ----------------
         N = self.N
@@ -319,7 +318,7 @@ class DemagFieldLindholmGPU:
         sums = (C.c_double * 1)()
         raw = self.A_raw
         nbp = 0 if b_part is None else b_part.numel()
-        amg = (self.amg_u1 if mode == 0 else self.amg_u2) if self.pc_type == "gamg" else None
+        amg = self.amg_u1 if mode == 0 else self.amg_u2
         if amg is not None:
             rc = L.smx_pcg_amg(self.N, A.width, A.ld, _capi.ptr(A.cols), _capi.ptr(A.vals), mode,
                                _capi.ptr(b), _capi.ptr(b_part), nbp, 1, raw.width, raw.ld,

@@ -306,27 +306,33 @@ int smx_pcg_jacobi(int64_t n, int32_t width, int64_t ld, const int32_t* cols,
  *   Q_l (n_l x n_{l+1})  = (I - w D^-1 A_l) P_l     (P_l smoothed prolongator)
  *   R_l (n_{l+1} x n_l)  = Q_l^T
  * and on the last level only S = a dense (pseudo-)inverse of the coarse
- * operator with full rows.  The preconditioner is M^-1 b = S_0 b + Q_0 M_1^-1
- * (R_0 b), recursively: one damped-Jacobi pre- and post-smoothing sweep per
- * level.  CSR operators use int32 offsets and columns; values are FP64 or FP32
- * (val_dtype, the same for every operator); s_tpr / r_tpr are the lanes per
- * row (1..64, a power of two) for the S (with Q) and R launches.  Levels >= 1
- * bring device vectors b and x of n_l doubles.
+ * operator.  The preconditioner is M^-1 b = S_0 b + Q_0 M_1^-1 (R_0 b),
+ * recursively: one damped-Jacobi pre- and post-smoothing sweep per level.
+ *
+ * Operators are sliced ELL for `lanes` lanes per row (1..64, a power of two:
+ * s_tpr for S and Q of a level, r_tpr for R): slice s holds rows
+ * s*rps .. s*rps+rps-1 with rps = 256/lanes (the last slice padded to rps
+ * rows), all of width W_s = (slice_off[s+1]-slice_off[s])/rps; slot j of
+ * local row i sits at slice_off[s] + j*rps + i when lanes == 1 and at
+ * slice_off[s] + i*W_s + j otherwise.  Padding slots hold value 0 and a valid
+ * column.  col == NULL: dense rows, the column of slot j is j.  Values are
+ * FP64 or FP32 (val_dtype, the same for every operator); offsets and columns
+ * int32.  Levels >= 1 bring device vectors b and x of n_l doubles.
  *
  * Everything else (ELL operator = plain A in solver order, modes, perm,
  * convergence test, status record, SMX_PCG_SUMX) is as smx_pcg_jacobi.
  * work: smx_pcg_amg_ws_size(n) doubles.
  * ---------------------------------------------------------------------- */
 typedef struct {
-  const int32_t* ptr;  /* rows + 1 offsets */
-  const int32_t* col;
+  const int32_t* slice_off;  /* nslices + 1 slot offsets */
+  const int32_t* col;        /* NULL: dense rows */
   const void* val;
   int64_t rows;
-} smx_csr_op;
+} smx_amg_op;
 
 typedef struct {
   int64_t n;
-  smx_csr_op S, Q, R;
+  smx_amg_op S, Q, R;
   int32_t s_tpr, r_tpr;
   double* b;
   double* x;

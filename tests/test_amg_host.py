@@ -20,8 +20,8 @@ import scipy.sparse as sp
 from scipy.sparse.csgraph import connected_components
 
 from smx import mesh as M
-from smx.amg import (aggregate, apply_vcycle, build_levels, lanes_per_row, spectral_radius_dinv_a,
-                     strength_graph)
+from smx.amg import (aggregate, apply_vcycle, build_levels, lanes_per_row, sell_layout, sell_matvec,
+                     spectral_radius_dinv_a, strength_graph)
 from smx.fem import assemble_fem, compact_csr
 
 
@@ -205,7 +205,7 @@ def test_solver_rejects_inconsistent_levels_on_the_host():
     """Shape validation happens before any device call (fake, never
     dereferenced addresses)."""
     from smx import _capi
-    from smx.amg import AmgLevel, CsrOp
+    from smx.amg import AmgLevel, AmgOp as CsrOp
     L = _capi.lib()
     lv = (AmgLevel * 2)()
     fake = 0x1000
@@ -230,3 +230,29 @@ def test_solver_rejects_inconsistent_levels_on_the_host():
     args[21] = _capi.SMX_BF16
     assert L.smx_pcg_amg(*args) == _capi.SMX_ERR_TYPE
     assert L.smx_amg_apply(C.addressof(lv), 2, _capi.SMX_F64, fake, None, fake, None) == _capi.SMX_ERR_VALUE
+
+
+@pytest.mark.parametrize("tpr", [1, 4, 64])
+def test_sliced_ell_layout(tpr):
+    """The device slot layout, restated on the host, reproduces the CSR
+    product for ragged rows, empty rows and a partial last slice."""
+    rng = np.random.default_rng(0)
+    n, m = 700, 300
+    lens = rng.integers(0, 40, n)
+    lens[5] = 0
+    rows = np.repeat(np.arange(n), lens)
+    cols = np.concatenate([np.sort(rng.choice(m, k, replace=False)) for k in lens])
+    M = sp.csr_matrix((rng.standard_normal(rows.size), (rows, cols)), shape=(n, m))
+    off, col, val = sell_layout(M, tpr)
+    rps = 256 // tpr
+    assert len(off) == (n + rps - 1) // rps + 1 and off[0] == 0
+    assert np.all(np.diff(off) % rps == 0)
+    v = rng.standard_normal(m)
+    assert np.allclose(sell_matvec(off, col, val, n, tpr, v), M @ v, rtol=1e-14, atol=1e-14)
+    D = sp.csr_matrix(rng.standard_normal((50, 50)))
+    off, col, val = sell_layout(D, tpr, dense=True)
+    assert col is None
+    v = rng.standard_normal(50)
+    assert np.allclose(sell_matvec(off, None, val, 50, tpr, v), D @ v, rtol=1e-14, atol=1e-14)
+    with pytest.raises(ValueError):
+        sell_layout(M, tpr, dense=True)
