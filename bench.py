@@ -42,7 +42,7 @@ HBM_PEAK_GBS = 8000.0
 GAMMA = 2.211e5
 
 
-PC_TYPE = "gamg"
+PC_TYPE = "auto"
 PC_TYPE_U2 = None
 
 
@@ -526,9 +526,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", choices=["film", "cube", "adjoint", "disk"], default="film")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--pc", choices=["gamg", "jacobi"], default="gamg",
-                    help="CG preconditioner of both Poisson solves (the reference uses GAMG)")
-    ap.add_argument("--pc-u2", choices=["gamg", "jacobi"], default=None,
+    ap.add_argument("--pc", choices=["gamg", "jacobi", "auto"], default="auto",
+                    help="CG preconditioner of both Poisson solves (the reference uses GAMG; auto: the faster "
+                         "of GAMG and Jacobi, timed at set-up)")
+    ap.add_argument("--pc-u2", choices=["gamg", "jacobi", "auto"], default=None,
                     help="preconditioner of the u2 solve when it differs from --pc")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--pmc-probe", action="store_true",

@@ -103,7 +103,7 @@ class MicromagneticEnergy:
         if self.demag is not None:
             self.q = z(self.demag.Nb)
             self.t = z(self.demag.Nb)
-            self.ws3 = z(int(_capi.lib().smx_pcg_ws_size(self.N)))
+            self.ws3 = z(max(int(_capi.lib().smx_pcg_ws_size(self.N)), int(_capi.lib().smx_pcg_amg_ws_size(self.N))))
             self.tmp_t = z(((self.demag.Nb + 255) // 256) * self.demag.Nb) \
                 if self.demag.boundary_backend == "dense" else None
 
@@ -114,22 +114,12 @@ class MicromagneticEnergy:
     def _check(self, rc, what):
         _capi.check(rc, what)
 
-    def _solve(self, mode, A, dinv, x, ws, b=None, b_part=None, project=1, gb=None):
+    def _solve(self, op, mode, x, ws, b=None, b_part=None, project=1, gb=None):
+        """A solve of the demag field's operator ``op`` with its preconditioner,
+        at the u1 tolerance (non-convergence is not an error, as there)."""
         d = self.demag
-        its = C.c_int32(0)
-        rn = C.c_double(0.0)
-        sums = (C.c_double * 1)()
-        raw = d.A_raw
-        nbp = 0 if b_part is None else b_part.numel()
-        rc = self._L().smx_pcg_jacobi(self.N, A.width, A.ld, _capi.ptr(A.cols), _capi.ptr(A.vals), _capi.ptr(dinv),
-                                      mode, _capi.ptr(b), _capi.ptr(b_part), nbp, int(project), raw.width, raw.ld,
-                                      _capi.ptr(raw.cols), _capi.ptr(raw.vals), _capi.ptr(d.bpos_cg), _capi.ptr(gb),
-                                      _capi.ptr(x), 0, _capi.ptr(d.cg_perm), float(d.ksp_rtol_u1), 0.0,
-                                      int(d.ksp_max_it_u1), 32, 0,
-                                      _capi.ptr(ws), C.byref(its), C.byref(rn), sums, _capi.stream_ptr())
-        if rc != _capi.SMX_ERR_NOT_CONVERGED:
-            _capi.check(rc, "smx_pcg_jacobi")
-        return int(its.value)
+        return d.solve(op, mode, x, ws, b=b, b_part=b_part, project=project, gb=gb,
+                       rtol=d.ksp_rtol_u1, max_it=d.ksp_max_it_u1)
 
     def _sum(self, partials) -> float:
         return float(np.sum(partials.cpu().numpy()))
@@ -165,10 +155,10 @@ class MicromagneticEnergy:
             self._check(L.smx_energy_node_div(self.N, _capi.ptr(self.inc_ptr), _capi.ptr(self.inc),
                                               _capi.ptr(self.cgv), _capi.ptr(self.s), _capi.ptr(self.b),
                                               _capi.ptr(self.pN), st), "smx_energy_node_div")
-            self.its_forward = [self._solve(0, d.A_u1, d.dinv_u1, d.u1, d.ws1, b=self.b, b_part=self.pN)]
+            self.its_forward = [self._solve("u1", 0, d.u1, d.ws1, b=self.b, b_part=self.pN)]
             d.boundary_matvec(d.u1, d.y_rows, gather=d.bverts_gpu, shift_ptr=d.u1_sum_ptr,
                               shift_scale=1.0 / self.N)
-            self.its_forward.append(self._solve(1, d.A_u2, d.dinv_u2, d.u2, d.ws2, gb=d.y_rows))
+            self.its_forward.append(self._solve("u2", 1, d.u2, d.ws2, gb=d.y_rows, project=0))
             self._glue(3, self.N, d.u1, d.u2, sx=d.u1_sum_ptr, out=self.phi)
             self._check(L.smx_dot_partials(self.N, _capi.ptr(self.b), _capi.ptr(self.phi), _capi.ptr(self.pN2), st),
                         "smx_dot_partials")
@@ -186,7 +176,7 @@ class MicromagneticEnergy:
         N, Nb = self.N, d.Nb
         # E^T y: interior Dirichlet solve, then the boundary residual
         self._glue(0, N, y, idx=d.bpos_gpu, out=self.y)
-        its = [self._solve(0, d.A_u2, d.dinv_u2, self.zv, self.ws3, b=self.y, project=0)]
+        its = [self._solve("u2", 0, self.zv, self.ws3, b=self.y, project=0)]
         self.stiff_graph.spmv(self.zv, self.Az, 0)
         self._glue(1, Nb, y, self.Az, idx=d.bverts_gpu, out=self.q)
         # B^T
@@ -200,7 +190,7 @@ class MicromagneticEnergy:
         self._glue(2, N, y, self.t, idx=d.bpos_gpu, out=self.w)
         self._check(L.smx_dot_partials(N, _capi.ptr(self.w), _capi.ptr(self.ones), _capi.ptr(self.pN2), st),
                     "smx_dot_partials")
-        its.append(self._solve(0, d.A_u1, d.dinv_u1, self.v, self.ws3, b=self.w, b_part=self.pN2, project=1))
+        its.append(self._solve("u1", 0, self.v, self.ws3, b=self.w, b_part=self.pN2, project=1))
         self._glue(3, N, self.v, self.v, a=0.0, b=1.0, sy=self.ws3.data_ptr() + 8 * _capi.PCG_SUMX, out=out)
         self.its_backward = its
         return out

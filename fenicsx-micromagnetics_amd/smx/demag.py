@@ -149,6 +149,13 @@ class DemagFieldLindholmGPU:
                  geometry: BoundaryGeometry | None = None, hmatrix_prebuilt=None,
                  hmatrix_near_fp32=False, cg_reorder=True, ksp_pc_type="gamg", ksp_pc_type_u2=None,
                  amg_fp32=None, amg_max_coarse=1024, amg_theta=0.0, verbose=False):
+        """Keywords as the reference's constructor (Demag_Lindholm_GPU.py:101-155),
+        plus, keyword-only: the solver settings the reference hard-codes
+        (ksp_atol, max_it), ``ksp_pc_type`` / ``ksp_pc_type_u2`` in {"gamg",
+        "jacobi", "auto"} for the u1 / u2 preconditioner (u2 defaults to u1's;
+        "auto" times both on a fixed right side at set-up and keeps the faster),
+        ``amg_fp32`` (FP32 AMG operator values; default: FP32 when the boundary
+        operator is stored below FP64), and prebuilt geometry / operators."""
         if not isinstance(mesh, TetMesh):
             raise TypeError("mesh must be a smx.mesh.TetMesh")
         self.mesh = mesh
@@ -201,19 +208,22 @@ class DemagFieldLindholmGPU:
         d2 = self._diag(fem.stiffness_bc())
         self.dinv_u1 = to_dev(1.0 / d1[sel], dev)
         self.dinv_u2 = to_dev(1.0 / d2[sel], dev)
-        # plain P A P^T: the lifting operator b = -A_raw g of the u2 solve
-        self.pc_type = str(ksp_pc_type).strip().lower()
-        self.pc_type_u2 = self.pc_type if ksp_pc_type_u2 is None else str(ksp_pc_type_u2).strip().lower()
-        if self.pc_type not in ("gamg", "jacobi") or self.pc_type_u2 not in ("gamg", "jacobi"):
-            raise ValueError("ksp_pc_type / ksp_pc_type_u2 must be 'gamg' or 'jacobi'")
+        pc1 = str(ksp_pc_type).strip().lower()
+        pc2 = pc1 if ksp_pc_type_u2 is None else str(ksp_pc_type_u2).strip().lower()
+        if pc1 not in ("gamg", "jacobi", "auto") or pc2 not in ("gamg", "jacobi", "auto"):
+            raise ValueError("ksp_pc_type / ksp_pc_type_u2 must be 'gamg', 'jacobi' or 'auto'")
         A1h = self.ops.compact_host(fem.stiff, perm)
         A2h = self.ops.compact_host(fem.stiffness_bc(), perm)
+        # plain P A P^T: the lifting operator b = -A_raw g of the u2 solve and
+        # the operator of the AMG-CG kernels; the Jacobi CG kernels take the
+        # solver form A D^-1 with the diagonal in slot 0
         self.A_raw = DeviceEll.from_csr(A1h.indptr, A1h.indices, A1h.data, self.N, dev)
         A_bc = DeviceEll.from_csr(A2h.indptr, A2h.indices, A2h.data, self.N, dev)
-        # the Jacobi CG kernels take A D^-1 with the diagonal in slot 0; the
-        # AMG-CG kernels take the plain operators
-        self.A_u1 = solver_operator(self.A_raw, self.dinv_u1) if self.pc_type == "jacobi" else self.A_raw
-        self.A_u2 = solver_operator(A_bc, self.dinv_u2) if self.pc_type_u2 == "jacobi" else A_bc
+        self._cg = {}
+        for key, plain, dinv, pc in (("u1", self.A_raw, self.dinv_u1, pc1), ("u2", A_bc, self.dinv_u2, pc2)):
+            self._cg[key] = {"plain": plain, "dinv": dinv, "request": pc, "pc": pc,
+                             "jacobi": solver_operator(plain, dinv) if pc in ("jacobi", "auto") else None,
+                             "amg": None}
         self._amg_host = (A1h, A2h, int(amg_max_coarse), float(amg_theta))
 
         # ---------------- boundary operator ----------------
@@ -259,7 +269,6 @@ class DemagFieldLindholmGPU:
         self.setup_boundary_seconds = time.perf_counter() - t0
 
         # ---------------- AMG hierarchies (PETSc GAMG counterpart) ----------------
-        self.amg_u1 = self.amg_u2 = None
         if amg_fp32 is None:
             # follow the boundary operator: a preconditioner in FP32 when the
             # boundary operator is stored below FP64 (the outer CG stays FP64)
@@ -271,12 +280,10 @@ class DemagFieldLindholmGPU:
         A1h, A2h, mc, th = self._amg_host
         del self._amg_host
         t1 = time.perf_counter()
-        if self.pc_type == "gamg" or self.pc_type_u2 == "gamg":
-            from .amg import AmgHierarchy
-            if self.pc_type == "gamg":
-                self.amg_u1 = AmgHierarchy(A1h, dev, fp32=self.amg_fp32, max_coarse=mc, theta=th)
-            if self.pc_type_u2 == "gamg":
-                self.amg_u2 = AmgHierarchy(A2h, dev, fp32=self.amg_fp32, max_coarse=mc, theta=th)
+        for key, Ah in (("u1", A1h), ("u2", A2h)):
+            if self._cg[key]["request"] in ("gamg", "auto"):
+                from .amg import AmgHierarchy
+                self._cg[key]["amg"] = AmgHierarchy(Ah, dev, fp32=self.amg_fp32, max_coarse=mc, theta=th)
         self.setup_amg_seconds = time.perf_counter() - t1
 
         # ---------------- per-evaluation buffers ----------------
@@ -295,6 +302,77 @@ class DemagFieldLindholmGPU:
         self.ws2 = z(nws)
         self._have_u = False
         self._energy_out = z(1)
+        if "auto" in (pc1, pc2):
+            self._choose_preconditioners()
+
+    # ------------------------------------------------------------------
+    @property
+    def pc_type(self) -> str:
+        """Preconditioner of the u1 solve ("gamg" or "jacobi")."""
+        return self._cg["u1"]["pc"]
+
+    @property
+    def pc_type_u2(self) -> str:
+        return self._cg["u2"]["pc"]
+
+    @property
+    def amg_u1(self):
+        return self._cg["u1"]["amg"]
+
+    @property
+    def amg_u2(self):
+        return self._cg["u2"]["amg"]
+
+    @property
+    def A_u1(self) -> DeviceEll:
+        """The operator the u1 solve iterates on (solver form for Jacobi)."""
+        c = self._cg["u1"]
+        return c["jacobi"] if c["pc"] == "jacobi" else c["plain"]
+
+    @property
+    def A_u2(self) -> DeviceEll:
+        c = self._cg["u2"]
+        return c["jacobi"] if c["pc"] == "jacobi" else c["plain"]
+
+    def _choose_preconditioners(self, reps: int = 3):
+        """For every solve asked to be "auto": solve a fixed right side (Div m of
+        a seeded random unit m for u1, seeded random boundary data for u2) with
+        each preconditioner, time it with device events, keep the faster.  Both
+        solve to the same tolerance; the choice only moves the time."""
+        rng = np.random.default_rng(0)
+        g = rng.standard_normal((self.N, 3))
+        m = to_dev((g / np.linalg.norm(g, axis=1)[:, None]).ravel(), self.device)
+        self.Div.spmv(m, self.b_u1, kind=1, partials=self.b_part)
+        gb = to_dev(rng.standard_normal(self.Nb), self.device)
+        self.choice_seconds = {}
+        for key in ("u1", "u2"):
+            c = self._cg[key]
+            if c["request"] != "auto":
+                continue
+            best = None
+            for pc in ("jacobi", "gamg"):
+                c["pc"] = pc
+                run = (lambda: self.solve("u1", 0, self.u1, self.ws1, b=self.b_u1, b_part=self.b_part,
+                                          expected=its)) if key == "u1" else \
+                      (lambda: self.solve("u2", 1, self.u2, self.ws2, gb=gb, project=0, expected=its))
+                its = 0
+                its = run()
+                ts = []
+                for _ in range(reps):
+                    e0 = torch.cuda.Event(enable_timing=True)
+                    e1 = torch.cuda.Event(enable_timing=True)
+                    e0.record()
+                    run()
+                    e1.record()
+                    e1.synchronize()
+                    ts.append(e0.elapsed_time(e1) * 1e-3)
+                self.choice_seconds[(key, pc)] = min(ts)
+                if best is None or min(ts) < best[0]:
+                    best = (min(ts), pc)
+            c["pc"] = best[1]
+        self.u1.zero_()
+        self.u2.zero_()
+        self.converged = True
 
     # ------------------------------------------------------------------
     def _diag(self, vals):
@@ -310,41 +388,49 @@ class DemagFieldLindholmGPU:
         """Device address of sum(u1) left by the u1 solve."""
         return self.ws1.data_ptr() + 8 * _capi.PCG_SUMX
 
-    def _pcg(self, mode, A: DeviceEll, dinv, x, ws, rtol, max_it, b=None, b_part=None,
-             gb=None, expected=0, warm=False):
+    def solve(self, op: str, mode: int, x: torch.Tensor, ws: torch.Tensor, *, b=None, b_part=None,
+              project: int = 1, gb=None, rtol=None, max_it=None, expected: int = 0, warm: bool = False) -> int:
+        """One preconditioned CG solve on operator ``op`` ("u1": the Neumann
+        stiffness; "u2": the stiffness with Dirichlet identity rows) with that
+        operator's preconditioner.  mode 0: A x = b - mean(b)*project (b and
+        x in node order); mode 1: the Dirichlet lifting of boundary data gb.
+        Leaves sum(x) in ws[PCG_SUMX].  Returns the iteration count."""
+        c = self._cg[op]
+        if rtol is None:
+            rtol = self.ksp_rtol_u1 if op == "u1" else self.ksp_rtol_u2
+        if max_it is None:
+            max_it = self.ksp_max_it_u1 if op == "u1" else self.ksp_max_it_u2
         L = _capi.lib()
         its = C.c_int32(0)
         rn = C.c_double(0.0)
         sums = (C.c_double * 1)()
         raw = self.A_raw
         nbp = 0 if b_part is None else b_part.numel()
-        amg = self.amg_u1 if mode == 0 else self.amg_u2
-        if amg is not None:
+        if c["pc"] == "gamg":
+            A, amg = c["plain"], c["amg"]
             rc = L.smx_pcg_amg(self.N, A.width, A.ld, _capi.ptr(A.cols), _capi.ptr(A.vals), mode,
-                               _capi.ptr(b), _capi.ptr(b_part), nbp, 1, raw.width, raw.ld,
+                               _capi.ptr(b), _capi.ptr(b_part), nbp, int(project), raw.width, raw.ld,
                                _capi.ptr(raw.cols), _capi.ptr(raw.vals), _capi.ptr(self.bpos_cg),
                                _capi.ptr(gb), _capi.ptr(x), int(warm), _capi.ptr(self.cg_perm),
                                C.addressof(amg.c_levels), amg.nlev, amg.val_dtype, float(rtol),
                                float(self.ksp_atol), int(max_it), 8, int(expected),
                                _capi.ptr(ws), C.byref(its), C.byref(rn), sums, _capi.stream_ptr())
-            if rc == _capi.SMX_ERR_NOT_CONVERGED:
-                self.converged = False
-            else:
-                _capi.check(rc, "smx_pcg_amg")
-            return int(its.value)
-        rc = L.smx_pcg_jacobi(self.N, A.width, A.ld, _capi.ptr(A.cols), _capi.ptr(A.vals),
-                              _capi.ptr(dinv), mode, _capi.ptr(b), _capi.ptr(b_part), nbp, 1,
-                              raw.width, raw.ld, _capi.ptr(raw.cols), _capi.ptr(raw.vals),
-                              _capi.ptr(self.bpos_cg), _capi.ptr(gb), _capi.ptr(x), int(warm),
-                              _capi.ptr(self.cg_perm),
-                              float(rtol), float(self.ksp_atol), int(max_it), 16, int(expected),
-                              _capi.ptr(ws), C.byref(its), C.byref(rn), sums,
-                              _capi.stream_ptr())
+            what = "smx_pcg_amg"
+        else:
+            A = c["jacobi"]
+            rc = L.smx_pcg_jacobi(self.N, A.width, A.ld, _capi.ptr(A.cols), _capi.ptr(A.vals),
+                                  _capi.ptr(c["dinv"]), mode, _capi.ptr(b), _capi.ptr(b_part), nbp,
+                                  int(project), raw.width, raw.ld, _capi.ptr(raw.cols), _capi.ptr(raw.vals),
+                                  _capi.ptr(self.bpos_cg), _capi.ptr(gb), _capi.ptr(x), int(warm),
+                                  _capi.ptr(self.cg_perm), float(rtol), float(self.ksp_atol), int(max_it), 16,
+                                  int(expected), _capi.ptr(ws), C.byref(its), C.byref(rn), sums,
+                                  _capi.stream_ptr())
+            what = "smx_pcg_jacobi"
         if rc == _capi.SMX_ERR_NOT_CONVERGED:
             # The reference never checks the KSP converged reason; keep going.
             self.converged = False
         else:
-            _capi.check(rc, "smx_pcg_jacobi")
+            _capi.check(rc, what)
         return int(its.value)
 
     def boundary_matvec(self, src: torch.Tensor, y: torch.Tensor, gather=None, shift_ptr=None,
@@ -366,14 +452,12 @@ class DemagFieldLindholmGPU:
         _capi.require_device(m_vec, "m_vec", numel=self.local_size)
         warm = self.warm_start and self._have_u
         self.Div.spmv(m_vec, self.b_u1, kind=1, partials=self.b_part)
-        self.last_its_u1 = self._pcg(0, self.A_u1, self.dinv_u1, self.u1, self.ws1,
-                                     self.ksp_rtol_u1, self.ksp_max_it_u1, b=self.b_u1,
-                                     b_part=self.b_part, expected=self.last_its_u1, warm=warm)
+        self.last_its_u1 = self.solve("u1", 0, self.u1, self.ws1, b=self.b_u1, b_part=self.b_part,
+                                      expected=self.last_its_u1, warm=warm)
         self.boundary_matvec(self.u1, self.y_rows, gather=self.bverts_gpu,
                              shift_ptr=self.u1_sum_ptr, shift_scale=1.0 / self.N)
-        self.last_its_u2 = self._pcg(1, self.A_u2, self.dinv_u2, self.u2, self.ws2,
-                                     self.ksp_rtol_u2, self.ksp_max_it_u2, gb=self.y_rows,
-                                     expected=self.last_its_u2, warm=warm)
+        self.last_its_u2 = self.solve("u2", 1, self.u2, self.ws2, gb=self.y_rows, project=0,
+                                      expected=self.last_its_u2, warm=warm)
         self._have_u = True
 
     def field_args(self, m_vec, out_h_demag=None) -> _capi.FieldArgs:

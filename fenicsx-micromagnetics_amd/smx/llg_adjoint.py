@@ -148,9 +148,9 @@ class LLGAdjointModel:
                                               _capi.ptr(e.e), _capi.ptr(e.pT), st), "smx_energy_cell_forward")
         _capi.check(L.smx_energy_node_div(self.N, _capi.ptr(e.inc_ptr), _capi.ptr(e.inc), _capi.ptr(e.cgv),
                                           _capi.ptr(e.s), _capi.ptr(e.b), _capi.ptr(e.pN), st), "smx_energy_node_div")
-        e._solve(0, d.A_u1, d.dinv_u1, d.u1, d.ws1, b=e.b, b_part=e.pN)
+        e._solve("u1", 0, d.u1, d.ws1, b=e.b, b_part=e.pN)
         d.boundary_matvec(d.u1, d.y_rows, gather=d.bverts_gpu, shift_ptr=d.u1_sum_ptr, shift_scale=1.0 / self.N)
-        e._solve(1, d.A_u2, d.dinv_u2, d.u2, d.ws2, gb=d.y_rows)
+        e._solve("u2", 1, d.u2, d.ws2, gb=d.y_rows, project=0)
 
     # ------------------------------------------------------------------
     def rhs(self, m: torch.Tensor, Ms_cell: torch.Tensor, out: torch.Tensor) -> torch.Tensor:

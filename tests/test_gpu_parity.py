@@ -172,7 +172,7 @@ def _demag_pair(m, bd, B, **kw):
     return d, ref
 
 
-@pytest.mark.parametrize("pc", ["gamg", "jacobi"])
+@pytest.mark.parametrize("pc", ["gamg", "jacobi", "auto"])
 def test_demag_field_c1_fp64(c1, pc):
     torch = _torch()
     m, bd, B = c1
