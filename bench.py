@@ -151,9 +151,11 @@ def cg_iteration_timing(d, K=200):
     return per, nbytes
 
 
-def matvec_timing(d, flush_mb=512):
+def matvec_timing(d, flush_mb=512, reps=12):
+    """Boundary matvec: (avg s after a cache-evicting sweep, avg s warm,
+    algorithmic bytes, kernel names).  reps=0: bytes and names only."""
     import torch
-    flush = torch.ones(flush_mb * 1024 * 1024 // 4, dtype=torch.float32, device="cuda")
+    flush = torch.ones(flush_mb * 1024 * 1024 // 4, dtype=torch.float32, device="cuda") if reps else None
     x = torch.randn(d.Nb, dtype=torch.float64, device="cuda")
     y = torch.empty_like(x)
     if d.boundary_backend == "dense":
@@ -169,9 +171,11 @@ def matvec_timing(d, flush_mb=512):
         H = d.B_hmatrix_gpu
         fn = lambda: H.matvec(x, y)
         nbytes = H.algorithmic_bytes()
-        name = "hm_gather + hm_far_z_kernel + hm_leaf_kernel (H-matrix boundary matvec)"
-    cold = event_time(fn, 12, flush)
-    warm = event_time(fn, 12)
+        name = "hm_gather_kernel + hm_far_z_kernel + hm_leaf_kernel (H-matrix boundary matvec)"
+    if not reps:
+        return None, None, nbytes, name
+    cold = event_time(fn, reps, flush)
+    warm = event_time(fn, reps)
     del flush
     return cold, warm, nbytes, name
 
@@ -290,22 +294,48 @@ def adjoint_through_time(steps=500, dt=1e-13, every=50, n_fd=2):
     print(json.dumps(out), flush=True)
 
 
-def measured_traffic(config):
-    """HBM-side bytes per CG-iteration launch from the committed counter passes
-    (FETCH_SIZE and WRITE_SIZE of `bench.py --pmc-probe`, each scaled by the
-    same counter's reading for a 256 MiB device copy), or None."""
-    f = ROOT / "profiles" / "cg_iter_traffic.json"
+def measured_traffic(config, what="matvec"):
+    """HBM-side bytes per launch (or per boundary matvec) from the committed
+    counter passes (FETCH_SIZE and WRITE_SIZE of `bench.py --pmc-probe WHAT`,
+    each scaled by the same counter's reading for a 256 MiB device copy), or
+    None when no pass for this workload is committed."""
+    f = ROOT / "profiles" / ("matvec_traffic.json" if what == "matvec" else "cg_iter_traffic.json")
     if not f.exists():
         return None
     rec = json.loads(f.read_text()).get(config)
     return None if rec is None else int(rec["traffic_bytes_per_launch"])
 
 
+def pmc_probe_matvec(config, reps=20):
+    """Counter-pass program for the boundary matvec: the workload's operator is
+    built (mixed precision: FP32 factors / FP32 dense B), then one 256 MiB
+    device copy (the calibration kernel: 256 MiB read + 256 MiB written), then
+    `reps` boundary matvecs."""
+    import torch
+    dev = torch.device("cuda", 0)
+    mesh, mat, h64, hmx, pool, work, extra, setup_s = build_case(config, dev)
+    d = hmx.demag_field
+    src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    x = torch.randn(d.Nb, dtype=torch.float64, device=dev)
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        d.boundary_matvec(x, y)
+    torch.cuda.synchronize()
+    _, _, nbytes, name = matvec_timing(d, reps=0)
+    print(json.dumps({"pmc_probe": config, "what": "matvec", "kernels": name, "matvecs": reps,
+                      "algorithmic_bytes_per_matvec": int(nbytes),
+                      "calibration_copy_bytes": 2 * src.numel() * 4}), flush=True)
+
+
 def pmc_probe(config):
     """Short program for hardware-counter passes: one 256 MiB device copy (the
     calibration kernel of known traffic: 256 MiB read + 256 MiB written), then
-    200 CG iterations of the workload's u1 operator (the same ELL matrix and
-    launch shape as an evaluation; no boundary operator is built)."""
+    200 Jacobi CG iterations of the workload's u1 operator (the same ELL matrix
+    and launch shape as an evaluation; no boundary operator is built)."""
     import numpy as np
     import torch
     from smx import _capi
@@ -532,12 +562,18 @@ def main():
     ap.add_argument("--pc-u2", choices=["gamg", "jacobi", "auto"], default=None,
                     help="preconditioner of the u2 solve when it differs from --pc")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
-    ap.add_argument("--pmc-probe", action="store_true",
-                    help="short program for rocprofv3 --pmc passes (calibration copy + CG iterations)")
+    ap.add_argument("--pmc-probe", nargs="?", const="cg", default=None, choices=["cg", "matvec"],
+                    help="short program for rocprofv3 --pmc passes (calibration copy + CG iterations, or "
+                         "+ boundary matvecs)")
+    ap.add_argument("--evals-only", action="store_true",
+                    help="after set-up run only the mixed-precision evaluations (for kernel traces)")
     args = ap.parse_args()
     global PC_TYPE, PC_TYPE_U2
     PC_TYPE = args.pc
     PC_TYPE_U2 = args.pc_u2
+    if args.pmc_probe == "matvec":
+        pmc_probe_matvec(args.config)
+        return
     if args.pmc_probe:
         pmc_probe(args.config)
         return
@@ -569,6 +605,10 @@ def main():
         torch.cuda.synchronize()
         return time.perf_counter() - t0, its
 
+    if args.evals_only:
+        tmx, itsmx = run(hmx, args.steps, args.warmup)
+        log(f"mixed {args.steps / tmx:.2f} evals/s (evaluations only)")
+        return
     t64, _ = run(h64, args.steps, args.warmup)
     log(f"fp64 {args.steps / t64:.2f} evals/s")
     tmx, itsmx = run(hmx, args.steps, args.warmup)
@@ -609,18 +649,21 @@ def main():
         "cg_iterations_u1_u2": [float(np.mean([a for a, _ in itsmx])), float(np.mean([b for _, b in itsmx]))],
         "pc": [hmx.demag_field.pc_type, hmx.demag_field.pc_type_u2],
         "amg_levels_u1": hmx.demag_field.amg_u1.describe() if hmx.demag_field.amg_u1 else None,
-        "roofline": {"bound": "hbm", "kernel": ("AMG-CG iteration (u1: update + V-cycle + A u launches)"
-                                                if hmx.demag_field.pc_type == "gamg"
-                                                else "cg_iter_kernel (Jacobi CG iteration, u1 operator)"),
-                     "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                     "traffic": (measured_traffic(args.config) if hmx.demag_field.pc_type == "jacobi" else None),
-                     "algorithmic_bytes": int(cg_bytes),
-                     "avg_us": cg_t * 1e6},
-        "matvec_roofline": {"bound": "hbm", "kernel": mv_name, "achieved": mv_ach, "peak": HBM_PEAK_GBS,
-                            "unit": "GB/s", "frac": mv_ach / HBM_PEAK_GBS, "algorithmic_bytes": int(mv_bytes),
-                            "avg_us_flushed": mv_cold * 1e6, "avg_us_cache_warm": mv_warm * 1e6,
-                            "fp64_factors_GBs": mv64_bytes / mv64_cold / 1e9,
-                            "fp64_factors_avg_us_flushed": mv64_cold * 1e6},
+        "roofline": {"bound": "hbm", "kernel": mv_name, "achieved": mv_ach, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": mv_ach / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(args.config, "matvec"), "algorithmic_bytes": int(mv_bytes),
+                     "avg_us": mv_cold * 1e6, "avg_us_cache_warm": mv_warm * 1e6,
+                     "note": "mixed precision boundary operator; timed after a 512 MB read sweep that evicts the "
+                             "Infinity Cache",
+                     "fp64_operator_GBs": mv64_bytes / mv64_cold / 1e9,
+                     "fp64_operator_avg_us": mv64_cold * 1e6},
+        "cg_roofline": {"bound": "hbm", "kernel": ("AMG-CG iteration of the u1 solve (update, V-cycle and A u "
+                                                   "launches)" if hmx.demag_field.pc_type == "gamg"
+                                                   else "cg_iter_kernel (Jacobi CG iteration, u1 operator)"),
+                        "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                        "traffic": (measured_traffic(args.config, "cg") if hmx.demag_field.pc_type == "jacobi"
+                                    else None),
+                        "algorithmic_bytes": int(cg_bytes), "avg_us": cg_t * 1e6},
     }
     if h64.demag_field.boundary_backend == "dense":
         res["multi_rhs_boundary_product"] = multi_rhs_timing(h64.demag_field)
