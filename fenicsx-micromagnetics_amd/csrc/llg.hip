@@ -23,6 +23,62 @@ namespace {
 
 constexpr int kBlock = 256;
 
+__device__ __forceinline__ void cross3(double ax, double ay, double az, double bx, double by,
+                                       double bz, double& cx, double& cy, double& cz) {
+  cx = ay * bz - az * by;
+  cy = az * bx - ax * bz;
+  cz = ax * by - ay * bx;
+}
+
+// Cubic axes of node k (9 doubles: c1, c2, c3).
+__device__ __forceinline__ void cubic_axes(const smx_field_args& a, int64_t k, double (&c)[9]) {
+#pragma unroll
+  for (int i = 0; i < 9; ++i) c[i] = a.cub_axes ? a.cub_axes[9 * k + i] : a.cub_axes_u[i];
+}
+
+// H_cub = pref (s1 c1 + s2 c2 + s3 c3), a_i = m.c_i, s1 = a1 (a2^2 + a3^2)
+// (cyclic); with v != NULL the derivative dH_cub/dm v instead.
+__device__ __forceinline__ void cubic_term(const smx_field_args& a, int64_t k, double mx, double my,
+                                           double mz, const double* v, double& hx, double& hy,
+                                           double& hz) {
+  double c[9];
+  cubic_axes(a, k, c);
+  const double a1 = mx * c[0] + my * c[1] + mz * c[2];
+  const double a2 = mx * c[3] + my * c[4] + mz * c[5];
+  const double a3 = mx * c[6] + my * c[7] + mz * c[8];
+  double s1, s2, s3;
+  if (!v) {
+    s1 = a1 * (a2 * a2 + a3 * a3);
+    s2 = a2 * (a3 * a3 + a1 * a1);
+    s3 = a3 * (a1 * a1 + a2 * a2);
+  } else {
+    const double vx = v[0], vy = v[1], vz = v[2];
+    const double d1 = vx * c[0] + vy * c[1] + vz * c[2];
+    const double d2 = vx * c[3] + vy * c[4] + vz * c[5];
+    const double d3 = vx * c[6] + vy * c[7] + vz * c[8];
+    s1 = d1 * (a2 * a2 + a3 * a3) + 2.0 * a1 * (a2 * d2 + a3 * d3);
+    s2 = d2 * (a3 * a3 + a1 * a1) + 2.0 * a2 * (a3 * d3 + a1 * d1);
+    s3 = d3 * (a1 * a1 + a2 * a2) + 2.0 * a3 * (a1 * d1 + a2 * d2);
+  }
+  const double p = a.cub_pref;
+  hx += p * (s1 * c[0] + s2 * c[3] + s3 * c[6]);
+  hy += p * (s1 * c[1] + s2 * c[4] + s3 * c[7]);
+  hz += p * (s1 * c[2] + s2 * c[5] + s3 * c[8]);
+}
+
+__device__ __forceinline__ void sot_p(const smx_field_args& a, int64_t k, double& px, double& py,
+                                      double& pz) {
+  if (a.sot_p) {
+    px = a.sot_p[3 * k];
+    py = a.sot_p[3 * k + 1];
+    pz = a.sot_p[3 * k + 2];
+  } else {
+    px = a.sot_p_u[0];
+    py = a.sot_p_u[1];
+    pz = a.sot_p_u[2];
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void field_rhs_kernel(smx_field_args a) {
   const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (k >= a.n) return;
@@ -31,9 +87,11 @@ __global__ __launch_bounds__(kBlock) void field_rhs_kernel(smx_field_args a) {
   const int64_t start = a.slice_ptr[s];
   const int64_t w = (a.slice_ptr[s + 1] - start) >> 6;
   const double shift = a.u1_sum ? (*a.u1_sum) / (double)a.n : 0.0;
-  const bool need_m = a.ex_vals || a.dmi_vals;
+  const bool stt = a.stt_vals && a.stt_pref != 0.0 && a.rhs;
+  const bool need_m = a.ex_vals || a.dmi_vals || stt;
   double hx = 0.0, hy = 0.0, hz = 0.0;
   double gx = 0.0, gy = 0.0, gz = 0.0;
+  double zx = 0.0, zy = 0.0, zz = 0.0;
   for (int64_t j = 0; j < w; ++j) {
     const int64_t e = start + j * 64 + lane;
     const int64_t c = a.cols[e];
@@ -44,6 +102,12 @@ __global__ __launch_bounds__(kBlock) void field_rhs_kernel(smx_field_args a) {
         hx = fma(sv, m0, hx);
         hy = fma(sv, m1, hy);
         hz = fma(sv, m2, hz);
+      }
+      if (stt) {
+        const double tv = a.stt_vals[e];
+        zx = fma(tv, m0, zx);
+        zy = fma(tv, m1, zy);
+        zz = fma(tv, m2, zz);
       }
       if (a.dmi_vals) {
         const double* D = a.dmi_vals + 9 * e;
@@ -76,6 +140,7 @@ __global__ __launch_bounds__(kBlock) void field_rhs_kernel(smx_field_args a) {
     hy += cdot * ny;
     hz += cdot * nz;
   }
+  if (a.cub_pref != 0.0) cubic_term(a, k, mx, my, mz, nullptr, hx, hy, hz);
   if (a.grad_vals) {
     const double iv = a.inv_vol[k];
     const double dx = gx * iv, dy = gy * iv, dz = gz * iv;
@@ -110,9 +175,29 @@ __global__ __launch_bounds__(kBlock) void field_rhs_kernel(smx_field_args a) {
     const double mcmy = mz * mcx - mx * mcz;
     const double mcmz = mx * mcy - my * mcx;
     const double stab = a.stab * (1.0 - (mx * mx + my * my + mz * mz));
-    a.rhs[3 * k] = a.P * (a.do_precess * mcx + a.alpha * mcmx) + stab * mx;
-    a.rhs[3 * k + 1] = a.P * (a.do_precess * mcy + a.alpha * mcmy) + stab * my;
-    a.rhs[3 * k + 2] = a.P * (a.do_precess * mcz + a.alpha * mcmz) + stab * mz;
+    double rx = a.P * (a.do_precess * mcx + a.alpha * mcmx) + stab * mx;
+    double ry = a.P * (a.do_precess * mcy + a.alpha * mcmy) + stab * my;
+    double rz = a.P * (a.do_precess * mcz + a.alpha * mcmz) + stab * mz;
+    if (stt) {  // - pref (a m x Z + b m x (m x Z))
+      double c1x, c1y, c1z, c2x, c2y, c2z;
+      cross3(mx, my, mz, zx, zy, zz, c1x, c1y, c1z);
+      cross3(mx, my, mz, c1x, c1y, c1z, c2x, c2y, c2z);
+      rx -= a.stt_pref * (a.stt_a * c1x + a.stt_b * c2x);
+      ry -= a.stt_pref * (a.stt_a * c1y + a.stt_b * c2y);
+      rz -= a.stt_pref * (a.stt_a * c1z + a.stt_b * c2z);
+    }
+    if (a.sot_fl != 0.0 || a.sot_dl != 0.0) {  // c_fl m x p + c_dl m x (m x p)
+      double px, py, pz, c1x, c1y, c1z, c2x, c2y, c2z;
+      sot_p(a, k, px, py, pz);
+      cross3(mx, my, mz, px, py, pz, c1x, c1y, c1z);
+      cross3(mx, my, mz, c1x, c1y, c1z, c2x, c2y, c2z);
+      rx += a.sot_fl * c1x + a.sot_dl * c2x;
+      ry += a.sot_fl * c1y + a.sot_dl * c2y;
+      rz += a.sot_fl * c1z + a.sot_dl * c2z;
+    }
+    a.rhs[3 * k] = rx;
+    a.rhs[3 * k + 1] = ry;
+    a.rhs[3 * k + 2] = rz;
   }
 }
 
@@ -128,7 +213,9 @@ __global__ __launch_bounds__(kBlock) void llg_jvp_kernel(smx_field_args a,
   const int64_t start = a.slice_ptr[s];
   const int64_t w = (a.slice_ptr[s + 1] - start) >> 6;
   double hvx = 0.0, hvy = 0.0, hvz = 0.0;
-  if (a.ex_vals || a.dmi_vals) {
+  const bool stt = a.stt_vals && a.stt_pref != 0.0;
+  double zmx = 0.0, zmy = 0.0, zmz = 0.0, zvx = 0.0, zvy = 0.0, zvz = 0.0;
+  if (a.ex_vals || a.dmi_vals || stt) {
     for (int64_t j = 0; j < w; ++j) {
       const int64_t e = start + j * 64 + lane;
       const int64_t c = a.cols[e];
@@ -138,6 +225,15 @@ __global__ __launch_bounds__(kBlock) void llg_jvp_kernel(smx_field_args a,
         hvx = fma(sv, v0, hvx);
         hvy = fma(sv, v1, hvy);
         hvz = fma(sv, v2, hvz);
+      }
+      if (stt) {
+        const double tv = a.stt_vals[e];
+        zvx = fma(tv, v0, zvx);
+        zvy = fma(tv, v1, zvy);
+        zvz = fma(tv, v2, zvz);
+        zmx = fma(tv, a.m[3 * c], zmx);
+        zmy = fma(tv, a.m[3 * c + 1], zmy);
+        zmz = fma(tv, a.m[3 * c + 2], zmz);
       }
       if (a.dmi_vals) {
         const double* D = a.dmi_vals + 9 * e;
@@ -165,6 +261,10 @@ __global__ __launch_bounds__(kBlock) void llg_jvp_kernel(smx_field_args a,
     hvz += cdot * nz;
   }
   const double mx = a.m[3 * k], my = a.m[3 * k + 1], mz = a.m[3 * k + 2];
+  if (a.cub_pref != 0.0) {
+    const double vk[3] = {vx, vy, vz};
+    cubic_term(a, k, mx, my, mz, vk, hvx, hvy, hvz);
+  }
   const double hx = hm[3 * k], hy = hm[3 * k + 1], hz = hm[3 * k + 2];
   const double cvx = vy * hz - vz * hy, cvy = vz * hx - vx * hz, cvz = vx * hy - vy * hx;
   const double cmx = my * hvz - mz * hvy, cmy = mz * hvx - mx * hvz, cmz = mx * hvy - my * hvx;
@@ -179,12 +279,128 @@ __global__ __launch_bounds__(kBlock) void llg_jvp_kernel(smx_field_args a,
   const double dampz = vz * mdHm - 2.0 * hz * mdv + mz * common - hvz * mdmm;
   const double coef = -gamma / (1.0 + a.alpha * a.alpha);
   const double sc = a.stab * (1.0 - mdmm);
-  out[3 * k] = coef * (a.do_precess * (cvx + cmx) + a.alpha * dampx) + sc * vx -
-               2.0 * a.stab * mx * mdv;
-  out[3 * k + 1] = coef * (a.do_precess * (cvy + cmy) + a.alpha * dampy) + sc * vy -
-                   2.0 * a.stab * my * mdv;
-  out[3 * k + 2] = coef * (a.do_precess * (cvz + cmz) + a.alpha * dampz) + sc * vz -
-                   2.0 * a.stab * mz * mdv;
+  double ox = coef * (a.do_precess * (cvx + cmx) + a.alpha * dampx) + sc * vx -
+              2.0 * a.stab * mx * mdv;
+  double oy = coef * (a.do_precess * (cvy + cmy) + a.alpha * dampy) + sc * vy -
+              2.0 * a.stab * my * mdv;
+  double oz = coef * (a.do_precess * (cvz + cmz) + a.alpha * dampz) + sc * vz -
+              2.0 * a.stab * mz * mdv;
+  if (stt) {
+    // d[m x Z] = v x Z + m x Zv;
+    // d[m x (m x Z)] = v (m.Z) + m (v.Z + m.Zv) - Zv |m|^2 - 2 Z (m.v)
+    double p1x, p1y, p1z, p2x, p2y, p2z;
+    cross3(vx, vy, vz, zmx, zmy, zmz, p1x, p1y, p1z);
+    cross3(mx, my, mz, zvx, zvy, zvz, p2x, p2y, p2z);
+    const double mZ = mx * zmx + my * zmy + mz * zmz;
+    const double vZ = vx * zmx + vy * zmy + vz * zmz;
+    const double mZv = mx * zvx + my * zvy + mz * zvz;
+    const double t = vZ + mZv;
+    const double dx = vx * mZ + mx * t - zvx * mdmm - 2.0 * zmx * mdv;
+    const double dy = vy * mZ + my * t - zvy * mdmm - 2.0 * zmy * mdv;
+    const double dz = vz * mZ + mz * t - zvz * mdmm - 2.0 * zmz * mdv;
+    ox -= a.stt_pref * (a.stt_a * (p1x + p2x) + a.stt_b * dx);
+    oy -= a.stt_pref * (a.stt_a * (p1y + p2y) + a.stt_b * dy);
+    oz -= a.stt_pref * (a.stt_a * (p1z + p2z) + a.stt_b * dz);
+  }
+  if (a.sot_fl != 0.0 || a.sot_dl != 0.0) {
+    // d[m x p] = v x p;  d[m x (m x p)] = v x (m x p) + m x (v x p)
+    double px, py, pz, cx, cy, cz, dcx, dcy, dcz, ex, ey, ez, fx, fy, fz;
+    sot_p(a, k, px, py, pz);
+    cross3(mx, my, mz, px, py, pz, cx, cy, cz);
+    cross3(vx, vy, vz, px, py, pz, dcx, dcy, dcz);
+    cross3(vx, vy, vz, cx, cy, cz, ex, ey, ez);
+    cross3(mx, my, mz, dcx, dcy, dcz, fx, fy, fz);
+    ox += a.sot_fl * dcx + a.sot_dl * (ex + fx);
+    oy += a.sot_fl * dcy + a.sot_dl * (ey + fy);
+    oz += a.sot_fl * dcz + a.sot_dl * (ez + fz);
+  }
+  out[3 * k] = ox;
+  out[3 * k + 1] = oy;
+  out[3 * k + 2] = oz;
+}
+
+__global__ __launch_bounds__(kBlock) void projdir_kernel(int64_t n, const double* __restrict__ m,
+                                                         const double* __restrict__ h,
+                                                         double* __restrict__ d) {
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (k >= n) return;
+  const double mx = m[3 * k], my = m[3 * k + 1], mz = m[3 * k + 2];
+  const double hx = h[3 * k], hy = h[3 * k + 1], hz = h[3 * k + 2];
+  const double mh = mx * hx + my * hy + mz * hz;
+  d[3 * k] = hx - mh * mx;
+  d[3 * k + 1] = hy - mh * my;
+  d[3 * k + 2] = hz - mh * mz;
+}
+
+// y = (I - C)^-1 (I + C) m, C v = c x v, c = (step/2) m x d:
+// b = m + c x m, y = (b + c x b + c (c.b)) / (1 + |c|^2).
+__global__ __launch_bounds__(kBlock) void cayley_kernel(int64_t n, const double* __restrict__ m,
+                                                        const double* __restrict__ d, double step,
+                                                        double* __restrict__ y) {
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (k >= n) return;
+  const double mx = m[3 * k], my = m[3 * k + 1], mz = m[3 * k + 2];
+  double cx, cy, cz;
+  cross3(mx, my, mz, d[3 * k], d[3 * k + 1], d[3 * k + 2], cx, cy, cz);
+  cx *= 0.5 * step;
+  cy *= 0.5 * step;
+  cz *= 0.5 * step;
+  double tx, ty, tz;
+  cross3(cx, cy, cz, mx, my, mz, tx, ty, tz);
+  const double bx = mx + tx, by = my + ty, bz = mz + tz;
+  double ux, uy, uz;
+  cross3(cx, cy, cz, bx, by, bz, ux, uy, uz);
+  const double cb = cx * bx + cy * by + cz * bz;
+  const double inv = 1.0 / (1.0 + (cx * cx + cy * cy + cz * cz));
+  y[3 * k] = (bx + ux + cx * cb) * inv;
+  y[3 * k + 1] = (by + uy + cy * cb) * inv;
+  y[3 * k + 2] = (bz + uz + cz * cb) * inv;
+}
+
+__global__ __launch_bounds__(kBlock) void normalize3_kernel(int64_t n, double* __restrict__ m) {
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (k >= n) return;
+  const double x = m[3 * k], y = m[3 * k + 1], z = m[3 * k + 2];
+  const double r = sqrt(x * x + y * y + z * z);
+  const double s = r > 1e-300 ? 1.0 / r : 1.0;
+  m[3 * k] = x * s;
+  m[3 * k + 1] = y * s;
+  m[3 * k + 2] = z * s;
+}
+
+// Per-block maxima of |v_i| into part[block]; the second kernel takes the max
+// over the blocks (fixed, order-independent result).
+__global__ __launch_bounds__(kBlock) void max_norm3_kernel(int64_t n, const double* __restrict__ v,
+                                                           double* __restrict__ part) {
+  __shared__ double sm[kBlock / 64];
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  double r = 0.0;
+  if (k < n) r = sqrt(v[3 * k] * v[3 * k] + v[3 * k + 1] * v[3 * k + 1] + v[3 * k + 2] * v[3 * k + 2]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) r = fmax(r, __shfl_xor(r, off, 64));
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = r;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = sm[0];
+    for (int i = 1; i < kBlock / 64; ++i) b = fmax(b, sm[i]);
+    part[blockIdx.x] = b;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void max_reduce_kernel(int64_t g, const double* __restrict__ part,
+                                                            double* __restrict__ out) {
+  __shared__ double sm[kBlock / 64];
+  double r = 0.0;
+  for (int64_t i = threadIdx.x; i < g; i += kBlock) r = fmax(r, part[i]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) r = fmax(r, __shfl_xor(r, off, 64));
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = r;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = sm[0];
+    for (int i = 1; i < kBlock / 64; ++i) b = fmax(b, sm[i]);
+    *out = b;
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void dot3_kernel(int64_t n, const double* __restrict__ a,
@@ -273,6 +489,53 @@ extern "C" int smx_llg_jvp(const smx_field_args* a, const double* hm, const doub
   if (a->n == 0) return SMX_OK;
   hipLaunchKernelGGL(llg_jvp_kernel, dim3(smx::grid_for(a->n, kBlock)), dim3(kBlock), 0,
                      smx::as_stream(stream), *a, hm, v, out, gamma);
+  SMX_LAUNCH_CHECK();
+  return SMX_OK;
+}
+
+extern "C" int smx_projected_direction(int64_t n, const double* m, const double* h, double* d,
+                                       void* stream) {
+  SMX_REQUIRE(n >= 0, SMX_ERR_SHAPE, "negative n");
+  SMX_REQUIRE(m && h && d, SMX_ERR_VALUE, "null pointer argument");
+  if (n == 0) return SMX_OK;
+  hipLaunchKernelGGL(projdir_kernel, dim3(smx::grid_for(n, kBlock)), dim3(kBlock), 0,
+                     smx::as_stream(stream), n, m, h, d);
+  SMX_LAUNCH_CHECK();
+  return SMX_OK;
+}
+
+extern "C" int smx_cayley_step(int64_t n, const double* m, const double* d, double step, double* out,
+                               void* stream) {
+  SMX_REQUIRE(n >= 0, SMX_ERR_SHAPE, "negative n");
+  SMX_REQUIRE(m && d && out, SMX_ERR_VALUE, "null pointer argument");
+  if (n == 0) return SMX_OK;
+  hipLaunchKernelGGL(cayley_kernel, dim3(smx::grid_for(n, kBlock)), dim3(kBlock), 0,
+                     smx::as_stream(stream), n, m, d, step, out);
+  SMX_LAUNCH_CHECK();
+  return SMX_OK;
+}
+
+extern "C" int smx_normalize3(int64_t n, double* m, void* stream) {
+  SMX_REQUIRE(n >= 0, SMX_ERR_SHAPE, "negative n");
+  SMX_REQUIRE(m, SMX_ERR_VALUE, "null pointer argument");
+  if (n == 0) return SMX_OK;
+  hipLaunchKernelGGL(normalize3_kernel, dim3(smx::grid_for(n, kBlock)), dim3(kBlock), 0,
+                     smx::as_stream(stream), n, m);
+  SMX_LAUNCH_CHECK();
+  return SMX_OK;
+}
+
+extern "C" int smx_max_norm3(int64_t n, const double* v, double* out, double* red, void* stream) {
+  SMX_REQUIRE(n >= 0, SMX_ERR_SHAPE, "negative n");
+  SMX_REQUIRE(v && out && red, SMX_ERR_VALUE, "null pointer argument");
+  hipStream_t st = smx::as_stream(stream);
+  const int g = smx::grid_for(n, kBlock);
+  if (n > 0) {
+    hipLaunchKernelGGL(max_norm3_kernel, dim3(g), dim3(kBlock), 0, st, n, v, red + 8);
+    SMX_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(max_reduce_kernel, dim3(1), dim3(kBlock), 0, st, n > 0 ? (int64_t)g : 0,
+                     red + 8, out);
   SMX_LAUNCH_CHECK();
   return SMX_OK;
 }

@@ -58,6 +58,9 @@ class FieldArgs(C.Structure):
         ("h0", P), ("h0_u", F64 * 3),
         ("h_eff", P), ("h_demag", P), ("rhs", P),
         ("P", F64), ("alpha", F64), ("do_precess", F64), ("stab", F64),
+        ("cub_pref", F64), ("cub_axes", P), ("cub_axes_u", F64 * 9),
+        ("stt_vals", P), ("stt_pref", F64), ("stt_a", F64), ("stt_b", F64),
+        ("sot_p", P), ("sot_p_u", F64 * 3), ("sot_fl", F64), ("sot_dl", F64),
     ]
 
 
@@ -101,6 +104,10 @@ _SIGS = {
     "smx_field_llg_rhs": ([C.POINTER(FieldArgs), P], C.c_int),
     "smx_llg_jvp": ([C.POINTER(FieldArgs), P, P, P, F64, P], C.c_int),
     "smx_dot3_weighted": ([I64, P, P, P, P, P, P], C.c_int),
+    "smx_projected_direction": ([I64, P, P, P, P], C.c_int),
+    "smx_cayley_step": ([I64, P, P, F64, P, P], C.c_int),
+    "smx_max_norm3": ([I64, P, P, P, P], C.c_int),
+    "smx_normalize3": ([I64, P, P], C.c_int),
     "smx_rk_combine": ([I64, P, F64, I32, P, P, P, F64, F64, P, P, P, P], C.c_int),
     "smx_energy_blocks": ([I64], I64),
     "smx_energy_cell_forward": ([I64, P, P, P, P, P, P, P, P, P, P], C.c_int),

@@ -284,3 +284,25 @@ def dmi_bulk_values(X, tets, fem: FEMSystem, D: float, Ms: float) -> np.ndarray:
     r = np.repeat(np.arange(N), np.diff(fem.graph.indptr))
     pref = 2.0 * D / (MU0 * Ms) * 1e9 / fem.volume
     return pref[r][:, None, None] * vals
+
+
+def zhang_li_values(fem: FEMSystem, tets: np.ndarray, Jdir) -> np.ndarray:
+    """Zhang-Li operator K_J[a, b] = int (J . grad phi_b) phi_a dx with its rows
+    scaled by 1/V_a (llg_stt_module_GPU.py:393-435: the UFL form
+    inner(dot(grad(u), J), v) dx, then diagonalScale(1/VolN, None)), on the node
+    graph.  J is one 3-vector or one per node (P1 interpolated); for P1 J the
+    integral is exact: int phi_c phi_a = |T| (1 + delta_ca) / 20, so
+    K_ab = |T|/20 (sum_c J_c + J_a) . grad phi_b."""
+    tets = np.asarray(tets, dtype=np.int64)
+    J = np.asarray(Jdir, dtype=np.float64).reshape(-1, 3)
+    if J.shape[0] == 1:
+        Jt = np.broadcast_to(J[0], (tets.shape[0], 4, 3))
+    elif J.shape[0] == fem.N:
+        Jt = J[tets]
+    else:
+        raise ValueError("Jdir_vec must hold one vector or one vector per node")
+    w = (fem.elem_vol / 20.0)[:, None, None] * (Jt.sum(axis=1)[:, None, :] + Jt)      # (T, a, 3)
+    elem = np.einsum("tak,tbk->tab", w, fem.elem_grad)                                  # (T, a, b)
+    vals = fem.graph.scatter(elem[..., None])[:, 0]
+    r = np.repeat(np.arange(fem.N), np.diff(fem.graph.indptr))
+    return vals / fem.volume[r]

@@ -24,20 +24,51 @@ import torch
 from . import _capi
 from .demag import DemagFieldLindholmGPU, OperatorSet
 from .device import default_device, reduce_workspace, to_dev
-from .fem import dmi_bulk_values, dmi_interfacial_values, exchange_values
+from .fem import dmi_bulk_values, dmi_interfacial_values, exchange_values, zhang_li_values
 from .mesh import TetMesh
 
 MU0 = 4.0 * math.pi * 1e-7
 
 
+E_CHARGE = 1.6021766e-19      # as llg_stt_module_GPU.py:569-570
+MU_B = 9.27400915e-24
+
+
+def cubic_axes(u1, u2, n: int) -> np.ndarray:
+    """Orthonormal cubic axes (c1, c2, c3) per node, (N, 9) or (1, 9) for
+    uniform axes: c1 = u1/|u1|, c2 = Gram-Schmidt of u2 against c1, c3 = c1 x c2
+    (Cubic_Anisotropy_GPU.py:179-190)."""
+    a = np.asarray(u1, dtype=np.float64).reshape(-1, 3).copy()
+    b = np.asarray(u2, dtype=np.float64).reshape(-1, 3).copy()
+    if a.shape[0] != b.shape[0] or a.shape[0] not in (1, n):
+        raise ValueError("u1_cub / u2_cub must be one axis or one axis per node each")
+
+    def unit(v):
+        nv = np.linalg.norm(v, axis=1)
+        return v / np.where(nv > 1e-30, nv, 1.0)[:, None]
+
+    a = unit(a)
+    b = unit(b - np.einsum("ij,ij->i", b, a)[:, None] * a)
+    c = unit(np.cross(a, b))
+    return np.concatenate([a, b, c], axis=1)
+
+
 class EffectiveField:
+    """Effective field and LLG right-hand side of one mesh.
+
+    Terms (each optional): exchange, uniaxial and cubic anisotropy, bulk and
+    interfacial DMI, FEM/BEM demag, static/time-dependent external field, and
+    in the RHS the Zhang-Li spin-transfer torque (``Jmagnitude``, ``Jdir_vec``,
+    ``P``, ``beta`` as EffectiveFieldSTTGPU, llg_stt_module_GPU.py:522-582) and
+    the spin-orbit torque (``H_dl``, ``H_fl``, ``polarization_vec`` as
+    EffectiveFieldSOTGPU, llg_SOT_module_GPU.py:545-817)."""
+
     def __init__(self, mesh: TetMesh, Ms, Aex, Ku=0.0, n_ani_vec=None, D_bulk=0.0, D_int=0.0,
                  n0_int_vec=None, Kc1=0.0, u1_cub=None, u2_cub=None, gamma=2.211e5, alpha=0.5,
                  do_precess=1, use_demag=False, demag_method="lindholm", demag_kwargs=None,
-                 H0_static=None, H_time_func=None, *, device=None,
+                 H0_static=None, H_time_func=None, *, Jmagnitude=0.0, Jdir_vec=None, P=0.0, beta=0.0,
+                 H_dl=0.0, H_fl=0.0, polarization_vec=None, device=None,
                  operators: OperatorSet | None = None):
-        if abs(float(Kc1)) > 0.0:
-            raise NotImplementedError("cubic anisotropy is not part of this path")
         self.mesh = mesh
         self.device = device or default_device()
         self.ops = operators or OperatorSet(mesh, self.device)
@@ -95,8 +126,45 @@ class EffectiveField:
                     raise ValueError("n_ani_vec must hold one axis or one axis per node")
                 self.ani_axis_gpu = to_dev(na.ravel(), dev)
 
-        if not (self.ex or self.dmi or self.ani_pref or self.use_demag):
-            raise ValueError("There are no active GPU terms. You must add exchange, anisotropy, DMI, or demag.")
+        # cubic anisotropy (Cubic_Anisotropy_GPU.py:33-110): H = 2 Kc1/(mu0 Ms) sum s_i c_i
+        self.Kc1 = float(Kc1)
+        self.cub_pref = 0.0
+        self.cub_axes_gpu = None
+        self.cub_axes_u = (0.0,) * 9
+        if abs(self.Kc1) > 0.0:
+            if u1_cub is None or u2_cub is None:
+                raise ValueError("Cubic anisotropy requires u1_cub and u2_cub.")
+            ax = cubic_axes(u1_cub, u2_cub, self.N)
+            self.cub_pref = 2.0 * self.Kc1 / (MU0 * self.Ms) if self.Ms != 0.0 else 0.0
+            if ax.shape[0] == 1:
+                self.cub_axes_u = tuple(float(v) for v in ax[0])
+            else:
+                self.cub_axes_gpu = to_dev(ax.ravel(), dev)
+
+        # Zhang-Li spin-transfer torque: Z = K_J m / V (llg_stt_module_GPU.py:393-435)
+        self.stt = None
+        self.P = float(P)
+        self.beta = float(beta)
+        self.Jmagnitude = float(Jmagnitude)
+        self.prefZhang = 0.0
+        if self.Jmagnitude != 0.0 and self.P != 0.0:
+            if Jdir_vec is None:
+                raise ValueError("Jdir_vec is required with a current")
+            self.stt = self.ops.graph_values(zhang_li_values(fem, mesh.tets, Jdir_vec), 1)
+            self.prefZhang = (self.Jmagnitude * self.P * MU_B / (E_CHARGE * self.Ms * (1.0 + self.beta ** 2))
+                              / (1.0 + self.alpha ** 2) / 1e-9)
+
+        # spin-orbit torque (llg_SOT_module_GPU.py:813-817)
+        self.H_dl = float(H_dl)
+        self.H_fl = float(H_fl)
+        self.sot_p_gpu = None
+        self.sot_p_u = (0.0, 1.0, 0.0)
+        self.c_fl = self.c_dl = 0.0
+        self.set_sot(polarization_vec=polarization_vec if polarization_vec is not None else (0.0, 1.0, 0.0))
+
+        if not (self.ex or self.dmi or self.ani_pref or self.use_demag or self.cub_pref):
+            raise ValueError("There are no active GPU terms. You must add exchange, anisotropy, DMI, "
+                             "cubic anisotropy, or demag.")
 
         z = lambda n: torch.zeros(n, dtype=torch.float64, device=dev)
         self.m_gpu = z(self.local_size)
@@ -159,7 +227,40 @@ class EffectiveField:
         a.alpha = self.alpha
         a.do_precess = self.do_precess
         a.stab = self.Stab
+        a.cub_pref = self.cub_pref
+        a.cub_axes = _capi.ptr(self.cub_axes_gpu)
+        a.cub_axes_u = (C.c_double * 9)(*self.cub_axes_u)
+        if self.stt is not None:
+            a.stt_vals = _capi.ptr(self.stt.vals)
+            a.stt_pref = self.prefZhang
+            a.stt_a = self.beta - self.alpha
+            a.stt_b = 1.0 + self.alpha * self.beta
+        a.sot_p = _capi.ptr(self.sot_p_gpu)
+        a.sot_p_u = (C.c_double * 3)(*self.sot_p_u)
+        a.sot_fl = self.c_fl
+        a.sot_dl = self.c_dl
         return a
+
+    def set_sot(self, H_dl=None, H_fl=None, polarization_vec=None):
+        """Spin-orbit torque amplitudes (A/m) and unit polarization, one vector
+        or one per node (llg_SOT_module_GPU.py:855-862, 813-817)."""
+        if H_dl is not None:
+            self.H_dl = float(H_dl)
+        if H_fl is not None:
+            self.H_fl = float(H_fl)
+        if polarization_vec is not None:
+            p = np.asarray(polarization_vec, dtype=np.float64).reshape(-1, 3)
+            p = p / np.maximum(np.linalg.norm(p, axis=1), 1e-300)[:, None]
+            if p.shape[0] == 1:
+                self.sot_p_u = tuple(float(v) for v in p[0])
+                self.sot_p_gpu = None
+            elif p.shape[0] == self.N:
+                self.sot_p_gpu = to_dev(p.ravel(), self.device)
+            else:
+                raise ValueError("polarization_vec must hold one vector or one per node")
+        den = 1.0 + self.alpha * self.alpha
+        self.c_fl = -self.gamma * (self.H_fl - self.alpha * self.H_dl) / den
+        self.c_dl = -self.gamma * (self.H_dl + self.alpha * self.H_fl) / den
 
     def _h_ext_uniform(self):
         if self.H_time_func is None:
@@ -243,36 +344,96 @@ class EffectiveField:
         """factor * mu0 Ms sum V_i m_i.H_i * 1e-27 (:1014-1051)."""
         return factor * self.mu0 * self.Ms * self._dot_weighted(m_vec, H_vec) * self.volume_scale_energy
 
+    def _term_field(self, m, h, lin=False, ani=False, cub=False):
+        """h = the chosen local terms of H(m) (fused kernel, other terms off)."""
+        a = self._args(m, h_eff=h, with_demag=False, with_h0=False)
+        if not lin:
+            a.ex_vals = None
+            a.dmi_vals = None
+        if not ani:
+            a.ani_pref = 0.0
+        if not cub:
+            a.cub_pref = 0.0
+        self._launch(a)
+
     def compute_Energy_terms(self, m_vec=None) -> dict:
+        """Lumped energies (J) of the active terms (llg_module_GPU.py:1014-1221):
+        E_lin = -1/2 mu0 Ms sum V m.H_lin, E_ani = -Ku sum V (m.n)^2
+        = -1/2 mu0 Ms sum V m.H_ani, E_cubic = -Kc1 sum V S = -1/4 mu0 Ms sum V
+        m.H_cub (m.H_cub = 2 pref S), E_demag, E_zeeman = -mu0 Ms sum V m.H0."""
         m = self.m_gpu if m_vec is None else m_vec
         out = {}
         h = torch.empty_like(m)
         if self.ex is not None or self.dmi is not None:
-            a = self._args(m, h_eff=h, with_demag=False, with_h0=False)
-            a.ani_pref = 0.0
-            self._launch(a)
+            self._term_field(m, h, lin=True)
             out["E_lin"] = self._energy_from_field_lumped_gpu(m, h, -0.5)
         if self.ani_pref:
-            n = self.ani_axis_gpu.view(-1, 3) if self.ani_axis_gpu is not None else torch.tensor(self.ani_axis_u, dtype=torch.float64, device=self.device)
-            mdn = (m.view(-1, 3) * n).sum(dim=1)
-            out["E_ani"] = float(-self.Ku * (self.ops.vol * mdn * mdn).sum().item() * 1e-27)
+            self._term_field(m, h, ani=True)
+            out["E_ani"] = self._energy_from_field_lumped_gpu(m, h, -0.5)
+        if self.cub_pref:
+            self._term_field(m, h, cub=True)
+            out["E_cubic"] = self._energy_from_field_lumped_gpu(m, h, -0.25)
         if self.demag_field is not None:
             self.demag_field.compute_vec(m, h)
             out["E_demag"] = self.demag_field.Energy_lumped_gpu(m, h)
+        if self.H0_gpu is not None or any(self.H0_u):
+            if self.H0_gpu is not None:
+                h.copy_(self.H0_gpu)
+            else:
+                h.copy_(torch.tensor(self.H0_u, dtype=torch.float64, device=self.device).repeat(self.N))
+            out["E_zeeman"] = self._energy_from_field_lumped_gpu(m, h, -1.0)
         out["E_total"] = sum(out.values())
         return out
 
+    def compute_Energy_terms_minimize_gpu(self, m_vec=None) -> dict:
+        return self.compute_Energy_terms(m_vec)
+
+
+class EffectiveFieldSTTGPU(EffectiveField):
+    """Reference name of the STT effective field (llg_stt_module_GPU.py:522):
+    the Zhang-Li torque is part of the fused RHS kernel."""
+
+    def llg_rhs_STT_vec(self, m_vec, out_vec):
+        self.llg_rhs_vec(m_vec, out_vec)
+
+    def update_jac_state_STT(self, m_vec):
+        self.update_jac_state(m_vec)
+
+    def jac_vec_times_STT_vec(self, v_vec, out_vec):
+        self.jac_vec_times_vec(v_vec, out_vec)
+
+
+class EffectiveFieldSOTGPU(EffectiveField):
+    """Reference name of the SOT effective field (llg_SOT_module_GPU.py:545)."""
+
+    def llg_rhs_SOT_vec(self, m_vec, out_vec):
+        self.llg_rhs_vec(m_vec, out_vec)
+
+    def update_jac_state_SOT(self, m_vec):
+        self.update_jac_state(m_vec)
+
+    def jac_vec_times_SOT_vec(self, v_vec, out_vec):
+        self.jac_vec_times_vec(v_vec, out_vec)
+
+
+HBAR = 1.054571817e-34        # as llg_SOT_module_GPU.py:1377-1379
+E_CHARGE_SOT = 1.602176634e-19
+
 
 class LLG_GPU:
-    """Explicit Runge-Kutta LLG driver (relax_explicit, llg_module_GPU.py:1724-1971)."""
+    """Explicit Runge-Kutta LLG driver (relax_explicit, llg_module_GPU.py:1724-1971),
+    with the term builders of LLG_GPU, LLG_STT_GPU (add_current,
+    llg_stt_module_GPU.py:1409-1414) and LLG_SOT_GPU (add_sot,
+    add_sot_from_spin_hall, llg_SOT_module_GPU.py:1359-1381)."""
 
     def __init__(self, mesh: TetMesh, Ms, Aex, Ku=0.0, n_ani_vec=None, D_bulk=0.0, D_int=0.0,
                  n0_int_vec=None, gamma=2.211e5, alpha=0.5, do_precess=1, H0_static=None,
-                 device=None):
+                 Kc1=0.0, u1_cub=None, u2_cub=None, device=None):
         self.mesh = mesh
+        self.Ms = float(Ms)
         self._kw = dict(Ms=Ms, Aex=Aex, Ku=Ku, n_ani_vec=n_ani_vec, D_bulk=D_bulk, D_int=D_int,
                         n0_int_vec=n0_int_vec, gamma=gamma, alpha=alpha, do_precess=do_precess,
-                        H0_static=H0_static)
+                        H0_static=H0_static, Kc1=Kc1, u1_cub=u1_cub, u2_cub=u2_cub)
         self.device = device or default_device()
         self.use_demag = False
         self.demag_kwargs = {}
@@ -282,6 +443,25 @@ class LLG_GPU:
         self.use_demag = True
         self.demag_method = method
         self.demag_kwargs = kwargs
+
+    def add_cubic_anisotropy(self, Kc1, u1_vec, u2_vec):
+        self._kw.update(Kc1=Kc1, u1_cub=u1_vec, u2_cub=u2_vec)
+
+    def add_current(self, Jmagnitude, Jdir_vec, P, beta):
+        """Zhang-Li spin-transfer torque of a current density J (A/m^2)."""
+        self._kw.update(Jmagnitude=Jmagnitude, Jdir_vec=Jdir_vec, P=P, beta=beta)
+
+    def add_sot(self, H_dl, H_fl=0.0, polarization_vec=(0.0, 1.0, 0.0)):
+        """Spin-orbit torque with amplitudes in A/m."""
+        self._kw.update(H_dl=H_dl, H_fl=H_fl, polarization_vec=polarization_vec)
+        if self.hef is not None:
+            self.hef.set_sot(H_dl=H_dl, H_fl=H_fl, polarization_vec=polarization_vec)
+
+    def add_sot_from_spin_hall(self, J, theta_sh, thickness, polarization_vec=(0.0, 1.0, 0.0), H_fl=0.0):
+        """H_dl = hbar theta_sh J / (2 e mu0 Ms t)."""
+        H_dl = HBAR * float(theta_sh) * float(J) / (2.0 * E_CHARGE_SOT * MU0 * self.Ms * float(thickness))
+        self.add_sot(H_dl=H_dl, H_fl=H_fl, polarization_vec=polarization_vec)
+        return H_dl
 
     def _build_effective_field(self):
         self.hef = EffectiveField(self.mesh, use_demag=self.use_demag,
@@ -400,3 +580,7 @@ class DormandPrince:
             self._fsal = True
             t += dt
         return t
+
+
+LLG_STT_GPU = LLG_GPU
+LLG_SOT_GPU = LLG_GPU

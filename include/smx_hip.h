@@ -360,8 +360,16 @@ int smx_pcg_amg(int64_t n, int32_t width, int64_t ld, const int32_t* cols, const
  * Anisotropy_GPU.py:25-38; Demag_Lindholm_GPU.py:660-670).  One thread per
  * node over the node-graph SELL-64 pattern:
  *   H = ex*m + dmi*m + pref_ani (m.n) n + [grad*(u1+u2-u1_sum/n)]/V + H0
+ *       + pref_cub (s1 c1 + s2 c2 + s3 c3)            (cubic anisotropy)
  *   rhs = P (p m x H + alpha m x (m x H)) + S (1 - |m|^2) m
- * Any operator pointer may be NULL (term absent).  Outputs may be NULL.
+ *       - stt_pref (stt_a m x Z + stt_b m x (m x Z))   (Zhang-Li STT, Z = K_J m)
+ *       + sot_fl m x p + sot_dl m x (m x p)             (spin-orbit torque)
+ * with a_i = m.c_i, s1 = a1 (a2^2 + a3^2) (cyclic) for the orthonormal cubic
+ * axes c_i (Cubic_Anisotropy_GPU.py:33-53, 205-276), K_J the Zhang-Li
+ * operator (J.grad) with rows scaled by 1/V_i (llg_stt_module_GPU.py:393-435,
+ * 101-139) and the spin-orbit coefficients of llg_SOT_module_GPU.py:170-205,
+ * 813-817.  Any operator pointer may be NULL (term absent); a zero
+ * prefactor disables its term.  Outputs may be NULL.
  * ---------------------------------------------------------------------- */
 typedef struct {
   int64_t n;                 /* nodes */
@@ -387,17 +395,43 @@ typedef struct {
   double alpha;
   double do_precess;
   double stab;               /* Ms gamma/(1+alpha^2)/2 */
+  double cub_pref;           /* 2 Kc1/(mu0 Ms); 0 disables */
+  const double* cub_axes;    /* 9N: c1, c2, c3 of every node, or NULL -> cub_axes_u */
+  double cub_axes_u[9];
+  const double* stt_vals;    /* k=1 Zhang-Li operator on the node graph, or NULL */
+  double stt_pref;           /* J P muB / (e Ms (1+beta^2)) / (1+alpha^2) / 1e-9 */
+  double stt_a;              /* beta - alpha */
+  double stt_b;              /* 1 + alpha beta */
+  const double* sot_p;       /* 3N spin polarization, or NULL -> sot_p_u */
+  double sot_p_u[3];
+  double sot_fl;             /* -gamma (H_fl - alpha H_dl)/(1+alpha^2) */
+  double sot_dl;             /* -gamma (H_dl + alpha H_fl)/(1+alpha^2) */
 } smx_field_args;
 
 int smx_field_llg_rhs(const smx_field_args* a, void* stream);
 
 /* Jacobian-vector product of the LLG RHS with the field derivative
- * Hv = ex*v + dmi*v + pref_ani (v.n) n computed in-kernel (demag and external
- * field excluded, as compute_H_jac_vec / apply_jacobian_field_vec do).
- * Replaces _JAC_VEC_KERNEL + jac_vec_times_vec (llg_module_GPU.py:125-164,
- * 1293-1323).  `a->m` is the Jacobian state, `hm` its field H_jac(m). */
+ * Hv = ex*v + dmi*v + pref_ani (v.n) n + dH_cub/dm v computed in-kernel (demag
+ * and external field excluded, as compute_H_jac_vec / apply_jacobian_field_vec
+ * do), plus the exact derivatives of the STT and SOT torques.  Replaces
+ * _JAC_VEC_KERNEL + jac_vec_times_vec (llg_module_GPU.py:125-164, 860-883,
+ * 1293-1323), _JAC_VEC_STT_KERNEL (llg_stt_module_GPU.py:142-215) and
+ * _JAC_VEC_SOT_KERNEL (llg_SOT_module_GPU.py:208-275).  `a->m` is the
+ * Jacobian state, `hm` its field H_jac(m). */
 int smx_llg_jvp(const smx_field_args* a, const double* hm, const double* v,
                 double* out, double gamma, void* stream);
+
+/* Energy minimizer steps (Minimizer_GPU.py:46-88, 306-416): the projected
+ * direction d = h - (m.h) m, and the norm-preserving Cayley update
+ * out = (I - C)^-1 (I + C) m with C v = c x v, c = (step/2) (m x d) (a rotation
+ * of m about c by 2 atan|c|; out = m + step d + O(step^2) for d tangent). */
+int smx_projected_direction(int64_t n, const double* m, const double* h, double* d, void* stream);
+int smx_cayley_step(int64_t n, const double* m, const double* d, double step, double* out,
+                    void* stream);
+/* m_i <- m_i / |m_i| (Minimizer_GPU.py:238-250). */
+int smx_normalize3(int64_t n, double* m, void* stream);
+/* out = max_i |v_i| over 3-vectors (deterministic); red as smx_sell_spmv. */
+int smx_max_norm3(int64_t n, const double* v, double* out, double* red, void* stream);
 
 /* out = sum_i w_i (a_i . b_i) over 3-vectors (w may be NULL -> 1); `red` as in
  * smx_sell_spmv.  Energies (Demag_Lindholm_GPU.py:706-724,

@@ -31,6 +31,13 @@ How each part is pinned (see tests/test_oracle_*.py):
                       admissibility by brute force
   fullaca_dense       ||A - UV||_F <= eps ||A||_F ; exact rank on rank-r input
   hmatrix matvec      validate_against_dense protocol (seed 1234, 3 trials)
+  cubic anisotropy    field = -(1/mu0 Ms) dE/dm (central FD of the energy);
+                      closed forms at m along an axis / a face diagonal
+  zhang_li_matrix     linear m: Z = (J.grad) m exactly at every node; the
+                      per-node J rule against the element closed form
+  torques             macrospin: damping-like SOT turns m toward p; STT/SOT
+                      torques orthogonal to m; JVPs by central FD
+  cayley_step         Rodrigues rotation by 2 atan|c| about c; |y| = |m|
 """
 from __future__ import annotations
 
@@ -865,3 +872,125 @@ def energy_per_cell(X, tets, m, Ms_cell, A_cell, Ku=0.0, n_ani=None, dem=None, L
         parts["E_d"] = 0.0
     E = parts["E_ex"] + parts["E_an"] + parts["E_d"]
     return E, parts, gm.ravel(), gMs, gA
+
+
+# --------------------------------------------------------------------------
+# Cubic anisotropy (gpu/fields/Cubic_Anisotropy_GPU.py:33-53, 179-190, 205-276,
+# 278-380, 501-531), Zhang-Li STT (gpu/solvers/llg_stt_module_GPU.py:101-139,
+# 393-435, 566-581), spin-orbit torque (gpu/solvers/llg_SOT_module_GPU.py:
+# 170-205, 813-817) and the minimizer's Cayley step / projected direction
+# (gpu/solvers/Minimizer_GPU.py:46-88)
+# --------------------------------------------------------------------------
+def cubic_axes_oracle(u1, u2):
+    """c1 = u1/|u1|, c2 = u2 - (u2.c1) c1 normalized, c3 = c1 x c2 (:179-190)."""
+    a = np.asarray(u1, dtype=np.float64).reshape(-1, 3)
+    b = np.asarray(u2, dtype=np.float64).reshape(-1, 3)
+    a = a / np.linalg.norm(a, axis=1)[:, None]
+    b = b - np.einsum("ij,ij->i", b, a)[:, None] * a
+    b = b / np.linalg.norm(b, axis=1)[:, None]
+    c = np.cross(a, b)
+    c = c / np.linalg.norm(c, axis=1)[:, None]
+    return a, b, c
+
+
+def _cubic_proj(m, axes):
+    m3 = np.asarray(m, dtype=np.float64).reshape(-1, 3)
+    c1, c2, c3 = (np.broadcast_to(c, m3.shape) for c in axes)
+    dot = lambda x, y: np.einsum("ij,ij->i", x, y)
+    return m3, (c1, c2, c3), (dot(m3, c1), dot(m3, c2), dot(m3, c3))
+
+
+def cubic_field(m, axes, K1, Ms):
+    """H = 2 K1/(mu0 Ms) (s1 c1 + s2 c2 + s3 c3), s1 = a1 (a2^2 + a3^2) cyclic,
+    a_i = m.c_i (:33-53, 205-276)."""
+    m3, (c1, c2, c3), (a1, a2, a3) = _cubic_proj(m, axes)
+    pref = 2.0 * K1 / (MU0 * Ms)
+    s1, s2, s3 = a1 * (a2 ** 2 + a3 ** 2), a2 * (a3 ** 2 + a1 ** 2), a3 * (a1 ** 2 + a2 ** 2)
+    return (pref * (s1[:, None] * c1 + s2[:, None] * c2 + s3[:, None] * c3)).ravel()
+
+
+def cubic_field_jvp(m, v, axes, K1, Ms):
+    """dH/dm v of cubic_field (:278-380)."""
+    m3, (c1, c2, c3), (a1, a2, a3) = _cubic_proj(m, axes)
+    v3 = np.asarray(v, dtype=np.float64).reshape(-1, 3)
+    d1, d2, d3 = (np.einsum("ij,ij->i", v3, c) for c in (c1, c2, c3))
+    pref = 2.0 * K1 / (MU0 * Ms)
+    s1 = d1 * (a2 ** 2 + a3 ** 2) + 2.0 * a1 * (a2 * d2 + a3 * d3)
+    s2 = d2 * (a3 ** 2 + a1 ** 2) + 2.0 * a2 * (a3 * d3 + a1 * d1)
+    s3 = d3 * (a1 ** 2 + a2 ** 2) + 2.0 * a3 * (a1 * d1 + a2 * d2)
+    return (pref * (s1[:, None] * c1 + s2[:, None] * c2 + s3[:, None] * c3)).ravel()
+
+
+def cubic_energy(m, axes, K1, V):
+    """E = -K1 sum_i V_i (a1^2 a2^2 + a2^2 a3^2 + a3^2 a1^2) * 1e-27 (:501-531)."""
+    _, _, (a1, a2, a3) = _cubic_proj(m, axes)
+    S = a1 ** 2 * a2 ** 2 + a2 ** 2 * a3 ** 2 + a3 ** 2 * a1 ** 2
+    return float(-K1 * np.sum(V * S) * 1e-27)
+
+
+def zhang_li_matrix(X, tets, V, Jdir):
+    """K_J / V: K_J[a, b] = int (J . grad phi_b) phi_a dx (UFL form
+    inner(dot(grad(u), J), v) dx, rows scaled by 1/VolN; :393-435), J one
+    vector or one per node (P1).  The product of the linear J with the linear
+    phi_a is integrated with the 4-point degree-2 rule on each tet."""
+    X = np.asarray(X, dtype=np.float64)
+    tets = np.asarray(tets, dtype=np.int64)
+    N = X.shape[0]
+    vol, g = element_geometry(X, tets)
+    J = np.asarray(Jdir, dtype=np.float64).reshape(-1, 3)
+    Jn = np.broadcast_to(J, (N, 3)) if J.shape[0] == 1 else J
+    # degree-2 rule: barycentric points (a, b, b, b) with a = 0.5854101966249685,
+    # b = 0.1381966011250105, equal weights |T|/4
+    qa, qb = 0.5854101966249685, 0.1381966011250105
+    lam = np.full((4, 4), qb) + np.eye(4) * (qa - qb)      # lam[q, c] at point q
+    Jq = np.einsum("qc,tck->tqk", lam, Jn[tets])          # J at the points
+    # int phi_a (J . g_b) = sum_q |T|/4 lam[q, a] (J_q . g_b)
+    elem = np.einsum("t,qa,tqk,tbk->tab", vol / 4.0, lam, Jq, g)
+    rows = np.repeat(tets[:, :, None], 4, axis=2)
+    cols = np.repeat(tets[:, None, :], 4, axis=1)
+    K = _coo(rows, cols, elem, (N, N))
+    return sp.diags(1.0 / V) @ K
+
+
+def llg_rhs_torques(m, H, gamma, alpha, Ms, Z=None, stt=None, p=None, H_dl=0.0, H_fl=0.0, do_precess=1.0):
+    """LLG RHS plus the Zhang-Li torque - pref (a m x Z + b m x (m x Z)) with
+    stt = (J, P, beta): pref = J P muB/(e Ms (1+beta^2))/(1+alpha^2)/1e-9,
+    a = beta - alpha, b = 1 + alpha beta (llg_stt_module_GPU.py:101-139,
+    566-581), plus the spin-orbit torque c_fl m x p + c_dl m x (m x p),
+    c_fl = -gamma (H_fl - alpha H_dl)/(1+alpha^2), c_dl = -gamma (H_dl +
+    alpha H_fl)/(1+alpha^2) (llg_SOT_module_GPU.py:170-205, 813-817)."""
+    r = llg_rhs(m, H, gamma, alpha, Ms, do_precess).reshape(-1, 3)
+    m3 = np.asarray(m, dtype=np.float64).reshape(-1, 3)
+    if stt is not None:
+        J, P, beta = stt
+        pref = J * P * 9.27400915e-24 / (1.6021766e-19 * Ms * (1.0 + beta ** 2)) / (1.0 + alpha ** 2) / 1e-9
+        z3 = np.asarray(Z, dtype=np.float64).reshape(-1, 3)
+        mz = np.cross(m3, z3)
+        r -= pref * ((beta - alpha) * mz + (1.0 + alpha * beta) * np.cross(m3, mz))
+    if p is not None and (H_dl or H_fl):
+        p3 = np.broadcast_to(np.asarray(p, dtype=np.float64).reshape(-1, 3), m3.shape)
+        p3 = p3 / np.linalg.norm(p3, axis=1)[:, None]
+        den = 1.0 + alpha * alpha
+        c_fl = -gamma * (H_fl - alpha * H_dl) / den
+        c_dl = -gamma * (H_dl + alpha * H_fl) / den
+        mp = np.cross(m3, p3)
+        r += c_fl * mp + c_dl * np.cross(m3, mp)
+    return r.ravel()
+
+
+def projected_direction(m, h):
+    """d = h - (m.h) m (Minimizer_GPU.py:75-88)."""
+    m3, h3 = np.asarray(m).reshape(-1, 3), np.asarray(h).reshape(-1, 3)
+    return (h3 - np.einsum("ij,ij->i", m3, h3)[:, None] * m3).ravel()
+
+
+def cayley_step(m, d, step):
+    """(I - C)^-1 (I + C) m, C v = c x v, c = (step/2) m x d, solved as a 3x3
+    linear system per node (Minimizer_GPU.py:46-72 states the closed form)."""
+    m3, d3 = np.asarray(m).reshape(-1, 3), np.asarray(d).reshape(-1, 3)
+    c = 0.5 * step * np.cross(m3, d3)
+    out = np.empty_like(m3)
+    for i in range(m3.shape[0]):
+        Cm = np.array([[0.0, -c[i, 2], c[i, 1]], [c[i, 2], 0.0, -c[i, 0]], [-c[i, 1], c[i, 0], 0.0]])
+        out[i] = np.linalg.solve(np.eye(3) - Cm, (np.eye(3) + Cm) @ m3[i])
+    return out.ravel()
