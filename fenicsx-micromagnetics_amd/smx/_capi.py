@@ -108,6 +108,11 @@ _SIGS = {
     "smx_cayley_step": ([I64, P, P, F64, P, P], C.c_int),
     "smx_max_norm3": ([I64, P, P, P, P], C.c_int),
     "smx_normalize3": ([I64, P, P], C.c_int),
+    "smx_llg_pc_build": ([C.POINTER(FieldArgs), P, F64, F64, F64, I32, I32, I32, P, P], C.c_int),
+    "smx_block3_apply": ([I64, P, P, P, P], C.c_int),
+    "smx_multi_dot_ws_size": ([I64, I32], I64),
+    "smx_multi_dot": ([I64, I32, P, I64, P, P, P, P], C.c_int),
+    "smx_lincomb": ([I64, I32, P, I64, P, F64, P, P], C.c_int),
     "smx_rk_combine": ([I64, P, F64, I32, P, P, P, F64, F64, P, P, P, P], C.c_int),
     "smx_energy_blocks": ([I64], I64),
     "smx_energy_cell_forward": ([I64, P, P, P, P, P, P, P, P, P, P], C.c_int),

@@ -421,8 +421,9 @@ E_CHARGE_SOT = 1.602176634e-19
 
 
 class LLG_GPU:
-    """Explicit Runge-Kutta LLG driver (relax_explicit, llg_module_GPU.py:1724-1971),
-    with the term builders of LLG_GPU, LLG_STT_GPU (add_current,
+    """LLG driver: explicit Runge-Kutta (relax_explicit, llg_module_GPU.py:
+    1724-1971) and implicit BDF Newton-Krylov (relax, :2183-2381), with the
+    term builders of LLG_GPU, LLG_STT_GPU (add_current,
     llg_stt_module_GPU.py:1409-1414) and LLG_SOT_GPU (add_sot,
     add_sot_from_spin_hall, llg_SOT_module_GPU.py:1359-1381)."""
 
@@ -486,6 +487,22 @@ class LLG_GPU:
                                   fixed_dt=fixed_dt, max_steps=max_steps)
         stats["elapsed_s"] = time.perf_counter() - t0
         return stats
+
+    def relax(self, m0_array, t0, t_final, dt_init, ts_rtol=1e-6, ts_atol=1e-6, snes_rtol=1e-2, snes_atol=1e-4,
+              ksp_rtol=1e-4, stopping_dmdt=0.0, monitor_fn=None, check_every_stop=5, return_stats=False,
+              use_pc_python=True, pc_form="exact", max_steps=5_000_000):
+        """Implicit BDF / Newton-Krylov relax (llg_module_GPU.py:2183-2381),
+        the block preconditioner when use_pc_python; returns hef (and the
+        integrator statistics when return_stats)."""
+        from .implicit import relax_implicit
+        if self.hef is None:
+            self._build_effective_field()
+        stats = relax_implicit(self.hef, t_final, dt_init=dt_init, ts_rtol=ts_rtol, ts_atol=ts_atol,
+                               snes_rtol=snes_rtol, snes_atol=snes_atol, ksp_rtol=ksp_rtol, m0_array=m0_array,
+                               use_python_pc=use_pc_python, max_steps=max_steps, t0=t0,
+                               stopping_dmdt=stopping_dmdt, check_every_stop=check_every_stop,
+                               monitor_fn=monitor_fn, pc_form=pc_form)
+        return (self.hef, stats) if return_stats else self.hef
 
 
 class DormandPrince:

@@ -421,6 +421,35 @@ int smx_field_llg_rhs(const smx_field_args* a, void* stream);
 int smx_llg_jvp(const smx_field_args* a, const double* hm, const double* v,
                 double* out, double gamma, void* stream);
 
+/* ---------------------------------------------------------------------- *
+ * (f) Implicit (BDF) LLG steps.  smx_llg_pc_build replaces
+ * JvContext.update_pc_full_fast_gpu + _PC_BUILD_INV3_KERNEL
+ * (llg_module_GPU.py:165-290, 1447-1505): inv (9N, row-major 3x3 per node) =
+ * [(shift + eps_reg) I - J_i]^-1 with the local block
+ * J_i = -+ c1 d(m x H)/dm -+ c2 d(m x (m x H))/dm + S ((1-|m|^2) I - 2 m m^T)
+ * (c1 = gamma p/(1+alpha^2), c2 = gamma alpha/(1+alpha^2); the derivatives
+ * taken with dH/dm = kappa I, kappa = mean (use_abs_kappa: mean |.|) of the
+ * diagonal of the linear field operator of `a`: exchange, DMI, uniaxial and
+ * cubic anisotropy) at the state a->m with field hm; upper signs (exact = 1):
+ * the derivative of the RHS; lower signs (exact = 0): the reference's block;
+ * the stabilization term when include_stab; |det| < det_eps is pushed to
+ * det_eps.  smx_block3_apply replaces JvContext.apply
+ * (_PC_APPLY_KERNEL, :293-307): y_i = inv_i x_i.
+ * smx_multi_dot: out[j] = V_j . w for j < k (V_j at V + j*ldv, len doubles;
+ * deterministic two-stage reduction; work: smx_multi_dot_ws_size doubles).
+ * smx_lincomb: out = beta out + sum_j c_j V_j (c: k device doubles).
+ * These carry the Arnoldi steps of the GMRES that replaces PETSc KSP gmres
+ * (:2074-2115). */
+int smx_llg_pc_build(const smx_field_args* a, const double* hm, double shift, double eps_reg,
+                     double det_eps, int32_t include_stab, int32_t use_abs_kappa, int32_t exact,
+                     double* inv, void* stream);
+int smx_block3_apply(int64_t n, const double* inv, const double* x, double* y, void* stream);
+int64_t smx_multi_dot_ws_size(int64_t len, int32_t k);
+int smx_multi_dot(int64_t len, int32_t k, const double* V, int64_t ldv, const double* w,
+                  double* out, double* work, void* stream);
+int smx_lincomb(int64_t len, int32_t k, const double* V, int64_t ldv, const double* c,
+                double beta, double* out, void* stream);
+
 /* Energy minimizer steps (Minimizer_GPU.py:46-88, 306-416): the projected
  * direction d = h - (m.h) m, and the norm-preserving Cayley update
  * out = (I - C)^-1 (I + C) m with C v = c x v, c = (step/2) (m x d) (a rotation

@@ -994,3 +994,31 @@ def cayley_step(m, d, step):
         Cm = np.array([[0.0, -c[i, 2], c[i, 1]], [c[i, 2], 0.0, -c[i, 0]], [-c[i, 1], c[i, 0], 0.0]])
         out[i] = np.linalg.solve(np.eye(3) - Cm, (np.eye(3) + Cm) @ m3[i])
     return out.ravel()
+
+
+# --------------------------------------------------------------------------
+# Implicit steps: local 3x3 block of the Newton operator
+# (gpu/solvers/llg_module_GPU.py:165-290, 1412-1505)
+# --------------------------------------------------------------------------
+def llg_local_jacobian(m_i, h_i, kappa, gamma, alpha, Ms, do_precess=1.0, include_stab=True):
+    """d r_i / d m_i of the LLG RHS r = -c1 m x H - c2 m x (m x H) + S (1-|m|^2) m
+    at one node, with the field's self-derivative dH/dm = kappa I.  Written
+    from the product rule: d(m x H) = dm x H + kappa m x dm;
+    d(m (m.H) - H |m|^2) = dm (m.H) + m (dm.H + kappa m.dm) - kappa dm |m|^2
+    - 2 H (m.dm)."""
+    m = np.asarray(m_i, dtype=np.float64)
+    h = np.asarray(h_i, dtype=np.float64)
+    den = 1.0 + alpha * alpha
+    c1, c2 = gamma * do_precess / den, gamma * alpha / den
+    S = Ms * gamma / den * 0.5
+    J = np.zeros((3, 3))
+    for c in range(3):
+        e = np.zeros(3)
+        e[c] = 1.0
+        dprec = np.cross(e, h) + kappa * np.cross(m, e)
+        ddamp = e * (m @ h) + m * (e @ h + kappa * (m @ e)) - kappa * e * (m @ m) - 2.0 * h * (m @ e)
+        col = -c1 * dprec - c2 * ddamp
+        if include_stab:
+            col = col + S * ((1.0 - m @ m) * e - 2.0 * m * (m @ e))
+        J[:, c] = col
+    return J
