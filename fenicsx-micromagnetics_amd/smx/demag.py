@@ -264,7 +264,14 @@ class DemagFieldLindholmGPU:
                 max_temporary_block_bytes=int(hmatrix_max_temporary_block_bytes),
                 use_fp32=bool(hmatrix_use_fp32), cache_path=hmatrix_cache_path,
                 force_rebuild=bool(hmatrix_force_rebuild), verbose=bool(hmatrix_verbose),
-                near_fp32=bool(hmatrix_near_fp32))
+                near_fp32=bool(hmatrix_near_fp32), compressor=str(hmatrix_compressor),
+                gpu_device_id=int(hmatrix_build_device_id), gpu_memory_fraction=float(hmatrix_gpu_memory_fraction),
+                gpu_fallback_to_cpu=bool(hmatrix_gpu_fallback_to_cpu),
+                gpu_aca_residual_check_interval=int(hmatrix_gpu_aca_residual_check_interval),
+                gpu_rsvd_initial_rank=int(hmatrix_gpu_rsvd_initial_rank),
+                gpu_rsvd_oversampling=int(hmatrix_gpu_rsvd_oversampling),
+                gpu_rsvd_power_iterations=int(hmatrix_gpu_rsvd_power_iterations),
+                gpu_progress_interval_seconds=float(hmatrix_gpu_progress_interval_seconds))
         torch.cuda.synchronize(dev)
         self.setup_boundary_seconds = time.perf_counter() - t0
 

@@ -1,12 +1,23 @@
 """H-matrix construction on the device.
 
-Partition rule (lindholm_hmatrix_gpu/builder.py:256-376): for a block (r, c)
-  * admissible and dense_bytes > max_temporary_block_bytes -> subdivide;
-  * admissible -> fill, full-pivot ACA; accept when rank > 0, rel. residual
-    <= eps (1 + 64 u) and (m+n) rank < 0.95 m n; else subdivide when the
-    tolerance failed and a side can split; else keep it dense;
+Partition rule (lindholm_hmatrix_gpu/builder.py:256-376, 630-760): for a
+block (r, c)
+  * admissible and its workspace does not fit (dense_bytes >
+    max_temporary_block_bytes, or dense_bytes x the compressor's workspace
+    multiplier above gpu_memory_fraction of the device memory available) ->
+    subdivide; an unsplittable block is kept dense (gpu_fallback_to_cpu) or
+    raises MemoryError;
+  * admissible -> fill, compress (full-pivot ACA, truncated SVD or adaptive
+    randomized SVD); accept when rank > 0, rel. residual <= eps (1 + 64 u)
+    and (m+n) rank < 0.95 m n; else subdivide when the tolerance failed and a
+    side can split; else keep it dense;
   * inadmissible leaf x leaf -> dense; otherwise subdivide the splittable side
     with the larger diameter (rows on ties).
+A batch that runs out of device memory is halved and retried; a single block
+that still does is subdivided (counted as an OOM subdivision) or, when it
+cannot split, kept dense as above.  The reference fills such a dense block on
+the CPU; here it is filled on the device in row slabs small enough to fit, so
+no block entry is ever computed on the host.
 Each block's fate depends only on the block itself, so the recursion is run
 breadth first: every level's candidate blocks are filled by ONE batched
 Lindholm launch and compressed by ONE batched ACA launch (a workgroup per
@@ -55,6 +66,14 @@ class HMatrixBuildConfig:
     max_temporary_block_bytes: int = 256 * 1024 * 1024
     low_rank_storage_factor: float = 0.95
     build_backend: str = "gpu"
+    gpu_device_id: int = 0
+    gpu_memory_fraction: float = 0.75
+    gpu_fallback_to_cpu: bool = True
+    gpu_aca_residual_check_interval: int = 1
+    gpu_rsvd_initial_rank: int = 16
+    gpu_rsvd_oversampling: int = 8
+    gpu_rsvd_power_iterations: int = 1
+    gpu_progress_interval_seconds: float = 5.0
     batch_bytes: int = 4 * 1024 ** 3
 
     def __post_init__(self):
@@ -69,11 +88,24 @@ class HMatrixBuildConfig:
         if not (0.0 < self.low_rank_storage_factor <= 1.0):
             raise ValueError("low_rank_storage_factor must be in (0, 1].")
         comp = str(self.compressor).strip().lower()
-        if comp == "aca":
-            comp = "fullaca"
-        if comp != "fullaca":
-            raise NotImplementedError("only the full-pivot ACA compressor is built for the device path")
-        self.compressor = comp
+        if comp not in {"fullaca", "aca", "svd", "rsvd", "randomized_svd", "randomized-svd"}:
+            raise ValueError("compressor must be 'fullaca', 'svd', or 'rsvd'.")
+        self.compressor = {"aca": "fullaca", "randomized_svd": "rsvd", "randomized-svd": "rsvd"}.get(comp, comp)
+        self.gpu_device_id = int(self.gpu_device_id)
+        if self.gpu_device_id < 0:
+            raise ValueError("gpu_device_id must be non-negative.")
+        if not (0.0 < float(self.gpu_memory_fraction) <= 1.0):
+            raise ValueError("gpu_memory_fraction must be in (0, 1].")
+        if self.gpu_aca_residual_check_interval < 1:
+            raise ValueError("gpu_aca_residual_check_interval must be positive.")
+        if self.gpu_rsvd_initial_rank < 1:
+            raise ValueError("gpu_rsvd_initial_rank must be positive.")
+        if self.gpu_rsvd_oversampling < 0:
+            raise ValueError("gpu_rsvd_oversampling must be non-negative.")
+        if self.gpu_rsvd_power_iterations < 0:
+            raise ValueError("gpu_rsvd_power_iterations must be non-negative.")
+        if self.gpu_progress_interval_seconds < 0.0:
+            raise ValueError("gpu_progress_interval_seconds must be non-negative.")
         self.build_backend = str(self.build_backend).strip().lower()
         if self.build_backend not in {"gpu", "auto"}:
             raise ValueError("build_backend must be 'gpu' (the device builds the H-matrix)")
@@ -104,6 +136,16 @@ class DeviceBuildOps:
 
     def alloc(self, n: int):
         return torch.empty(max(int(n), 1), dtype=torch.float64, device=self.device)
+
+    def available_bytes(self) -> int:
+        """Free device memory plus what torch's caching allocator holds
+        unused (builder.py:520-548 counts the pool's reusable bytes alike)."""
+        free, _ = torch.cuda.mem_get_info(self.device)
+        pool = torch.cuda.memory_reserved(self.device) - torch.cuda.memory_allocated(self.device)
+        return int(free) + max(int(pool), 0)
+
+    def release(self):
+        torch.cuda.empty_cache()
 
     def host(self, t) -> np.ndarray:
         return t.cpu().numpy()
@@ -166,6 +208,32 @@ class DeviceBuildOps:
                                                 st_d.data_ptr(), _capi.stream_ptr()), "smx_aca_batched")
         return rank_d.cpu().numpy(), res_d.cpu().numpy(), st_d.cpu().numpy()
 
+    def compress(self, method, A, tasks, U, V, rtol, max_rank, seeds, cfg):
+        if method == "fullaca":
+            return self.aca(A, tasks, U, V, rtol)
+        from .compressors import rsvd_batched, svd_batched
+        if method == "svd":
+            return svd_batched(A, tasks, U, V, rtol, max_rank)
+        return rsvd_batched(A, tasks, U, V, rtol, seeds, max_rank, cfg.gpu_rsvd_initial_rank,
+                            cfg.gpu_rsvd_oversampling, cfg.gpu_rsvd_power_iterations)
+
+
+def is_device_oom(exc: BaseException) -> bool:
+    """Allocation failures of the device (torch's OutOfMemoryError, the C
+    ABI's SMX_ERR_OOM as MemoryError, or an allocator message), through the
+    exception chain (builder.py:591-628)."""
+    seen = set()
+    cur = exc
+    while cur is not None and id(cur) not in seen:
+        seen.add(id(cur))
+        if isinstance(cur, (MemoryError, torch.cuda.OutOfMemoryError)):
+            return True
+        msg = str(cur).lower()
+        if any(t in msg for t in ("out of memory", "outofmemory", "memory allocation", "alloc failed")):
+            return True
+        cur = cur.__cause__ or cur.__context__
+    return False
+
 
 def _can_split(node: ClusterNode) -> bool:
     return not node.is_leaf
@@ -189,6 +257,9 @@ def _max_ok_rank(m: int, n: int, factor: float) -> int:
 
 def build_hmatrix_device(geo: BoundaryGeometry, config: HMatrixBuildConfig | None = None, device=None,
                          verbose: bool = False, ops=None) -> HMatrixCPUData:
+    from collections import deque
+
+    from .compressors import block_random_seed, workspace_multiplier
     cfg = config or HMatrixBuildConfig()
     t_start = time.perf_counter()
     tree = build_cluster_tree(geo.Xb, leaf_size=cfg.leaf_size)
@@ -196,13 +267,16 @@ def build_hmatrix_device(geo: BoundaryGeometry, config: HMatrixBuildConfig | Non
     if ops is None:
         ops = DeviceBuildOps(geo, device or torch.device("cuda", torch.cuda.current_device()))
     margin = 1.0 + 64.0 * np.finfo(np.float64).eps
+    mult = workspace_multiplier(cfg.compressor)
     stats = dict(dense_blocks=0, low_rank_blocks=0, rejected_low_rank_blocks=0,
                  forced_subdivisions_memory=0, compression_tolerance_subdivisions=0,
                  block_entry_evaluations=0, max_observed_rank=0, sum_observed_rank=0,
-                 aca_batches=0, fill_batches=0)
+                 aca_batches=0, fill_batches=0, gpu_workspace_subdivisions=0, gpu_oom_subdivisions=0,
+                 fallback_dense_blocks=0, oom_batch_splits=0, slab_filled_blocks=0, cpu_fallback_blocks=0)
     decision: dict = {}
     far_of: dict = {}
     near_rows, near_cols, near_vals = [], [], []
+    last_report = [time.perf_counter()]
 
     def key(r, c):
         return (id(r), id(c))
@@ -210,45 +284,165 @@ def build_hmatrix_device(geo: BoundaryGeometry, config: HMatrixBuildConfig | Non
     def idx(node):
         return perm[node.start:node.stop].astype(np.int64)
 
-    def add_near(pairs):
+    def release():
+        rel = getattr(ops, "release", None)
+        if rel is not None:
+            rel()
+
+    def keep_nonzeros(host, offs, chunk, row_shift=None):
+        nz = np.flatnonzero(host)
+        if not nz.size:
+            return
+        b = np.searchsorted(offs, nz, side="right") - 1
+        local = nz - offs[b]
+        ncol = np.array([c.size for _, c in chunk], dtype=np.int64)[b]
+        rs = np.array([r.start for r, _ in chunk], dtype=np.int64)[b]
+        if row_shift is not None:
+            rs = rs + row_shift
+        cs = np.array([c.start for _, c in chunk], dtype=np.int64)[b]
+        near_rows.append(rs + local // ncol)
+        near_cols.append(cs + local % ncol)
+        near_vals.append(host[nz])
+
+    def fill_slabs(r, c):
+        """One dense block filled in row slabs, halving the slab until it fits."""
+        rows, cols = idx(r), idx(c)
+        slab = max(1, r.size // 2)
+        a = 0
+        while a < r.size:
+            b = min(a + slab, r.size)
+            try:
+                buf = ops.alloc((b - a) * c.size)
+                ops.fill([(rows[a:b], cols)], buf, np.zeros(1, np.int64))
+                host = ops.host(buf)[:(b - a) * c.size].copy()
+            except Exception as exc:
+                if not is_device_oom(exc) or slab == 1:
+                    raise
+                buf = None
+                release()
+                slab = max(1, slab // 2)
+                continue
+            del buf
+            stats["fill_batches"] += 1
+            keep_nonzeros(host, np.zeros(1, np.int64), [(r, c)], row_shift=a)
+            a = b
+
+    def add_near(pairs, budget):
         """Fill blocks densely and keep their nonzero entries (permuted coords)."""
         if not pairs:
             return
-        for chunk in _batches(pairs, cfg.batch_bytes):
+        work = deque(_batches(pairs, budget))
+        while work:
+            chunk = work.popleft()
             sizes = np.array([r.size * c.size for r, c in chunk], dtype=np.int64)
             offs = np.concatenate([[0], np.cumsum(sizes)[:-1]])
-            buf = ops.alloc(int(sizes.sum()))
-            ops.fill([(idx(r), idx(c)) for r, c in chunk], buf, offs)
+            try:
+                buf = ops.alloc(int(sizes.sum()))
+                ops.fill([(idx(r), idx(c)) for r, c in chunk], buf, offs)
+                host = ops.host(buf)[:int(sizes.sum())].copy()
+            except Exception as exc:
+                if not is_device_oom(exc):
+                    raise
+                buf = None
+                release()
+                if len(chunk) > 1:
+                    stats["oom_batch_splits"] += 1
+                    h = len(chunk) // 2
+                    work.appendleft(chunk[h:])
+                    work.appendleft(chunk[:h])
+                    continue
+                if not cfg.gpu_fallback_to_cpu:
+                    raise MemoryError(f"device OOM filling a dense {chunk[0][0].size}x{chunk[0][1].size} block") \
+                        from exc
+                fill_slabs(*chunk[0])
+                stats["slab_filled_blocks"] += 1
+                stats["block_entry_evaluations"] += int(sizes.sum())
+                continue
+            del buf
             stats["fill_batches"] += 1
             stats["block_entry_evaluations"] += int(sizes.sum())
-            host = ops.host(buf)[:int(sizes.sum())]
-            nz = np.flatnonzero(host)
-            if nz.size:
-                b = np.searchsorted(offs, nz, side="right") - 1
-                local = nz - offs[b]
-                ncol = np.array([c.size for _, c in chunk], dtype=np.int64)[b]
-                rs = np.array([r.start for r, _ in chunk], dtype=np.int64)[b]
-                cs = np.array([c.start for _, c in chunk], dtype=np.int64)[b]
-                near_rows.append(rs + local // ncol)
-                near_cols.append(cs + local % ncol)
-                near_vals.append(host[nz])
-            del buf
+            keep_nonzeros(host, offs, chunk)
+
+    def compress_chunk(chunk):
+        """Fill + compress one batch; returns per-block results and the host
+        factors of the blocks that pass; commits nothing."""
+        m_ = np.array([r.size for r, _ in chunk], dtype=np.int64)
+        n_ = np.array([c.size for _, c in chunk], dtype=np.int64)
+        sizes = m_ * n_
+        offs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+        rcap = np.empty(len(chunk), dtype=np.int64)
+        storage_cap = np.zeros(len(chunk), dtype=bool)
+        for i in range(len(chunk)):
+            ok = _max_ok_rank(int(m_[i]), int(n_[i]), cfg.low_rank_storage_factor)
+            full = int(min(m_[i], n_[i]))
+            if cfg.max_rank is None:
+                rcap[i] = min(ok + 1, full)
+                storage_cap[i] = rcap[i] == ok + 1
+            else:
+                rcap[i] = min(max(int(cfg.max_rank), 0), full)
+        uoff = np.concatenate([[0], np.cumsum(m_ * rcap)[:-1]]).astype(np.int64)
+        voff = np.concatenate([[0], np.cumsum(n_ * rcap)[:-1]]).astype(np.int64)
+        A = ops.alloc(int(sizes.sum()))
+        U = ops.alloc(int((m_ * rcap).sum()))
+        V = ops.alloc(int((n_ * rcap).sum()))
+        ops.fill([(idx(r), idx(c)) for r, c in chunk], A, offs)
+        tasks = np.zeros(len(chunk), dtype=ACA_TASK_DTYPE)
+        tasks["a_off"], tasks["u_off"], tasks["v_off"] = offs, uoff, voff
+        tasks["m"], tasks["n"], tasks["rcap"] = m_, n_, rcap
+        seeds = np.array([block_random_seed(r.start, r.stop, c.start, c.stop) for r, c in chunk], dtype=np.int64)
+        ranks, rels, status = ops.compress(cfg.compressor, A, tasks, U, V, float(cfg.epsilon), cfg.max_rank,
+                                           seeds, cfg)
+        fate = []
+        for i, (r, c) in enumerate(chunk):
+            rank, rel = int(ranks[i]), float(rels[i])
+            if cfg.compressor == "fullaca" and status[i] == 2 and storage_cap[i]:
+                fate.append("reject")
+                continue
+            meets = np.isfinite(rel) and rel <= cfg.epsilon * margin
+            if rank > 0 and meets and (m_[i] + n_[i]) * rank < cfg.low_rank_storage_factor * sizes[i]:
+                fate.append("far")
+            elif not meets and (_can_split(r) or _can_split(c)):
+                fate.append("split")
+            else:
+                fate.append("reject" if status[i] != 3 else "zero")
+        acc = [i for i, f in enumerate(fate) if f == "far"]
+        Uh = Vh = None
+        if acc:
+            Uh = ops.gather(U, [(int(uoff[i]), int(uoff[i]) + int(m_[i]) * int(ranks[i])) for i in acc])
+            Vh = ops.gather(V, [(int(voff[i]), int(voff[i]) + int(n_[i]) * int(ranks[i])) for i in acc])
+        del A, U, V
+        return fate, ranks, rels, m_, n_, Uh, Vh, int(sizes.sum())
 
     queue = [(tree.root, tree.root)]
     while queue:
-        nxt, cand, dense_pairs = [], [], []
+        avail = float(ops.available_bytes()) * cfg.gpu_memory_fraction if hasattr(ops, "available_bytes") \
+            else math.inf
+        nxt, cand, dense_pairs, rejected = [], [], [], []
+
+        def split(r, c):
+            which, ch = _children(r, c)
+            decision[key(r, c)] = ("split", which)
+            nxt.extend(ch)
+
+        def split_or_fallback(r, c, stat, sink):
+            if _can_split(r) or _can_split(c):
+                stats[stat] += 1
+                split(r, c)
+                return
+            if not cfg.gpu_fallback_to_cpu:
+                reason = "device OOM" if stat == "gpu_oom_subdivisions" else "device workspace limit"
+                raise MemoryError(f"{reason} for an unsplittable H-matrix block {r.size}x{c.size}.")
+            decision[key(r, c)] = ("dense",)
+            sink.append((r, c))
+            stats["dense_blocks"] += 1
+            stats["fallback_dense_blocks"] += 1
+
         for r, c in queue:
             adm = admissible(r, c, cfg.eta)
-            if adm and r.size * c.size * 8 > cfg.max_temporary_block_bytes:
+            nbytes = r.size * c.size * 8
+            if adm and (nbytes > cfg.max_temporary_block_bytes or nbytes * mult > avail):
                 stats["forced_subdivisions_memory"] += 1
-                if not _can_split(r) and not _can_split(c):
-                    decision[key(r, c)] = ("dense",)
-                    dense_pairs.append((r, c))
-                    stats["dense_blocks"] += 1
-                else:
-                    which, ch = _children(r, c)
-                    decision[key(r, c)] = ("split", which)
-                    nxt += ch
+                split_or_fallback(r, c, "gpu_workspace_subdivisions", dense_pairs)
             elif adm:
                 cand.append((r, c))
             elif r.is_leaf and c.is_leaf:
@@ -256,71 +450,34 @@ def build_hmatrix_device(geo: BoundaryGeometry, config: HMatrixBuildConfig | Non
                 dense_pairs.append((r, c))
                 stats["dense_blocks"] += 1
             else:
-                which, ch = _children(r, c)
-                decision[key(r, c)] = ("split", which)
-                nxt += ch
-        add_near(dense_pairs)
-        rejected = []
-        for chunk in _batches(cand, cfg.batch_bytes // 2):
-            m_ = np.array([r.size for r, _ in chunk], dtype=np.int64)
-            n_ = np.array([c.size for _, c in chunk], dtype=np.int64)
-            sizes = m_ * n_
-            offs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
-            rcap = np.empty(len(chunk), dtype=np.int64)
-            storage_cap = np.zeros(len(chunk), dtype=bool)
-            for i in range(len(chunk)):
-                ok = _max_ok_rank(int(m_[i]), int(n_[i]), cfg.low_rank_storage_factor)
-                full = int(min(m_[i], n_[i]))
-                if cfg.max_rank is None:
-                    rcap[i] = min(ok + 1, full)
-                    storage_cap[i] = rcap[i] == ok + 1
+                split(r, c)
+        fill_budget = int(max(min(cfg.batch_bytes, avail), 8))
+        add_near(dense_pairs, fill_budget)
+        work = deque(_batches(cand, int(max(min(cfg.batch_bytes // 2, avail / mult), 8))))
+        while work:
+            chunk = work.popleft()
+            try:
+                fate, ranks, rels, m_, n_, Uh, Vh, nent = compress_chunk(chunk)
+            except Exception as exc:
+                if not is_device_oom(exc):
+                    raise
+                release()
+                if len(chunk) > 1:
+                    stats["oom_batch_splits"] += 1
+                    h = len(chunk) // 2
+                    work.appendleft(chunk[h:])
+                    work.appendleft(chunk[:h])
                 else:
-                    user = min(max(int(cfg.max_rank), 0), full)
-                    rcap[i] = user
-            uoff = np.concatenate([[0], np.cumsum(m_ * rcap)[:-1]]).astype(np.int64)
-            voff = np.concatenate([[0], np.cumsum(n_ * rcap)[:-1]]).astype(np.int64)
-            A = ops.alloc(int(sizes.sum()))
-            U = ops.alloc(int((m_ * rcap).sum()))
-            V = ops.alloc(int((n_ * rcap).sum()))
-            ops.fill([(idx(r), idx(c)) for r, c in chunk], A, offs)
+                    split_or_fallback(chunk[0][0], chunk[0][1], "gpu_oom_subdivisions", rejected)
+                continue
             stats["fill_batches"] += 1
-            stats["block_entry_evaluations"] += int(sizes.sum())
-            tasks = np.zeros(len(chunk), dtype=ACA_TASK_DTYPE)
-            tasks["a_off"], tasks["u_off"], tasks["v_off"] = offs, uoff, voff
-            tasks["m"], tasks["n"], tasks["rcap"] = m_, n_, rcap
-            ranks, rels, status = ops.aca(A, tasks, U, V, float(cfg.epsilon))
             stats["aca_batches"] += 1
-            acc_idx = []
+            stats["block_entry_evaluations"] += nent
+            pu = pv = 0
             for i, (r, c) in enumerate(chunk):
-                rank, rel = int(ranks[i]), float(rels[i])
-                if status[i] == 2 and storage_cap[i]:
-                    decision[key(r, c)] = ("dense",)
-                    rejected.append((r, c))
-                    stats["dense_blocks"] += 1
-                    stats["rejected_low_rank_blocks"] += 1
-                    continue
-                meets = np.isfinite(rel) and rel <= cfg.epsilon * margin
-                if rank > 0 and meets and (m_[i] + n_[i]) * rank < cfg.low_rank_storage_factor * sizes[i]:
-                    acc_idx.append(i)
+                f = fate[i]
+                if f == "far":
                     decision[key(r, c)] = ("far",)
-                    continue
-                if not meets and (_can_split(r) or _can_split(c)):
-                    stats["compression_tolerance_subdivisions"] += 1
-                    which, ch = _children(r, c)
-                    decision[key(r, c)] = ("split", which)
-                    nxt += ch
-                    continue
-                decision[key(r, c)] = ("dense",)
-                stats["dense_blocks"] += 1
-                stats["rejected_low_rank_blocks"] += 1
-                if status[i] != 3:
-                    rejected.append((r, c))
-            if acc_idx:
-                Uh = ops.gather(U, [(int(uoff[i]), int(uoff[i]) + int(m_[i]) * int(ranks[i])) for i in acc_idx])
-                Vh = ops.gather(V, [(int(voff[i]), int(voff[i]) + int(n_[i]) * int(ranks[i])) for i in acc_idx])
-                pu = pv = 0
-                for i in acc_idx:
-                    r, c = chunk[i]
                     mm, nn, rk = int(m_[i]), int(n_[i]), int(ranks[i])
                     Ub = np.ascontiguousarray(Uh[pu:pu + mm * rk].reshape(rk, mm).T)
                     Vb = Vh[pv:pv + nn * rk].reshape(rk, nn).copy()
@@ -330,8 +487,21 @@ def build_hmatrix_device(geo: BoundaryGeometry, config: HMatrixBuildConfig | Non
                     stats["low_rank_blocks"] += 1
                     stats["max_observed_rank"] = max(stats["max_observed_rank"], rk)
                     stats["sum_observed_rank"] += rk
-            del A, U, V
-        add_near(rejected)
+                elif f == "split":
+                    stats["compression_tolerance_subdivisions"] += 1
+                    split(r, c)
+                else:
+                    decision[key(r, c)] = ("dense",)
+                    stats["dense_blocks"] += 1
+                    stats["rejected_low_rank_blocks"] += 1
+                    if f == "reject":
+                        rejected.append((r, c))
+            if verbose and cfg.gpu_progress_interval_seconds > 0 and \
+                    time.perf_counter() - last_report[0] >= cfg.gpu_progress_interval_seconds:
+                last_report[0] = time.perf_counter()
+                print(f"[HMatrix GPU build] far={stats['low_rank_blocks']}, near={stats['dense_blocks']}, "
+                      f"entries={stats['block_entry_evaluations']:,}", flush=True)
+        add_near(rejected, fill_budget)
         queue = nxt
 
     # far blocks in the order of the depth-first recursion

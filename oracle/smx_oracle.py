@@ -30,6 +30,14 @@ How each part is pinned (see tests/test_oracle_*.py):
   tree / partition    permutation + interval invariants, leaf sizes,
                       admissibility by brute force
   fullaca_dense       ||A - UV||_F <= eps ||A||_F ; exact rank on rank-r input
+  truncated_svd       matrices of prescribed singular values: the rank is the
+                      first r with sqrt(sum_{i>r} s_i^2) <= rtol ||s||, the
+                      reported residual equals ||A - UV||_F / ||A||_F
+  randomized_svd      exact recovery of rank-r input; reported residual equals
+                      ||A - UV||_F / ||A||_F (the projection identity);
+                      adaptive growth from a too-small initial rank
+  build_hmatrix       workspace / OOM hooks: every far block fits, matvec
+                      against dense B
   hmatrix matvec      validate_against_dense protocol (seed 1234, 3 trials)
   cubic anisotropy    field = -(1/mu0 Ms) dE/dm (central FD of the energy);
                       closed forms at m along an axis / a face diagonal
@@ -704,6 +712,74 @@ def fullaca_dense(A, rtol=1e-6, max_rank=None):
     return np.column_stack(U), np.vstack(V), rel
 
 
+def _rank_from_singular_values(s, norm_sq, rtol, max_rank):
+    """First rank whose captured energy leaves a relative residual <= rtol,
+    else the largest usable rank (compressors.py:395-401, 466-474)."""
+    usable = min(int(max_rank), int(s.size))
+    captured = np.cumsum(s[:usable] ** 2)
+    rels = np.sqrt(np.maximum(norm_sq - captured, 0.0)) / math.sqrt(norm_sq)
+    ok = np.flatnonzero(rels <= rtol)
+    rank = int(ok[0] + 1) if ok.size else usable
+    rel = float(rels[rank - 1]) if rank else 1.0
+    return rank, rel, bool(ok.size)
+
+
+def truncated_svd(A, rtol=1e-6, max_rank=None):
+    """Truncated SVD compressor, device form (compressors.py:373-414): the
+    residual from ||A||_F^2 minus the captured singular energy."""
+    A = np.asarray(A, dtype=np.float64)
+    m, n = A.shape
+    max_rank = min(m, n) if max_rank is None else min(max(int(max_rank), 0), m, n)
+    norm_sq = float(np.sum(A * A))
+    if norm_sq == 0.0 or max_rank == 0:
+        return np.zeros((m, 0)), np.zeros((0, n)), 0.0
+    U, s, Vh = np.linalg.svd(A, full_matrices=False)
+    rank, rel, _ = _rank_from_singular_values(s, norm_sq, rtol, max_rank)
+    return U[:, :rank] * s[:rank][None, :], Vh[:rank].copy(), rel
+
+
+def randomized_svd(A, rtol=1e-6, max_rank=None, initial_rank=16, oversampling=8, power_iterations=1,
+                   random_seed=12345, omega=None):
+    """Adaptive randomized SVD (compressors.py:148-235, 417-492): sketch
+    Y = A Omega with target + oversampling columns, power iterations
+    Y <- A (A^T qr(Y)), Q = qr(Y), SVD of Q^T A; the target doubles until a
+    rank meets rtol or the sketch reaches max_rank.  omega(n, k) supplies the
+    test matrices (default: numpy's generator seeded with random_seed)."""
+    A = np.asarray(A, dtype=np.float64)
+    m, n = A.shape
+    max_rank = min(m, n) if max_rank is None else min(max(int(max_rank), 0), m, n)
+    initial_rank = max(1, int(initial_rank))
+    oversampling = max(0, int(oversampling))
+    norm_sq = float(np.sum(A * A))
+    if norm_sq == 0.0 or max_rank == 0:
+        return np.zeros((m, 0)), np.zeros((0, n)), 0.0
+    if omega is None:
+        rng = np.random.default_rng(int(random_seed))
+        omega = lambda nn, kk: rng.standard_normal((nn, kk))
+    target = min(initial_rank, max_rank)
+    while True:
+        k = min(max_rank, target + oversampling)
+        Y = A @ omega(n, k)
+        for _ in range(max(0, int(power_iterations))):
+            Q, _ = np.linalg.qr(Y)
+            Y = A @ (A.T @ Q)
+        Q, _ = np.linalg.qr(Y)
+        Ub, s, Vh = np.linalg.svd(Q.T @ A, full_matrices=False)
+        rank, rel, ok = _rank_from_singular_values(s, norm_sq, rtol, max_rank)
+        if ok or k >= max_rank:
+            break
+        target = min(max_rank, max(target + 1, 2 * target))
+    if rank == 0:
+        return np.zeros((m, 0)), np.zeros((0, n)), rel
+    return (Q @ Ub[:, :rank]) * s[:rank][None, :], Vh[:rank].copy(), rel
+
+
+def block_random_seed(r0, r1, c0, c1):
+    """Per-block seed of the randomized compressor (builder.py:170-183)."""
+    v = int(r0) * 73856093 ^ int(r1) * 19349663 ^ int(c0) * 83492791 ^ int(c1) * 2654435761
+    return int(v & 0x7FFFFFFF)
+
+
 @dataclass
 class HMatrixOracle:
     perm: np.ndarray
@@ -725,15 +801,29 @@ class HMatrixOracle:
 
 def build_hmatrix(bd: BoundaryData, eps=1e-6, eta=2.0, leaf_size=64,
                   max_temporary_block_bytes=256 * 1024 ** 2, storage_factor=0.95,
-                  max_rank=None, fill=None):
-    """HMatrixBuilder._build_block / _subdivide (builder.py:256-376) with
-    full-pivot ACA.  ``fill(rows, cols)`` defaults to block_B."""
+                  max_rank=None, fill=None, compressor="fullaca", workspace_fits=None, oom=None,
+                  fallback_dense=True):
+    """HMatrixBuilder._build_block / _subdivide (builder.py:256-376), and the
+    device builder's memory rules (builder.py:520-548, 630-760) when
+    ``workspace_fits(dense_bytes)`` / ``oom(m, n)`` are given: an admissible
+    block whose workspace does not fit, or whose fill/compression runs out of
+    memory, is subdivided (counted as workspace / OOM subdivisions), and kept
+    dense when it cannot split (``fallback_dense``; else MemoryError).
+    ``fill(rows, cols)`` defaults to block_B; ``compressor`` is "fullaca",
+    "svd" or a callable (A, rtol, max_rank, (r0, r1, c0, c1)) -> (U, V, rel)."""
     fill = fill or (lambda r, c: block_B(bd, r, c))
+    if compressor == "fullaca":
+        compress = lambda A, key: fullaca_dense(A, eps, max_rank)
+    elif compressor == "svd":
+        compress = lambda A, key: truncated_svd(A, eps, max_rank)
+    else:
+        compress = lambda A, key: compressor(A, eps, max_rank, key)
     perm, root = cluster_tree(bd.Xb, leaf_size)
     nr, nc, nd = [], [], []
     far = []
     stats = dict(dense_blocks=0, low_rank_blocks=0, rejected_low_rank_blocks=0,
-                 forced_subdivisions_memory=0, compression_tolerance_subdivisions=0)
+                 forced_subdivisions_memory=0, compression_tolerance_subdivisions=0,
+                 gpu_workspace_subdivisions=0, gpu_oom_subdivisions=0, fallback_dense_blocks=0)
     margin = 1.0 + 64.0 * np.finfo(np.float64).eps
 
     def add_dense(r, c, A=None):
@@ -758,15 +848,34 @@ def build_hmatrix(bd: BoundaryData, eps=1e-6, eta=2.0, leaf_size=64,
             build(r, c.left)
             build(r, c.right)
 
+    def split_or_fallback(r, c, key):
+        if not r.is_leaf or not c.is_leaf:
+            stats[key] += 1
+            subdivide(r, c)
+            return
+        if not fallback_dense:
+            raise MemoryError(f"unsplittable block {r.size}x{c.size}")
+        add_dense(r, c)
+        stats["fallback_dense_blocks"] += 1
+
     def build(r, c):
         adm = admissible(r, c, eta)
-        if adm and r.size * c.size * 8 > max_temporary_block_bytes:
+        nbytes = r.size * c.size * 8
+        if workspace_fits is not None and adm and (nbytes > max_temporary_block_bytes
+                                                   or not workspace_fits(nbytes)):
+            stats["forced_subdivisions_memory"] += 1
+            split_or_fallback(r, c, "gpu_workspace_subdivisions")
+            return
+        if adm and nbytes > max_temporary_block_bytes:
             stats["forced_subdivisions_memory"] += 1
             subdivide(r, c)
             return
+        if adm and oom is not None and oom(r.size, c.size):
+            split_or_fallback(r, c, "gpu_oom_subdivisions")
+            return
         if adm:
             A = fill(perm[r.start:r.stop], perm[c.start:c.stop])
-            U, V, rel = fullaca_dense(A, eps, max_rank)
+            U, V, rel = compress(A, (r.start, r.stop, c.start, c.stop))
             rank = U.shape[1]
             ok = np.isfinite(rel) and rel <= eps * margin
             if rank > 0 and ok and U.size + V.size < storage_factor * A.size:

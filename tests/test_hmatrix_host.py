@@ -1,10 +1,12 @@
 """H-matrix host logic on the CPU.
 
-The builder is driven through a host stand-in of the two device operations
-(batched Lindholm fill, batched full-pivot ACA).  The stand-in checks every
-task of the tables the builder produces against the buffer sizes the kernels
-would index, and evaluates the tasks with the oracle's Lindholm weights.  The
-resulting partition must equal the oracle's depth-first build.
+The builder is driven through a host stand-in of the device operations
+(batched Lindholm fill, batched compression, memory queries).  The stand-in
+checks every task of the tables the builder produces against the buffer sizes
+the kernels would index, and evaluates the tasks with the oracle's Lindholm
+weights and compressors.  The resulting partition must equal the oracle's
+depth-first build, also under the device builder's memory rules (a workspace
+limit, allocations that fail above a size) and with the SVD compressors.
 """
 import math
 
@@ -19,14 +21,51 @@ from smx.hmatrix.builder import DeviceBuildOps
 
 
 class HostOps:
-    def __init__(self, geo, bd):
+    def __init__(self, geo, bd, avail=None, alloc_limit=None):
         self.geo = geo
         self.bd = bd
         self.inc = geo.incidence()
         self.evaluations = 0
+        self.avail = avail
+        self.alloc_limit = alloc_limit
+        self.failed_allocs = 0
+
+    def available_bytes(self):
+        return float("inf") if self.avail is None else self.avail
 
     def alloc(self, n):
+        if self.alloc_limit is not None and 8 * int(n) > self.alloc_limit:
+            self.failed_allocs += 1
+            raise MemoryError("out of memory (host stand-in)")
         return np.full(max(int(n), 1), np.nan)
+
+    def compress(self, method, A, tasks, U, V, rtol, max_rank, seeds, cfg):
+        if method == "fullaca":
+            return self.aca(A, tasks, U, V, rtol)
+        n_t = tasks.size
+        ranks, rels, status = np.zeros(n_t, np.int32), np.zeros(n_t), np.zeros(n_t, np.int32)
+        for q, t in enumerate(tasks):
+            m, n, rcap = int(t["m"]), int(t["n"]), int(t["rcap"])
+            a0, u0, v0 = int(t["a_off"]), int(t["u_off"]), int(t["v_off"])
+            assert a0 + m * n <= A.size and u0 + rcap * m <= U.size and v0 + rcap * n <= V.size
+            R = A[a0:a0 + m * n].reshape(m, n)
+            if not np.any(R) or (max_rank is not None and max_rank == 0):
+                status[q] = 3
+                continue
+            if method == "svd":
+                Uf, Vf, rel = O.truncated_svd(R, rtol, max_rank)
+            else:
+                Uf, Vf, rel = O.randomized_svd(R, rtol, max_rank, cfg.gpu_rsvd_initial_rank,
+                                               cfg.gpu_rsvd_oversampling, cfg.gpu_rsvd_power_iterations,
+                                               random_seed=int(seeds[q]))
+            rk = Uf.shape[1]
+            ranks[q], rels[q] = rk, rel
+            if rk > rcap:
+                status[q] = 2
+                continue
+            U[u0:u0 + rk * m] = Uf.T.ravel()
+            V[v0:v0 + rk * n] = Vf.ravel()
+        return ranks, rels, status
 
     def host(self, t):
         return t
@@ -303,3 +342,91 @@ def test_pack_fp32_near_values(c1):
     ref = _pack(data)
     assert np.array_equal(pk["near_idx"], ref["near_idx"])
     assert np.array_equal(pk["near_val"][nz], ref["near_val"][nz].astype(np.float32))
+
+
+@pytest.mark.parametrize("compressor", ["svd", "rsvd"])
+def test_svd_compressors_partition_matches_oracle(c1, compressor):
+    _, geo, bd = c1
+    eps = 1e-6
+    cfg = HMatrixBuildConfig(epsilon=eps, leaf_size=32, compressor=compressor, batch_bytes=1 << 20)
+    data = build_hmatrix_device(geo, cfg, ops=HostOps(geo, bd))
+    if compressor == "svd":
+        comp = "svd"
+    else:
+        comp = lambda A, rtol, mr, k: O.randomized_svd(A, rtol, mr, random_seed=O.block_random_seed(*k))
+    ref = O.build_hmatrix(bd, eps=eps, leaf_size=32, compressor=comp)
+    got = [(b.row_start, b.row_stop, b.col_start, b.col_stop, b.rank) for b in data.far_blocks]
+    assert got == [(r0, r1, c0, c1, U.shape[1]) for r0, r1, c0, c1, U, V, _ in ref.far]
+    _same_near(data.near_csr, ref.near)
+    B = O.dense_B_rows(bd)
+    x = np.random.default_rng(5).standard_normal(bd.Nb)
+    assert np.linalg.norm(_matvec(data, x) - B @ x) / np.linalg.norm(B @ x) < 10 * eps
+
+
+def test_config_compressor_names():
+    assert HMatrixBuildConfig(compressor="randomized-svd").compressor == "rsvd"
+    assert HMatrixBuildConfig(compressor="ACA").compressor == "fullaca"
+    with pytest.raises(ValueError):
+        HMatrixBuildConfig(compressor="qr")
+    with pytest.raises(ValueError):
+        HMatrixBuildConfig(gpu_memory_fraction=0.0)
+
+
+def test_workspace_limit_subdivides_like_the_device_rule(c1):
+    """A workspace limit: admissible blocks whose dense bytes x 3.5 exceed
+    the available fraction split (counted), as in the oracle with the same
+    predicate; the operator stays within tolerance."""
+    _, geo, bd = c1
+    avail = 100_000.0
+    cfg = HMatrixBuildConfig(epsilon=1e-6, leaf_size=32, gpu_memory_fraction=0.5)
+    data = build_hmatrix_device(geo, cfg, ops=HostOps(geo, bd, avail=avail))
+    ref = O.build_hmatrix(bd, eps=1e-6, leaf_size=32, workspace_fits=lambda nb: nb * 3.5 <= avail * 0.5)
+    assert data.metadata["gpu_workspace_subdivisions"] == ref.stats["gpu_workspace_subdivisions"] > 0
+    assert data.metadata["forced_subdivisions_memory"] == ref.stats["forced_subdivisions_memory"]
+    got = [(b.row_start, b.row_stop, b.col_start, b.col_stop, b.rank) for b in data.far_blocks]
+    assert got == [(r0, r1, c0, c1, U.shape[1]) for r0, r1, c0, c1, U, V, _ in ref.far]
+    assert max(b.size for b in [type("x", (), {"size": (f.row_stop - f.row_start) * (f.col_stop - f.col_start)})
+                                for f in data.far_blocks]) * 8 * 3.5 <= avail * 0.5
+
+
+@pytest.mark.parametrize("limit", [6 * 1024, 2 * 1024])
+def test_oom_halves_batches_then_subdivides(c1, limit):
+    """Allocations above 6 KiB (2 KiB) fail: batches are halved, single
+    admissible blocks whose dense fill does not fit are subdivided (OOM
+    count), leaf blocks that do not fit are filled in row slabs; the partition equals the
+    oracle's with the same OOM predicate, and the operator stays within
+    tolerance."""
+    _, geo, bd = c1
+    ops = HostOps(geo, bd, alloc_limit=limit)
+    cfg = HMatrixBuildConfig(epsilon=1e-6, leaf_size=32)
+    data = build_hmatrix_device(geo, cfg, ops=ops)
+    assert ops.failed_allocs > 0
+    ref = O.build_hmatrix(bd, eps=1e-6, leaf_size=32, oom=lambda m, n: m * n * 8 > limit)
+    assert data.metadata["gpu_oom_subdivisions"] == ref.stats["gpu_oom_subdivisions"] > 0
+    assert data.metadata["fallback_dense_blocks"] == ref.stats["fallback_dense_blocks"]
+    assert data.metadata["oom_batch_splits"] > 0
+    if limit < 19 * 19 * 8:             # smaller than the smallest leaf block
+        assert data.metadata["slab_filled_blocks"] > 0 and data.metadata["fallback_dense_blocks"] > 0
+    else:
+        assert len(data.far_blocks) > 0
+    assert data.metadata["cpu_fallback_blocks"] == 0
+    got = [(b.row_start, b.row_stop, b.col_start, b.col_stop, b.rank) for b in data.far_blocks]
+    assert got == [(r0, r1, c0, c1, U.shape[1]) for r0, r1, c0, c1, U, V, _ in ref.far]
+    _same_near(data.near_csr, ref.near)
+    B = O.dense_B_rows(bd)
+    x = np.random.default_rng(6).standard_normal(bd.Nb)
+    assert np.linalg.norm(_matvec(data, x) - B @ x) / np.linalg.norm(B @ x) < 1e-5
+
+
+def test_unsplittable_without_fallback_raises(c1):
+    _, geo, bd = c1
+    cfg = HMatrixBuildConfig(epsilon=1e-6, leaf_size=32, max_temporary_block_bytes=1024, gpu_fallback_to_cpu=False)
+    with pytest.raises(MemoryError):
+        build_hmatrix_device(geo, cfg, ops=HostOps(geo, bd))
+
+
+def _same_near(a, b):
+    """Same sparsity pattern; values equal to the last bits (the stand-in sums
+    a tile's triangle contributions in another order than the oracle)."""
+    assert np.array_equal(a.indptr, b.indptr) and np.array_equal(a.indices, b.indices)
+    assert np.abs(a.data - b.data).max() < 1e-15
