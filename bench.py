@@ -294,6 +294,360 @@ def adjoint_through_time(steps=500, dt=1e-13, every=50, n_fd=2):
     print(json.dumps(out), flush=True)
 
 
+def _kernel_roofline(name, fn, nbytes, flush, reps=20):
+    """HIP-event time of one launch (Infinity Cache evicted before each) and
+    its HBM roofline from the algorithmic bytes."""
+    sec = event_time(fn, reps, flush=flush)
+    return {"kernel": name, "bound": "hbm", "avg_us": sec * 1e6, "algorithmic_bytes": int(nbytes),
+            "achieved": nbytes / sec / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": nbytes / sec / 1e9 / HBM_PEAK_GBS}
+
+
+def _cpu_local_blocks(m, Hm, kappa, gamma, alpha, Ms, shift):
+    """Vectorized inverse of shift I - J_i with J_i the exact local derivative
+    of the LLG RHS (field self-derivative kappa_i I), for the CPU port."""
+    import numpy as np
+    m3, h3 = m.reshape(-1, 3), Hm.reshape(-1, 3)
+    n = m3.shape[0]
+    c1 = gamma / (1 + alpha ** 2)
+    c2 = c1 * alpha
+    S = Ms * gamma / (1 + alpha ** 2) * 0.5
+
+    def skew(v):
+        K = np.zeros((v.shape[0], 3, 3))
+        K[:, 0, 1], K[:, 0, 2] = -v[:, 2], v[:, 1]
+        K[:, 1, 0], K[:, 1, 2] = v[:, 2], -v[:, 0]
+        K[:, 2, 0], K[:, 2, 1] = -v[:, 1], v[:, 0]
+        return K
+
+    mm = np.einsum("ni,ni->n", m3, m3)
+    mh = np.einsum("ni,ni->n", m3, h3)
+    I3 = np.eye(3)[None]
+    outer_mm = np.einsum("ni,nj->nij", m3, m3)
+    B = np.einsum("ni,nj->nij", m3, h3) - 2.0 * np.einsum("ni,nj->nij", h3, m3)
+    diag = mh[:, None] - h3 * m3
+    idx = np.arange(3)
+    B[:, idx, idx] = diag
+    k = kappa[:, None, None]
+    dprec = -skew(h3) + k * skew(m3)
+    ddamp = B + k * (outer_mm - mm[:, None, None] * I3)
+    stab = S * ((1.0 - mm)[:, None, None] * I3 - 2.0 * outer_mm)
+    J = -c1 * dprec - c2 * ddamp + stab
+    return np.linalg.inv((shift + 1e-14) * I3 - J), J
+
+
+def _cpu_bdf_port(mesh, mat, B_host, m0, t_final, dt0, budget_s, tol=1e-6, snes_rtol=1e-2, snes_atol=1e-4,
+                  ksp_rtol=1e-4):
+    """Host port of BDFIntegrator + GMRES + exact block preconditioner on the
+    FP64 oracle operators (splu Poisson solves, dense B), run for at most
+    budget_s seconds.  Returns (steps, elapsed, t reached, counters)."""
+    import math
+    import numpy as np
+    import scipy.sparse.linalg as spla
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import smx_oracle as O
+    Ms, A_ex, alpha, Ku = mat["Ms"], mat["Aex"], mat["alpha"], mat["Ku"]
+    nax = np.asarray(mat["n_ani_vec"], dtype=float)
+    fem = O.fem_operators(mesh.X, mesh.tets, Ms)
+    Kex = O.exchange_matrix(mesh.X, mesh.tets, fem.V, A_ex, Ms).tocsr()
+    bd = O.boundary_data(mesh.X, mesh.tets)
+    lu1 = spla.splu(_pinned(fem.A).tocsc())
+    lu2 = spla.splu(fem.A_bc.tocsc())
+    Vn = np.repeat(fem.V, 3)
+    pref = 2 * Ku / (O.MU0 * Ms)
+    kappa = (Kex.diagonal().reshape(-1, 3) + pref * nax ** 2).mean(axis=1)
+    cnt = dict(rhs=0, jv=0, newton=0, gmres=0)
+
+    def h_lin(v):
+        return Kex @ v + O.anisotropy_field(v, nax, Ku, Ms)
+
+    def rhs(y):
+        cnt["rhs"] += 1
+        b = fem.Div @ y
+        b -= b.mean()
+        u1 = _pin_solve(lu1, b)
+        u1 -= u1.mean()
+        g = np.zeros(fem.N)
+        g[bd.bverts] = B_host @ u1[bd.bverts]
+        bb = -(fem.A @ g)
+        bb[fem.bnodes] = g[fem.bnodes]
+        u2 = lu2.solve(bb)
+        H = h_lin(y) + (fem.G @ (u1 + u2)) / Vn
+        return O.llg_rhs(y, H, GAMMA, alpha, Ms)
+
+    def gmres(mult, pc, b, rtol, max_it=200, restart=30):
+        x = np.zeros_like(b)
+        bn = np.linalg.norm(b)
+        r = b - mult(x)
+        beta = np.linalg.norm(r)
+        its = 0
+        while beta > rtol * bn and its < max_it:
+            Vk = [r / beta]
+            Z = []
+            Hh = np.zeros((restart + 1, restart))
+            for j in range(restart):
+                Z.append(pc(Vk[j]))
+                w = mult(Z[j])
+                for _ in range(2):
+                    Vm = np.array(Vk)
+                    h = Vm @ w
+                    w = w - h @ Vm
+                    Hh[:j + 1, j] += h
+                Hh[j + 1, j] = np.linalg.norm(w)
+                Vk.append(w / Hh[j + 1, j] if Hh[j + 1, j] > 0 else w)
+                its += 1
+                e1 = np.zeros(j + 2)
+                e1[0] = beta
+                yls = np.linalg.lstsq(Hh[:j + 2, :j + 1], e1, rcond=None)[0]
+                res = np.linalg.norm(Hh[:j + 2, :j + 1] @ yls - e1)
+                if res <= rtol * bn or its >= max_it:
+                    break
+            x = x + np.array(Z).T @ yls
+            r = b - mult(x)
+            beta = np.linalg.norm(r)
+        cnt["gmres"] += its
+        return x
+
+    y = m0.copy()
+    hist, times = [y.copy()], [0.0]
+    t, dt = 0.0, dt0
+    steps = 0
+    t_start = time.perf_counter()
+    while t < t_final and time.perf_counter() - t_start < budget_s:
+        dt = min(dt, t_final - t)
+        q = 2 if len(hist) >= 2 else 1
+        if q == 2:
+            w = dt / (times[0] - times[1])
+            a0, a1, a2 = (1 + 2 * w) / (1 + w), -(1 + w), w * w / (1 + w)
+            shift = a0 / dt
+            chist = (-a1 * hist[0] - a2 * hist[1]) / dt
+        else:
+            shift = 1.0 / dt
+            chist = hist[0] / dt
+        tn = times[:3]
+        tq = t + dt
+        lag = [math.prod((tq - tn[j]) / (tn[i] - tn[j]) for j in range(len(tn)) if j != i)
+               for i in range(len(tn))]
+        ypred = sum(c * h for c, h in zip(lag, hist[:len(tn)]))
+        yv = ypred.copy()
+        F = shift * yv - chist - rhs(yv)
+        Fn = np.linalg.norm(F)
+        tol_n = max(snes_atol, snes_rtol * Fn)
+        ok = False
+        for it in range(8):
+            if Fn <= tol_n and it > 0:
+                ok = True
+                break
+            cnt["newton"] += 1
+            Hm = h_lin(yv)
+            inv, _ = _cpu_local_blocks(yv, Hm, kappa, GAMMA, alpha, Ms, shift)
+
+            def mult(v, yv=yv, Hm=Hm):
+                cnt["jv"] += 1
+                return shift * v - O.jac_vec(yv, Hm, v, h_lin(v), GAMMA, alpha, Ms)
+
+            pc = lambda v: np.einsum("nij,nj->ni", inv, v.reshape(-1, 3)).ravel()
+            d = gmres(mult, pc, -F, ksp_rtol)
+            lam = 1.0
+            for _ in range(5):
+                yt = yv + lam * d
+                Ft = shift * yt - chist - rhs(yt)
+                Ftn = np.linalg.norm(Ft)
+                if Ftn <= (1 - 1e-4 * lam) * Fn or lam < 0.1:
+                    break
+                lam *= 0.5
+            yv, F, Fn = yt, Ft, Ftn
+            if Fn <= tol_n:
+                ok = True
+                break
+        if not ok:
+            dt *= 0.25
+            continue
+        if len(tn) >= q + 1:
+            coef = 2.0 / 11.0 if q == 2 else 0.5
+            sc = tol + tol * np.maximum(np.abs(hist[0]), np.abs(yv))
+            err = math.sqrt(np.mean((coef * (yv - ypred) / sc) ** 2))
+        else:
+            err = 0.0
+        if err <= 1.0:
+            hist = [yv.copy()] + hist[:2]
+            times = [t + dt] + times[:2]
+            t += dt
+            steps += 1
+            fac = 0.9 * (1.0 / max(err, 1e-16)) ** (1.0 / (q + 1)) if err > 0 else 3.0
+            dt = min(max(dt * min(max(fac, 0.1), 3.0), 1e-17), 1e-10)
+        else:
+            dt = max(dt * min(max(0.9 * (1.0 / err) ** (1.0 / (q + 1)), 0.1), 1.0), 1e-17)
+    return steps, time.perf_counter() - t_start, t, cnt
+
+
+def extended_paths(cpu_budget=20.0, no_cpu=False):
+    """Measurements of the paths beside the demag + RHS evaluation: torque
+    terms in the fused field kernel, the implicit BDF Newton-Krylov relax and
+    its kernels, the minimizer, and H-matrix construction per compressor."""
+    import numpy as np
+    import torch
+    from smx import mesh as M
+    from smx.demag import OperatorSet
+    from smx.implicit import GMRES, JvContext, relax_implicit
+    from smx.llg import EffectiveField
+    from smx.minimizer import LaBonteBBMinimizerGPU
+    dev = torch.device("cuda", 0)
+    flush = torch.empty(64 * 1024 * 1024, dtype=torch.float64, device=dev)
+    flush.fill_(1.0)
+    out = {"metric": "extended paths: torque RHS, implicit relax, minimizer, H-matrix build", "n_gpus": 1,
+           "data": "synthetic (seeded)"}
+
+    # ---- torque terms in the fused field kernel (film, no demag) ----
+    film = M.film_c3()
+    ops = OperatorSet(film, dev)
+    base = dict(Ms=8e5, Aex=1.3e-11, Ku=5e5, n_ani_vec=(0.0, 0.0, 1.0), D_int=3e-3, n0_int_vec=(0.0, 0.0, 1.0),
+                alpha=0.5)
+    extra_terms = dict(Kc1=-4.5e4, u1_cub=(1.0, 0.0, 0.0), u2_cub=(0.0, 1.0, 0.0), Jmagnitude=1e12,
+                       Jdir_vec=(1.0, 0.0, 0.0), P=0.6, beta=0.04, H_dl=2e4, H_fl=-7e3,
+                       polarization_vec=(0.0, 1.0, 0.0))
+    hb = EffectiveField(film, operators=ops, device=dev, **base)
+    ha = EffectiveField(film, operators=ops, device=dev, **base, **extra_terms)
+    N = film.num_nodes
+    nnz = int(ops.graph.nnz)
+    m = torch.from_numpy(random_m(N, seed=1)).to(dev)
+    r = torch.empty_like(m)
+    v = torch.from_numpy(random_m(N, seed=2)).to(dev)
+    jv = torch.empty_like(m)
+    rb = _kernel_roofline("field_rhs_kernel (exchange, DMI, uniaxial)", lambda: hb.llg_rhs_vec(m, r),
+                          nnz * (4 + 8 + 72) + N * 72, flush)
+    ra = _kernel_roofline("field_rhs_kernel (+ cubic, Zhang-Li STT, SOT)", lambda: ha.llg_rhs_vec(m, r),
+                          nnz * (4 + 8 + 72 + 8) + N * 72, flush)
+    ha.update_jac_state(m)
+    rj = _kernel_roofline("llg_jvp_kernel (all terms)", lambda: ha.jac_vec_times_vec(v, jv),
+                          nnz * (4 + 8 + 72 + 8) + N * 96, flush)
+    tor = {"workload": f"film500x500x10nm (N={N}, graph nnz {nnz}), no demag", "rhs_base": rb, "rhs_all_terms": ra,
+           "jvp_all_terms": rj, "value": 1.0 / (ra["avg_us"] * 1e-6), "unit": "field+RHS evals/s (all terms)"}
+    if not no_cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import smx_oracle as O
+        log("torques: host operator assembly for the CPU port")
+        t0 = time.perf_counter()
+        fem = O.fem_operators(film.X, film.tets, 8e5)
+        Kl = (O.exchange_matrix(film.X, film.tets, fem.V, 1.3e-11, 8e5)
+              + O.dmi_interfacial_matrix(film.X, film.tets, fem.V, 3e-3, 8e5, (0.0, 0.0, 1.0))).tocsr()
+        KJ = O.zhang_li_matrix(film.X, film.tets, fem.V, np.tile([1.0, 0.0, 0.0], (N, 1))).tocsr()
+        axes = O.cubic_axes_oracle((1.0, 0.0, 0.0), (0.0, 1.0, 0.0))
+        setup = time.perf_counter() - t0
+        mh = m.cpu().numpy()
+        n_ev, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < min(cpu_budget, 10.0) and n_ev < 200:
+            H = Kl @ mh + O.anisotropy_field(mh, (0.0, 0.0, 1.0), 5e5, 8e5) + O.cubic_field(mh, axes, -4.5e4, 8e5)
+            Z = np.column_stack([KJ @ mh.reshape(-1, 3)[:, k] for k in range(3)]).ravel()
+            O.llg_rhs_torques(mh, H, GAMMA, 0.5, 8e5, Z=Z, stt=(1e12, 0.6, 0.04), p=(0.0, 1.0, 0.0),
+                              H_dl=2e4, H_fl=-7e3)
+            n_ev += 1
+        el = time.perf_counter() - t0
+        tor["cpu_baseline"] = {"value": n_ev / el, "unit": "field+RHS evals/s (all terms)", "cores": 1,
+                               "kind": "port", "sample": f"{n_ev} oracle evaluations (scipy sparse + numpy) in "
+                                                         f"{el:.1f}s, operator assembly {setup:.1f}s excluded"}
+    out["torques"] = tor
+    log(f"torques: rhs {ra['avg_us']:.1f} us ({ra['frac']:.2f} of HBM), jvp {rj['avg_us']:.1f} us")
+
+    # ---- implicit-path kernels at film size ----
+    ctx = JvContext(ha)
+    ctx.update_pc_full_fast_gpu(2e12, exact=True, use_abs_kappa=False)
+    y = torch.empty_like(m)
+    kr = {"pc_build": _kernel_roofline("pc_build_kernel", lambda: ctx.update_pc_full_fast_gpu(
+              2e12, exact=True, use_abs_kappa=False), nnz * 4 + N * (8 + 72 + 24 + 24 + 72), flush),
+          "block3_apply": _kernel_roofline("block3_apply_kernel", lambda: ctx.apply(v, y), N * 120, flush)}
+    g = GMRES(3 * N, dev)
+    from smx import _capi
+    L = _capi.lib()
+    k = 31
+    g.V[:k].normal_()
+    kr["multi_dot_31"] = _kernel_roofline(
+        "multi_dot_kernel + reduce (31 vectors)",
+        lambda: _capi.check(L.smx_multi_dot(3 * N, k, g.V.data_ptr(), 3 * N, v.data_ptr(), g.h_dev.data_ptr(),
+                                            g.work.data_ptr(), _capi.stream_ptr())), (k + 1) * 8 * 3 * N, flush)
+    coef = torch.randn(k, dtype=torch.float64, device=dev)       # one coefficient per vector
+    assert g.V.shape[0] >= k and g.h_dev.numel() >= k and g.V.shape[1] == 3 * N
+    kr["lincomb_31"] = _kernel_roofline(
+        "lincomb_kernel (31 vectors)",
+        lambda: _capi.check(L.smx_lincomb(3 * N, k, g.V.data_ptr(), 3 * N, coef.data_ptr(), 1.0, y.data_ptr(),
+                                          _capi.stream_ptr())), (k + 2) * 8 * 3 * N, flush)
+    del g, coef
+
+    # ---- implicit relax (150 nm cube, dense FP64 B) ----
+    cube = M.cube_c2()
+    mat = dict(Ms=8e5, Aex=1.3e-11, Ku=5e5, n_ani_vec=(0.0, 0.0, 1.0), alpha=0.5)
+    hc = EffectiveField(cube, use_demag=True, device=dev, demag_kwargs=dict(
+        B_dtype="float64", ksp_rtol_u1=1e-6, ksp_rtol_u2=1e-6, ksp_atol=1e-12), **mat)
+    m0 = vortex_m(cube.X, seed=0)
+    t_final = 50e-12
+    torch.cuda.synchronize()
+    st = relax_implicit(hc, t_final, dt_init=1e-14, m0_array=m0)
+    torch.cuda.synchronize()
+    imp = {"workload": "cube150nm_30x30x30_kuhn (N=29791, Nb=5402, dense FP64 B), exchange + uniaxial + demag, "
+                       "alpha 0.5, vortex start, BDF to 50 ps (ts_rtol = ts_atol = 1e-6, snes_rtol 1e-2, "
+                       "ksp_rtol 1e-4, exact block preconditioner)",
+           "stats": {k2: (None if isinstance(v2, float) and v2 != v2 else v2) for k2, v2 in st.items()},
+           "value": st["steps"] / st["elapsed_s"], "unit": "BDF steps/s", "kernels_at_film_size": kr}
+    if not no_cpu:
+        B_host = hc.demag_field.B_dense_gpu.cpu().numpy()[:, :hc.demag_field.opB.Nb]
+        steps_c, el_c, t_c, cnt = _cpu_bdf_port(cube, mat, B_host, m0, t_final, 1e-14, cpu_budget)
+        imp["cpu_baseline"] = {"value": steps_c / el_c, "unit": "BDF steps/s", "cores": 1, "kind": "port",
+                               "sample": f"first {steps_c} BDF steps (to t = {t_c:.3e} s) of the same run by a host "
+                                         f"port (splu Poisson solves, numpy GMRES and block preconditioner) in "
+                                         f"{el_c:.1f}s; {cnt}"}
+    out["implicit"] = imp
+    log(f"implicit: {st['steps']} steps in {st['elapsed_s']:.2f}s ({imp['value']:.0f} steps/s)")
+
+    # ---- minimizer (150 nm cube with demag) ----
+    hc.set_m_from_cpu(random_m(cube.num_nodes, seed=5))
+    mz = LaBonteBBMinimizerGPU(hc)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = mz.minimize(max_iter=400, tol=1e-3)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    mr = {"projected_direction": _kernel_roofline("projdir_kernel", lambda: _capi.check(L.smx_projected_direction(
+              N, m.data_ptr(), v.data_ptr(), y.data_ptr(), _capi.stream_ptr())), N * 72, flush),
+          "cayley_step": _kernel_roofline("cayley_kernel", lambda: _capi.check(L.smx_cayley_step(
+              N, m.data_ptr(), v.data_ptr(), 0.1, y.data_ptr(), _capi.stream_ptr())), N * 72, flush)}
+    out["minimizer"] = {"workload": "cube150nm_30x30x30_kuhn with demag, random start, LaBonte/BB adaptive, "
+                                    "400 iterations at most", "iterations": res["iterations"],
+                        "stop_reason": res["stop_reason"], "elapsed_s": el,
+                        "value": res["iterations"] / el, "unit": "minimizer iterations/s",
+                        "kernels_at_film_size": mr}
+    log(f"minimizer: {res['iterations']} iterations in {el:.2f}s")
+    del hc
+
+    # ---- H-matrix construction per compressor (150 nm cube) ----
+    from smx.boundary import build_boundary_geometry
+    from smx.demag import assemble_dense_B
+    from smx.hmatrix import HMatrixBuildConfig, HMatrixGPUPackedFused, build_hmatrix_device
+    geo = build_boundary_geometry(cube.X, cube.tets)
+    Bd, _ = assemble_dense_B(geo, dev)
+    Bh = Bd[:, :geo.Nb]
+    hb_out = {}
+    for comp in ("fullaca", "svd", "rsvd"):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        data = build_hmatrix_device(geo, HMatrixBuildConfig(epsilon=1e-6, leaf_size=64, compressor=comp), dev,
+                                    verbose=True)
+        torch.cuda.synchronize()
+        tb = time.perf_counter() - t0
+        Hm_ = HMatrixGPUPackedFused(data, use_fp32=False)
+        x = torch.from_numpy(np.random.default_rng(1234).standard_normal(geo.Nb)).to(dev)
+        ref = Bh @ x
+        err = float(torch.linalg.norm(Hm_.matvec(x) - ref) / torch.linalg.norm(ref))
+        md = data.metadata
+        hb_out[comp] = {"build_s": tb, "far_blocks": md["low_rank_blocks"], "dense_blocks": md["dense_blocks"],
+                        "average_far_rank": md["average_far_rank"], "far_storage_entries": md["far_storage_entries"],
+                        "matvec_rel_err_vs_dense": err}
+        log(f"hmatrix {comp}: build {tb:.2f}s far {md['low_rank_blocks']} rank {md['average_far_rank']:.1f} "
+            f"err {err:.2e}")
+        del Hm_, data
+    out["hmatrix_build"] = {"workload": "cube150nm (Nb=5402), leaf 64, eta 2, eps 1e-6", **hb_out}
+    print(json.dumps(out), flush=True)
+
+
 def measured_traffic(config, what="matvec"):
     """HBM-side bytes per launch (or per boundary matvec) from the committed
     counter passes (FETCH_SIZE and WRITE_SIZE of `bench.py --pmc-probe WHAT`,
@@ -554,7 +908,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", choices=["film", "cube", "adjoint", "disk"], default="film")
+    ap.add_argument("--config", choices=["film", "cube", "adjoint", "disk", "paths"], default="film")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pc", choices=["gamg", "jacobi", "auto"], default="auto",
                     help="CG preconditioner of both Poisson solves (the reference uses GAMG; auto: the faster "
@@ -582,6 +936,9 @@ def main():
         return
     if args.config == "disk":
         energy_gradient_disk()
+        return
+    if args.config == "paths":
+        extended_paths(args.cpu_budget, args.no_cpu)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))

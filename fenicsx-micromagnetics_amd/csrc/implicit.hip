@@ -141,43 +141,72 @@ __global__ __launch_bounds__(kBlock) void block3_apply_kernel(int64_t n, const d
   y[3 * k + 2] = o[6] * x0 + o[7] * x1 + o[8] * x2;
 }
 
-// part[j*G + b] = sum over block b's entries of V_j . w (j < k), fixed order.
+// part[j*G + b] = sum over workgroup b's entries of V_j . w (j < k), fixed
+// order.  Each thread keeps kMdEpt entries of w in registers and sums kMdJ
+// vectors at a time, so a workgroup needs one barrier per kMdJ vectors
+// (double-buffered LDS) instead of two per vector.
+constexpr int kMdEpt = 1;
+constexpr int kMdJ = 32;
+constexpr int kMdSpan = kBlock * kMdEpt;
+
 __global__ __launch_bounds__(kBlock) void multi_dot_kernel(int64_t len, int32_t k, const double* __restrict__ V,
                                                            int64_t ldv, const double* __restrict__ w,
                                                            double* __restrict__ part) {
-  __shared__ double scr[kBlock / 64];
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  const double wi = i < len ? w[i] : 0.0;
-  for (int32_t j = 0; j < k; ++j) {
-    double v = i < len ? V[j * ldv + i] * wi : 0.0;
-    v = smx::wave_total(v);
-    if ((threadIdx.x & 63) == 0) scr[threadIdx.x >> 6] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double s = 0.0;
-      for (int q = 0; q < kBlock / 64; ++q) s += scr[q];
-      part[(int64_t)j * gridDim.x + blockIdx.x] = s;
+  __shared__ double scr[2][kMdJ][kBlock / 64];
+  const int64_t base = (int64_t)blockIdx.x * kMdSpan + threadIdx.x;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double wv[kMdEpt];
+#pragma unroll
+  for (int q = 0; q < kMdEpt; ++q) {
+    const int64_t i = base + (int64_t)q * kBlock;
+    wv[q] = i < len ? w[i] : 0.0;
+  }
+  int buf = 0;
+  for (int32_t j0 = 0; j0 < k; j0 += kMdJ, buf ^= 1) {
+    double acc[kMdJ];
+#pragma unroll
+    for (int jj = 0; jj < kMdJ; ++jj) {
+      acc[jj] = 0.0;
+      if (j0 + jj < k) {
+        const double* __restrict__ Vj = V + (int64_t)(j0 + jj) * ldv;
+#pragma unroll
+        for (int q = 0; q < kMdEpt; ++q) {
+          const int64_t i = base + (int64_t)q * kBlock;
+          if (i < len) acc[jj] = fma(Vj[i], wv[q], acc[jj]);
+        }
+      }
+    }
+#pragma unroll
+    for (int jj = 0; jj < kMdJ; ++jj) {
+      const double t = smx::wave_total(acc[jj]);
+      if (lane == 0) scr[buf][jj][wave] = t;
     }
     __syncthreads();
+    if (threadIdx.x < kMdJ && j0 + (int)threadIdx.x < k) {
+      double sum = 0.0;
+#pragma unroll
+      for (int q = 0; q < kBlock / 64; ++q) sum += scr[buf][threadIdx.x][q];
+      part[(int64_t)(j0 + threadIdx.x) * gridDim.x + blockIdx.x] = sum;
+    }
+    // the other LDS buffer is written next: no second barrier needed
   }
 }
 
+// out[j] = sum over b of part[j*G + b]: one workgroup per j, fixed order.
 __global__ __launch_bounds__(kBlock) void multi_dot_reduce_kernel(int32_t k, int64_t G,
                                                                   const double* __restrict__ part,
                                                                   double* __restrict__ out) {
   __shared__ double scr[kBlock / 64];
-  for (int32_t j = 0; j < k; ++j) {
-    double v = 0.0;
-    for (int64_t b = threadIdx.x; b < G; b += kBlock) v += part[j * G + b];
-    v = smx::wave_total(v);
-    if ((threadIdx.x & 63) == 0) scr[threadIdx.x >> 6] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double s = 0.0;
-      for (int q = 0; q < kBlock / 64; ++q) s += scr[q];
-      out[j] = s;
-    }
-    __syncthreads();
+  const int32_t j = blockIdx.x;
+  double v = 0.0;
+  for (int64_t b = threadIdx.x; b < G; b += kBlock) v += part[(int64_t)j * G + b];
+  v = smx::wave_total(v);
+  if ((threadIdx.x & 63) == 0) scr[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int q = 0; q < kBlock / 64; ++q) s += scr[q];
+    out[j] = s;
   }
 }
 
@@ -220,7 +249,7 @@ extern "C" int smx_block3_apply(int64_t n, const double* inv, const double* x, d
 }
 
 extern "C" int64_t smx_multi_dot_ws_size(int64_t len, int32_t k) {
-  return (int64_t)smx::grid_for(len, kBlock) * (k > 0 ? k : 1);
+  return (int64_t)smx::grid_for(len, kMdSpan) * (k > 0 ? k : 1);
 }
 
 extern "C" int smx_multi_dot(int64_t len, int32_t k, const double* V, int64_t ldv, const double* w,
@@ -229,10 +258,15 @@ extern "C" int smx_multi_dot(int64_t len, int32_t k, const double* V, int64_t ld
   SMX_REQUIRE((k == 0 || (V && w)) && out && work, SMX_ERR_VALUE, "null pointer argument");
   if (k == 0) return SMX_OK;
   hipStream_t st = smx::as_stream(stream);
-  const int G = smx::grid_for(len, kBlock);
+  if (len == 0) {
+    hipLaunchKernelGGL(multi_dot_reduce_kernel, dim3((unsigned)k), dim3(kBlock), 0, st, k, (int64_t)0, work, out);
+    SMX_LAUNCH_CHECK();
+    return SMX_OK;
+  }
+  const int G = smx::grid_for(len, kMdSpan);
   hipLaunchKernelGGL(multi_dot_kernel, dim3(G), dim3(kBlock), 0, st, len, k, V, ldv, w, work);
   SMX_LAUNCH_CHECK();
-  hipLaunchKernelGGL(multi_dot_reduce_kernel, dim3(1), dim3(kBlock), 0, st, k, (int64_t)G, work, out);
+  hipLaunchKernelGGL(multi_dot_reduce_kernel, dim3((unsigned)k), dim3(kBlock), 0, st, k, (int64_t)G, work, out);
   SMX_LAUNCH_CHECK();
   return SMX_OK;
 }

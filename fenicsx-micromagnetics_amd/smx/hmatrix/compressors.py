@@ -24,7 +24,7 @@ import math
 import numpy as np
 import torch
 
-STATUS_STORED, STATUS_OVER_CAP, STATUS_ZERO = 0, 2, 3
+STATUS_STORED, STATUS_OVER_CAP, STATUS_ZERO, STATUS_FAILED = 0, 2, 3, 4
 
 
 def block_random_seed(r0: int, r1: int, c0: int, c1: int) -> int:
@@ -67,8 +67,32 @@ def _max_rank(m, n, max_rank):
     return min(m, n) if max_rank is None else min(max(int(max_rank), 0), m, n)
 
 
+def _svd(X):
+    """Batched SVD as (U, s, Vh, s on the host) per element (one host copy of
+    the singular values per batch); where the library's iteration does not
+    converge for some element, the batch is redone element by element and a
+    failing element comes back as None."""
+    try:
+        Us, s, Vh = torch.linalg.svd(X, full_matrices=False)
+        sh = s.cpu().numpy()
+        return [(Us[b], s[b], Vh[b], sh[b]) for b in range(X.shape[0])]
+    except RuntimeError:
+        out = []
+        for b in range(X.shape[0]):
+            try:
+                u, sv, vh = torch.linalg.svd(X[b], full_matrices=False)
+                out.append((u, sv, vh, sv.cpu().numpy()))
+            except RuntimeError:
+                out.append(None)
+        return out
+
+
 def svd_batched(A, tasks, U, V, rtol: float, max_rank=None):
-    """Truncated SVD of every task's block (compressors.py:373-414)."""
+    """Truncated SVD of every task's block (compressors.py:373-414).  Zero
+    blocks never reach the SVD (the library reports non-convergence on an
+    exactly zero matrix); a block whose SVD does not converge is returned
+    with an infinite residual (status 4), so the builder subdivides it or
+    keeps it dense."""
     nt = int(tasks.size)
     ranks = np.zeros(nt, np.int32)
     rels = np.zeros(nt)
@@ -76,20 +100,26 @@ def svd_batched(A, tasks, U, V, rtol: float, max_rank=None):
     for (m, n), ids in _groups(tasks, range(nt)).items():
         X = _stack(A, tasks, ids, m, n)
         norm_sq = (X * X).sum(dim=(1, 2)).cpu().numpy()
-        Us, s, Vh = torch.linalg.svd(X, full_matrices=False)
-        s_h = s.cpu().numpy()
-        for b, i in enumerate(ids):
-            mr = _max_rank(m, n, max_rank)
-            if norm_sq[b] == 0.0 or mr == 0:
-                ranks[i], rels[i], status[i] = 0, 0.0, STATUS_ZERO
+        mr = _max_rank(m, n, max_rank)
+        live = [b for b in range(len(ids)) if norm_sq[b] > 0.0 and mr > 0]
+        for b in sorted(set(range(len(ids))) - set(live)):
+            ranks[ids[b]], rels[ids[b]], status[ids[b]] = 0, 0.0, STATUS_ZERO
+        if not live:
+            continue
+        res = _svd(X[live] if len(live) < len(ids) else X)
+        for q, b in enumerate(live):
+            i = ids[b]
+            if res[q] is None:
+                ranks[i], rels[i], status[i] = 0, math.inf, STATUS_FAILED
                 continue
-            rank, rel, _ = _rank_rule(s_h[b], float(norm_sq[b]), rtol, mr)
+            Ub_, sb_, Vhb_, sh_ = res[q]
+            rank, rel, _ = _rank_rule(sh_, float(norm_sq[b]), rtol, mr)
             ranks[i], rels[i] = rank, rel
             if rank > int(tasks["rcap"][i]):
                 status[i] = STATUS_OVER_CAP
                 continue
-            _store(U, V, tasks, i, m, n, Us[b, :, :rank] * s[b, :rank][None, :], Vh[b, :rank, :], rank)
-        del X, Us, s, Vh
+            _store(U, V, tasks, i, m, n, Ub_[:, :rank] * sb_[:rank][None, :], Vhb_[:rank, :], rank)
+        del X, res
     return ranks, rels, status
 
 
@@ -141,10 +171,13 @@ def rsvd_batched(A, tasks, U, V, rtol: float, seeds, max_rank=None, initial_rank
                 Y = torch.matmul(X, torch.matmul(Xt, Q))
             Q, _ = torch.linalg.qr(Y, mode="reduced")
             Bm = torch.matmul(Q.transpose(1, 2).contiguous(), X)
-            Ub, s, Vh = torch.linalg.svd(Bm, full_matrices=False)
-            s_h = s.cpu().numpy()
+            res = _svd(Bm)
             for b, i in enumerate(ids):
-                rank, rel, ok = _rank_rule(s_h[b], norm_sq[i], rtol, mr[i])
+                if res[b] is None:
+                    ranks[i], rels[i], status[i] = 0, math.inf, STATUS_FAILED
+                    continue
+                Ub_, sb_, Vhb_, sh_ = res[b]
+                rank, rel, ok = _rank_rule(sh_, norm_sq[i], rtol, mr[i])
                 if not ok and k < mr[i]:
                     target[i] = min(mr[i], max(target[i] + 1, 2 * target[i]))
                     nxt.append(i)
@@ -153,9 +186,9 @@ def rsvd_batched(A, tasks, U, V, rtol: float, seeds, max_rank=None, initial_rank
                 if rank > int(tasks["rcap"][i]):
                     status[i] = STATUS_OVER_CAP
                     continue
-                Uf = torch.matmul(Q[b], Ub[b, :, :rank]) * s[b, :rank][None, :]
-                _store(U, V, tasks, i, m, n, Uf, Vh[b, :rank, :], rank)
-            del X, Xt, Om, Y, Q, Bm, Ub, s, Vh
+                Uf = torch.matmul(Q[b], Ub_[:, :rank]) * sb_[:rank][None, :]
+                _store(U, V, tasks, i, m, n, Uf, Vhb_[:rank, :], rank)
+            del X, Xt, Om, Y, Q, Bm, res
         active = nxt
     return ranks, rels, status
 

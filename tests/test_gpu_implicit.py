@@ -77,10 +77,10 @@ def test_pc_blocks_match_oracle():
     assert np.linalg.norm(y.view(N, 3).cpu().numpy() - ref) <= 1e-14 * np.linalg.norm(ref)
 
 
-def test_multi_dot_and_lincomb():
+@pytest.mark.parametrize("n,k", [(10007, 7), (5000, 31), (3, 1), (2048, 16)])
+def test_multi_dot_and_lincomb(n, k):
     from smx import _capi
     L = _capi.lib()
-    n, k = 10007, 7
     V = torch.randn(k, n, dtype=torch.float64, device=DEV)
     w = torch.randn(n, dtype=torch.float64, device=DEV)
     out = torch.zeros(k, dtype=torch.float64, device=DEV)
@@ -88,7 +88,11 @@ def test_multi_dot_and_lincomb():
     _capi.check(L.smx_multi_dot(n, k, V.data_ptr(), n, w.data_ptr(), out.data_ptr(), work.data_ptr(),
                                 _capi.stream_ptr()))
     ref = V.cpu().numpy() @ w.cpu().numpy()
-    assert np.allclose(out.cpu().numpy(), ref, rtol=1e-12)
+    assert np.allclose(out.cpu().numpy(), ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+    again = torch.zeros_like(out)
+    _capi.check(L.smx_multi_dot(n, k, V.data_ptr(), n, w.data_ptr(), again.data_ptr(), work.data_ptr(),
+                                _capi.stream_ptr()))
+    assert torch.equal(again, out)                      # fixed reduction order
     c = torch.randn(k, dtype=torch.float64, device=DEV)
     y = torch.randn(n, dtype=torch.float64, device=DEV)
     y0 = y.cpu().numpy().copy()
