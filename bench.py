@@ -303,184 +303,6 @@ def _kernel_roofline(name, fn, nbytes, flush, reps=20):
             "frac": nbytes / sec / 1e9 / HBM_PEAK_GBS}
 
 
-def _cpu_local_blocks(m, Hm, kappa, gamma, alpha, Ms, shift):
-    """Vectorized inverse of shift I - J_i with J_i the exact local derivative
-    of the LLG RHS (field self-derivative kappa_i I), for the CPU port."""
-    import numpy as np
-    m3, h3 = m.reshape(-1, 3), Hm.reshape(-1, 3)
-    n = m3.shape[0]
-    c1 = gamma / (1 + alpha ** 2)
-    c2 = c1 * alpha
-    S = Ms * gamma / (1 + alpha ** 2) * 0.5
-
-    def skew(v):
-        K = np.zeros((v.shape[0], 3, 3))
-        K[:, 0, 1], K[:, 0, 2] = -v[:, 2], v[:, 1]
-        K[:, 1, 0], K[:, 1, 2] = v[:, 2], -v[:, 0]
-        K[:, 2, 0], K[:, 2, 1] = -v[:, 1], v[:, 0]
-        return K
-
-    mm = np.einsum("ni,ni->n", m3, m3)
-    mh = np.einsum("ni,ni->n", m3, h3)
-    I3 = np.eye(3)[None]
-    outer_mm = np.einsum("ni,nj->nij", m3, m3)
-    B = np.einsum("ni,nj->nij", m3, h3) - 2.0 * np.einsum("ni,nj->nij", h3, m3)
-    diag = mh[:, None] - h3 * m3
-    idx = np.arange(3)
-    B[:, idx, idx] = diag
-    k = kappa[:, None, None]
-    dprec = -skew(h3) + k * skew(m3)
-    ddamp = B + k * (outer_mm - mm[:, None, None] * I3)
-    stab = S * ((1.0 - mm)[:, None, None] * I3 - 2.0 * outer_mm)
-    J = -c1 * dprec - c2 * ddamp + stab
-    return np.linalg.inv((shift + 1e-14) * I3 - J), J
-
-
-def _cpu_bdf_port(mesh, mat, B_host, m0, t_final, dt0, budget_s, tol=1e-6, snes_rtol=1e-2, snes_atol=1e-4,
-                  ksp_rtol=1e-4):
-    """Host port of BDFIntegrator + GMRES + exact block preconditioner on the
-    FP64 oracle operators (splu Poisson solves, dense B), run for at most
-    budget_s seconds.  Returns (steps, elapsed, t reached, counters)."""
-    import math
-    import numpy as np
-    import scipy.sparse.linalg as spla
-    sys.path.insert(0, str(ROOT / "oracle"))
-    import smx_oracle as O
-    Ms, A_ex, alpha, Ku = mat["Ms"], mat["Aex"], mat["alpha"], mat["Ku"]
-    nax = np.asarray(mat["n_ani_vec"], dtype=float)
-    fem = O.fem_operators(mesh.X, mesh.tets, Ms)
-    Kex = O.exchange_matrix(mesh.X, mesh.tets, fem.V, A_ex, Ms).tocsr()
-    bd = O.boundary_data(mesh.X, mesh.tets)
-    lu1 = spla.splu(_pinned(fem.A).tocsc())
-    lu2 = spla.splu(fem.A_bc.tocsc())
-    Vn = np.repeat(fem.V, 3)
-    pref = 2 * Ku / (O.MU0 * Ms)
-    kappa = (Kex.diagonal().reshape(-1, 3) + pref * nax ** 2).mean(axis=1)
-    cnt = dict(rhs=0, jv=0, newton=0, gmres=0)
-
-    def h_lin(v):
-        return Kex @ v + O.anisotropy_field(v, nax, Ku, Ms)
-
-    def rhs(y):
-        cnt["rhs"] += 1
-        b = fem.Div @ y
-        b -= b.mean()
-        u1 = _pin_solve(lu1, b)
-        u1 -= u1.mean()
-        g = np.zeros(fem.N)
-        g[bd.bverts] = B_host @ u1[bd.bverts]
-        bb = -(fem.A @ g)
-        bb[fem.bnodes] = g[fem.bnodes]
-        u2 = lu2.solve(bb)
-        H = h_lin(y) + (fem.G @ (u1 + u2)) / Vn
-        return O.llg_rhs(y, H, GAMMA, alpha, Ms)
-
-    def gmres(mult, pc, b, rtol, max_it=200, restart=30):
-        x = np.zeros_like(b)
-        bn = np.linalg.norm(b)
-        r = b - mult(x)
-        beta = np.linalg.norm(r)
-        its = 0
-        while beta > rtol * bn and its < max_it:
-            Vk = [r / beta]
-            Z = []
-            Hh = np.zeros((restart + 1, restart))
-            for j in range(restart):
-                Z.append(pc(Vk[j]))
-                w = mult(Z[j])
-                for _ in range(2):
-                    Vm = np.array(Vk)
-                    h = Vm @ w
-                    w = w - h @ Vm
-                    Hh[:j + 1, j] += h
-                Hh[j + 1, j] = np.linalg.norm(w)
-                Vk.append(w / Hh[j + 1, j] if Hh[j + 1, j] > 0 else w)
-                its += 1
-                e1 = np.zeros(j + 2)
-                e1[0] = beta
-                yls = np.linalg.lstsq(Hh[:j + 2, :j + 1], e1, rcond=None)[0]
-                res = np.linalg.norm(Hh[:j + 2, :j + 1] @ yls - e1)
-                if res <= rtol * bn or its >= max_it:
-                    break
-            x = x + np.array(Z).T @ yls
-            r = b - mult(x)
-            beta = np.linalg.norm(r)
-        cnt["gmres"] += its
-        return x
-
-    y = m0.copy()
-    hist, times = [y.copy()], [0.0]
-    t, dt = 0.0, dt0
-    steps = 0
-    t_start = time.perf_counter()
-    while t < t_final and time.perf_counter() - t_start < budget_s:
-        dt = min(dt, t_final - t)
-        q = 2 if len(hist) >= 2 else 1
-        if q == 2:
-            w = dt / (times[0] - times[1])
-            a0, a1, a2 = (1 + 2 * w) / (1 + w), -(1 + w), w * w / (1 + w)
-            shift = a0 / dt
-            chist = (-a1 * hist[0] - a2 * hist[1]) / dt
-        else:
-            shift = 1.0 / dt
-            chist = hist[0] / dt
-        tn = times[:3]
-        tq = t + dt
-        lag = [math.prod((tq - tn[j]) / (tn[i] - tn[j]) for j in range(len(tn)) if j != i)
-               for i in range(len(tn))]
-        ypred = sum(c * h for c, h in zip(lag, hist[:len(tn)]))
-        yv = ypred.copy()
-        F = shift * yv - chist - rhs(yv)
-        Fn = np.linalg.norm(F)
-        tol_n = max(snes_atol, snes_rtol * Fn)
-        ok = False
-        for it in range(8):
-            if Fn <= tol_n and it > 0:
-                ok = True
-                break
-            cnt["newton"] += 1
-            Hm = h_lin(yv)
-            inv, _ = _cpu_local_blocks(yv, Hm, kappa, GAMMA, alpha, Ms, shift)
-
-            def mult(v, yv=yv, Hm=Hm):
-                cnt["jv"] += 1
-                return shift * v - O.jac_vec(yv, Hm, v, h_lin(v), GAMMA, alpha, Ms)
-
-            pc = lambda v: np.einsum("nij,nj->ni", inv, v.reshape(-1, 3)).ravel()
-            d = gmres(mult, pc, -F, ksp_rtol)
-            lam = 1.0
-            for _ in range(5):
-                yt = yv + lam * d
-                Ft = shift * yt - chist - rhs(yt)
-                Ftn = np.linalg.norm(Ft)
-                if Ftn <= (1 - 1e-4 * lam) * Fn or lam < 0.1:
-                    break
-                lam *= 0.5
-            yv, F, Fn = yt, Ft, Ftn
-            if Fn <= tol_n:
-                ok = True
-                break
-        if not ok:
-            dt *= 0.25
-            continue
-        if len(tn) >= q + 1:
-            coef = 2.0 / 11.0 if q == 2 else 0.5
-            sc = tol + tol * np.maximum(np.abs(hist[0]), np.abs(yv))
-            err = math.sqrt(np.mean((coef * (yv - ypred) / sc) ** 2))
-        else:
-            err = 0.0
-        if err <= 1.0:
-            hist = [yv.copy()] + hist[:2]
-            times = [t + dt] + times[:2]
-            t += dt
-            steps += 1
-            fac = 0.9 * (1.0 / max(err, 1e-16)) ** (1.0 / (q + 1)) if err > 0 else 3.0
-            dt = min(max(dt * min(max(fac, 0.1), 3.0), 1e-17), 1e-10)
-        else:
-            dt = max(dt * min(max(0.9 * (1.0 / err) ** (1.0 / (q + 1)), 0.1), 1.0), 1e-17)
-    return steps, time.perf_counter() - t_start, t, cnt
-
-
 def extended_paths(cpu_budget=20.0, no_cpu=False):
     """Measurements of the paths beside the demag + RHS evaluation: torque
     terms in the fused field kernel, the implicit BDF Newton-Krylov relax and
@@ -589,12 +411,18 @@ def extended_paths(cpu_budget=20.0, no_cpu=False):
            "stats": {k2: (None if isinstance(v2, float) and v2 != v2 else v2) for k2, v2 in st.items()},
            "value": st["steps"] / st["elapsed_s"], "unit": "BDF steps/s", "kernels_at_film_size": kr}
     if not no_cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import smx_oracle as O
         B_host = hc.demag_field.B_dense_gpu.cpu().numpy()[:, :hc.demag_field.opB.Nb]
-        steps_c, el_c, t_c, cnt = _cpu_bdf_port(cube, mat, B_host, m0, t_final, 1e-14, cpu_budget)
-        imp["cpu_baseline"] = {"value": steps_c / el_c, "unit": "BDF steps/s", "cores": 1, "kind": "port",
-                               "sample": f"first {steps_c} BDF steps (to t = {t_c:.3e} s) of the same run by a host "
-                                         f"port (splu Poisson solves, numpy GMRES and block preconditioner) in "
-                                         f"{el_c:.1f}s; {cnt}"}
+        dem = O.DemagOracle(cube.X, cube.tets, mat["Ms"], B=B_host)
+        t0 = time.perf_counter()
+        _, t_c, cst = O.llg_bdf_relax(cube.X, cube.tets, m0, t_final, 1e-14, mat["Ms"], mat["Aex"], mat["alpha"],
+                                      Ku=mat["Ku"], n_ani=mat["n_ani_vec"], dem=dem, budget_s=cpu_budget)
+        el_c = time.perf_counter() - t0
+        imp["cpu_baseline"] = {"value": cst["steps"] / el_c, "unit": "BDF steps/s", "cores": 1, "kind": "port",
+                               "sample": f"first {cst['steps']} BDF steps (to t = {t_c:.3e} s) of the same run by "
+                                         f"the oracle's BDF Newton-Krylov (splu Poisson solves, numpy GMRES and "
+                                         f"block preconditioner) in {el_c:.1f}s; {cst}"}
     out["implicit"] = imp
     log(f"implicit: {st['steps']} steps in {st['elapsed_s']:.2f}s ({imp['value']:.0f} steps/s)")
 

@@ -19,7 +19,10 @@ Time integration (BDF of order 1 on the first step, then 2, variable step):
   local error (Milne): C (y - y_pred), y_pred the polynomial extrapolation of
   the last three (BDF2; C = 2/11) or two (BDF1; C = 1/2) accepted states,
   weighted RMS norm with atol + rtol max(|y_n|, |y_{n+1}|), step factor
-  0.9 err^(-1/(q+1)) clipped to [0.1, 3], x0.25 after a failed solve.
+  0.9 err^(-1/(q+1)) clipped to [0.1, 3], x0.25 after a failed solve; a
+  step that would leave less than 1e-3 of itself before t1 is stretched to
+  land on t1 (no sliver step, whose huge shift the Newton tolerance cannot
+  resolve).
 """
 from __future__ import annotations
 
@@ -311,7 +314,9 @@ class BDFIntegrator:
         t_check = t
         self.stats["stopped_on_dmdt"] = False
         while t < t1 * (1 - 1e-15) and steps < max_steps:
-            dt = min(dt, t1 - t)
+            rem = t1 - t
+            if dt >= rem or rem - dt < 1e-3 * dt:       # land on t1: no sliver step before it
+                dt = rem
             q = 2 if self.nhist >= 2 else 1
             yn = self.hist[0]
             if q == 2:

@@ -46,6 +46,12 @@ How each part is pinned (see tests/test_oracle_*.py):
   torques             macrospin: damping-like SOT turns m toward p; STT/SOT
                       torques orthogonal to m; JVPs by central FD
   cayley_step         Rodrigues rotation by 2 atan|c| about c; |y| = |m|
+  gmres_right         solves against numpy's dense solve; restarts and the
+                      right preconditioner change the iterates, not the answer
+  bdf_integrate       fixed-step BDF2 error ratio ~4 on halving dt (second
+                      order) on y' = A y against expm; the adaptive run meets
+                      its tolerance there and on the damped macrospin's
+                      closed-form solution
 """
 from __future__ import annotations
 
@@ -1109,6 +1115,161 @@ def cayley_step(m, d, step):
 # Implicit steps: local 3x3 block of the Newton operator
 # (gpu/solvers/llg_module_GPU.py:165-290, 1412-1505)
 # --------------------------------------------------------------------------
+def gmres_right(mult, pc, b, x, rtol, atol=0.0, max_it=200, restart=30):
+    """Right-preconditioned restarted GMRES with classical Gram-Schmidt applied
+    twice and Givens rotations (the KSP gmres of the implicit relax,
+    llg_module_GPU.py:2112-2140, restart 30).  x is updated in place;
+    returns (iterations, final true residual norm)."""
+    bnorm = float(np.linalg.norm(b))
+    tol = max(rtol * bnorm, atol)
+    r = b - mult(x)
+    beta = float(np.linalg.norm(r))
+    total = 0
+    while beta > tol and total < max_it:
+        V = [r / beta]
+        Z = []
+        H = np.zeros((restart + 1, restart))
+        cs, sn = np.zeros(restart), np.zeros(restart)
+        g = np.zeros(restart + 1)
+        g[0] = beta
+        k = 0
+        for j in range(restart):
+            Z.append(pc(V[j]))
+            w = mult(Z[j])
+            h = np.zeros(j + 1)
+            for _ in range(2):
+                c = np.array([v @ w for v in V])
+                w = w - sum(ci * v for ci, v in zip(c, V))
+                h += c
+            H[:j + 1, j] = h
+            hn = float(np.linalg.norm(w))
+            H[j + 1, j] = hn
+            V.append(w / hn if hn > 0.0 else w)
+            for i in range(j):
+                a, c2 = H[i, j], H[i + 1, j]
+                H[i, j], H[i + 1, j] = cs[i] * a + sn[i] * c2, -sn[i] * a + cs[i] * c2
+            den = math.hypot(H[j, j], H[j + 1, j])
+            cs[j], sn[j] = (1.0, 0.0) if den == 0.0 else (H[j, j] / den, H[j + 1, j] / den)
+            H[j, j] = cs[j] * H[j, j] + sn[j] * H[j + 1, j]
+            H[j + 1, j] = 0.0
+            g[j + 1] = -sn[j] * g[j]
+            g[j] = cs[j] * g[j]
+            k = j + 1
+            total += 1
+            if abs(g[j + 1]) <= tol or total >= max_it or hn == 0.0:
+                break
+        y = np.zeros(k)
+        for i in range(k - 1, -1, -1):
+            y[i] = (g[i] - H[i, i + 1:k] @ y[i + 1:k]) / H[i, i]
+        x += sum(yi * zi for yi, zi in zip(y, Z))
+        r = b - mult(x)
+        beta = float(np.linalg.norm(r))
+    return total, beta
+
+
+def bdf_integrate(f, newton_ops, y0, t0, t1, dt0, rtol=1e-6, atol=1e-6, snes_rtol=1e-2, snes_atol=1e-4,
+                  snes_max_it=8, ksp_rtol=1e-4, ksp_max_it=200, dt_min=1e-17, dt_max=1e-10, fixed_dt=False,
+                  max_steps=10 ** 7, budget_s=None):
+    """Variable-step BDF(1, 2) with Newton-Krylov solves: the implicit relax of
+    llg_module_GPU.py:2052-2381 (PETSc TS bdf with the basic adaptor, SNES
+    newtonls with a backtracking line search, KSP gmres).  Restated from the
+    published method (BDF2 with variable step ratio w = h/h_prev:
+    a0 = (1+2w)/(1+w), a1 = -(1+w), a2 = w^2/(1+w); Milne error estimate
+    C (y - y_pred) with C = 2/11 (BDF2) or 1/2 (BDF1) against a polynomial
+    predictor through the last accepted states, weighted RMS norm with
+    atol + rtol max(|y_n|, |y|), step factor 0.9 err^(-1/(q+1)) clipped to
+    [0.1, 3], x0.25 after a failed solve; a step that would leave less than
+    1e-3 of itself before t1 is stretched to land on t1).
+
+    f(t, y) is the right-hand side; newton_ops(y, shift) returns (mult, pc),
+    the operator x -> shift x - J(y) x and its preconditioner.  fixed_dt
+    accepts every step at dt (no error control).  budget_s stops after that
+    many seconds of wall time.  Returns (y, t, stats)."""
+    import time as _time
+    y = np.array(y0, dtype=np.float64)
+    hist, times = [y.copy()], [float(t0)]
+    t = float(t0)
+    dt = min(max(float(dt0), dt_min), dt_max)
+    st = dict(steps=0, rejected=0, failed_solves=0, newton=0, gmres=0, rhs_evals=0)
+    start = _time.perf_counter()
+
+    def rhs(tt, yy):
+        st["rhs_evals"] += 1
+        return f(tt, yy)
+
+    while t < t1 * (1 - 1e-15) and st["steps"] < max_steps:
+        if budget_s is not None and _time.perf_counter() - start >= budget_s:
+            break
+        rem = t1 - t
+        if dt >= rem or rem - dt < 1e-3 * dt:      # land on t1: no sliver step before it
+            dt = rem
+        q = 2 if len(hist) >= 2 else 1
+        if q == 2:
+            w = dt / (times[0] - times[1])
+            a0, a1, a2 = (1 + 2 * w) / (1 + w), -(1 + w), w * w / (1 + w)
+            shift = a0 / dt
+            c_hist = (-a1 / dt) * hist[0] + (-a2 / dt) * hist[1]
+        else:
+            shift = 1.0 / dt
+            c_hist = hist[0] / dt
+        tn = times[:3]
+        tq = t + dt
+        lag = [math.prod((tq - tn[j]) / (tn[i] - tn[j]) for j in range(len(tn)) if j != i)
+               for i in range(len(tn))]
+        ypred = hist[0].copy() if len(tn) == 1 else sum(c * h for c, h in zip(lag, hist))
+        yv = ypred.copy()
+        F = shift * yv - rhs(tq, yv) - c_hist
+        Fn = float(np.linalg.norm(F))
+        tol = max(snes_atol, snes_rtol * Fn)
+        ok = False
+        for it in range(snes_max_it):
+            if Fn <= tol and it > 0:
+                ok = True
+                break
+            st["newton"] += 1
+            mult, pc = newton_ops(yv, shift)
+            d = np.zeros_like(yv)
+            its, _ = gmres_right(mult, pc, -F, d, ksp_rtol, 0.0, ksp_max_it)
+            st["gmres"] += its
+            lam = 1.0
+            for _ in range(5):
+                yt = yv + lam * d
+                Ft = shift * yt - rhs(tq, yt) - c_hist
+                Ftn = float(np.linalg.norm(Ft))
+                if Ftn <= (1.0 - 1e-4 * lam) * Fn or lam < 0.1:
+                    break
+                lam *= 0.5
+            yv, F, Fn = yt, Ft, Ftn
+            if Fn <= tol:
+                ok = True
+                break
+        if not ok:
+            st["failed_solves"] += 1
+            dt = max(dt * 0.25, dt_min)
+            if dt <= dt_min * (1 + 1e-12) and st["failed_solves"] > 50:
+                raise RuntimeError("BDF: nonlinear solves keep failing at dt_min")
+            continue
+        if fixed_dt or len(tn) < q + 1:
+            err = 0.0
+        else:
+            coef = 2.0 / 11.0 if q == 2 else 0.5
+            sc = atol + rtol * np.maximum(np.abs(hist[0]), np.abs(yv))
+            err = math.sqrt(float(np.mean((coef * (yv - ypred) / sc) ** 2)))
+        if err <= 1.0:
+            hist = [yv.copy()] + hist[:2]
+            times = [tq] + times[:2]
+            t = tq
+            st["steps"] += 1
+            if not fixed_dt:
+                fac = 0.9 * (1.0 / max(err, 1e-16)) ** (1.0 / (q + 1)) if err > 0 else 3.0
+                dt = min(max(dt * min(max(fac, 0.1), 3.0), dt_min), dt_max)
+        else:
+            st["rejected"] += 1
+            fac = 0.9 * (1.0 / err) ** (1.0 / (q + 1))
+            dt = max(dt * min(max(fac, 0.1), 1.0), dt_min)
+    return hist[0], t, st
+
+
 def llg_local_jacobian(m_i, h_i, kappa, gamma, alpha, Ms, do_precess=1.0, include_stab=True):
     """d r_i / d m_i of the LLG RHS r = -c1 m x H - c2 m x (m x H) + S (1-|m|^2) m
     at one node, with the field's self-derivative dH/dm = kappa I.  Written
@@ -1131,3 +1292,63 @@ def llg_local_jacobian(m_i, h_i, kappa, gamma, alpha, Ms, do_precess=1.0, includ
             col = col + S * ((1.0 - m @ m) * e - 2.0 * m * (m @ e))
         J[:, c] = col
     return J
+
+
+def llg_local_jacobians(m, h, kappa, gamma, alpha, Ms, do_precess=1.0, include_stab=True):
+    """llg_local_jacobian for every node at once: m, h (3N), kappa (N) ->
+    (N, 3, 3), the same product-rule columns written with einsum."""
+    m3 = np.asarray(m, dtype=np.float64).reshape(-1, 3)
+    h3 = np.asarray(h, dtype=np.float64).reshape(-1, 3)
+    kap = np.broadcast_to(np.asarray(kappa, dtype=np.float64), (m3.shape[0],))
+    den = 1.0 + alpha * alpha
+    c1, c2 = gamma * do_precess / den, gamma * alpha / den
+    S = Ms * gamma / den * 0.5
+    E = np.eye(3)
+    mh = np.einsum("ni,ni->n", m3, h3)
+    mm = np.einsum("ni,ni->n", m3, m3)
+    # column c: e_c x h + kappa m x e_c
+    dprec = np.cross(E[None, :, :], h3[:, None, :]) + kap[:, None, None] * np.cross(m3[:, None, :], E[None, :, :])
+    # column c: e_c (m.h) + m (h_c + kappa m_c) - kappa e_c |m|^2 - 2 h m_c
+    ddamp = (E[None] * mh[:, None, None] + m3[:, None, :] * (h3 + kap[:, None] * m3)[:, :, None]
+             - kap[:, None, None] * E[None] * mm[:, None, None] - 2.0 * h3[:, None, :] * m3[:, :, None])
+    cols = -c1 * dprec - c2 * ddamp                           # (N, c, i): column c, component i
+    if include_stab:
+        cols = cols + S * ((1.0 - mm)[:, None, None] * E[None] - 2.0 * m3[:, None, :] * m3[:, :, None])
+    return np.transpose(cols, (0, 2, 1))
+
+
+def llg_bdf_relax(X, tets, m0, t1, dt0, Ms, A_ex, alpha, Ku=0.0, n_ani=(0.0, 0.0, 1.0), gamma=2.211e5, B=None,
+                  dem=None, **bdf_kw):
+    """The implicit relax of an exchange + uniaxial + FEM/BEM demag magnet
+    (llg_module_GPU.py:2052-2381) on the oracle's operators: RHS with the
+    demag field from DemagOracle (LU solves), Newton operator shift I - J
+    with J the JVP without demag (llg_module_GPU.py:1293-1323) at the field
+    without demag, block preconditioner inv(shift I - J_i) with J_i the exact
+    local derivative at kappa_i = mean of the linear operator's diagonal.
+    Returns (m, t, stats)."""
+    X = np.asarray(X, dtype=np.float64)
+    dem = dem if dem is not None else DemagOracle(X, tets, Ms, B=B)
+    fem = dem.fem
+    Kex = exchange_matrix(X, tets, fem.V, A_ex, Ms).tocsr()
+    nax = np.asarray(n_ani, dtype=np.float64)
+    pref = 2.0 * Ku / (MU0 * Ms)
+    kappa = (Kex.diagonal().reshape(-1, 3) + pref * nax[None, :] ** 2).mean(axis=1)
+
+    def h_lin(v):
+        out = Kex @ v
+        if Ku:
+            out = out + anisotropy_field(v, nax, Ku, Ms)
+        return out
+
+    def f(t, y):
+        return llg_rhs(y, h_lin(y) + dem.compute(y), gamma, alpha, Ms)
+
+    def newton_ops(y, shift):
+        Hm = h_lin(y)
+        J = llg_local_jacobians(y, Hm, kappa, gamma, alpha, Ms)
+        inv = np.linalg.inv((shift + 1e-14) * np.eye(3)[None] - J)
+        mult = lambda v: shift * v - jac_vec(y, Hm, v, h_lin(v), gamma, alpha, Ms)
+        pc = lambda v: np.einsum("nij,nj->ni", inv, v.reshape(-1, 3)).ravel()
+        return mult, pc
+
+    return bdf_integrate(f, newton_ops, m0, 0.0, t1, dt0, **bdf_kw)

@@ -385,8 +385,8 @@ def test_workspace_limit_subdivides_like_the_device_rule(c1):
     assert data.metadata["forced_subdivisions_memory"] == ref.stats["forced_subdivisions_memory"]
     got = [(b.row_start, b.row_stop, b.col_start, b.col_stop, b.rank) for b in data.far_blocks]
     assert got == [(r0, r1, c0, c1, U.shape[1]) for r0, r1, c0, c1, U, V, _ in ref.far]
-    assert max(b.size for b in [type("x", (), {"size": (f.row_stop - f.row_start) * (f.col_stop - f.col_start)})
-                                for f in data.far_blocks]) * 8 * 3.5 <= avail * 0.5
+    assert max((f.row_stop - f.row_start) * (f.col_stop - f.col_start) for f in data.far_blocks) * 8 * 3.5 \
+        <= avail * 0.5
 
 
 @pytest.mark.parametrize("limit", [6 * 1024, 2 * 1024])
