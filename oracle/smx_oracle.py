@@ -48,6 +48,9 @@ How each part is pinned (see tests/test_oracle_*.py):
   cayley_step         Rodrigues rotation by 2 atan|c| about c; |y| = |m|
   gmres_right         solves against numpy's dense solve; restarts and the
                       right preconditioner change the iterates, not the answer
+  labonte_bb_minimize macrospin in a field along the easy axis -> m = n;
+                      exchange + uniaxial box -> E = -Ku V; BB rules on
+                      hand-computed s, y
   bdf_integrate       fixed-step BDF2 error ratio ~4 on halving dt (second
                       order) on y' = A y against expm; the adaptive run meets
                       its tolerance there and on the damped macrospin's
@@ -1352,3 +1355,124 @@ def llg_bdf_relax(X, tets, m0, t1, dt0, Ms, A_ex, alpha, Ku=0.0, n_ani=(0.0, 0.0
         return mult, pc
 
     return bdf_integrate(f, newton_ops, m0, 0.0, t1, dt0, **bdf_kw)
+
+
+def bb_step_rule(sts, sty, yty, alpha_old, alpha_min, alpha_max, variant, iteration, state):
+    """Barzilai-Borwein step from s = m_n - m_{n-1}, y = g_n - g_{n-1}
+    (Minimizer_GPU.py:418-535): BB1 = s.s/s.y, BB2 = s.y/y.y (Barzilai and
+    Borwein 1988), alternating, or the adaptive rule of Loris, Bertero, De
+    Mol, Zanella and Zanni (2009): take the smallest BB2 of the last
+    Malpha + 1 when BB2/BB1 <= tau (then tau *= 0.9), else BB1 (tau *= 1.1).
+    state holds tau, Malpha, tau_min, tau_max and the BB2 history; returns
+    (alpha, rule)."""
+    clamp = lambda a: min(max(float(a), alpha_min), alpha_max)
+    if sts < 1e-300 or yty < 1e-300:
+        return clamp(alpha_old), "reuse_old_degenerate"
+    key = str(variant).lower().replace("-", "_")
+    if key in ("adaptive", "gpss", "loris", "exl", "loris_zanni"):
+        if sty <= 1e-300:
+            return float(alpha_max), "adaptive_alpha_max_nonpositive_sty"
+        a1, a2 = clamp(sts / sty), clamp(sty / yty)
+        state["hist"] = (state["hist"] + [a2])[-(state["Malpha"] + 1):]
+        if a2 / max(a1, 1e-300) <= state["tau"]:
+            state["tau"] = max(state["tau"] * 0.9, state["tau_min"])
+            return min(state["hist"]), "adaptive_recent_min_bb2"
+        state["tau"] = min(state["tau"] * 1.1, state["tau_max"])
+        return a1, "adaptive_bb1"
+    if sty <= 1e-300:
+        return clamp(alpha_old), "nonpositive_sty"
+    if key == "bb1" or (key == "alternate" and iteration % 2 == 0):
+        return clamp(sts / sty), "bb1"
+    if key in ("bb2", "alternate"):
+        return clamp(sty / yty), "bb2"
+    raise ValueError(f"unknown bb_variant {variant!r}")
+
+
+def labonte_bb_minimize(energy, heff, m0, max_iter=5000, tol=1.0, alpha0=5e-11, alpha_min=1e-13, alpha_max=1e-8,
+                        bb_variant="adaptive", bb_adaptive_tau0=0.5, bb_adaptive_Malpha=2, nonmonotone_window=10,
+                        max_backtracking=10, shrink=0.5, energy_accept_rtol=1e-10, energy_accept_atol=1e-24,
+                        energy_stagnation_rtol=1e-10, energy_stagnation_atol=1e-24, stagnation_window=30,
+                        max_rejected_at_alpha_min=30, alpha_restart=5e-11, max_alpha_restarts=3,
+                        restart_residual_factor=50.0, stagnation_residual_stop=50.0):
+    """LaBonte steepest descent on the unit sphere with Barzilai-Borwein steps
+    (LaBonteBBMinimizerGPU.minimize, Minimizer_GPU.py:537-846): direction
+    d = H - (m.H) m, Cayley update m <- (I - C)^-1 (I + C) m with
+    c = (step/2) m x d, nonmonotone acceptance against the largest of the
+    last nonmonotone_window energies with backtracking, restarts of the step
+    on energy stagnation, stop on max_i |d_i| < tol.  energy(m) -> float,
+    heff(m) -> H.  Returns (m, info)."""
+    def direction(mm):
+        return projected_direction(mm, heff(mm))
+
+    def max_norm3(v):
+        return float(np.linalg.norm(np.asarray(v).reshape(-1, 3), axis=1).max())
+
+    state = dict(tau=min(max(float(bb_adaptive_tau0), 1e-12), 1e12), Malpha=max(int(bb_adaptive_Malpha), 0),
+                 tau_min=1e-12, tau_max=1e12, hist=[])
+    rule = "init"
+    m3 = np.asarray(m0, dtype=np.float64).reshape(-1, 3)
+    m = (m3 / np.linalg.norm(m3, axis=1)[:, None]).ravel()
+    E = energy(m)
+    hist = [E]
+    alpha = min(max(float(alpha0), alpha_min), alpha_max)
+    rejected = restarts = 0
+    stop = "max_iter"
+    d = direction(m)
+    it = 0
+    for it in range(max_iter):
+        max_g = max_norm3(d)
+        if max_g < tol:
+            stop = "projected_field_tol"
+            break
+        if len(hist) > stagnation_window:
+            E_old, E_new = hist[-stagnation_window], hist[-1]
+            if abs(E_new - E_old) <= energy_stagnation_atol + energy_stagnation_rtol * max(abs(E_old), abs(E_new),
+                                                                                            1e-300):
+                if max_g <= restart_residual_factor * tol:
+                    stop = "energy_stagnation_low_residual"
+                    break
+                if restarts < max_alpha_restarts:
+                    alpha = min(max(alpha_restart, alpha_min), alpha_max)
+                    restarts += 1
+                    rejected = 0
+                    hist = [E]
+                    d = direction(m)
+                    continue
+                if max_g < stagnation_residual_stop:
+                    stop = "energy_stagnation_high_residual"
+                    break
+        m_old, d_old = m.copy(), d.copy()
+        E_ref = max(hist[-nonmonotone_window:])
+        accepted = False
+        a_try = alpha
+        for _ in range(max_backtracking + 1):
+            m_trial = cayley_step(m_old, d, a_try)
+            E_trial = energy(m_trial)
+            if E_trial <= E_ref + energy_accept_atol + energy_accept_rtol * max(abs(E_ref), abs(E_trial), 1e-300):
+                m, E = m_trial, E_trial
+                hist.append(E)
+                accepted = True
+                rejected = 0
+                break
+            a_try = max(a_try * shrink, alpha_min)
+        if not accepted:
+            m = m_old
+            rejected += 1
+            alpha = alpha_min
+            if rejected >= max_rejected_at_alpha_min:
+                stop = "repeated_rejection_at_alpha_min"
+                break
+            d = direction(m)
+            continue
+        d = direction(m)
+        s, y = m - m_old, d_old - d
+        alpha, rule = bb_step_rule(float(s @ s), float(s @ y), float(y @ y), a_try, alpha_min, alpha_max, bb_variant,
+                                   it, state)
+        alpha = min(max(alpha, alpha_min), alpha_max)
+    m3 = m.reshape(-1, 3)
+    m = (m3 / np.linalg.norm(m3, axis=1)[:, None]).ravel()
+    d = direction(m)
+    info = dict(iterations=int(it), energy=float(energy(m)), max_projected_field=max_norm3(d), alpha=float(alpha),
+                bb_last_rule=rule, alpha_restarts=int(restarts), rejected_at_alpha_min=int(rejected),
+                stop_reason=stop, converged=stop in ("projected_field_tol", "energy_stagnation_low_residual"))
+    return m, info

@@ -190,3 +190,36 @@ def test_minimizer_reaches_uniform_minimum(variant):
     assert res["converged"]
     assert mz.min() > 1.0 - 1e-8
     assert abs(res["energy"] - Emin) <= 1e-8 * abs(Emin)
+
+
+def test_minimizer_matches_oracle():
+    """The device minimizer against the oracle's restatement
+    (labonte_bb_minimize) on an exchange + tilted uniaxial + Zeeman box:
+    same iteration count and stop reason, energies to 1e-9, states to 1e-6."""
+    from smx.llg import EffectiveField
+    from smx.minimizer import LaBonteBBMinimizerGPU
+    m = _mesh()
+    N = m.num_nodes
+    Ms, A, Ku = 8e5, 1.3e-11, 5e5
+    n_ax, H0 = (0.0, 0.6, 0.8), (2e4, 0.0, -1e4)
+    hef = EffectiveField(m, Ms=Ms, Aex=A, Ku=Ku, n_ani_vec=n_ax, H0_static=H0, device=DEV)
+    m0 = _unit(N, 13, bias=(0.0, 0.0, 1.0))
+    hef.set_m_from_cpu(m0)
+    kw = dict(max_iter=150, tol=1e-6)
+    res = LaBonteBBMinimizerGPU(hef).minimize(**kw)
+    fem = O.fem_operators(m.X, m.tets, Ms)
+    K = O.exchange_matrix(m.X, m.tets, fem.V, A, Ms)
+    Vt = np.repeat(fem.V, 3) * 1e-27
+    h0 = np.tile(np.asarray(H0), N)
+    hani = lambda mm: O.anisotropy_field(mm, n_ax, Ku, Ms)
+    energy = lambda mm: float(MU0 * Ms * np.sum(Vt * mm * (-0.5 * (K @ mm) - 0.5 * hani(mm) - h0)))
+    heff = lambda mm: K @ mm + hani(mm) + h0
+    mo, info = O.labonte_bb_minimize(energy, heff, m0, **kw)
+    mg = hef.m_gpu.cpu().numpy()
+    dm = float(np.linalg.norm(mg - mo) / np.linalg.norm(mo))
+    print(f"[minimizer vs oracle] device {res['iterations']} its {res['stop_reason']} E {res['energy']:.12e}; "
+          f"oracle {info['iterations']} its {info['stop_reason']} E {info['energy']:.12e}; |dm| {dm:.2e}",
+          flush=True)
+    assert res["iterations"] == info["iterations"] and res["stop_reason"] == info["stop_reason"]
+    assert abs(res["energy"] - info["energy"]) <= 1e-9 * abs(info["energy"])
+    assert dm <= 1e-6
