@@ -57,6 +57,9 @@ enum {
   ST_SIZE = 8,
   HDR = 32
 };
+static_assert(SS_DONE == SMX_PCG_DONE && SS_ITERS == SMX_PCG_ITERS && SS_FAIL == SMX_PCG_FAIL &&
+                  SS_SUMX == SMX_PCG_SUMX,
+              "status block layout of the C ABI");
 
 struct AmgPtrs {
   double* ss;
@@ -575,8 +578,13 @@ int run_amg(int64_t n, int32_t W, int64_t ld, const int32_t* cols, const double*
   SellOp R0 = {nullptr, nullptr, nullptr, 0};
   if (nlev > 1) R0 = to_op(lv[0].R);
   double* b1 = nlev > 1 ? lv[1].b : nullptr;
+  // check_every < 0: speculative launch of expected_its + 1 - check_every
+  // iterations with no wait (iterations past convergence return at once on
+  // the done flag); the caller reads the status block later
+  const bool spec = check_every < 0 && expected_its > 0;
+  const int32_t spec_batch = spec ? expected_its + 1 - check_every : 0;
   if (check_every < 1) check_every = 8;
-  int32_t batch = expected_its > 0 ? expected_its + 1 : check_every;
+  int32_t batch = spec ? spec_batch : (expected_its > 0 ? expected_its + 1 : check_every);
   int32_t launched = 0;
   bool done = false;
   while (!done && launched <= max_it) {
@@ -605,6 +613,7 @@ int run_amg(int64_t n, int32_t W, int64_t ld, const int32_t* cols, const double*
       SMX_LAUNCH_CHECK();
     }
     launched += nb;
+    if (spec) break;
     SMX_HIP_CHECK(hipMemcpyAsync(hst, P.ss, sizeof(double) * SS_SIZE, hipMemcpyDeviceToHost, st));
     SMX_HIP_CHECK(hipStreamSynchronize(st));
     done = hst[SS_DONE] != 0.0;
@@ -693,6 +702,8 @@ extern "C" int smx_pcg_amg(int64_t n, int32_t width, int64_t ld, const int32_t* 
   SMX_REQUIRE(n >= 0, SMX_ERR_SHAPE, "negative size");
   SMX_REQUIRE(mode == 0 || mode == 1, SMX_ERR_VALUE, "mode must be 0 or 1");
   SMX_REQUIRE(max_it >= 0, SMX_ERR_VALUE, "max_it must be >= 0");
+  SMX_REQUIRE(check_every >= 0 || (expected_its > 0 && check_every >= -64), SMX_ERR_VALUE,
+              "a speculative launch (check_every in [-64, -1]) needs expected_its > 0");
   SMX_REQUIRE(width >= 1 && width <= 32 && ld >= n, SMX_ERR_SHAPE,
               "ELL width must be in [1, 32] and ld >= n (width %d, ld %lld)", width, (long long)ld);
   SMX_REQUIRE(mode == 0 || (raw_width >= 1 && raw_width <= 32 && raw_ld >= n), SMX_ERR_SHAPE,
@@ -757,6 +768,10 @@ extern "C" int smx_pcg_amg(int64_t n, int32_t width, int64_t ld, const int32_t* 
 #undef SMX_AMG_W
 #undef SMX_AMG_CALL
   if (rc != SMX_OK) return rc;
+  if (check_every < 0) {  // speculative: nothing waited for, the status stays on the device
+    if (its_out) *its_out = -1;
+    return SMX_OK;
+  }
   if (its_out) *its_out = (int32_t)hst[SS_ITERS];
   if (rnorm_out) *rnorm_out = sqrt(hst[SS_RR]);
   if (sums_out) sums_out[0] = hst[SS_SUMX];

@@ -19,7 +19,7 @@ SMX_OK, SMX_ERR_SHAPE, SMX_ERR_VALUE, SMX_ERR_NOT_CONVERGED, SMX_ERR_OOM, SMX_ER
 SMX_F64, SMX_F32, SMX_BF16 = 0, 1, 2
 TRI_FIELDS = 34
 LINDHOLM_TILE_COLS = 1024
-PCG_SUMX = 6
+PCG_DONE, PCG_ITERS, PCG_FAIL, PCG_SUMX = 0, 1, 5, 6
 HMAT_Z_GROUP = 8
 HMAT_KW_MAX = 16384
 

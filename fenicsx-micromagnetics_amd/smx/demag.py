@@ -147,7 +147,8 @@ class DemagFieldLindholmGPU:
                  *, ksp_atol=1e-12, ksp_max_it_u1=2000, ksp_max_it_u2=12000,
                  warm_start=False, device=None, operators: OperatorSet | None = None,
                  geometry: BoundaryGeometry | None = None, hmatrix_prebuilt=None,
-                 hmatrix_near_fp32=False, cg_reorder=True, ksp_pc_type="gamg", ksp_pc_type_u2=None,
+                 hmatrix_near_fp32=False, cg_reorder=True, cg_speculative=True, ksp_pc_type="gamg",
+                 ksp_pc_type_u2=None,
                  amg_fp32=None, amg_max_coarse=1024, amg_theta=0.0, verbose=False):
         """Keywords as the reference's constructor (Demag_Lindholm_GPU.py:101-155),
         plus, keyword-only: the solver settings the reference hard-codes
@@ -179,6 +180,14 @@ class DemagFieldLindholmGPU:
         self.warm_start = bool(warm_start)
         self.verbose = bool(verbose)
         self.converged = True
+        # speculative solves: once both iteration counts are known, each solve
+        # launches last count + margin iterations without a host wait and the
+        # status of both is checked once, after the kernel that consumes them
+        self.cg_speculative = bool(cg_speculative)
+        self._its_known = False
+        self._pending_m = None
+        self.speculative_redos = 0
+        self._status_host = torch.zeros(16, dtype=torch.float64).pin_memory()
 
         if VolN is not None:
             vol = np.asarray(VolN, dtype=np.float64).reshape(-1, 3)[:, 0]
@@ -396,13 +405,17 @@ class DemagFieldLindholmGPU:
         return self.ws1.data_ptr() + 8 * _capi.PCG_SUMX
 
     def solve(self, op: str, mode: int, x: torch.Tensor, ws: torch.Tensor, *, b=None, b_part=None,
-              project: int = 1, gb=None, rtol=None, max_it=None, expected: int = 0, warm: bool = False) -> int:
+              project: int = 1, gb=None, rtol=None, max_it=None, expected: int = 0, warm: bool = False,
+              speculative: bool = False) -> int:
         """One preconditioned CG solve on operator ``op`` ("u1": the Neumann
         stiffness; "u2": the stiffness with Dirichlet identity rows) with that
         operator's preconditioner.  mode 0: A x = b - mean(b)*project (b and
         x in node order); mode 1: the Dirichlet lifting of boundary data gb.
-        Leaves sum(x) in ws[PCG_SUMX].  Returns the iteration count."""
+        Leaves sum(x) in ws[PCG_SUMX].  Returns the iteration count, or -1
+        for a speculative launch (expected + margin iterations enqueued, no
+        wait; the status block ws[0:8] tells later whether it converged)."""
         c = self._cg[op]
+        spec = bool(speculative) and expected > 0
         if rtol is None:
             rtol = self.ksp_rtol_u1 if op == "u1" else self.ksp_rtol_u2
         if max_it is None:
@@ -420,7 +433,7 @@ class DemagFieldLindholmGPU:
                                _capi.ptr(raw.cols), _capi.ptr(raw.vals), _capi.ptr(self.bpos_cg),
                                _capi.ptr(gb), _capi.ptr(x), int(warm), _capi.ptr(self.cg_perm),
                                C.addressof(amg.c_levels), amg.nlev, amg.val_dtype, float(rtol),
-                               float(self.ksp_atol), int(max_it), 8, int(expected),
+                               float(self.ksp_atol), int(max_it), -1 if spec else 8, int(expected),
                                _capi.ptr(ws), C.byref(its), C.byref(rn), sums, _capi.stream_ptr())
             what = "smx_pcg_amg"
         else:
@@ -429,8 +442,8 @@ class DemagFieldLindholmGPU:
                                   _capi.ptr(c["dinv"]), mode, _capi.ptr(b), _capi.ptr(b_part), nbp,
                                   int(project), raw.width, raw.ld, _capi.ptr(raw.cols), _capi.ptr(raw.vals),
                                   _capi.ptr(self.bpos_cg), _capi.ptr(gb), _capi.ptr(x), int(warm),
-                                  _capi.ptr(self.cg_perm), float(rtol), float(self.ksp_atol), int(max_it), 16,
-                                  int(expected), _capi.ptr(ws), C.byref(its), C.byref(rn), sums,
+                                  _capi.ptr(self.cg_perm), float(rtol), float(self.ksp_atol), int(max_it),
+                                  -2 if spec else 16, int(expected), _capi.ptr(ws), C.byref(its), C.byref(rn), sums,
                                   _capi.stream_ptr())
             what = "smx_pcg_jacobi"
         if rc == _capi.SMX_ERR_NOT_CONVERGED:
@@ -454,18 +467,59 @@ class DemagFieldLindholmGPU:
             self.B_hmatrix_gpu.matvec_from(src, y, gather=gather, shift_ptr=shift_ptr,
                                            shift_scale=shift_scale)
 
-    def solve_potential(self, m_vec: torch.Tensor):
-        """u1, u2 for magnetization m (3N); leaves sum(u1) at u1_sum_ptr."""
+    def solve_potential(self, m_vec: torch.Tensor, speculative=None):
+        """u1, u2 for magnetization m (3N); leaves sum(u1) at u1_sum_ptr.
+        With speculative solves (the default once both iteration counts are
+        known) nothing waits for the device: the caller enqueues what reads
+        u1/u2 and then calls finish_potential, which checks both solves and,
+        if one had not converged within its launches, solves again waiting
+        and returns True (the caller then recomputes what it enqueued)."""
         _capi.require_device(m_vec, "m_vec", numel=self.local_size)
+        if self._pending_m is not None:
+            self.finish_potential()
+        if speculative is None:
+            speculative = self.cg_speculative and self._its_known
         warm = self.warm_start and self._have_u
         self.Div.spmv(m_vec, self.b_u1, kind=1, partials=self.b_part)
-        self.last_its_u1 = self.solve("u1", 0, self.u1, self.ws1, b=self.b_u1, b_part=self.b_part,
-                                      expected=self.last_its_u1, warm=warm)
+        its1 = self.solve("u1", 0, self.u1, self.ws1, b=self.b_u1, b_part=self.b_part,
+                          expected=self.last_its_u1, warm=warm, speculative=speculative)
         self.boundary_matvec(self.u1, self.y_rows, gather=self.bverts_gpu,
                              shift_ptr=self.u1_sum_ptr, shift_scale=1.0 / self.N)
-        self.last_its_u2 = self.solve("u2", 1, self.u2, self.ws2, gb=self.y_rows, project=0,
-                                      expected=self.last_its_u2, warm=warm)
+        its2 = self.solve("u2", 1, self.u2, self.ws2, gb=self.y_rows, project=0,
+                          expected=self.last_its_u2, warm=warm, speculative=speculative)
         self._have_u = True
+        if its1 < 0 or its2 < 0:
+            self._status_host[:8].copy_(self.ws1[:8], non_blocking=True)
+            self._status_host[8:].copy_(self.ws2[:8], non_blocking=True)
+            self._pending_m = m_vec
+        else:
+            self.last_its_u1, self.last_its_u2 = its1, its2
+            self._its_known = True
+
+    def finish_potential(self) -> bool:
+        """Check the speculative solves of the last solve_potential (one wait
+        for the stream); returns True when they had to be solved again."""
+        m_vec = self._pending_m
+        if m_vec is None:
+            return False
+        self._pending_m = None
+        torch.cuda.current_stream().synchronize()
+        st = self._status_host.numpy()
+        redo = False
+        for off, name in ((0, "u1"), (8, "u2")):
+            done, its, fail = st[off + _capi.PCG_DONE], st[off + _capi.PCG_ITERS], st[off + _capi.PCG_FAIL]
+            if fail == 3.0:
+                raise ValueError(f"{name} CG operator is not in solver form")
+            if done == 0.0:
+                redo = True
+                continue
+            if fail != 0.0:
+                self.converged = False      # the reference never checks KSP convergence: keep going
+            setattr(self, "last_its_" + name, int(its))
+        if redo:
+            self.speculative_redos += 1
+            self.solve_potential(m_vec, speculative=False)
+        return redo
 
     def field_args(self, m_vec, out_h_demag=None) -> _capi.FieldArgs:
         a = _capi.FieldArgs()
@@ -487,6 +541,8 @@ class DemagFieldLindholmGPU:
         self.solve_potential(m_vec)
         a = self.field_args(m_vec, out_vec)
         _capi.check(_capi.lib().smx_field_llg_rhs(C.byref(a), _capi.stream_ptr()), "smx_field_llg_rhs")
+        if self.finish_potential():
+            _capi.check(_capi.lib().smx_field_llg_rhs(C.byref(a), _capi.stream_ptr()), "smx_field_llg_rhs")
 
     def Energy_lumped_gpu(self, m_vec: torch.Tensor, h_vec: torch.Tensor) -> float:
         """E = -1/2 mu0 Ms sum_i V_i m_i.H_i / volume_scale (:706-724)."""

@@ -295,6 +295,8 @@ class EffectiveField:
         if self.demag_field is not None:
             self.demag_field.solve_potential(m_vec)
         self._launch(self._args(m_vec, h_eff=out_vec))
+        if self.demag_field is not None and self.demag_field.finish_potential():
+            self._launch(self._args(m_vec, h_eff=out_vec))
 
     def llg_rhs_vec(self, m_vec, out_vec):
         """dm/dt = LLG(m, H_eff(m)) into out_vec (:1261-1285)."""
@@ -304,6 +306,8 @@ class EffectiveField:
         if self.demag_field is not None:
             self.demag_field.solve_potential(m_vec)
         self._launch(self._args(m_vec, h_eff=self.H_eff_gpu, rhs=out_vec))
+        if self.demag_field is not None and self.demag_field.finish_potential():
+            self._launch(self._args(m_vec, h_eff=self.H_eff_gpu, rhs=out_vec))
 
     def compute_H_jac_vec(self, m_vec, out_vec):
         """Field for the Jacobian: everything except demag (:1232-1257)."""

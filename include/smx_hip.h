@@ -272,8 +272,23 @@ int64_t smx_reduce_ws_size(int64_t n);
  * work: smx_pcg_ws_size(n) doubles.  On return work[SMX_PCG_SUMX] holds
  * sum(x) on the device (the nullspace projection of x, read by later kernels).
  * its_out / rnorm_out / sums_out: host pointers filled on return (the call
- * synchronizes `stream` every `check_every` iterations).
+ * synchronizes `stream` every `check_every` iterations; the first wait comes
+ * after expected_its + 1 iterations when expected_its > 0).
+ * Speculative launch (check_every in [-64, -1], expected_its > 0): the call
+ * enqueues expected_its + 1 - check_every iterations and returns at once,
+ * without waiting (its_out = -1, rnorm_out and sums_out untouched).
+ * Iterations past convergence return at once on the device's done flag, so
+ * the result equals the waiting call's whenever the solve converged within
+ * them.  The status block at the start of work then tells, once the stream
+ * has reached it: work[SMX_PCG_DONE] != 0 when the solve stopped,
+ * work[SMX_PCG_ITERS] its iteration count, work[SMX_PCG_FAIL] 0 (converged),
+ * 1 (breakdown), 2 (max_it) or 3 (operator not in solver form).  A caller
+ * that finds DONE == 0 must solve again (waiting) before using x.  The same
+ * holds for smx_pcg_amg.
  * ---------------------------------------------------------------------- */
+#define SMX_PCG_DONE 0
+#define SMX_PCG_ITERS 1
+#define SMX_PCG_FAIL 5
 #define SMX_PCG_SUMX 6
 int64_t smx_pcg_ws_size(int64_t n);
 /* Solver form of a CG operator (replaces nothing in the reference: PETSc's

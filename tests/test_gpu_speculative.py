@@ -1,0 +1,56 @@
+"""Speculative CG launches in the demag potential solves.
+
+Once both iteration counts are known, each solve enqueues last count +
+margin iterations without waiting and the evaluation checks both status
+blocks once at its end.  Iterations past convergence return at once on the
+device's done flag, so every result must equal the waiting path's bit for
+bit; an iteration guess that falls short must be caught and repaired (the
+solve runs again, waiting).  Both preconditioners (AMG and Jacobi), the
+dense and the H-matrix boundary operator.
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device("cuda:0")
+
+
+def _unit(n, seed):
+    g = np.random.default_rng(seed).standard_normal((n, 3))
+    return (g / np.linalg.norm(g, axis=1)[:, None]).ravel()
+
+
+@pytest.mark.parametrize("pc,backend", [("gamg", "dense"), ("jacobi", "dense"), ("gamg", "hmatrix")])
+def test_speculative_equals_waiting(pc, backend):
+    from smx import mesh as M
+    from smx.llg import EffectiveField
+    m = M.cube_c1()
+    N = m.num_nodes
+    dk = dict(ksp_pc_type=pc, boundary_backend=backend, ksp_rtol_u1=1e-6, ksp_rtol_u2=1e-6, ksp_atol=1e-12)
+    kw = dict(Ms=8e5, Aex=1.3e-11, Ku=5e5, n_ani_vec=(0.0, 0.0, 1.0), alpha=0.5, use_demag=True, device=DEV)
+    hs = EffectiveField(m, demag_kwargs=dict(cg_speculative=True, **dk), **kw)
+    hw = EffectiveField(m, demag_kwargs=dict(cg_speculative=False, **dk), **kw)
+    states = [torch.from_numpy(_unit(N, s)).to(DEV) for s in range(6)]
+    rs, rw = torch.empty_like(states[0]), torch.empty_like(states[0])
+    for k, x in enumerate(states):
+        if k == 4:
+            hs.demag_field.last_its_u1 = 1      # a guess that falls short: must be repaired
+        hs.llg_rhs_vec(x, rs)
+        hw.llg_rhs_vec(x, rw)
+        torch.cuda.synchronize()
+        assert torch.equal(rs, rw), k
+        assert (hs.demag_field.last_its_u1, hs.demag_field.last_its_u2) == \
+            (hw.demag_field.last_its_u1, hw.demag_field.last_its_u2), k
+    d = hs.demag_field
+    print(f"[speculative {pc}/{backend}] its {d.last_its_u1}/{d.last_its_u2}, redos {d.speculative_redos}",
+          flush=True)
+    assert d.speculative_redos >= 1
+    # H_demag through compute_vec, and the energy path
+    hd_s, hd_w = torch.empty_like(rs), torch.empty_like(rs)
+    hs.demag_field.compute_vec(states[5], hd_s)
+    hw.demag_field.compute_vec(states[5], hd_w)
+    torch.cuda.synchronize()
+    assert torch.equal(hd_s, hd_w)
+    assert hs.compute_Energy_terms(states[2])["E_demag"] == hw.compute_Energy_terms(states[2])["E_demag"]
