@@ -17,13 +17,21 @@ and "mixed" (FP32 H-matrix factors / FP32 dense B, FP64 accumulation);
 `value` is the mixed-mode rate, the FP64 rate and the relative RHS error of
 mixed vs FP64 are reported beside it.
 
-`roofline` describes the dominant kernel of a step (the CG iteration kernel):
-algorithmic bytes per launch = ELL matrix nonzeros x 12 B + 88 B per row
-(reads of r, w, s, p, x, D^-1 and writes of r, w, s, p, x), divided by its
-average duration measured with HIP events on the launch stream.
-`matvec_roofline` gives the same for the boundary matvec (HBM-resident: the
-Infinity Cache is flushed before every timed launch).  `cpu_baseline` times
-the FP64 oracle path on the host on a bounded sample.
+`roofline` describes the boundary matvec, the metric's named kernel and the
+largest single operation of a film step (hm_gather + hm_far_z + hm_leaf):
+algorithmic bytes per matvec (factor entries x 4 B + near nonzeros x 12 B +
+the vectors) divided by its average duration, measured with HIP events on the
+launch stream after a read-only sweep that evicts the Infinity Cache;
+`traffic` is the HBM bytes of the same matvec from two rocprofv3 counter
+passes (FETCH_SIZE, WRITE_SIZE) of `--pmc-probe matvec`, scaled by a plain
+copy of known size.  `cg_roofline` gives one AMG-CG iteration of the u1 solve.
+`cpu_baseline` times the FP64 oracle path on the host on a bounded sample.
+
+Other modes: `--config cube` (dense B, plus the 64-RHS matrix-core product),
+`--config disk` (energy-gradient check), `--config adjoint` (gradient through
+500 Dormand-Prince steps), `--config paths` (torque terms, implicit BDF relax,
+minimizer, H-matrix build per compressor, with their kernel rooflines and host
+baselines).
 """
 from __future__ import annotations
 
