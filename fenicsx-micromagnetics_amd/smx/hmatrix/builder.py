@@ -556,11 +556,59 @@ def _batches(pairs, budget_bytes):
     return out
 
 
-def build_or_load_hmatrix(cache_path, geo: BoundaryGeometry, config: HMatrixBuildConfig | None = None,
-                          device=None, force_rebuild=False, verbose=False, cache_compressed=False):
+def build_or_load_hmatrix(cache_path, geo, *args, config: HMatrixBuildConfig | None = None, device=None,
+                          force_rebuild=False, verbose=None, cache_compressed=None, diag_jump=None, provider=None):
     """Reuse a cache when its geometry hash, build configuration and construction
-    version match (builder.py:814-927); otherwise build on the device and save."""
+    version match (builder.py:814-927); otherwise build on the device and save.
+
+    Two call forms: (cache_path, geo, config=None, device=None, ...) with the
+    boundary geometry, or the reference's (cache_path, Xb, diag_jump,
+    provider, config=None, force_rebuild=False, verbose=True,
+    cache_compressed=True) with a LindholmEntryProviderGPU (its geometry must
+    hold the same Xb; diag_jump is stored as given).  verbose and
+    cache_compressed default to False in the first form and to True (the
+    reference's defaults) in the second."""
     from pathlib import Path
+    if not isinstance(geo, BoundaryGeometry):
+        # the reference's positional form
+        Xb = np.asarray(geo, dtype=np.float64)
+        pos = list(args)
+        names = ["diag_jump", "provider", "config", "force_rebuild", "verbose", "cache_compressed"]
+        vals = dict(diag_jump=diag_jump, provider=provider, config=config, force_rebuild=force_rebuild,
+                    verbose=True if verbose is None else verbose,
+                    cache_compressed=True if cache_compressed is None else cache_compressed)
+        for k, v in zip(names, pos):
+            vals[k] = v
+        prov = vals["provider"]
+        if prov is None or not isinstance(getattr(prov, "geo", None), BoundaryGeometry):
+            raise TypeError("provider must be a LindholmEntryProviderGPU (from_opB of the boundary geometry)")
+        if prov.geo.Xb.shape != Xb.shape or not np.array_equal(prov.geo.Xb, Xb):
+            raise ValueError("Xb differs from the provider's boundary geometry")
+        dj = vals["diag_jump"]
+        if dj is None or np.asarray(dj).shape != (Xb.shape[0],):
+            raise ValueError("diag_jump must hold one value per boundary node")
+        data = build_or_load_hmatrix(cache_path, prov.geo, config=vals["config"],
+                                     device=device or torch.device("cuda", prov.device_id),
+                                     force_rebuild=vals["force_rebuild"], verbose=vals["verbose"],
+                                     cache_compressed=vals["cache_compressed"])
+        dj = np.asarray(dj, dtype=np.float64)
+        data.metadata["provider_signature"] = prov.cache_signature
+        if not np.array_equal(data.diag_jump, dj):
+            data.diag_jump = dj.copy()
+            if cache_path is not None:
+                data.save(cache_path, compressed=vals["cache_compressed"])
+        return data
+    if args:
+        # (cache_path, geo, config, device, force_rebuild, verbose, cache_compressed) positionally
+        names = ["config", "device", "force_rebuild", "verbose", "cache_compressed"]
+        kw = dict(zip(names, args))
+        config = kw.get("config", config)
+        device = kw.get("device", device)
+        force_rebuild = kw.get("force_rebuild", force_rebuild)
+        verbose = kw.get("verbose", verbose)
+        cache_compressed = kw.get("cache_compressed", cache_compressed)
+    verbose = bool(verbose) if verbose is not None else False
+    cache_compressed = bool(cache_compressed) if cache_compressed is not None else False
     cfg = config or HMatrixBuildConfig()
     if cache_path is not None and not force_rebuild and Path(cache_path).exists():
         try:

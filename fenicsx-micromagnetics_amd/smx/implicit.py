@@ -68,12 +68,18 @@ class JvContext:
                     "smx_llg_pc_build")
         self._pc_ready = True
 
-    def mult(self, x, y):
+    def mult(self, *args):
+        """y = shift x - J x; called as mult(x, y) or, like the reference's
+        matrix-free shell context, mult(A, x, y) (llg_module_GPU.py:1510-1517)."""
+        x, y = args[-2], args[-1]
         self.calls += 1
         self.hef.jac_vec_times_vec(x, self.Jv)
         _glue_axpby(self.hef.local_size, self.shift, x, -1.0, self.Jv, y)
 
-    def apply(self, x, y):
+    def apply(self, *args):
+        """y = P^-1 x with the 3x3 blocks; apply(x, y) or apply(pc, x, y)
+        (llg_module_GPU.py:1519-1544)."""
+        x, y = args[-2], args[-1]
         self.callsPre += 1
         if not self._pc_ready:
             y.copy_(x)

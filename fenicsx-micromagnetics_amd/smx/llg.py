@@ -329,6 +329,20 @@ class EffectiveField:
         self.current_time = float(t)
         self.llg_rhs_vec(y, f)
 
+    def ifunction(self, ts, t, y, ydot, f):
+        """f = ydot - RHS(t, y), the implicit residual of the time stepper
+        (llg_module_GPU.py:1328-1339)."""
+        self.current_time = float(t)
+        if getattr(self, "_dmdt", None) is None or self._dmdt.numel() != self.local_size:
+            self._dmdt = torch.empty(self.local_size, dtype=torch.float64, device=self.device)
+        self.llg_rhs_vec(y, self._dmdt)
+        _capi.require_device(ydot, "ydot", numel=self.local_size)
+        _capi.require_device(f, "f", numel=self.local_size)
+        _capi.check(_capi.lib().smx_vec_glue(3, self.local_size, _capi.ptr(ydot), _capi.ptr(self._dmdt), None, 1.0,
+                                             -1.0, None, None, 1.0, _capi.ptr(f), _capi.stream_ptr()),
+                    "smx_vec_glue")
+        return 0
+
     # ------------------------------------------------------------------
     def set_m_from_cpu(self, m0_array):
         self.m_gpu.copy_(torch.as_tensor(np.asarray(m0_array, dtype=np.float64).ravel()))

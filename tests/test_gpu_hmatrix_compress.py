@@ -219,3 +219,31 @@ def test_device_workspace_limit_keeps_everything_dense(c1):
     with pytest.raises(MemoryError):
         build_hmatrix_device(geo, HMatrixBuildConfig(leaf_size=32, gpu_memory_fraction=1e-12,
                                                      gpu_fallback_to_cpu=False), DEV)
+
+
+def test_provider_and_reference_call_form(c1, tmp_path):
+    """LindholmEntryProviderGPU fills blocks equal to the oracle's Lindholm
+    blocks; build_or_load_hmatrix in the reference's form (cache_path, Xb,
+    diag_jump, provider, config) builds the same operator as the geometry
+    form, saves the cache, and reloads it."""
+    from smx.hmatrix import HMatrixBuildConfig, LindholmEntryProviderGPU, build_or_load_hmatrix
+    m, geo, bd, B = c1
+    prov = LindholmEntryProviderGPU.from_opB(geo)
+    rng = np.random.default_rng(4)
+    rows = np.sort(rng.choice(geo.Nb, 70, replace=False))
+    cols = np.sort(rng.choice(geo.Nb, 90, replace=False))
+    blk = prov.fill_block(rows, cols)
+    ref = O.block_B(bd, rows, cols)
+    assert np.abs(blk - ref).max() <= 1e-13 * np.abs(ref).max()
+    assert prov.fill_block_device(rows, cols).shape == (70, 90) and prov.size == geo.Nb
+    cfg = HMatrixBuildConfig(epsilon=1e-6, leaf_size=32)
+    p = tmp_path / "h.npz"
+    a = build_or_load_hmatrix(p, geo.Xb, geo.diag_jump, prov, cfg, False, False)
+    b = build_or_load_hmatrix(None, geo, config=cfg, device=DEV)
+    assert [(x.row_start, x.col_start, x.rank) for x in a.far_blocks] == \
+        [(x.row_start, x.col_start, x.rank) for x in b.far_blocks]
+    assert np.array_equal(a.diag_jump, geo.diag_jump)
+    c = build_or_load_hmatrix(p, geo.Xb, geo.diag_jump, prov, cfg, verbose=False)
+    assert c.metadata.get("cache_loaded") is True
+    with pytest.raises(ValueError):
+        build_or_load_hmatrix(None, geo.Xb + 1.0, geo.diag_jump, prov, cfg)

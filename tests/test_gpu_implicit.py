@@ -197,3 +197,31 @@ def test_bdf_matches_oracle_integrator():
     assert abs(st["t"] - T) <= 1e-9 * T and abs(t_o - T) <= 1e-9 * T
     assert abs(st["steps"] - st_o["steps"]) <= 2
     assert err <= 1e-6
+
+
+def test_ifunction_and_shell_context_forms():
+    """ifunction gives ydot - RHS; the (A, x, y) / (pc, x, y) forms of the
+    shell context equal the two-argument ones."""
+    from smx.implicit import JvContext
+    from smx.llg import EffectiveField
+    m = _box()
+    N = m.num_nodes
+    hef = EffectiveField(m, Ms=8e5, Aex=1.3e-11, Ku=5e5, n_ani_vec=(0.0, 0.0, 1.0), alpha=0.1, device=DEV)
+    y = torch.from_numpy(_unit(N, 4)).to(DEV)
+    ydot = torch.randn(3 * N, dtype=torch.float64, device=DEV)
+    f = torch.empty_like(y)
+    r = torch.empty_like(y)
+    assert hef.ifunction(None, 1e-12, y, ydot, f) == 0
+    hef.llg_rhs_vec(y, r)
+    assert torch.equal(f, ydot - r)
+    hef.update_jac_state(y)
+    ctx = JvContext(hef)
+    ctx.update_pc_full_fast_gpu(1e12, exact=True, use_abs_kappa=False)
+    x = torch.randn_like(y)
+    a, b = torch.empty_like(y), torch.empty_like(y)
+    ctx.mult(x, a)
+    ctx.mult(None, x, b)
+    assert torch.equal(a, b)
+    ctx.apply(x, a)
+    ctx.apply(None, x, b)
+    assert torch.equal(a, b)
