@@ -68,7 +68,7 @@ class EffectiveField:
                  do_precess=1, use_demag=False, demag_method="lindholm", demag_kwargs=None,
                  H0_static=None, H_time_func=None, *, Jmagnitude=0.0, Jdir_vec=None, P=0.0, beta=0.0,
                  H_dl=0.0, H_fl=0.0, polarization_vec=None, device=None,
-                 operators: OperatorSet | None = None):
+                 operators: OperatorSet | None = None, overlap_local_terms=False):
         self.mesh = mesh
         self.device = device or default_device()
         self.ops = operators or OperatorSet(mesh, self.device)
@@ -171,6 +171,15 @@ class EffectiveField:
         self.dmdt_gpu = z(self.local_size)
         self.H_eff_gpu = z(self.local_size)
         self.H_demag_gpu = z(self.local_size) if self.use_demag else None
+        # overlap_local_terms: with demag, the local terms (exchange, DMI,
+        # anisotropies, external field) run on a second stream while the
+        # potentials are solved and the last kernel adds the demag gradient
+        # and forms the RHS.  Off by default: on the 500 nm film it measured
+        # 695 evaluations/s against 720 for the single fused kernel (the side
+        # kernel takes CUs from the latency-bound solver launches)
+        self.overlap_local_terms = bool(overlap_local_terms) and self.use_demag
+        self._h_loc = z(self.local_size) if self.overlap_local_terms else None
+        self._side_stream = torch.cuda.Stream(device=self.device) if self.overlap_local_terms else None
         self.Hm_gpu = z(self.local_size)
         self.Hv_gpu = z(self.local_size)
         self.M_state_gpu = z(self.local_size)
@@ -290,13 +299,42 @@ class EffectiveField:
         """out = K_lin x + H_ani(x) (:962-970)."""
         self._launch(self._args(x_vec, h_eff=out_vec, with_demag=False, with_h0=False))
 
+    def _field_with_demag(self, m_vec, h_eff, rhs):
+        """H_eff (and the RHS) with demag.  Overlapped form: kernel 1 (local
+        terms + external field -> _h_loc) on the side stream while the main
+        stream solves u1, applies B and solves u2; kernel 2 on the main
+        stream adds the demag gradient to _h_loc and forms the RHS."""
+        d = self.demag_field
+        if not self.overlap_local_terms:
+            d.solve_potential(m_vec)
+            a = self._args(m_vec, h_eff=h_eff, rhs=rhs)
+            self._launch(a)
+            if d.finish_potential():
+                self._launch(a)
+            return
+        main = torch.cuda.current_stream()
+        self._side_stream.wait_stream(main)
+        with torch.cuda.stream(self._side_stream):
+            self._launch(self._args(m_vec, h_eff=self._h_loc, with_demag=False))
+        d.solve_potential(m_vec)
+        main.wait_stream(self._side_stream)
+        a = self._args(m_vec, h_eff=h_eff, rhs=rhs, with_h0=False)
+        a.ex_vals = None
+        a.dmi_vals = None
+        a.ani_pref = 0.0
+        a.cub_pref = 0.0
+        a.h0 = _capi.ptr(self._h_loc)
+        a.h0_u = (C.c_double * 3)(0.0, 0.0, 0.0)
+        self._launch(a)
+        if d.finish_potential():
+            self._launch(a)
+
     def compute_H_eff_vec(self, m_vec, out_vec):
         """Full H_eff(m) (:972-1006)."""
         if self.demag_field is not None:
-            self.demag_field.solve_potential(m_vec)
+            self._field_with_demag(m_vec, out_vec, None)
+            return
         self._launch(self._args(m_vec, h_eff=out_vec))
-        if self.demag_field is not None and self.demag_field.finish_potential():
-            self._launch(self._args(m_vec, h_eff=out_vec))
 
     def llg_rhs_vec(self, m_vec, out_vec):
         """dm/dt = LLG(m, H_eff(m)) into out_vec (:1261-1285)."""
@@ -304,10 +342,9 @@ class EffectiveField:
         _capi.require_device(m_vec, "m_vec", numel=self.local_size)
         _capi.require_device(out_vec, "out_vec", numel=self.local_size)
         if self.demag_field is not None:
-            self.demag_field.solve_potential(m_vec)
+            self._field_with_demag(m_vec, self.H_eff_gpu, out_vec)
+            return
         self._launch(self._args(m_vec, h_eff=self.H_eff_gpu, rhs=out_vec))
-        if self.demag_field is not None and self.demag_field.finish_potential():
-            self._launch(self._args(m_vec, h_eff=self.H_eff_gpu, rhs=out_vec))
 
     def compute_H_jac_vec(self, m_vec, out_vec):
         """Field for the Jacobian: everything except demag (:1232-1257)."""

@@ -54,3 +54,34 @@ def test_speculative_equals_waiting(pc, backend):
     torch.cuda.synchronize()
     assert torch.equal(hd_s, hd_w)
     assert hs.compute_Energy_terms(states[2])["E_demag"] == hw.compute_Energy_terms(states[2])["E_demag"]
+
+
+def test_overlapped_local_terms_equal_fused():
+    """Local terms on the side stream + demag/RHS kernel against the single
+    fused kernel, all terms on (exchange, DMI, uniaxial, cubic, Zhang-Li,
+    SOT, static field): the two sum the field in another order, so equal to
+    rounding (1e-14 relative); the RHS, H_eff and H_demag."""
+    from smx import mesh as M
+    from smx.llg import EffectiveField
+    m = M.cube_c1()
+    N = m.num_nodes
+    kw = dict(Ms=8e5, Aex=1.3e-11, Ku=5e5, n_ani_vec=(0.0, 0.6, 0.8), D_int=2e-3, n0_int_vec=(0.0, 0.0, 1.0),
+              Kc1=-4e4, u1_cub=(1.0, 0.0, 0.0), u2_cub=(0.0, 1.0, 0.0), Jmagnitude=5e11, Jdir_vec=(1.0, 0.0, 0.0),
+              P=0.6, beta=0.04, H_dl=2e4, H_fl=-5e3, polarization_vec=(0.0, 1.0, 0.0), H0_static=(1e4, 0.0, 0.0),
+              alpha=0.3, use_demag=True, device=DEV, demag_kwargs=dict(ksp_rtol_u1=1e-12, ksp_rtol_u2=1e-12))
+    ho = EffectiveField(m, overlap_local_terms=True, **kw)
+    hf = EffectiveField(m, overlap_local_terms=False, **kw)
+    assert ho.overlap_local_terms and not hf.overlap_local_terms
+    for seed in range(3):
+        x = torch.from_numpy(_unit(N, 20 + seed)).to(DEV)
+        ro, rf = torch.empty_like(x), torch.empty_like(x)
+        ho.llg_rhs_vec(x, ro)
+        hf.llg_rhs_vec(x, rf)
+        torch.cuda.synchronize()
+        for a, b in ((ro, rf), (ho.H_eff_gpu, hf.H_eff_gpu), (ho.H_demag_gpu, hf.H_demag_gpu)):
+            assert float(torch.linalg.norm(a - b) / torch.linalg.norm(b)) <= 1e-14
+        ha, hb = torch.empty_like(x), torch.empty_like(x)
+        ho.compute_H_eff_vec(x, ha)
+        hf.compute_H_eff_vec(x, hb)
+        torch.cuda.synchronize()
+        assert float(torch.linalg.norm(ha - hb) / torch.linalg.norm(hb)) <= 1e-14
