@@ -179,7 +179,10 @@ def matvec_timing(d, flush_mb=512, reps=12):
         H = d.B_hmatrix_gpu
         fn = lambda: H.matvec(x, y)
         nbytes = H.algorithmic_bytes()
-        name = "hm_gather_kernel + hm_far_z_kernel + hm_leaf_kernel (H-matrix boundary matvec)"
+        where = "near launch" if H.near_in_z else "leaf launch"
+        log(f"near rows in the {where}; set-up times {H.near_mode_seconds}")
+        name = (f"hm_gather_kernel + {'hm_near_kernel + ' if H.near_in_z else ''}hm_far_z_kernel + hm_leaf_kernel "
+                f"(H-matrix boundary matvec; near rows in the {where}, chosen at set-up)")
     if not reps:
         return None, None, nbytes, name
     cold = event_time(fn, reps, flush)

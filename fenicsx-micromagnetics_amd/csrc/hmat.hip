@@ -195,6 +195,30 @@ static_assert(kZG == 8, "the reduce-scatter below is written for 8 ranks per tas
 //    SIMD.  With a scalar record the compiler serializes the row loads
 //    (290-470 us), so the record stays per-lane and the body stays branch-free
 //    with row kc-1 re-read (250 us).
+// Near rows in a launch of their own: one wavefront per row,
+// ynear[p] = Near[p, :] . xp (pairs of entries per lane, as the leaf kernel's
+// near loop), written by lane 0.  Spreads the near field over the whole
+// device when the leaves are too few to fill it.
+template <typename TN>
+__global__ __launch_bounds__(256) void hm_near_kernel(smx_hmatrix H) {
+  const int64_t p = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (p >= H.nb) return;  // wave-uniform; no barrier
+  const TN* __restrict__ NV = reinterpret_cast<const TN*>(H.near_val);
+  const double* __restrict__ xp = H.xp;
+  double s = 0.0;
+  const int64_t n1 = H.near_ptr[p + 1];
+  for (int64_t e = H.near_ptr[p] + 2 * lane; e < n1; e += 128) {
+    double v0, v1;
+    load2(NV + e, v0, v1);
+    const int2 c = *reinterpret_cast<const int2*>(H.near_idx + e);
+    s = fma(v0, xp[c.x], s);
+    s = fma(v1, xp[c.y], s);
+  }
+  s = smx::wave_total(s);
+  if (lane == 0) H.ynear[p] = s;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256, 8) void hm_far_z_kernel(smx_hmatrix H) {
   constexpr bool kF32 = sizeof(T) == 4;
@@ -275,7 +299,7 @@ __global__ __launch_bounds__(NW * 64) void hm_leaf_kernel(smx_hmatrix H, double*
   for (int32_t i = wid; i < l1 - l0; i += NW) {
     const int64_t p = l0 + i;
     double s = 0.0;
-    if (H.near_ptr) {
+    if (H.near_ptr && !H.ynear) {
       const int64_t n1 = H.near_ptr[p + 1];
       for (int64_t e = H.near_ptr[p] + 2 * lane; e < n1; e += 128) {
         double v0, v1;
@@ -295,6 +319,7 @@ __global__ __launch_bounds__(NW * 64) void hm_leaf_kernel(smx_hmatrix H, double*
       s = fma(u[3], zl[k + 3], s);
     }
     s = smx::wave_total(s);
+    if (H.ynear && H.near_ptr) s += H.ynear[p];  // near row summed in the z launch
     if (lane == 0) y[H.perm[p]] = H.diag_p[p] * xp[p] + s;
   }
 }
@@ -345,12 +370,22 @@ extern "C" int smx_hmat_matvec(const smx_hmatrix* H, const double* src, const do
               "far-field arrays missing");
   SMX_REQUIRE(H->kw_max >= 0 && H->kw_max <= SMX_HMAT_KW_MAX && (H->kw_max & 3) == 0, SMX_ERR_SHAPE,
               "kw_max %d must be a multiple of 4 in [0, %d]", H->kw_max, SMX_HMAT_KW_MAX);
-  SMX_REQUIRE(H->nleaf <= 0x7fffffffLL && H->nztask / 4 < 0x7fffffffLL, SMX_ERR_SHAPE,
+  SMX_REQUIRE(H->nleaf <= 0x7fffffffLL && H->nztask / 4 < 0x7fffffffLL && H->nb / 4 < 0x7fffffffLL, SMX_ERR_SHAPE,
               "too many leaves or z tasks");
+  SMX_REQUIRE(!H->ynear || !H->near_ptr || (H->near_idx && H->near_val), SMX_ERR_VALUE,
+              "near arrays missing");
   hipStream_t st = smx::as_stream(stream);
   hipLaunchKernelGGL(hm_gather_kernel, dim3(smx::grid_for(H->nb, 256)), dim3(256), 0, st, H->nb,
                      src, H->gidx, shift, shift_scale, H->xp);
   SMX_LAUNCH_CHECK();
+  if (H->ynear && H->near_ptr) {
+    const unsigned gn = (unsigned)((H->nb + 3) / 4);
+    if (H->near_dtype == SMX_F32)
+      hipLaunchKernelGGL(hm_near_kernel<float>, dim3(gn), dim3(256), 0, st, *H);
+    else
+      hipLaunchKernelGGL(hm_near_kernel<double>, dim3(gn), dim3(256), 0, st, *H);
+    SMX_LAUNCH_CHECK();
+  }
   if (H->nztask > 0) {
     const unsigned g = (unsigned)((H->nztask + 3) / 4);
     if (H->factor_dtype == SMX_F32)

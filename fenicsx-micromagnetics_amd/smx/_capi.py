@@ -71,7 +71,7 @@ class HMatrixArgs(C.Structure):
         ("nztask", I64), ("ztask", P), ("V", P),
         ("nleaf", I64), ("leaf_start", P), ("leaf_stop", P), ("leaf_u_off", P), ("leaf_kw", P),
         ("leaf_zg_off", P), ("leaf_zg", P), ("kw_max", I32), ("pad", I32),
-        ("U", P), ("xp", P), ("z", P),
+        ("U", P), ("xp", P), ("z", P), ("ynear", P),
     ]
 
 

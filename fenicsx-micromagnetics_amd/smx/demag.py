@@ -147,7 +147,8 @@ class DemagFieldLindholmGPU:
                  *, ksp_atol=1e-12, ksp_max_it_u1=2000, ksp_max_it_u2=12000,
                  warm_start=False, device=None, operators: OperatorSet | None = None,
                  geometry: BoundaryGeometry | None = None, hmatrix_prebuilt=None,
-                 hmatrix_near_fp32=False, cg_reorder=True, cg_speculative=True, ksp_pc_type="gamg",
+                 hmatrix_near_fp32=False, hmatrix_near_in_z="auto", cg_reorder=True, cg_speculative=True,
+                 ksp_pc_type="gamg",
                  ksp_pc_type_u2=None,
                  amg_fp32=None, amg_max_coarse=1024, amg_theta=0.0, verbose=False):
         """Keywords as the reference's constructor (Demag_Lindholm_GPU.py:101-155),
@@ -264,6 +265,7 @@ class DemagFieldLindholmGPU:
             self.B_hmatrix_cpu = hmatrix_prebuilt
             self.B_hmatrix_gpu = HMatrixGPUPackedFused(hmatrix_prebuilt, use_fp32=bool(hmatrix_use_fp32),
                                                        device=dev, near_fp32=bool(hmatrix_near_fp32),
+                                                       near_in_z=hmatrix_near_in_z,
                                                        boundary_gather=geo.bverts)
         else:
             from .hmatrix import build_device_hmatrix
@@ -273,7 +275,7 @@ class DemagFieldLindholmGPU:
                 max_temporary_block_bytes=int(hmatrix_max_temporary_block_bytes),
                 use_fp32=bool(hmatrix_use_fp32), cache_path=hmatrix_cache_path,
                 force_rebuild=bool(hmatrix_force_rebuild), verbose=bool(hmatrix_verbose),
-                near_fp32=bool(hmatrix_near_fp32), compressor=str(hmatrix_compressor),
+                near_fp32=bool(hmatrix_near_fp32), near_in_z=hmatrix_near_in_z, compressor=str(hmatrix_compressor),
                 gpu_device_id=int(hmatrix_build_device_id), gpu_memory_fraction=float(hmatrix_gpu_memory_fraction),
                 gpu_fallback_to_cpu=bool(hmatrix_gpu_fallback_to_cpu),
                 gpu_aca_residual_check_interval=int(hmatrix_gpu_aca_residual_check_interval),

@@ -175,8 +175,15 @@ def pack_operator(nb, row_start, row_stop, col_start, col_stop, rank, Ufac, Vfac
 
 class HMatrixGPUPackedFused:
     def __init__(self, data: HMatrixCPUData, threads_per_block: int = 128, use_fp32: bool = True,
-                 device=None, near_fp32: bool = False, boundary_gather: np.ndarray | None = None):
+                 device=None, near_fp32: bool = False, boundary_gather: np.ndarray | None = None,
+                 near_in_z="auto"):
         self.data = data
+        # near_in_z: the near rows are summed by a launch of their own (one
+        # wavefront per row) instead of inside the leaf launch.  It pays when
+        # the leaves are too few to fill the device (the 150 nm cube has 84);
+        # "auto" times both once (cache evicted) and keeps the faster.
+        self._near_mode_request = near_in_z
+        self.near_in_z = bool(near_in_z) if near_in_z != "auto" else True
         self.device = device or torch.device("cuda", torch.cuda.current_device())
         dev = self.device
         self.use_fp32 = bool(use_fp32)
@@ -205,8 +212,41 @@ class HMatrixGPUPackedFused:
         self.U, self.V = self.fwd["U"], self.fwd["V"]
         self.xp = torch.zeros(nb, dtype=torch.float64, device=dev)
         self.z = torch.zeros(max(host["nz"], 1), dtype=torch.float64, device=dev)
+        self.ynear = torch.zeros(max(nb, 1), dtype=torch.float64, device=dev)
         self._H = self._struct(self.fwd, self.gidx_plain)
         self._Hb = self._struct(self.fwd, self.gidx_boundary) if self.gidx_boundary is not None else None
+        self.near_mode_seconds = {}
+        if near_in_z == "auto" and self.near_nnz > 0:
+            self._choose_near_mode()
+
+    def _choose_near_mode(self, reps: int = 5):
+        """Time the matvec with the near rows in the z launch and in the leaf
+        launch (HIP events, after a warm-up, each timed call after a read
+        sweep that evicts the 256 MB Infinity Cache, as inside an evaluation
+        where the operator streams from HBM) and keep the faster."""
+        x = torch.randn(self.nb, dtype=torch.float64, device=self.device)
+        out = torch.empty_like(x)
+        sweep = torch.ones(128 * 1024 * 1024, dtype=torch.float32, device=self.device)   # 512 MB
+        for mode in (True, False):
+            self.near_in_z = mode
+            self._H = self._struct(self.fwd, self.gidx_plain)
+            for _ in range(2):
+                self.matvec(x, out)
+            ts = []
+            for _ in range(reps):
+                sweep.sum()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                self.matvec(x, out)
+                e1.record()
+                e1.synchronize()
+                ts.append(e0.elapsed_time(e1) * 1e-3)
+            self.near_mode_seconds[mode] = min(ts)
+        del sweep
+        self.near_in_z = self.near_mode_seconds[True] < self.near_mode_seconds[False]
+        self._H = self._struct(self.fwd, self.gidx_plain)
+        if self.gidx_boundary is not None:
+            self._Hb = self._struct(self.fwd, self.gidx_boundary)
 
     def _upload(self, host):
         return {k: (to_dev(v, self.device) if isinstance(v, np.ndarray) else v) for k, v in host.items()}
@@ -234,6 +274,7 @@ class HMatrixGPUPackedFused:
         H.kw_max = int(pk["kw_max"])
         H.U = p(pk["U"])
         H.xp, H.z = p(self.xp), p(self.z)
+        H.ynear = p(self.ynear) if (self.near_in_z and has_near) else None
         return H
 
     def transposed_pack(self):

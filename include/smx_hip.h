@@ -168,6 +168,9 @@ int smx_aca_batched(double* A, const smx_aca_task* tasks, int64_t ntasks, double
  *                                          (one wavefront per z task t)
  *   yp[p] = d_p xp[p] + Near[p,:] xp + U_L[p - l0, :] . zl_L;  y[perm[p]] = yp[p]
  *                                          (one workgroup per leaf L)
+ * With ynear set, Near[p,:] xp is computed by a launch of its own before the
+ * z launch, one wavefront per row, and the leaf launch streams U only: this
+ * spreads the near field over the whole device when the leaves are few.
  * Layout (every factor load is 16 bytes and aligned):
  *  - a z task covers kc <= SMX_HMAT_Z_GROUP rank rows of one far block's V:
  *    row g at V + v_off + g*ld, ld = n rounded up to 4 entries (zero padded),
@@ -221,6 +224,10 @@ typedef struct {
   const void* U;
   double* xp; /* workspace nb */
   double* z;  /* workspace: sum of ranks */
+  /* optional workspace nb (NULL: off): the near rows are summed by a launch
+   * of their own (one wavefront per row) into ynear[p], and the leaf launch
+   * reads that sum instead of the near row */
+  double* ynear;
 } smx_hmatrix;
 
 int smx_hmat_matvec(const smx_hmatrix* H, const double* src, const double* shift,

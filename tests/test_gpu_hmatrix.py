@@ -129,3 +129,22 @@ def test_c2_build_and_matvec():
         print(f"[c2 hmatrix fp32={fp32}] rel err {worst:.3e}, {bm['average_seconds']*1e6:.1f} us/matvec, "
               f"{H.algorithmic_bytes()/bm['average_seconds']/1e9:.0f} GB/s", flush=True)
         assert worst <= 1e-5
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_near_rows_in_z_launch(c1, fp32):
+    """Near rows summed by their own launch against the leaf kernel's own
+    near loop: equal to rounding (the two sum near and far parts of a row in
+    separate wave totals), forward and transposed."""
+    import torch
+    from smx.hmatrix import HMatrixBuildConfig, HMatrixGPUPackedFused, build_hmatrix_device
+    m, geo, bd, B = c1
+    data = build_hmatrix_device(geo, HMatrixBuildConfig(epsilon=1e-6, leaf_size=32), torch.device("cuda"))
+    Ha = HMatrixGPUPackedFused(data, use_fp32=fp32, near_in_z=True)
+    Hb = HMatrixGPUPackedFused(data, use_fp32=fp32, near_in_z=False)
+    x = torch.from_numpy(np.random.default_rng(9).standard_normal(geo.Nb)).cuda()
+    for fa, fb in ((Ha.matvec, Hb.matvec), (Ha.rmatvec, Hb.rmatvec)):
+        ya, yb = fa(x), fb(x)
+        assert float(torch.linalg.norm(ya - yb) / torch.linalg.norm(yb)) <= 1e-15
+    assert np.linalg.norm(Ha.matvec_numpy(x.cpu().numpy()) - B @ x.cpu().numpy()) <= \
+        (10 * 1e-6 + (1e-6 if fp32 else 0.0)) * np.linalg.norm(B @ x.cpu().numpy())
