@@ -70,7 +70,7 @@ class HMatrixArgs(C.Structure):
         ("near_ptr", P), ("near_idx", P), ("near_val", P), ("near_dtype", I32), ("factor_dtype", I32),
         ("nztask", I64), ("ztask", P), ("V", P),
         ("nleaf", I64), ("leaf_start", P), ("leaf_stop", P), ("leaf_u_off", P), ("leaf_kw", P),
-        ("leaf_zg_off", P), ("leaf_zg", P), ("kw_max", I32), ("pad", I32),
+        ("leaf_zg_off", P), ("leaf_zg", P), ("kw_max", I32), ("leaf_waves", I32),
         ("U", P), ("xp", P), ("z", P), ("ynear", P),
     ]
 

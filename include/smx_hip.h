@@ -220,7 +220,8 @@ typedef struct {
   const int64_t* leaf_zg_off; /* nleaf + 1 */
   const int32_t* leaf_zg;
   int32_t kw_max;
-  int32_t pad;
+  int32_t leaf_waves; /* wavefronts per leaf workgroup: 4, 8 or 16 (0: 8 for
+                       * FP32 factors, 16 for FP64) */
   const void* U;
   double* xp; /* workspace nb */
   double* z;  /* workspace: sum of ranks */

@@ -148,3 +148,25 @@ def test_near_rows_in_z_launch(c1, fp32):
         assert float(torch.linalg.norm(ya - yb) / torch.linalg.norm(yb)) <= 1e-15
     assert np.linalg.norm(Ha.matvec_numpy(x.cpu().numpy()) - B @ x.cpu().numpy()) <= \
         (10 * 1e-6 + (1e-6 if fp32 else 0.0)) * np.linalg.norm(B @ x.cpu().numpy())
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_leaf_workgroup_shapes_agree(c1, fp32):
+    """4, 8 and 16 wavefronts per leaf workgroup give the same bits (each
+    row is one wavefront's dot product either way); other values are
+    rejected before any launch."""
+    import torch
+    from smx import _capi
+    from smx.hmatrix import HMatrixBuildConfig, HMatrixGPUPackedFused, build_hmatrix_device
+    m, geo, bd, B = c1
+    data = build_hmatrix_device(geo, HMatrixBuildConfig(epsilon=1e-6, leaf_size=32), torch.device("cuda"))
+    H = HMatrixGPUPackedFused(data, use_fp32=fp32, near_in_z=False)
+    x = torch.from_numpy(np.random.default_rng(10).standard_normal(geo.Nb)).cuda()
+    ref = H.matvec(x).clone()
+    for nw in (4, 8, 16, 0):
+        H._H.leaf_waves = nw
+        assert torch.equal(H.matvec(x), ref), nw
+    H._H.leaf_waves = 12
+    with pytest.raises(ValueError):
+        H.matvec(x)
+    H._H.leaf_waves = 0
