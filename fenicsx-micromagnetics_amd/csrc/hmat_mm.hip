@@ -1,0 +1,294 @@
+// Multi-RHS H-matrix product Y = B X (up to 64 right-hand sides) with the
+// low-rank products on the matrix cores.
+//
+// Same packed operator as smx_hmat_matvec (z tasks over V, leaves over U,
+// near CSR); with 64 columns of X the far-field work becomes small GEMMs:
+//   stage:  XR[p][j] = X[j][gidx[p]] in the factor type, p < nb, j < 64
+//   z:      Z[z_off + g][j] = V_t[g, :] . XR[c0 .. c0+n)[j]      (MFMA)
+//   leaf:   YF[p][j] = U_L[p - l0, :] . Z[zg_L[k]][j]           (MFMA)
+//   near:   Y[j][perm[p]] = d_p XR[p][j] + Near[p,:] XR[:, j] + YF[p][j]
+// Every operand array is row-major with the 64 columns contiguous, so the
+// 16 lanes of a lane group read 64 contiguous bytes of one row.
+// MFMA per factor type (lane l, g = l / 16, c = l % 16), as the dense
+// multi-RHS product: FP32 v_mfma_f32_16x16x4_f32, D[m = 4g + i][n = c];
+// FP64 v_mfma_f64_16x16x4_f64, D[m = g + 4i][n = c]; operand A[m = c][k],
+// B[k][n = c], the k of MFMA j in a step of lane group g being
+// kk + KL g + j for both operands.  A z task fills rows 0..kc-1 of a
+// 16-row tile (rows past kc load zeros); a leaf workgroup's four wavefronts
+// cover its rows in 16-row tiles.  FP32 factors: X, Z in FP32 (one rounding
+// each), FP32 accumulation in the tiles; the near field and all sums in FP64.
+#include "smx_common.h"
+
+namespace {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kR = 64;  // right-hand sides per launch (columns of XR, Z, YF)
+
+template <typename T>
+struct Mt;
+
+// FP32: a step is 32 k; a lane loads 8 consecutive k = 8 MFMAs of 4.
+template <>
+struct Mt<float> {
+  using acc_t = f32x4;
+  static constexpr int kStep = 32;
+  static constexpr int kLane = 8;
+  // 8 factor entries from p, `room` entries (a multiple of 4) left in the row
+  __device__ static void load_a(const float* p, int64_t room, float (&v)[8]) {
+    if (room >= 4) {
+      const float4 a = *reinterpret_cast<const float4*>(p);
+      v[0] = a.x, v[1] = a.y, v[2] = a.z, v[3] = a.w;
+    } else {
+      v[0] = v[1] = v[2] = v[3] = 0.0f;
+    }
+    if (room >= 8) {
+      const float4 b = *reinterpret_cast<const float4*>(p + 4);
+      v[4] = b.x, v[5] = b.y, v[6] = b.z, v[7] = b.w;
+    } else {
+      v[4] = v[5] = v[6] = v[7] = 0.0f;
+    }
+  }
+  __device__ static void mma(acc_t& c, const float (&a)[8], const float (&b)[8]) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[j], b[j], c, 0, 0, 0);
+  }
+  __device__ static int row_of(int g, int i) { return 4 * g + i; }
+};
+
+// FP64: a step is 16 k; a lane loads 4 consecutive k = 4 MFMAs of 4.
+template <>
+struct Mt<double> {
+  using acc_t = f64x4;
+  static constexpr int kStep = 16;
+  static constexpr int kLane = 4;
+  __device__ static void load_a(const double* p, int64_t room, double (&v)[4]) {
+    if (room >= 4) {
+      const double2 a = *reinterpret_cast<const double2*>(p);
+      const double2 b = *reinterpret_cast<const double2*>(p + 2);
+      v[0] = a.x, v[1] = a.y, v[2] = b.x, v[3] = b.y;
+    } else {
+      v[0] = v[1] = v[2] = v[3] = 0.0;
+    }
+  }
+  __device__ static void mma(acc_t& c, const double (&a)[4], const double (&b)[4]) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) c = __builtin_amdgcn_mfma_f64_16x16x4f64(a[j], b[j], c, 0, 0, 0);
+  }
+  __device__ static int row_of(int g, int i) { return g + 4 * i; }
+};
+
+template <typename T>
+__global__ void hmm_stage_kernel(int64_t nb, int32_t nrhs, const double* __restrict__ X, int64_t ldx,
+                                 const int32_t* __restrict__ gidx, T* __restrict__ XR) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb * kR) return;
+  const int64_t p = i / kR;
+  const int j = (int)(i - p * kR);
+  XR[i] = (T)(j < nrhs ? X[(int64_t)j * ldx + gidx[p]] : 0.0);
+}
+
+// One wavefront per z task; four per workgroup.
+template <typename T>
+__global__ __launch_bounds__(256) void hmm_z_kernel(smx_hmatrix H, const T* __restrict__ XR,
+                                                    T* __restrict__ Z) {
+  using M = Mt<T>;
+  const int lane = threadIdx.x & 63;
+  const int64_t t = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  if (t >= H.nztask) return;  // wave-uniform; no barrier
+  const int g = lane >> 4, c = lane & 15;
+  const smx_hmat_ztask task = H.ztask[t];
+  const int64_t n = task.n;
+  const int kc = task.kc;
+  const int64_t ld = (n + 3) & ~(int64_t)3;
+  const bool row_ok = c < kc;
+  const T* __restrict__ Vr = reinterpret_cast<const T*>(H.V) + task.v_off + (int64_t)(row_ok ? c : 0) * ld;
+  const T* __restrict__ xs = XR + (int64_t)task.c0 * kR + c;  // column c of this task's x segment
+  typename M::acc_t acc[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) acc[q][i] = 0;
+  for (int64_t kk = 0; kk < n; kk += M::kStep) {
+    const int64_t kl = kk + M::kLane * g;
+    T a[M::kLane];
+    M::load_a(Vr + kl, row_ok ? ld - kl : 0, a);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      T b[M::kLane];
+#pragma unroll
+      for (int i = 0; i < M::kLane; ++i) b[i] = (kl + i < n) ? xs[(kl + i) * kR + 16 * q] : (T)0;
+      M::mma(acc[q], a, b);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int m = M::row_of(g, i);
+      if (m < kc) Z[(task.z_off + m) * kR + 16 * q + c] = (T)acc[q][i];
+    }
+}
+
+// One workgroup (four wavefronts) per leaf; wavefront w covers rows
+// 16 w + (0..15), 16 (w + 4) + (0..15), ... of the leaf.
+template <typename T>
+__global__ __launch_bounds__(256) void hmm_leaf_kernel(smx_hmatrix H, const T* __restrict__ Z,
+                                                       double* __restrict__ YF) {
+  using M = Mt<T>;
+  const int64_t L = blockIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int g = lane >> 4, c = lane & 15;
+  const int32_t l0 = H.leaf_start[L], rows = H.leaf_stop[L] - l0;
+  const int64_t kw = H.leaf_kw[L];
+  const int64_t z0 = H.leaf_zg_off[L];
+  const int64_t K = H.leaf_zg_off[L + 1] - z0;
+  const int32_t* __restrict__ zg = H.leaf_zg + z0;
+  for (int r0 = 16 * w; r0 < rows; r0 += 64) {
+    const int row = r0 + c;
+    const bool row_ok = row < rows;
+    const T* __restrict__ Ur = reinterpret_cast<const T*>(H.U) + H.leaf_u_off[L] + (int64_t)(row_ok ? row : 0) * kw;
+    typename M::acc_t acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) acc[q][i] = 0;
+    for (int64_t kk = 0; kk < kw; kk += M::kStep) {
+      const int64_t kl = kk + M::kLane * g;
+      T a[M::kLane];
+      M::load_a(Ur + kl, row_ok ? kw - kl : 0, a);
+      int64_t zi[M::kLane];
+#pragma unroll
+      for (int i = 0; i < M::kLane; ++i) zi[i] = (kl + i < K) ? (int64_t)zg[kl + i] : -1;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        T b[M::kLane];
+#pragma unroll
+        for (int i = 0; i < M::kLane; ++i) b[i] = zi[i] >= 0 ? Z[zi[i] * kR + 16 * q + c] : (T)0;
+        M::mma(acc[q], a, b);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int rr = r0 + M::row_of(g, i);
+        if (rr < rows) YF[(int64_t)(l0 + rr) * kR + 16 * q + c] = (double)acc[q][i];
+      }
+  }
+}
+
+// One wavefront per permuted row, lane j = right-hand side j.
+template <typename TN, typename T>
+__global__ __launch_bounds__(256) void hmm_near_kernel(smx_hmatrix H, int32_t nrhs,
+                                                       const T* __restrict__ XN,
+                                                       const double* __restrict__ YF, double* __restrict__ Y,
+                                                       int64_t ldy) {
+  const int64_t p = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int j = threadIdx.x & 63;
+  if (p >= H.nb) return;  // wave-uniform
+  double s = H.diag_p[p] * (double)XN[p * kR + j];
+  if (H.near_ptr) {
+    // the lanes load 64 entries of the row at once; each entry's value and
+    // column then go to every lane by readlane (uniform index), so the
+    // gathers of several entries are in flight together
+    const TN* __restrict__ NV = reinterpret_cast<const TN*>(H.near_val);
+    const int64_t e0 = H.near_ptr[p], e1 = H.near_ptr[p + 1];
+    for (int64_t eb = e0; eb < e1; eb += 64) {
+      const int64_t e = eb + j;
+      const double v = e < e1 ? (double)NV[e] : 0.0;
+      const int col = e < e1 ? H.near_idx[e] : 0;
+      const int cnt = (int)((e1 - eb) < 64 ? (e1 - eb) : 64);
+      const int lo = __double2loint(v), hi = __double2hiint(v);
+#pragma unroll 4
+      for (int t = 0; t < cnt; ++t) {
+        const int ct = __builtin_amdgcn_readlane(col, t);
+        const double vt = __hiloint2double(__builtin_amdgcn_readlane(hi, t), __builtin_amdgcn_readlane(lo, t));
+        s = fma(vt, (double)XN[(int64_t)ct * kR + j], s);
+      }
+    }
+  }
+  s += YF[p * kR + j];
+  if (j < nrhs) Y[(int64_t)j * ldy + H.perm[p]] = s;
+}
+
+// Workspace: XR (factor type, nb x 64), YF (FP64, nb x 64), Z (factor
+// type, nz x 64), each 256-byte aligned.
+struct MmLayout {
+  int64_t xr, yf, z;
+};
+
+MmLayout mm_layout(const smx_hmatrix* H, int64_t nz) {
+  const int64_t ts = H->factor_dtype == SMX_F32 ? 4 : 8;
+  auto up = [](int64_t b) { return (b + 255) / 256 * 256; };
+  MmLayout L;
+  L.xr = up(ts * kR * H->nb);
+  L.yf = up(8 * kR * H->nb);
+  L.z = up(ts * kR * (nz > 0 ? nz : 1));
+  return L;
+}
+
+template <typename T>
+int run_hmm(const smx_hmatrix* H, int64_t nz, const double* X, int64_t ldx, int32_t nrhs, double* Y,
+            int64_t ldy, char* work, hipStream_t st) {
+  const MmLayout Lo = mm_layout(H, nz);
+  T* XR = reinterpret_cast<T*>(work);
+  double* YF = reinterpret_cast<double*>(work + Lo.xr);
+  T* Z = reinterpret_cast<T*>(work + Lo.xr + Lo.yf);
+  const int64_t ns = H->nb * kR;
+  hipLaunchKernelGGL(hmm_stage_kernel<T>, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, st, H->nb, nrhs, X,
+                     ldx, H->gidx, XR);
+  SMX_LAUNCH_CHECK();
+  if (H->nztask > 0 && nz > 0) {
+    hipLaunchKernelGGL(hmm_z_kernel<T>, dim3((unsigned)((H->nztask + 3) / 4)), dim3(256), 0, st, *H, XR, Z);
+    SMX_LAUNCH_CHECK();
+  }
+  if (H->nleaf > 0) {
+    hipLaunchKernelGGL(hmm_leaf_kernel<T>, dim3((unsigned)H->nleaf), dim3(256), 0, st, *H, Z, YF);
+    SMX_LAUNCH_CHECK();
+  }
+  const unsigned gn = (unsigned)((H->nb + 3) / 4);
+  if (H->near_dtype == SMX_F32)
+    hipLaunchKernelGGL((hmm_near_kernel<float, T>), dim3(gn), dim3(256), 0, st, *H, nrhs, XR, YF, Y, ldy);
+  else
+    hipLaunchKernelGGL((hmm_near_kernel<double, T>), dim3(gn), dim3(256), 0, st, *H, nrhs, XR, YF, Y, ldy);
+  SMX_LAUNCH_CHECK();
+  return SMX_OK;
+}
+
+}  // namespace
+
+extern "C" int64_t smx_hmat_matmat_ws_size(const smx_hmatrix* H, int64_t nz) {
+  if (!H || H->nb <= 0) return 0;
+  const MmLayout Lo = mm_layout(H, nz);
+  return Lo.xr + Lo.yf + Lo.z;
+}
+
+extern "C" int smx_hmat_matmat(const smx_hmatrix* H, int64_t nz, const double* X, int64_t ldx, int32_t nrhs,
+                               double* Y, int64_t ldy, void* work, int64_t work_bytes, void* stream) {
+  SMX_REQUIRE(H, SMX_ERR_VALUE, "null H");
+  SMX_REQUIRE(H->nb >= 0 && H->nleaf >= 0 && H->nztask >= 0 && nz >= 0, SMX_ERR_SHAPE, "negative size");
+  SMX_REQUIRE(nrhs >= 1 && nrhs <= kR, SMX_ERR_SHAPE, "nrhs %d must be in [1, %d]", nrhs, kR);
+  if (H->nb == 0) return SMX_OK;
+  SMX_REQUIRE(ldx >= H->nb && ldy >= H->nb, SMX_ERR_SHAPE, "ldx/ldy below nb");
+  SMX_REQUIRE(X && Y && work && H->gidx && H->perm && H->diag_p && H->leaf_start && H->leaf_stop &&
+                  H->leaf_kw && H->leaf_u_off && H->leaf_zg_off && H->U,
+              SMX_ERR_VALUE, "null pointer argument");
+  SMX_REQUIRE(H->factor_dtype == SMX_F32 || H->factor_dtype == SMX_F64, SMX_ERR_TYPE,
+              "factor dtype must be FP32 or FP64");
+  SMX_REQUIRE(H->near_dtype == SMX_F32 || H->near_dtype == SMX_F64, SMX_ERR_TYPE,
+              "near dtype must be FP32 or FP64");
+  SMX_REQUIRE(H->nztask == 0 || (H->ztask && H->V && H->leaf_zg), SMX_ERR_VALUE, "far-field arrays missing");
+  SMX_REQUIRE(!H->near_ptr || (H->near_idx && H->near_val), SMX_ERR_VALUE, "near arrays missing");
+  SMX_REQUIRE(H->nleaf <= 0x7fffffffLL && H->nztask / 4 < 0x7fffffffLL && H->nb * kR < 0x7fffffffLL * 256LL,
+              SMX_ERR_SHAPE, "operator too large");
+  const int64_t need = smx_hmat_matmat_ws_size(H, nz);
+  SMX_REQUIRE(work_bytes >= need, SMX_ERR_OOM, "workspace of %lld bytes, %lld needed", (long long)work_bytes,
+              (long long)need);
+  hipStream_t st = smx::as_stream(stream);
+  if (H->factor_dtype == SMX_F32)
+    return run_hmm<float>(H, nz, X, ldx, nrhs, Y, ldy, reinterpret_cast<char*>(work), st);
+  return run_hmm<double>(H, nz, X, ldx, nrhs, Y, ldy, reinterpret_cast<char*>(work), st);
+}

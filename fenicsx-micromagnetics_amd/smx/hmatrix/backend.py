@@ -25,6 +25,8 @@ leaves, near^T), packed on first use.
 """
 from __future__ import annotations
 
+import ctypes as C
+
 import numpy as np
 import torch
 
@@ -85,19 +87,27 @@ def pack_operator(nb, row_start, row_stop, col_start, col_stop, rank, Ufac, Vfac
     z_off = np.zeros(nfar, dtype=np.int64)
     if nfar:
         z_off[1:] = np.cumsum(rk)[:-1]
-    ng = (rk + _ZG - 1) // _ZG
-    tb = np.repeat(np.arange(nfar, dtype=np.int64), ng)
-    k0 = (np.arange(int(ng.sum()), dtype=np.int64) - np.repeat(np.cumsum(ng) - ng, ng)) * _ZG
-    kc = np.minimum(_ZG, rk[tb] - k0)
-    order = np.argsort(-(n[tb] * kc), kind="stable")
-    tb, k0, kc = tb[order], k0[order], kc[order]
-    zt = np.zeros(max(tb.size, 1), dtype=ZTASK_DTYPE)
-    zt["v_off"][:tb.size] = v_off[tb] + k0 * ld[tb]
-    zt["z_off"][:tb.size] = z_off[tb] + k0
-    zt["c0"][:tb.size] = c0[tb]
-    zt["n"][:tb.size] = n[tb]
-    zt["kc"][:tb.size] = kc
-    P.update(V=V, ztask=zt.view(np.int64), nztask=int(tb.size))
+    def tasks(group):
+        """Tasks of <= group rank rows of one block (rows contiguous in V),
+        largest first."""
+        ng = (rk + group - 1) // group
+        tb = np.repeat(np.arange(nfar, dtype=np.int64), ng)
+        k0 = (np.arange(int(ng.sum()), dtype=np.int64) - np.repeat(np.cumsum(ng) - ng, ng)) * group
+        kc = np.minimum(group, rk[tb] - k0)
+        order = np.argsort(-(n[tb] * kc), kind="stable")
+        tb, k0, kc = tb[order], k0[order], kc[order]
+        zt = np.zeros(max(tb.size, 1), dtype=ZTASK_DTYPE)
+        zt["v_off"][:tb.size] = v_off[tb] + k0 * ld[tb]
+        zt["z_off"][:tb.size] = z_off[tb] + k0
+        zt["c0"][:tb.size] = c0[tb]
+        zt["n"][:tb.size] = n[tb]
+        zt["kc"][:tb.size] = kc
+        return zt, int(tb.size)
+
+    zt, nzt = tasks(_ZG)
+    # 16-row tasks for the multi-RHS product: one MFMA tile of rank rows each
+    zt16, nzt16 = tasks(16)
+    P.update(V=V, ztask=zt.view(np.int64), nztask=nzt, ztask16=zt16.view(np.int64), nztask16=nzt16)
     # ---- leaves ----
     starts, stops = leaf_intervals(nb, r0, r1)
     nleaf = starts.size
@@ -212,6 +222,7 @@ class HMatrixGPUPackedFused:
         self.U, self.V = self.fwd["U"], self.fwd["V"]
         self.xp = torch.zeros(nb, dtype=torch.float64, device=dev)
         self.z = torch.zeros(max(host["nz"], 1), dtype=torch.float64, device=dev)
+        self._nz = int(host["nz"])
         self.ynear = torch.zeros(max(nb, 1), dtype=torch.float64, device=dev)
         self._H = self._struct(self.fwd, self.gidx_plain)
         self._Hb = self._struct(self.fwd, self.gidx_boundary) if self.gidx_boundary is not None else None
@@ -312,6 +323,34 @@ class HMatrixGPUPackedFused:
             out = torch.empty_like(x_gpu)
         _capi.check(_capi.lib().smx_hmat_matvec(_capi.byref(self._H), x_gpu.data_ptr(), None, 0.0,
                                                 out.data_ptr(), _capi.stream_ptr()), "smx_hmat_matvec")
+        return out
+
+    def matmat(self, X: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        """Y = B X for up to 64 right-hand sides: X and Y are (k, nb) device
+        tensors, row j = right-hand side j (boundary ordering).  The low-rank
+        stages run as MFMA tiles over the columns (smx_hmat_matmat)."""
+        if X.dim() != 2 or X.shape[1] != self.nb or not 1 <= X.shape[0] <= 64:
+            raise ValueError(f"Expected X of shape (k <= 64, {self.nb}), got {tuple(X.shape)}.")
+        _capi.require_device(X, "X", numel=X.numel())
+        X = X.contiguous()
+        k = int(X.shape[0])
+        if out is None:
+            out = torch.empty_like(X)
+        _capi.require_device(out, "out", numel=X.numel())
+        nz = int(self._nz)
+        if getattr(self, "_Hmm", None) is None:
+            # the multi-RHS product walks the 16-row task table (one MFMA tile each)
+            Hm = self._struct(self.fwd, self.gidx_plain)
+            Hm.ztask = _capi.ptr(self.fwd["ztask16"])
+            Hm.nztask = int(self.fwd["nztask16"])
+            Hm.ynear = None
+            self._Hmm = Hm
+        need = int(_capi.lib().smx_hmat_matmat_ws_size(C.byref(self._Hmm), nz))
+        if getattr(self, "_mm_work", None) is None or self._mm_work.numel() < need:
+            self._mm_work = torch.empty(max(need, 1), dtype=torch.uint8, device=self.device)
+        _capi.check(_capi.lib().smx_hmat_matmat(C.byref(self._Hmm), nz, _capi.ptr(X), self.nb, k, _capi.ptr(out),
+                                                self.nb, _capi.ptr(self._mm_work), self._mm_work.numel(),
+                                                _capi.stream_ptr()), "smx_hmat_matmat")
         return out
 
     def matvec_numpy(self, x: np.ndarray) -> np.ndarray:

@@ -231,6 +231,19 @@ typedef struct {
   double* ynear;
 } smx_hmatrix;
 
+/* Y = B X for nrhs <= 64 right-hand sides (column j of X at X + j*ldx, of
+ * Y at Y + j*ldy, indexed through H->gidx like smx_hmat_matvec's src, no
+ * shift), with the low-rank products on the matrix cores: the z stage
+ * Z = V_b X_b and the leaf stage U_L Z_L become 16 x 16 MFMA tiles over the
+ * 64 columns (FP32 factors: v_mfma_f32_16x16x4_f32 with X and Z rounded to
+ * FP32 once; FP64: v_mfma_f64_16x16x4_f64); the near field and the diagonal
+ * sum in FP64 over the staged X.  No counterpart call in the reference, whose matvec takes one
+ * vector (backend_fused.py:496-512): it replaces a loop of matvecs.  nz: the
+ * packed operator's z entries (the largest z_off + kc); work: at least
+ * smx_hmat_matmat_ws_size(H, nz) bytes. */
+int64_t smx_hmat_matmat_ws_size(const smx_hmatrix* H, int64_t nz);
+int smx_hmat_matmat(const smx_hmatrix* H, int64_t nz, const double* X, int64_t ldx, int32_t nrhs,
+                    double* Y, int64_t ldy, void* work, int64_t work_bytes, void* stream);
 int smx_hmat_matvec(const smx_hmatrix* H, const double* src, const double* shift,
                     double shift_scale, double* y, void* stream);
 

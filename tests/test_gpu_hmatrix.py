@@ -170,3 +170,24 @@ def test_leaf_workgroup_shapes_agree(c1, fp32):
     with pytest.raises(ValueError):
         H.matvec(x)
     H._H.leaf_waves = 0
+
+
+@pytest.mark.parametrize("fp32,k", [(False, 64), (False, 7), (True, 64), (True, 1)])
+def test_matmat_against_matvecs(fp32, k):
+    """Multi-RHS product (MFMA low-rank stages) against k separate matvecs on
+    the 150 nm cube: FP64 factors 1e-12 (the MFMA tiles sum in another order),
+    FP32 factors 1e-6 (X and Z rounded to FP32 once) relative per column."""
+    import torch
+    from smx import mesh as M
+    from smx.boundary import build_boundary_geometry
+    from smx.hmatrix import HMatrixBuildConfig, HMatrixGPUPackedFused, build_hmatrix_device
+    m = M.cube_c2()
+    geo = build_boundary_geometry(m.X, m.tets)
+    data = build_hmatrix_device(geo, HMatrixBuildConfig(epsilon=1e-6, leaf_size=64), torch.device("cuda"))
+    H = HMatrixGPUPackedFused(data, use_fp32=fp32, near_in_z=False)
+    X = torch.from_numpy(np.random.default_rng(k).standard_normal((k, geo.Nb))).cuda()
+    Y = H.matmat(X)
+    ref = torch.stack([H.matvec(X[j]) for j in range(k)])
+    err = (torch.linalg.norm(Y - ref, dim=1) / torch.linalg.norm(ref, dim=1)).max().item()
+    print(f"[matmat fp32={fp32} k={k}] max column rel err {err:.2e}", flush=True)
+    assert err <= (1e-6 if fp32 else 1e-12)

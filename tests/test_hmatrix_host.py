@@ -430,3 +430,35 @@ def _same_near(a, b):
     a tile's triangle contributions in another order than the oracle)."""
     assert np.array_equal(a.indptr, b.indptr) and np.array_equal(a.indices, b.indices)
     assert np.abs(a.data - b.data).max() < 1e-15
+
+
+def test_sixteen_row_task_table(c1):
+    """The multi-RHS product's 16-row z tasks: in bounds of V and z, every z
+    entry produced once, and the same z as the matvec's 8-row tasks."""
+    from smx.hmatrix.backend import ZTASK_DTYPE, pack_operator
+    _, geo, bd = c1
+    d = build_hmatrix_device(geo, HMatrixBuildConfig(leaf_size=16, epsilon=1e-4), ops=HostOps(geo, bd))
+    fb = d.far_blocks
+    pk = pack_operator(d.size, [b.row_start for b in fb], [b.row_stop for b in fb], [b.col_start for b in fb],
+                       [b.col_stop for b in fb], [b.rank for b in fb], [b.U for b in fb], [b.V for b in fb],
+                       d.near_csr, np.float32, np.float64)
+    V = pk["V"].astype(np.float64)
+    xp = np.random.default_rng(0).standard_normal(d.size)[np.asarray(d.perm)]
+
+    def zvec(tasks, cap):
+        z = np.full(pk["nz"], np.nan)
+        hits = np.zeros(pk["nz"], dtype=int)
+        for t in tasks:
+            n, kc, vo, zo, c0 = int(t["n"]), int(t["kc"]), int(t["v_off"]), int(t["z_off"]), int(t["c0"])
+            ld = (n + 3) // 4 * 4
+            assert 1 <= kc <= cap and vo % 4 == 0 and vo + kc * ld <= V.size and zo + kc <= pk["nz"]
+            assert 0 <= c0 and c0 + n <= d.size
+            z[zo:zo + kc] = V[vo:vo + kc * ld].reshape(kc, ld)[:, :n] @ xp[c0:c0 + n]
+            hits[zo:zo + kc] += 1
+        assert np.all(hits == 1)
+        return z
+
+    z8 = zvec(pk["ztask"].view(ZTASK_DTYPE)[:pk["nztask"]], 8)
+    z16 = zvec(pk["ztask16"].view(ZTASK_DTYPE)[:pk["nztask16"]], 16)
+    assert pk["nztask16"] < pk["nztask"]
+    assert np.abs(z8 - z16).max() <= 1e-12
