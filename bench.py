@@ -623,15 +623,21 @@ def hmatrix_multi_rhs_timing(h64, hmx, k=64, flush_mb=512):
     for name, H in (("fp64", H64), ("fp32", Hmx)):
         Y = H.matmat(X)
         t = event_time(lambda: H.matmat(X, Y), 8, flush)
+        tc = event_time(lambda: H.matmat(X, Y, near_tiles=False), 8, flush)
+        H.matmat(X, Y)
         tv = event_time(lambda: H.matvec(X[0]), 8, flush)
         flops = 2.0 * H.far_storage_entries * k
         out[name] = {"us": t * 1e6, "us_per_rhs": t * 1e6 / k, "matvec_us": tv * 1e6,
+                     "us_near_csr_rows": tc * 1e6,
                      "speedup_vs_k_matvecs": k * tv / t, "lowrank_TFLOPs": flops / t / 1e12,
                      "rel_err_vs_fp64_matvecs": float((torch.linalg.norm(Y - ref, dim=1)
-                                                       / torch.linalg.norm(ref, dim=1)).max())}
-        log(f"H-matrix {k}-RHS product {name}: {t * 1e6:.0f} us ({k * tv / t:.1f}x {k} matvecs)")
+                                                       / torch.linalg.norm(ref, dim=1)).max()),
+                     "near_tiles": dict(H.near_tile_stats)}
+        log(f"H-matrix {k}-RHS product {name}: {t * 1e6:.0f} us ({k * tv / t:.1f}x {k} matvecs; "
+            f"near as CSR rows {tc * 1e6:.0f} us) tiles {H.near_tile_stats}")
     return {"nrhs": k, "kernel": "hmm_z_kernel + hmm_leaf_kernel (v_mfma_f32_16x16x4 / f64_16x16x4) + "
-                                 "hmm_near_kernel", "note": "times include the X staging launch", **out}
+                                 "hmm_near_tile_kernel (v_mfma_f64_16x16x4 over dense 16-row near tiles)",
+            "note": "times include the X staging launch", **out}
 
 
 def build_case(config, dev):
@@ -787,6 +793,8 @@ def main():
                          "+ boundary matvecs)")
     ap.add_argument("--evals-only", action="store_true",
                     help="after set-up run only the mixed-precision evaluations (for kernel traces)")
+    ap.add_argument("--multi-rhs-only", action="store_true",
+                    help="film: after set-up time only the 64-column H-matrix product (for kernel traces)")
     args = ap.parse_args()
     global PC_TYPE, PC_TYPE_U2
     PC_TYPE = args.pc
@@ -814,6 +822,11 @@ def main():
     dev = torch.device("cuda", 0)
     mesh, mat, h64, hmx, pool, work, extra, setup_s = build_case(args.config, dev)
     log(f"setup {setup_s:.1f}s")
+    if args.multi_rhs_only:
+        if args.config != "film":
+            raise SystemExit("--multi-rhs-only needs the film H-matrix operator")
+        print(json.dumps({"workload": work, "multi_rhs_boundary_product": hmatrix_multi_rhs_timing(h64, hmx)}))
+        return
     out = torch.empty(3 * mesh.num_nodes, dtype=torch.float64, device=dev)
 
     def run(hef, steps, warmup):

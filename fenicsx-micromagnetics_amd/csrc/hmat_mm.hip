@@ -7,6 +7,7 @@
 //   z:      Z[z_off + g][j] = V_t[g, :] . XR[c0 .. c0+n)[j]      (MFMA)
 //   leaf:   YF[p][j] = U_L[p - l0, :] . Z[zg_L[k]][j]           (MFMA)
 //   near:   Y[j][perm[p]] = d_p XR[p][j] + Near[p,:] XR[:, j] + YF[p][j]
+//           (dense 16-row near tiles on the FP64 matrix cores, or CSR rows)
 // Every operand array is row-major with the 64 columns contiguous, so the
 // 16 lanes of a lane group read 64 contiguous bytes of one row.
 // MFMA per factor type (lane l, g = l / 16, c = l % 16), as the dense
@@ -16,7 +17,10 @@
 // kk + KL g + j for both operands.  A z task fills rows 0..kc-1 of a
 // 16-row tile (rows past kc load zeros); a leaf workgroup's four wavefronts
 // cover its rows in 16-row tiles.  FP32 factors: X, Z in FP32 (one rounding
-// each), FP32 accumulation in the tiles; the near field and all sums in FP64.
+// each), FP32 accumulation in the tiles; the near field (over an FP64 copy of
+// X) and all sums in FP64.  Z tasks and leaves run largest first; the
+// near-tile launch maps workgroups to XCDs by contiguous eighths of its
+// (surface-ordered) tiles.
 #include "smx_common.h"
 
 namespace {
@@ -25,6 +29,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kR = 64;  // right-hand sides per launch (columns of XR, Z, YF)
+constexpr int kNtCols = 1024;  // near-tile columns staged in LDS at a time (a multiple of 64)
 
 template <typename T>
 struct Mt;
@@ -81,13 +86,24 @@ struct Mt<double> {
 
 template <typename T>
 __global__ void hmm_stage_kernel(int64_t nb, int32_t nrhs, const double* __restrict__ X, int64_t ldx,
-                                 const int32_t* __restrict__ gidx, T* __restrict__ XR) {
+                                 const int32_t* __restrict__ gidx, T* __restrict__ XR, double* __restrict__ XD) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nb * kR) return;
   const int64_t p = i / kR;
   const int j = (int)(i - p * kR);
-  XR[i] = (T)(j < nrhs ? X[(int64_t)j * ldx + gidx[p]] : 0.0);
+  const double v = j < nrhs ? X[(int64_t)j * ldx + gidx[p]] : 0.0;
+  XR[i] = (T)v;
+  if (XD) XD[i] = v;
 }
+
+// Workgroup b of a launch over n items runs on XCD b % 8; item
+// (b % 8) * ceil(n / 8) + b / 8 gives each XCD a contiguous eighth of the
+// items (stored in surface order), so its L2 serves one part of the surface.
+__device__ inline int64_t xcd_item(int64_t n) {
+  const int64_t per = (n + 7) / 8;
+  return (int64_t)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+}
+inline unsigned xcd_grid(int64_t n) { return (unsigned)((n + 7) / 8 * 8); }
 
 // One wavefront per z task; four per workgroup.
 template <typename T>
@@ -181,15 +197,15 @@ __global__ __launch_bounds__(256) void hmm_leaf_kernel(smx_hmatrix H, const T* _
 }
 
 // One wavefront per permuted row, lane j = right-hand side j.
-template <typename TN, typename T>
+template <typename TN>
 __global__ __launch_bounds__(256) void hmm_near_kernel(smx_hmatrix H, int32_t nrhs,
-                                                       const T* __restrict__ XN,
+                                                       const double* __restrict__ XN,
                                                        const double* __restrict__ YF, double* __restrict__ Y,
                                                        int64_t ldy) {
   const int64_t p = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
   const int j = threadIdx.x & 63;
   if (p >= H.nb) return;  // wave-uniform
-  double s = H.diag_p[p] * (double)XN[p * kR + j];
+  double s = H.diag_p[p] * XN[p * kR + j];
   if (H.near_ptr) {
     // the lanes load 64 entries of the row at once; each entry's value and
     // column then go to every lane by readlane (uniform index), so the
@@ -206,7 +222,7 @@ __global__ __launch_bounds__(256) void hmm_near_kernel(smx_hmatrix H, int32_t nr
       for (int t = 0; t < cnt; ++t) {
         const int ct = __builtin_amdgcn_readlane(col, t);
         const double vt = __hiloint2double(__builtin_amdgcn_readlane(hi, t), __builtin_amdgcn_readlane(lo, t));
-        s = fma(vt, (double)XN[(int64_t)ct * kR + j], s);
+        s = fma(vt, XN[(int64_t)ct * kR + j], s);
       }
     }
   }
@@ -214,10 +230,130 @@ __global__ __launch_bounds__(256) void hmm_near_kernel(smx_hmatrix H, int32_t nr
   if (j < nrhs) Y[(int64_t)j * ldy + H.perm[p]] = s;
 }
 
+// Near field from dense tiles: one workgroup per tile, the tile's columns
+// staged in LDS, its four wavefronts taking every fourth 16-column step (v_mfma_f64_16x16x4_f64 whatever the
+// value and factor types; A = the tile's rows, row m of the tile on lane
+// c = m; B = the staged X rows of its columns).  The four partial tiles meet
+// in LDS and are summed in wavefront order, wavefront w finishing the
+// right-hand sides 16 w .. 16 w + 15: diagonal and far sum added, the row
+// written back over YF[p] (whole 128-byte segments); hmm_unstage_kernel then
+// moves the rows to Y.
+template <typename TN>
+__device__ inline void load_near4(const TN* p, bool ok, double (&v)[4]);
+template <>
+__device__ inline void load_near4<double>(const double* p, bool ok, double (&v)[4]) {
+  if (ok) {
+    const double2 a = *reinterpret_cast<const double2*>(p);
+    const double2 b = *reinterpret_cast<const double2*>(p + 2);
+    v[0] = a.x, v[1] = a.y, v[2] = b.x, v[3] = b.y;
+  } else {
+    v[0] = v[1] = v[2] = v[3] = 0.0;
+  }
+}
+template <>
+__device__ inline void load_near4<float>(const float* p, bool ok, double (&v)[4]) {
+  if (ok) {
+    const float4 a = *reinterpret_cast<const float4*>(p);
+    v[0] = a.x, v[1] = a.y, v[2] = a.z, v[3] = a.w;
+  } else {
+    v[0] = v[1] = v[2] = v[3] = 0.0;
+  }
+}
+
+template <typename TN>
+__global__ __launch_bounds__(256) void hmm_near_tile_kernel(smx_hmatrix H, smx_hmat_near_tiles NT,
+                                                            const double* __restrict__ XN,
+                                                            double* __restrict__ YF) {
+  using M = Mt<double>;
+  __shared__ double part[4][4][4][64];  // [wavefront][q][i][lane], 32 KB
+  __shared__ int32_t cs[kNtCols];       // the tile's columns, kNtCols at a time
+  const int lane = threadIdx.x & 63;
+  // tiles are stored in row order; XCD x (workgroups b with b % 8 == x) takes
+  // the x-th eighth of them, so each XCD's L2 holds the X rows of one part of
+  // the surface
+  const int64_t t = xcd_item(NT.ntile);
+  if (t >= NT.ntile) return;  // workgroup-uniform, before any barrier
+  const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int g = lane >> 4, c = lane & 15;
+  const smx_hmat_ntile tile = NT.tile[t];
+  const int64_t w = tile.w;
+  const int64_t ld = (w + 3) & ~(int64_t)3;
+  const int nrows = tile.nrows;
+  const bool row_ok = c < nrows;
+  const TN* __restrict__ Ar = reinterpret_cast<const TN*>(NT.val) + tile.a_off + (int64_t)(row_ok ? c : 0) * ld;
+  const int32_t* __restrict__ cl = NT.col + tile.col_off;
+  const double* __restrict__ xs = XN + c;
+  f64x4 acc[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) acc[q][i] = 0;
+  for (int64_t k0 = 0; k0 < w; k0 += kNtCols) {
+    const int kn = (int)(w - k0 < kNtCols ? w - k0 : kNtCols);
+    __syncthreads();  // the previous chunk's columns are consumed
+    for (int e = threadIdx.x; e < kn; e += 256) cs[e] = cl[k0 + e];
+    __syncthreads();
+    for (int kk = M::kStep * wv; kk < kn; kk += 4 * M::kStep) {
+      const int kc = kk + M::kLane * g;  // column within the chunk
+      double a[4];
+      load_near4<TN>(Ar + k0 + kc, row_ok && k0 + kc < ld, a);
+      int64_t ci[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) ci[i] = (kc + i < kn) ? (int64_t)cs[kc + i] : -1;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        double b[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) b[i] = ci[i] >= 0 ? xs[ci[i] * kR + 16 * q] : 0.0;
+        M::mma(acc[q], a, b);
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) part[wv][q][i][lane] = acc[q][i];
+  __syncthreads();
+  const int32_t* __restrict__ rows = NT.row + 16 * t;
+  const int j = 16 * wv + c;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int m = M::row_of(g, i);
+    if (m >= nrows) continue;
+    const int64_t p = rows[m];
+    double s = part[0][wv][i][lane];
+    s += part[1][wv][i][lane];
+    s += part[2][wv][i][lane];
+    s += part[3][wv][i][lane];
+    s += H.diag_p[p] * XN[p * kR + j] + YF[p * kR + j];
+    YF[p * kR + j] = s;
+  }
+}
+
+// Y[j][i] = YF[iperm[i]][j]: one workgroup per 64 output indices, the rows
+// read whole (512 bytes each) into LDS and written out along i.
+__global__ __launch_bounds__(256) void hmm_unstage_kernel(int64_t nb, int32_t nrhs, const double* __restrict__ YF,
+                                                          const int32_t* __restrict__ iperm,
+                                                          double* __restrict__ Y, int64_t ldy) {
+  __shared__ double tb[64][kR + 1];
+  const int64_t i0 = (int64_t)blockIdx.x * 64;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wv = tid >> 6;
+  for (int r = wv; r < 64; r += 4) {
+    const int64_t i = i0 + r;
+    if (i < nb) tb[r][lane] = YF[(int64_t)iperm[i] * kR + lane];
+  }
+  __syncthreads();
+  const int64_t i = i0 + lane;
+  if (i >= nb) return;
+  for (int j = wv; j < nrhs; j += 4) Y[(int64_t)j * ldy + i] = tb[lane][j];
+}
+
 // Workspace: XR (factor type, nb x 64), YF (FP64, nb x 64), Z (factor
-// type, nz x 64), each 256-byte aligned.
+// type, nz x 64), and for FP32 factors XD (FP64 copy of XR for the near
+// field and the diagonal), each 256-byte aligned.
 struct MmLayout {
-  int64_t xr, yf, z;
+  int64_t xr, yf, z, xd;
 };
 
 MmLayout mm_layout(const smx_hmatrix* H, int64_t nz) {
@@ -227,19 +363,22 @@ MmLayout mm_layout(const smx_hmatrix* H, int64_t nz) {
   L.xr = up(ts * kR * H->nb);
   L.yf = up(8 * kR * H->nb);
   L.z = up(ts * kR * (nz > 0 ? nz : 1));
+  L.xd = ts == 8 ? 0 : up(8 * kR * H->nb);  // FP64 copy of X for the near field
   return L;
 }
 
 template <typename T>
-int run_hmm(const smx_hmatrix* H, int64_t nz, const double* X, int64_t ldx, int32_t nrhs, double* Y,
-            int64_t ldy, char* work, hipStream_t st) {
+int run_hmm(const smx_hmatrix* H, const smx_hmat_near_tiles* NT, int64_t nz, const double* X, int64_t ldx,
+            int32_t nrhs, double* Y, int64_t ldy, char* work, hipStream_t st) {
   const MmLayout Lo = mm_layout(H, nz);
   T* XR = reinterpret_cast<T*>(work);
   double* YF = reinterpret_cast<double*>(work + Lo.xr);
   T* Z = reinterpret_cast<T*>(work + Lo.xr + Lo.yf);
+  double* XD = Lo.xd ? reinterpret_cast<double*>(work + Lo.xr + Lo.yf + Lo.z) : nullptr;
+  const double* XN = XD ? XD : reinterpret_cast<const double*>(XR);  // T is double when XD is absent
   const int64_t ns = H->nb * kR;
   hipLaunchKernelGGL(hmm_stage_kernel<T>, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, st, H->nb, nrhs, X,
-                     ldx, H->gidx, XR);
+                     ldx, H->gidx, XR, XD);
   SMX_LAUNCH_CHECK();
   if (H->nztask > 0 && nz > 0) {
     hipLaunchKernelGGL(hmm_z_kernel<T>, dim3((unsigned)((H->nztask + 3) / 4)), dim3(256), 0, st, *H, XR, Z);
@@ -249,11 +388,23 @@ int run_hmm(const smx_hmatrix* H, int64_t nz, const double* X, int64_t ldx, int3
     hipLaunchKernelGGL(hmm_leaf_kernel<T>, dim3((unsigned)H->nleaf), dim3(256), 0, st, *H, Z, YF);
     SMX_LAUNCH_CHECK();
   }
+  if (NT) {
+    if (NT->val_dtype == SMX_F32)
+      hipLaunchKernelGGL(hmm_near_tile_kernel<float>, dim3(xcd_grid(NT->ntile)), dim3(256), 0, st, *H, *NT, XN, YF);
+    else
+      hipLaunchKernelGGL(hmm_near_tile_kernel<double>, dim3(xcd_grid(NT->ntile)), dim3(256), 0, st, *H, *NT, XN,
+                         YF);
+    SMX_LAUNCH_CHECK();
+    hipLaunchKernelGGL(hmm_unstage_kernel, dim3((unsigned)((H->nb + 63) / 64)), dim3(256), 0, st, H->nb, nrhs, YF,
+                       NT->iperm, Y, ldy);
+    SMX_LAUNCH_CHECK();
+    return SMX_OK;
+  }
   const unsigned gn = (unsigned)((H->nb + 3) / 4);
   if (H->near_dtype == SMX_F32)
-    hipLaunchKernelGGL((hmm_near_kernel<float, T>), dim3(gn), dim3(256), 0, st, *H, nrhs, XR, YF, Y, ldy);
+    hipLaunchKernelGGL(hmm_near_kernel<float>, dim3(gn), dim3(256), 0, st, *H, nrhs, XN, YF, Y, ldy);
   else
-    hipLaunchKernelGGL((hmm_near_kernel<double, T>), dim3(gn), dim3(256), 0, st, *H, nrhs, XR, YF, Y, ldy);
+    hipLaunchKernelGGL(hmm_near_kernel<double>, dim3(gn), dim3(256), 0, st, *H, nrhs, XN, YF, Y, ldy);
   SMX_LAUNCH_CHECK();
   return SMX_OK;
 }
@@ -263,11 +414,12 @@ int run_hmm(const smx_hmatrix* H, int64_t nz, const double* X, int64_t ldx, int3
 extern "C" int64_t smx_hmat_matmat_ws_size(const smx_hmatrix* H, int64_t nz) {
   if (!H || H->nb <= 0) return 0;
   const MmLayout Lo = mm_layout(H, nz);
-  return Lo.xr + Lo.yf + Lo.z;
+  return Lo.xr + Lo.yf + Lo.z + Lo.xd;
 }
 
-extern "C" int smx_hmat_matmat(const smx_hmatrix* H, int64_t nz, const double* X, int64_t ldx, int32_t nrhs,
-                               double* Y, int64_t ldy, void* work, int64_t work_bytes, void* stream) {
+extern "C" int smx_hmat_matmat(const smx_hmatrix* H, const smx_hmat_near_tiles* NT, int64_t nz, const double* X,
+                               int64_t ldx, int32_t nrhs, double* Y, int64_t ldy, void* work, int64_t work_bytes,
+                               void* stream) {
   SMX_REQUIRE(H, SMX_ERR_VALUE, "null H");
   SMX_REQUIRE(H->nb >= 0 && H->nleaf >= 0 && H->nztask >= 0 && nz >= 0, SMX_ERR_SHAPE, "negative size");
   SMX_REQUIRE(nrhs >= 1 && nrhs <= kR, SMX_ERR_SHAPE, "nrhs %d must be in [1, %d]", nrhs, kR);
@@ -282,13 +434,20 @@ extern "C" int smx_hmat_matmat(const smx_hmatrix* H, int64_t nz, const double* X
               "near dtype must be FP32 or FP64");
   SMX_REQUIRE(H->nztask == 0 || (H->ztask && H->V && H->leaf_zg), SMX_ERR_VALUE, "far-field arrays missing");
   SMX_REQUIRE(!H->near_ptr || (H->near_idx && H->near_val), SMX_ERR_VALUE, "near arrays missing");
-  SMX_REQUIRE(H->nleaf <= 0x7fffffffLL && H->nztask / 4 < 0x7fffffffLL && H->nb * kR < 0x7fffffffLL * 256LL,
+  SMX_REQUIRE(H->nleaf <= 0x7ffffff0LL && H->nztask / 4 < 0x7ffffff0LL && H->nb * kR < 0x7fffffffLL * 256LL,
               SMX_ERR_SHAPE, "operator too large");
+  if (NT) {
+    SMX_REQUIRE(NT->ntile >= (H->nb + 15) / 16 && NT->ntile <= H->nb && NT->ntile <= 0x7ffffff0LL,
+                SMX_ERR_SHAPE, "near tiles: %lld tiles for %lld rows", (long long)NT->ntile, (long long)H->nb);
+    SMX_REQUIRE(NT->tile && NT->row && NT->col && NT->val && NT->iperm, SMX_ERR_VALUE, "near tile arrays missing");
+    SMX_REQUIRE(NT->val_dtype == SMX_F32 || NT->val_dtype == SMX_F64, SMX_ERR_TYPE,
+                "near tile dtype must be FP32 or FP64");
+  }
   const int64_t need = smx_hmat_matmat_ws_size(H, nz);
   SMX_REQUIRE(work_bytes >= need, SMX_ERR_OOM, "workspace of %lld bytes, %lld needed", (long long)work_bytes,
               (long long)need);
   hipStream_t st = smx::as_stream(stream);
   if (H->factor_dtype == SMX_F32)
-    return run_hmm<float>(H, nz, X, ldx, nrhs, Y, ldy, reinterpret_cast<char*>(work), st);
-  return run_hmm<double>(H, nz, X, ldx, nrhs, Y, ldy, reinterpret_cast<char*>(work), st);
+    return run_hmm<float>(H, NT, nz, X, ldx, nrhs, Y, ldy, reinterpret_cast<char*>(work), st);
+  return run_hmm<double>(H, NT, nz, X, ldx, nrhs, Y, ldy, reinterpret_cast<char*>(work), st);
 }

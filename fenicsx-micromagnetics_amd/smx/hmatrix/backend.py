@@ -183,6 +183,127 @@ def pack_operator(nb, row_start, row_stop, col_start, col_stop, rank, Ufac, Vfac
     return P
 
 
+NTILE_DTYPE = np.dtype([("a_off", np.int64), ("col_off", np.int64), ("w", np.int32),
+                        ("nrows", np.int32)])   # smx_hmat_ntile, 24 bytes
+NTILE_ROWS = 16
+
+
+def pack_near_tiles(nb, near_csr, leaf_starts, near_dt):
+    """The near rows (permuted coordinates) as dense tiles of <= 16 rows for
+    the multi-RHS product: tile t holds rows row[16 t + s] (s < nrows, -1
+    beyond) and the union of their columns, col[col_off .. col_off + w), its
+    values row-major at a_off with rows padded to a multiple of 4 entries
+    (zeros where a row has no entry).
+
+    Rows of one leaf interval (``leaf_starts``: the ascending starts of the
+    intervals that partition 0..nb) share their near blocks, but their
+    patterns still differ: on a flat face the coplanar entries are exact
+    zeros and are not stored.  The rows of a leaf are therefore grouped by
+    their column pattern (an order-free hash of the column set); each group
+    gives full 16-row tiles, and the groups' remainders share tiles in
+    pattern order.  Tiles are stored in the order of their first row (the
+    device maps contiguous eighths of them to the eight XCDs, so each XCD's
+    L2 serves the X rows of one part of the surface).  Every row is in
+    exactly one tile, rows without near entries included."""
+    near = near_csr.tocsr().copy()
+    near.sum_duplicates()
+    near.sort_indices()
+    ptr = near.indptr.astype(np.int64)
+    cols = near.indices.astype(np.int64)
+    cnt = np.diff(ptr)
+    key = np.zeros(nb, dtype=np.uint64)
+    if cols.size:
+        h = (cols.astype(np.uint64) + np.uint64(1)) * np.uint64(0x9E3779B97F4A7C15)
+        h ^= h >> np.uint64(31)
+        h *= np.uint64(0xBF58476D1CE4E5B9)
+        live = cnt > 0
+        key[live] = np.add.reduceat(h, ptr[:-1][live])
+    leaf = np.searchsorted(np.asarray(leaf_starts, dtype=np.int64), np.arange(nb, dtype=np.int64),
+                           side="right") - 1
+    o = np.lexsort((np.arange(nb), key, leaf))          # rows by leaf, pattern, index
+    lf, ky = leaf[o], key[o]
+    newg = np.ones(nb, dtype=bool)
+    newg[1:] = (lf[1:] != lf[:-1]) | (ky[1:] != ky[:-1])
+    gid = np.cumsum(newg) - 1
+    gstart = np.flatnonzero(newg)
+    gsize = np.diff(np.append(gstart, nb))
+    full = (np.arange(nb) - gstart[gid]) < (gsize[gid] // NTILE_ROWS) * NTILE_ROWS
+    rows_full = o[full].reshape(-1, NTILE_ROWS)         # whole groups of 16 equal patterns
+    rest, rest_leaf = o[~full], lf[~full]
+    nr = rest.size
+    newl = np.ones(nr, dtype=bool)
+    newl[1:] = rest_leaf[1:] != rest_leaf[:-1]
+    lstart = np.flatnonzero(newl)
+    r_in_l = np.arange(nr) - lstart[np.cumsum(newl) - 1]
+    chunk_new = newl | (r_in_l % NTILE_ROWS == 0)       # remainders of a leaf, 16 at a time
+    cid = np.cumsum(chunk_new) - 1
+    nrest_t = int(chunk_new.sum())
+    ntile = rows_full.shape[0] + nrest_t
+    tile_of = np.empty(nb, dtype=np.int64)
+    slot_of = np.empty(nb, dtype=np.int64)
+    tile_of[rows_full.ravel()] = np.repeat(np.arange(rows_full.shape[0]), NTILE_ROWS)
+    slot_of[rows_full.ravel()] = np.tile(np.arange(NTILE_ROWS), rows_full.shape[0])
+    cstart = np.flatnonzero(chunk_new)
+    tile_of[rest] = rows_full.shape[0] + cid
+    slot_of[rest] = np.arange(nr) - cstart[cid]
+    nrows = np.bincount(tile_of, minlength=ntile).astype(np.int64)
+    # column union per tile
+    rowid = np.repeat(np.arange(nb, dtype=np.int64), cnt)
+    te = tile_of[rowid]
+    u, inv = np.unique(te * nb + cols, return_inverse=True)
+    ut = u // nb
+    w = np.bincount(ut, minlength=ntile).astype(np.int64)
+    # storage order: by first row
+    first = np.full(ntile, nb, dtype=np.int64)
+    np.minimum.at(first, tile_of, np.arange(nb, dtype=np.int64))
+    order = np.argsort(first, kind="stable")            # storage slot -> tile
+    rank = np.empty(ntile, dtype=np.int64)
+    rank[order] = np.arange(ntile)
+    ld = _round4(w)
+    a_sz = nrows * ld
+    a_off = np.zeros(ntile, dtype=np.int64)
+    a_off[order] = np.concatenate([[0], np.cumsum(a_sz[order])[:-1]]) if ntile else []
+    col_off = np.zeros(ntile, dtype=np.int64)
+    col_off[order] = np.concatenate([[0], np.cumsum(w[order])[:-1]]) if ntile else []
+    # u is sorted by (tile, column): its tile runs start at the old-order offsets
+    old_off = np.concatenate([[0], np.cumsum(w)[:-1]]) if ntile else np.zeros(0, np.int64)
+    pos = inv - old_off[te]
+    col = np.empty(max(int(w.sum()), 1), dtype=np.int32)
+    col[:1] = 0
+    col[col_off[ut] + (np.arange(u.size) - old_off[ut])] = (u % nb).astype(np.int32)
+    val = np.zeros(max(int(a_sz.sum()), 4), dtype=near_dt)
+    val[a_off[te] + slot_of[rowid] * ld[te] + pos] = near.data
+    row = np.full((ntile, NTILE_ROWS), -1, dtype=np.int32)
+    row[rank[tile_of], slot_of] = np.arange(nb, dtype=np.int32)
+    tiles = np.zeros(ntile, dtype=NTILE_DTYPE)
+    tiles["a_off"] = a_off[order]
+    tiles["col_off"] = col_off[order]
+    tiles["w"] = w[order]
+    tiles["nrows"] = nrows[order]
+    return dict(tile=tiles, row=row.ravel(), col=col, val=val, ntile=ntile,
+                dense_entries=int(a_sz.sum()), mfma_entries=int((NTILE_ROWS * ld).sum()), nnz=int(near.nnz),
+                w_max=int(w.max()) if ntile else 0, w_mean=float(w.mean()) if ntile else 0.0)
+
+
+def near_from_tiles(nb, tiles, row, col, val):
+    """The near matrix (permuted coordinates, scipy CSR) a tile pack encodes
+    (host check of pack_near_tiles)."""
+    import scipy.sparse as sp
+    rs, cs, vs = [], [], []
+    row = np.asarray(row).reshape(-1, NTILE_ROWS)
+    for t in range(tiles.size):
+        a0, c0, w, nr = (int(tiles["a_off"][t]), int(tiles["col_off"][t]), int(tiles["w"][t]),
+                         int(tiles["nrows"][t]))
+        ld = (w + 3) // 4 * 4
+        A = np.asarray(val[a0:a0 + nr * ld]).reshape(nr, ld)[:, :w]
+        rr, kk = np.nonzero(A)
+        rs.append(row[t, rr])
+        cs.append(np.asarray(col[c0:c0 + w])[kk])
+        vs.append(A[rr, kk])
+    cat = (lambda a: np.concatenate(a) if a else np.zeros(0))
+    return sp.csr_matrix((cat(vs), (cat(rs).astype(np.int64), cat(cs).astype(np.int64))), shape=(nb, nb))
+
+
 class HMatrixGPUPackedFused:
     def __init__(self, data: HMatrixCPUData, threads_per_block: int = 128, use_fp32: bool = True,
                  device=None, near_fp32: bool = False, boundary_gather: np.ndarray | None = None,
@@ -325,10 +446,34 @@ class HMatrixGPUPackedFused:
                                                 out.data_ptr(), _capi.stream_ptr()), "smx_hmat_matvec")
         return out
 
-    def matmat(self, X: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+    def near_tiles(self):
+        """The near rows as dense 16-row tiles on the device (pack_near_tiles),
+        built on first use; returns the smx_hmat_near_tiles argument."""
+        if getattr(self, "_NT", None) is None:
+            fb = self.data.far_blocks
+            starts, _ = leaf_intervals(self.nb, [b.row_start for b in fb], [b.row_stop for b in fb])
+            host = pack_near_tiles(self.nb, self.data.near_csr, starts, self._ndt)
+            self.near_tile_stats = {k: host[k] for k in ("ntile", "dense_entries", "mfma_entries", "nnz", "w_max", "w_mean")}
+            dev = self._upload({k: host[k] for k in ("row", "col", "val")})
+            dev["tile"] = to_dev(host["tile"].view(np.int64), self.device)
+            perm = np.asarray(self.data.perm, dtype=np.int64)
+            iperm = np.empty(self.nb, dtype=np.int32)
+            iperm[perm] = np.arange(self.nb, dtype=np.int32)
+            dev["iperm"] = to_dev(iperm, self.device)
+            NT = _capi.NearTilesArgs()
+            NT.ntile = int(host["ntile"])
+            NT.tile, NT.row, NT.col, NT.val, NT.iperm = (_capi.ptr(dev[k]) for k in ("tile", "row", "col", "val",
+                                                                                    "iperm"))
+            NT.val_dtype = self.near_dtype
+            self._nt_dev, self._NT = dev, NT
+        return self._NT
+
+    def matmat(self, X: torch.Tensor, out: torch.Tensor | None = None, near_tiles: bool = True) -> torch.Tensor:
         """Y = B X for up to 64 right-hand sides: X and Y are (k, nb) device
         tensors, row j = right-hand side j (boundary ordering).  The low-rank
-        stages run as MFMA tiles over the columns (smx_hmat_matmat)."""
+        stages run as MFMA tiles over the columns (smx_hmat_matmat); the near
+        field as FP64 MFMA tiles over dense 16-row near tiles (near_tiles=True,
+        packed on the first call) or one wavefront per CSR row."""
         if X.dim() != 2 or X.shape[1] != self.nb or not 1 <= X.shape[0] <= 64:
             raise ValueError(f"Expected X of shape (k <= 64, {self.nb}), got {tuple(X.shape)}.")
         _capi.require_device(X, "X", numel=X.numel())
@@ -348,7 +493,8 @@ class HMatrixGPUPackedFused:
         need = int(_capi.lib().smx_hmat_matmat_ws_size(C.byref(self._Hmm), nz))
         if getattr(self, "_mm_work", None) is None or self._mm_work.numel() < need:
             self._mm_work = torch.empty(max(need, 1), dtype=torch.uint8, device=self.device)
-        _capi.check(_capi.lib().smx_hmat_matmat(C.byref(self._Hmm), nz, _capi.ptr(X), self.nb, k, _capi.ptr(out),
+        NT = C.byref(self.near_tiles()) if near_tiles else None
+        _capi.check(_capi.lib().smx_hmat_matmat(C.byref(self._Hmm), NT, nz, _capi.ptr(X), self.nb, k, _capi.ptr(out),
                                                 self.nb, _capi.ptr(self._mm_work), self._mm_work.numel(),
                                                 _capi.stream_ptr()), "smx_hmat_matmat")
         return out

@@ -231,19 +231,47 @@ typedef struct {
   double* ynear;
 } smx_hmatrix;
 
+/* The near rows as dense tiles for smx_hmat_matmat: tile t covers rows
+ * row[16 t + s] (permuted indices; s < nrows <= 16, -1 beyond) and the
+ * columns col[col_off .. col_off + w) (permuted indices, the union of its
+ * rows' near columns); its values sit row-major at val + a_off (val_dtype
+ * SMX_F64 or SMX_F32), rows padded to a multiple of 4 entries with zeros,
+ * a_off a multiple of 4.  Every row 0..nb-1 is in exactly one tile.  With
+ * tiles the rows are finished in permuted order and written to Y by an LDS
+ * transpose through iperm (contiguous stores), not scattered per entry. */
+typedef struct {
+  int64_t a_off;
+  int64_t col_off;
+  int32_t w;
+  int32_t nrows;
+} smx_hmat_ntile;
+
+typedef struct {
+  int64_t ntile;
+  const smx_hmat_ntile* tile;
+  const int32_t* row; /* 16 * ntile */
+  const int32_t* col;
+  const void* val;
+  const int32_t* iperm; /* inverse of H->perm: the permuted row of output index i */
+  int32_t val_dtype;
+  int32_t pad;
+} smx_hmat_near_tiles;
+
 /* Y = B X for nrhs <= 64 right-hand sides (column j of X at X + j*ldx, of
  * Y at Y + j*ldy, indexed through H->gidx like smx_hmat_matvec's src, no
  * shift), with the low-rank products on the matrix cores: the z stage
  * Z = V_b X_b and the leaf stage U_L Z_L become 16 x 16 MFMA tiles over the
  * 64 columns (FP32 factors: v_mfma_f32_16x16x4_f32 with X and Z rounded to
  * FP32 once; FP64: v_mfma_f64_16x16x4_f64); the near field and the diagonal
- * sum in FP64 over the staged X.  No counterpart call in the reference, whose matvec takes one
+ * sum in FP64 over the staged X: with NT (optional) as v_mfma_f64_16x16x4_f64
+ * tiles over NT's dense near tiles, else one wavefront per CSR row.  No counterpart call in the reference, whose matvec takes one
  * vector (backend_fused.py:496-512): it replaces a loop of matvecs.  nz: the
  * packed operator's z entries (the largest z_off + kc); work: at least
  * smx_hmat_matmat_ws_size(H, nz) bytes. */
 int64_t smx_hmat_matmat_ws_size(const smx_hmatrix* H, int64_t nz);
-int smx_hmat_matmat(const smx_hmatrix* H, int64_t nz, const double* X, int64_t ldx, int32_t nrhs,
-                    double* Y, int64_t ldy, void* work, int64_t work_bytes, void* stream);
+int smx_hmat_matmat(const smx_hmatrix* H, const smx_hmat_near_tiles* NT, int64_t nz, const double* X,
+                    int64_t ldx, int32_t nrhs, double* Y, int64_t ldy, void* work, int64_t work_bytes,
+                    void* stream);
 int smx_hmat_matvec(const smx_hmatrix* H, const double* src, const double* shift,
                     double shift_scale, double* y, void* stream);
 

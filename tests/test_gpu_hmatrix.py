@@ -172,11 +172,14 @@ def test_leaf_workgroup_shapes_agree(c1, fp32):
     H._H.leaf_waves = 0
 
 
-@pytest.mark.parametrize("fp32,k", [(False, 64), (False, 7), (True, 64), (True, 1)])
-def test_matmat_against_matvecs(fp32, k):
+@pytest.mark.parametrize("fp32,k,near_fp32", [(False, 64, False), (False, 7, False), (True, 64, False),
+                                               (True, 1, False), (True, 64, True)])
+def test_matmat_against_matvecs(fp32, k, near_fp32):
     """Multi-RHS product (MFMA low-rank stages) against k separate matvecs on
-    the 150 nm cube: FP64 factors 1e-12 (the MFMA tiles sum in another order),
-    FP32 factors 1e-6 (X and Z rounded to FP32 once) relative per column."""
+    the 150 nm cube, with the near field from the dense near tiles (FP64 MFMA)
+    and from the CSR rows: FP64 factors 1e-12 (the MFMA tiles sum in another
+    order), FP32 factors 1e-6 (X and Z rounded to FP32 once) relative per
+    column; the two near forms agree to 1e-13 of the column norm."""
     import torch
     from smx import mesh as M
     from smx.boundary import build_boundary_geometry
@@ -184,10 +187,20 @@ def test_matmat_against_matvecs(fp32, k):
     m = M.cube_c2()
     geo = build_boundary_geometry(m.X, m.tets)
     data = build_hmatrix_device(geo, HMatrixBuildConfig(epsilon=1e-6, leaf_size=64), torch.device("cuda"))
-    H = HMatrixGPUPackedFused(data, use_fp32=fp32, near_in_z=False)
+    H = HMatrixGPUPackedFused(data, use_fp32=fp32, near_in_z=False, near_fp32=near_fp32)
     X = torch.from_numpy(np.random.default_rng(k).standard_normal((k, geo.Nb))).cuda()
+    print(f"[matmat fp32={fp32} k={k} near_fp32={near_fp32}] Nb={geo.Nb}", flush=True)
     Y = H.matmat(X)
+    Yc = H.matmat(X, near_tiles=False)
+    torch.cuda.synchronize()
+    st = H.near_tile_stats
+    assert st["nnz"] == H.near_nnz and st["dense_entries"] >= st["nnz"]
     ref = torch.stack([H.matvec(X[j]) for j in range(k)])
-    err = (torch.linalg.norm(Y - ref, dim=1) / torch.linalg.norm(ref, dim=1)).max().item()
-    print(f"[matmat fp32={fp32} k={k}] max column rel err {err:.2e}", flush=True)
-    assert err <= (1e-6 if fp32 else 1e-12)
+    nrm = torch.linalg.norm(ref, dim=1)
+    err = (torch.linalg.norm(Y - ref, dim=1) / nrm).max().item()
+    errc = (torch.linalg.norm(Yc - ref, dim=1) / nrm).max().item()
+    dforms = (torch.linalg.norm(Y - Yc, dim=1) / nrm).max().item()
+    print(f"  max column rel err tiles {err:.2e} csr {errc:.2e} between {dforms:.2e}; "
+          f"{st['ntile']} tiles, {st['dense_entries']} dense entries for {st['nnz']} near", flush=True)
+    assert err <= (1e-6 if fp32 else 1e-12) and errc <= (1e-6 if fp32 else 1e-12)
+    assert dforms <= 1e-13

@@ -462,3 +462,40 @@ def test_sixteen_row_task_table(c1):
     z16 = zvec(pk["ztask16"].view(ZTASK_DTYPE)[:pk["nztask16"]], 16)
     assert pk["nztask16"] < pk["nztask"]
     assert np.abs(z8 - z16).max() <= 1e-12
+
+
+@pytest.mark.parametrize("leaf", [16, 32])
+def test_near_tiles_reproduce_near_csr(c1, leaf):
+    """The multi-RHS product's dense near tiles: every row in exactly one tile,
+    every index in bounds, the near matrix recovered entry for entry, and the
+    tile products equal to the CSR product."""
+    from smx.hmatrix.backend import NTILE_ROWS, leaf_intervals, near_from_tiles, pack_near_tiles
+    _, geo, bd = c1
+    d = build_hmatrix_device(geo, HMatrixBuildConfig(leaf_size=leaf), ops=HostOps(geo, bd))
+    fb = d.far_blocks
+    starts, _ = leaf_intervals(d.size, [b.row_start for b in fb], [b.row_stop for b in fb])
+    pk = pack_near_tiles(d.size, d.near_csr, starts, np.float64)
+    t, row, col, val = pk["tile"], pk["row"], pk["col"], pk["val"]
+    R = row.reshape(-1, NTILE_ROWS)
+    assert R.shape[0] == t.size == pk["ntile"]
+    assert np.all((t["nrows"] >= 1) & (t["nrows"] <= NTILE_ROWS))
+    for q in range(t.size):
+        nr = int(t["nrows"][q])
+        assert np.all(R[q, :nr] >= 0) and np.all(R[q, nr:] == -1)
+        ld = (int(t["w"][q]) + 3) // 4 * 4
+        assert t["a_off"][q] + nr * ld <= val.size and t["col_off"][q] + t["w"][q] <= col.size
+        assert t["a_off"][q] % 4 == 0
+    got = np.sort(R[R >= 0])
+    assert np.array_equal(got, np.arange(d.size))
+    assert np.all(np.diff(R.min(axis=1, where=R >= 0, initial=d.size)) > 0)   # stored in row order
+    ref = d.near_csr.tocsr()
+    back = near_from_tiles(d.size, t, row, col, val)
+    assert (back != ref).nnz == 0
+    assert pk["nnz"] == ref.nnz and pk["dense_entries"] >= ref.nnz
+    X = np.random.default_rng(2).standard_normal((d.size, 5))
+    Y = np.zeros_like(X)
+    for q in range(t.size):
+        nr, w, a0, c0 = int(t["nrows"][q]), int(t["w"][q]), int(t["a_off"][q]), int(t["col_off"][q])
+        ld = (w + 3) // 4 * 4
+        Y[R[q, :nr]] = val[a0:a0 + nr * ld].reshape(nr, ld)[:, :w] @ X[col[c0:c0 + w]]
+    assert np.abs(Y - ref @ X).max() <= 1e-12 * max(1.0, np.abs(ref @ X).max())
