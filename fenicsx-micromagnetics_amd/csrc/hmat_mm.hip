@@ -4,12 +4,15 @@
 // Same packed operator as smx_hmat_matvec (z tasks over V, leaves over U,
 // near CSR); with 64 columns of X the far-field work becomes small GEMMs:
 //   stage:  XR[p][j] = X[j][gidx[p]] in the factor type, p < nb, j < 64
+//           (and XD, the same in FP64)
 //   z:      Z[z_off + g][j] = V_t[g, :] . XR[c0 .. c0+n)[j]      (MFMA)
 //   leaf:   YF[p][j] = U_L[p - l0, :] . Z[zg_L[k]][j]           (MFMA)
 //   near:   Y[j][perm[p]] = d_p XR[p][j] + Near[p,:] XR[:, j] + YF[p][j]
 //           (dense 16-row near tiles on the FP64 matrix cores, or CSR rows)
-// Every operand array is row-major with the 64 columns contiguous, so the
-// 16 lanes of a lane group read 64 contiguous bytes of one row.
+// Every operand array is row-major with a row's 64 columns contiguous; XR
+// (and Z for FP64 factors) interleave them (column 16 q + c at 4 c + q), so
+// the 16 lanes of a lane group read one whole row, each lane its four column
+// blocks at once.
 // MFMA per factor type (lane l, g = l / 16, c = l % 16), as the dense
 // multi-RHS product: FP32 v_mfma_f32_16x16x4_f32, D[m = 4g + i][n = c];
 // FP64 v_mfma_f64_16x16x4_f64, D[m = g + 4i][n = c]; operand A[m = c][k],
@@ -40,6 +43,9 @@ struct Mt<float> {
   using acc_t = f32x4;
   static constexpr int kStep = 32;
   static constexpr int kLane = 8;
+  // Z in column order: the leaf stage's eight Z rows per step gather best
+  // as single 4-byte loads per column block (measured on the film)
+  static constexpr bool kZInterleaved = false;
   // 8 factor entries from p, `room` entries (a multiple of 4) left in the row
   __device__ static void load_a(const float* p, int64_t room, float (&v)[8]) {
     if (room >= 4) {
@@ -59,6 +65,10 @@ struct Mt<float> {
 #pragma unroll
     for (int j = 0; j < 8; ++j) c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[j], b[j], c, 0, 0, 0);
   }
+  __device__ static void mma4(acc_t& c, const float* a, const float (&b)[4]) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[j], b[j], c, 0, 0, 0);
+  }
   __device__ static int row_of(int g, int i) { return 4 * g + i; }
 };
 
@@ -68,6 +78,7 @@ struct Mt<double> {
   using acc_t = f64x4;
   static constexpr int kStep = 16;
   static constexpr int kLane = 4;
+  static constexpr bool kZInterleaved = true;
   __device__ static void load_a(const double* p, int64_t room, double (&v)[4]) {
     if (room >= 4) {
       const double2 a = *reinterpret_cast<const double2*>(p);
@@ -81,8 +92,33 @@ struct Mt<double> {
 #pragma unroll
     for (int j = 0; j < 4; ++j) c = __builtin_amdgcn_mfma_f64_16x16x4f64(a[j], b[j], c, 0, 0, 0);
   }
+  __device__ static void mma4(acc_t& c, const double* a, const double (&b)[4]) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) c = __builtin_amdgcn_mfma_f64_16x16x4f64(a[j], b[j], c, 0, 0, 0);
+  }
   __device__ static int row_of(int g, int i) { return g + 4 * i; }
 };
+
+// XR (and Z for FP64 factors) keep a row's 64 columns interleaved: column
+// 16 q + c at 4 c + q,
+// so lane c of a lane group reads (or writes) its four column blocks
+// q = 0..3 as one 16-byte (FP32) or two 16-byte (FP64) accesses.
+__device__ inline void load_q4(const float* p, float (&v)[4]) {
+  const float4 a = *reinterpret_cast<const float4*>(p);
+  v[0] = a.x, v[1] = a.y, v[2] = a.z, v[3] = a.w;
+}
+__device__ inline void load_q4(const double* p, double (&v)[4]) {
+  const double2 a = *reinterpret_cast<const double2*>(p);
+  const double2 b = *reinterpret_cast<const double2*>(p + 2);
+  v[0] = a.x, v[1] = a.y, v[2] = b.x, v[3] = b.y;
+}
+__device__ inline void store_q4(float* p, float v0, float v1, float v2, float v3) {
+  *reinterpret_cast<float4*>(p) = make_float4(v0, v1, v2, v3);
+}
+__device__ inline void store_q4(double* p, double v0, double v1, double v2, double v3) {
+  *reinterpret_cast<double2*>(p) = make_double2(v0, v1);
+  *reinterpret_cast<double2*>(p + 2) = make_double2(v2, v3);
+}
 
 template <typename T>
 __global__ void hmm_stage_kernel(int64_t nb, int32_t nrhs, const double* __restrict__ X, int64_t ldx,
@@ -90,10 +126,11 @@ __global__ void hmm_stage_kernel(int64_t nb, int32_t nrhs, const double* __restr
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nb * kR) return;
   const int64_t p = i / kR;
-  const int j = (int)(i - p * kR);
+  const int r = (int)(i - p * kR);
+  const int j = 16 * (r & 3) + (r >> 2);  // interleaved slot r holds column j
   const double v = j < nrhs ? X[(int64_t)j * ldx + gidx[p]] : 0.0;
   XR[i] = (T)v;
-  if (XD) XD[i] = v;
+  XD[p * kR + j] = v;
 }
 
 // Workgroup b of a launch over n items runs on XCD b % 8; item
@@ -120,7 +157,7 @@ __global__ __launch_bounds__(256) void hmm_z_kernel(smx_hmatrix H, const T* __re
   const int64_t ld = (n + 3) & ~(int64_t)3;
   const bool row_ok = c < kc;
   const T* __restrict__ Vr = reinterpret_cast<const T*>(H.V) + task.v_off + (int64_t)(row_ok ? c : 0) * ld;
-  const T* __restrict__ xs = XR + (int64_t)task.c0 * kR + c;  // column c of this task's x segment
+  const T* __restrict__ xs = XR + (int64_t)task.c0 * kR + 4 * c;  // lane c's columns of the x segment
   typename M::acc_t acc[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q)
@@ -130,21 +167,33 @@ __global__ __launch_bounds__(256) void hmm_z_kernel(smx_hmatrix H, const T* __re
     const int64_t kl = kk + M::kLane * g;
     T a[M::kLane];
     M::load_a(Vr + kl, row_ok ? ld - kl : 0, a);
+    T b[4][M::kLane];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      T b[M::kLane];
+    for (int i = 0; i < M::kLane; ++i) {
+      T v[4];
+      if (kl + i < n) {
+        load_q4(xs + (kl + i) * kR, v);
+      } else {
+        v[0] = v[1] = v[2] = v[3] = (T)0;
+      }
 #pragma unroll
-      for (int i = 0; i < M::kLane; ++i) b[i] = (kl + i < n) ? xs[(kl + i) * kR + 16 * q] : (T)0;
-      M::mma(acc[q], a, b);
+      for (int q = 0; q < 4; ++q) b[q][i] = v[q];
     }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) M::mma(acc[q], a, b[q]);
   }
 #pragma unroll
-  for (int q = 0; q < 4; ++q)
+  for (int i = 0; i < 4; ++i) {
+    const int m = M::row_of(g, i);
+    if (m >= kc) continue;
+    T* __restrict__ zr = Z + (task.z_off + m) * kR;
+    if constexpr (M::kZInterleaved) {
+      store_q4(zr + 4 * c, (T)acc[0][i], (T)acc[1][i], (T)acc[2][i], (T)acc[3][i]);
+    } else {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int m = M::row_of(g, i);
-      if (m < kc) Z[(task.z_off + m) * kR + 16 * q + c] = (T)acc[q][i];
+      for (int q = 0; q < 4; ++q) zr[16 * q + c] = (T)acc[q][i];
     }
+  }
 }
 
 // One workgroup (four wavefronts) per leaf; wavefront w covers rows
@@ -178,12 +227,33 @@ __global__ __launch_bounds__(256) void hmm_leaf_kernel(smx_hmatrix H, const T* _
       int64_t zi[M::kLane];
 #pragma unroll
       for (int i = 0; i < M::kLane; ++i) zi[i] = (kl + i < K) ? (int64_t)zg[kl + i] : -1;
+      if constexpr (!M::kZInterleaved) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        T b[M::kLane];
+        for (int q = 0; q < 4; ++q) {
+          T b[M::kLane];
 #pragma unroll
-        for (int i = 0; i < M::kLane; ++i) b[i] = zi[i] >= 0 ? Z[zi[i] * kR + 16 * q + c] : (T)0;
-        M::mma(acc[q], a, b);
+          for (int i = 0; i < M::kLane; ++i) b[i] = zi[i] >= 0 ? Z[zi[i] * kR + 16 * q + c] : (T)0;
+          M::mma(acc[q], a, b);
+        }
+        continue;
+      }
+      // four k rows at a time: their Z fragments, then their MFMAs
+#pragma unroll
+      for (int h = 0; h < M::kLane; h += 4) {
+        T b[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          T v[4];
+          if (zi[h + i] >= 0) {
+            load_q4(Z + zi[h + i] * kR + 4 * c, v);
+          } else {
+            v[0] = v[1] = v[2] = v[3] = (T)0;
+          }
+#pragma unroll
+          for (int q = 0; q < 4; ++q) b[q][i] = v[q];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) M::mma4(acc[q], a + h, b[q]);
       }
     }
 #pragma unroll
@@ -349,9 +419,10 @@ __global__ __launch_bounds__(256) void hmm_unstage_kernel(int64_t nb, int32_t nr
   for (int j = wv; j < nrhs; j += 4) Y[(int64_t)j * ldy + i] = tb[lane][j];
 }
 
-// Workspace: XR (factor type, nb x 64), YF (FP64, nb x 64), Z (factor
-// type, nz x 64), and for FP32 factors XD (FP64 copy of XR for the near
-// field and the diagonal), each 256-byte aligned.
+// Workspace: XR (factor type, nb x 64, columns interleaved), YF (FP64,
+// nb x 64), Z (factor type, nz x 64, columns interleaved for FP64), XD (FP64 copy of
+// X, nb x 64 in column order, for the near field and the diagonal), each
+// 256-byte aligned.
 struct MmLayout {
   int64_t xr, yf, z, xd;
 };
@@ -363,7 +434,7 @@ MmLayout mm_layout(const smx_hmatrix* H, int64_t nz) {
   L.xr = up(ts * kR * H->nb);
   L.yf = up(8 * kR * H->nb);
   L.z = up(ts * kR * (nz > 0 ? nz : 1));
-  L.xd = ts == 8 ? 0 : up(8 * kR * H->nb);  // FP64 copy of X for the near field
+  L.xd = up(8 * kR * H->nb);  // FP64 copy of X in column order for the near field
   return L;
 }
 
@@ -374,8 +445,8 @@ int run_hmm(const smx_hmatrix* H, const smx_hmat_near_tiles* NT, int64_t nz, con
   T* XR = reinterpret_cast<T*>(work);
   double* YF = reinterpret_cast<double*>(work + Lo.xr);
   T* Z = reinterpret_cast<T*>(work + Lo.xr + Lo.yf);
-  double* XD = Lo.xd ? reinterpret_cast<double*>(work + Lo.xr + Lo.yf + Lo.z) : nullptr;
-  const double* XN = XD ? XD : reinterpret_cast<const double*>(XR);  // T is double when XD is absent
+  double* XD = reinterpret_cast<double*>(work + Lo.xr + Lo.yf + Lo.z);
+  const double* XN = XD;
   const int64_t ns = H->nb * kR;
   hipLaunchKernelGGL(hmm_stage_kernel<T>, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, st, H->nb, nrhs, X,
                      ldx, H->gidx, XR, XD);
