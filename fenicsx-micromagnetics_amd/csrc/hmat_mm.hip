@@ -120,17 +120,28 @@ __device__ inline void store_q4(double* p, double v0, double v1, double v2, doub
   *reinterpret_cast<double2*>(p + 2) = make_double2(v2, v3);
 }
 
+// One workgroup per 64 permuted rows: the wavefronts read right-hand sides
+// j = w, w + 4, ... for 64 consecutive rows (lanes along p, through gidx)
+// into LDS, then write each row whole: XD in column order, XR interleaved
+// (slot r holds column 16 (r mod 4) + r / 4) in the factor type.
 template <typename T>
-__global__ void hmm_stage_kernel(int64_t nb, int32_t nrhs, const double* __restrict__ X, int64_t ldx,
-                                 const int32_t* __restrict__ gidx, T* __restrict__ XR, double* __restrict__ XD) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nb * kR) return;
-  const int64_t p = i / kR;
-  const int r = (int)(i - p * kR);
-  const int j = 16 * (r & 3) + (r >> 2);  // interleaved slot r holds column j
-  const double v = j < nrhs ? X[(int64_t)j * ldx + gidx[p]] : 0.0;
-  XR[i] = (T)v;
-  XD[p * kR + j] = v;
+__global__ __launch_bounds__(256) void hmm_stage_kernel(int64_t nb, int32_t nrhs, const double* __restrict__ X,
+                                                        int64_t ldx, const int32_t* __restrict__ gidx,
+                                                        T* __restrict__ XR, double* __restrict__ XD) {
+  __shared__ double tb[64][kR + 1];
+  const int64_t p0 = (int64_t)blockIdx.x * 64;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t pl = p0 + lane;
+  const int64_t src = pl < nb ? (int64_t)gidx[pl] : 0;
+  for (int j = wv; j < kR; j += 4) tb[lane][j] = (j < nrhs && pl < nb) ? X[(int64_t)j * ldx + src] : 0.0;
+  __syncthreads();
+  const int jr = 16 * (lane & 3) + (lane >> 2);
+  for (int r = wv; r < 64; r += 4) {
+    const int64_t p = p0 + r;
+    if (p >= nb) break;
+    XD[p * kR + lane] = tb[r][lane];
+    XR[p * kR + lane] = (T)tb[r][jr];
+  }
 }
 
 // Workgroup b of a launch over n items runs on XCD b % 8; item
@@ -447,8 +458,7 @@ int run_hmm(const smx_hmatrix* H, const smx_hmat_near_tiles* NT, int64_t nz, con
   T* Z = reinterpret_cast<T*>(work + Lo.xr + Lo.yf);
   double* XD = reinterpret_cast<double*>(work + Lo.xr + Lo.yf + Lo.z);
   const double* XN = XD;
-  const int64_t ns = H->nb * kR;
-  hipLaunchKernelGGL(hmm_stage_kernel<T>, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, st, H->nb, nrhs, X,
+  hipLaunchKernelGGL(hmm_stage_kernel<T>, dim3((unsigned)((H->nb + 63) / 64)), dim3(256), 0, st, H->nb, nrhs, X,
                      ldx, H->gidx, XR, XD);
   SMX_LAUNCH_CHECK();
   if (H->nztask > 0 && nz > 0) {
