@@ -33,6 +33,7 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kR = 64;  // right-hand sides per launch (columns of XR, Z, YF)
 constexpr int kNtCols = 1024;  // near-tile columns staged in LDS at a time (a multiple of 64)
+constexpr int kLeafCols = 2048;  // leaf Z indices staged in LDS at a time (a multiple of 32)
 
 template <typename T>
 struct Mt;
@@ -65,10 +66,6 @@ struct Mt<float> {
 #pragma unroll
     for (int j = 0; j < 8; ++j) c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[j], b[j], c, 0, 0, 0);
   }
-  __device__ static void mma4(acc_t& c, const float* a, const float (&b)[4]) {
-#pragma unroll
-    for (int j = 0; j < 4; ++j) c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[j], b[j], c, 0, 0, 0);
-  }
   __device__ static int row_of(int g, int i) { return 4 * g + i; }
 };
 
@@ -89,10 +86,6 @@ struct Mt<double> {
     }
   }
   __device__ static void mma(acc_t& c, const double (&a)[4], const double (&b)[4]) {
-#pragma unroll
-    for (int j = 0; j < 4; ++j) c = __builtin_amdgcn_mfma_f64_16x16x4f64(a[j], b[j], c, 0, 0, 0);
-  }
-  __device__ static void mma4(acc_t& c, const double* a, const double (&b)[4]) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) c = __builtin_amdgcn_mfma_f64_16x16x4f64(a[j], b[j], c, 0, 0, 0);
   }
@@ -207,12 +200,14 @@ __global__ __launch_bounds__(256) void hmm_z_kernel(smx_hmatrix H, const T* __re
   }
 }
 
-// One workgroup (four wavefronts) per leaf; wavefront w covers rows
-// 16 w + (0..15), 16 (w + 4) + (0..15), ... of the leaf.
+// One workgroup (four wavefronts) per leaf of at most 64 rows; wavefront w
+// covers rows 16 w + (0..15).  The leaf's Z index list is staged in LDS,
+// kLeafCols entries at a time, for all four wavefronts.
 template <typename T>
 __global__ __launch_bounds__(256) void hmm_leaf_kernel(smx_hmatrix H, const T* __restrict__ Z,
                                                        double* __restrict__ YF) {
   using M = Mt<T>;
+  __shared__ int32_t zs[kLeafCols];
   const int64_t L = blockIdx.x;
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
@@ -222,22 +217,30 @@ __global__ __launch_bounds__(256) void hmm_leaf_kernel(smx_hmatrix H, const T* _
   const int64_t z0 = H.leaf_zg_off[L];
   const int64_t K = H.leaf_zg_off[L + 1] - z0;
   const int32_t* __restrict__ zg = H.leaf_zg + z0;
-  for (int r0 = 16 * w; r0 < rows; r0 += 64) {
-    const int row = r0 + c;
-    const bool row_ok = row < rows;
-    const T* __restrict__ Ur = reinterpret_cast<const T*>(H.U) + H.leaf_u_off[L] + (int64_t)(row_ok ? row : 0) * kw;
-    typename M::acc_t acc[4];
+  const int r0 = 16 * w;
+  const bool active = r0 < rows;  // wave-uniform; every wavefront takes the barriers
+  const int row = r0 + c;
+  const bool row_ok = row < rows;
+  const T* __restrict__ Ur = reinterpret_cast<const T*>(H.U) + H.leaf_u_off[L] + (int64_t)(row_ok ? row : 0) * kw;
+  typename M::acc_t acc[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
+  for (int q = 0; q < 4; ++q)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) acc[q][i] = 0;
-    for (int64_t kk = 0; kk < kw; kk += M::kStep) {
-      const int64_t kl = kk + M::kLane * g;
+    for (int i = 0; i < 4; ++i) acc[q][i] = 0;
+  for (int64_t k0 = 0; k0 < kw; k0 += kLeafCols) {
+    const int kn = (int)(K - k0 < kLeafCols ? (K - k0 > 0 ? K - k0 : 0) : kLeafCols);
+    const int kspan = (int)(kw - k0 < kLeafCols ? kw - k0 : kLeafCols);
+    __syncthreads();  // the previous chunk's indices are consumed
+    for (int e = threadIdx.x; e < kn; e += 256) zs[e] = zg[k0 + e];
+    __syncthreads();
+    if (!active) continue;
+    for (int kk = 0; kk < kspan; kk += M::kStep) {
+      const int kc = kk + M::kLane * g;  // column within the chunk
       T a[M::kLane];
-      M::load_a(Ur + kl, row_ok ? kw - kl : 0, a);
+      M::load_a(Ur + k0 + kc, row_ok ? kw - k0 - kc : 0, a);
       int64_t zi[M::kLane];
 #pragma unroll
-      for (int i = 0; i < M::kLane; ++i) zi[i] = (kl + i < K) ? (int64_t)zg[kl + i] : -1;
+      for (int i = 0; i < M::kLane; ++i) zi[i] = (kc + i < kn) ? (int64_t)zs[kc + i] : -1;
       if constexpr (!M::kZInterleaved) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -246,17 +249,13 @@ __global__ __launch_bounds__(256) void hmm_leaf_kernel(smx_hmatrix H, const T* _
           for (int i = 0; i < M::kLane; ++i) b[i] = zi[i] >= 0 ? Z[zi[i] * kR + 16 * q + c] : (T)0;
           M::mma(acc[q], a, b);
         }
-        continue;
-      }
-      // four k rows at a time: their Z fragments, then their MFMAs
+      } else {
+        T b[4][M::kLane];
 #pragma unroll
-      for (int h = 0; h < M::kLane; h += 4) {
-        T b[4][4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
+        for (int i = 0; i < M::kLane; ++i) {
           T v[4];
-          if (zi[h + i] >= 0) {
-            load_q4(Z + zi[h + i] * kR + 4 * c, v);
+          if (zi[i] >= 0) {
+            load_q4(Z + zi[i] * kR + 4 * c, v);
           } else {
             v[0] = v[1] = v[2] = v[3] = (T)0;
           }
@@ -264,17 +263,18 @@ __global__ __launch_bounds__(256) void hmm_leaf_kernel(smx_hmatrix H, const T* _
           for (int q = 0; q < 4; ++q) b[q][i] = v[q];
         }
 #pragma unroll
-        for (int q = 0; q < 4; ++q) M::mma4(acc[q], a + h, b[q]);
+        for (int q = 0; q < 4; ++q) M::mma(acc[q], a, b[q]);
       }
     }
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int rr = r0 + M::row_of(g, i);
-        if (rr < rows) YF[(int64_t)(l0 + rr) * kR + 16 * q + c] = (double)acc[q][i];
-      }
   }
+  if (!active) return;
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int rr = r0 + M::row_of(g, i);
+      if (rr < rows) YF[(int64_t)(l0 + rr) * kR + 16 * q + c] = (double)acc[q][i];
+    }
 }
 
 // One wavefront per permuted row, lane j = right-hand side j.
