@@ -76,39 +76,40 @@ class HMatrixBuildConfig:
     gpu_progress_interval_seconds: float = 5.0
     batch_bytes: int = 4 * 1024 ** 3
 
+    # (field, predicate, what the value must satisfy); checked in this order
+    _CHECKS = (
+        ("epsilon", lambda v: v > 0.0, "> 0"),
+        ("eta", lambda v: v > 0.0, "> 0"),
+        ("leaf_size", lambda v: v >= 1, ">= 1"),
+        ("max_rank", lambda v: v is None or int(v) >= 0, "None or >= 0"),
+        ("max_temporary_block_bytes", lambda v: v >= 8, ">= 8 (one FP64 entry)"),
+        ("low_rank_storage_factor", lambda v: 0.0 < v <= 1.0, "in (0, 1]"),
+        ("gpu_device_id", lambda v: int(v) >= 0, ">= 0"),
+        ("gpu_memory_fraction", lambda v: 0.0 < float(v) <= 1.0, "in (0, 1]"),
+        ("gpu_aca_residual_check_interval", lambda v: v >= 1, ">= 1"),
+        ("gpu_rsvd_initial_rank", lambda v: v >= 1, ">= 1"),
+        ("gpu_rsvd_oversampling", lambda v: v >= 0, ">= 0"),
+        ("gpu_rsvd_power_iterations", lambda v: v >= 0, ">= 0"),
+        ("gpu_progress_interval_seconds", lambda v: v >= 0.0, ">= 0"),
+    )
+    _COMPRESSOR_ALIASES = {"fullaca": "fullaca", "aca": "fullaca", "svd": "svd", "rsvd": "rsvd",
+                           "randomized_svd": "rsvd", "randomized-svd": "rsvd"}
+
     def __post_init__(self):
-        if self.epsilon <= 0.0:
-            raise ValueError("epsilon must be positive.")
-        if self.eta <= 0.0:
-            raise ValueError("eta must be positive.")
-        if self.leaf_size < 1:
-            raise ValueError("leaf_size must be positive.")
-        if self.max_temporary_block_bytes < 8:
-            raise ValueError("max_temporary_block_bytes is too small.")
-        if not (0.0 < self.low_rank_storage_factor <= 1.0):
-            raise ValueError("low_rank_storage_factor must be in (0, 1].")
-        comp = str(self.compressor).strip().lower()
-        if comp not in {"fullaca", "aca", "svd", "rsvd", "randomized_svd", "randomized-svd"}:
-            raise ValueError("compressor must be 'fullaca', 'svd', or 'rsvd'.")
-        self.compressor = {"aca": "fullaca", "randomized_svd": "rsvd", "randomized-svd": "rsvd"}.get(comp, comp)
+        for name, ok, rule in self._CHECKS:
+            value = getattr(self, name)
+            if not ok(value):
+                raise ValueError(f"HMatrixBuildConfig.{name} = {value!r} is invalid: it must be {rule}")
+        comp = self._COMPRESSOR_ALIASES.get(str(self.compressor).strip().lower())
+        if comp is None:
+            raise ValueError(f"HMatrixBuildConfig.compressor = {self.compressor!r} is not one of "
+                             f"{sorted(set(self._COMPRESSOR_ALIASES))}")
+        self.compressor = comp
         self.gpu_device_id = int(self.gpu_device_id)
-        if self.gpu_device_id < 0:
-            raise ValueError("gpu_device_id must be non-negative.")
-        if not (0.0 < float(self.gpu_memory_fraction) <= 1.0):
-            raise ValueError("gpu_memory_fraction must be in (0, 1].")
-        if self.gpu_aca_residual_check_interval < 1:
-            raise ValueError("gpu_aca_residual_check_interval must be positive.")
-        if self.gpu_rsvd_initial_rank < 1:
-            raise ValueError("gpu_rsvd_initial_rank must be positive.")
-        if self.gpu_rsvd_oversampling < 0:
-            raise ValueError("gpu_rsvd_oversampling must be non-negative.")
-        if self.gpu_rsvd_power_iterations < 0:
-            raise ValueError("gpu_rsvd_power_iterations must be non-negative.")
-        if self.gpu_progress_interval_seconds < 0.0:
-            raise ValueError("gpu_progress_interval_seconds must be non-negative.")
         self.build_backend = str(self.build_backend).strip().lower()
         if self.build_backend not in {"gpu", "auto"}:
-            raise ValueError("build_backend must be 'gpu' (the device builds the H-matrix)")
+            raise ValueError(f"HMatrixBuildConfig.build_backend = {self.build_backend!r}: this library builds "
+                             "the H-matrix on the device only ('gpu' or 'auto')")
 
 
 def geometry_hash(Xb, tri_lid=None) -> str:
@@ -403,8 +404,14 @@ def build_hmatrix_device(geo: BoundaryGeometry, config: HMatrixBuildConfig | Non
                 fate.append("far")
             elif not meets and (_can_split(r) or _can_split(c)):
                 fate.append("split")
+            elif status[i] == 3 and rcap[i] > 0:
+                # the block is exactly zero: dense, with no entries to store
+                fate.append("zero")
             else:
-                fate.append("reject" if status[i] != 3 else "zero")
+                # includes a rank cap of 0 (max_rank=0) on a nonzero block: the
+                # compressor returns rank 0, so the block is kept dense with its
+                # entries, as builder.py:343-371 keeps it as a dense rejection
+                fate.append("reject")
         acc = [i for i, f in enumerate(fate) if f == "far"]
         Uh = Vh = None
         if acc:

@@ -1,12 +1,22 @@
 """Geometric cluster tree and eta-admissibility (host).
 
-Restates lindholm_hmatrix_gpu/tree.py:10-118 so that the permutation and the
-block tree are bit-identical to the reference's: recursive bisection where the
-split axis is the FIRST argmax of the bounding-box extent, points ordered by a
-stable mergesort of that coordinate and the left child taking the first n//2;
-leaves hold <= leaf_size points; a block (a, b) is admissible iff the bbox
-distance is > 0 and max(diam a, diam b) <= eta * dist (diam = 2-norm of the
-bbox diagonal).
+Follows lindholm_hmatrix_gpu/tree.py:10-118, whose rules fix the permutation
+and the block tree bit for bit:
+
+- a cluster with more than ``leaf_size`` points is bisected along the FIRST
+  axis of largest bounding-box extent (``np.argmax`` of max - min);
+- its points are ranked by a stable mergesort of that coordinate (ties keep
+  the incoming order) and the lower ``n // 2`` go to the left child;
+- the permutation lists the leaves left to right;
+- a block (a, b) is admissible iff the bounding boxes are a positive distance
+  apart and max(diam a, diam b) <= eta * dist, with diam the 2-norm of the
+  bounding-box diagonal.
+
+The construction is an explicit work stack rather than recursion: nodes are
+created when popped, children are pushed right first so the left subtree is
+always finished first, and each leaf's points are appended to the permutation
+in that order.  Since ``leaf_size >= 1``, a split cluster holds at least two
+points and both halves are non-empty.
 """
 from __future__ import annotations
 
@@ -61,43 +71,62 @@ class ClusterTree:
 
 
 def build_cluster_tree(points: np.ndarray, leaf_size: int = 64) -> ClusterTree:
+    """Bisection tree over ``points`` (N x 3) and the leaf-order permutation."""
     pts = np.ascontiguousarray(points, dtype=np.float64)
     if pts.ndim != 2 or pts.shape[1] != 3:
-        raise ValueError("points must have shape (N, 3).")
+        raise ValueError(f"cluster tree: expected an (N, 3) array of points, got shape {pts.shape}")
     n = int(pts.shape[0])
-    if n == 0:
-        raise ValueError("At least one point is required.")
+    if n < 1:
+        raise ValueError("cluster tree: the point set is empty")
+    leaf_size = int(leaf_size)
     if leaf_size < 1:
-        raise ValueError("leaf_size must be positive.")
-    order: list[np.ndarray] = []
-    count = [0]
+        raise ValueError(f"cluster tree: leaf_size must be at least 1, got {leaf_size}")
 
-    def split(idx: np.ndarray) -> ClusterNode:
-        sub = pts[idx]
+    perm = np.empty(n, dtype=np.int32)
+    filled = 0
+    root_holder: list[ClusterNode] = []
+    # Work items: (point ids in the incoming order, parent node, is_left_child).
+    # A node's interval is known only once its subtree is done, so the stop of
+    # every internal node is patched on the way back (the "close" items).
+    work: list[tuple] = [("open", np.arange(n, dtype=np.int32), None, False)]
+    while work:
+        item = work.pop()
+        if item[0] == "close":
+            node = item[1]
+            node.stop = node.right.stop
+            continue
+        _, ids, parent, is_left = item
+        sub = pts[ids]
         lo = sub.min(axis=0)
         hi = sub.max(axis=0)
-        if idx.size <= leaf_size:
-            s = count[0]
-            order.append(idx)
-            count[0] += idx.size
-            return ClusterNode(s, count[0], lo, hi)
+        node = ClusterNode(filled, filled, lo, hi)
+        if parent is None:
+            root_holder.append(node)
+        elif is_left:
+            parent.left = node
+        else:
+            parent.right = node
+        if ids.size <= leaf_size:
+            perm[filled:filled + ids.size] = ids
+            filled += ids.size
+            node.stop = filled
+            continue
         axis = int(np.argmax(hi - lo))
-        ranked = idx[np.argsort(sub[:, axis], kind="mergesort")]
-        half = ranked.size // 2
-        if half == 0 or half == ranked.size:
-            half = max(1, ranked.size // 2)
-        a = split(ranked[:half])
-        b = split(ranked[half:])
-        return ClusterNode(a.start, b.stop, lo, hi, a, b)
+        ranked = ids[np.argsort(sub[:, axis], kind="mergesort")]
+        cut = ranked.size // 2
+        # the left subtree must be completed before the right one starts, and
+        # this node closes after both
+        work.append(("close", node))
+        work.append(("open", ranked[cut:], node, False))
+        work.append(("open", ranked[:cut], node, True))
 
-    root = split(np.arange(n, dtype=np.int32))
-    perm = np.concatenate(order).astype(np.int32)
-    if perm.size != n or np.unique(perm).size != n:
-        raise RuntimeError("Invalid cluster-tree permutation.")
-    return ClusterTree(points=pts, perm=perm, root=root, leaf_size=int(leaf_size))
+    if filled != n or np.bincount(perm, minlength=n).max() != 1:
+        raise RuntimeError("cluster tree: the leaf order is not a permutation of the points")
+    return ClusterTree(points=pts, perm=perm, root=root_holder[0], leaf_size=leaf_size)
 
 
 def bbox_distance(a: ClusterNode, b: ClusterNode) -> float:
+    """Euclidean gap between the two axis-aligned bounding boxes (0 if they touch)."""
     gap = np.maximum(np.maximum(a.bbox_min - b.bbox_max, b.bbox_min - a.bbox_max), 0.0)
     return float(np.linalg.norm(gap))
 

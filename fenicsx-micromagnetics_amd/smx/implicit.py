@@ -36,6 +36,7 @@ import torch
 from . import _capi
 from .device import reduce_workspace
 from .llg import EffectiveField
+from .stepcontrol import BasicAdaptor, StepRejectedError, check_final_time_mode
 
 
 class JvContext:
@@ -191,13 +192,16 @@ class BDFIntegrator:
 
     def __init__(self, hef: EffectiveField, rtol=1e-6, atol=1e-6, snes_rtol=1e-2, snes_atol=1e-4,
                  snes_max_it=8, ksp_rtol=1e-4, ksp_max_it=200, dt_min=1e-17, dt_max=1e-10, safety=0.9,
-                 clip=(0.1, 3.0), scale_solve_failed=0.25, use_pc=True, pc_form="exact"):
+                 clip=(0.1, 3.0), scale_solve_failed=0.25, use_pc=True, pc_form="exact", reject_safety=0.1,
+                 max_reject=10):
         self.hef = hef
         self.rtol, self.atol = float(rtol), float(atol)
         self.snes_rtol, self.snes_atol, self.snes_max_it = float(snes_rtol), float(snes_atol), int(snes_max_it)
         self.ksp_rtol, self.ksp_max_it = float(ksp_rtol), int(ksp_max_it)
         self.dt_min, self.dt_max = float(dt_min), float(dt_max)
         self.safety, self.clip, self.scale_failed = safety, clip, scale_solve_failed
+        self.adaptor = BasicAdaptor(safety=float(safety), reject_safety=float(reject_safety), clip=tuple(clip),
+                                    dt_min=self.dt_min, dt_max=self.dt_max, max_reject=int(max_reject))
         self.use_pc = bool(use_pc)
         if pc_form not in ("exact", "reference"):
             raise ValueError(f"pc_form must be 'exact' or 'reference', got {pc_form!r}")
@@ -305,11 +309,17 @@ class BDFIntegrator:
         self.u_prev.copy_(self.hist[0])
         return float(self._s.item()) / (t - t_prev) * (180.0 / math.pi) * 1e-9
 
-    def integrate(self, y, t0, t1, dt, max_steps=10 ** 7, stopping_dmdt=0.0, check_every=5, monitor_fn=None):
-        """Advance y (device, in place) from t0 to t1.  stopping_dmdt > 0:
+    def integrate(self, y, t0, t1, dt, max_steps=10 ** 7, stopping_dmdt=0.0, check_every=5, monitor_fn=None,
+                  exact_final_time="stepover"):
+        """Advance y (device, in place) from t0 until t >= t1.  stopping_dmdt > 0:
         stop early once max |dm/dt| over the last check_every accepted steps
         falls below it (deg/ns).  monitor_fn(step, t, y) after each accepted
-        step."""
+        step.  exact_final_time "stepover" (the reference's) lets the last
+        step pass t1; "matchstep" lands on t1 (a step that would leave less
+        than 1e-3 of itself before t1 is stretched to reach it)."""
+        match = check_final_time_mode(exact_final_time) == "matchstep"
+        failed = False
+        reject_run = 0
         t = float(t0)
         self.hist[0].copy_(y)
         self.times[0] = t
@@ -321,7 +331,7 @@ class BDFIntegrator:
         self.stats["stopped_on_dmdt"] = False
         while t < t1 * (1 - 1e-15) and steps < max_steps:
             rem = t1 - t
-            if dt >= rem or rem - dt < 1e-3 * dt:       # land on t1: no sliver step before it
+            if match and (dt >= rem or rem - dt < 1e-3 * dt):   # land on t1: no sliver step before it
                 dt = rem
             q = 2 if self.nhist >= 2 else 1
             yn = self.hist[0]
@@ -353,7 +363,10 @@ class BDFIntegrator:
                 err = self._error_norm(2.0 / 11.0 if q == 2 else 0.5)
             else:
                 err = 0.0                                  # no estimate yet: accept
-            if err <= 1.0:
+            accept, dt_next = self.adaptor.choose(err, dt, q + 1, failed)
+            if accept:
+                failed = False
+                reject_run = 0
                 # rotate history
                 self.hist = [self.hist[2], self.hist[0], self.hist[1]]
                 self.hist[0].copy_(self.y)
@@ -362,8 +375,7 @@ class BDFIntegrator:
                 t += dt
                 steps += 1
                 self.stats["steps"] += 1
-                fac = self.safety * (1.0 / max(err, 1e-16)) ** (1.0 / (q + 1)) if err > 0 else self.clip[1]
-                dt = min(max(dt * min(max(fac, self.clip[0]), self.clip[1]), self.dt_min), self.dt_max)
+                dt = dt_next
                 if monitor_fn is not None:
                     monitor_fn(steps, t, self.hist[0])
                 if stopping_dmdt > 0.0 and steps % max(int(check_every), 1) == 0:
@@ -374,15 +386,20 @@ class BDFIntegrator:
                         break
             else:
                 self.stats["rejected"] += 1
-                fac = self.safety * (1.0 / err) ** (1.0 / (q + 1))
-                dt = max(dt * min(max(fac, self.clip[0]), 1.0), self.dt_min)
+                reject_run = (reject_run + 1) if failed else 1
+                if reject_run > self.adaptor.max_reject:
+                    raise StepRejectedError(f"BDF: {reject_run} consecutive rejections at t = {t:.6e} "
+                                            f"(dt = {dt:.3e}, error norm {err:.3e})")
+                failed = True
+                dt = dt_next
         y.copy_(self.hist[0])
         return {"t": t, "dt": dt, "maxdmdt_deg_ns": self.last_max_dmdt_deg_ns, **self.stats}
 
 
 def relax_implicit(hef: EffectiveField, t_final, dt_init=1e-14, ts_rtol=1e-6, ts_atol=1e-6, snes_rtol=1e-2,
                    snes_atol=1e-4, ksp_rtol=1e-4, m0_array=None, use_python_pc=True, max_steps=5_000_000,
-                   t0=0.0, stopping_dmdt=0.0, check_every_stop=5, monitor_fn=None, pc_form="exact"):
+                   t0=0.0, stopping_dmdt=0.0, check_every_stop=5, monitor_fn=None, pc_form="exact",
+                   exact_final_time="stepover"):
     """Implicit relax of hef.m_gpu from t0 to t_final (llg_module_GPU.py:
     2052-2381); the output-directory and snapshot machinery of the reference
     is left to monitor_fn(step, t, m)."""
@@ -392,6 +409,7 @@ def relax_implicit(hef: EffectiveField, t_final, dt_init=1e-14, ts_rtol=1e-6, ts
                           ksp_rtol=ksp_rtol, use_pc=use_python_pc, pc_form=pc_form)
     t0w = time.perf_counter()
     out = integ.integrate(hef.m_gpu, float(t0), float(t_final), float(dt_init), max_steps=max_steps,
-                          stopping_dmdt=stopping_dmdt, check_every=check_every_stop, monitor_fn=monitor_fn)
+                          stopping_dmdt=stopping_dmdt, check_every=check_every_stop, monitor_fn=monitor_fn,
+                          exact_final_time=exact_final_time)
     out["elapsed_s"] = time.perf_counter() - t0w
     return out

@@ -26,6 +26,7 @@ from .demag import DemagFieldLindholmGPU, OperatorSet
 from .device import default_device, reduce_workspace, to_dev
 from .fem import dmi_bulk_values, dmi_interfacial_values, exchange_values, zhang_li_values
 from .mesh import TetMesh
+from .stepcontrol import BasicAdaptor, StepRejectedError, check_final_time_mode
 
 MU0 = 4.0 * math.pi * 1e-7
 
@@ -531,7 +532,8 @@ class LLG_GPU:
         self.hef.set_m_from_cpu(m0_array)
 
     def relax_explicit(self, t_final, dt_init=1e-14, ts_rtol=1e-5, ts_atol=1e-5, m0_array=None,
-                       max_steps=50_000_000, fixed_dt=False, dt_min=1e-18, dt_max=1e-10):
+                       max_steps=50_000_000, fixed_dt=False, dt_min=1e-18, dt_max=1e-10,
+                       exact_final_time="stepover"):
         if self.hef is None:
             self._build_effective_field()
         if m0_array is not None:
@@ -539,13 +541,13 @@ class LLG_GPU:
         stepper = DormandPrince(self.hef, rtol=ts_rtol, atol=ts_atol, dt_min=dt_min, dt_max=dt_max)
         t0 = time.perf_counter()
         stats = stepper.integrate(self.hef.m_gpu, 0.0, float(t_final), float(dt_init),
-                                  fixed_dt=fixed_dt, max_steps=max_steps)
+                                  fixed_dt=fixed_dt, max_steps=max_steps, exact_final_time=exact_final_time)
         stats["elapsed_s"] = time.perf_counter() - t0
         return stats
 
     def relax(self, m0_array, t0, t_final, dt_init, ts_rtol=1e-6, ts_atol=1e-6, snes_rtol=1e-2, snes_atol=1e-4,
               ksp_rtol=1e-4, stopping_dmdt=0.0, monitor_fn=None, check_every_stop=5, return_stats=False,
-              use_pc_python=True, pc_form="exact", max_steps=5_000_000):
+              use_pc_python=True, pc_form="exact", max_steps=5_000_000, exact_final_time="stepover"):
         """Implicit BDF / Newton-Krylov relax (llg_module_GPU.py:2183-2381),
         the block preconditioner when use_pc_python; returns hef (and the
         integrator statistics when return_stats)."""
@@ -556,7 +558,7 @@ class LLG_GPU:
                                snes_rtol=snes_rtol, snes_atol=snes_atol, ksp_rtol=ksp_rtol, m0_array=m0_array,
                                use_python_pc=use_pc_python, max_steps=max_steps, t0=t0,
                                stopping_dmdt=stopping_dmdt, check_every_stop=check_every_stop,
-                               monitor_fn=monitor_fn, pc_form=pc_form)
+                               monitor_fn=monitor_fn, pc_form=pc_form, exact_final_time=exact_final_time)
         return (self.hef, stats) if return_stats else self.hef
 
 
@@ -573,11 +575,13 @@ class DormandPrince:
     B4 = [5179 / 57600, 0.0, 7571 / 16695, 393 / 640, -92097 / 339200, 187 / 2100, 1 / 40]
 
     def __init__(self, hef: EffectiveField, rtol=1e-5, atol=1e-5, dt_min=1e-18, dt_max=1e-10,
-                 safety=0.9, clip=(0.1, 3.0), reject_safety=0.1):
+                 safety=0.9, clip=(0.1, 3.0), reject_safety=0.1, max_reject=10):
         self.hef = hef
         self.rtol, self.atol = float(rtol), float(atol)
         self.dt_min, self.dt_max = float(dt_min), float(dt_max)
         self.safety, self.clip, self.reject_safety = safety, clip, reject_safety
+        self.adaptor = BasicAdaptor(safety=float(safety), reject_safety=float(reject_safety), clip=tuple(clip),
+                                    dt_min=self.dt_min, dt_max=self.dt_max, max_reject=int(max_reject))
         n = hef.local_size
         dev = hef.device
         self.k = [torch.zeros(n, dtype=torch.float64, device=dev) for _ in range(7)]
@@ -619,27 +623,42 @@ class DormandPrince:
         self._combine(y, dt, 7, self.B5, self.ynew, ecoef if want_err else None)
         return float(self.err.item()) if want_err else 0.0
 
-    def integrate(self, y, t0, t1, dt, fixed_dt=False, max_steps=10 ** 9):
+    def integrate(self, y, t0, t1, dt, fixed_dt=False, max_steps=10 ** 9, exact_final_time="stepover"):
+        """Advance y (device, in place) from t0 until t >= t1.  The step-size
+        decision is the reference's PETSc "basic" adaptor (BasicAdaptor);
+        exact_final_time "stepover" (the reference's) lets the last step pass
+        t1, "matchstep" shortens it to land on t1."""
+        match = check_final_time_mode(exact_final_time) == "matchstep"
         t = t0
         steps = rejected = 0
+        failed = False            # the previous attempt of this step was rejected
+        reject_run = 0
         self._fsal = False
         while t < t1 * (1 - 1e-15) and steps < max_steps:
-            dt = min(dt, t1 - t) if not fixed_dt else dt
+            if match:
+                dt = min(dt, t1 - t)
             err = self.step(y, t, dt, want_err=not fixed_dt)
-            if fixed_dt or err <= 1.0:
+            if fixed_dt:
+                accept, dt_next = True, dt
+            else:
+                accept, dt_next = self.adaptor.choose(err, dt, 5, failed)
+            # the stages of a rejected attempt are discarded, but k[0] = f(t, y) stays valid
+            self._fsal = True
+            if accept:
                 y.copy_(self.ynew)
                 self.k[0], self.k[6] = self.k[6], self.k[0]
-                self._fsal = True
                 t += dt
                 steps += 1
-                if not fixed_dt:
-                    fac = self.safety * (1.0 / max(err, 1e-16)) ** 0.2
-                    dt = min(max(dt * min(max(fac, self.clip[0]), self.clip[1]), self.dt_min), self.dt_max)
+                failed = False
+                reject_run = 0
             else:
                 rejected += 1
-                self._fsal = True
-                fac = self.safety * (1.0 / err) ** 0.2
-                dt = max(dt * min(max(fac, self.clip[0]), 1.0), self.dt_min)
+                reject_run = (reject_run + 1) if failed else 1
+                if reject_run > self.adaptor.max_reject:
+                    raise StepRejectedError(f"Dormand-Prince: {reject_run} consecutive rejections at t = {t:.6e} "
+                                            f"(dt = {dt:.3e}, error norm {err:.3e})")
+                failed = True
+            dt = dt_next
         return {"t": t, "steps": steps, "rejected": rejected, "rhs_evals": self.rhs_evals, "dt": dt}
 
     def run_steps(self, y, t, dt, nsteps):

@@ -13,6 +13,11 @@
  *   - Functions are asynchronous on `stream` (a hipStream_t passed as void*),
  *     allocate no device memory (workspaces are caller-provided) and return a
  *     smx_status.  smx_last_error() describes the last failure of the thread.
+ *     One exception: the waiting form of the CG solvers (smx_pcg_jacobi and
+ *     smx_pcg_amg with check_every >= 0) blocks the calling host thread on
+ *     hipStreamSynchronize(stream) between batches of iterations, because it
+ *     returns the iteration count and residual in host memory; their
+ *     speculative form (check_every < 0) returns without waiting.
  *   - Reductions are deterministic: fixed-order, no floating-point atomics.
  *
  * Reference interfaces replaced (paths relative to src/spinfemx of the

@@ -148,11 +148,11 @@ def test_bdf_matches_dormand_prince():
     h1 = EffectiveField(m, **kw)
     y1 = m0.clone()
     dp = DormandPrince(h1, rtol=1e-9, atol=1e-9)
-    s1 = dp.integrate(y1, 0.0, t1, 1e-15)
+    s1 = dp.integrate(y1, 0.0, t1, 1e-15, exact_final_time="matchstep")
     h2 = EffectiveField(m, **kw)
     y2 = m0.clone()
     bdf = BDFIntegrator(h2, rtol=1e-7, atol=1e-7, snes_rtol=1e-6, snes_atol=1e-10, ksp_rtol=1e-8)
-    s2 = bdf.integrate(y2, 0.0, t1, 1e-15)
+    s2 = bdf.integrate(y2, 0.0, t1, 1e-15, exact_final_time="matchstep")
     err = float(torch.linalg.norm(y2 - y1) / torch.linalg.norm(y1))
     print(f"[bdf] {s2} vs dp5 steps {s1['steps']}: rel diff {err:.2e}", flush=True)
     assert abs(s2["t"] - t1) <= 1e-9 * t1
@@ -191,7 +191,7 @@ def test_bdf_matches_oracle_integrator():
     hef = EffectiveField(m, Ms=Ms, Aex=A, Ku=Ku, n_ani_vec=(0.0, 0.0, 1.0), alpha=alpha, device=DEV,
                          use_demag=True, demag_kwargs=dict(ksp_rtol_u1=1e-13, ksp_rtol_u2=1e-13, ksp_atol=0.0))
     y = torch.from_numpy(m0).to(DEV)
-    st = BDFIntegrator(hef, **tol).integrate(y, 0.0, T, 1e-15)
+    st = BDFIntegrator(hef, **tol).integrate(y, 0.0, T, 1e-15, exact_final_time="matchstep")
     err = float(np.linalg.norm(y.cpu().numpy() - y_o) / np.linalg.norm(y_o))
     print(f"[bdf vs oracle] device {st} oracle {st_o}: rel diff {err:.2e}", flush=True)
     assert abs(st["t"] - T) <= 1e-9 * T and abs(t_o - T) <= 1e-9 * T

@@ -499,3 +499,31 @@ def test_near_tiles_reproduce_near_csr(c1, leaf):
         ld = (w + 3) // 4 * 4
         Y[R[q, :nr]] = val[a0:a0 + nr * ld].reshape(nr, ld)[:, :w] @ X[col[c0:c0 + w]]
     assert np.abs(Y - ref @ X).max() <= 1e-12 * max(1.0, np.abs(ref @ X).max())
+
+
+@pytest.mark.parametrize("compressor", ["fullaca", "svd"])
+def test_max_rank_zero_keeps_blocks_dense(c1, compressor):
+    """max_rank = 0: every admissible block compresses to rank 0 and must stay
+    in the operator as a dense rejection (builder.py:343-371), so the
+    H-matrix equals the dense B exactly up to rounding of the near sums."""
+    _, geo, bd = c1
+    cfg = HMatrixBuildConfig(epsilon=1e-6, leaf_size=32, max_rank=0, compressor=compressor, batch_bytes=1 << 20)
+    data = build_hmatrix_device(geo, cfg, ops=HostOps(geo, bd))
+    ref = O.build_hmatrix(bd, eps=1e-6, leaf_size=32, max_rank=0, compressor=compressor)
+    assert not data.far_blocks and not ref.far
+    assert data.metadata["rejected_low_rank_blocks"] == ref.stats["rejected_low_rank_blocks"] > 0
+    assert np.array_equal(data.near_csr.indices, ref.near.indices)
+    B = O.dense_B_rows(bd)
+    rng = np.random.default_rng(1234)
+    x = rng.standard_normal(bd.Nb)
+    assert np.linalg.norm(_matvec(data, x) - B @ x) <= 1e-13 * np.linalg.norm(B @ x)
+
+
+def test_build_config_rejects_bad_values():
+    for kw in (dict(epsilon=0.0), dict(eta=-1.0), dict(leaf_size=0), dict(max_rank=-1),
+               dict(low_rank_storage_factor=1.5), dict(compressor="lu"), dict(build_backend="cpu")):
+        with pytest.raises(ValueError):
+            HMatrixBuildConfig(**kw)
+    assert HMatrixBuildConfig(compressor="ACA").compressor == "fullaca"
+    assert HMatrixBuildConfig(compressor="randomized-svd").compressor == "rsvd"
+    assert HMatrixBuildConfig(max_rank=0).max_rank == 0
