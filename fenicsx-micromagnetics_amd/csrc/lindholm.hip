@@ -6,8 +6,13 @@
 //
 // Mapping: one wavefront owns one target row r.  Its 64 lanes take 64
 // different triangles of the relevant-triangle list; the (w0,w1,w2) of each
-// (row, triangle) pair is accumulated into an LDS copy of the output row
-// (ds_add_f64) and the finished row is written with coalesced stores.  The
+// (row, triangle) pair is accumulated into an LDS copy of the output row and
+// the finished row is written with coalesced stores.  Lanes whose triangles
+// share a vertex hit the same column, so the accumulator is 64-bit fixed
+// point (units of 2^-60, range +-8) updated with integer LDS atomics:
+// integer addition is associative, so every fill of a block gives the same
+// bits whatever order the lanes land in.  Each weight is rounded once to
+// 2^-60 (8.7e-19 absolute); the entries of B are below 1 in magnitude.  The
 // triangle geometry (34 doubles per triangle) is staged once per chunk of 64
 // triangles in LDS, structure-of-arrays, and shared by the 4 wavefronts
 // (4 target rows) of a workgroup.  Everything is FP64: the weights are
@@ -21,6 +26,16 @@ constexpr int kThreads = kWaves * smx::kWave;
 constexpr int kChunk = 64;
 constexpr int kTile = SMX_LINDHOLM_TILE_COLS;
 constexpr double kTiny = 1e-300;
+// fixed-point scale of the row accumulator: 2^60 units per 1.0
+constexpr double kFixScale = 1152921504606846976.0;
+constexpr double kFixInv = 1.0 / 1152921504606846976.0;
+constexpr double kFixLimit = 8.0 * 1152921504606846976.0 - 4096.0;
+
+__device__ __forceinline__ unsigned long long to_fixed(double w) {
+  double q = w * kFixScale;
+  q = q > kFixLimit ? kFixLimit : (q < -kFixLimit ? -kFixLimit : q);
+  return (unsigned long long)__double2ll_rn(q);
+}
 
 __device__ __forceinline__ double p_log1p(double ri, double rj, double s) {
   double denom = ri + rj - s;
@@ -29,7 +44,7 @@ __device__ __forceinline__ double p_log1p(double ri, double rj, double s) {
 }
 
 struct FillSmem {
-  double acc[kWaves][kTile];
+  unsigned long long acc[kWaves][kTile];
   double g[SMX_TRI_FIELDS][kChunk];
   int32_t lcol[3][kChunk];
   int32_t lgid[3][kChunk];
@@ -48,7 +63,7 @@ __device__ __forceinline__ void fill_rows(
   const int64_t r = row0 + wid;
   const bool active = r < nrows;
 
-  for (int c = lane; c < ncols; c += smx::kWave) sm.acc[wid][c] = 0.0;
+  for (int c = lane; c < ncols; c += smx::kWave) sm.acc[wid][c] = 0ull;
 
   int32_t grow = -1;
   double x0 = 0.0, x1 = 0.0, x2 = 0.0;
@@ -122,16 +137,17 @@ __device__ __forceinline__ void fill_rows(
         const double w1 = s2 * c * (e2 * omega - h * gP1);
         const double w2 = s0 * c * (e0 * omega - h * gP2);
         const int32_t c0 = sm.lcol[0][t], c1 = sm.lcol[1][t], c2 = sm.lcol[2][t];
-        if (c0 >= 0) atomicAdd(&sm.acc[wid][c0], w0);
-        if (c1 >= 0) atomicAdd(&sm.acc[wid][c1], w1);
-        if (c2 >= 0) atomicAdd(&sm.acc[wid][c2], w2);
+        if (c0 >= 0) atomicAdd(&sm.acc[wid][c0], to_fixed(w0));
+        if (c1 >= 0) atomicAdd(&sm.acc[wid][c1], to_fixed(w1));
+        if (c2 >= 0) atomicAdd(&sm.acc[wid][c2], to_fixed(w2));
       }
     }
   }
   __syncthreads();
   if (active) {
     double* orow = out + r * ld_row;
-    for (int c = lane; c < ncols; c += smx::kWave) orow[(int64_t)c * ld_col] = sm.acc[wid][c];
+    for (int c = lane; c < ncols; c += smx::kWave)
+      orow[(int64_t)c * ld_col] = (double)(long long)sm.acc[wid][c] * kFixInv;
   }
 }
 

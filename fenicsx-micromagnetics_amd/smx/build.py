@@ -25,7 +25,6 @@ HIPCC_FLAGS = [
     f"--offload-arch={ARCH}",
     # packed FP32 instructions are kept out of every kernel
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-munsafe-fp-atomics",
     "-Wno-unused-command-line-argument",
 ]
 
@@ -41,19 +40,41 @@ def sources():
     return sorted(CSRC.glob("*.hip"))
 
 
-def build(verbose: bool = True) -> Path:
-    """Compile all kernels into LIB_PATH (always recompiles)."""
+def build(verbose: bool = True, jobs: int | None = None) -> Path:
+    """Compile all kernels into LIB_PATH (always recompiles): every source to
+    an object in parallel, then one link."""
+    from concurrent.futures import ThreadPoolExecutor
+    import tempfile
     srcs = sources()
     if not srcs:
         raise RuntimeError(f"no HIP sources under {CSRC}")
-    tmp = LIB_PATH.with_suffix(".so.tmp")
-    cmd = [hipcc(), *HIPCC_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-o", str(tmp), *map(str, srcs)]
-    if verbose:
-        print("[smx.build]", " ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError(f"hipcc failed with exit code {res.returncode}")
+    exe = hipcc()
+    objflags = [f for f in HIPCC_FLAGS if f != "-shared"]
+    with tempfile.TemporaryDirectory(prefix="smx_build_") as tmpd:
+        objs = [Path(tmpd) / (src.stem + ".o") for src in srcs]
+
+        def compile_one(pair):
+            src, obj = pair
+            cmd = [exe, *objflags, f"-I{INCLUDE}", f"-I{CSRC}", "-c", str(src), "-o", str(obj)]
+            return cmd, subprocess.run(cmd, capture_output=True, text=True)
+
+        n = jobs or min(len(srcs), max(1, (os.cpu_count() or 1)))
+        with ThreadPoolExecutor(max_workers=n) as pool:
+            results = list(pool.map(compile_one, zip(srcs, objs)))
+        for cmd, res in results:
+            if verbose:
+                print("[smx.build]", " ".join(cmd), flush=True)
+            if res.returncode != 0:
+                sys.stderr.write(res.stdout + res.stderr)
+                raise RuntimeError(f"hipcc failed with exit code {res.returncode}")
+        tmp = LIB_PATH.with_suffix(".so.tmp")
+        cmd = [exe, *HIPCC_FLAGS, "-o", str(tmp), *map(str, objs)]
+        if verbose:
+            print("[smx.build]", " ".join(cmd), flush=True)
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout + res.stderr)
+            raise RuntimeError(f"hipcc link failed with exit code {res.returncode}")
     os.replace(tmp, LIB_PATH)
     if verbose:
         print(f"[smx.build] wrote {LIB_PATH}", flush=True)

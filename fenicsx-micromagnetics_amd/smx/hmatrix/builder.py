@@ -511,12 +511,17 @@ def build_hmatrix_device(geo: BoundaryGeometry, config: HMatrixBuildConfig | Non
         add_near(rejected, fill_budget)
         queue = nxt
 
-    # far blocks in the order of the depth-first recursion
+    # far blocks in the order of the depth-first recursion; the block tree
+    # (every visited block and its fate) is kept for inspection
     far_blocks = []
+    visited = []
+    fate_code = {"far": 0, "dense": 1}
     stack = [(tree.root, tree.root)]
     while stack:
         r, c = stack.pop()
         d = decision[key(r, c)]
+        code = fate_code.get(d[0], 2 if d[0] == "split" and d[1] == "row" else 3)
+        visited.append((r.start, r.stop, c.start, c.stop, code))
         if d[0] == "far":
             far_blocks.append(far_of[key(r, c)])
         elif d[0] == "split":
@@ -538,6 +543,9 @@ def build_hmatrix_device(geo: BoundaryGeometry, config: HMatrixBuildConfig | Non
                 lindholm_pair_evaluations=int(getattr(ops, "evaluations", 0)), **stats)
     data = HMatrixCPUData(perm=perm.astype(np.int32), diag_jump=geo.diag_jump.astype(np.float64),
                           near_csr=near, far_blocks=far_blocks, metadata=meta)
+    # rows (row_start, row_stop, col_start, col_stop, fate) in depth-first
+    # order; fate 0 far, 1 dense, 2 split by rows, 3 split by columns
+    data.block_tree = np.asarray(visited, dtype=np.int64).reshape(-1, 5)
     data.metadata["near_nnz"] = int(near.nnz)
     data.metadata["far_storage_entries"] = int(data.far_storage_entries)
     data.metadata["stored_entries_total"] = int(data.stored_entries)

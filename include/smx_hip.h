@@ -18,7 +18,10 @@
  *     hipStreamSynchronize(stream) between batches of iterations, because it
  *     returns the iteration count and residual in host memory; their
  *     speculative form (check_every < 0) returns without waiting.
- *   - Reductions are deterministic: fixed-order, no floating-point atomics.
+ *   - Results are deterministic (bitwise, run to run): reductions sum in a
+ *     fixed order, and no floating-point atomics are used.  The one atomic
+ *     accumulation, the Lindholm row assembly, adds 64-bit fixed-point
+ *     integers (units of 2^-60), which is independent of the order.
  *
  * Reference interfaces replaced (paths relative to src/spinfemx of the
  * reference) are named at each entry point.

@@ -244,6 +244,8 @@ def _hef_pair(m, bd, B, **extra):
     fem = O.fem_operators(m.X, m.tets, 1.0)
     Kex = O.exchange_matrix(m.X, m.tets, fem.V, kw["Aex"], kw["Ms"])
     Kd = O.dmi_interfacial_matrix(m.X, m.tets, fem.V, kw["D_int"], kw["Ms"], kw["n0_int_vec"])
+    if kw.get("D_bulk", 0.0):
+        Kd = Kd + O.dmi_bulk_matrix(m.X, m.tets, fem.V, kw["D_bulk"], kw["Ms"])
 
     def field(mv, demag=True):
         H = Kex @ mv + Kd @ mv + O.anisotropy_field(mv, kw["n_ani_vec"], kw["Ku"], kw["Ms"])
@@ -270,6 +272,34 @@ def test_llg_rhs_c1(c1):
     assert eH <= 1e-10 and er <= 1e-10
 
 
+@pytest.mark.parametrize("D_int", [0.0, 3e-3])
+def test_llg_rhs_bulk_dmi_c1(c1, D_int):
+    """Bulk DMI (D = 3e-3 J/m^2) alone and with interfacial DMI, through the
+    fused field kernel, against the oracle's dmi_bulk_matrix (DMI_Bulk_GPU.py:
+    43-86, 248-264): H and RHS within 1e-10 relative L2; the JVP too."""
+    torch = _torch()
+    m, bd, B = c1
+    hef, field, kw = _hef_pair(m, bd, B, D_bulk=3e-3, D_int=D_int)
+    mv = random_unit(m.num_nodes, 14)
+    out = torch.zeros(3 * m.num_nodes, dtype=torch.float64, device="cuda")
+    mt = torch.from_numpy(mv).cuda()
+    hef.llg_rhs_vec(mt, out)
+    torch.cuda.synchronize()
+    Hr = field(mv)
+    rr = O.llg_rhs(mv, Hr, 2.211e5, kw["alpha"], kw["Ms"])
+    eH = rel_l2(hef.H_eff_gpu.cpu().numpy(), Hr)
+    er = rel_l2(out.cpu().numpy(), rr)
+    v = np.random.default_rng(15).standard_normal(3 * m.num_nodes)
+    hef.update_jac_state(mt)
+    jv = torch.zeros_like(mt)
+    hef.jac_vec_times_vec(torch.from_numpy(v).cuda(), jv)
+    torch.cuda.synchronize()
+    ej = rel_l2(jv.cpu().numpy(), O.jac_vec(mv, field(mv, demag=False), v, field(v, demag=False), 2.211e5,
+                                            kw["alpha"], kw["Ms"]))
+    print(f"[llg rhs bulk DMI c1, D_int={D_int}] H {eH:.3e} rhs {er:.3e} jvp {ej:.3e}", flush=True)
+    assert eH <= 1e-10 and er <= 1e-10 and ej <= 1e-10
+
+
 def test_llg_jvp_c1(c1):
     torch = _torch()
     m, bd, B = c1
@@ -291,6 +321,11 @@ def test_llg_jvp_c1(c1):
 
 
 def test_dp5_fixed_steps_c1(c1):
+    """The 40 nm cube workload: exchange + FEM/BEM demag, alpha 0.5, seeded
+    random m, 100 fixed Dormand-Prince steps of 1e-13 s (601 RHS
+    evaluations) on the device against the oracle's (LU Poisson solves).
+    Bound: 1e-9 relative L2 on the final state (each evaluation agrees to
+    ~1e-12; the bound leaves room for their accumulation over 100 steps)."""
     torch = _torch()
     from smx.llg import DormandPrince
     m, bd, B = c1
@@ -299,18 +334,21 @@ def test_dp5_fixed_steps_c1(c1):
     y = torch.from_numpy(mv.copy()).cuda()
     dp = DormandPrince(hef)
     dt = 1e-13
-    nsteps = 5
-    dp.run_steps(y, 0.0, dt, nsteps)
-    torch.cuda.synchronize()
+    nsteps = 100
     f = lambda t, yy: O.llg_rhs(yy, field(yy), 2.211e5, kw["alpha"], kw["Ms"])
     yr = mv.copy()
     k1 = None
-    for _ in range(nsteps):
-        yr, _, k1 = O.dp5_step(f, 0.0, yr, dt, k1)
-    e = rel_l2(y.cpu().numpy(), yr)
+    worst = 0.0
+    for chunk in range(4):
+        dp.run_steps(y, chunk * 25 * dt, dt, 25)
+        torch.cuda.synchronize()
+        for _ in range(25):
+            yr, _, k1 = O.dp5_step(f, 0.0, yr, dt, k1)
+        worst = max(worst, rel_l2(y.cpu().numpy(), yr))
     d = rel_l2(y.cpu().numpy(), mv)
-    print(f"[dp5 c1] rel err {e:.3e} (change {d:.3e})", flush=True)
-    assert e <= 1e-10 * max(1.0, d / 1e-3)
+    print(f"[dp5 c1] {nsteps} steps: worst rel err {worst:.3e} (change of m {d:.3e}, "
+          f"{dp.rhs_evals} device RHS evaluations)", flush=True)
+    assert worst <= 1e-9 and d > 1e-2
 
 
 def test_demag_against_golden_fixture():
