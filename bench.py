@@ -117,6 +117,42 @@ def amg_iteration_bytes(d):
     return nb
 
 
+def eval_breakdown(hef, pool, reps=10):
+    """Device time of each stage of one demag + RHS evaluation (events on the
+    launch stream between the stages of the product's own sequence, as
+    DemagFieldLindholmGPU.solve_potential and the fused field kernel run it,
+    speculative solves): divergence, u1 solve, boundary matvec, u2 solve,
+    field kernel.  Averages over `reps` evaluations (ms)."""
+    import torch
+    d = hef.demag_field
+    names = ("div", "u1_solve", "boundary_matvec", "u2_solve", "field_rhs")
+    acc = dict.fromkeys(names, 0.0)
+    out = torch.empty_like(pool[0])
+    for i in range(reps + 1):
+        m = pool[i % len(pool)]
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
+        spec = d.cg_speculative and d._its_known
+        ev[0].record()
+        d.Div.spmv(m, d.b_u1, kind=1, partials=d.b_part)
+        ev[1].record()
+        d.solve("u1", 0, d.u1, d.ws1, b=d.b_u1, b_part=d.b_part, expected=d._its_split["u1"][1],
+                speculative=spec)
+        ev[2].record()
+        d.boundary_matvec(d.u1, d.y_rows, gather=d.bverts_gpu, shift_ptr=d.u1_sum_ptr, shift_scale=1.0 / d.N)
+        ev[3].record()
+        d.solve("u2", 1, d.u2, d.ws2, gb=d.y_rows, project=0, expected=d._its_split["u2"][1], speculative=spec)
+        ev[4].record()
+        hef._launch(hef._args(m, h_eff=hef.H_eff_gpu, rhs=out))
+        ev[5].record()
+        ev[5].synchronize()
+        if i == 0:
+            continue                                   # warm-up
+        for k, nm in enumerate(names):
+            acc[nm] += ev[k].elapsed_time(ev[k + 1]) / reps
+    acc["total"] = sum(acc[nm] for nm in names)
+    return acc
+
+
 def cg_iteration_timing(d, K=200):
     """Average duration of one CG iteration (u1 operator), events on the
     launch stream: (solve with K iterations - solve with 0) / K.  For the AMG
@@ -656,7 +692,11 @@ def build_case(config, dev):
         h64 = EffectiveField(mesh, use_demag=True, operators=ops, demag_kwargs=dict(B_dtype="float64", **common),
                              **mat)
         hmx = EffectiveField(mesh, use_demag=True, operators=ops,
-                             demag_kwargs=dict(B_dtype="float32", geometry=h64.demag_field.opB, **common), **mat)
+                             demag_kwargs=dict(B_dtype="float32", geometry=h64.demag_field.opB, cg_mixed=True,
+                                               **common), **mat)
+        h_f64cg = EffectiveField(mesh, use_demag=True, operators=ops,
+                                 demag_kwargs=dict(B_dtype="float32", geometry=h64.demag_field.opB, **common),
+                                 **mat)
         pool = [torch.from_numpy(vortex_m(mesh.X, seed=s)).to(dev) for s in range(8)]
         work = "cube150nm_30x30x30_kuhn (N=29791, Nb=5402, dense B)"
         extra = {}
@@ -676,8 +716,10 @@ def build_case(config, dev):
         hk = dict(boundary_backend="hmatrix", geometry=geo, hmatrix_prebuilt=data, **common)
         h64 = EffectiveField(mesh, use_demag=True, operators=ops, demag_kwargs=dict(hmatrix_use_fp32=False, **hk),
                              **mat)
-        hmx = EffectiveField(mesh, use_demag=True, operators=ops, demag_kwargs=dict(hmatrix_use_fp32=True, **hk),
-                             **mat)
+        hmx = EffectiveField(mesh, use_demag=True, operators=ops,
+                             demag_kwargs=dict(hmatrix_use_fp32=True, cg_mixed=True, **hk), **mat)
+        h_f64cg = EffectiveField(mesh, use_demag=True, operators=ops,
+                                 demag_kwargs=dict(hmatrix_use_fp32=True, **hk), **mat)
         pool = [torch.from_numpy(random_m(mesh.num_nodes, seed=s)).to(dev) for s in range(4)]
         work = "film500x500x10nm_200x200x4_kuhn (N=202005, Nb=83202, H-matrix leaf 64, eta 2, eps 1e-6)"
         md = data.metadata
@@ -686,6 +728,7 @@ def build_case(config, dev):
                  "average_far_rank": round(float(md["average_far_rank"]), 2),
                  "lindholm_pair_evaluations": int(md["lindholm_pair_evaluations"])}
         extra["_data"] = data
+    extra["_h_f64cg"] = h_f64cg
     torch.cuda.synchronize()
     return mesh, mat, h64, hmx, pool, work, extra, time.perf_counter() - t0
 
@@ -749,17 +792,31 @@ def cpu_baseline(config, mesh, mat, pool, extra, h64, budget_s):
             H = H + O.anisotropy_field(mv, mat["n_ani_vec"], mat["Ku"], Ms)
         return O.llg_rhs(mv, H, GAMMA, alpha, Ms)
 
-    n = 0
-    t0 = time.perf_counter()
-    while True:
-        rhs(pool[n % len(pool)])
-        n += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s or n >= 200:
-            break
-    threads = os.environ.get("OMP_NUM_THREADS") or os.environ.get("OPENBLAS_NUM_THREADS") or str(os.cpu_count())
+    def timed(budget):
+        n = 0
+        t0 = time.perf_counter()
+        while True:
+            rhs(pool[n % len(pool)])
+            n += 1
+            el = time.perf_counter() - t0
+            if el >= budget or n >= 200:
+                return n, el
+
+    from threadpoolctl import threadpool_info, threadpool_limits
+    threads = max((p.get("num_threads", 1) for p in threadpool_info()), default=1)
+    n, el = timed(budget_s)
+    with threadpool_limits(limits=1):
+        n1, el1 = timed(budget_s / 2)
+    cpu = ""
+    try:
+        cpu = next(l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
     return {"value": n / el, "unit": "evals/s", "cores": int(threads), "kind": "port",
-            "sample": f"{n} FP64 RHS evals of the same workload in {el:.1f}s ({how}; setup {setup:.1f}s excluded)"}
+            "sample": f"{n} FP64 RHS evals of the same workload in {el:.1f}s ({how}; setup {setup:.1f}s excluded)",
+            "cpu_model": cpu,
+            "single_thread": {"value": n1 / el1, "unit": "evals/s", "cores": 1,
+                              "sample": f"{n1} evals in {el1:.1f}s with the BLAS pool limited to 1 thread"}}
 
 
 def _pinned(A):
@@ -852,6 +909,12 @@ def main():
     tmx, itsmx = run(hmx, args.steps, args.warmup)
     redos_mx = run.redos
     log(f"mixed {args.steps / tmx:.2f} evals/s ({redos_mx} speculative solve repairs)")
+    h_f64cg = extra["_h_f64cg"]
+    tmf, _ = run(h_f64cg, args.steps, args.warmup)
+    log(f"mixed boundary, FP64 CG {args.steps / tmf:.2f} evals/s")
+    bd64 = eval_breakdown(h64, pool)
+    bdmx = eval_breakdown(hmx, pool)
+    log(f"stage times (ms) fp64 {bd64} mixed {bdmx}")
     r64 = torch.empty_like(out)
     rmx = torch.empty_like(out)
     errs, herr = [], []
@@ -860,6 +923,22 @@ def main():
         hmx.llg_rhs_vec(mv, rmx)
         errs.append(float(torch.linalg.norm(rmx - r64) / torch.linalg.norm(r64)))
         herr.append(float(torch.linalg.norm(hmx.H_demag_gpu - h64.H_demag_gpu) / torch.linalg.norm(h64.H_demag_gpu)))
+    dmx = hmx.demag_field
+    krylov64 = (bd64["u1_solve"] + bd64["u2_solve"]) / bd64["total"]
+    krylovmx = (bdmx["u1_solve"] + bdmx["u2_solve"]) / bdmx["total"]
+    precision = {
+        "mixed": ("FP32 H-matrix factors; FP32 inner CG (vectors and iteration operator) with FP64 x, scalars and "
+                  "compensated inner products, one FP64 iterative-refinement restart per solve "
+                  f"(inner solve to rtol^(1/2)); FP32 AMG values"),
+        "mixed_fp64_cg_evals_per_s": args.steps / tmf,
+        "stage_ms_fp64": {k: round(v, 4) for k, v in bd64.items()},
+        "stage_ms_mixed": {k: round(v, 4) for k, v in bdmx.items()},
+        "krylov_share_fp64": krylov64,
+        "krylov_share_mixed": krylovmx,
+        "amdahl_ratio_if_mixed_matvec_free": bd64["total"] / (bdmx["total"] - bdmx["boundary_matvec"]),
+        "amdahl_ratio_if_mixed_krylov_free": bd64["total"] / (bdmx["total"] - bdmx["u1_solve"] - bdmx["u2_solve"]),
+        "cg_iterations_mixed_inner_last": {"u1": list(dmx._its_split["u1"]), "u2": list(dmx._its_split["u2"])},
+    }
     cg_t, cg_bytes = cg_iteration_timing(hmx.demag_field)
     mv_cold, mv_warm, mv_bytes, mv_name = matvec_timing(hmx.demag_field)
     mv64_cold, _, mv64_bytes, _ = matvec_timing(h64.demag_field)
@@ -880,13 +959,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 (boundary operator f32)",
+        "dtype": "f64 (boundary operator and inner CG f32)",
         "data": "synthetic (seeded)",
         "config": cfg,
         "fp64": {"value": args.steps / t64, "ms_per_step": 1e3 * t64 / args.steps},
         "mixed_over_fp64": t64 / tmx,
         "mixed_vs_fp64_rel_err_rhs": max(errs),
         "mixed_vs_fp64_rel_err_hdemag": max(herr),
+        "precision_study": precision,
         "cg_iterations_u1_u2": [float(np.mean([a for a, _ in itsmx])), float(np.mean([b for _, b in itsmx]))],
         "pc": [hmx.demag_field.pc_type, hmx.demag_field.pc_type_u2],
         "amg_levels_u1": hmx.demag_field.amg_u1.describe() if hmx.demag_field.amg_u1 else None,

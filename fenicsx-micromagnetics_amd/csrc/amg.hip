@@ -38,8 +38,13 @@
 // row pointer and every lane of a slice runs the same trip count.  Each lane
 // sums every TPR-th slot of its row and the lanes of a row meet in a fixed
 // butterfly, so results do not depend on scheduling.  Values are FP64 or FP32
-// (the FP32 hierarchy halves the bytes of the preconditioner; the outer CG and
-// every vector stay FP64).
+// (the FP32 hierarchy halves the bytes of the preconditioner).
+//
+// Mixed precision (smx_pcg_amg_mixed): the fine-level iteration vectors
+// (r, s, w, u, p) and the operator of w = A u are FP32; x, the coarse-level
+// vectors, the scalars and the inner products stay FP64 (exact products,
+// compensated sums), and iterative refinement restarts the solve from the
+// current x with the residual recomputed from the FP64 operator.
 #include "smx_common.h"
 
 namespace {
@@ -52,6 +57,7 @@ enum {
   SS_BB = 4,
   SS_FAIL = 5,  // 1 breakdown, 2 max_it reached
   SS_SUMX = 6,
+  SS_ITERS0 = 7,  // iterations of the inner solves before the last (mixed precision)
   SS_SIZE = 8,
   ST_OFF = 8,   // [gamma, alpha, 1/gamma, 1/alpha] x 2 parities
   ST_SIZE = 8,
@@ -61,6 +67,8 @@ static_assert(SS_DONE == SMX_PCG_DONE && SS_ITERS == SMX_PCG_ITERS && SS_FAIL ==
                   SS_SUMX == SMX_PCG_SUMX,
               "status block layout of the C ABI");
 
+// Fine-level iteration vectors are FP64 or FP32 (a kernel template
+// parameter); x and the coarse vectors are FP64.
 struct AmgPtrs {
   double* ss;
   double* st[2];
@@ -68,13 +76,24 @@ struct AmgPtrs {
   double* prr[2];   // 2 x Gf partials (rr, sum x), by iteration parity
   double* pg;       // Gf partials of (r, u)
   double* pd;       // Gf partials of (w, u)
-  double* r[2];
-  double* s[2];
-  double* w;
-  double* u;
-  double* p;
+  void* r[2];
+  void* s[2];
+  void* w;
+  void* u;
+  void* p;
   double* z1;       // R_0 s (coarse, carried)
   int Gf;           // fine workgroups (one row per thread)
+};
+
+template <typename V>
+__host__ __device__ __forceinline__ V* vec(void* p) {
+  return static_cast<V*>(p);
+}
+
+// the vector types of a launch are given explicitly, not deduced
+template <typename T>
+struct Ident {
+  using type = T;
 };
 
 struct SellOp {
@@ -94,12 +113,15 @@ __device__ __forceinline__ int xcd_map(int b, int G) {
   return x * q + (x < r ? x : r) + k;
 }
 
-template <int BS, int NQ, bool ALL>
-__device__ __forceinline__ void block_total(double (&v)[NQ], double* scr) {
+// COMP: every addition compensated (TwoSum), v[q] + c[q] the per-thread input.
+template <int BS, int NQ, bool ALL, bool COMP = false>
+__device__ __forceinline__ void block_total(double (&v)[NQ], double* scr, const double* c = nullptr) {
   constexpr int nw = BS / 64;
   const int wid = threadIdx.x >> 6;
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) v[q] = smx::wave_total(v[q]);
+  for (int q = 0; q < NQ; ++q) {
+    v[q] = COMP ? smx::wave_total_comp(v[q], c ? c[q] : 0.0) : smx::wave_total(v[q]);
+  }
   if ((threadIdx.x & 63) == 0) {
 #pragma unroll
     for (int q = 0; q < NQ; ++q) scr[wid * NQ + q] = v[q];
@@ -108,10 +130,16 @@ __device__ __forceinline__ void block_total(double (&v)[NQ], double* scr) {
   if (ALL || threadIdx.x == 0) {
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-      double s = 0.0;
+      double s = 0.0, e = 0.0;
 #pragma unroll
-      for (int w = 0; w < nw; ++w) s += scr[w * NQ + q];
-      v[q] = s;
+      for (int w = 0; w < nw; ++w) {
+        if (COMP) {
+          smx::comp_add(s, e, scr[w * NQ + q]);
+        } else {
+          s += scr[w * NQ + q];
+        }
+      }
+      v[q] = COMP ? s + e : s;
     }
   }
 }
@@ -136,8 +164,8 @@ __device__ __forceinline__ double ldv(const void* v, int64_t k) {
 // exact zero).  Slots are column-major in the slice for TPR == 1 (lanes of a
 // wavefront read consecutive rows) and row-major otherwise (lanes of a row
 // read consecutive slots).
-template <int TPR, typename VT>
-__device__ __forceinline__ double sell_row(const SellOp& M, const double* __restrict__ v, int32_t off,
+template <int TPR, typename VT, typename XT>
+__device__ __forceinline__ double sell_row(const SellOp& M, const XT* __restrict__ v, int32_t off,
                                            int32_t W, int i, int lane) {
   constexpr int U = 4;
   constexpr int RPS = kSliceThreads / TPR;
@@ -152,7 +180,7 @@ __device__ __forceinline__ double sell_row(const SellOp& M, const double* __rest
       const int32_t pos = TPR == 1 ? off + jj * RPS + i : off + i * W + jj;
       const int32_t c = M.col ? M.col[pos] : jj;
       vv[u] = ok ? ldv<VT>(M.val, pos) : 0.0;
-      g[u] = v[c];
+      g[u] = (double)v[c];
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) acc = fma(vv[u], g[u], acc);
@@ -170,13 +198,13 @@ __device__ __forceinline__ double next_r(double r, double a, double sn) { return
 // r0 = b - A x0 (mode 0) or the lifted residual (mode 1); s = p = 0.
 // Partials: ||b||^2 (pbb) and (||r0||^2, sum x0) (prr[0]).  Block 0 resets the
 // status record.
-template <int BS, int WMAX>
+template <int BS, int WMAX, typename V>
 __global__ __launch_bounds__(BS) void amg_init_kernel(
     int64_t n, int32_t mode, int32_t W, int64_t ld, const int32_t* __restrict__ cols,
     const double* __restrict__ vals, const double* __restrict__ b,
     const double* __restrict__ b_part, int64_t nb_part, int32_t project,
     const int32_t* __restrict__ bpos, const double* __restrict__ gb, double* __restrict__ x,
-    int32_t warm, const int32_t* __restrict__ perm, AmgPtrs P, int64_t n1) {
+    int32_t warm, const int32_t* __restrict__ perm, AmgPtrs P, int64_t n1, int32_t phase) {
   __shared__ double scr[BS / 64];
   __shared__ double scr_pub[BS / 64 * 3];
   const int blk = xcd_map(blockIdx.x, P.Gf);
@@ -236,11 +264,12 @@ __global__ __launch_bounds__(BS) void amg_init_kernel(
     // rows of mode 1 (their neighbours read gb instead).
     if (!warm && (mode == 0 || bpos[row] < 0)) x[row] = 0.0;
     if (mode == 1 && bpos[row] >= 0) x[row] = x0;
-    P.r[0][row] = r0;
-    P.s[0][row] = 0.0;
-    P.p[row] = 0.0;
+    const V r0v = (V)r0;
+    vec<V>(P.r[0])[row] = r0v;
+    vec<V>(P.s[0])[row] = (V)0.0;
+    vec<V>(P.p)[row] = (V)0.0;
     v[0] = bk * bk;
-    v[1] = r0 * r0;
+    v[1] = (double)r0v * (double)r0v;
     v[2] = x0;
   }
   block_total<BS, 3, false>(v, scr_pub);
@@ -250,6 +279,8 @@ __global__ __launch_bounds__(BS) void amg_init_kernel(
     P.prr[0][P.Gf + blk] = v[2];
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
+    // a refinement restart (phase > 0) adds the previous solves' count
+    P.ss[SS_ITERS0] = phase == 0 ? 0.0 : P.ss[SS_ITERS0] + P.ss[SS_ITERS];
     P.ss[SS_DONE] = 0.0;
     P.ss[SS_FAIL] = 0.0;
     P.ss[SS_ITERS] = 0.0;
@@ -258,12 +289,12 @@ __global__ __launch_bounds__(BS) void amg_init_kernel(
 
 // One level launch: y = M1 v1 (+ M2 v2), one slice of both operators per
 // workgroup (M1 and M2 have the same rows and slicing).
-template <int TPR, typename VT>
+template <int TPR, typename VT, typename X1, typename X2, typename Y>
 __global__ __launch_bounds__(kSliceThreads) void amg_level_kernel(SellOp M1,
-                                                                  const double* __restrict__ v1,
+                                                                  const X1* __restrict__ v1,
                                                                   SellOp M2,
-                                                                  const double* __restrict__ v2,
-                                                                  double* __restrict__ y,
+                                                                  const X2* __restrict__ v2,
+                                                                  Y* __restrict__ y,
                                                                   const double* __restrict__ ss) {
   constexpr int RPS = kSliceThreads / TPR;
   // the done flag (written by an earlier launch) is tested once the slice
@@ -282,7 +313,7 @@ __global__ __launch_bounds__(kSliceThreads) void amg_level_kernel(SellOp M1,
   const int64_t row = (int64_t)sl * RPS + i;
   double acc = sell_row<TPR, VT>(M1, v1, a0, (a1 - a0) / RPS, i, lane);
   if (M2.off) acc += sell_row<TPR, VT>(M2, v2, b0, (b1 - b0) / RPS, i, lane);
-  if (lane == 0 && row < M1.rows) y[row] = acc;
+  if (lane == 0 && row < M1.rows) y[row] = (Y)acc;
 }
 
 // Start of iteration `it`: scalars from the partial sums, convergence test,
@@ -290,7 +321,7 @@ __global__ __launch_bounds__(kSliceThreads) void amg_level_kernel(SellOp M1,
 // slices of R_0 per workgroup.  Everything that does not need the scalars
 // (the fine vectors, and all of R_0 w) is loaded or computed before the
 // reduction, so the reduction is the only wait on the partial sums.
-template <int BS, int TPR, typename VT>
+template <int BS, int TPR, typename VT, typename V, bool COMP>
 __global__ __launch_bounds__(BS) void amg_update_kernel(int64_t n, AmgPtrs P, double* __restrict__ x,
                                                         SellOp R0, double* __restrict__ b1,
                                                         int32_t it, int32_t max_it, double rtol,
@@ -303,15 +334,15 @@ __global__ __launch_bounds__(BS) void amg_update_kernel(int64_t n, AmgPtrs P, do
   const bool fine = (int)blockIdx.x < Gf;
   // partial sums first: the reduction waits only on these
   const int b0 = (int)threadIdx.x < Gf ? (int)threadIdx.x : 0;
-  double t[5];
+  double t[5], tc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   t[0] = P.pg[b0];
   t[1] = P.pd[b0];
   t[2] = P.prr[q][b0];
   t[3] = P.prr[q][Gf + b0];
   t[4] = P.pbb[b0];
-  const double* __restrict__ rq = P.r[q];
-  const double* __restrict__ sq = P.s[q];
-  const double* __restrict__ w = P.w;
+  const V* __restrict__ rq = vec<V>(P.r[q]);
+  const V* __restrict__ sq = vec<V>(P.s[q]);
+  const V* __restrict__ w = vec<V>(P.w);
   // fine role: the row's vectors; restriction role: (R_0 w)_j and b_1, z_1
   const int blk = fine ? xcd_map(blockIdx.x, Gf) : 0;
   const int64_t row = fine ? (int64_t)blk * BS + threadIdx.x
@@ -319,12 +350,12 @@ __global__ __launch_bounds__(BS) void amg_update_kernel(int64_t n, AmgPtrs P, do
   double f0 = 0.0, f1 = 0.0, f2 = 0.0, f3 = 0.0, f4 = 0.0, f5 = 0.0;
   if (fine) {
     const int64_t rc = row < n ? row : n - 1;
-    f0 = w[rc];
-    f1 = sq[rc];
-    f2 = P.p[rc];
-    f3 = P.u[rc];
+    f0 = (double)w[rc];
+    f1 = (double)sq[rc];
+    f2 = (double)vec<V>(P.p)[rc];
+    f3 = (double)vec<V>(P.u)[rc];
     f4 = x[rc];
-    f5 = rq[rc];
+    f5 = (double)rq[rc];
   } else {
     constexpr int RPS = kSliceThreads / TPR;
     const int sl = (int)(row / RPS);
@@ -343,13 +374,17 @@ __global__ __launch_bounds__(BS) void amg_update_kernel(int64_t n, AmgPtrs P, do
     for (int k = 0; k < 5; ++k) t[k] = 0.0;
   }
   for (int b = threadIdx.x + BS; b < Gf; b += BS) {  // only when Gf > BS
-    t[0] += P.pg[b];
-    t[1] += P.pd[b];
-    t[2] += P.prr[q][b];
-    t[3] += P.prr[q][Gf + b];
-    t[4] += P.pbb[b];
+    const double e[5] = {P.pg[b], P.pd[b], P.prr[q][b], P.prr[q][Gf + b], P.pbb[b]};
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      if (COMP) {
+        smx::comp_add(t[k], tc[k], e[k]);
+      } else {
+        t[k] += e[k];
+      }
+    }
   }
-  block_total<BS, 5, true>(t, scr);
+  block_total<BS, 5, true, COMP>(t, scr, tc);
   const double gam = t[0], del = t[1], rr = t[2];
   const double tol2 = fmax(rtol * rtol * t[4], atol * atol);
   double alpha, beta;
@@ -385,18 +420,18 @@ __global__ __launch_bounds__(BS) void amg_update_kernel(int64_t n, AmgPtrs P, do
   if (fine) {
     double v[2] = {0.0, 0.0};
     if (row < n) {
-      const double s_n = next_s(f0, beta, f1);
-      const double p_n = fma(beta, f2, f3);
-      const double x_n = fma(alpha, p_n, f4);
-      const double r_n = next_r(f5, alpha, s_n);
-      __builtin_nontemporal_store(s_n, &P.s[1 - q][row]);
-      __builtin_nontemporal_store(r_n, &P.r[1 - q][row]);
-      __builtin_nontemporal_store(p_n, &P.p[row]);
+      const V s_n = (V)next_s(f0, beta, f1);
+      const V p_n = (V)fma(beta, f2, f3);
+      const double x_n = fma(alpha, (double)p_n, f4);
+      const V r_n = (V)next_r(f5, alpha, (double)s_n);
+      __builtin_nontemporal_store(s_n, &vec<V>(P.s[1 - q])[row]);
+      __builtin_nontemporal_store(r_n, &vec<V>(P.r[1 - q])[row]);
+      __builtin_nontemporal_store(p_n, &vec<V>(P.p)[row]);
       __builtin_nontemporal_store(x_n, &x[row]);
-      v[0] = r_n * r_n;
+      v[0] = (double)r_n * (double)r_n;
       v[1] = x_n;
     }
-    block_total<BS, 2, false>(v, scr_pub);
+    block_total<BS, 2, false, COMP>(v, scr_pub);
     if (threadIdx.x == 0) {
       P.prr[1 - q][blk] = v[0];
       P.prr[1 - q][Gf + blk] = v[1];
@@ -410,11 +445,11 @@ __global__ __launch_bounds__(BS) void amg_update_kernel(int64_t n, AmgPtrs P, do
 
 // w = A u (plain operator, ELL), partials of (w, u) and (r, u), r the
 // residual the preconditioner was applied to.
-template <int BS, int WMAX, bool EXACT>
+template <int BS, int WMAX, bool EXACT, typename V, typename AV, bool COMP>
 __global__ __launch_bounds__(BS) void amg_matvec_kernel(int64_t n, int32_t W, int64_t ld,
                                                         const int32_t* __restrict__ cols,
-                                                        const double* __restrict__ vals, AmgPtrs P,
-                                                        const double* __restrict__ r) {
+                                                        const AV* __restrict__ vals, AmgPtrs P,
+                                                        const V* __restrict__ r) {
   __shared__ double scr[BS / 64 * 2];
   const bool was_done = P.ss[SS_DONE] != 0.0;  // tested once the matrix loads are in flight
   const int blk = xcd_map(blockIdx.x, P.Gf);
@@ -426,21 +461,22 @@ __global__ __launch_bounds__(BS) void amg_matvec_kernel(int64_t n, int32_t W, in
 #pragma unroll
   for (int j = 0; j < WMAX; ++j) {
     c[j] = j < Wk ? cols[j * ld + rowc] : (int32_t)rowc;
-    av[j] = j < Wk ? vals[j * ld + rowc] : 0.0;
+    av[j] = j < Wk ? (double)vals[j * ld + rowc] : 0.0;
   }
   if (was_done) return;  // uniform
   double t[2] = {0.0, 0.0};
   if (row < n) {
-    const double* __restrict__ u = P.u;
+    const V* __restrict__ u = vec<V>(P.u);
     double acc = 0.0;
 #pragma unroll
-    for (int j = 0; j < WMAX; ++j) acc = fma(av[j], u[c[j]], acc);
-    __builtin_nontemporal_store(acc, &P.w[row]);
-    const double uk = u[row];
-    t[0] = acc * uk;
-    t[1] = r[row] * uk;
+    for (int j = 0; j < WMAX; ++j) acc = fma(av[j], (double)u[c[j]], acc);
+    const V w_n = (V)acc;
+    __builtin_nontemporal_store(w_n, &vec<V>(P.w)[row]);
+    const double uk = (double)u[row];
+    t[0] = (double)w_n * uk;
+    t[1] = (double)r[row] * uk;
   }
-  block_total<BS, 2, false>(t, scr);
+  block_total<BS, 2, false, COMP>(t, scr);
   if (threadIdx.x == 0) {
     P.pd[blk] = t[0];
     P.pg[blk] = t[1];
@@ -475,17 +511,18 @@ int64_t slices(int64_t rows, int tpr) {
   return (rows + rps - 1) / rps;
 }
 
-template <typename VT>
-int launch_level(const smx_amg_op& m1, const double* v1, const smx_amg_op* m2, const double* v2,
-                 double* y, const double* ss, int tpr, hipStream_t st) {
+template <typename VT, typename X1 = double, typename X2 = double, typename Y = double>
+int launch_level(const smx_amg_op& m1, const typename Ident<X1>::type* v1, const smx_amg_op* m2,
+                 const typename Ident<X2>::type* v2, typename Ident<Y>::type* y, const double* ss,
+                 int tpr, hipStream_t st) {
   const SellOp a = to_op(m1);
   SellOp b = {nullptr, nullptr, nullptr, 0};
   if (m2 && m2->rows > 0) b = to_op(*m2);
   if (a.rows <= 0) return SMX_OK;
   const unsigned g = (unsigned)slices(a.rows, tpr);
 #define SMX_AMG_LVL(T)                                                                          \
-  hipLaunchKernelGGL((amg_level_kernel<T, VT>), dim3(g), dim3(kSliceThreads), 0, st, a, v1, b, \
-                     v2, y, ss)
+  hipLaunchKernelGGL((amg_level_kernel<T, VT, X1, X2, Y>), dim3(g), dim3(kSliceThreads), 0, st, \
+                     a, v1, b, v2, y, ss)
   switch (tpr) {
     case 1: SMX_AMG_LVL(1); break;
     case 2: SMX_AMG_LVL(2); break;
@@ -515,8 +552,8 @@ bool valid_tpr(int t) { return t == 1 || t == 2 || t == 4 || t == 8 || t == 16 |
 
 // The V-cycle below the fine level and the up0 launch (u = M^-1 r_fine).
 // On entry levels[1].b holds R_0 r_fine (or, with one level, nothing).
-template <typename VT>
-int vcycle_tail(const smx_amg_level* lv, int nlev, const double* r_fine, AmgPtrs& P,
+template <typename VT, typename V>
+int vcycle_tail(const smx_amg_level* lv, int nlev, const V* r_fine, AmgPtrs& P,
                 hipStream_t st) {
   int rc;
   for (int l = 1; l + 1 < nlev; ++l) {  // down: b_{l+1} = R_l b_l
@@ -535,16 +572,22 @@ int vcycle_tail(const smx_amg_level* lv, int nlev, const double* r_fine, AmgPtrs
   }
   const smx_amg_op* q0 = nlev > 1 ? &lv[0].Q : nullptr;
   const double* x1 = nlev > 1 ? lv[1].x : nullptr;
-  return launch_level<VT>(lv[0].S, r_fine, q0, x1, P.u, P.ss, lv[0].s_tpr, st);
+  return launch_level<VT, V, double, V>(lv[0].S, r_fine, q0, x1, vec<V>(P.u), P.ss, lv[0].s_tpr, st);
 }
 
-template <int BS, int WMAX, bool EXACT, typename VT>
-int run_amg(int64_t n, int32_t W, int64_t ld, const int32_t* cols, const double* vals, int32_t mode,
-            const double* b, const double* b_part, int64_t nb_part, int32_t project, int32_t raw_W,
-            int64_t raw_ld, const int32_t* raw_cols, const double* raw_vals, const int32_t* bpos,
-            const double* gb, double* x_user, int32_t warm, const int32_t* perm, double* x_perm,
-            const smx_amg_level* lv, int nlev, double rtol, double atol, int32_t max_it,
-            int32_t check_every, int32_t expected_its, AmgPtrs P, hipStream_t st, double* hst) {
+// The solve as nphase CG solves in a row (iterative refinement, as in
+// pcg.hip): every solve after the first starts from the current x with its
+// residual recomputed from the FP64 operator by the init kernel; solves before
+// the last stop at rtol_inner.  ivals: values of the w = A u operator.
+template <int BS, int WMAX, bool EXACT, typename VT, typename V, typename AV, bool COMP>
+int run_amg(int64_t n, int32_t W, int64_t ld, const int32_t* cols, const double* vals,
+            const AV* ivals, int32_t mode, const double* b, const double* b_part, int64_t nb_part,
+            int32_t project, int32_t raw_W, int64_t raw_ld, const int32_t* raw_cols,
+            const double* raw_vals, const int32_t* bpos, const double* gb, double* x_user,
+            int32_t warm, const int32_t* perm, double* x_perm, const smx_amg_level* lv, int nlev,
+            double rtol, double atol, int32_t max_it, int32_t check_every, int32_t expected_its,
+            int32_t nphase, double rtol_inner, int32_t expected_inner, AmgPtrs P, hipStream_t st,
+            double* hst) {
   const int Gf = P.Gf;
   const unsigned gs = (unsigned)((n + 255) / 256);
   double* x = perm ? x_perm : x_user;
@@ -553,71 +596,81 @@ int run_amg(int64_t n, int32_t W, int64_t ld, const int32_t* cols, const double*
     hipLaunchKernelGGL(amg_gather_kernel, dim3(gs), dim3(256), 0, st, n, perm, x_user, x);
     SMX_LAUNCH_CHECK();
   }
-  if (mode == 0) {
-    hipLaunchKernelGGL((amg_init_kernel<BS, WMAX>), dim3(Gf), dim3(BS), 0, st, n, mode, W, ld, cols,
-                       vals, b, b_part, nb_part, project, bpos, gb, x, warm, perm, P, n1);
-  } else {
-    hipLaunchKernelGGL((amg_init_kernel<BS, WMAX>), dim3(Gf), dim3(BS), 0, st, n, mode, raw_W,
-                       raw_ld, raw_cols, raw_vals, b, b_part, nb_part, project, bpos, gb, x, warm,
-                       perm, P, n1);
-  }
-  SMX_LAUNCH_CHECK();
-  int rc;
-  // u0 = M^-1 r0, w0 = A u0
-  if (nlev > 1) {
-    rc = launch_level<VT>(lv[0].R, P.r[0], nullptr, nullptr, lv[1].b, P.ss, lv[0].r_tpr, st);
-    if (rc) return rc;
-  }
-  rc = vcycle_tail<VT>(lv, nlev, P.r[0], P, st);
-  if (rc) return rc;
-  hipLaunchKernelGGL((amg_matvec_kernel<BS, WMAX, EXACT>), dim3(Gf), dim3(BS), 0, st, n, W, ld,
-                     cols, vals, P, P.r[0]);
-  SMX_LAUNCH_CHECK();
   const int rt = nlev > 1 ? lv[0].r_tpr : 1;
   const int64_t Gr = nlev > 1 ? (n1 + (BS / rt) - 1) / (BS / rt) : 0;
   SellOp R0 = {nullptr, nullptr, nullptr, 0};
   if (nlev > 1) R0 = to_op(lv[0].R);
   double* b1 = nlev > 1 ? lv[1].b : nullptr;
-  // check_every < 0: speculative launch of expected_its + 1 - check_every
-  // iterations with no wait (iterations past convergence return at once on
-  // the done flag); the caller reads the status block later
+  // check_every < 0: speculative launch of expected + 1 - check_every
+  // iterations per solve with no wait (iterations past convergence return at
+  // once on the done flag); the caller reads the status block later
   const bool spec = check_every < 0 && expected_its > 0;
-  const int32_t spec_batch = spec ? expected_its + 1 - check_every : 0;
-  if (check_every < 1) check_every = 8;
-  int32_t batch = spec ? spec_batch : (expected_its > 0 ? expected_its + 1 : check_every);
-  int32_t launched = 0;
-  bool done = false;
-  while (!done && launched <= max_it) {
-    const int32_t nb = batch < (max_it + 1 - launched) ? batch : (max_it + 1 - launched);
-    for (int32_t k = 0; k < nb; ++k) {
-      const int32_t it = launched + k;
-      const unsigned grid = (unsigned)(Gf + Gr);
-#define SMX_AMG_UPD(T)                                                                        \
-  hipLaunchKernelGGL((amg_update_kernel<BS, T, VT>), dim3(grid), dim3(BS), 0, st, n, P, x, R0, \
-                     b1, it, max_it, rtol, atol)
-      switch (rt) {
-        case 1: SMX_AMG_UPD(1); break;
-        case 2: SMX_AMG_UPD(2); break;
-        case 4: SMX_AMG_UPD(4); break;
-        case 8: SMX_AMG_UPD(8); break;
-        case 16: SMX_AMG_UPD(16); break;
-        case 32: SMX_AMG_UPD(32); break;
-        default: SMX_AMG_UPD(64); break;
-      }
-#undef SMX_AMG_UPD
-      SMX_LAUNCH_CHECK();
-      rc = vcycle_tail<VT>(lv, nlev, P.r[1 - (it & 1)], P, st);
-      if (rc) return rc;
-      hipLaunchKernelGGL((amg_matvec_kernel<BS, WMAX, EXACT>), dim3(Gf), dim3(BS), 0, st, n, W,
-                         ld, cols, vals, P, P.r[1 - (it & 1)]);
-      SMX_LAUNCH_CHECK();
+  const int32_t every = check_every < 1 ? 8 : check_every;
+  int rc;
+  for (int32_t phase = 0; phase < nphase; ++phase) {
+    const bool last = phase + 1 == nphase;
+    const int32_t wph = phase == 0 ? warm : 1;
+    // inner solve k reduces the residual to rtol_inner^(k+1) of ||b||
+    const double tol = last ? rtol : fmax(rtol, pow(rtol_inner, (double)(phase + 1)));
+    const int32_t expect = last ? expected_its : (expected_inner > 0 ? expected_inner : expected_its);
+    if (mode == 0) {
+      hipLaunchKernelGGL((amg_init_kernel<BS, WMAX, V>), dim3(Gf), dim3(BS), 0, st, n, mode, W, ld,
+                         cols, vals, b, b_part, nb_part, project, bpos, gb, x, wph, perm, P, n1,
+                         phase);
+    } else {
+      hipLaunchKernelGGL((amg_init_kernel<BS, WMAX, V>), dim3(Gf), dim3(BS), 0, st, n, mode, raw_W,
+                         raw_ld, raw_cols, raw_vals, b, b_part, nb_part, project, bpos, gb, x, wph,
+                         perm, P, n1, phase);
     }
-    launched += nb;
-    if (spec) break;
-    SMX_HIP_CHECK(hipMemcpyAsync(hst, P.ss, sizeof(double) * SS_SIZE, hipMemcpyDeviceToHost, st));
-    SMX_HIP_CHECK(hipStreamSynchronize(st));
-    done = hst[SS_DONE] != 0.0;
-    batch = check_every;
+    SMX_LAUNCH_CHECK();
+    // u0 = M^-1 r0, w0 = A u0
+    const V* r0 = vec<V>(P.r[0]);
+    if (nlev > 1) {
+      rc = launch_level<VT, V, double, double>(lv[0].R, r0, nullptr, nullptr, lv[1].b, P.ss,
+                                               lv[0].r_tpr, st);
+      if (rc) return rc;
+    }
+    rc = vcycle_tail<VT, V>(lv, nlev, r0, P, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL((amg_matvec_kernel<BS, WMAX, EXACT, V, AV, COMP>), dim3(Gf), dim3(BS), 0, st,
+                       n, W, ld, cols, ivals, P, r0);
+    SMX_LAUNCH_CHECK();
+    int32_t batch = spec ? expect + 1 - check_every : (expect > 0 ? expect + 1 : every);
+    int32_t launched = 0;
+    bool done = false;
+    while (!done && launched <= max_it) {
+      const int32_t nb = batch < (max_it + 1 - launched) ? batch : (max_it + 1 - launched);
+      for (int32_t k = 0; k < nb; ++k) {
+        const int32_t it = launched + k;
+        const unsigned grid = (unsigned)(Gf + Gr);
+#define SMX_AMG_UPD(T)                                                                               \
+  hipLaunchKernelGGL((amg_update_kernel<BS, T, VT, V, COMP>), dim3(grid), dim3(BS), 0, st, n, P, x, \
+                     R0, b1, it, max_it, tol, atol)
+        switch (rt) {
+          case 1: SMX_AMG_UPD(1); break;
+          case 2: SMX_AMG_UPD(2); break;
+          case 4: SMX_AMG_UPD(4); break;
+          case 8: SMX_AMG_UPD(8); break;
+          case 16: SMX_AMG_UPD(16); break;
+          case 32: SMX_AMG_UPD(32); break;
+          default: SMX_AMG_UPD(64); break;
+        }
+#undef SMX_AMG_UPD
+        SMX_LAUNCH_CHECK();
+        const V* rn = vec<V>(P.r[1 - (it & 1)]);
+        rc = vcycle_tail<VT, V>(lv, nlev, rn, P, st);
+        if (rc) return rc;
+        hipLaunchKernelGGL((amg_matvec_kernel<BS, WMAX, EXACT, V, AV, COMP>), dim3(Gf), dim3(BS), 0,
+                           st, n, W, ld, cols, ivals, P, rn);
+        SMX_LAUNCH_CHECK();
+      }
+      launched += nb;
+      if (spec) break;
+      SMX_HIP_CHECK(hipMemcpyAsync(hst, P.ss, sizeof(double) * SS_SIZE, hipMemcpyDeviceToHost, st));
+      SMX_HIP_CHECK(hipStreamSynchronize(st));
+      done = hst[SS_DONE] != 0.0;
+      batch = every;
+    }
   }
   if (perm) {
     hipLaunchKernelGGL(amg_scatter_kernel, dim3(gs), dim3(256), 0, st, n, perm, x, x_user);
@@ -666,7 +719,7 @@ int apply_once(const smx_amg_level* lv, int nlev, const double* r, double* u, do
     const int rc = launch_level<VT>(lv[0].R, r, nullptr, nullptr, lv[1].b, P.ss, lv[0].r_tpr, st);
     if (rc) return rc;
   }
-  return vcycle_tail<VT>(lv, nlev, r, P, st);
+  return vcycle_tail<VT, double>(lv, nlev, r, P, st);
 }
 
 }  // namespace
@@ -689,16 +742,16 @@ extern "C" int64_t smx_pcg_amg_ws_size(int64_t n) {
   return HDR + 7 * L.Gf + 9 * L.nvec + 64;
 }
 
-extern "C" int smx_pcg_amg(int64_t n, int32_t width, int64_t ld, const int32_t* cols,
-                           const double* vals, int32_t mode, const double* b,
-                           const double* b_part, int64_t nb_part, int32_t project,
-                           int32_t raw_width, int64_t raw_ld, const int32_t* raw_cols,
-                           const double* raw_vals, const int32_t* bpos, const double* gb,
-                           double* x, int32_t warm, const int32_t* perm,
-                           const smx_amg_level* levels, int32_t nlev, int32_t val_dtype,
-                           double rtol, double atol, int32_t max_it, int32_t check_every,
-                           int32_t expected_its, double* work, int32_t* its_out,
-                           double* rnorm_out, double* sums_out, void* stream) {
+namespace {
+
+int amg_entry(int64_t n, int32_t width, int64_t ld, const int32_t* cols, const double* vals,
+              int32_t mode, const double* b, const double* b_part, int64_t nb_part,
+              int32_t project, int32_t raw_width, int64_t raw_ld, const int32_t* raw_cols,
+              const double* raw_vals, const int32_t* bpos, const double* gb, double* x,
+              int32_t warm, const int32_t* perm, const smx_amg_level* levels, int32_t nlev,
+              int32_t val_dtype, double rtol, double atol, int32_t max_it, int32_t check_every,
+              int32_t expected_its, double* work, int32_t* its_out, double* rnorm_out,
+              double* sums_out, const smx_cg_mixed* mx, void* stream) {
   SMX_REQUIRE(n >= 0, SMX_ERR_SHAPE, "negative size");
   SMX_REQUIRE(mode == 0 || mode == 1, SMX_ERR_VALUE, "mode must be 0 or 1");
   SMX_REQUIRE(max_it >= 0, SMX_ERR_VALUE, "max_it must be >= 0");
@@ -710,6 +763,13 @@ extern "C" int smx_pcg_amg(int64_t n, int32_t width, int64_t ld, const int32_t* 
               "raw operator width must be in [1, 32]");
   SMX_REQUIRE(val_dtype == SMX_F64 || val_dtype == SMX_F32, SMX_ERR_TYPE,
               "AMG operator values must be FP64 or FP32");
+  if (mx) {
+    SMX_REQUIRE(mx->vals32, SMX_ERR_VALUE, "mixed precision needs the FP32 operator values");
+    SMX_REQUIRE(mx->refinements >= 0 && mx->refinements <= 8, SMX_ERR_VALUE,
+                "refinements must be in [0, 8]");
+    SMX_REQUIRE(mx->rtol_inner > 0.0 && mx->rtol_inner < 1.0, SMX_ERR_VALUE,
+                "rtol_inner must be in (0, 1)");
+  }
   if (its_out) *its_out = 0;
   if (rnorm_out) *rnorm_out = 0.0;
   if (n == 0) return SMX_OK;
@@ -750,20 +810,35 @@ extern "C" int smx_pcg_amg(int64_t n, int32_t width, int64_t ld, const int32_t* 
   const int bs = fine_bs(n);
   const int32_t wmax = (mode == 1 && raw_width > width) ? raw_width : width;
   const bool w7 = width == 7 && wmax == 7;
+  const int32_t nphase = mx ? 1 + mx->refinements : 1;
+  const double rin = mx ? mx->rtol_inner : rtol;
+  const int32_t ein = mx ? mx->expected_inner : 0;
   int rc;
-#define SMX_AMG_CALL(BS, WM, EX, VT)                                                            \
-  run_amg<BS, WM, EX, VT>(n, width, ld, cols, vals, mode, b, b_part, nb_part, project,          \
-                          raw_width, raw_ld, raw_cols, raw_vals, bpos, gb, x, warm, perm,       \
-                          x_perm, levels, nlev, rtol, atol, max_it, check_every, expected_its, \
-                          P, st, hst)
-#define SMX_AMG_W(BS, VT)                                                                \
-  (w7 ? SMX_AMG_CALL(BS, 7, true, VT)                                                    \
-      : wmax <= 8 ? SMX_AMG_CALL(BS, 8, false, VT)                                       \
-                  : wmax <= 16 ? SMX_AMG_CALL(BS, 16, false, VT) : SMX_AMG_CALL(BS, 32, false, VT))
-  if (val_dtype == SMX_F64) {
-    rc = bs == 256 ? SMX_AMG_W(256, double) : SMX_AMG_W(512, double);
+#define SMX_AMG_CALL(BS, WM, EX, VT, V, AV, CP, IV)                                              \
+  run_amg<BS, WM, EX, VT, V, AV, CP>(n, width, ld, cols, vals, IV, mode, b, b_part, nb_part,     \
+                                     project, raw_width, raw_ld, raw_cols, raw_vals, bpos, gb, x, \
+                                     warm, perm, x_perm, levels, nlev, rtol, atol, max_it,       \
+                                     check_every, expected_its, nphase, rin, ein, P, st, hst)
+#define SMX_AMG_W(BS, VT, V, AV, CP, IV)                                                          \
+  (w7 ? SMX_AMG_CALL(BS, 7, true, VT, V, AV, CP, IV)                                              \
+      : wmax <= 8 ? SMX_AMG_CALL(BS, 8, false, VT, V, AV, CP, IV)                                 \
+                  : wmax <= 16 ? SMX_AMG_CALL(BS, 16, false, VT, V, AV, CP, IV)                   \
+                               : SMX_AMG_CALL(BS, 32, false, VT, V, AV, CP, IV))
+  if (mx) {
+    const float* v32 = mx->vals32;
+    if (val_dtype == SMX_F64) {
+      rc = bs == 256 ? SMX_AMG_W(256, double, float, float, true, v32)
+                     : SMX_AMG_W(512, double, float, float, true, v32);
+    } else {
+      rc = bs == 256 ? SMX_AMG_W(256, float, float, float, true, v32)
+                     : SMX_AMG_W(512, float, float, float, true, v32);
+    }
+  } else if (val_dtype == SMX_F64) {
+    rc = bs == 256 ? SMX_AMG_W(256, double, double, double, false, vals)
+                   : SMX_AMG_W(512, double, double, double, false, vals);
   } else {
-    rc = bs == 256 ? SMX_AMG_W(256, float) : SMX_AMG_W(512, float);
+    rc = bs == 256 ? SMX_AMG_W(256, float, double, double, false, vals)
+                   : SMX_AMG_W(512, float, double, double, false, vals);
   }
 #undef SMX_AMG_W
 #undef SMX_AMG_CALL
@@ -772,7 +847,7 @@ extern "C" int smx_pcg_amg(int64_t n, int32_t width, int64_t ld, const int32_t* 
     if (its_out) *its_out = -1;
     return SMX_OK;
   }
-  if (its_out) *its_out = (int32_t)hst[SS_ITERS];
+  if (its_out) *its_out = (int32_t)(hst[SS_ITERS] + hst[SS_ITERS0]);
   if (rnorm_out) *rnorm_out = sqrt(hst[SS_RR]);
   if (sums_out) sums_out[0] = hst[SS_SUMX];
   if (hst[SS_FAIL] == 1.0) {
@@ -786,4 +861,40 @@ extern "C" int smx_pcg_amg(int64_t n, int32_t width, int64_t ld, const int32_t* 
     return SMX_ERR_NOT_CONVERGED;
   }
   return SMX_OK;
+}
+
+}  // namespace
+
+extern "C" int smx_pcg_amg(int64_t n, int32_t width, int64_t ld, const int32_t* cols,
+                           const double* vals, int32_t mode, const double* b,
+                           const double* b_part, int64_t nb_part, int32_t project,
+                           int32_t raw_width, int64_t raw_ld, const int32_t* raw_cols,
+                           const double* raw_vals, const int32_t* bpos, const double* gb,
+                           double* x, int32_t warm, const int32_t* perm,
+                           const smx_amg_level* levels, int32_t nlev, int32_t val_dtype,
+                           double rtol, double atol, int32_t max_it, int32_t check_every,
+                           int32_t expected_its, double* work, int32_t* its_out,
+                           double* rnorm_out, double* sums_out, void* stream) {
+  return amg_entry(n, width, ld, cols, vals, mode, b, b_part, nb_part, project, raw_width, raw_ld,
+                   raw_cols, raw_vals, bpos, gb, x, warm, perm, levels, nlev, val_dtype, rtol, atol,
+                   max_it, check_every, expected_its, work, its_out, rnorm_out, sums_out, nullptr,
+                   stream);
+}
+
+extern "C" int smx_pcg_amg_mixed(int64_t n, int32_t width, int64_t ld, const int32_t* cols,
+                                 const double* vals, int32_t mode, const double* b,
+                                 const double* b_part, int64_t nb_part, int32_t project,
+                                 int32_t raw_width, int64_t raw_ld, const int32_t* raw_cols,
+                                 const double* raw_vals, const int32_t* bpos, const double* gb,
+                                 double* x, int32_t warm, const int32_t* perm,
+                                 const smx_amg_level* levels, int32_t nlev, int32_t val_dtype,
+                                 double rtol, double atol, int32_t max_it, int32_t check_every,
+                                 int32_t expected_its, double* work, int32_t* its_out,
+                                 double* rnorm_out, double* sums_out, const smx_cg_mixed* mixed,
+                                 void* stream) {
+  SMX_REQUIRE(mixed, SMX_ERR_VALUE, "null mixed-precision options");
+  return amg_entry(n, width, ld, cols, vals, mode, b, b_part, nb_part, project, raw_width, raw_ld,
+                   raw_cols, raw_vals, bpos, gb, x, warm, perm, levels, nlev, val_dtype, rtol, atol,
+                   max_it, check_every, expected_its, work, its_out, rnorm_out, sums_out, mixed,
+                   stream);
 }

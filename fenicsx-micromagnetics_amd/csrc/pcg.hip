@@ -45,6 +45,16 @@
 //  - a stencil of width 7 is compiled with its exact width.
 // Gathering the neighbour values before the first reduction needs more than
 // 128 VGPRs, which halves the resident workgroups per CU and loses.
+//
+// Mixed precision (smx_pcg_jacobi_mixed): the iteration vectors r, w, s, p
+// and the iteration operator are FP32, halving the bytes an iteration
+// streams; x, the scalars and every inner product stay FP64.  A product of
+// two FP32 numbers is exact in FP64, and the sums are compensated (TwoSum in
+// the wave butterfly and across workgroups), so the inner products carry no
+// more than a few FP64 roundings.  Accuracy is recovered by iterative
+// refinement: after an inner solve to rtol_inner, a new solve starts from the
+// current x with its residual b - A x recomputed in FP64 from the FP64
+// operator (the init kernels), and the last solve runs to rtol.
 #include "smx_common.h"
 
 namespace {
@@ -58,6 +68,7 @@ enum {
   SS_BB = 4,
   SS_FAIL = 5,  // 1 breakdown, 2 max_it reached, 3 operator not in solver form
   SS_SUMX = 6,
+  SS_ITERS0 = 7,  // iterations of the inner solves before the last (mixed precision)
   SS_SIZE = 8,
   // scalars carried to the next launch: [gamma, alpha, 1/gamma, 1/alpha]
   ST_OFF = 8,
@@ -68,17 +79,24 @@ static_assert(SS_DONE == SMX_PCG_DONE && SS_ITERS == SMX_PCG_ITERS && SS_FAIL ==
                   SS_SUMX == SMX_PCG_SUMX,
               "status block layout of the C ABI");
 
+// Iteration vectors are FP64 or FP32 (the element type is a kernel template
+// parameter); x is always FP64.
 struct CGPtrs {
   double* ss;        // status record
   double* st[2];     // parity-buffered scalars
   double* part[2];   // parity-buffered partials, 4 x G
   double* pbb;       // 2 x G partials: ||b||^2 and rows whose slot 0 is not the diagonal
-  double* r[2];
-  double* w[2];
-  double* s[2];
-  double* p;
+  void* r[2];
+  void* w[2];
+  void* s[2];
+  void* p;
   int G;
 };
+
+template <typename V>
+__host__ __device__ __forceinline__ V* vec(void* p) {
+  return static_cast<V*>(p);
+}
 
 
 // Hardware dispatch places workgroup b on XCD b % 8.  Renumbering so that each
@@ -103,12 +121,15 @@ __device__ __forceinline__ T ldo(const T* __restrict__ base, uint32_t byte_off) 
 // same launch would need a second barrier.  ALL: every thread gets the totals
 // (else only thread 0).  The summation order is fixed, so every workgroup that
 // reduces the same values gets bit-identical totals.
-template <int BS, int NQ, bool ALL>
-__device__ __forceinline__ void block_total(double (&v)[NQ], double* scr) {
+// COMP: every addition compensated (TwoSum), v[q] + c[q] the per-thread input.
+template <int BS, int NQ, bool ALL, bool COMP = false>
+__device__ __forceinline__ void block_total(double (&v)[NQ], double* scr, const double* c = nullptr) {
   constexpr int nw = BS / 64;
   const int wid = threadIdx.x >> 6;
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) v[q] = smx::wave_total(v[q]);
+  for (int q = 0; q < NQ; ++q) {
+    v[q] = COMP ? smx::wave_total_comp(v[q], c ? c[q] : 0.0) : smx::wave_total(v[q]);
+  }
   if ((threadIdx.x & 63) == 0) {
 #pragma unroll
     for (int q = 0; q < NQ; ++q) scr[wid * NQ + q] = v[q];
@@ -117,33 +138,46 @@ __device__ __forceinline__ void block_total(double (&v)[NQ], double* scr) {
   if (ALL || threadIdx.x == 0) {
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-      double s = 0.0;
+      double s = 0.0, e = 0.0;
 #pragma unroll
-      for (int w = 0; w < nw; ++w) s += scr[w * NQ + q];
-      v[q] = s;
+      for (int w = 0; w < nw; ++w) {
+        if (COMP) {
+          smx::comp_add(s, e, scr[w * NQ + q]);
+        } else {
+          s += scr[w * NQ + q];
+        }
+      }
+      v[q] = COMP ? s + e : s;
     }
   }
 }
 
 // Sum of NQ x G partials (quantity q of block b at part[q*G + b]), identical
 // in every workgroup and every thread.
-template <int BS, int NQ>
+template <int BS, int NQ, bool COMP = false>
 __device__ __forceinline__ void reduce_partials(const double* __restrict__ part, int G,
                                                 double (&t)[NQ], double* scr) {
+  double c[NQ];
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) t[q] = 0.0;
+  for (int q = 0; q < NQ; ++q) t[q] = c[q] = 0.0;
   for (int b = threadIdx.x; b < G; b += BS) {
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) t[q] += part[q * G + b];
+    for (int q = 0; q < NQ; ++q) {
+      if (COMP) {
+        smx::comp_add(t[q], c[q], part[q * G + b]);
+      } else {
+        t[q] += part[q * G + b];
+      }
+    }
   }
-  block_total<BS, NQ, true>(t, scr);
+  block_total<BS, NQ, true, COMP>(t, scr, c);
 }
 
-template <int BS, int NQ>
+template <int BS, int NQ, bool COMP = false>
 __device__ __forceinline__ void publish_partials(double (&v)[NQ], double* __restrict__ part, int G,
                                                  double* scr, int blk = -1) {
   if (blk < 0) blk = blockIdx.x;
-  block_total<BS, NQ, false>(v, scr);
+  block_total<BS, NQ, false, COMP>(v, scr);
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int q = 0; q < NQ; ++q) part[q * G + blk] = v[q];
@@ -188,7 +222,7 @@ __global__ __launch_bounds__(256) void cg_prepare_kernel(int64_t n, int32_t W, i
 // r0 = b - A x0 (mode 0) or the lifted residual (mode 1); partials of ||b||^2
 // and of the rows whose slot 0 is not their diagonal (acols: the solver-form
 // operator, checked in both modes).
-template <int BS, int WMAX>
+template <int BS, int WMAX, typename V>
 __global__ __launch_bounds__(BS) void cg_init_a_kernel(
     int64_t n, int32_t mode, int32_t W, int64_t ld, const int32_t* __restrict__ cols,
     const double* __restrict__ vals, const int32_t* __restrict__ acols,
@@ -254,9 +288,9 @@ __global__ __launch_bounds__(BS) void cg_init_a_kernel(
         bk = -sg;
       }
     }
-    P.r[0][row] = r0;
-    P.s[0][row] = 0.0;
-    P.p[row] = 0.0;
+    vec<V>(P.r[0])[row] = (V)r0;
+    vec<V>(P.s[0])[row] = (V)0.0;
+    vec<V>(P.p)[row] = (V)0.0;
     v[0] = bk * bk;
     v[1] = acols[row] != (int32_t)row ? 1.0 : 0.0;
   }
@@ -277,14 +311,17 @@ __global__ void cg_scatter_kernel(int64_t n, const int32_t* __restrict__ perm, c
   if (i < n) x[perm[i]] = xp[i];
 }
 
-// w0 = A D^-1 r0; partials of (g0, d0, rr0, sum x0)
-template <int BS, int WMAX>
+// w0 = A D^-1 r0 with the iteration operator (values AV); partials of
+// (g0, d0, rr0, sum x0).  tol_rel: the relative tolerance of this solve;
+// phase > 0 (a refinement restart): the previous solves' iteration counts are
+// added to SS_ITERS0 before SS_ITERS restarts at 0.
+template <int BS, int WMAX, typename V, typename AV, bool COMP>
 __global__ __launch_bounds__(BS) void cg_init_b_kernel(int64_t n, int32_t W, int64_t ld,
                                                        const int32_t* __restrict__ cols,
-                                                       const double* __restrict__ vals,
+                                                       const AV* __restrict__ vals,
                                                        const double* __restrict__ dinv,
                                                        const double* __restrict__ x, CGPtrs P,
-                                                       double rtol, double atol) {
+                                                       double rtol, double atol, int32_t phase) {
   __shared__ double scr[BS / 64 * 2];
   __shared__ double scr_pub[BS / 64 * 4];
   double t[2];
@@ -293,6 +330,7 @@ __global__ __launch_bounds__(BS) void cg_init_b_kernel(int64_t n, int32_t W, int
     const double t1 = rtol * rtol * t[0], t2 = atol * atol;
     P.ss[SS_BB] = t[0];
     P.ss[SS_TOL2] = t1 > t2 ? t1 : t2;
+    P.ss[SS_ITERS0] = phase == 0 ? 0.0 : P.ss[SS_ITERS0] + P.ss[SS_ITERS];
     P.ss[SS_ITERS] = 0.0;
     P.ss[SS_DONE] = t[1] > 0.0 ? 1.0 : 0.0;  // a layout error stops before iteration 0
     P.ss[SS_FAIL] = t[1] > 0.0 ? 3.0 : 0.0;
@@ -300,28 +338,31 @@ __global__ __launch_bounds__(BS) void cg_init_b_kernel(int64_t n, int32_t W, int
   const int64_t row = (int64_t)blockIdx.x * BS + threadIdx.x;
   double v[4] = {0.0, 0.0, 0.0, 0.0};
   if (row < n) {
-    const double* r = P.r[0];
+    const V* r = vec<V>(P.r[0]);
     double acc = 0.0;
 #pragma unroll
     for (int j = 0; j < WMAX; ++j) {
-      if (j < W) acc = fma(vals[j * ld + row], r[cols[j * ld + row]], acc);
+      if (j < W) acc = fma((double)vals[j * ld + row], (double)r[cols[j * ld + row]], acc);
     }
-    P.w[0][row] = acc;
-    const double rk = r[row];
+    const V w0 = (V)acc;
+    vec<V>(P.w[0])[row] = w0;
+    const double rk = (double)r[row];
     const double uk = rk * dinv[row];
     v[0] = rk * uk;
-    v[1] = acc * uk;
+    v[1] = (double)w0 * uk;
     v[2] = rk * rk;
     v[3] = x[row];
   }
-  publish_partials<BS, 4>(v, P.part[0], P.G, scr_pub);
+  publish_partials<BS, 4, COMP>(v, P.part[0], P.G, scr_pub);
 }
 
 // One CG iteration.  EXACT: the ELL width is WMAX (no per-slot predicate).
-template <int BS, int WMAX, bool EXACT>
+// V: iteration vectors, AV: iteration operator values (FP64 or FP32); the
+// arithmetic is FP64 throughout.
+template <int BS, int WMAX, bool EXACT, typename V, typename AV, bool COMP>
 __global__ __launch_bounds__(BS) void cg_iter_kernel(int64_t n, int32_t W, int64_t ld,
                                                      const int32_t* __restrict__ cols,
-                                                     const double* __restrict__ vals,
+                                                     const AV* __restrict__ vals,
                                                      const double* __restrict__ dinv,
                                                      double* __restrict__ x, CGPtrs P, int32_t it,
                                                      int32_t max_it) {
@@ -330,29 +371,31 @@ __global__ __launch_bounds__(BS) void cg_iter_kernel(int64_t n, int32_t W, int64
   // Written by an earlier launch; tested once the loads below are in flight.
   const bool was_done = P.ss[SS_DONE] != 0.0;
   const int q = it & 1;
-  const double* __restrict__ rq = P.r[q];
-  const double* __restrict__ wq = P.w[q];
-  const double* __restrict__ sq = P.s[q];
+  const V* __restrict__ rq = vec<V>(P.r[q]);
+  const V* __restrict__ wq = vec<V>(P.w[q]);
+  const V* __restrict__ sq = vec<V>(P.s[q]);
   const int G = P.G;
   const double* __restrict__ part = P.part[q];
   const int b0 = (int)threadIdx.x < G ? (int)threadIdx.x : 0;
-  double t[4];
+  double t[4], tc[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int k = 0; k < 4; ++k) t[k] = part[k * G + b0];
   const int blk = xcd_block(G);
   const int64_t row = (int64_t)blk * BS + threadIdx.x;
   const int64_t rowc = row < n ? row : n - 1;
   const int Wk = EXACT ? WMAX : W;
+  constexpr uint32_t SV = sizeof(V), SA = sizeof(AV);
   const uint32_t o4 = (uint32_t)rowc * 4u, o8 = (uint32_t)rowc * 8u;
+  const uint32_t oa = (uint32_t)rowc * SA, ov = (uint32_t)rowc * SV;
   int32_t c[WMAX];  // byte offsets are formed after the reduction, where they are used
   double av[WMAX];
 #pragma unroll
   for (int j = 0; j < WMAX; ++j) {
     c[j] = j < Wk ? ldo(cols + j * ld, o4) : (int32_t)rowc;
-    av[j] = j < Wk ? ldo(vals + j * ld, o8) : 0.0;
+    av[j] = j < Wk ? (double)ldo(vals + j * ld, oa) : 0.0;
   }
   const double dk = ldo(dinv, o8);
-  const double p0 = ldo((const double*)P.p, o8);
+  const double p0 = (double)ldo((const V*)P.p, ov);
   const double x0 = ldo((const double*)x, o8);
   if ((int)threadIdx.x >= G) {
 #pragma unroll
@@ -360,9 +403,15 @@ __global__ __launch_bounds__(BS) void cg_iter_kernel(int64_t n, int32_t W, int64
   }
   for (int b = threadIdx.x + BS; b < G; b += BS) {  // only when G > BS
 #pragma unroll
-    for (int k = 0; k < 4; ++k) t[k] += part[k * G + b];
+    for (int k = 0; k < 4; ++k) {
+      if (COMP) {
+        smx::comp_add(t[k], tc[k], part[k * G + b]);
+      } else {
+        t[k] += part[k * G + b];
+      }
+    }
   }
-  block_total<BS, 4, true>(t, scr);
+  block_total<BS, 4, true, COMP>(t, scr, tc);
   const double gam = t[0], del = t[1], rr = t[2];
   const double tol2 = P.ss[SS_TOL2];
   double alpha, beta;
@@ -399,33 +448,39 @@ __global__ __launch_bounds__(BS) void cg_iter_kernel(int64_t n, int32_t W, int64
     double rj[WMAX], wj[WMAX], sj[WMAX];
 #pragma unroll
     for (int j = 0; j < WMAX; ++j) {
-      const uint32_t oc = (uint32_t)c[j] * 8u;
-      rj[j] = ldo(rq, oc);
-      wj[j] = ldo(wq, oc);
-      sj[j] = ldo(sq, oc);
+      const uint32_t oc = (uint32_t)c[j] * SV;
+      rj[j] = (double)ldo(rq, oc);
+      wj[j] = (double)ldo(wq, oc);
+      sj[j] = (double)ldo(sq, oc);
     }
+    // neighbour values of r_{i+1}, by the same explicit fmas and roundings as
+    // the row's own r_n below, so w_{i+1} is A D^-1 applied to the stored r
     double acc = 0.0;
 #pragma unroll
-    for (int j = 0; j < WMAX; ++j) acc = fma(av[j], rj[j] - alpha * (wj[j] + beta * sj[j]), acc);
+    for (int j = 0; j < WMAX; ++j) {
+      const double sn_j = (double)(V)fma(beta, sj[j], wj[j]);
+      acc = fma(av[j], (double)(V)fma(-alpha, sn_j, rj[j]), acc);
+    }
     const double r0 = rj[0], w0 = wj[0], s0 = sj[0];  // slot 0 is the row itself
-    const double s_n = w0 + beta * s0;
-    const double p_n = r0 * dk + beta * p0;
-    const double x_n = x0 + alpha * p_n;
-    const double r_n = r0 - alpha * s_n;
-    const double u_n = r_n * dk;
+    const V s_n = (V)fma(beta, s0, w0);
+    const V p_n = (V)fma(beta, p0, r0 * dk);
+    const double x_n = fma(alpha, (double)p_n, x0);
+    const V r_n = (V)fma(-alpha, (double)s_n, r0);
+    const V w_n = (V)acc;
+    const double u_n = (double)r_n * dk;
     // streaming stores: the next launch reads these from another XCD anyway,
     // and lines that bypass L2 need no write-back at the kernel boundary
-    __builtin_nontemporal_store(s_n, &P.s[1 - q][row]);
-    __builtin_nontemporal_store(r_n, &P.r[1 - q][row]);
-    __builtin_nontemporal_store(acc, &P.w[1 - q][row]);
-    __builtin_nontemporal_store(p_n, &P.p[row]);
+    __builtin_nontemporal_store(s_n, &vec<V>(P.s[1 - q])[row]);
+    __builtin_nontemporal_store(r_n, &vec<V>(P.r[1 - q])[row]);
+    __builtin_nontemporal_store(w_n, &vec<V>(P.w[1 - q])[row]);
+    __builtin_nontemporal_store(p_n, &vec<V>(P.p)[row]);
     __builtin_nontemporal_store(x_n, &x[row]);
-    v[0] = r_n * u_n;
-    v[1] = acc * u_n;
-    v[2] = r_n * r_n;
+    v[0] = (double)r_n * u_n;
+    v[1] = (double)w_n * u_n;
+    v[2] = (double)r_n * (double)r_n;
     v[3] = x_n;
   }
-  publish_partials<BS, 4>(v, P.part[1 - q], G, scr_pub, blk);
+  publish_partials<BS, 4, COMP>(v, P.part[1 - q], G, scr_pub, blk);
 }
 
 int block_size_for(int64_t n) { return n <= 65536 ? 256 : 512; }
@@ -446,14 +501,18 @@ Layout layout(int64_t n) {
   return L;
 }
 
-template <int BS, int WMAX, bool EXACT>
+// The solve as nphase CG solves in a row: solve 0 from x0 (warm) or 0, every
+// later one from the current x with its residual recomputed in FP64 by the
+// init kernels (iterative refinement); solves before the last stop at
+// rtol_inner, the last at rtol.  ivals: the iteration operator's values.
+template <int BS, int WMAX, bool EXACT, typename V, typename AV, bool COMP>
 int run_pcg(int64_t n, int32_t W, int64_t ld, const int32_t* cols, const double* vals,
-            const double* dinv, int32_t mode, const double* b, const double* b_part,
-            int64_t nb_part, int32_t project, int32_t raw_W, int64_t raw_ld,
+            const AV* ivals, const double* dinv, int32_t mode, const double* b,
+            const double* b_part, int64_t nb_part, int32_t project, int32_t raw_W, int64_t raw_ld,
             const int32_t* raw_cols, const double* raw_vals, const int32_t* bpos, const double* gb,
             double* x_user, int32_t warm, const int32_t* perm, double* x_perm, double rtol,
-            double atol, int32_t max_it, int32_t check_every, int32_t expected_its, CGPtrs P,
-            hipStream_t st, double* hst) {
+            double atol, int32_t max_it, int32_t check_every, int32_t expected_its, int32_t nphase,
+            double rtol_inner, int32_t expected_inner, CGPtrs P, hipStream_t st, double* hst) {
   const int G = P.G;
   const unsigned gs = (unsigned)((n + 255) / 256);
   double* x = perm ? x_perm : x_user;  // the kernels work in solver (permuted) row order
@@ -461,41 +520,47 @@ int run_pcg(int64_t n, int32_t W, int64_t ld, const int32_t* cols, const double*
     hipLaunchKernelGGL(cg_gather_kernel, dim3(gs), dim3(256), 0, st, n, perm, x_user, x);
     SMX_LAUNCH_CHECK();
   }
-  if (mode == 0) {
-    hipLaunchKernelGGL((cg_init_a_kernel<BS, WMAX>), dim3(G), dim3(BS), 0, st, n, mode, W, ld,
-                       cols, vals, cols, dinv, b, b_part, nb_part, project, bpos, gb, x, warm,
-                       perm, P);
-  } else {
-    hipLaunchKernelGGL((cg_init_a_kernel<BS, WMAX>), dim3(G), dim3(BS), 0, st, n, mode, raw_W,
-                       raw_ld, raw_cols, raw_vals, cols, dinv, b, b_part, nb_part, project, bpos,
-                       gb, x, warm, perm, P);
-  }
-  SMX_LAUNCH_CHECK();
-  hipLaunchKernelGGL((cg_init_b_kernel<BS, WMAX>), dim3(G), dim3(BS), 0, st, n, W, ld, cols, vals,
-                     dinv, x, P, rtol, atol);
-  SMX_LAUNCH_CHECK();
-  // check_every < 0: speculative launch of expected_its + 1 - check_every
-  // iterations with no wait (iterations past convergence return at once on
-  // the done flag); the caller reads the status block later
+  // check_every < 0: speculative launch of expected + 1 - check_every
+  // iterations per solve with no wait (iterations past convergence return at
+  // once on the done flag); the caller reads the status block later
   const bool spec = check_every < 0 && expected_its > 0;
-  const int32_t spec_batch = spec ? expected_its + 1 - check_every : 0;
-  if (check_every < 1) check_every = 16;
-  int32_t batch = spec ? spec_batch : (expected_its > 0 ? expected_its + 1 : check_every);
-  int32_t launched = 0;  // iterations launched: it = 0 .. launched-1
-  bool done = false;
-  while (!done && launched <= max_it) {
-    const int32_t nb = batch < (max_it + 1 - launched) ? batch : (max_it + 1 - launched);
-    for (int32_t k = 0; k < nb; ++k) {
-      hipLaunchKernelGGL((cg_iter_kernel<BS, WMAX, EXACT>), dim3(G), dim3(BS), 0, st, n, W, ld,
-                         cols, vals, dinv, x, P, launched + k, max_it);
+  const int32_t every = check_every < 1 ? 16 : check_every;
+  for (int32_t phase = 0; phase < nphase; ++phase) {
+    const bool last = phase + 1 == nphase;
+    const int32_t wph = phase == 0 ? warm : 1;
+    // inner solve k reduces the residual to rtol_inner^(k+1) of ||b||
+    const double tol = last ? rtol : fmax(rtol, pow(rtol_inner, (double)(phase + 1)));
+    const int32_t expect = last ? expected_its : (expected_inner > 0 ? expected_inner : expected_its);
+    if (mode == 0) {
+      hipLaunchKernelGGL((cg_init_a_kernel<BS, WMAX, V>), dim3(G), dim3(BS), 0, st, n, mode, W, ld,
+                         cols, vals, cols, dinv, b, b_part, nb_part, project, bpos, gb, x, wph,
+                         perm, P);
+    } else {
+      hipLaunchKernelGGL((cg_init_a_kernel<BS, WMAX, V>), dim3(G), dim3(BS), 0, st, n, mode, raw_W,
+                         raw_ld, raw_cols, raw_vals, cols, dinv, b, b_part, nb_part, project, bpos,
+                         gb, x, wph, perm, P);
     }
     SMX_LAUNCH_CHECK();
-    launched += nb;
-    if (spec) break;
-    SMX_HIP_CHECK(hipMemcpyAsync(hst, P.ss, sizeof(double) * SS_SIZE, hipMemcpyDeviceToHost, st));
-    SMX_HIP_CHECK(hipStreamSynchronize(st));
-    done = hst[SS_DONE] != 0.0;
-    batch = check_every;
+    hipLaunchKernelGGL((cg_init_b_kernel<BS, WMAX, V, AV, COMP>), dim3(G), dim3(BS), 0, st, n, W,
+                       ld, cols, ivals, dinv, x, P, tol, atol, phase);
+    SMX_LAUNCH_CHECK();
+    int32_t batch = spec ? expect + 1 - check_every : (expect > 0 ? expect + 1 : every);
+    int32_t launched = 0;  // iterations launched: it = 0 .. launched-1
+    bool done = false;
+    while (!done && launched <= max_it) {
+      const int32_t nb = batch < (max_it + 1 - launched) ? batch : (max_it + 1 - launched);
+      for (int32_t k = 0; k < nb; ++k) {
+        hipLaunchKernelGGL((cg_iter_kernel<BS, WMAX, EXACT, V, AV, COMP>), dim3(G), dim3(BS), 0,
+                           st, n, W, ld, cols, ivals, dinv, x, P, launched + k, max_it);
+      }
+      SMX_LAUNCH_CHECK();
+      launched += nb;
+      if (spec) break;
+      SMX_HIP_CHECK(hipMemcpyAsync(hst, P.ss, sizeof(double) * SS_SIZE, hipMemcpyDeviceToHost, st));
+      SMX_HIP_CHECK(hipStreamSynchronize(st));
+      done = hst[SS_DONE] != 0.0;
+      batch = every;
+    }
   }
   if (perm) {
     hipLaunchKernelGGL(cg_scatter_kernel, dim3(gs), dim3(256), 0, st, n, perm, x, x_user);
@@ -525,16 +590,16 @@ extern "C" int64_t smx_pcg_ws_size(int64_t n) {
   return HDR + 10 * L.G + 8 * L.nvec + 64;
 }
 
-extern "C" int smx_pcg_jacobi(int64_t n, int32_t width, int64_t ld, const int32_t* cols,
-                              const double* vals, const double* dinv, int32_t mode,
-                              const double* b, const double* b_part, int64_t nb_part,
-                              int32_t project, int32_t raw_width, int64_t raw_ld,
-                              const int32_t* raw_cols, const double* raw_vals,
-                              const int32_t* bpos, const double* gb, double* x, int32_t warm,
-                              const int32_t* perm, double rtol, double atol, int32_t max_it,
-                              int32_t check_every,
-                              int32_t expected_its, double* work, int32_t* its_out,
-                              double* rnorm_out, double* sums_out, void* stream) {
+namespace {
+
+int pcg_entry(int64_t n, int32_t width, int64_t ld, const int32_t* cols, const double* vals,
+              const double* dinv, int32_t mode, const double* b, const double* b_part,
+              int64_t nb_part, int32_t project, int32_t raw_width, int64_t raw_ld,
+              const int32_t* raw_cols, const double* raw_vals, const int32_t* bpos,
+              const double* gb, double* x, int32_t warm, const int32_t* perm, double rtol,
+              double atol, int32_t max_it, int32_t check_every, int32_t expected_its,
+              double* work, int32_t* its_out, double* rnorm_out, double* sums_out,
+              const smx_cg_mixed* mx, void* stream) {
   SMX_REQUIRE(n >= 0, SMX_ERR_SHAPE, "negative size");
   SMX_REQUIRE(mode == 0 || mode == 1, SMX_ERR_VALUE, "mode must be 0 or 1");
   SMX_REQUIRE(max_it >= 0, SMX_ERR_VALUE, "max_it must be >= 0");
@@ -546,6 +611,13 @@ extern "C" int smx_pcg_jacobi(int64_t n, int32_t width, int64_t ld, const int32_
               "raw operator width must be in [1, 32]");
   SMX_REQUIRE(n < (int64_t)1 << 28, SMX_ERR_SHAPE,
               "CG operator too large for 32-bit byte offsets (n %lld)", (long long)n);
+  if (mx) {
+    SMX_REQUIRE(mx->vals32, SMX_ERR_VALUE, "mixed precision needs the FP32 operator values");
+    SMX_REQUIRE(mx->refinements >= 0 && mx->refinements <= 8, SMX_ERR_VALUE,
+                "refinements must be in [0, 8]");
+    SMX_REQUIRE(mx->rtol_inner > 0.0 && mx->rtol_inner < 1.0, SMX_ERR_VALUE,
+                "rtol_inner must be in (0, 1)");
+  }
   if (its_out) *its_out = 0;
   if (rnorm_out) *rnorm_out = 0.0;
   if (n == 0) return SMX_OK;
@@ -575,29 +647,38 @@ extern "C" int smx_pcg_jacobi(int64_t n, int32_t width, int64_t ld, const int32_
   double* x_perm = v0 + 7 * Lo.nvec;
   double hst[SS_SIZE] = {0};
   const int bs = block_size_for(n);
+  const int32_t nphase = mx ? 1 + mx->refinements : 1;
+  const double rin = mx ? mx->rtol_inner : rtol;
+  const int32_t ein = mx ? mx->expected_inner : 0;
   int rc;
-#define SMX_PCG_CALL(BS, WM, EX)                                                                 \
-  run_pcg<BS, WM, EX>(n, width, ld, cols, vals, dinv, mode, b, b_part, nb_part, project,        \
-                      raw_width, raw_ld, raw_cols, raw_vals, bpos, gb, x, warm, perm, x_perm,   \
-                      rtol, atol, max_it, check_every, expected_its, P, st, hst)
+#define SMX_PCG_CALL(BS, WM, EX, V, AV, CP, IV)                                                   \
+  run_pcg<BS, WM, EX, V, AV, CP>(n, width, ld, cols, vals, IV, dinv, mode, b, b_part, nb_part,    \
+                                 project, raw_width, raw_ld, raw_cols, raw_vals, bpos, gb, x, warm, \
+                                 perm, x_perm, rtol, atol, max_it, check_every, expected_its,     \
+                                 nphase, rin, ein, P, st, hst)
+#define SMX_PCG_WIDTH(BS, V, AV, CP, IV)                                                          \
+  (w7 ? SMX_PCG_CALL(BS, 7, true, V, AV, CP, IV)                                                  \
+      : wmax <= 8 ? SMX_PCG_CALL(BS, 8, false, V, AV, CP, IV)                                     \
+                  : wmax <= 16 ? SMX_PCG_CALL(BS, 16, false, V, AV, CP, IV)                       \
+                               : SMX_PCG_CALL(BS, 32, false, V, AV, CP, IV))
   const int32_t wmax = (mode == 1 && raw_width > width) ? raw_width : width;
   const bool w7 = width == 7 && wmax == 7;  // the tetrahedral Kuhn-grid stencil
-  if (bs == 256) {
-    rc = w7 ? SMX_PCG_CALL(256, 7, true)
-            : wmax <= 8 ? SMX_PCG_CALL(256, 8, false)
-                        : wmax <= 16 ? SMX_PCG_CALL(256, 16, false) : SMX_PCG_CALL(256, 32, false);
+  if (mx) {
+    const float* v32 = mx->vals32;
+    rc = bs == 256 ? SMX_PCG_WIDTH(256, float, float, true, v32)
+                   : SMX_PCG_WIDTH(512, float, float, true, v32);
   } else {
-    rc = w7 ? SMX_PCG_CALL(512, 7, true)
-            : wmax <= 8 ? SMX_PCG_CALL(512, 8, false)
-                        : wmax <= 16 ? SMX_PCG_CALL(512, 16, false) : SMX_PCG_CALL(512, 32, false);
+    rc = bs == 256 ? SMX_PCG_WIDTH(256, double, double, false, vals)
+                   : SMX_PCG_WIDTH(512, double, double, false, vals);
   }
+#undef SMX_PCG_WIDTH
 #undef SMX_PCG_CALL
   if (rc != SMX_OK) return rc;
   if (check_every < 0) {  // speculative: nothing waited for, the status stays on the device
     if (its_out) *its_out = -1;
     return SMX_OK;
   }
-  if (its_out) *its_out = (int32_t)hst[SS_ITERS];
+  if (its_out) *its_out = (int32_t)(hst[SS_ITERS] + hst[SS_ITERS0]);
   if (rnorm_out) *rnorm_out = sqrt(hst[SS_RR]);
   if (sums_out) sums_out[0] = hst[SS_SUMX];
   if (hst[SS_FAIL] == 3.0) {
@@ -616,4 +697,37 @@ extern "C" int smx_pcg_jacobi(int64_t n, int32_t width, int64_t ld, const int32_
     return SMX_ERR_NOT_CONVERGED;
   }
   return SMX_OK;
+}
+
+}  // namespace
+
+extern "C" int smx_pcg_jacobi(int64_t n, int32_t width, int64_t ld, const int32_t* cols,
+                              const double* vals, const double* dinv, int32_t mode,
+                              const double* b, const double* b_part, int64_t nb_part,
+                              int32_t project, int32_t raw_width, int64_t raw_ld,
+                              const int32_t* raw_cols, const double* raw_vals,
+                              const int32_t* bpos, const double* gb, double* x, int32_t warm,
+                              const int32_t* perm, double rtol, double atol, int32_t max_it,
+                              int32_t check_every,
+                              int32_t expected_its, double* work, int32_t* its_out,
+                              double* rnorm_out, double* sums_out, void* stream) {
+  return pcg_entry(n, width, ld, cols, vals, dinv, mode, b, b_part, nb_part, project, raw_width,
+                   raw_ld, raw_cols, raw_vals, bpos, gb, x, warm, perm, rtol, atol, max_it,
+                   check_every, expected_its, work, its_out, rnorm_out, sums_out, nullptr, stream);
+}
+
+extern "C" int smx_pcg_jacobi_mixed(int64_t n, int32_t width, int64_t ld, const int32_t* cols,
+                                    const double* vals, const double* dinv, int32_t mode,
+                                    const double* b, const double* b_part, int64_t nb_part,
+                                    int32_t project, int32_t raw_width, int64_t raw_ld,
+                                    const int32_t* raw_cols, const double* raw_vals,
+                                    const int32_t* bpos, const double* gb, double* x,
+                                    int32_t warm, const int32_t* perm, double rtol, double atol,
+                                    int32_t max_it, int32_t check_every, int32_t expected_its,
+                                    double* work, int32_t* its_out, double* rnorm_out,
+                                    double* sums_out, const smx_cg_mixed* mixed, void* stream) {
+  SMX_REQUIRE(mixed, SMX_ERR_VALUE, "null mixed-precision options");
+  return pcg_entry(n, width, ld, cols, vals, dinv, mode, b, b_part, nb_part, project, raw_width,
+                   raw_ld, raw_cols, raw_vals, bpos, gb, x, warm, perm, rtol, atol, max_it,
+                   check_every, expected_its, work, its_out, rnorm_out, sums_out, mixed, stream);
 }

@@ -91,6 +91,49 @@ __device__ __forceinline__ double wave_total(double v) {
   return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
 }
 
+// Error-free sum (Knuth's TwoSum): s + e == a + b exactly, s = fl(a + b).
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+  s = a + b;
+  const double bv = s - a;
+  e = (a - (s - bv)) + (b - bv);
+}
+
+// Compensated accumulation of v into (s, c): s + c carries the running sum
+// with the rounding error of every addition kept in c (Neumaier's variant).
+__device__ __forceinline__ void comp_add(double& s, double& c, double v) {
+  double t, e;
+  two_sum(s, v, t, e);
+  s = t;
+  c += e;
+}
+
+// wave_total with every addition compensated: the lanes' (sum, correction)
+// pairs travel through the same DPP butterfly, and the result is the rounded
+// s + c, identical in every lane.
+template <int CTRL>
+__device__ __forceinline__ void dpp_comp_step(double& s, double& c) {
+  const double bs = dpp_move<CTRL>(s), bc = dpp_move<CTRL>(c);
+  double t, e;
+  two_sum(s, bs, t, e);
+  s = t;
+  c = c + bc + e;
+}
+
+__device__ __forceinline__ double wave_total_comp(double v, double c = 0.0) {
+  double s = v;
+  dpp_comp_step<0xB1>(s, c);
+  dpp_comp_step<0x4E>(s, c);
+  dpp_comp_step<0x141>(s, c);
+  dpp_comp_step<0x140>(s, c);
+  double S = read_lane(s, 0), C = read_lane(c, 0);
+#pragma unroll
+  for (int l = 16; l < 64; l += 16) {
+    comp_add(S, C, read_lane(s, l));
+    C += read_lane(c, l);
+  }
+  return S + C;
+}
+
 // Workgroup barrier that orders LDS accesses only: outstanding global loads
 // and stores of the calling wave are not waited for.
 __device__ __forceinline__ void lds_barrier() {

@@ -19,7 +19,7 @@ SMX_OK, SMX_ERR_SHAPE, SMX_ERR_VALUE, SMX_ERR_NOT_CONVERGED, SMX_ERR_OOM, SMX_ER
 SMX_F64, SMX_F32, SMX_BF16 = 0, 1, 2
 TRI_FIELDS = 34
 LINDHOLM_TILE_COLS = 1024
-PCG_DONE, PCG_ITERS, PCG_FAIL, PCG_SUMX = 0, 1, 5, 6
+PCG_DONE, PCG_ITERS, PCG_FAIL, PCG_SUMX, PCG_ITERS0 = 0, 1, 5, 6, 7
 HMAT_Z_GROUP = 8
 HMAT_KW_MAX = 16384
 
@@ -64,6 +64,11 @@ class FieldArgs(C.Structure):
     ]
 
 
+class CGMixed(C.Structure):
+    """smx_cg_mixed: FP32 inner CG with FP64 iterative refinement."""
+    _fields_ = [("vals32", P), ("rtol_inner", F64), ("refinements", I32), ("expected_inner", I32)]
+
+
 class NearTilesArgs(C.Structure):
     """smx_hmat_near_tiles."""
     _fields_ = [("ntile", I64), ("tile", P), ("row", P), ("col", P), ("val", P), ("iperm", P),
@@ -105,10 +110,16 @@ _SIGS = {
     "smx_pcg_jacobi": ([I64, I32, I64, P, P, P, I32, P, P, I64, I32, I32, I64, P, P, P, P, P, I32, P, F64,
                         F64, I32, I32, I32, P, C.POINTER(I32), C.POINTER(F64), C.POINTER(F64), P],
                        C.c_int),
+    "smx_pcg_jacobi_mixed": ([I64, I32, I64, P, P, P, I32, P, P, I64, I32, I32, I64, P, P, P, P, P, I32, P,
+                              F64, F64, I32, I32, I32, P, C.POINTER(I32), C.POINTER(F64), C.POINTER(F64),
+                              C.POINTER(CGMixed), P], C.c_int),
     "smx_pcg_amg_ws_size": ([I64], I64),
     "smx_amg_apply": ([P, I32, I32, P, P, P, P], C.c_int),
     "smx_pcg_amg": ([I64, I32, I64, P, P, I32, P, P, I64, I32, I32, I64, P, P, P, P, P, I32, P, P, I32, I32,
                      F64, F64, I32, I32, I32, P, C.POINTER(I32), C.POINTER(F64), C.POINTER(F64), P], C.c_int),
+    "smx_pcg_amg_mixed": ([I64, I32, I64, P, P, I32, P, P, I64, I32, I32, I64, P, P, P, P, P, I32, P, P, I32,
+                           I32, F64, F64, I32, I32, I32, P, C.POINTER(I32), C.POINTER(F64), C.POINTER(F64),
+                           C.POINTER(CGMixed), P], C.c_int),
     "smx_field_llg_rhs": ([C.POINTER(FieldArgs), P], C.c_int),
     "smx_llg_jvp": ([C.POINTER(FieldArgs), P, P, P, F64, P], C.c_int),
     "smx_dot3_weighted": ([I64, P, P, P, P, P, P], C.c_int),

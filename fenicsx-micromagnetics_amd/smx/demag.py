@@ -150,14 +150,19 @@ class DemagFieldLindholmGPU:
                  hmatrix_near_fp32=False, hmatrix_near_in_z="auto", cg_reorder=True, cg_speculative=True,
                  ksp_pc_type="gamg",
                  ksp_pc_type_u2=None,
-                 amg_fp32=None, amg_max_coarse=1024, amg_theta=0.0, verbose=False):
+                 amg_fp32=None, amg_max_coarse=1024, amg_theta=0.0, cg_mixed=False, cg_refinements=1,
+                 cg_rtol_inner=None, verbose=False):
         """Keywords as the reference's constructor (Demag_Lindholm_GPU.py:101-155),
         plus, keyword-only: the solver settings the reference hard-codes
         (ksp_atol, max_it), ``ksp_pc_type`` / ``ksp_pc_type_u2`` in {"gamg",
         "jacobi", "auto"} for the u1 / u2 preconditioner (u2 defaults to u1's;
         "auto" times both on a fixed right side at set-up and keeps the faster),
         ``amg_fp32`` (FP32 AMG operator values; default: FP32 when the boundary
-        operator is stored below FP64), and prebuilt geometry / operators."""
+        operator is stored below FP64), ``cg_mixed`` (the mixed-precision
+        Poisson solves: FP32 inner CG, FP64 iterative refinement; off unless
+        asked for) with ``cg_refinements`` FP64 residual restarts, inner solve
+        k stopping at ``cg_rtol_inner``^(k+1) of ||b|| (default: the factor
+        rtol^(1/(refinements+1)) per solve), and prebuilt geometry / operators."""
         if not isinstance(mesh, TetMesh):
             raise TypeError("mesh must be a smx.mesh.TetMesh")
         self.mesh = mesh
@@ -295,6 +300,16 @@ class DemagFieldLindholmGPU:
             else:
                 amg_fp32 = bool(hmatrix_use_fp32)
         self.amg_fp32 = bool(amg_fp32)
+        # mixed-precision Poisson solves (FP32 inner CG, FP64 refinement)
+        self.cg_mixed = bool(cg_mixed)
+        self.cg_refinements = int(cg_refinements)
+        self.cg_rtol_inner = None if cg_rtol_inner is None else float(cg_rtol_inner)
+        if self.cg_mixed:
+            for c in self._cg.values():
+                c["plain32"] = c["plain"].vals.float()
+                c["jacobi32"] = c["jacobi"].vals.float() if c["jacobi"] is not None else None
+        # iteration counts of the last solves: (earlier inner solves, last solve)
+        self._its_split = {"u1": (0, 1), "u2": (0, 1)}
         A1h, A2h, mc, th = self._amg_host
         del self._amg_host
         t1 = time.perf_counter()
@@ -428,31 +443,51 @@ class DemagFieldLindholmGPU:
         sums = (C.c_double * 1)()
         raw = self.A_raw
         nbp = 0 if b_part is None else b_part.numel()
-        if c["pc"] == "gamg":
+        amg_pc = c["pc"] == "gamg"
+        mixed = None
+        if self.cg_mixed:
+            v32 = c["plain32"] if amg_pc else c["jacobi32"]
+            # default: every solve gains the same factor, rtol^(1 / (refinements + 1))
+            rin = self.cg_rtol_inner if self.cg_rtol_inner is not None else \
+                float(rtol) ** (1.0 / (self.cg_refinements + 1))
+            mixed = _capi.CGMixed(v32.data_ptr(), min(max(rin, 1e-300), 0.5), self.cg_refinements,
+                                  int(self._its_split[op][0]))
+        if amg_pc:
             A, amg = c["plain"], c["amg"]
-            rc = L.smx_pcg_amg(self.N, A.width, A.ld, _capi.ptr(A.cols), _capi.ptr(A.vals), mode,
-                               _capi.ptr(b), _capi.ptr(b_part), nbp, int(project), raw.width, raw.ld,
-                               _capi.ptr(raw.cols), _capi.ptr(raw.vals), _capi.ptr(self.bpos_cg),
-                               _capi.ptr(gb), _capi.ptr(x), int(warm), _capi.ptr(self.cg_perm),
-                               C.addressof(amg.c_levels), amg.nlev, amg.val_dtype, float(rtol),
-                               float(self.ksp_atol), int(max_it), -1 if spec else 8, int(expected),
-                               _capi.ptr(ws), C.byref(its), C.byref(rn), sums, _capi.stream_ptr())
-            what = "smx_pcg_amg"
+            args = [self.N, A.width, A.ld, _capi.ptr(A.cols), _capi.ptr(A.vals), mode,
+                    _capi.ptr(b), _capi.ptr(b_part), nbp, int(project), raw.width, raw.ld,
+                    _capi.ptr(raw.cols), _capi.ptr(raw.vals), _capi.ptr(self.bpos_cg),
+                    _capi.ptr(gb), _capi.ptr(x), int(warm), _capi.ptr(self.cg_perm),
+                    C.addressof(amg.c_levels), amg.nlev, amg.val_dtype, float(rtol),
+                    float(self.ksp_atol), int(max_it), -1 if spec else 8, int(expected),
+                    _capi.ptr(ws), C.byref(its), C.byref(rn), sums]
+            if mixed is None:
+                rc, what = L.smx_pcg_amg(*args, _capi.stream_ptr()), "smx_pcg_amg"
+            else:
+                rc, what = L.smx_pcg_amg_mixed(*args, C.byref(mixed), _capi.stream_ptr()), "smx_pcg_amg_mixed"
         else:
             A = c["jacobi"]
-            rc = L.smx_pcg_jacobi(self.N, A.width, A.ld, _capi.ptr(A.cols), _capi.ptr(A.vals),
-                                  _capi.ptr(c["dinv"]), mode, _capi.ptr(b), _capi.ptr(b_part), nbp,
-                                  int(project), raw.width, raw.ld, _capi.ptr(raw.cols), _capi.ptr(raw.vals),
-                                  _capi.ptr(self.bpos_cg), _capi.ptr(gb), _capi.ptr(x), int(warm),
-                                  _capi.ptr(self.cg_perm), float(rtol), float(self.ksp_atol), int(max_it),
-                                  -2 if spec else 16, int(expected), _capi.ptr(ws), C.byref(its), C.byref(rn), sums,
-                                  _capi.stream_ptr())
-            what = "smx_pcg_jacobi"
+            args = [self.N, A.width, A.ld, _capi.ptr(A.cols), _capi.ptr(A.vals),
+                    _capi.ptr(c["dinv"]), mode, _capi.ptr(b), _capi.ptr(b_part), nbp,
+                    int(project), raw.width, raw.ld, _capi.ptr(raw.cols), _capi.ptr(raw.vals),
+                    _capi.ptr(self.bpos_cg), _capi.ptr(gb), _capi.ptr(x), int(warm),
+                    _capi.ptr(self.cg_perm), float(rtol), float(self.ksp_atol), int(max_it),
+                    -2 if spec else 16, int(expected), _capi.ptr(ws), C.byref(its), C.byref(rn), sums]
+            if mixed is None:
+                rc, what = L.smx_pcg_jacobi(*args, _capi.stream_ptr()), "smx_pcg_jacobi"
+            else:
+                rc, what = L.smx_pcg_jacobi_mixed(*args, C.byref(mixed), _capi.stream_ptr()), "smx_pcg_jacobi_mixed"
         if rc == _capi.SMX_ERR_NOT_CONVERGED:
             # The reference never checks the KSP converged reason; keep going.
             self.converged = False
         else:
             _capi.check(rc, what)
+        if its.value >= 0 and mixed is not None:
+            # the waiting form synchronized the stream: split the count by solve
+            st = ws[:8].cpu().numpy()
+            self._its_split[op] = (int(st[_capi.PCG_ITERS0]), int(st[_capi.PCG_ITERS]))
+        elif its.value >= 0:
+            self._its_split[op] = (0, int(its.value))
         return int(its.value)
 
     def boundary_matvec(self, src: torch.Tensor, y: torch.Tensor, gather=None, shift_ptr=None,
@@ -484,11 +519,11 @@ class DemagFieldLindholmGPU:
         warm = self.warm_start and self._have_u
         self.Div.spmv(m_vec, self.b_u1, kind=1, partials=self.b_part)
         its1 = self.solve("u1", 0, self.u1, self.ws1, b=self.b_u1, b_part=self.b_part,
-                          expected=self.last_its_u1, warm=warm, speculative=speculative)
+                          expected=self._its_split["u1"][1], warm=warm, speculative=speculative)
         self.boundary_matvec(self.u1, self.y_rows, gather=self.bverts_gpu,
                              shift_ptr=self.u1_sum_ptr, shift_scale=1.0 / self.N)
         its2 = self.solve("u2", 1, self.u2, self.ws2, gb=self.y_rows, project=0,
-                          expected=self.last_its_u2, warm=warm, speculative=speculative)
+                          expected=self._its_split["u2"][1], warm=warm, speculative=speculative)
         self._have_u = True
         if its1 < 0 or its2 < 0:
             self._status_host[:8].copy_(self.ws1[:8], non_blocking=True)
@@ -517,7 +552,9 @@ class DemagFieldLindholmGPU:
                 continue
             if fail != 0.0:
                 self.converged = False      # the reference never checks KSP convergence: keep going
-            setattr(self, "last_its_" + name, int(its))
+            its0 = st[off + _capi.PCG_ITERS0] if self.cg_mixed else 0.0
+            self._its_split[name] = (int(its0), int(its))
+            setattr(self, "last_its_" + name, int(its0 + its))
         if redo:
             self.speculative_redos += 1
             self.solve_potential(m_vec, speculative=False)

@@ -13,8 +13,9 @@
  *   - Functions are asynchronous on `stream` (a hipStream_t passed as void*),
  *     allocate no device memory (workspaces are caller-provided) and return a
  *     smx_status.  smx_last_error() describes the last failure of the thread.
- *     One exception: the waiting form of the CG solvers (smx_pcg_jacobi and
- *     smx_pcg_amg with check_every >= 0) blocks the calling host thread on
+ *     One exception: the waiting form of the CG solvers (smx_pcg_jacobi,
+ *     smx_pcg_amg and their _mixed forms with check_every >= 0) blocks the
+ *     calling host thread on
  *     hipStreamSynchronize(stream) between batches of iterations, because it
  *     returns the iteration count and residual in host memory; their
  *     speculative form (check_every < 0) returns without waiting.
@@ -366,6 +367,45 @@ int smx_pcg_jacobi(int64_t n, int32_t width, int64_t ld, const int32_t* cols,
                    const int32_t* perm, double rtol, double atol, int32_t max_it,
                    int32_t check_every, int32_t expected_its, double* work,
                    int32_t* its_out, double* rnorm_out, double* sums_out, void* stream);
+
+/* Mixed-precision Poisson solve: FP32 inner CG, FP64 iterative refinement.
+ * The same solve as smx_pcg_jacobi / smx_pcg_amg (same operators, tolerance
+ * test, modes, workspace and status block), with
+ *   - the iteration vectors (r, w, s, p, and u for the AMG form) in FP32,
+ *     and the iteration operator read from `vals32`: FP32 values in the
+ *     layout of `vals` (the solver form for the Jacobi solver, the plain A
+ *     for the AMG solver);
+ *   - x, the CG scalars and the inner products in FP64: each product of two
+ *     FP32 values is exact in FP64 and the sums are compensated (TwoSum);
+ *   - `refinements` restarts: inner solve k (k = 0 .. refinements-1) stops
+ *     at a residual of max(rtol, rtol_inner^(k+1)) * ||b||, and the next
+ *     starts from the current x with the residual b - A x recomputed from
+ *     the FP64 operator; the last solve stops at rtol.  refinements = 0 is
+ *     a single FP32-vector solve.
+ * Speculative launches (check_every < 0) enqueue expected_inner + 1 -
+ * check_every iterations for each inner solve before the last and
+ * expected_its + 1 - check_every for the last.  The status block then holds
+ * the last solve's count at SMX_PCG_ITERS and the earlier solves' total at
+ * SMX_PCG_ITERS0; its_out (waiting form) is their sum.  This is the
+ * mixed-precision form of the reference's KSP cg solves
+ * (Demag_Lindholm_GPU.py:216-226, 271-281, 569-640), which run in FP64. */
+typedef struct {
+  const float* vals32;      /* FP32 iteration operator values */
+  double rtol_inner;        /* residual reduction of each inner solve but the last, (0, 1) */
+  int32_t refinements;      /* FP64 residual restarts, 0..8 */
+  int32_t expected_inner;   /* speculative: iterations expected of the earlier solves */
+} smx_cg_mixed;
+#define SMX_PCG_ITERS0 7
+int smx_pcg_jacobi_mixed(int64_t n, int32_t width, int64_t ld, const int32_t* cols,
+                         const double* vals, const double* dinv, int32_t mode,
+                         const double* b, const double* b_part, int64_t nb_part,
+                         int32_t project, int32_t raw_width, int64_t raw_ld,
+                         const int32_t* raw_cols, const double* raw_vals,
+                         const int32_t* bpos, const double* gb, double* x, int32_t warm,
+                         const int32_t* perm, double rtol, double atol, int32_t max_it,
+                         int32_t check_every, int32_t expected_its, double* work,
+                         int32_t* its_out, double* rnorm_out, double* sums_out,
+                         const smx_cg_mixed* mixed, void* stream);
 
 /* ---------------------------------------------------------------------- *
  * (c') CG preconditioned by one smoothed-aggregation AMG V-cycle.  Replaces

@@ -17,8 +17,9 @@ workload, on the device, against the oracle.
   H-matrix.
 * Demag (FP64 factors): the uniform-m patch test (u1 = Ms z + const, so the
   u1 part of H is -Ms m) and constant-shift invariance of u1 + u2.
-* Mixed (FP32 factors, FP32 AMG values) against FP64 on the film material:
-  H_demag and the LLG RHS within 1e-6 relative L2.
+* Mixed (FP32 factors, FP32 inner CG with one FP64 refinement restart, FP32
+  AMG values) against FP64 on the film material: H_demag and the LLG RHS
+  within 1e-6 relative L2.
 """
 import math
 
@@ -252,8 +253,8 @@ def test_film_mixed_vs_fp64(film):
               ksp_rtol_u1=1e-6, ksp_rtol_u2=1e-6, ksp_atol=1e-12)
     h64 = EffectiveField(m, use_demag=True, operators=film["ops"], demag_kwargs=dict(hmatrix_use_fp32=False, **hk),
                          **mat)
-    hmx = EffectiveField(m, use_demag=True, operators=film["ops"], demag_kwargs=dict(hmatrix_use_fp32=True, **hk),
-                         **mat)
+    hmx = EffectiveField(m, use_demag=True, operators=film["ops"],
+                         demag_kwargs=dict(hmatrix_use_fp32=True, cg_mixed=True, **hk), **mat)
     g = np.random.default_rng(0).standard_normal((m.num_nodes, 3))
     mv = torch.from_numpy((g / np.linalg.norm(g, axis=1)[:, None]).ravel()).to(film["dev"])
     r64, rmx = torch.empty_like(mv), torch.empty_like(mv)
