@@ -43,8 +43,9 @@
 // Mixed precision (smx_pcg_amg_mixed): the fine-level iteration vectors
 // (r, s, w, u, p) and the operator of w = A u are FP32; x, the coarse-level
 // vectors, the scalars and the inner products stay FP64 (exact products,
-// compensated sums), and iterative refinement restarts the solve from the
-// current x with the residual recomputed from the FP64 operator.
+// compensated sums).  Iterative refinement: after the FP32 solve the residual
+// b - A x is recomputed from the FP64 operator and checked (two launches); a
+// correction solve from the current x runs only when the check fails.
 #include "smx_common.h"
 
 namespace {
@@ -113,15 +114,16 @@ __device__ __forceinline__ int xcd_map(int b, int G) {
   return x * q + (x < r ? x : r) + k;
 }
 
-// COMP: every addition compensated (TwoSum), v[q] + c[q] the per-thread input.
+// COMP: the sums across waves (and, in the callers, across workgroups' partials)
+// are compensated (TwoSum); v[q] + c[q] is the per-thread input.  The wave
+// totals stay a plain DPP butterfly: it sits on every iteration's dependent
+// chain, and its inputs are FP64 sums of products that are exact in FP64.
 template <int BS, int NQ, bool ALL, bool COMP = false>
 __device__ __forceinline__ void block_total(double (&v)[NQ], double* scr, const double* c = nullptr) {
   constexpr int nw = BS / 64;
   const int wid = threadIdx.x >> 6;
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) {
-    v[q] = COMP ? smx::wave_total_comp(v[q], c ? c[q] : 0.0) : smx::wave_total(v[q]);
-  }
+  for (int q = 0; q < NQ; ++q) v[q] = smx::wave_total(COMP && c ? v[q] + c[q] : v[q]);
   if ((threadIdx.x & 63) == 0) {
 #pragma unroll
     for (int q = 0; q < NQ; ++q) scr[wid * NQ + q] = v[q];
@@ -278,12 +280,44 @@ __global__ __launch_bounds__(BS) void amg_init_kernel(
     P.prr[0][blk] = v[1];
     P.prr[0][P.Gf + blk] = v[2];
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    // a refinement restart (phase > 0) adds the previous solves' count
-    P.ss[SS_ITERS0] = phase == 0 ? 0.0 : P.ss[SS_ITERS0] + P.ss[SS_ITERS];
+  // phase > 0 (a refinement check): the status is the check kernel's to write
+  if (phase == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
+    P.ss[SS_ITERS0] = 0.0;
     P.ss[SS_DONE] = 0.0;
     P.ss[SS_FAIL] = 0.0;
     P.ss[SS_ITERS] = 0.0;
+  }
+}
+
+// Refinement check, one workgroup: the FP64 residual ||b - A x|| left by
+// amg_init_kernel against the tolerance.  Met: the solve is done.  Not met:
+// the iterations so far move to SS_ITERS0 and the status is reset for a
+// correction solve from the current x.
+template <int BS>
+__global__ __launch_bounds__(BS) void amg_check_kernel(AmgPtrs P, double rtol, double atol) {
+  __shared__ double scr[BS / 64 * 3];
+  const int Gf = P.Gf;
+  double t[3] = {0.0, 0.0, 0.0}, c[3] = {0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < Gf; b += BS) {
+    smx::comp_add(t[0], c[0], P.pbb[b]);
+    smx::comp_add(t[1], c[1], P.prr[0][b]);
+    smx::comp_add(t[2], c[2], P.prr[0][Gf + b]);
+  }
+  block_total<BS, 3, false, true>(t, scr, c);
+  if (threadIdx.x == 0) {
+    const double tol2 = fmax(rtol * rtol * t[0], atol * atol);
+    P.ss[SS_BB] = t[0];
+    P.ss[SS_TOL2] = tol2;
+    P.ss[SS_RR] = t[1];
+    P.ss[SS_SUMX] = t[2];
+    P.ss[SS_FAIL] = 0.0;
+    if (t[1] <= tol2) {
+      P.ss[SS_DONE] = 1.0;
+    } else {
+      P.ss[SS_ITERS0] = P.ss[SS_ITERS0] + P.ss[SS_ITERS];
+      P.ss[SS_ITERS] = 0.0;
+      P.ss[SS_DONE] = 0.0;
+    }
   }
 }
 
@@ -575,10 +609,11 @@ int vcycle_tail(const smx_amg_level* lv, int nlev, const V* r_fine, AmgPtrs& P,
   return launch_level<VT, V, double, V>(lv[0].S, r_fine, q0, x1, vec<V>(P.u), P.ss, lv[0].s_tpr, st);
 }
 
-// The solve as nphase CG solves in a row (iterative refinement, as in
-// pcg.hip): every solve after the first starts from the current x with its
-// residual recomputed from the FP64 operator by the init kernel; solves before
-// the last stop at rtol_inner.  ivals: values of the w = A u operator.
+// The solve: AMG-CG from x0 (warm) or 0 to rtol_inner (rtol without
+// refinement), then up to `refinements` times the FP64 residual check
+// (amg_init_kernel + amg_check_kernel) and, if it fails, a correction solve
+// from the current x.  A speculative launch enqueues the first solve and one
+// check: a failed check leaves DONE = 0 and the caller solves again waiting.
 template <int BS, int WMAX, bool EXACT, typename VT, typename V, typename AV, bool COMP>
 int run_amg(int64_t n, int32_t W, int64_t ld, const int32_t* cols, const double* vals,
             const AV* ivals, int32_t mode, const double* b, const double* b_part, int64_t nb_part,
@@ -586,8 +621,7 @@ int run_amg(int64_t n, int32_t W, int64_t ld, const int32_t* cols, const double*
             const double* raw_vals, const int32_t* bpos, const double* gb, double* x_user,
             int32_t warm, const int32_t* perm, double* x_perm, const smx_amg_level* lv, int nlev,
             double rtol, double atol, int32_t max_it, int32_t check_every, int32_t expected_its,
-            int32_t nphase, double rtol_inner, int32_t expected_inner, AmgPtrs P, hipStream_t st,
-            double* hst) {
+            int32_t refinements, double rtol_inner, AmgPtrs P, hipStream_t st, double* hst) {
   const int Gf = P.Gf;
   const unsigned gs = (unsigned)((n + 255) / 256);
   double* x = perm ? x_perm : x_user;
@@ -602,27 +636,25 @@ int run_amg(int64_t n, int32_t W, int64_t ld, const int32_t* cols, const double*
   if (nlev > 1) R0 = to_op(lv[0].R);
   double* b1 = nlev > 1 ? lv[1].b : nullptr;
   // check_every < 0: speculative launch of expected + 1 - check_every
-  // iterations per solve with no wait (iterations past convergence return at
-  // once on the done flag); the caller reads the status block later
+  // iterations with no wait (iterations past convergence return at once on
+  // the done flag); the caller reads the status block later
   const bool spec = check_every < 0 && expected_its > 0;
   const int32_t every = check_every < 1 ? 8 : check_every;
-  int rc;
-  for (int32_t phase = 0; phase < nphase; ++phase) {
-    const bool last = phase + 1 == nphase;
-    const int32_t wph = phase == 0 ? warm : 1;
-    // inner solve k reduces the residual to rtol_inner^(k+1) of ||b||
-    const double tol = last ? rtol : fmax(rtol, pow(rtol_inner, (double)(phase + 1)));
-    const int32_t expect = last ? expected_its : (expected_inner > 0 ? expected_inner : expected_its);
+  auto init = [&](int32_t w, int32_t phase) -> int {
     if (mode == 0) {
       hipLaunchKernelGGL((amg_init_kernel<BS, WMAX, V>), dim3(Gf), dim3(BS), 0, st, n, mode, W, ld,
-                         cols, vals, b, b_part, nb_part, project, bpos, gb, x, wph, perm, P, n1,
-                         phase);
+                         cols, vals, b, b_part, nb_part, project, bpos, gb, x, w, perm, P, n1, phase);
     } else {
       hipLaunchKernelGGL((amg_init_kernel<BS, WMAX, V>), dim3(Gf), dim3(BS), 0, st, n, mode, raw_W,
-                         raw_ld, raw_cols, raw_vals, b, b_part, nb_part, project, bpos, gb, x, wph,
-                         perm, P, n1, phase);
+                         raw_ld, raw_cols, raw_vals, b, b_part, nb_part, project, bpos, gb, x, w, perm,
+                         P, n1, phase);
     }
     SMX_LAUNCH_CHECK();
+    return SMX_OK;
+  };
+  // one AMG-CG solve on the residual in P.r[0] (left by amg_init_kernel)
+  auto iterate = [&](double tol, int32_t expect) -> int {
+    int rc;
     // u0 = M^-1 r0, w0 = A u0
     const V* r0 = vec<V>(P.r[0]);
     if (nlev > 1) {
@@ -671,10 +703,32 @@ int run_amg(int64_t n, int32_t W, int64_t ld, const int32_t* cols, const double*
       done = hst[SS_DONE] != 0.0;
       batch = every;
     }
+    return SMX_OK;
+  };
+  int rc = init(warm, 0);
+  if (rc) return rc;
+  rc = iterate(refinements > 0 ? fmax(rtol, rtol_inner) : rtol, expected_its);
+  if (rc) return rc;
+  for (int32_t k = 1; k <= refinements; ++k) {
+    rc = init(1, k);
+    if (rc) return rc;
+    hipLaunchKernelGGL((amg_check_kernel<1024>), dim3(1), dim3(1024), 0, st, P, rtol, atol);
+    SMX_LAUNCH_CHECK();
+    if (spec) break;
+    SMX_HIP_CHECK(hipMemcpyAsync(hst, P.ss, sizeof(double) * SS_SIZE, hipMemcpyDeviceToHost, st));
+    SMX_HIP_CHECK(hipStreamSynchronize(st));
+    if (hst[SS_DONE] != 0.0) break;
+    // correction k stops at rtol_inner^(k+1) of ||b||, the last at rtol
+    rc = iterate(k == refinements ? rtol : fmax(rtol, pow(rtol_inner, (double)(k + 1))), 0);
+    if (rc) return rc;
   }
   if (perm) {
     hipLaunchKernelGGL(amg_scatter_kernel, dim3(gs), dim3(256), 0, st, n, perm, x, x_user);
     SMX_LAUNCH_CHECK();
+  }
+  if (!spec) {
+    SMX_HIP_CHECK(hipMemcpyAsync(hst, P.ss, sizeof(double) * SS_SIZE, hipMemcpyDeviceToHost, st));
+    SMX_HIP_CHECK(hipStreamSynchronize(st));
   }
   return SMX_OK;
 }
@@ -810,15 +864,14 @@ int amg_entry(int64_t n, int32_t width, int64_t ld, const int32_t* cols, const d
   const int bs = fine_bs(n);
   const int32_t wmax = (mode == 1 && raw_width > width) ? raw_width : width;
   const bool w7 = width == 7 && wmax == 7;
-  const int32_t nphase = mx ? 1 + mx->refinements : 1;
+  const int32_t nref = mx ? mx->refinements : 0;
   const double rin = mx ? mx->rtol_inner : rtol;
-  const int32_t ein = mx ? mx->expected_inner : 0;
   int rc;
 #define SMX_AMG_CALL(BS, WM, EX, VT, V, AV, CP, IV)                                              \
   run_amg<BS, WM, EX, VT, V, AV, CP>(n, width, ld, cols, vals, IV, mode, b, b_part, nb_part,     \
                                      project, raw_width, raw_ld, raw_cols, raw_vals, bpos, gb, x, \
                                      warm, perm, x_perm, levels, nlev, rtol, atol, max_it,       \
-                                     check_every, expected_its, nphase, rin, ein, P, st, hst)
+                                     check_every, expected_its, nref, rin, P, st, hst)
 #define SMX_AMG_W(BS, VT, V, AV, CP, IV)                                                          \
   (w7 ? SMX_AMG_CALL(BS, 7, true, VT, V, AV, CP, IV)                                              \
       : wmax <= 8 ? SMX_AMG_CALL(BS, 8, false, VT, V, AV, CP, IV)                                 \

@@ -52,9 +52,9 @@
 // two FP32 numbers is exact in FP64, and the sums are compensated (TwoSum in
 // the wave butterfly and across workgroups), so the inner products carry no
 // more than a few FP64 roundings.  Accuracy is recovered by iterative
-// refinement: after an inner solve to rtol_inner, a new solve starts from the
-// current x with its residual b - A x recomputed in FP64 from the FP64
-// operator (the init kernels), and the last solve runs to rtol.
+// refinement: after the FP32 solve, the residual b - A x is recomputed in FP64
+// from the FP64 operator and checked against the tolerance (two launches);
+// only when it fails does a correction solve run, from the current x.
 #include "smx_common.h"
 
 namespace {
@@ -85,7 +85,7 @@ struct CGPtrs {
   double* ss;        // status record
   double* st[2];     // parity-buffered scalars
   double* part[2];   // parity-buffered partials, 4 x G
-  double* pbb;       // 2 x G partials: ||b||^2 and rows whose slot 0 is not the diagonal
+  double* pbb;       // 3 x G partials: ||b||^2, rows whose slot 0 is not the diagonal, ||b - A x0||^2
   void* r[2];
   void* w[2];
   void* s[2];
@@ -121,15 +121,16 @@ __device__ __forceinline__ T ldo(const T* __restrict__ base, uint32_t byte_off) 
 // same launch would need a second barrier.  ALL: every thread gets the totals
 // (else only thread 0).  The summation order is fixed, so every workgroup that
 // reduces the same values gets bit-identical totals.
-// COMP: every addition compensated (TwoSum), v[q] + c[q] the per-thread input.
+// COMP: the sums across waves (and, in the callers, across workgroups' partials)
+// are compensated (TwoSum); v[q] + c[q] is the per-thread input.  The wave
+// totals stay a plain DPP butterfly: it sits on every iteration's dependent
+// chain, and its inputs are FP64 sums of products that are exact in FP64.
 template <int BS, int NQ, bool ALL, bool COMP = false>
 __device__ __forceinline__ void block_total(double (&v)[NQ], double* scr, const double* c = nullptr) {
   constexpr int nw = BS / 64;
   const int wid = threadIdx.x >> 6;
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) {
-    v[q] = COMP ? smx::wave_total_comp(v[q], c ? c[q] : 0.0) : smx::wave_total(v[q]);
-  }
+  for (int q = 0; q < NQ; ++q) v[q] = smx::wave_total(COMP && c ? v[q] + c[q] : v[q]);
   if ((threadIdx.x & 63) == 0) {
 #pragma unroll
     for (int q = 0; q < NQ; ++q) scr[wid * NQ + q] = v[q];
@@ -219,9 +220,10 @@ __global__ __launch_bounds__(256) void cg_prepare_kernel(int64_t n, int32_t W, i
   }
 }
 
-// r0 = b - A x0 (mode 0) or the lifted residual (mode 1); partials of ||b||^2
-// and of the rows whose slot 0 is not their diagonal (acols: the solver-form
-// operator, checked in both modes).
+// r0 = b - A x0 (mode 0) or the lifted residual (mode 1), in FP64 from the
+// FP64 operator; partials of ||b||^2, of the rows whose slot 0 is not their
+// diagonal (acols: the solver-form operator, checked in both modes) and of
+// ||r0||^2 (read by the refinement check).
 template <int BS, int WMAX, typename V>
 __global__ __launch_bounds__(BS) void cg_init_a_kernel(
     int64_t n, int32_t mode, int32_t W, int64_t ld, const int32_t* __restrict__ cols,
@@ -231,7 +233,7 @@ __global__ __launch_bounds__(BS) void cg_init_a_kernel(
     const int32_t* __restrict__ bpos, const double* __restrict__ gb, double* __restrict__ x,
     int32_t warm, const int32_t* __restrict__ perm, CGPtrs P) {
   __shared__ double scr[BS / 64 * 1];
-  __shared__ double scr_pub[BS / 64 * 2];
+  __shared__ double scr_pub[BS / 64 * 3];
   const int64_t row = (int64_t)blockIdx.x * BS + threadIdx.x;
   double shift = 0.0;
   if (mode == 0 && project && b_part) {
@@ -241,7 +243,7 @@ __global__ __launch_bounds__(BS) void cg_init_a_kernel(
     block_total<BS, 1, true>(t, scr);
     shift = t[0] / (double)n;
   }
-  double v[2] = {0.0, 0.0};
+  double v[3] = {0.0, 0.0, 0.0};
   if (row < n) {
     double r0, bk;
     if (mode == 0) {
@@ -293,8 +295,35 @@ __global__ __launch_bounds__(BS) void cg_init_a_kernel(
     vec<V>(P.p)[row] = (V)0.0;
     v[0] = bk * bk;
     v[1] = acols[row] != (int32_t)row ? 1.0 : 0.0;
+    v[2] = r0 * r0;
   }
-  publish_partials<BS, 2>(v, P.pbb, P.G, scr_pub);
+  publish_partials<BS, 3>(v, P.pbb, P.G, scr_pub);
+}
+
+// Refinement check, one workgroup: the FP64 residual ||b - A x|| left by
+// cg_init_a against the tolerance.  Met: the solve is done (x stands).  Not
+// met: the iterations so far move to SS_ITERS0 and the status is reset for a
+// correction solve from the current x.
+template <int BS>
+__global__ __launch_bounds__(BS) void cg_check_kernel(CGPtrs P, double rtol, double atol) {
+  __shared__ double scr[BS / 64 * 3];
+  double t[3];
+  reduce_partials<BS, 3, true>(P.pbb, P.G, t, scr);
+  if (threadIdx.x == 0) {
+    const double tol2 = fmax(rtol * rtol * t[0], atol * atol);
+    const bool ok = t[2] <= tol2 && t[1] == 0.0;
+    P.ss[SS_BB] = t[0];
+    P.ss[SS_TOL2] = tol2;
+    P.ss[SS_RR] = t[2];
+    P.ss[SS_FAIL] = t[1] > 0.0 ? 3.0 : 0.0;
+    if (ok || t[1] > 0.0) {
+      P.ss[SS_DONE] = 1.0;
+    } else {
+      P.ss[SS_ITERS0] = P.ss[SS_ITERS0] + P.ss[SS_ITERS];
+      P.ss[SS_ITERS] = 0.0;
+      P.ss[SS_DONE] = 0.0;
+    }
+  }
 }
 
 // x_p[i] = x[perm[i]] (warm start in solver order)
@@ -312,9 +341,8 @@ __global__ void cg_scatter_kernel(int64_t n, const int32_t* __restrict__ perm, c
 }
 
 // w0 = A D^-1 r0 with the iteration operator (values AV); partials of
-// (g0, d0, rr0, sum x0).  tol_rel: the relative tolerance of this solve;
-// phase > 0 (a refinement restart): the previous solves' iteration counts are
-// added to SS_ITERS0 before SS_ITERS restarts at 0.
+// (g0, d0, rr0, sum x0).  rtol: the relative tolerance of this solve; phase 0
+// clears SS_ITERS0 (a correction solve keeps what the check moved there).
 template <int BS, int WMAX, typename V, typename AV, bool COMP>
 __global__ __launch_bounds__(BS) void cg_init_b_kernel(int64_t n, int32_t W, int64_t ld,
                                                        const int32_t* __restrict__ cols,
@@ -330,7 +358,7 @@ __global__ __launch_bounds__(BS) void cg_init_b_kernel(int64_t n, int32_t W, int
     const double t1 = rtol * rtol * t[0], t2 = atol * atol;
     P.ss[SS_BB] = t[0];
     P.ss[SS_TOL2] = t1 > t2 ? t1 : t2;
-    P.ss[SS_ITERS0] = phase == 0 ? 0.0 : P.ss[SS_ITERS0] + P.ss[SS_ITERS];
+    if (phase == 0) P.ss[SS_ITERS0] = 0.0;
     P.ss[SS_ITERS] = 0.0;
     P.ss[SS_DONE] = t[1] > 0.0 ? 1.0 : 0.0;  // a layout error stops before iteration 0
     P.ss[SS_FAIL] = t[1] > 0.0 ? 3.0 : 0.0;
@@ -501,18 +529,20 @@ Layout layout(int64_t n) {
   return L;
 }
 
-// The solve as nphase CG solves in a row: solve 0 from x0 (warm) or 0, every
-// later one from the current x with its residual recomputed in FP64 by the
-// init kernels (iterative refinement); solves before the last stop at
-// rtol_inner, the last at rtol.  ivals: the iteration operator's values.
+// The solve: CG from x0 (warm) or 0 to rtol_inner (rtol without
+// refinement), then up to `refinements` times: the FP64 residual b - A x and
+// its check (cg_init_a + cg_check_kernel); if it fails, a correction solve
+// from the current x to the next tolerance.  A speculative launch enqueues
+// the first solve and one check and returns: a failed check leaves DONE = 0
+// in the status block, and the caller solves again waiting.
 template <int BS, int WMAX, bool EXACT, typename V, typename AV, bool COMP>
 int run_pcg(int64_t n, int32_t W, int64_t ld, const int32_t* cols, const double* vals,
             const AV* ivals, const double* dinv, int32_t mode, const double* b,
             const double* b_part, int64_t nb_part, int32_t project, int32_t raw_W, int64_t raw_ld,
             const int32_t* raw_cols, const double* raw_vals, const int32_t* bpos, const double* gb,
             double* x_user, int32_t warm, const int32_t* perm, double* x_perm, double rtol,
-            double atol, int32_t max_it, int32_t check_every, int32_t expected_its, int32_t nphase,
-            double rtol_inner, int32_t expected_inner, CGPtrs P, hipStream_t st, double* hst) {
+            double atol, int32_t max_it, int32_t check_every, int32_t expected_its,
+            int32_t refinements, double rtol_inner, CGPtrs P, hipStream_t st, double* hst) {
   const int G = P.G;
   const unsigned gs = (unsigned)((n + 255) / 256);
   double* x = perm ? x_perm : x_user;  // the kernels work in solver (permuted) row order
@@ -521,26 +551,24 @@ int run_pcg(int64_t n, int32_t W, int64_t ld, const int32_t* cols, const double*
     SMX_LAUNCH_CHECK();
   }
   // check_every < 0: speculative launch of expected + 1 - check_every
-  // iterations per solve with no wait (iterations past convergence return at
-  // once on the done flag); the caller reads the status block later
+  // iterations with no wait (iterations past convergence return at once on
+  // the done flag); the caller reads the status block later
   const bool spec = check_every < 0 && expected_its > 0;
   const int32_t every = check_every < 1 ? 16 : check_every;
-  for (int32_t phase = 0; phase < nphase; ++phase) {
-    const bool last = phase + 1 == nphase;
-    const int32_t wph = phase == 0 ? warm : 1;
-    // inner solve k reduces the residual to rtol_inner^(k+1) of ||b||
-    const double tol = last ? rtol : fmax(rtol, pow(rtol_inner, (double)(phase + 1)));
-    const int32_t expect = last ? expected_its : (expected_inner > 0 ? expected_inner : expected_its);
+  auto init_a = [&](int32_t w) -> int {
     if (mode == 0) {
       hipLaunchKernelGGL((cg_init_a_kernel<BS, WMAX, V>), dim3(G), dim3(BS), 0, st, n, mode, W, ld,
-                         cols, vals, cols, dinv, b, b_part, nb_part, project, bpos, gb, x, wph,
-                         perm, P);
+                         cols, vals, cols, dinv, b, b_part, nb_part, project, bpos, gb, x, w, perm, P);
     } else {
       hipLaunchKernelGGL((cg_init_a_kernel<BS, WMAX, V>), dim3(G), dim3(BS), 0, st, n, mode, raw_W,
-                         raw_ld, raw_cols, raw_vals, cols, dinv, b, b_part, nb_part, project, bpos,
-                         gb, x, wph, perm, P);
+                         raw_ld, raw_cols, raw_vals, cols, dinv, b, b_part, nb_part, project, bpos, gb,
+                         x, w, perm, P);
     }
     SMX_LAUNCH_CHECK();
+    return SMX_OK;
+  };
+  // one CG solve on the residual in P.r[0] (left by cg_init_a)
+  auto iterate = [&](int32_t phase, double tol, int32_t expect) -> int {
     hipLaunchKernelGGL((cg_init_b_kernel<BS, WMAX, V, AV, COMP>), dim3(G), dim3(BS), 0, st, n, W,
                        ld, cols, ivals, dinv, x, P, tol, atol, phase);
     SMX_LAUNCH_CHECK();
@@ -561,10 +589,33 @@ int run_pcg(int64_t n, int32_t W, int64_t ld, const int32_t* cols, const double*
       done = hst[SS_DONE] != 0.0;
       batch = every;
     }
+    return SMX_OK;
+  };
+  int rc = init_a(warm);
+  if (rc) return rc;
+  rc = iterate(0, refinements > 0 ? fmax(rtol, rtol_inner) : rtol, expected_its);
+  if (rc) return rc;
+  for (int32_t k = 1; k <= refinements; ++k) {
+    rc = init_a(1);
+    if (rc) return rc;
+    hipLaunchKernelGGL((cg_check_kernel<1024>), dim3(1), dim3(1024), 0, st, P, rtol, atol);
+    SMX_LAUNCH_CHECK();
+    if (spec) break;
+    SMX_HIP_CHECK(hipMemcpyAsync(hst, P.ss, sizeof(double) * SS_SIZE, hipMemcpyDeviceToHost, st));
+    SMX_HIP_CHECK(hipStreamSynchronize(st));
+    if (hst[SS_DONE] != 0.0) break;
+    // correction k stops at rtol_inner^(k+1) of ||b||, the last at rtol
+    const double tol = k == refinements ? rtol : fmax(rtol, pow(rtol_inner, (double)(k + 1)));
+    rc = iterate(k, tol, 0);
+    if (rc) return rc;
   }
   if (perm) {
     hipLaunchKernelGGL(cg_scatter_kernel, dim3(gs), dim3(256), 0, st, n, perm, x, x_user);
     SMX_LAUNCH_CHECK();
+  }
+  if (!spec) {
+    SMX_HIP_CHECK(hipMemcpyAsync(hst, P.ss, sizeof(double) * SS_SIZE, hipMemcpyDeviceToHost, st));
+    SMX_HIP_CHECK(hipStreamSynchronize(st));
   }
   return SMX_OK;
 }
@@ -587,7 +638,7 @@ extern "C" int smx_pcg_prepare(int64_t n, int32_t width, int64_t ld, const int32
 
 extern "C" int64_t smx_pcg_ws_size(int64_t n) {
   const Layout L = layout(n);
-  return HDR + 10 * L.G + 8 * L.nvec + 64;
+  return HDR + 11 * L.G + 8 * L.nvec + 64;
 }
 
 namespace {
@@ -635,7 +686,7 @@ int pcg_entry(int64_t n, int32_t width, int64_t ld, const int32_t* cols, const d
   P.part[0] = work + HDR;
   P.part[1] = work + HDR + 4 * Lo.G;
   P.pbb = work + HDR + 8 * Lo.G;
-  double* v0 = work + HDR + 10 * Lo.G;
+  double* v0 = work + HDR + 11 * Lo.G;
   v0 += (64 - (((uintptr_t)v0 / sizeof(double)) & 63)) & 63;  // 512-byte aligned vectors
   P.r[0] = v0;
   P.r[1] = v0 + Lo.nvec;
@@ -647,15 +698,14 @@ int pcg_entry(int64_t n, int32_t width, int64_t ld, const int32_t* cols, const d
   double* x_perm = v0 + 7 * Lo.nvec;
   double hst[SS_SIZE] = {0};
   const int bs = block_size_for(n);
-  const int32_t nphase = mx ? 1 + mx->refinements : 1;
+  const int32_t nref = mx ? mx->refinements : 0;
   const double rin = mx ? mx->rtol_inner : rtol;
-  const int32_t ein = mx ? mx->expected_inner : 0;
   int rc;
 #define SMX_PCG_CALL(BS, WM, EX, V, AV, CP, IV)                                                   \
   run_pcg<BS, WM, EX, V, AV, CP>(n, width, ld, cols, vals, IV, dinv, mode, b, b_part, nb_part,    \
                                  project, raw_width, raw_ld, raw_cols, raw_vals, bpos, gb, x, warm, \
                                  perm, x_perm, rtol, atol, max_it, check_every, expected_its,     \
-                                 nphase, rin, ein, P, st, hst)
+                                 nref, rin, P, st, hst)
 #define SMX_PCG_WIDTH(BS, V, AV, CP, IV)                                                          \
   (w7 ? SMX_PCG_CALL(BS, 7, true, V, AV, CP, IV)                                                  \
       : wmax <= 8 ? SMX_PCG_CALL(BS, 8, false, V, AV, CP, IV)                                     \

@@ -107,33 +107,6 @@ __device__ __forceinline__ void comp_add(double& s, double& c, double v) {
   c += e;
 }
 
-// wave_total with every addition compensated: the lanes' (sum, correction)
-// pairs travel through the same DPP butterfly, and the result is the rounded
-// s + c, identical in every lane.
-template <int CTRL>
-__device__ __forceinline__ void dpp_comp_step(double& s, double& c) {
-  const double bs = dpp_move<CTRL>(s), bc = dpp_move<CTRL>(c);
-  double t, e;
-  two_sum(s, bs, t, e);
-  s = t;
-  c = c + bc + e;
-}
-
-__device__ __forceinline__ double wave_total_comp(double v, double c = 0.0) {
-  double s = v;
-  dpp_comp_step<0xB1>(s, c);
-  dpp_comp_step<0x4E>(s, c);
-  dpp_comp_step<0x141>(s, c);
-  dpp_comp_step<0x140>(s, c);
-  double S = read_lane(s, 0), C = read_lane(c, 0);
-#pragma unroll
-  for (int l = 16; l < 64; l += 16) {
-    comp_add(S, C, read_lane(s, l));
-    C += read_lane(c, l);
-  }
-  return S + C;
-}
-
 // Workgroup barrier that orders LDS accesses only: outstanding global loads
 // and stores of the calling wave are not waited for.
 __device__ __forceinline__ void lds_barrier() {

@@ -66,7 +66,7 @@ class FieldArgs(C.Structure):
 
 class CGMixed(C.Structure):
     """smx_cg_mixed: FP32 inner CG with FP64 iterative refinement."""
-    _fields_ = [("vals32", P), ("rtol_inner", F64), ("refinements", I32), ("expected_inner", I32)]
+    _fields_ = [("vals32", P), ("rtol_inner", F64), ("refinements", I32), ("pad", I32)]
 
 
 class NearTilesArgs(C.Structure):

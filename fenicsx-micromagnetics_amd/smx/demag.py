@@ -160,9 +160,9 @@ class DemagFieldLindholmGPU:
         ``amg_fp32`` (FP32 AMG operator values; default: FP32 when the boundary
         operator is stored below FP64), ``cg_mixed`` (the mixed-precision
         Poisson solves: FP32 inner CG, FP64 iterative refinement; off unless
-        asked for) with ``cg_refinements`` FP64 residual restarts, inner solve
-        k stopping at ``cg_rtol_inner``^(k+1) of ||b|| (default: the factor
-        rtol^(1/(refinements+1)) per solve), and prebuilt geometry / operators."""
+        asked for): the FP32 solve to max(rtol, ``cg_rtol_inner``) (default
+        rtol), then up to ``cg_refinements`` FP64 residual checks, each followed
+        by a correction solve when it fails; and prebuilt geometry / operators."""
         if not isinstance(mesh, TetMesh):
             raise TypeError("mesh must be a smx.mesh.TetMesh")
         self.mesh = mesh
@@ -447,11 +447,10 @@ class DemagFieldLindholmGPU:
         mixed = None
         if self.cg_mixed:
             v32 = c["plain32"] if amg_pc else c["jacobi32"]
-            # default: every solve gains the same factor, rtol^(1 / (refinements + 1))
-            rin = self.cg_rtol_inner if self.cg_rtol_inner is not None else \
-                float(rtol) ** (1.0 / (self.cg_refinements + 1))
-            mixed = _capi.CGMixed(v32.data_ptr(), min(max(rin, 1e-300), 0.5), self.cg_refinements,
-                                  int(self._its_split[op][0]))
+            # default: the FP32 solve runs to rtol itself; the FP64 check then
+            # rarely asks for a correction solve
+            rin = float(rtol) if self.cg_rtol_inner is None else self.cg_rtol_inner
+            mixed = _capi.CGMixed(v32.data_ptr(), min(max(rin, 1e-300), 0.5), self.cg_refinements, 0)
         if amg_pc:
             A, amg = c["plain"], c["amg"]
             args = [self.N, A.width, A.ld, _capi.ptr(A.cols), _capi.ptr(A.vals), mode,
@@ -519,11 +518,11 @@ class DemagFieldLindholmGPU:
         warm = self.warm_start and self._have_u
         self.Div.spmv(m_vec, self.b_u1, kind=1, partials=self.b_part)
         its1 = self.solve("u1", 0, self.u1, self.ws1, b=self.b_u1, b_part=self.b_part,
-                          expected=self._its_split["u1"][1], warm=warm, speculative=speculative)
+                          expected=self.last_its_u1, warm=warm, speculative=speculative)
         self.boundary_matvec(self.u1, self.y_rows, gather=self.bverts_gpu,
                              shift_ptr=self.u1_sum_ptr, shift_scale=1.0 / self.N)
         its2 = self.solve("u2", 1, self.u2, self.ws2, gb=self.y_rows, project=0,
-                          expected=self._its_split["u2"][1], warm=warm, speculative=speculative)
+                          expected=self.last_its_u2, warm=warm, speculative=speculative)
         self._have_u = True
         if its1 < 0 or its2 < 0:
             self._status_host[:8].copy_(self.ws1[:8], non_blocking=True)

@@ -376,24 +376,26 @@ int smx_pcg_jacobi(int64_t n, int32_t width, int64_t ld, const int32_t* cols,
  *     layout of `vals` (the solver form for the Jacobi solver, the plain A
  *     for the AMG solver);
  *   - x, the CG scalars and the inner products in FP64: each product of two
- *     FP32 values is exact in FP64 and the sums are compensated (TwoSum);
- *   - `refinements` restarts: inner solve k (k = 0 .. refinements-1) stops
- *     at a residual of max(rtol, rtol_inner^(k+1)) * ||b||, and the next
- *     starts from the current x with the residual b - A x recomputed from
- *     the FP64 operator; the last solve stops at rtol.  refinements = 0 is
- *     a single FP32-vector solve.
- * Speculative launches (check_every < 0) enqueue expected_inner + 1 -
- * check_every iterations for each inner solve before the last and
- * expected_its + 1 - check_every for the last.  The status block then holds
- * the last solve's count at SMX_PCG_ITERS and the earlier solves' total at
- * SMX_PCG_ITERS0; its_out (waiting form) is their sum.  This is the
- * mixed-precision form of the reference's KSP cg solves
+ *     FP32 values is exact in FP64, the wave sums are FP64 and the sums
+ *     across waves and workgroups are compensated (TwoSum);
+ *   - iterative refinement: the FP32 solve runs to max(rtol, rtol_inner);
+ *     then, up to `refinements` times, the residual b - A x is recomputed
+ *     from the FP64 operator (`vals`) and checked against rtol; when it
+ *     fails, a correction solve (FP32 again, from the current x) runs to
+ *     max(rtol, rtol_inner^(k+1)) of ||b||, the last one to rtol.
+ *     refinements = 0: the FP32 solve alone.
+ * A speculative launch (check_every < 0) enqueues the FP32 solve
+ * (expected_its + 1 - check_every iterations) and one FP64 check; a failed
+ * check leaves SMX_PCG_DONE = 0 and the caller solves again waiting.  The
+ * status block holds the last solve's iteration count at SMX_PCG_ITERS and
+ * the earlier solves' total at SMX_PCG_ITERS0; its_out (waiting form) is
+ * their sum.  Mixed-precision form of the reference's KSP cg solves
  * (Demag_Lindholm_GPU.py:216-226, 271-281, 569-640), which run in FP64. */
 typedef struct {
   const float* vals32;      /* FP32 iteration operator values */
-  double rtol_inner;        /* residual reduction of each inner solve but the last, (0, 1) */
-  int32_t refinements;      /* FP64 residual restarts, 0..8 */
-  int32_t expected_inner;   /* speculative: iterations expected of the earlier solves */
+  double rtol_inner;        /* tolerance factor of the FP32 solves before the last, (0, 1) */
+  int32_t refinements;      /* FP64 residual checks / correction solves, 0..8 */
+  int32_t pad;
 } smx_cg_mixed;
 #define SMX_PCG_ITERS0 7
 int smx_pcg_jacobi_mixed(int64_t n, int32_t width, int64_t ld, const int32_t* cols,

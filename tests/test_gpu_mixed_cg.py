@@ -4,8 +4,10 @@ FP32 inner CG, FP64 iterative refinement, FP64 compensated inner products.
 * Both solves of the demag field (u1: Neumann, mode 0; u2: Dirichlet lifting,
   mode 1), both preconditioners (AMG, Jacobi), on a 14 x 12 x 10 Kuhn box:
   the true FP64 residual of the mixed solution (recomputed on the host) meets
-  the requested tolerance, for rtol 1e-6 (one refinement) and 1e-10 (two
-  refinements, the solution within 1e-8 of the FP64 solver's at 1e-13).
+  the requested tolerance, for rtol 1e-6 (the FP32 solve to rtol and one FP64
+  check) and 1e-10 (the FP32 solve to 1e-4, then FP64 checks and correction
+  solves to 1e-8 and 1e-10; the solution within 1e-8 of the FP64 solver's at
+  1e-13).
 * Without refinement the FP32 solve still converges in its recurrence, and
   its true residual is within 10x the tolerance at rtol 1e-6.
 * The waiting call reports the iteration count of all its solves; the
@@ -83,10 +85,10 @@ def _solve_both(d, b, gb):
 
 
 @pytest.mark.parametrize("pc", ["gamg", "jacobi"])
-@pytest.mark.parametrize("rtol,refinements", [(1e-6, 1), (1e-10, 2)])
-def test_mixed_solves_meet_tolerance(pc, rtol, refinements):
+@pytest.mark.parametrize("rtol,refinements,rtol_inner", [(1e-6, 1, None), (1e-10, 2, 1e-4)])
+def test_mixed_solves_meet_tolerance(pc, rtol, refinements, rtol_inner):
     m = _box()
-    d = _demag(m, pc, True, refinements, rtol)
+    d = _demag(m, pc, True, refinements, rtol, cg_rtol_inner=rtol_inner)
     A = _stiff(d)
     b, gb = _rhs(d, 3)
     its1, its2, x1, x2 = _solve_both(d, b, gb)
@@ -94,7 +96,9 @@ def test_mixed_solves_meet_tolerance(pc, rtol, refinements):
     split1, split2 = d._its_split["u1"], d._its_split["u2"]
     print(f"[mixed cg {pc} rtol={rtol:g} ref={refinements}] its u1 {its1} {split1} u2 {its2} {split2}; "
           f"true residuals {r1:.2e} {r2:.2e}", flush=True)
-    assert sum(split1) == its1 and sum(split2) == its2 and split1[0] > 0
+    assert sum(split1) == its1 and sum(split2) == its2
+    if rtol_inner is not None:      # the FP32 solve stops at 1e-4: the corrections must run
+        assert split1[0] > 0 and split2[0] > 0
     assert r1 <= 1.5 * rtol and r2 <= 1.5 * rtol
     if rtol < 1e-8:
         d64 = _demag(m, pc, False, rtol=1e-13)
@@ -128,8 +132,8 @@ def test_speculative_equals_waiting(pc):
     d.u1.zero_()
     d.u2.zero_()
     y = gb.clone()
-    assert d.solve("u1", 0, d.u1, d.ws1, b=b, b_part=d.b_part, expected=e1[1], speculative=True) == -1
-    assert d.solve("u2", 1, d.u2, d.ws2, gb=y, project=0, expected=e2[1], speculative=True) == -1
+    assert d.solve("u1", 0, d.u1, d.ws1, b=b, b_part=d.b_part, expected=sum(e1), speculative=True) == -1
+    assert d.solve("u2", 1, d.u2, d.ws2, gb=y, project=0, expected=sum(e2), speculative=True) == -1
     torch.cuda.synchronize()
     s1, s2 = d.ws1[:8].cpu().numpy(), d.ws2[:8].cpu().numpy()
     assert s1[0] == 1.0 and s2[0] == 1.0 and s1[5] == 0.0 and s2[5] == 0.0
