@@ -610,7 +610,7 @@ int vcycle_tail(const smx_amg_level* lv, int nlev, const V* r_fine, AmgPtrs& P,
 }
 
 // The solve: AMG-CG from x0 (warm) or 0 to rtol_inner (rtol without
-// refinement), then up to `refinements` times the FP64 residual check
+// refinement; rtol_inner may be below rtol), then up to `refinements` times the FP64 residual check
 // (amg_init_kernel + amg_check_kernel) and, if it fails, a correction solve
 // from the current x.  A speculative launch enqueues the first solve and one
 // check: a failed check leaves DONE = 0 and the caller solves again waiting.
@@ -707,7 +707,7 @@ int run_amg(int64_t n, int32_t W, int64_t ld, const int32_t* cols, const double*
   };
   int rc = init(warm, 0);
   if (rc) return rc;
-  rc = iterate(refinements > 0 ? fmax(rtol, rtol_inner) : rtol, expected_its);
+  rc = iterate(refinements > 0 ? rtol_inner : rtol, expected_its);
   if (rc) return rc;
   for (int32_t k = 1; k <= refinements; ++k) {
     rc = init(1, k);

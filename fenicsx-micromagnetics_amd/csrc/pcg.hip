@@ -530,7 +530,7 @@ Layout layout(int64_t n) {
 }
 
 // The solve: CG from x0 (warm) or 0 to rtol_inner (rtol without
-// refinement), then up to `refinements` times: the FP64 residual b - A x and
+// refinement; rtol_inner may be below rtol), then up to `refinements` times: the FP64 residual b - A x and
 // its check (cg_init_a + cg_check_kernel); if it fails, a correction solve
 // from the current x to the next tolerance.  A speculative launch enqueues
 // the first solve and one check and returns: a failed check leaves DONE = 0
@@ -593,7 +593,7 @@ int run_pcg(int64_t n, int32_t W, int64_t ld, const int32_t* cols, const double*
   };
   int rc = init_a(warm);
   if (rc) return rc;
-  rc = iterate(0, refinements > 0 ? fmax(rtol, rtol_inner) : rtol, expected_its);
+  rc = iterate(0, refinements > 0 ? rtol_inner : rtol, expected_its);
   if (rc) return rc;
   for (int32_t k = 1; k <= refinements; ++k) {
     rc = init_a(1);

@@ -378,8 +378,9 @@ int smx_pcg_jacobi(int64_t n, int32_t width, int64_t ld, const int32_t* cols,
  *   - x, the CG scalars and the inner products in FP64: each product of two
  *     FP32 values is exact in FP64, the wave sums are FP64 and the sums
  *     across waves and workgroups are compensated (TwoSum);
- *   - iterative refinement: the FP32 solve runs to max(rtol, rtol_inner);
- *     then, up to `refinements` times, the residual b - A x is recomputed
+ *   - iterative refinement: the FP32 solve runs to rtol_inner (below rtol
+ *     leaves room for the FP32 drift of the recurrence residual; above it
+ *     stages the solve); then, up to `refinements` times, the residual b - A x is recomputed
  *     from the FP64 operator (`vals`) and checked against rtol; when it
  *     fails, a correction solve (FP32 again, from the current x) runs to
  *     max(rtol, rtol_inner^(k+1)) of ||b||, the last one to rtol.
@@ -393,7 +394,7 @@ int smx_pcg_jacobi(int64_t n, int32_t width, int64_t ld, const int32_t* cols,
  * (Demag_Lindholm_GPU.py:216-226, 271-281, 569-640), which run in FP64. */
 typedef struct {
   const float* vals32;      /* FP32 iteration operator values */
-  double rtol_inner;        /* tolerance factor of the FP32 solves before the last, (0, 1) */
+  double rtol_inner;        /* tolerance of the first FP32 solve; powers of it stage the corrections */
   int32_t refinements;      /* FP64 residual checks / correction solves, 0..8 */
   int32_t pad;
 } smx_cg_mixed;
