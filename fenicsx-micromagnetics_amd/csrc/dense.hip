@@ -216,7 +216,8 @@ extern "C" int smx_dense_bmatvec(const void* B, int32_t dtype, int64_t n, int64_
     case SMX_F64:
       return launch_gemv<double, 2>(B, n, ldb, x, y, st);
     case SMX_F32:
-      return launch_gemv<float, 2>(B, n, ldb, x, y, st);
+      // 4 rows per workgroup: 150 nm cube (5402 rows), cache evicted, 30.2 -> 28.0 us
+      return launch_gemv<float, 4>(B, n, ldb, x, y, st);
     case SMX_BF16:
       return launch_gemv<uint16_t, 4>(B, n, ldb, x, y, st);
     default:
