@@ -12,6 +12,7 @@
 // are rows of the column-major U slice: 256 B coalesced per rank), reduced in
 // LDS in a fixed order and scattered to the original boundary order.
 #include <cfloat>
+#include <mutex>
 
 #include "smx_common.h"
 
@@ -380,8 +381,10 @@ extern "C" int smx_aca_batched(double* A, const smx_aca_task* tasks, int64_t nta
   return SMX_OK;
 }
 
-extern "C" int smx_hmat_matvec(const smx_hmatrix* H, const double* src, const double* shift,
-                               double shift_scale, double* y, void* stream) {
+namespace {
+
+int hmat_matvec_impl(const smx_hmatrix* H, const double* src, const double* shift,
+                     double shift_scale, double* y, void* stream) {
   SMX_REQUIRE(H, SMX_ERR_VALUE, "null H");
   SMX_REQUIRE(H->nb >= 0 && H->nleaf >= 0 && H->nztask >= 0, SMX_ERR_SHAPE, "negative size");
   if (H->nb == 0) return SMX_OK;
@@ -430,4 +433,60 @@ extern "C" int smx_hmat_matvec(const smx_hmatrix* H, const double* src, const do
     SMX_LAUNCH_CHECK();
   }
   return SMX_OK;
+}
+
+// Scratch of the entry point that allocates nothing: xp, z and ynear of
+// smx_hmat_matvec live in this array instead of caller buffers.  Each GPU the
+// library runs on holds one copy until the process ends.
+constexpr int64_t kScratchDoubles = SMX_HMAT_NOALLOC_MAX_DOUBLES;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device (the host-side name of a __device__ array is not its address).
+int scratch_ptr(double** out) {
+  int dev = 0;
+  SMX_HIP_CHECK(hipGetDevice(&dev));
+  SMX_REQUIRE(dev >= 0 && dev < kMaxDevices, SMX_ERR_VALUE, "device ordinal %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    SMX_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    SMX_REQUIRE(p, SMX_ERR_HIP, "hipGetSymbolAddress returned a null scratch address");
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return SMX_OK;
+}
+
+}  // namespace
+
+extern "C" int smx_hmat_matvec(const smx_hmatrix* H, const double* src, const double* shift,
+                               double shift_scale, double* y, void* stream) {
+  return hmat_matvec_impl(H, src, shift, shift_scale, y, stream);
+}
+
+extern "C" int smx_hmat_matvec_noalloc(const smx_hmatrix* H, int64_t nz, const double* src,
+                                       const double* shift, double shift_scale, double* y,
+                                       void* stream) {
+  SMX_REQUIRE(H, SMX_ERR_VALUE, "null H");
+  SMX_REQUIRE(H->nb >= 0 && nz >= 0, SMX_ERR_SHAPE, "negative size");
+  const int64_t need = H->nb + nz + (H->ynear ? H->nb : 0);
+  SMX_REQUIRE(need <= kScratchDoubles, SMX_ERR_SHAPE,
+              "smx_hmat_matvec_noalloc: the operator needs %lld scratch doubles (nb %lld, %lld z "
+              "entries%s), more than the %lld of the fixed scratch array; use smx_hmat_matvec with "
+              "caller workspaces", (long long)need, (long long)H->nb, (long long)nz,
+              H->ynear ? ", near rows in a launch of their own" : "", (long long)kScratchDoubles);
+  if (H->nb == 0) return SMX_OK;
+  double* p = nullptr;
+  const int rc = scratch_ptr(&p);
+  if (rc) return rc;
+  smx_hmatrix Hs = *H;
+  Hs.xp = p;
+  Hs.z = p + H->nb;
+  Hs.ynear = H->ynear ? p + H->nb + nz : nullptr;
+  return hmat_matvec_impl(&Hs, src, shift, shift_scale, y, stream);
 }

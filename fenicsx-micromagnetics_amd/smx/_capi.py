@@ -97,6 +97,7 @@ _SIGS = {
     "smx_aca_batched": ([P, P, I64, F64, P, P, P, P, P, P], C.c_int),
     "smx_hmat_matvec": ([C.POINTER(HMatrixArgs), P, P, F64, P, P], C.c_int),
     "smx_hmat_matmat_ws_size": ([C.POINTER(HMatrixArgs), I64], I64),
+    "smx_hmat_matvec_noalloc": ([C.POINTER(HMatrixArgs), I64, P, P, F64, P, P], C.c_int),
     "smx_hmat_matmat": ([C.POINTER(HMatrixArgs), C.c_void_p, I64, P, I64, I32, P, I64, P, I64, P], C.c_int),
     "smx_dense_bmatvec": ([P, I32, I64, I64, P, P, P, F64, P, P, P], C.c_int),
     "smx_dense_bmatvec_t": ([P, I32, I64, I64, P, P, P, P], C.c_int),

@@ -446,6 +446,20 @@ class HMatrixGPUPackedFused:
                                                 out.data_ptr(), _capi.stream_ptr()), "smx_hmat_matvec")
         return out
 
+    def matvec_noalloc(self, x_gpu: torch.Tensor, out: torch.Tensor) -> torch.Tensor:
+        """y = B x through smx_hmat_matvec_noalloc: the same kernels with the
+        x, z and near-sum scratch in the library's fixed device array instead
+        of this object's workspaces (for callers that cannot allocate; out is
+        the caller's)."""
+        if x_gpu.shape != (self.nb,) or out.shape != (self.nb,):
+            raise ValueError(f"Expected x.shape == out.shape == {(self.nb,)}.")
+        _capi.require_device(x_gpu, "x")
+        _capi.require_device(out, "out")
+        _capi.check(_capi.lib().smx_hmat_matvec_noalloc(_capi.byref(self._H), self._nz, x_gpu.data_ptr(), None,
+                                                        0.0, out.data_ptr(), _capi.stream_ptr()),
+                    "smx_hmat_matvec_noalloc")
+        return out
+
     def near_tiles(self):
         """The near rows as dense 16-row tiles on the device (pack_near_tiles),
         built on first use; returns the smx_hmat_near_tiles argument."""

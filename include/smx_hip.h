@@ -284,6 +284,25 @@ int smx_hmat_matmat(const smx_hmatrix* H, const smx_hmat_near_tiles* NT, int64_t
 int smx_hmat_matvec(const smx_hmatrix* H, const double* src, const double* shift,
                     double shift_scale, double* y, void* stream);
 
+/* The H-matrix matvec without caller workspaces.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same operator, arguments and result as smx_hmat_matvec (the
+ * same kernels), except that H->xp, H->z and H->ynear are not used: the
+ * gathered x, the z entries and the near sums live in a fixed array in
+ * device memory that the library holds on each GPU it runs on until the
+ * process ends (H->ynear != NULL still selects the near rows in a launch of
+ * their own).  nz is the number of z entries (the sum of the far blocks'
+ * ranks).  Needs nb + nz (+ nb with ynear) <= SMX_HMAT_NOALLOC_MAX_DOUBLES
+ * (8 MiB of doubles; the 83 202-node film needs about 0.62 M), else
+ * SMX_ERR_SHAPE before any launch.  Calls nothing that allocates.  Every
+ * call on one GPU uses the same array, so calls that overlap on one GPU
+ * (other streams or host threads) race; every scratch entry a call reads is
+ * written earlier in the same call, so nothing needs clearing between calls. */
+#define SMX_HMAT_NOALLOC_MAX_DOUBLES (1 << 20)
+int smx_hmat_matvec_noalloc(const smx_hmatrix* H, int64_t nz, const double* src,
+                            const double* shift, double shift_scale, double* y, void* stream);
+
 /* ---------------------------------------------------------------------- *
  * (c) SELL-64 sparse kernels (replace PETSc AIJCUSPARSE MatMult on Div, A_u2_raw,
  * G and K_lin: Demag_Lindholm_GPU.py:570, 632, 662; llg_module_GPU.py:963-964).
