@@ -527,11 +527,55 @@ def measured_traffic(config, what="matvec"):
     counter passes (FETCH_SIZE and WRITE_SIZE of `bench.py --pmc-probe WHAT`,
     each scaled by the same counter's reading for a 256 MiB device copy), or
     None when no pass for this workload is committed."""
-    f = ROOT / "profiles" / ("matvec_traffic.json" if what == "matvec" else "cg_iter_traffic.json")
+    f = ROOT / "profiles" / {"matvec": "matvec_traffic.json", "cg": "cg_iter_traffic.json",
+                             "amg": "amg_iter_traffic.json"}[what]
     if not f.exists():
         return None
     rec = json.loads(f.read_text()).get(config)
     return None if rec is None else int(rec["traffic_bytes_per_launch"])
+
+
+def pmc_probe_amg(config, K=40):
+    """Counter-pass program for the AMG-CG iteration of the u1 solve: the
+    workload's Poisson operator and AMG hierarchy (FP32 values, as in the mixed
+    evaluation) are built, one 256 MiB device copy runs (calibration), then one
+    solve of exactly K iterations (smx_pcg_amg, the initial V-cycle included;
+    the parser drops the solve's first seven launches, its initialisation)."""
+    import numpy as np
+    import torch
+    from smx import _capi
+    from smx import mesh as M
+    from smx.amg import AmgHierarchy
+    from smx.demag import OperatorSet
+    from smx.fem import assemble_fem
+    dev = torch.device("cuda", 0)
+    src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    mesh = M.film_c3() if config == "film" else M.cube_c2()
+    fem = assemble_fem(mesh.X, mesh.tets)
+    ops = OperatorSet(mesh, dev, fem)
+    perm = ops.solver_order()
+    A = ops.compact(fem.stiff, perm)
+    h = AmgHierarchy(ops.compact_host(fem.stiff, perm), dev, fp32=True)
+    b = np.random.default_rng(0).standard_normal(fem.N)
+    b -= b.mean()
+    bd = torch.from_numpy(b).to(dev)
+    bpart = torch.zeros(1, dtype=torch.float64, device=dev)
+    pd = torch.from_numpy(perm.astype(np.int32)).to(dev)
+    x = torch.zeros(fem.N, dtype=torch.float64, device=dev)
+    L = _capi.lib()
+    ws = torch.zeros(int(L.smx_pcg_amg_ws_size(fem.N)), dtype=torch.float64, device=dev)
+    its, rn, sums = C.c_int32(0), C.c_double(0.0), (C.c_double * 1)()
+    L.smx_pcg_amg(fem.N, A.width, A.ld, A.cols.data_ptr(), A.vals.data_ptr(), 0, bd.data_ptr(), bpart.data_ptr(), 1,
+                  1, A.width, A.ld, A.cols.data_ptr(), A.vals.data_ptr(), None, None, x.data_ptr(), 0, pd.data_ptr(),
+                  C.addressof(h.c_levels), h.nlev, h.val_dtype, 0.0, 0.0, K, 10 ** 6, K, ws.data_ptr(),
+                  C.byref(its), C.byref(rn), sums, _capi.stream_ptr())
+    torch.cuda.synchronize()
+    print(json.dumps({"pmc_probe": "amg", "config": config, "N": fem.N, "cg_iterations": K,
+                      "levels": h.describe(), "calibration_copy_bytes": 2 * src.numel() * 4}), flush=True)
 
 
 def pmc_probe_matvec(config, reps=20):
@@ -691,11 +735,10 @@ def build_case(config, dev):
         h64 = EffectiveField(mesh, use_demag=True, operators=ops, demag_kwargs=dict(B_dtype="float64", **common),
                              **mat)
         hmx = EffectiveField(mesh, use_demag=True, operators=ops,
-                             demag_kwargs=dict(B_dtype="float32", geometry=h64.demag_field.opB, cg_mixed=True,
-                                               **common), **mat)
-        h_f64cg = EffectiveField(mesh, use_demag=True, operators=ops,
-                                 demag_kwargs=dict(B_dtype="float32", geometry=h64.demag_field.opB, **common),
-                                 **mat)
+                             demag_kwargs=dict(B_dtype="float32", geometry=h64.demag_field.opB, **common), **mat)
+        h_n1 = EffectiveField(mesh, use_demag=True, operators=ops,
+                              demag_kwargs=dict(B_dtype="float32", geometry=h64.demag_field.opB, cg_mixed=True,
+                                                **common), **mat)
         pool = [torch.from_numpy(vortex_m(mesh.X, seed=s)).to(dev) for s in range(8)]
         work = "cube150nm_30x30x30_kuhn (N=29791, Nb=5402, dense B)"
         extra = {}
@@ -716,9 +759,9 @@ def build_case(config, dev):
         h64 = EffectiveField(mesh, use_demag=True, operators=ops, demag_kwargs=dict(hmatrix_use_fp32=False, **hk),
                              **mat)
         hmx = EffectiveField(mesh, use_demag=True, operators=ops,
-                             demag_kwargs=dict(hmatrix_use_fp32=True, cg_mixed=True, **hk), **mat)
-        h_f64cg = EffectiveField(mesh, use_demag=True, operators=ops,
-                                 demag_kwargs=dict(hmatrix_use_fp32=True, **hk), **mat)
+                             demag_kwargs=dict(hmatrix_use_fp32=True, **hk), **mat)
+        h_n1 = EffectiveField(mesh, use_demag=True, operators=ops,
+                              demag_kwargs=dict(hmatrix_use_fp32=True, cg_mixed=True, **hk), **mat)
         pool = [torch.from_numpy(random_m(mesh.num_nodes, seed=s)).to(dev) for s in range(4)]
         work = "film500x500x10nm_200x200x4_kuhn (N=202005, Nb=83202, H-matrix leaf 64, eta 2, eps 1e-6)"
         md = data.metadata
@@ -727,7 +770,7 @@ def build_case(config, dev):
                  "average_far_rank": round(float(md["average_far_rank"]), 2),
                  "lindholm_pair_evaluations": int(md["lindholm_pair_evaluations"])}
         extra["_data"] = data
-    extra["_h_f64cg"] = h_f64cg
+    extra["_h_n1"] = h_n1
     torch.cuda.synchronize()
     return mesh, mat, h64, hmx, pool, work, extra, time.perf_counter() - t0
 
@@ -844,9 +887,9 @@ def main():
     ap.add_argument("--pc-u2", choices=["gamg", "jacobi", "auto"], default=None,
                     help="preconditioner of the u2 solve when it differs from --pc")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
-    ap.add_argument("--pmc-probe", nargs="?", const="cg", default=None, choices=["cg", "matvec"],
-                    help="short program for rocprofv3 --pmc passes (calibration copy + CG iterations, or "
-                         "+ boundary matvecs)")
+    ap.add_argument("--pmc-probe", nargs="?", const="cg", default=None, choices=["cg", "matvec", "amg"],
+                    help="short program for rocprofv3 --pmc passes: calibration copy + Jacobi CG iterations "
+                         "(cg), + boundary matvecs (matvec) or + AMG-CG iterations (amg)")
     ap.add_argument("--evals-only", action="store_true",
                     help="after set-up run only the mixed-precision evaluations (for kernel traces)")
     ap.add_argument("--multi-rhs-only", action="store_true",
@@ -857,6 +900,9 @@ def main():
     PC_TYPE_U2 = args.pc_u2
     if args.pmc_probe == "matvec":
         pmc_probe_matvec(args.config)
+        return
+    if args.pmc_probe == "amg":
+        pmc_probe_amg(args.config)
         return
     if args.pmc_probe:
         pmc_probe(args.config)
@@ -908,36 +954,51 @@ def main():
     tmx, itsmx = run(hmx, args.steps, args.warmup)
     redos_mx = run.redos
     log(f"mixed {args.steps / tmx:.2f} evals/s ({redos_mx} speculative solve repairs)")
-    h_f64cg = extra["_h_f64cg"]
-    tmf, _ = run(h_f64cg, args.steps, args.warmup)
-    log(f"mixed boundary, FP64 CG {args.steps / tmf:.2f} evals/s")
+    h_n1 = extra["_h_n1"]
+    tn1, _ = run(h_n1, args.steps, args.warmup)
+    redos_n1 = run.redos
+    log(f"mixed with FP32 inner CG and FP64 refinement {args.steps / tn1:.2f} evals/s ({redos_n1} repairs)")
     bd64 = eval_breakdown(h64, pool)
     bdmx = eval_breakdown(hmx, pool)
-    log(f"stage times (ms) fp64 {bd64} mixed {bdmx}")
+    bdn1 = eval_breakdown(h_n1, pool)
+    log(f"stage times (ms) fp64 {bd64} mixed {bdmx} fp32 inner cg {bdn1}")
     r64 = torch.empty_like(out)
     rmx = torch.empty_like(out)
     errs, herr = [], []
+    herr_n1, errs_n1 = [], []
+    rn1 = torch.empty_like(out)
     for mv in pool[:2]:
         h64.llg_rhs_vec(mv, r64)
         hmx.llg_rhs_vec(mv, rmx)
+        h_n1.llg_rhs_vec(mv, rn1)
         errs.append(float(torch.linalg.norm(rmx - r64) / torch.linalg.norm(r64)))
         herr.append(float(torch.linalg.norm(hmx.H_demag_gpu - h64.H_demag_gpu) / torch.linalg.norm(h64.H_demag_gpu)))
-    dmx = hmx.demag_field
+        errs_n1.append(float(torch.linalg.norm(rn1 - r64) / torch.linalg.norm(r64)))
+        herr_n1.append(float(torch.linalg.norm(h_n1.H_demag_gpu - h64.H_demag_gpu)
+                             / torch.linalg.norm(h64.H_demag_gpu)))
+    dn1 = h_n1.demag_field
     krylov64 = (bd64["u1_solve"] + bd64["u2_solve"]) / bd64["total"]
     krylovmx = (bdmx["u1_solve"] + bdmx["u2_solve"]) / bdmx["total"]
     precision = {
-        "mixed": ("FP32 H-matrix factors; FP32 inner CG (vectors and iteration operator) with FP64 x, scalars "
-                  "and inner products (exact products, compensated cross-wave and cross-workgroup sums), then one "
-                  "FP64 iterative-refinement step per solve (FP64 residual check, correction solve if it fails); "
-                  "FP32 AMG values"),
-        "mixed_fp64_cg_evals_per_s": args.steps / tmf,
+        "mixed": ("the value: FP32 boundary operator (H-matrix factors or dense B), FP32 AMG preconditioner "
+                  "values, CG vectors, scalars and inner products FP64"),
+        "fp32_inner_cg": {
+            "what": ("FP32 boundary operator and FP32 inner CG (vectors and iteration operator) with FP64 x, scalars "
+                     "and inner products (exact products, compensated cross-wave and cross-workgroup sums), the "
+                     "FP32 solve to rtol/2, then one FP64 iterative-refinement step per solve (FP64 residual check, "
+                     "correction solve if it fails); FP32 AMG values"),
+            "evals_per_s": args.steps / tn1, "over_fp64": t64 / tn1,
+            "rel_err_hdemag_vs_fp64": max(herr_n1), "rel_err_rhs_vs_fp64": max(errs_n1),
+            "speculative_repairs_in_timed_steps": int(redos_n1),
+            "stage_ms": {k: round(v, 4) for k, v in bdn1.items()},
+            "cg_iterations_before_and_in_last_solve": {"u1": list(dn1._its_split["u1"]),
+                                                       "u2": list(dn1._its_split["u2"])}},
         "stage_ms_fp64": {k: round(v, 4) for k, v in bd64.items()},
         "stage_ms_mixed": {k: round(v, 4) for k, v in bdmx.items()},
         "krylov_share_fp64": krylov64,
         "krylov_share_mixed": krylovmx,
         "amdahl_ratio_if_mixed_matvec_free": bd64["total"] / (bdmx["total"] - bdmx["boundary_matvec"]),
         "amdahl_ratio_if_mixed_krylov_free": bd64["total"] / (bdmx["total"] - bdmx["u1_solve"] - bdmx["u2_solve"]),
-        "cg_iterations_mixed_inner_last": {"u1": list(dmx._its_split["u1"]), "u2": list(dmx._its_split["u2"])},
     }
     cg_t, cg_bytes = cg_iteration_timing(hmx.demag_field)
     mv_cold, mv_warm, mv_bytes, mv_name = matvec_timing(hmx.demag_field)
@@ -959,7 +1020,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 (boundary operator and inner CG f32)",
+        "dtype": "f64 (boundary operator and AMG values f32)",
         "data": "synthetic (seeded)",
         "config": cfg,
         "fp64": {"value": args.steps / t64, "ms_per_step": 1e3 * t64 / args.steps},
@@ -982,8 +1043,8 @@ def main():
                                                    "launches)" if hmx.demag_field.pc_type == "gamg"
                                                    else "cg_iter_kernel (Jacobi CG iteration, u1 operator)"),
                         "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                        "traffic": (measured_traffic(args.config, "cg") if hmx.demag_field.pc_type == "jacobi"
-                                    else None),
+                        "traffic": measured_traffic(args.config, "cg" if hmx.demag_field.pc_type == "jacobi"
+                                                    else "amg"),
                         "algorithmic_bytes": int(cg_bytes), "avg_us": cg_t * 1e6},
     }
     if h64.demag_field.boundary_backend == "dense":
