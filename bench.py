@@ -522,16 +522,17 @@ def extended_paths(cpu_budget=20.0, no_cpu=False):
     print(json.dumps(out), flush=True)
 
 
-def measured_traffic(config, what="matvec"):
+def measured_traffic(config, what="matvec", variant=None):
     """HBM-side bytes per launch (or per boundary matvec) from the committed
     counter passes (FETCH_SIZE and WRITE_SIZE of `bench.py --pmc-probe WHAT`,
     each scaled by the same counter's reading for a 256 MiB device copy), or
-    None when no pass for this workload is committed."""
+    None when no pass for this workload is committed.  `variant` names a launch
+    variant of the same workload (the H-matrix matvec's "near_launch")."""
     f = ROOT / "profiles" / {"matvec": "matvec_traffic.json", "cg": "cg_iter_traffic.json",
                              "amg": "amg_iter_traffic.json"}[what]
     if not f.exists():
         return None
-    rec = json.loads(f.read_text()).get(config)
+    rec = json.loads(f.read_text()).get(config if variant is None else f"{config}:{variant}")
     return None if rec is None else int(rec["traffic_bytes_per_launch"])
 
 
@@ -594,11 +595,20 @@ def pmc_probe_matvec(config, reps=20):
     torch.cuda.synchronize()
     x = torch.randn(d.Nb, dtype=torch.float64, device=dev)
     y = torch.empty_like(x)
-    for _ in range(reps):
-        d.boundary_matvec(x, y)
-    torch.cuda.synchronize()
+    # the H-matrix runs both near-row variants: `reps` matvecs with the near
+    # rows in the leaf launch, then `reps` with them in a launch of their own
+    H = d.B_hmatrix_gpu if d.boundary_backend == "hmatrix" else None
+    modes = (False, True) if H is not None else (None,)
+    for mode in modes:
+        if H is not None:
+            H.set_near_mode(mode)
+        for _ in range(reps):
+            d.boundary_matvec(x, y)
+        torch.cuda.synchronize()
     _, _, nbytes, name = matvec_timing(d, reps=0)
-    print(json.dumps({"pmc_probe": config, "what": "matvec", "kernels": name, "matvecs": reps,
+    print(json.dumps({"pmc_probe": config, "what": "matvec", "matvecs_per_variant": reps,
+                      "variants": (["near rows in the leaf launch", "near rows in their own launch"]
+                                   if H is not None else [name]),
                       "algorithmic_bytes_per_matvec": int(nbytes),
                       "calibration_copy_bytes": 2 * src.numel() * 4}), flush=True)
 
@@ -1002,6 +1012,8 @@ def main():
     }
     cg_t, cg_bytes = cg_iteration_timing(hmx.demag_field)
     mv_cold, mv_warm, mv_bytes, mv_name = matvec_timing(hmx.demag_field)
+    mv_variant = ("near_launch" if hmx.demag_field.boundary_backend == "hmatrix"
+                  and hmx.demag_field.B_hmatrix_gpu.near_in_z else None)
     mv64_cold, _, mv64_bytes, _ = matvec_timing(h64.demag_field)
     ach = cg_bytes / cg_t / 1e9
     mv_ach = mv_bytes / mv_cold / 1e9
@@ -1033,7 +1045,8 @@ def main():
         "amg_levels_u1": hmx.demag_field.amg_u1.describe() if hmx.demag_field.amg_u1 else None,
         "roofline": {"bound": "hbm", "kernel": mv_name, "achieved": mv_ach, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": mv_ach / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(args.config, "matvec"), "algorithmic_bytes": int(mv_bytes),
+                     "traffic": measured_traffic(args.config, "matvec", mv_variant),
+                     "algorithmic_bytes": int(mv_bytes),
                      "avg_us": mv_cold * 1e6, "avg_us_cache_warm": mv_warm * 1e6,
                      "note": "mixed precision boundary operator; timed after a 512 MB read sweep that evicts the "
                              "Infinity Cache",

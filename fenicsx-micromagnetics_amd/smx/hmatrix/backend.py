@@ -360,8 +360,7 @@ class HMatrixGPUPackedFused:
         out = torch.empty_like(x)
         sweep = torch.ones(128 * 1024 * 1024, dtype=torch.float32, device=self.device)   # 512 MB
         for mode in (True, False):
-            self.near_in_z = mode
-            self._H = self._struct(self.fwd, self.gidx_plain)
+            self.set_near_mode(mode)
             for _ in range(2):
                 self.matvec(x, out)
             ts = []
@@ -375,7 +374,12 @@ class HMatrixGPUPackedFused:
                 ts.append(e0.elapsed_time(e1) * 1e-3)
             self.near_mode_seconds[mode] = min(ts)
         del sweep
-        self.near_in_z = self.near_mode_seconds[True] < self.near_mode_seconds[False]
+        self.set_near_mode(self.near_mode_seconds[True] < self.near_mode_seconds[False])
+
+    def set_near_mode(self, near_in_z: bool):
+        """Sum the near rows in a launch of their own (True) or inside the leaf
+        launch (False) from the next matvec on."""
+        self.near_in_z = bool(near_in_z)
         self._H = self._struct(self.fwd, self.gidx_plain)
         if self.gidx_boundary is not None:
             self._Hb = self._struct(self.fwd, self.gidx_boundary)
