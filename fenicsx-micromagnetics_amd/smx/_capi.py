@@ -22,6 +22,7 @@ LINDHOLM_TILE_COLS = 1024
 PCG_DONE, PCG_ITERS, PCG_FAIL, PCG_SUMX, PCG_ITERS0 = 0, 1, 5, 6, 7
 HMAT_Z_GROUP = 8
 HMAT_KW_MAX = 16384
+SVD_KMAX = 128
 
 DTYPE_CODE = {torch.float64: SMX_F64, torch.float32: SMX_F32, torch.bfloat16: SMX_BF16}
 
@@ -95,6 +96,9 @@ _SIGS = {
     "smx_lindholm_fill_batched": ([P, P, I64, P, P, P, P, P, P, P, I64, P, I32, P], C.c_int),
     "smx_dense_finish": ([P, I64, I64, P, P, I32, P], C.c_int),
     "smx_aca_batched": ([P, P, I64, F64, P, P, P, P, P, P], C.c_int),
+    "smx_svd_ws_doubles": ([I32, I32, I32], I64),
+    "smx_svd_batched": ([P, P, I64, F64, P, P, P, P, P, P, P], C.c_int),
+    "smx_rsvd_round": ([P, P, I64, P, F64, I32, P, P, P, P, P, P, P], C.c_int),
     "smx_hmat_matvec": ([C.POINTER(HMatrixArgs), P, P, F64, P, P], C.c_int),
     "smx_hmat_matmat_ws_size": ([C.POINTER(HMatrixArgs), I64], I64),
     "smx_hmat_matvec_noalloc": ([C.POINTER(HMatrixArgs), I64, P, P, F64, P, P], C.c_int),

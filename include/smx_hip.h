@@ -169,6 +169,59 @@ int smx_aca_batched(double* A, const smx_aca_task* tasks, int64_t ntasks, double
                     int32_t* status_out, void* stream);
 
 /* ---------------------------------------------------------------------- *
+ * (a/b) Truncated-SVD and randomized-SVD compression of many dense blocks,
+ * one workgroup per block, no library calls.  Block, U and V layouts as
+ * smx_aca_batched (A row-major m x n at a_off; column r of U, scaled by s_r,
+ * at u_off + r m; row r of V at v_off + r n).  Rank rule of the reference
+ * (compressors.py:402-406): with s descending, rel_r = sqrt(max(||A||^2 -
+ * sum_{i<=r} s_i^2, 0)) / ||A||; rank = first r with rel_r <= rtol, else
+ * min(mr, min(m, n)).  status: 0 factors stored, 2 rank above rcap (nothing
+ * stored), 3 zero block or mr = 0 (rank 0, rel 0), 4 not resolved (rel inf),
+ * 7 (randomized only) the sketch found no rank meeting rtol and s < mr:
+ * call again with a wider sketch.  `work` holds smx_svd_ws_doubles(m, n, s)
+ * doubles per task at w_off.
+ * ---------------------------------------------------------------------- */
+#define SMX_SVD_KMAX 128   /* rows of a Jacobi problem: QR steps / sketch width */
+
+typedef struct {
+  int64_t a_off;
+  int64_t u_off;
+  int64_t v_off;
+  int64_t w_off;   /* doubles into `work` */
+  int64_t om_off;  /* randomized: the n x s test matrix (row-major) in `omega` */
+  int32_t m;
+  int32_t n;
+  int32_t rcap;    /* rank capacity of the U/V slots */
+  int32_t mr;      /* usable rank: min(m, n), or min(max_rank, m, n) */
+  int32_t s;       /* randomized: sketch width of this call (1..SMX_SVD_KMAX) */
+  int32_t pad;
+} smx_svd_task;
+
+/* Workspace doubles of one task: s = 0 for smx_svd_batched, the sketch width
+ * for smx_rsvd_round; -1 for invalid sizes. */
+int64_t smx_svd_ws_doubles(int32_t m, int32_t n, int32_t s);
+
+/* Truncated SVD (replaces truncated_svd_gpu, compressors.py:373-414):
+ * Householder QR with column pivoting of A^T, stopped once the trailing
+ * Frobenius norm is <= 1e-4 rtol ||A||_F (or after SMX_SVD_KMAX steps, if it
+ * is then <= 1e-2 rtol ||A||_F), then a one-sided Jacobi SVD of the leading
+ * rows of the triangular factor: captured energies within that floor squared
+ * of A's, and the reported residual is that of the stored factors.  A block
+ * that does not reach the 1e-2 floor within SMX_SVD_KMAX steps gets status 4.
+ * A is overwritten (rows permuted, reflectors stored). */
+int smx_svd_batched(double* A, const smx_svd_task* tasks, int64_t ntasks, double rtol, double* U,
+                    double* V, double* work, int32_t* rank_out, double* resid_out,
+                    int32_t* status_out, void* stream);
+
+/* One round of the adaptive randomized SVD (replaces the loop body of
+ * randomized_svd_gpu, compressors.py:417-492): Y = A Omega; power_iterations
+ * times Q = orth(Y), Y = A A^T Q; Q = orth(Y); B = Q^T A; SVD of B by
+ * one-sided Jacobi; rank rule over the s values.  A is not modified. */
+int smx_rsvd_round(const double* A, const smx_svd_task* tasks, int64_t ntasks, const double* omega,
+                   double rtol, int32_t power_iterations, double* U, double* V, double* work,
+                   int32_t* rank_out, double* resid_out, int32_t* status_out, void* stream);
+
+/* ---------------------------------------------------------------------- *
  * (b) H-matrix boundary matvec y = B x (replaces HMatrixGPUPackedFused.matvec,
  * lindholm_hmatrix_gpu/backend_fused.py:496-512, with the semantics of
  * HMatrixCPUData.matvec_cpu, storage.py:284-299).  Three launches:

@@ -15,7 +15,10 @@ H-matrix builder.
 * device build with "rsvd" on the 40 nm and 150 nm cubes: every far block
   within eps of its Lindholm block; validate_against_dense matvec <= 10 eps;
 * a device workspace limit (gpu_memory_fraction tiny): every admissible block
-  splits down to the leaves and is kept dense; the operator equals dense B.
+  splits down to the leaves and is kept dense; the operator equals dense B;
+* the compressor kernels on a block needing ~100 QR steps (rank and factors
+  against the exact SVD), on one beyond their 128-row limit (reported
+  unresolved), and bitwise repeatability.
 """
 import numpy as np
 import pytest
@@ -247,3 +250,45 @@ def test_provider_and_reference_call_form(c1, tmp_path):
     assert c.metadata.get("cache_loaded") is True
     with pytest.raises(ValueError):
         build_or_load_hmatrix(None, geo.Xb + 1.0, geo.diag_jump, prov, cfg)
+
+
+def test_device_svd_kernels_large_unresolved_and_deterministic():
+    """A 400 x 300 block (spectrum 0.8^i: rank 62 at 1e-6, about 100 QR steps
+    to the 1e-4 rtol floor): rank equal to the exact SVD's, factors within
+    1e-9 of numpy's truncated SVD, residual identity.  A 300 x 300 block with
+    spectrum 0.95^i needs rank ~270 > 128: both compressors report it
+    unresolved (status 4, infinite residual), so the builder subdivides it.
+    Two truncated-SVD runs give the same bits."""
+    from smx.hmatrix.compressors import rsvd_batched, svd_batched
+    rtol = 1e-6
+    shapes = [(400, 300), (300, 300)]
+    blocks = [_prescribed(400, 300, 0.8 ** np.arange(300), 11), _prescribed(300, 300, 0.95 ** np.arange(300), 12)]
+    exp0 = O.truncated_svd(blocks[0], rtol)
+    rcaps = [300, 300]
+    tasks = _tasks(shapes, rcaps)
+    host = np.concatenate([b.ravel() for b in blocks])
+    outs = []
+    for rep in range(2):
+        A = torch.from_numpy(host).to(DEV)
+        U = torch.zeros(int((tasks["m"] * tasks["rcap"]).sum()), dtype=torch.float64, device=DEV)
+        V = torch.zeros(int((tasks["n"] * tasks["rcap"]).sum()), dtype=torch.float64, device=DEV)
+        ranks, rels, status = svd_batched(A, tasks, U, V, rtol)
+        outs.append((ranks.copy(), U.cpu().numpy(), V.cpu().numpy()))
+    print(f"[svd large] ranks {ranks} (exact {exp0[0].shape[1]}), rels {rels}, status {status}", flush=True)
+    assert ranks[0] == exp0[0].shape[1] and status[0] == 0
+    assert status[1] == 4 and np.isinf(rels[1]) and ranks[1] == 0
+    r, (m, n) = int(ranks[0]), shapes[0]
+    Uf = outs[0][1][:r * m].reshape(r, m).T
+    Vf = outs[0][2][:r * n].reshape(r, n)
+    ref = exp0[0] @ exp0[1]
+    assert np.linalg.norm(Uf @ Vf - ref) <= 1e-9 * np.linalg.norm(ref)
+    true = np.linalg.norm(blocks[0] - Uf @ Vf) / np.linalg.norm(blocks[0])
+    assert abs(rels[0] ** 2 - true ** 2) <= 64 * 400 * np.finfo(float).eps
+    assert np.array_equal(outs[0][1], outs[1][1]) and np.array_equal(outs[0][2], outs[1][2])
+    A = torch.from_numpy(host).to(DEV)
+    U = torch.zeros(int((tasks["m"] * tasks["rcap"]).sum()), dtype=torch.float64, device=DEV)
+    V = torch.zeros(int((tasks["n"] * tasks["rcap"]).sum()), dtype=torch.float64, device=DEV)
+    ranks, rels, status = rsvd_batched(A, tasks, U, V, rtol, np.array([3, 5]))
+    print(f"[rsvd large] ranks {ranks}, rels {rels}, status {status}", flush=True)
+    assert status[0] == 0 and exp0[0].shape[1] <= ranks[0] <= exp0[0].shape[1] + 8 and rels[0] <= rtol
+    assert status[1] == 4 and np.isinf(rels[1])
