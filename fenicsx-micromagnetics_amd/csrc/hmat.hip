@@ -99,6 +99,10 @@ __global__ __launch_bounds__(kAcaThreads) void aca_kernel(double* __restrict__ A
     rel = 0.0;
   } else {
     for (int k = 0; k < T.rcap; ++k) {
+      if (bi < 0 || bi >= (int64_t)m * n) {  // no valid pivot (a block of NaNs)
+        status = 1;
+        break;
+      }
       const int pi = (int)(bi / n);
       const int pj = (int)(bi - (int64_t)pi * n);
       const double piv = R[bi];
@@ -139,6 +143,149 @@ __global__ __launch_bounds__(kAcaThreads) void aca_kernel(double* __restrict__ A
     resid_out[blockIdx.x] = rel;
     status_out[blockIdx.x] = status;
   }
+}
+
+
+// Large blocks: the same full-pivot ACA, each rank step spread over row
+// tiles of kAcaTileRows rows (one workgroup per tile, the tiles of every block
+// of the call in one launch), then one pivot workgroup per block.  Per
+// element the update is the single-workgroup kernel's (unfused multiply and
+// subtract); the first-occurrence argmax does not depend on the tiling, and
+// ||R||_F^2 is summed per tile, then over the tiles in order.
+// Per-block state (8 doubles): normA, rel, done, rank, status, pi, pj, piv.
+constexpr int kAcaTileRows = SMX_ACA_TILE_ROWS;
+enum { AS_NORM = 0, AS_REL, AS_DONE, AS_RANK, AS_STATUS, AS_PI, AS_PJ, AS_PIV, AS_SIZE };
+
+// Step k of every open block: apply R -= u_{k-1} v_{k-1}^T to the tile's rows
+// (k > 0), then the tile's (sum of squares, argmax |R|) partials.
+__global__ __launch_bounds__(kAcaThreads) void aca_tile_kernel(double* __restrict__ A,
+                                                               const smx_aca_task* __restrict__ tasks,
+                                                               const int32_t* __restrict__ tile_task,
+                                                               const int32_t* __restrict__ tile_row0,
+                                                               const double* __restrict__ U,
+                                                               const double* __restrict__ V,
+                                                               const double* __restrict__ state,
+                                                               double* __restrict__ part, int32_t k) {
+  __shared__ double ss[kAcaWaves + 1];
+  __shared__ double sv[kAcaWaves + 1];
+  __shared__ int64_t si[kAcaWaves + 1];
+  const int b = tile_task[blockIdx.x];
+  if (state[(int64_t)b * AS_SIZE + AS_DONE] != 0.0) return;  // uniform
+  const smx_aca_task T = tasks[b];
+  const int m = T.m, n = T.n;
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  const int r0 = tile_row0[blockIdx.x];
+  const int r1 = r0 + kAcaTileRows < m ? r0 + kAcaTileRows : m;
+  double* __restrict__ R = A + T.a_off;
+  const double* __restrict__ u = k > 0 ? U + T.u_off + (int64_t)(k - 1) * m : nullptr;
+  const double* __restrict__ v = k > 0 ? V + T.v_off + (int64_t)(k - 1) * n : nullptr;
+  double s = 0.0, bv = -1.0;
+  int64_t bi = INT64_MAX;
+  for (int r = r0 + wid; r < r1; r += kAcaWaves) {
+    double* row = R + (int64_t)r * n;
+    const double ur = u ? u[r] : 0.0;
+    for (int c = lane; c < n; c += 64) {
+      double nv = row[c];
+      if (u) {
+        nv = __dsub_rn(nv, __dmul_rn(ur, v[c]));
+        row[c] = nv;
+      }
+      s += nv * nv;
+      am_merge(bv, bi, fabs(nv), (int64_t)r * n + c);
+    }
+  }
+  block_sum_argmax(s, bv, bi, ss, sv, si);
+  if (threadIdx.x == 0) {
+    part[3 * (int64_t)blockIdx.x] = s;
+    part[3 * (int64_t)blockIdx.x + 1] = bv;
+    part[3 * (int64_t)blockIdx.x + 2] = (double)bi;
+  }
+}
+
+// Step k of block b: the tiles' partials reduced in tile order, the stopping
+// tests of aca_kernel in its order (zero block; rel <= rtol after k updates;
+// k == rcap; pivot below eps ||A||_F), and, for an open block, u_k = R[:, pj]
+// and v_k = R[pi, :] / pivot into U and V.  A block that stops writes its
+// rank, residual and status and counts itself in done_count.
+__global__ __launch_bounds__(kAcaThreads) void aca_pivot_kernel(double* __restrict__ A,
+                                                                const smx_aca_task* __restrict__ tasks,
+                                                                const int32_t* __restrict__ tile_first,
+                                                                double rtol, double* __restrict__ U,
+                                                                double* __restrict__ V,
+                                                                double* __restrict__ state,
+                                                                const double* __restrict__ part,
+                                                                int32_t k, int32_t* rank_out,
+                                                                double* resid_out, int32_t* status_out,
+                                                                int32_t* done_count) {
+  __shared__ double ss[kAcaWaves + 1];
+  __shared__ double sv[kAcaWaves + 1];
+  __shared__ int64_t si[kAcaWaves + 1];
+  __shared__ double dec[4];
+  const int b = blockIdx.x;
+  double* st = state + (int64_t)b * AS_SIZE;
+  if (st[AS_DONE] != 0.0) return;  // uniform
+  const smx_aca_task T = tasks[b];
+  const int m = T.m, n = T.n;
+  double* __restrict__ R = A + T.a_off;
+  double s = 0.0, bv = -1.0;
+  int64_t bi = INT64_MAX;
+  const int t0 = tile_first[b], t1 = tile_first[b + 1];
+  for (int t = t0 + (int)threadIdx.x; t < t1; t += kAcaThreads) {
+    s += part[3 * (int64_t)t];
+    am_merge(bv, bi, part[3 * (int64_t)t + 1], (int64_t)part[3 * (int64_t)t + 2]);
+  }
+  block_sum_argmax(s, bv, bi, ss, sv, si);
+  if (threadIdx.x == 0) {
+    double normA = st[AS_NORM], rel = k == 0 ? 1.0 : st[AS_REL];
+    int rank = k, status = -1;
+    if (k == 0) {
+      normA = sqrt(s);
+      st[AS_NORM] = normA;
+      if (normA == 0.0 || T.rcap == 0) {
+        status = 3;
+        rel = 0.0;
+      }
+    } else {
+      rel = sqrt(s) / normA;
+      if (rel <= rtol) status = 0;
+    }
+    if (status < 0 && k == T.rcap) status = 2;
+    double piv = 0.0;
+    if (status < 0) {
+      // no valid pivot (a block of NaNs) stops as a pivot break
+      piv = (bi >= 0 && bi < (int64_t)m * n) ? R[bi] : 0.0;
+      if (!(fabs(piv) > DBL_EPSILON * normA)) status = 1;
+    }
+    st[AS_REL] = rel;
+    st[AS_RANK] = (double)rank;
+    if (status >= 0) {
+      st[AS_STATUS] = (double)status;
+      st[AS_DONE] = 1.0;
+      rank_out[b] = rank;
+      resid_out[b] = rel;
+      status_out[b] = status;
+      atomicAdd(done_count, 1);
+    }
+    dec[0] = status >= 0 ? 1.0 : 0.0;
+    dec[1] = (double)(bi / n);
+    dec[2] = (double)(bi - (bi / n) * n);
+    dec[3] = piv;
+  }
+  __syncthreads();
+  if (dec[0] != 0.0) return;  // uniform
+  const int pi = (int)dec[1], pj = (int)dec[2];
+  const double piv = dec[3];
+  double* __restrict__ u = U + T.u_off + (int64_t)k * m;
+  double* __restrict__ v = V + T.v_off + (int64_t)k * n;
+  for (int r = threadIdx.x; r < m; r += kAcaThreads) u[r] = R[(int64_t)r * n + pj];
+  for (int c = threadIdx.x; c < n; c += kAcaThreads) v[c] = R[(int64_t)pi * n + c] / piv;
+}
+
+__global__ void aca_state_init_kernel(double* __restrict__ state, int64_t nblocks, int32_t* done_count) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nblocks * AS_SIZE) state[i] = 0.0;
+  if (i == 0) *done_count = 0;
 }
 
 __global__ void hm_gather_kernel(int64_t nb, const double* __restrict__ src,
@@ -378,6 +525,56 @@ extern "C" int smx_aca_batched(double* A, const smx_aca_task* tasks, int64_t nta
   hipLaunchKernelGGL(aca_kernel, dim3((unsigned)ntasks), dim3(kAcaThreads), 0,
                      smx::as_stream(stream), A, tasks, rtol, U, V, rank_out, resid_out, status_out);
   SMX_LAUNCH_CHECK();
+  return SMX_OK;
+}
+
+
+extern "C" int64_t smx_aca_large_ws_doubles(int64_t ntasks, int64_t ntiles) {
+  if (ntasks < 0 || ntiles < 0) return -1;
+  return ntasks * AS_SIZE + 3 * ntiles + 2;
+}
+
+extern "C" int smx_aca_large(double* A, const smx_aca_task* tasks, int64_t ntasks, const int32_t* tile_task,
+                             const int32_t* tile_row0, const int32_t* tile_first, int64_t ntiles,
+                             int32_t max_rcap, double rtol, double* U, double* V, int32_t* rank_out,
+                             double* resid_out, int32_t* status_out, double* work, int64_t work_doubles,
+                             void* stream) {
+  SMX_REQUIRE(ntasks >= 0 && ntiles >= 0 && max_rcap >= 0, SMX_ERR_SHAPE, "negative size");
+  if (ntasks == 0) return SMX_OK;
+  SMX_REQUIRE(ntasks <= 0x7fffffffLL && ntiles <= 0x7fffffffLL, SMX_ERR_SHAPE, "too many tasks or tiles");
+  SMX_REQUIRE(A && tasks && tile_task && tile_row0 && tile_first && U && V && rank_out && resid_out &&
+                  status_out && work,
+              SMX_ERR_VALUE, "null pointer argument");
+  SMX_REQUIRE(work_doubles >= smx_aca_large_ws_doubles(ntasks, ntiles), SMX_ERR_OOM,
+              "workspace of %lld doubles, %lld needed", (long long)work_doubles,
+              (long long)smx_aca_large_ws_doubles(ntasks, ntiles));
+  hipStream_t st = smx::as_stream(stream);
+  double* state = work;
+  double* part = work + ntasks * AS_SIZE;
+  int32_t* done = reinterpret_cast<int32_t*>(part + 3 * ntiles);
+  const int64_t ninit = ntasks * AS_SIZE;
+  hipLaunchKernelGGL(aca_state_init_kernel, dim3((unsigned)((ninit + 255) / 256)), dim3(256), 0, st, state,
+                     ntasks, done);
+  SMX_LAUNCH_CHECK();
+  int32_t hdone = 0;
+  constexpr int kCheckEvery = 8;
+  for (int32_t k = 0; k <= max_rcap; ++k) {
+    if (ntiles > 0) {
+      hipLaunchKernelGGL(aca_tile_kernel, dim3((unsigned)ntiles), dim3(kAcaThreads), 0, st, A, tasks, tile_task,
+                         tile_row0, U, V, state, part, k);
+      SMX_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(aca_pivot_kernel, dim3((unsigned)ntasks), dim3(kAcaThreads), 0, st, A, tasks, tile_first,
+                       rtol, U, V, state, part, k, rank_out, resid_out, status_out, done);
+    SMX_LAUNCH_CHECK();
+    if ((k + 1) % kCheckEvery == 0 || k == max_rcap) {
+      SMX_HIP_CHECK(hipMemcpyAsync(&hdone, done, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+      SMX_HIP_CHECK(hipStreamSynchronize(st));
+      if (hdone >= ntasks) break;
+    }
+  }
+  SMX_REQUIRE(hdone >= ntasks, SMX_ERR_VALUE, "%d of %lld blocks still open after %d steps (max_rcap too small)",
+              hdone, (long long)ntasks, max_rcap + 1);
   return SMX_OK;
 }
 

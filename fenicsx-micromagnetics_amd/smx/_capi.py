@@ -23,6 +23,7 @@ PCG_DONE, PCG_ITERS, PCG_FAIL, PCG_SUMX, PCG_ITERS0 = 0, 1, 5, 6, 7
 HMAT_Z_GROUP = 8
 HMAT_KW_MAX = 16384
 SVD_KMAX = 128
+ACA_TILE_ROWS = 64
 
 DTYPE_CODE = {torch.float64: SMX_F64, torch.float32: SMX_F32, torch.bfloat16: SMX_BF16}
 
@@ -96,6 +97,8 @@ _SIGS = {
     "smx_lindholm_fill_batched": ([P, P, I64, P, P, P, P, P, P, P, I64, P, I32, P], C.c_int),
     "smx_dense_finish": ([P, I64, I64, P, P, I32, P], C.c_int),
     "smx_aca_batched": ([P, P, I64, F64, P, P, P, P, P, P], C.c_int),
+    "smx_aca_large_ws_doubles": ([I64, I64], I64),
+    "smx_aca_large": ([P, P, I64, P, P, P, I64, I32, F64, P, P, P, P, P, P, I64, P], C.c_int),
     "smx_svd_ws_doubles": ([I32, I32, I32], I64),
     "smx_svd_batched": ([P, P, I64, F64, P, P, P, P, P, P, P], C.c_int),
     "smx_rsvd_round": ([P, P, I64, P, F64, I32, P, P, P, P, P, P, P], C.c_int),

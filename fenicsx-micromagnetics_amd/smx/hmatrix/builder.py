@@ -54,6 +54,10 @@ FILL_TASK_DTYPE = np.dtype([("out_off", np.int64), ("ld_row", np.int64), ("ld_co
 ACA_TASK_DTYPE = np.dtype([("a_off", np.int64), ("u_off", np.int64), ("v_off", np.int64),
                            ("m", np.int32), ("n", np.int32), ("rcap", np.int32), ("pad", np.int32)])
 assert FILL_TASK_DTYPE.itemsize == 56 and ACA_TASK_DTYPE.itemsize == 40
+# blocks of at least this many entries (16 MiB) are compressed with every ACA
+# rank step spread over 64-row tiles (smx_aca_large)
+ACA_LARGE_ENTRIES = 1 << 21
+ACA_TILE_ROWS = _capi.ACA_TILE_ROWS
 
 
 @dataclass
@@ -198,16 +202,45 @@ class DeviceBuildOps:
         torch.cuda.current_stream().synchronize()
 
     def aca(self, A, tasks: np.ndarray, U, V, rtol: float):
+        """Full-pivot ACA of every task's block: blocks of at least
+        ACA_LARGE_ENTRIES entries through smx_aca_large (each rank step
+        spread over 64-row tiles, one workgroup each), the others through
+        smx_aca_batched (one workgroup per block)."""
         dev = self.device
         nt = int(tasks.size)
-        tasks_d = to_dev(tasks.view(np.uint8), dev)
-        rank_d = torch.empty(nt, dtype=torch.int32, device=dev)
-        res_d = torch.empty(nt, dtype=torch.float64, device=dev)
-        st_d = torch.empty(nt, dtype=torch.int32, device=dev)
-        _capi.check(_capi.lib().smx_aca_batched(A.data_ptr(), tasks_d.data_ptr(), nt, float(rtol), U.data_ptr(),
-                                                V.data_ptr(), rank_d.data_ptr(), res_d.data_ptr(),
-                                                st_d.data_ptr(), _capi.stream_ptr()), "smx_aca_batched")
-        return rank_d.cpu().numpy(), res_d.cpu().numpy(), st_d.cpu().numpy()
+        L = _capi.lib()
+        ranks, rels, status = np.zeros(nt, np.int32), np.zeros(nt), np.zeros(nt, np.int32)
+        mn = tasks["m"].astype(np.int64) * tasks["n"].astype(np.int64)
+        for ids, large in ((np.flatnonzero(mn < ACA_LARGE_ENTRIES), False),
+                           (np.flatnonzero(mn >= ACA_LARGE_ENTRIES), True)):
+            if ids.size == 0:
+                continue
+            sub = np.ascontiguousarray(tasks[ids])
+            nb = int(sub.size)
+            tasks_d = to_dev(sub.view(np.uint8), dev)
+            rank_d = torch.empty(nb, dtype=torch.int32, device=dev)
+            res_d = torch.empty(nb, dtype=torch.float64, device=dev)
+            st_d = torch.empty(nb, dtype=torch.int32, device=dev)
+            if not large:
+                _capi.check(L.smx_aca_batched(A.data_ptr(), tasks_d.data_ptr(), nb, float(rtol), U.data_ptr(),
+                                              V.data_ptr(), rank_d.data_ptr(), res_d.data_ptr(), st_d.data_ptr(),
+                                              _capi.stream_ptr()), "smx_aca_batched")
+            else:
+                ntile = (sub["m"].astype(np.int64) + ACA_TILE_ROWS - 1) // ACA_TILE_ROWS
+                first = np.concatenate([[0], np.cumsum(ntile)]).astype(np.int32)
+                tile_task = np.repeat(np.arange(nb, dtype=np.int32), ntile)
+                tile_row0 = (ACA_TILE_ROWS * (np.arange(int(ntile.sum())) - np.repeat(first[:-1], ntile))
+                             ).astype(np.int32)
+                nws = int(L.smx_aca_large_ws_doubles(nb, int(ntile.sum())))
+                work = torch.empty(nws, dtype=torch.float64, device=dev)
+                keep = [to_dev(a, dev) for a in (tile_task, tile_row0, first)]
+                _capi.check(L.smx_aca_large(A.data_ptr(), tasks_d.data_ptr(), nb, keep[0].data_ptr(),
+                                            keep[1].data_ptr(), keep[2].data_ptr(), int(ntile.sum()),
+                                            int(sub["rcap"].max()), float(rtol), U.data_ptr(), V.data_ptr(),
+                                            rank_d.data_ptr(), res_d.data_ptr(), st_d.data_ptr(), work.data_ptr(),
+                                            nws, _capi.stream_ptr()), "smx_aca_large")
+            ranks[ids], rels[ids], status[ids] = rank_d.cpu().numpy(), res_d.cpu().numpy(), st_d.cpu().numpy()
+        return ranks, rels, status
 
     def compress(self, method, A, tasks, U, V, rtol, max_rank, seeds, cfg):
         if method == "fullaca":

@@ -18,7 +18,9 @@
  *     calling host thread on
  *     hipStreamSynchronize(stream) between batches of iterations, because it
  *     returns the iteration count and residual in host memory; their
- *     speculative form (check_every < 0) returns without waiting.
+ *     speculative form (check_every < 0) returns without waiting.  The
+ *     tiled ACA for large blocks (smx_aca_large) waits likewise every 8 rank
+ *     steps.
  *   - Results are deterministic (bitwise, run to run): reductions sum in a
  *     fixed order, and no floating-point atomics are used.  The one atomic
  *     accumulation, the Lindholm row assembly, adds 64-bit fixed-point
@@ -167,6 +169,28 @@ typedef struct {
 int smx_aca_batched(double* A, const smx_aca_task* tasks, int64_t ntasks, double rtol,
                     double* U, double* V, int32_t* rank_out, double* resid_out,
                     int32_t* status_out, void* stream);
+
+/* The same ACA for large blocks, each rank step spread over row tiles of
+ * SMX_ACA_TILE_ROWS rows: one launch over the tiles of every open block
+ * (rank-1 update, partial ||R||_F^2 and first argmax |R|), then one pivot
+ * workgroup per block (the partials in tile order, the stopping tests of
+ * smx_aca_batched in its order, the next u and v).  Same per-element
+ * arithmetic and pivots as smx_aca_batched; ||R||_F is summed per tile.
+ * Tile t covers rows tile_row0[t] .. +SMX_ACA_TILE_ROWS of block
+ * tile_task[t]; block b owns tiles tile_first[b] .. tile_first[b+1]-1
+ * (consecutive, rows ascending).  max_rcap >= every task's rcap.  Waits on
+ * `stream` every 8 rank steps to read how many blocks are done (the waiting
+ * form, as the CG solvers).  Replaces the per-step rank1_residual_update
+ * loop of fullaca_dense_gpu (lindholm_hmatrix_gpu/compressors.py:251-372)
+ * for blocks too large for one workgroup.  work:
+ * smx_aca_large_ws_doubles(ntasks, ntiles) doubles. */
+#define SMX_ACA_TILE_ROWS 64
+int64_t smx_aca_large_ws_doubles(int64_t ntasks, int64_t ntiles);
+int smx_aca_large(double* A, const smx_aca_task* tasks, int64_t ntasks, const int32_t* tile_task,
+                  const int32_t* tile_row0, const int32_t* tile_first, int64_t ntiles,
+                  int32_t max_rcap, double rtol, double* U, double* V, int32_t* rank_out,
+                  double* resid_out, int32_t* status_out, double* work, int64_t work_doubles,
+                  void* stream);
 
 /* ---------------------------------------------------------------------- *
  * (a/b) Truncated-SVD and randomized-SVD compression of many dense blocks,

@@ -292,3 +292,82 @@ def test_device_svd_kernels_large_unresolved_and_deterministic():
     print(f"[rsvd large] ranks {ranks}, rels {rels}, status {status}", flush=True)
     assert status[0] == 0 and exp0[0].shape[1] <= ranks[0] <= exp0[0].shape[1] + 8 and rels[0] <= rtol
     assert status[1] == 4 and np.isinf(rels[1])
+
+
+@pytest.mark.parametrize("compressor", ["fullaca", "svd", "rsvd"])
+def test_device_build_max_rank_zero_keeps_blocks_dense(c1, compressor):
+    """max_rank = 0: every admissible block compresses to rank 0 on the
+    device; the builder keeps it as a dense rejection (builder.py:343-371),
+    so the operator equals dense B."""
+    from smx.hmatrix import HMatrixBuildConfig, build_hmatrix_device
+    m, geo, bd, B = c1
+    cfg = HMatrixBuildConfig(epsilon=1e-6, leaf_size=32, max_rank=0, compressor=compressor)
+    data = build_hmatrix_device(geo, cfg, DEV)
+    err = _matvec_err(data, B, geo)
+    print(f"[max_rank 0 {compressor}] far {len(data.far_blocks)}, matvec vs dense {err:.2e}", flush=True)
+    assert len(data.far_blocks) == 0 and err <= 1e-13
+
+
+def test_tiled_aca_matches_single_workgroup_aca():
+    """smx_aca_large (each rank step over 64-row tiles) against
+    smx_aca_batched (one workgroup per block) on Lindholm blocks of the
+    150 nm cube (one of 1600 x 1500, above the builder's large-block size),
+    an all-zero block and a block capped below its rank: the same ranks,
+    statuses and factors bit for bit (same pivots, same per-element update),
+    residuals to the rounding of the norm sums."""
+    import ctypes as C
+    from smx import _capi
+    from smx import mesh as M
+    from smx.boundary import build_boundary_geometry
+    from smx.hmatrix import LindholmEntryProviderGPU
+    from smx.hmatrix.builder import ACA_TASK_DTYPE, ACA_TILE_ROWS
+    m = M.cube_c2()
+    geo = build_boundary_geometry(m.X, m.tets)
+    prov = LindholmEntryProviderGPU.from_opB(geo)
+    # boundary nodes sorted along x: the first and last slabs are far apart
+    order = np.argsort(geo.Xb[:, 0], kind="stable")
+    shapes = [(order[:1600], order[-1500:]), (order[:300], order[-700:]), (order[100:400], order[-260:]),
+              (order[:90], order[-50:])]
+    blocks = [prov.fill_block_device(r, c).reshape(-1) for r, c in shapes]
+    blocks[3] = torch.zeros_like(blocks[3])
+    t = np.zeros(len(shapes), dtype=ACA_TASK_DTYPE)
+    t["m"] = [r.size for r, _ in shapes]
+    t["n"] = [c.size for _, c in shapes]
+    t["rcap"] = [200, 200, 4, 50]
+    t["a_off"] = np.concatenate([[0], np.cumsum(t["m"].astype(np.int64) * t["n"])[:-1]])
+    t["u_off"] = np.concatenate([[0], np.cumsum(t["m"].astype(np.int64) * t["rcap"])[:-1]])
+    t["v_off"] = np.concatenate([[0], np.cumsum(t["n"].astype(np.int64) * t["rcap"])[:-1]])
+    L = _capi.lib()
+    rtol = 1e-6
+    res = []
+    for large in (False, True):
+        A = torch.cat(blocks).clone()
+        U = torch.zeros(int((t["m"] * t["rcap"]).sum()), dtype=torch.float64, device=DEV)
+        V = torch.zeros(int((t["n"] * t["rcap"]).sum()), dtype=torch.float64, device=DEV)
+        td = torch.from_numpy(t.view(np.uint8).copy()).to(DEV)
+        rk = torch.zeros(len(shapes), dtype=torch.int32, device=DEV)
+        rl = torch.zeros(len(shapes), dtype=torch.float64, device=DEV)
+        st = torch.zeros(len(shapes), dtype=torch.int32, device=DEV)
+        if not large:
+            _capi.check(L.smx_aca_batched(A.data_ptr(), td.data_ptr(), len(shapes), rtol, U.data_ptr(), V.data_ptr(),
+                                          rk.data_ptr(), rl.data_ptr(), st.data_ptr(), _capi.stream_ptr()))
+        else:
+            ntile = (t["m"].astype(np.int64) + ACA_TILE_ROWS - 1) // ACA_TILE_ROWS
+            first = np.concatenate([[0], np.cumsum(ntile)]).astype(np.int32)
+            tt = np.repeat(np.arange(len(shapes), dtype=np.int32), ntile)
+            r0 = (ACA_TILE_ROWS * (np.arange(int(ntile.sum())) - np.repeat(first[:-1], ntile))).astype(np.int32)
+            dd = [torch.from_numpy(a).to(DEV) for a in (tt, r0, first)]
+            nws = int(L.smx_aca_large_ws_doubles(len(shapes), int(ntile.sum())))
+            work = torch.empty(nws, dtype=torch.float64, device=DEV)
+            _capi.check(L.smx_aca_large(A.data_ptr(), td.data_ptr(), len(shapes), dd[0].data_ptr(), dd[1].data_ptr(),
+                                        dd[2].data_ptr(), int(ntile.sum()), int(t["rcap"].max()), rtol,
+                                        U.data_ptr(), V.data_ptr(), rk.data_ptr(), rl.data_ptr(), st.data_ptr(),
+                                        work.data_ptr(), nws, _capi.stream_ptr()))
+        torch.cuda.synchronize()
+        res.append((rk.cpu().numpy(), rl.cpu().numpy(), st.cpu().numpy(), U.cpu().numpy(), V.cpu().numpy()))
+    (r0_, l0, s0, U0, V0), (r1_, l1, s1, U1, V1) = res
+    print(f"[tiled aca] ranks {r0_} / {r1_}, status {s0} / {s1}, rel {l0} / {l1}", flush=True)
+    assert np.array_equal(r0_, r1_) and np.array_equal(s0, s1)
+    assert s0[2] == 2 and r0_[2] == 4 and s0[3] == 3 and r0_[0] > 4
+    assert np.array_equal(U0, U1) and np.array_equal(V0, V1)
+    assert np.allclose(l0, l1, rtol=1e-13, atol=0)
