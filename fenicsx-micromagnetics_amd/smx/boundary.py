@@ -105,9 +105,10 @@ class BoundaryGeometry:
         np.add.at(ptr, lid + 1, 1)
         return np.cumsum(ptr), tri[order]
 
-    def relevant_triangles(self, cols: np.ndarray, inc=None):
+    def relevant_triangles(self, cols: np.ndarray, inc=None, scratch=None):
         """Triangles touching at least one of ``cols`` (ascending) and the local
-        column of each vertex or -1 (entry_provider.py:187-223)."""
+        column of each vertex or -1 (entry_provider.py:187-223).  ``scratch``:
+        an Nb-array of -1 to use as the column map (left as -1 on return)."""
         cols = np.asarray(cols, dtype=np.int64)
         ptr, tri = inc if inc is not None else self.incidence()
         counts = ptr[cols + 1] - ptr[cols]
@@ -115,9 +116,12 @@ class BoundaryGeometry:
             return np.empty(0, np.int32), np.empty((0, 3), np.int32)
         idx = np.repeat(ptr[cols], counts) + (np.arange(counts.sum()) - np.repeat(np.cumsum(counts) - counts, counts))
         rel = np.unique(tri[idx])
-        col_of = np.full(self.Nb, -1, dtype=np.int64)
+        col_of = scratch if scratch is not None else np.full(self.Nb, -1, dtype=np.int64)
         col_of[cols] = np.arange(cols.size)
-        return rel.astype(np.int32), col_of[self.tri_lid[rel]].astype(np.int32)
+        out = col_of[self.tri_lid[rel]].astype(np.int32)
+        if scratch is not None:
+            col_of[cols] = -1
+        return rel.astype(np.int32), out
 
 
 def build_boundary_geometry(X: np.ndarray, tets: np.ndarray, incident_policy: str = "skip") -> BoundaryGeometry:

@@ -139,6 +139,19 @@ class DeviceBuildOps:
         self.inc = geo.incidence()
         self.evaluations = 0
 
+    def relevant(self, cols: np.ndarray):
+        """Relevant triangles of a column tile, cached per build by the tile's
+        columns (a cluster's tiles recur as the columns of many blocks)."""
+        cache = self.__dict__.setdefault("_rel_cache", {})
+        key = (cols.size, hash(cols.tobytes()))
+        hit = cache.get(key)
+        if hit is not None and np.array_equal(hit[0], cols):
+            return hit[1], hit[2]
+        scratch = self.__dict__.setdefault("_col_of", np.full(self.geo.Nb, -1, dtype=np.int64))
+        rel, rc = self.geo.relevant_triangles(cols, self.inc, scratch=scratch)
+        cache[key] = (cols.copy(), rel, rc)
+        return rel, rc
+
     def alloc(self, n: int):
         return torch.empty(max(int(n), 1), dtype=torch.float64, device=self.device)
 
@@ -171,7 +184,7 @@ class DeviceBuildOps:
             rows_parts.append(rows)
             for t0 in range(0, n, T):
                 tc = cols[t0:t0 + T]
-                rel, rc = self.geo.relevant_triangles(tc, self.inc)
+                rel, rc = DeviceBuildOps.relevant(self, tc)
                 tasks.append((int(offsets[b]) + t0, n, 1, nrows_tot, nrel_tot, m, rel.size, tc.size, 0))
                 rel_parts.append(rel)
                 rc_parts.append(rc.reshape(-1))
