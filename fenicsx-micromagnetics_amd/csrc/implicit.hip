@@ -42,9 +42,12 @@ __device__ __forceinline__ void field_self_diag(const smx_field_args& a, int64_t
         d[2] += a.ex_vals[e];
       }
       if (a.dmi_vals) {
-        d[0] += a.dmi_vals[9 * e];
-        d[1] += a.dmi_vals[9 * e + 4];
-        d[2] += a.dmi_vals[9 * e + 8];
+        const int nc = a.dmi_ncomp > 0 ? a.dmi_ncomp : 9;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const int o = a.dmi_ncomp > 0 ? a.dmi_off[4 * c] : 4 * c;  // diagonal component (c, c)
+          if (o >= 0) d[c] += a.dmi_vals[(int64_t)nc * e + o];
+        }
       }
     }
   }

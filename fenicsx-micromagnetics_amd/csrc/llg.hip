@@ -79,6 +79,28 @@ __device__ __forceinline__ void sot_p(const smx_field_args& a, int64_t k, double
   }
 }
 
+// Row i of the DMI block at graph slot e times (v0, v1, v2): the components
+// present in the layout, in column order, the first as a product and the
+// rest by fma (absent components are exact zeros), so the full and the
+// compact layouts give the same bits.  0 when the row has no component.
+__device__ __forceinline__ double dmi_row(const smx_field_args& a, int64_t e, int i, double v0, double v1,
+                                          double v2) {
+  const int nc = a.dmi_ncomp > 0 ? a.dmi_ncomp : 9;
+  const double* __restrict__ D = a.dmi_vals + (int64_t)nc * e;
+  const double v[3] = {v0, v1, v2};
+  double t = 0.0;
+  bool any = false;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const int o = a.dmi_ncomp > 0 ? a.dmi_off[3 * i + j] : 3 * i + j;
+    if (o >= 0) {
+      t = any ? fma(D[o], v[j], t) : __dmul_rn(D[o], v[j]);
+      any = true;
+    }
+  }
+  return t;
+}
+
 __global__ __launch_bounds__(kBlock) void field_rhs_kernel(smx_field_args a) {
   const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (k >= a.n) return;
@@ -110,10 +132,9 @@ __global__ __launch_bounds__(kBlock) void field_rhs_kernel(smx_field_args a) {
         zz = fma(tv, m2, zz);
       }
       if (a.dmi_vals) {
-        const double* D = a.dmi_vals + 9 * e;
-        hx += D[0] * m0 + D[1] * m1 + D[2] * m2;
-        hy += D[3] * m0 + D[4] * m1 + D[5] * m2;
-        hz += D[6] * m0 + D[7] * m1 + D[8] * m2;
+        hx += dmi_row(a, e, 0, m0, m1, m2);
+        hy += dmi_row(a, e, 1, m0, m1, m2);
+        hz += dmi_row(a, e, 2, m0, m1, m2);
       }
     }
     if (a.grad_vals) {
@@ -236,10 +257,9 @@ __global__ __launch_bounds__(kBlock) void llg_jvp_kernel(smx_field_args a,
         zmz = fma(tv, a.m[3 * c + 2], zmz);
       }
       if (a.dmi_vals) {
-        const double* D = a.dmi_vals + 9 * e;
-        hvx += D[0] * v0 + D[1] * v1 + D[2] * v2;
-        hvy += D[3] * v0 + D[4] * v1 + D[5] * v2;
-        hvz += D[6] * v0 + D[7] * v1 + D[8] * v2;
+        hvx += dmi_row(a, e, 0, v0, v1, v2);
+        hvy += dmi_row(a, e, 1, v0, v1, v2);
+        hvz += dmi_row(a, e, 2, v0, v1, v2);
       }
     }
   }

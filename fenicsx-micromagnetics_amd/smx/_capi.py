@@ -63,6 +63,7 @@ class FieldArgs(C.Structure):
         ("cub_pref", F64), ("cub_axes", P), ("cub_axes_u", F64 * 9),
         ("stt_vals", P), ("stt_pref", F64), ("stt_a", F64), ("stt_b", F64),
         ("sot_p", P), ("sot_p_u", F64 * 3), ("sot_fl", F64), ("sot_dl", F64),
+        ("dmi_ncomp", I32), ("dmi_off", I32 * 9),
     ]
 
 

@@ -107,8 +107,16 @@ class EffectiveField:
         if abs(self.D_bulk) > 0.0:
             db = dmi_bulk_values(X, tets, fem, self.D_bulk, self.Ms)
             dmi = db if dmi is None else dmi + db
+        self.dmi_off = np.arange(9, dtype=np.int32)
         if dmi is not None:
-            self.dmi = self.ops.graph_values(dmi.reshape(-1, 9), 9)
+            # only the components that are nonzero somewhere are stored
+            # (interfacial DMI along z: 5 of 9; bulk DMI: the 6 off-diagonal)
+            flat = dmi.reshape(-1, 9)
+            present = np.flatnonzero(np.any(flat != 0.0, axis=0))
+            if present.size:
+                self.dmi_off = np.full(9, -1, dtype=np.int32)
+                self.dmi_off[present] = np.arange(present.size, dtype=np.int32)
+                self.dmi = self.ops.graph_values(np.ascontiguousarray(flat[:, present]), int(present.size))
 
         # uniaxial anisotropy: nodal axis normalized per node (Anisotropy_GPU.py:123-130)
         self.ani_pref = 0.0
@@ -215,6 +223,9 @@ class EffectiveField:
         a.cols = _capi.ptr(self.ops.graph.cols)
         a.ex_vals = _capi.ptr(self.ex.vals) if self.ex is not None else None
         a.dmi_vals = _capi.ptr(self.dmi.vals) if self.dmi is not None else None
+        if self.dmi is not None:
+            a.dmi_ncomp = int(np.count_nonzero(self.dmi_off >= 0))
+            a.dmi_off = (C.c_int32 * 9)(*(int(o) for o in self.dmi_off))
         if with_demag and self.demag_field is not None:
             d = self.demag_field
             a.grad_vals = _capi.ptr(d.G.vals)

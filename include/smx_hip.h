@@ -587,7 +587,7 @@ typedef struct {
   const int64_t* slice_ptr;  /* node graph (shared by ex/dmi/grad) */
   const int32_t* cols;
   const double* ex_vals;     /* k=1 exchange (prefactor folded), or NULL */
-  const double* dmi_vals;    /* k=9 DMI blocks (prefactor folded), or NULL */
+  const double* dmi_vals;    /* DMI blocks (prefactor folded; layout: dmi_ncomp), or NULL */
   const double* grad_vals;   /* k=3 demag gradient G, or NULL */
   const double* u1;          /* N */
   const double* u2;          /* N */
@@ -617,6 +617,15 @@ typedef struct {
   double sot_p_u[3];
   double sot_fl;             /* -gamma (H_fl - alpha H_dl)/(1+alpha^2) */
   double sot_dl;             /* -gamma (H_dl + alpha H_fl)/(1+alpha^2) */
+  /* DMI value layout: dmi_ncomp values per graph slot (k = dmi_ncomp in the
+   * SELL convention); component q = 3 i + j of the 3x3 block (field
+   * component i, m component j) at dmi_off[q], absent (exactly zero) when
+   * dmi_off[q] < 0.  dmi_ncomp == 0: all nine, in order.  Interfacial DMI
+   * with n0 along z needs 5 (xz, yz, zx, zy, zz), bulk DMI 6 (no diagonal).
+   * Either layout gives the same bits: a row's present components are taken
+   * in column order, the first as a product and the rest by fma. */
+  int32_t dmi_ncomp;
+  int32_t dmi_off[9];
 } smx_field_args;
 
 int smx_field_llg_rhs(const smx_field_args* a, void* stream);
