@@ -929,7 +929,7 @@ def main():
         extended_paths(args.cpu_budget, args.no_cpu)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        print(json.dumps({"error": "the benchmark runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
     import numpy as np
     import torch

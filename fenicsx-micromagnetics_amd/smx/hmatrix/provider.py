@@ -5,7 +5,7 @@ entry_provider_gpu.py:352-777): the object the H-matrix builder asks for
 blocks of the double-layer operator B.  Here every entry is computed by the
 batched Lindholm fill kernel on the device (smx_lindholm_fill_batched):
 fill_block_device returns the block as a device tensor, fill_block and
-fill_block_cpu return it as a host array (this build has no host fill).  The
+fill_block_cpu return it as a host array (no entry is computed on the host).  The
 provider wraps the boundary geometry that the demag class keeps as ``opB``.
 """
 from __future__ import annotations

@@ -1,7 +1,7 @@
 """Tetrahedral mesh generators for the demag / LLG hot path.
 
 The reference reads DOLFINx meshes from XDMF (gpu/io_utils/mesh.py:8-31).  DOLFINx
-is not part of this build, so meshes are generated here as plain arrays:
+is not used here: meshes are generated as plain arrays:
 
 * ``X``    (N, 3) float64 vertex coordinates in nm,
 * ``tets`` (T, 4) int32 vertex indices per tetrahedron.

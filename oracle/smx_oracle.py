@@ -144,7 +144,7 @@ _TET_FACES = ((1, 2, 3), (0, 2, 3), (0, 1, 3), (0, 1, 2))
 
 def exterior_facets(tets):
     """Exterior facets as (sorted vertex triple, owning cell), ordered
-    lexicographically by the sorted triple (the facet numbering of this build)."""
+    lexicographically by the sorted triple (the facet numbering used throughout)."""
     T = tets.shape[0]
     faces = []
     owner = []
@@ -969,8 +969,8 @@ def build_hmatrix(bd: BoundaryData, eps=1e-6, eta=2.0, leaf_size=64,
 
 # --------------------------------------------------------------------------
 # Energy with per-cell Ms and A, and its gradient from the dense FK map
-# (the adjoint of this build is checked against this; the reference has no
-# adjoint).  E_d = 1/2 mu0 1e-27 b^T L b with b = Div_Ms m; for uniform Ms this
+# (the product's adjoint kernels are checked against this; the reference has
+# no adjoint).  E_d = 1/2 mu0 1e-27 b^T L b with b = Div_Ms m; for uniform Ms this
 # is the lumped energy of Demag_Lindholm_GPU.py:706-724 (G = -Div^T/Ms).
 # --------------------------------------------------------------------------
 def div_per_cell(X, tets, Ms_cell):
