@@ -60,6 +60,21 @@ ACA_LARGE_ENTRIES = 1 << 21
 ACA_TILE_ROWS = _capi.ACA_TILE_ROWS
 
 
+def aca_tile_table(m: np.ndarray):
+    """Row tiles of smx_aca_large for blocks of m[b] rows: (tile_task,
+    tile_row0, tile_first), block b owning tiles tile_first[b] ..
+    tile_first[b+1]-1 in row order, ACA_TILE_ROWS rows each (the last one
+    shorter)."""
+    m = np.asarray(m, dtype=np.int64)
+    if np.any(m <= 0):
+        raise ValueError("every block needs at least one row")
+    ntile = (m + ACA_TILE_ROWS - 1) // ACA_TILE_ROWS
+    first = np.concatenate([[0], np.cumsum(ntile)]).astype(np.int32)
+    tile_task = np.repeat(np.arange(m.size, dtype=np.int32), ntile)
+    tile_row0 = (ACA_TILE_ROWS * (np.arange(int(ntile.sum())) - np.repeat(first[:-1], ntile))).astype(np.int32)
+    return tile_task, tile_row0, first
+
+
 @dataclass
 class HMatrixBuildConfig:
     epsilon: float = 1e-6
@@ -239,16 +254,12 @@ class DeviceBuildOps:
                                               V.data_ptr(), rank_d.data_ptr(), res_d.data_ptr(), st_d.data_ptr(),
                                               _capi.stream_ptr()), "smx_aca_batched")
             else:
-                ntile = (sub["m"].astype(np.int64) + ACA_TILE_ROWS - 1) // ACA_TILE_ROWS
-                first = np.concatenate([[0], np.cumsum(ntile)]).astype(np.int32)
-                tile_task = np.repeat(np.arange(nb, dtype=np.int32), ntile)
-                tile_row0 = (ACA_TILE_ROWS * (np.arange(int(ntile.sum())) - np.repeat(first[:-1], ntile))
-                             ).astype(np.int32)
-                nws = int(L.smx_aca_large_ws_doubles(nb, int(ntile.sum())))
+                tile_task, tile_row0, first = aca_tile_table(sub["m"])
+                nws = int(L.smx_aca_large_ws_doubles(nb, int(tile_task.size)))
                 work = torch.empty(nws, dtype=torch.float64, device=dev)
                 keep = [to_dev(a, dev) for a in (tile_task, tile_row0, first)]
                 _capi.check(L.smx_aca_large(A.data_ptr(), tasks_d.data_ptr(), nb, keep[0].data_ptr(),
-                                            keep[1].data_ptr(), keep[2].data_ptr(), int(ntile.sum()),
+                                            keep[1].data_ptr(), keep[2].data_ptr(), int(tile_task.size),
                                             int(sub["rcap"].max()), float(rtol), U.data_ptr(), V.data_ptr(),
                                             rank_d.data_ptr(), res_d.data_ptr(), st_d.data_ptr(), work.data_ptr(),
                                             nws, _capi.stream_ptr()), "smx_aca_large")

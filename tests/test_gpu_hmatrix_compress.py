@@ -320,7 +320,7 @@ def test_tiled_aca_matches_single_workgroup_aca():
     from smx import mesh as M
     from smx.boundary import build_boundary_geometry
     from smx.hmatrix import LindholmEntryProviderGPU
-    from smx.hmatrix.builder import ACA_TASK_DTYPE, ACA_TILE_ROWS
+    from smx.hmatrix.builder import ACA_TASK_DTYPE, aca_tile_table
     m = M.cube_c2()
     geo = build_boundary_geometry(m.X, m.tets)
     prov = LindholmEntryProviderGPU.from_opB(geo)
@@ -352,15 +352,12 @@ def test_tiled_aca_matches_single_workgroup_aca():
             _capi.check(L.smx_aca_batched(A.data_ptr(), td.data_ptr(), len(shapes), rtol, U.data_ptr(), V.data_ptr(),
                                           rk.data_ptr(), rl.data_ptr(), st.data_ptr(), _capi.stream_ptr()))
         else:
-            ntile = (t["m"].astype(np.int64) + ACA_TILE_ROWS - 1) // ACA_TILE_ROWS
-            first = np.concatenate([[0], np.cumsum(ntile)]).astype(np.int32)
-            tt = np.repeat(np.arange(len(shapes), dtype=np.int32), ntile)
-            r0 = (ACA_TILE_ROWS * (np.arange(int(ntile.sum())) - np.repeat(first[:-1], ntile))).astype(np.int32)
+            tt, r0, first = aca_tile_table(t["m"])
             dd = [torch.from_numpy(a).to(DEV) for a in (tt, r0, first)]
-            nws = int(L.smx_aca_large_ws_doubles(len(shapes), int(ntile.sum())))
+            nws = int(L.smx_aca_large_ws_doubles(len(shapes), int(tt.size)))
             work = torch.empty(nws, dtype=torch.float64, device=DEV)
             _capi.check(L.smx_aca_large(A.data_ptr(), td.data_ptr(), len(shapes), dd[0].data_ptr(), dd[1].data_ptr(),
-                                        dd[2].data_ptr(), int(ntile.sum()), int(t["rcap"].max()), rtol,
+                                        dd[2].data_ptr(), int(tt.size), int(t["rcap"].max()), rtol,
                                         U.data_ptr(), V.data_ptr(), rk.data_ptr(), rl.data_ptr(), st.data_ptr(),
                                         work.data_ptr(), nws, _capi.stream_ptr()))
         torch.cuda.synchronize()
