@@ -170,7 +170,7 @@ def cg_iteration_timing(d, K=200):
         h = d.amg_u1
 
         def solve_amg(k):
-            L.smx_pcg_amg(d.N, A.width, A.ld, A.cols.data_ptr(), A.vals.data_ptr(), 0,
+            L.smx_pcg_amg(d.N, 0, A.width, A.ld, A.cols.data_ptr(), A.vals.data_ptr(), 0,
                           d.b_u1.data_ptr(), d.b_part.data_ptr(), d.b_part.numel(), 1, raw.width, raw.ld,
                           raw.cols.data_ptr(), raw.vals.data_ptr(), None, None, x.data_ptr(), 0,
                           _capi.ptr(d.cg_perm), C.addressof(h.c_levels), h.nlev, h.val_dtype, 0.0, 0.0, k,
@@ -181,7 +181,7 @@ def cg_iteration_timing(d, K=200):
         return (t_k - t_0) / K, amg_iteration_bytes(d)
 
     def solve(k):
-        L.smx_pcg_jacobi(d.N, A.width, A.ld, A.cols.data_ptr(), A.vals.data_ptr(), d.dinv_u1.data_ptr(), 0,
+        L.smx_pcg_jacobi(d.N, 0, A.width, A.ld, A.cols.data_ptr(), A.vals.data_ptr(), d.dinv_u1.data_ptr(), 0,
                          d.b_u1.data_ptr(), d.b_part.data_ptr(), d.b_part.numel(), 1, raw.width, raw.ld,
                          raw.cols.data_ptr(), raw.vals.data_ptr(), None, None, x.data_ptr(), 0,
                          _capi.ptr(d.cg_perm), 0.0, 0.0, k, 10 ** 6,
@@ -572,7 +572,7 @@ def pmc_probe_amg(config, K=40):
     L = _capi.lib()
     ws = torch.zeros(int(L.smx_pcg_amg_ws_size(fem.N)), dtype=torch.float64, device=dev)
     its, rn, sums = C.c_int32(0), C.c_double(0.0), (C.c_double * 1)()
-    L.smx_pcg_amg(fem.N, A.width, A.ld, A.cols.data_ptr(), A.vals.data_ptr(), 0, bd.data_ptr(), bpart.data_ptr(), 1,
+    L.smx_pcg_amg(fem.N, 0, A.width, A.ld, A.cols.data_ptr(), A.vals.data_ptr(), 0, bd.data_ptr(), bpart.data_ptr(), 1,
                   1, A.width, A.ld, A.cols.data_ptr(), A.vals.data_ptr(), None, None, x.data_ptr(), 0, pd.data_ptr(),
                   C.addressof(h.c_levels), h.nlev, h.val_dtype, 0.0, 0.0, K, 10 ** 6, K, ws.data_ptr(),
                   C.byref(its), C.byref(rn), sums, _capi.stream_ptr())
@@ -655,7 +655,7 @@ def pmc_probe(config):
     ws = torch.zeros(int(L.smx_pcg_ws_size(fem.N)), dtype=torch.float64, device=dev)
     its, rn, sums = C.c_int32(0), C.c_double(0.0), (C.c_double * 1)()
     K = 200
-    L.smx_pcg_jacobi(fem.N, A.width, A.ld, A.cols.data_ptr(), A.vals.data_ptr(), dinv.data_ptr(), 0,
+    L.smx_pcg_jacobi(fem.N, 0, A.width, A.ld, A.cols.data_ptr(), A.vals.data_ptr(), dinv.data_ptr(), 0,
                      bd.data_ptr(), bpart.data_ptr(), 1, 1, A_plain.width, A_plain.ld, A_plain.cols.data_ptr(),
                      A_plain.vals.data_ptr(),
                      None, None, x.data_ptr(), 0, pd.data_ptr(), 0.0, 0.0, K, 10 ** 6, K, ws.data_ptr(), C.byref(its),

@@ -202,7 +202,7 @@ __device__ __forceinline__ double next_r(double r, double a, double sn) { return
 // status record.
 template <int BS, int WMAX, typename V>
 __global__ __launch_bounds__(BS) void amg_init_kernel(
-    int64_t n, int32_t mode, int32_t W, int64_t ld, const int32_t* __restrict__ cols,
+    int64_t n, int64_t nd, int32_t mode, int32_t W, int64_t ld, const int32_t* __restrict__ cols,
     const double* __restrict__ vals, const double* __restrict__ b,
     const double* __restrict__ b_part, int64_t nb_part, int32_t project,
     const int32_t* __restrict__ bpos, const double* __restrict__ gb, double* __restrict__ x,
@@ -273,6 +273,13 @@ __global__ __launch_bounds__(BS) void amg_init_kernel(
     v[0] = bk * bk;
     v[1] = (double)r0v * (double)r0v;
     v[2] = x0;
+  }
+  // trailing identity rows n .. n+nd-1: x is their data (b, or gb at bpos),
+  // their residual is 0 and they count in ||b|| as rows of the full system
+  for (int64_t t = n + row; t < n + nd; t += (int64_t)gridDim.x * BS) {
+    const double bt = mode == 0 ? b[perm ? perm[t] : t] : gb[bpos[t]];
+    x[t] = bt;
+    v[0] = fma(bt, bt, v[0]);
   }
   block_total<BS, 3, false>(v, scr_pub);
   if (threadIdx.x == 0) {
@@ -575,10 +582,12 @@ struct Layout {
   int64_t Gf, nvec;
 };
 
-Layout layout(int64_t n) {
+// Gf from the n iterated rows; vectors (x in solver order among them) hold
+// ntot = n + nd rows, so smx_pcg_amg_ws_size(ntot) covers the layout
+Layout layout(int64_t n, int64_t ntot) {
   Layout L;
   L.Gf = (n + fine_bs(n) - 1) / fine_bs(n);
-  L.nvec = (n + 63) / 64 * 64;
+  L.nvec = (ntot + 63) / 64 * 64;
   return L;
 }
 
@@ -615,7 +624,7 @@ int vcycle_tail(const smx_amg_level* lv, int nlev, const V* r_fine, AmgPtrs& P,
 // from the current x.  A speculative launch enqueues the first solve and one
 // check: a failed check leaves DONE = 0 and the caller solves again waiting.
 template <int BS, int WMAX, bool EXACT, typename VT, typename V, typename AV, bool COMP>
-int run_amg(int64_t n, int32_t W, int64_t ld, const int32_t* cols, const double* vals,
+int run_amg(int64_t n, int64_t nd, int32_t W, int64_t ld, const int32_t* cols, const double* vals,
             const AV* ivals, int32_t mode, const double* b, const double* b_part, int64_t nb_part,
             int32_t project, int32_t raw_W, int64_t raw_ld, const int32_t* raw_cols,
             const double* raw_vals, const int32_t* bpos, const double* gb, double* x_user,
@@ -623,11 +632,11 @@ int run_amg(int64_t n, int32_t W, int64_t ld, const int32_t* cols, const double*
             double rtol, double atol, int32_t max_it, int32_t check_every, int32_t expected_its,
             int32_t refinements, double rtol_inner, AmgPtrs P, hipStream_t st, double* hst) {
   const int Gf = P.Gf;
-  const unsigned gs = (unsigned)((n + 255) / 256);
+  const unsigned gs = (unsigned)((n + nd + 255) / 256);
   double* x = perm ? x_perm : x_user;
   const int64_t n1 = nlev > 1 ? lv[0].R.rows : 0;
   if (perm && warm) {
-    hipLaunchKernelGGL(amg_gather_kernel, dim3(gs), dim3(256), 0, st, n, perm, x_user, x);
+    hipLaunchKernelGGL(amg_gather_kernel, dim3(gs), dim3(256), 0, st, n + nd, perm, x_user, x);
     SMX_LAUNCH_CHECK();
   }
   const int rt = nlev > 1 ? lv[0].r_tpr : 1;
@@ -642,10 +651,10 @@ int run_amg(int64_t n, int32_t W, int64_t ld, const int32_t* cols, const double*
   const int32_t every = check_every < 1 ? 8 : check_every;
   auto init = [&](int32_t w, int32_t phase) -> int {
     if (mode == 0) {
-      hipLaunchKernelGGL((amg_init_kernel<BS, WMAX, V>), dim3(Gf), dim3(BS), 0, st, n, mode, W, ld,
+      hipLaunchKernelGGL((amg_init_kernel<BS, WMAX, V>), dim3(Gf), dim3(BS), 0, st, n, nd, mode, W, ld,
                          cols, vals, b, b_part, nb_part, project, bpos, gb, x, w, perm, P, n1, phase);
     } else {
-      hipLaunchKernelGGL((amg_init_kernel<BS, WMAX, V>), dim3(Gf), dim3(BS), 0, st, n, mode, raw_W,
+      hipLaunchKernelGGL((amg_init_kernel<BS, WMAX, V>), dim3(Gf), dim3(BS), 0, st, n, nd, mode, raw_W,
                          raw_ld, raw_cols, raw_vals, b, b_part, nb_part, project, bpos, gb, x, w, perm,
                          P, n1, phase);
     }
@@ -723,7 +732,7 @@ int run_amg(int64_t n, int32_t W, int64_t ld, const int32_t* cols, const double*
     if (rc) return rc;
   }
   if (perm) {
-    hipLaunchKernelGGL(amg_scatter_kernel, dim3(gs), dim3(256), 0, st, n, perm, x, x_user);
+    hipLaunchKernelGGL(amg_scatter_kernel, dim3(gs), dim3(256), 0, st, n + nd, perm, x, x_user);
     SMX_LAUNCH_CHECK();
   }
   if (!spec) {
@@ -792,13 +801,13 @@ extern "C" int smx_amg_apply(const smx_amg_level* levels, int32_t nlev, int32_t 
 }
 
 extern "C" int64_t smx_pcg_amg_ws_size(int64_t n) {
-  const Layout L = layout(n);
+  const Layout L = layout(n, n);
   return HDR + 7 * L.Gf + 9 * L.nvec + 64;
 }
 
 namespace {
 
-int amg_entry(int64_t n, int32_t width, int64_t ld, const int32_t* cols, const double* vals,
+int amg_entry(int64_t n, int64_t nd, int32_t width, int64_t ld, const int32_t* cols, const double* vals,
               int32_t mode, const double* b, const double* b_part, int64_t nb_part,
               int32_t project, int32_t raw_width, int64_t raw_ld, const int32_t* raw_cols,
               const double* raw_vals, const int32_t* bpos, const double* gb, double* x,
@@ -806,8 +815,10 @@ int amg_entry(int64_t n, int32_t width, int64_t ld, const int32_t* cols, const d
               int32_t val_dtype, double rtol, double atol, int32_t max_it, int32_t check_every,
               int32_t expected_its, double* work, int32_t* its_out, double* rnorm_out,
               double* sums_out, const smx_cg_mixed* mx, void* stream) {
-  SMX_REQUIRE(n >= 0, SMX_ERR_SHAPE, "negative size");
+  SMX_REQUIRE(n >= 0 && nd >= 0, SMX_ERR_SHAPE, "negative size");
   SMX_REQUIRE(mode == 0 || mode == 1, SMX_ERR_VALUE, "mode must be 0 or 1");
+  SMX_REQUIRE(nd == 0 || mode == 1 || !project, SMX_ERR_VALUE,
+              "trailing identity rows (nd > 0) in mode 0 need project = 0");
   SMX_REQUIRE(max_it >= 0, SMX_ERR_VALUE, "max_it must be >= 0");
   SMX_REQUIRE(check_every >= 0 || (expected_its > 0 && check_every >= -64), SMX_ERR_VALUE,
               "a speculative launch (check_every in [-64, -1]) needs expected_its > 0");
@@ -826,6 +837,7 @@ int amg_entry(int64_t n, int32_t width, int64_t ld, const int32_t* cols, const d
   }
   if (its_out) *its_out = 0;
   if (rnorm_out) *rnorm_out = 0.0;
+  SMX_REQUIRE(n > 0 || nd == 0, SMX_ERR_SHAPE, "trailing identity rows need n >= 1");
   if (n == 0) return SMX_OK;
   SMX_REQUIRE(cols && vals && x && work, SMX_ERR_VALUE, "null pointer argument");
   SMX_REQUIRE(mode == 1 || b, SMX_ERR_VALUE, "mode 0 needs b");
@@ -835,9 +847,10 @@ int amg_entry(int64_t n, int32_t width, int64_t ld, const int32_t* cols, const d
     const int rc = check_levels(levels, nlev, n);
     if (rc) return rc;
   }
-  SMX_REQUIRE(n < (int64_t)1 << 30, SMX_ERR_SHAPE, "operator too large (n %lld)", (long long)n);
+  SMX_REQUIRE(n + nd < (int64_t)1 << 30, SMX_ERR_SHAPE, "operator too large (n %lld)",
+              (long long)(n + nd));
   hipStream_t st = smx::as_stream(stream);
-  const Layout Lo = layout(n);
+  const Layout Lo = layout(n, n + nd);
   AmgPtrs P;
   P.Gf = (int)Lo.Gf;
   P.ss = work;
@@ -868,7 +881,7 @@ int amg_entry(int64_t n, int32_t width, int64_t ld, const int32_t* cols, const d
   const double rin = mx ? mx->rtol_inner : rtol;
   int rc;
 #define SMX_AMG_CALL(BS, WM, EX, VT, V, AV, CP, IV)                                              \
-  run_amg<BS, WM, EX, VT, V, AV, CP>(n, width, ld, cols, vals, IV, mode, b, b_part, nb_part,     \
+  run_amg<BS, WM, EX, VT, V, AV, CP>(n, nd, width, ld, cols, vals, IV, mode, b, b_part, nb_part, \
                                      project, raw_width, raw_ld, raw_cols, raw_vals, bpos, gb, x, \
                                      warm, perm, x_perm, levels, nlev, rtol, atol, max_it,       \
                                      check_every, expected_its, nref, rin, P, st, hst)
@@ -918,7 +931,7 @@ int amg_entry(int64_t n, int32_t width, int64_t ld, const int32_t* cols, const d
 
 }  // namespace
 
-extern "C" int smx_pcg_amg(int64_t n, int32_t width, int64_t ld, const int32_t* cols,
+extern "C" int smx_pcg_amg(int64_t n, int64_t nd, int32_t width, int64_t ld, const int32_t* cols,
                            const double* vals, int32_t mode, const double* b,
                            const double* b_part, int64_t nb_part, int32_t project,
                            int32_t raw_width, int64_t raw_ld, const int32_t* raw_cols,
@@ -928,13 +941,13 @@ extern "C" int smx_pcg_amg(int64_t n, int32_t width, int64_t ld, const int32_t* 
                            double rtol, double atol, int32_t max_it, int32_t check_every,
                            int32_t expected_its, double* work, int32_t* its_out,
                            double* rnorm_out, double* sums_out, void* stream) {
-  return amg_entry(n, width, ld, cols, vals, mode, b, b_part, nb_part, project, raw_width, raw_ld,
+  return amg_entry(n, nd, width, ld, cols, vals, mode, b, b_part, nb_part, project, raw_width, raw_ld,
                    raw_cols, raw_vals, bpos, gb, x, warm, perm, levels, nlev, val_dtype, rtol, atol,
                    max_it, check_every, expected_its, work, its_out, rnorm_out, sums_out, nullptr,
                    stream);
 }
 
-extern "C" int smx_pcg_amg_mixed(int64_t n, int32_t width, int64_t ld, const int32_t* cols,
+extern "C" int smx_pcg_amg_mixed(int64_t n, int64_t nd, int32_t width, int64_t ld, const int32_t* cols,
                                  const double* vals, int32_t mode, const double* b,
                                  const double* b_part, int64_t nb_part, int32_t project,
                                  int32_t raw_width, int64_t raw_ld, const int32_t* raw_cols,
@@ -946,7 +959,7 @@ extern "C" int smx_pcg_amg_mixed(int64_t n, int32_t width, int64_t ld, const int
                                  double* rnorm_out, double* sums_out, const smx_cg_mixed* mixed,
                                  void* stream) {
   SMX_REQUIRE(mixed, SMX_ERR_VALUE, "null mixed-precision options");
-  return amg_entry(n, width, ld, cols, vals, mode, b, b_part, nb_part, project, raw_width, raw_ld,
+  return amg_entry(n, nd, width, ld, cols, vals, mode, b, b_part, nb_part, project, raw_width, raw_ld,
                    raw_cols, raw_vals, bpos, gb, x, warm, perm, levels, nlev, val_dtype, rtol, atol,
                    max_it, check_every, expected_its, work, its_out, rnorm_out, sums_out, mixed,
                    stream);

@@ -226,7 +226,7 @@ __global__ __launch_bounds__(256) void cg_prepare_kernel(int64_t n, int32_t W, i
 // ||r0||^2 (read by the refinement check).
 template <int BS, int WMAX, typename V>
 __global__ __launch_bounds__(BS) void cg_init_a_kernel(
-    int64_t n, int32_t mode, int32_t W, int64_t ld, const int32_t* __restrict__ cols,
+    int64_t n, int64_t nd, int32_t mode, int32_t W, int64_t ld, const int32_t* __restrict__ cols,
     const double* __restrict__ vals, const int32_t* __restrict__ acols,
     const double* __restrict__ dinv, const double* __restrict__ b,
     const double* __restrict__ b_part, int64_t nb_part, int32_t project,
@@ -296,6 +296,13 @@ __global__ __launch_bounds__(BS) void cg_init_a_kernel(
     v[0] = bk * bk;
     v[1] = acols[row] != (int32_t)row ? 1.0 : 0.0;
     v[2] = r0 * r0;
+  }
+  // trailing identity rows n .. n+nd-1: x is their data (b, or gb at bpos),
+  // their residual is 0 and they count in ||b|| as rows of the full system
+  for (int64_t t = n + row; t < n + nd; t += (int64_t)gridDim.x * BS) {
+    const double bt = mode == 0 ? b[perm ? perm[t] : t] : gb[bpos[t]];
+    x[t] = bt;
+    v[0] = fma(bt, bt, v[0]);
   }
   publish_partials<BS, 3>(v, P.pbb, P.G, scr_pub);
 }
@@ -522,10 +529,12 @@ struct Layout {
   int64_t G, nvec;
 };
 
-Layout layout(int64_t n) {
+// G from the n iterated rows; vectors (x in solver order among them) hold
+// ntot = n + nd rows, so smx_pcg_ws_size(ntot) covers the layout
+Layout layout(int64_t n, int64_t ntot) {
   Layout L;
   L.G = grid_for_n(n);
-  L.nvec = (n + 63) / 64 * 64;
+  L.nvec = (ntot + 63) / 64 * 64;
   return L;
 }
 
@@ -536,7 +545,7 @@ Layout layout(int64_t n) {
 // the first solve and one check and returns: a failed check leaves DONE = 0
 // in the status block, and the caller solves again waiting.
 template <int BS, int WMAX, bool EXACT, typename V, typename AV, bool COMP>
-int run_pcg(int64_t n, int32_t W, int64_t ld, const int32_t* cols, const double* vals,
+int run_pcg(int64_t n, int64_t nd, int32_t W, int64_t ld, const int32_t* cols, const double* vals,
             const AV* ivals, const double* dinv, int32_t mode, const double* b,
             const double* b_part, int64_t nb_part, int32_t project, int32_t raw_W, int64_t raw_ld,
             const int32_t* raw_cols, const double* raw_vals, const int32_t* bpos, const double* gb,
@@ -544,10 +553,10 @@ int run_pcg(int64_t n, int32_t W, int64_t ld, const int32_t* cols, const double*
             double atol, int32_t max_it, int32_t check_every, int32_t expected_its,
             int32_t refinements, double rtol_inner, CGPtrs P, hipStream_t st, double* hst) {
   const int G = P.G;
-  const unsigned gs = (unsigned)((n + 255) / 256);
+  const unsigned gs = (unsigned)((n + nd + 255) / 256);
   double* x = perm ? x_perm : x_user;  // the kernels work in solver (permuted) row order
   if (perm && warm) {
-    hipLaunchKernelGGL(cg_gather_kernel, dim3(gs), dim3(256), 0, st, n, perm, x_user, x);
+    hipLaunchKernelGGL(cg_gather_kernel, dim3(gs), dim3(256), 0, st, n + nd, perm, x_user, x);
     SMX_LAUNCH_CHECK();
   }
   // check_every < 0: speculative launch of expected + 1 - check_every
@@ -557,10 +566,10 @@ int run_pcg(int64_t n, int32_t W, int64_t ld, const int32_t* cols, const double*
   const int32_t every = check_every < 1 ? 16 : check_every;
   auto init_a = [&](int32_t w) -> int {
     if (mode == 0) {
-      hipLaunchKernelGGL((cg_init_a_kernel<BS, WMAX, V>), dim3(G), dim3(BS), 0, st, n, mode, W, ld,
+      hipLaunchKernelGGL((cg_init_a_kernel<BS, WMAX, V>), dim3(G), dim3(BS), 0, st, n, nd, mode, W, ld,
                          cols, vals, cols, dinv, b, b_part, nb_part, project, bpos, gb, x, w, perm, P);
     } else {
-      hipLaunchKernelGGL((cg_init_a_kernel<BS, WMAX, V>), dim3(G), dim3(BS), 0, st, n, mode, raw_W,
+      hipLaunchKernelGGL((cg_init_a_kernel<BS, WMAX, V>), dim3(G), dim3(BS), 0, st, n, nd, mode, raw_W,
                          raw_ld, raw_cols, raw_vals, cols, dinv, b, b_part, nb_part, project, bpos, gb,
                          x, w, perm, P);
     }
@@ -610,7 +619,7 @@ int run_pcg(int64_t n, int32_t W, int64_t ld, const int32_t* cols, const double*
     if (rc) return rc;
   }
   if (perm) {
-    hipLaunchKernelGGL(cg_scatter_kernel, dim3(gs), dim3(256), 0, st, n, perm, x, x_user);
+    hipLaunchKernelGGL(cg_scatter_kernel, dim3(gs), dim3(256), 0, st, n + nd, perm, x, x_user);
     SMX_LAUNCH_CHECK();
   }
   if (!spec) {
@@ -637,13 +646,13 @@ extern "C" int smx_pcg_prepare(int64_t n, int32_t width, int64_t ld, const int32
 }
 
 extern "C" int64_t smx_pcg_ws_size(int64_t n) {
-  const Layout L = layout(n);
+  const Layout L = layout(n, n);
   return HDR + 11 * L.G + 8 * L.nvec + 64;
 }
 
 namespace {
 
-int pcg_entry(int64_t n, int32_t width, int64_t ld, const int32_t* cols, const double* vals,
+int pcg_entry(int64_t n, int64_t nd, int32_t width, int64_t ld, const int32_t* cols, const double* vals,
               const double* dinv, int32_t mode, const double* b, const double* b_part,
               int64_t nb_part, int32_t project, int32_t raw_width, int64_t raw_ld,
               const int32_t* raw_cols, const double* raw_vals, const int32_t* bpos,
@@ -651,8 +660,10 @@ int pcg_entry(int64_t n, int32_t width, int64_t ld, const int32_t* cols, const d
               double atol, int32_t max_it, int32_t check_every, int32_t expected_its,
               double* work, int32_t* its_out, double* rnorm_out, double* sums_out,
               const smx_cg_mixed* mx, void* stream) {
-  SMX_REQUIRE(n >= 0, SMX_ERR_SHAPE, "negative size");
+  SMX_REQUIRE(n >= 0 && nd >= 0, SMX_ERR_SHAPE, "negative size");
   SMX_REQUIRE(mode == 0 || mode == 1, SMX_ERR_VALUE, "mode must be 0 or 1");
+  SMX_REQUIRE(nd == 0 || mode == 1 || !project, SMX_ERR_VALUE,
+              "trailing identity rows (nd > 0) in mode 0 need project = 0");
   SMX_REQUIRE(max_it >= 0, SMX_ERR_VALUE, "max_it must be >= 0");
   SMX_REQUIRE(check_every >= 0 || (expected_its > 0 && check_every >= -64), SMX_ERR_VALUE,
               "a speculative launch (check_every in [-64, -1]) needs expected_its > 0");
@@ -660,8 +671,8 @@ int pcg_entry(int64_t n, int32_t width, int64_t ld, const int32_t* cols, const d
               "ELL width must be in [1, 32] and ld >= n (width %d, ld %lld)", width, (long long)ld);
   SMX_REQUIRE(mode == 0 || (raw_width >= 1 && raw_width <= 32 && raw_ld >= n), SMX_ERR_SHAPE,
               "raw operator width must be in [1, 32]");
-  SMX_REQUIRE(n < (int64_t)1 << 28, SMX_ERR_SHAPE,
-              "CG operator too large for 32-bit byte offsets (n %lld)", (long long)n);
+  SMX_REQUIRE(n + nd < (int64_t)1 << 28, SMX_ERR_SHAPE,
+              "CG operator too large for 32-bit byte offsets (n %lld)", (long long)(n + nd));
   if (mx) {
     SMX_REQUIRE(mx->vals32, SMX_ERR_VALUE, "mixed precision needs the FP32 operator values");
     SMX_REQUIRE(mx->refinements >= 0 && mx->refinements <= 8, SMX_ERR_VALUE,
@@ -671,13 +682,14 @@ int pcg_entry(int64_t n, int32_t width, int64_t ld, const int32_t* cols, const d
   }
   if (its_out) *its_out = 0;
   if (rnorm_out) *rnorm_out = 0.0;
+  SMX_REQUIRE(n > 0 || nd == 0, SMX_ERR_SHAPE, "trailing identity rows need n >= 1");
   if (n == 0) return SMX_OK;
   SMX_REQUIRE(cols && vals && dinv && x && work, SMX_ERR_VALUE, "null pointer argument");
   SMX_REQUIRE(mode == 1 || b, SMX_ERR_VALUE, "mode 0 needs b");
   SMX_REQUIRE(mode == 0 || (raw_cols && raw_vals && bpos && gb), SMX_ERR_VALUE,
               "mode 1 needs the raw operator, bpos and gb");
   hipStream_t st = smx::as_stream(stream);
-  const Layout Lo = layout(n);
+  const Layout Lo = layout(n, n + nd);
   CGPtrs P;
   P.G = (int)Lo.G;
   P.ss = work;
@@ -702,7 +714,7 @@ int pcg_entry(int64_t n, int32_t width, int64_t ld, const int32_t* cols, const d
   const double rin = mx ? mx->rtol_inner : rtol;
   int rc;
 #define SMX_PCG_CALL(BS, WM, EX, V, AV, CP, IV)                                                   \
-  run_pcg<BS, WM, EX, V, AV, CP>(n, width, ld, cols, vals, IV, dinv, mode, b, b_part, nb_part,    \
+  run_pcg<BS, WM, EX, V, AV, CP>(n, nd, width, ld, cols, vals, IV, dinv, mode, b, b_part, nb_part,\
                                  project, raw_width, raw_ld, raw_cols, raw_vals, bpos, gb, x, warm, \
                                  perm, x_perm, rtol, atol, max_it, check_every, expected_its,     \
                                  nref, rin, P, st, hst)
@@ -751,7 +763,7 @@ int pcg_entry(int64_t n, int32_t width, int64_t ld, const int32_t* cols, const d
 
 }  // namespace
 
-extern "C" int smx_pcg_jacobi(int64_t n, int32_t width, int64_t ld, const int32_t* cols,
+extern "C" int smx_pcg_jacobi(int64_t n, int64_t nd, int32_t width, int64_t ld, const int32_t* cols,
                               const double* vals, const double* dinv, int32_t mode,
                               const double* b, const double* b_part, int64_t nb_part,
                               int32_t project, int32_t raw_width, int64_t raw_ld,
@@ -761,12 +773,12 @@ extern "C" int smx_pcg_jacobi(int64_t n, int32_t width, int64_t ld, const int32_
                               int32_t check_every,
                               int32_t expected_its, double* work, int32_t* its_out,
                               double* rnorm_out, double* sums_out, void* stream) {
-  return pcg_entry(n, width, ld, cols, vals, dinv, mode, b, b_part, nb_part, project, raw_width,
+  return pcg_entry(n, nd, width, ld, cols, vals, dinv, mode, b, b_part, nb_part, project, raw_width,
                    raw_ld, raw_cols, raw_vals, bpos, gb, x, warm, perm, rtol, atol, max_it,
                    check_every, expected_its, work, its_out, rnorm_out, sums_out, nullptr, stream);
 }
 
-extern "C" int smx_pcg_jacobi_mixed(int64_t n, int32_t width, int64_t ld, const int32_t* cols,
+extern "C" int smx_pcg_jacobi_mixed(int64_t n, int64_t nd, int32_t width, int64_t ld, const int32_t* cols,
                                     const double* vals, const double* dinv, int32_t mode,
                                     const double* b, const double* b_part, int64_t nb_part,
                                     int32_t project, int32_t raw_width, int64_t raw_ld,
@@ -777,7 +789,7 @@ extern "C" int smx_pcg_jacobi_mixed(int64_t n, int32_t width, int64_t ld, const 
                                     double* work, int32_t* its_out, double* rnorm_out,
                                     double* sums_out, const smx_cg_mixed* mixed, void* stream) {
   SMX_REQUIRE(mixed, SMX_ERR_VALUE, "null mixed-precision options");
-  return pcg_entry(n, width, ld, cols, vals, dinv, mode, b, b_part, nb_part, project, raw_width,
+  return pcg_entry(n, nd, width, ld, cols, vals, dinv, mode, b, b_part, nb_part, project, raw_width,
                    raw_ld, raw_cols, raw_vals, bpos, gb, x, warm, perm, rtol, atol, max_it,
                    check_every, expected_its, work, its_out, rnorm_out, sums_out, mixed, stream);
 }

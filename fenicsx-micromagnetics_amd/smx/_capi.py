@@ -116,17 +116,17 @@ _SIGS = {
     "smx_reduce_ws_size": ([I64], I64),
     "smx_pcg_ws_size": ([I64], I64),
     "smx_pcg_prepare": ([I64, I32, I64, P, P, P, P, P, P], I32),
-    "smx_pcg_jacobi": ([I64, I32, I64, P, P, P, I32, P, P, I64, I32, I32, I64, P, P, P, P, P, I32, P, F64,
+    "smx_pcg_jacobi": ([I64, I64, I32, I64, P, P, P, I32, P, P, I64, I32, I32, I64, P, P, P, P, P, I32, P, F64,
                         F64, I32, I32, I32, P, C.POINTER(I32), C.POINTER(F64), C.POINTER(F64), P],
                        C.c_int),
-    "smx_pcg_jacobi_mixed": ([I64, I32, I64, P, P, P, I32, P, P, I64, I32, I32, I64, P, P, P, P, P, I32, P,
+    "smx_pcg_jacobi_mixed": ([I64, I64, I32, I64, P, P, P, I32, P, P, I64, I32, I32, I64, P, P, P, P, P, I32, P,
                               F64, F64, I32, I32, I32, P, C.POINTER(I32), C.POINTER(F64), C.POINTER(F64),
                               C.POINTER(CGMixed), P], C.c_int),
     "smx_pcg_amg_ws_size": ([I64], I64),
     "smx_amg_apply": ([P, I32, I32, P, P, P, P], C.c_int),
-    "smx_pcg_amg": ([I64, I32, I64, P, P, I32, P, P, I64, I32, I32, I64, P, P, P, P, P, I32, P, P, I32, I32,
+    "smx_pcg_amg": ([I64, I64, I32, I64, P, P, I32, P, P, I64, I32, I32, I64, P, P, P, P, P, I32, P, P, I32, I32,
                      F64, F64, I32, I32, I32, P, C.POINTER(I32), C.POINTER(F64), C.POINTER(F64), P], C.c_int),
-    "smx_pcg_amg_mixed": ([I64, I32, I64, P, P, I32, P, P, I64, I32, I32, I64, P, P, P, P, P, I32, P, P, I32,
+    "smx_pcg_amg_mixed": ([I64, I64, I32, I64, P, P, I32, P, P, I64, I32, I32, I64, P, P, P, P, P, I32, P, P, I32,
                            I32, F64, F64, I32, I32, I32, P, C.POINTER(I32), C.POINTER(F64), C.POINTER(F64),
                            C.POINTER(CGMixed), P], C.c_int),
     "smx_field_llg_rhs": ([C.POINTER(FieldArgs), P], C.c_int),

@@ -219,26 +219,47 @@ class DemagFieldLindholmGPU:
         self.cg_perm = to_dev(perm.astype(np.int32), dev) if perm is not None else None
         ident = np.arange(self.N)
         sel = perm if perm is not None else ident
+        # u2 in an order of its own: the interior rows first (in the order of
+        # u1), the Dirichlet rows of A_bc after them.  The interior rows of
+        # A_bc do not couple to the Dirichlet columns, so the solver iterates
+        # the interior block alone and writes the Dirichlet rows' data once
+        # (smx_pcg_* trailing identity rows): the same iterates as the full
+        # system, without streaming its identity rows every iteration.
+        isb = fem.is_boundary[sel]
+        self.n_int = int(np.count_nonzero(~isb))
+        split = 0 < self.n_int < self.N
+        sel2 = np.concatenate([sel[~isb], sel[isb]]) if split else sel
+        n2 = self.n_int if split else self.N
+        self.cg_perm2 = to_dev(sel2.astype(np.int32), dev)
         d1 = self._diag(fem.stiff)
         d2 = self._diag(fem.stiffness_bc())
         self.dinv_u1 = to_dev(1.0 / d1[sel], dev)
-        self.dinv_u2 = to_dev(1.0 / d2[sel], dev)
+        self.dinv_u2 = to_dev(1.0 / d2[sel2][:n2], dev)
         pc1 = str(ksp_pc_type).strip().lower()
         pc2 = pc1 if ksp_pc_type_u2 is None else str(ksp_pc_type_u2).strip().lower()
         if pc1 not in ("gamg", "jacobi", "auto") or pc2 not in ("gamg", "jacobi", "auto"):
             raise ValueError("ksp_pc_type / ksp_pc_type_u2 must be 'gamg', 'jacobi' or 'auto'")
         A1h = self.ops.compact_host(fem.stiff, perm)
-        A2h = self.ops.compact_host(fem.stiffness_bc(), perm)
-        # plain P A P^T: the lifting operator b = -A_raw g of the u2 solve and
-        # the operator of the AMG-CG kernels; the Jacobi CG kernels take the
-        # solver form A D^-1 with the diagonal in slot 0
+        A2f = self.ops.compact_host(fem.stiffness_bc(), sel2)
+        if split and A2f[:n2, n2:].nnz:
+            raise RuntimeError("interior rows of the Dirichlet operator couple to Dirichlet columns")
+        A2h = A2f[:n2, :n2].tocsr()
+        # plain P A P^T: the operator of the AMG-CG kernels (u1: also the
+        # lifting operator); the Jacobi CG kernels take the solver form A D^-1
+        # with the diagonal in slot 0.  u2 lifts b = -A_raw g with the raw
+        # stiffness rows of its iterated rows, in its own order.
         self.A_raw = DeviceEll.from_csr(A1h.indptr, A1h.indices, A1h.data, self.N, dev)
-        A_bc = DeviceEll.from_csr(A2h.indptr, A2h.indices, A2h.data, self.N, dev)
+        R2 = self.ops.compact_host(fem.stiff, sel2)[:n2].tocsr()
+        raw2 = DeviceEll.from_csr(R2.indptr, R2.indices, R2.data, n2, dev)
+        A_bc = DeviceEll.from_csr(A2h.indptr, A2h.indices, A2h.data, n2, dev)
         self._cg = {}
-        for key, plain, dinv, pc in (("u1", self.A_raw, self.dinv_u1, pc1), ("u2", A_bc, self.dinv_u2, pc2)):
+        for key, plain, dinv, pc, n, raw, cperm in (
+                ("u1", self.A_raw, self.dinv_u1, pc1, self.N, self.A_raw, self.cg_perm),
+                ("u2", A_bc, self.dinv_u2, pc2, n2, raw2, self.cg_perm2)):
             self._cg[key] = {"plain": plain, "dinv": dinv, "request": pc, "pc": pc,
                              "jacobi": solver_operator(plain, dinv) if pc in ("jacobi", "auto") else None,
-                             "amg": None}
+                             "amg": None, "n": n, "nd": self.N - n, "raw": raw, "perm": cperm}
+        self._sel = (sel, sel2)
         self._amg_host = (A1h, A2h, int(amg_max_coarse), float(amg_theta))
 
         # ---------------- boundary operator ----------------
@@ -247,7 +268,14 @@ class DemagFieldLindholmGPU:
         self.Nb = geo.Nb
         self.bverts_gpu = to_dev(geo.bverts.astype(np.int32), dev)
         self.bpos_gpu = to_dev(geo.bpos.astype(np.int32), dev)
-        self.bpos_cg = to_dev(geo.bpos[sel].astype(np.int32), dev)   # solver row order
+        # boundary positions in each solve's row order
+        for key, order in zip(("u1", "u2"), self._sel):
+            bp = geo.bpos[order]
+            c = self._cg[key]
+            if key == "u2" and c["nd"] and not (np.all(bp[:c["n"]] < 0) and np.all(bp[c["n"]:] >= 0)):
+                raise RuntimeError("the boundary geometry and the Dirichlet rows of the FEM system differ")
+            c["bpos"] = to_dev(bp.astype(np.int32), dev)
+        del self._sel
         self.diag_jump_gpu = to_dev(geo.diag_jump, dev)
         self.B_dtype = _torch_dtype(B_dtype)
 
@@ -441,7 +469,8 @@ class DemagFieldLindholmGPU:
         its = C.c_int32(0)
         rn = C.c_double(0.0)
         sums = (C.c_double * 1)()
-        raw = self.A_raw
+        raw = c["raw"]
+        n, nd = c["n"], c["nd"]
         nbp = 0 if b_part is None else b_part.numel()
         amg_pc = c["pc"] == "gamg"
         mixed = None
@@ -454,10 +483,10 @@ class DemagFieldLindholmGPU:
             mixed = _capi.CGMixed(v32.data_ptr(), min(max(rin, 1e-300), 0.5), self.cg_refinements, 0)
         if amg_pc:
             A, amg = c["plain"], c["amg"]
-            args = [self.N, A.width, A.ld, _capi.ptr(A.cols), _capi.ptr(A.vals), mode,
+            args = [n, nd, A.width, A.ld, _capi.ptr(A.cols), _capi.ptr(A.vals), mode,
                     _capi.ptr(b), _capi.ptr(b_part), nbp, int(project), raw.width, raw.ld,
-                    _capi.ptr(raw.cols), _capi.ptr(raw.vals), _capi.ptr(self.bpos_cg),
-                    _capi.ptr(gb), _capi.ptr(x), int(warm), _capi.ptr(self.cg_perm),
+                    _capi.ptr(raw.cols), _capi.ptr(raw.vals), _capi.ptr(c["bpos"]),
+                    _capi.ptr(gb), _capi.ptr(x), int(warm), _capi.ptr(c["perm"]),
                     C.addressof(amg.c_levels), amg.nlev, amg.val_dtype, float(rtol),
                     float(self.ksp_atol), int(max_it), -1 if spec else 8, int(expected),
                     _capi.ptr(ws), C.byref(its), C.byref(rn), sums]
@@ -467,11 +496,11 @@ class DemagFieldLindholmGPU:
                 rc, what = L.smx_pcg_amg_mixed(*args, C.byref(mixed), _capi.stream_ptr()), "smx_pcg_amg_mixed"
         else:
             A = c["jacobi"]
-            args = [self.N, A.width, A.ld, _capi.ptr(A.cols), _capi.ptr(A.vals),
+            args = [n, nd, A.width, A.ld, _capi.ptr(A.cols), _capi.ptr(A.vals),
                     _capi.ptr(c["dinv"]), mode, _capi.ptr(b), _capi.ptr(b_part), nbp,
                     int(project), raw.width, raw.ld, _capi.ptr(raw.cols), _capi.ptr(raw.vals),
-                    _capi.ptr(self.bpos_cg), _capi.ptr(gb), _capi.ptr(x), int(warm),
-                    _capi.ptr(self.cg_perm), float(rtol), float(self.ksp_atol), int(max_it),
+                    _capi.ptr(c["bpos"]), _capi.ptr(gb), _capi.ptr(x), int(warm),
+                    _capi.ptr(c["perm"]), float(rtol), float(self.ksp_atol), int(max_it),
                     -2 if spec else 16, int(expected), _capi.ptr(ws), C.byref(its), C.byref(rn), sums]
             if mixed is None:
                 rc, what = L.smx_pcg_jacobi(*args, _capi.stream_ptr()), "smx_pcg_jacobi"

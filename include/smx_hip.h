@@ -417,14 +417,25 @@ int64_t smx_reduce_ws_size(int64_t n);
  * mode 1 (u2, Dirichlet lifting): A = A_bc; boundary rows take the data
  *     x_k = gb[bpos[k]] where bpos[k] >= 0; interior rows solve
  *     sum_j A_raw_kj x_j = 0 (raw stiffness in ELL, raw_width <= 32).
+ * nd (trailing identity rows, 0 for none): the system has n + nd rows, and
+ *     rows n .. n+nd-1 of the solver order are identity rows (the Dirichlet
+ *     rows of A_bc): their x is their data (mode 1: gb[bpos[row]]; mode 0:
+ *     b of that row, which needs project = 0), their residual is 0, and they
+ *     count in ||b|| as rows of the full system, so the tolerance is the full
+ *     system's.  No iteration touches them: cols/vals describe rows 0..n-1
+ *     only and must reference columns < n only (A_bc's interior block); the
+ *     raw operator of mode 1 (rows 0..n-1) may reference every column, and
+ *     bpos, perm, b and x cover n + nd rows.  An iteration then streams the
+ *     n interior rows alone.  With nd = 0 Dirichlet rows may also sit among
+ *     the n rows (mode 1 treats any row with bpos >= 0 as one).
  * perm (optional, NULL = identity): solver row i is caller row perm[i].  The
  *     matrices (cols/vals, raw), dinv and bpos are then given in solver order
  *     (P A P^T etc.); b, gb and x stay in the caller's order (b is gathered
  *     on entry, x scattered back on exit).  A bandwidth-reducing order keeps
  *     the SpMV neighbours of each XCD's contiguous row range in its own L2.
  * Convergence: ||r||_2 <= max(rtol*||b||_2, atol), b the (lifted) right side.
- * work: smx_pcg_ws_size(n) doubles.  On return work[SMX_PCG_SUMX] holds
- * sum(x) on the device (the nullspace projection of x, read by later kernels).
+ * work: smx_pcg_ws_size(n + nd) doubles.  On return work[SMX_PCG_SUMX] holds
+ * sum(x) over the n iterated rows on the device (the nullspace projection of x, read by later kernels).
  * its_out / rnorm_out / sums_out: host pointers filled on return (the call
  * synchronizes `stream` every `check_every` iterations; the first wait comes
  * after expected_its + 1 iterations when expected_its > 0).
@@ -454,7 +465,7 @@ int64_t smx_pcg_ws_size(int64_t n);
 int smx_pcg_prepare(int64_t n, int32_t width, int64_t ld, const int32_t* cols,
                     const double* vals, const double* dinv, int32_t* cols_out,
                     double* vals_out, void* stream);
-int smx_pcg_jacobi(int64_t n, int32_t width, int64_t ld, const int32_t* cols,
+int smx_pcg_jacobi(int64_t n, int64_t nd, int32_t width, int64_t ld, const int32_t* cols,
                    const double* vals, const double* dinv, int32_t mode,
                    const double* b, const double* b_part, int64_t nb_part,
                    int32_t project, int32_t raw_width, int64_t raw_ld,
@@ -495,7 +506,7 @@ typedef struct {
   int32_t pad;
 } smx_cg_mixed;
 #define SMX_PCG_ITERS0 7
-int smx_pcg_jacobi_mixed(int64_t n, int32_t width, int64_t ld, const int32_t* cols,
+int smx_pcg_jacobi_mixed(int64_t n, int64_t nd, int32_t width, int64_t ld, const int32_t* cols,
                          const double* vals, const double* dinv, int32_t mode,
                          const double* b, const double* b_part, int64_t nb_part,
                          int32_t project, int32_t raw_width, int64_t raw_ld,
@@ -532,7 +543,7 @@ int smx_pcg_jacobi_mixed(int64_t n, int32_t width, int64_t ld, const int32_t* co
  *
  * Everything else (ELL operator = plain A in solver order, modes, perm,
  * convergence test, status record, SMX_PCG_SUMX) is as smx_pcg_jacobi.
- * work: smx_pcg_amg_ws_size(n) doubles.
+ * work: smx_pcg_amg_ws_size(n + nd) doubles; levels[0].n == n.
  * ---------------------------------------------------------------------- */
 typedef struct {
   const int32_t* slice_off;  /* nslices + 1 slot offsets */
@@ -555,7 +566,7 @@ int64_t smx_pcg_amg_ws_size(int64_t n);
  * least smx_pcg_amg_ws_size(levels[0].n) doubles. */
 int smx_amg_apply(const smx_amg_level* levels, int32_t nlev, int32_t val_dtype, const double* r,
                   double* u, double* work, void* stream);
-int smx_pcg_amg(int64_t n, int32_t width, int64_t ld, const int32_t* cols, const double* vals,
+int smx_pcg_amg(int64_t n, int64_t nd, int32_t width, int64_t ld, const int32_t* cols, const double* vals,
                 int32_t mode, const double* b, const double* b_part, int64_t nb_part,
                 int32_t project, int32_t raw_width, int64_t raw_ld, const int32_t* raw_cols,
                 const double* raw_vals, const int32_t* bpos, const double* gb, double* x,
@@ -563,6 +574,17 @@ int smx_pcg_amg(int64_t n, int32_t width, int64_t ld, const int32_t* cols, const
                 int32_t val_dtype, double rtol, double atol, int32_t max_it,
                 int32_t check_every, int32_t expected_its, double* work, int32_t* its_out,
                 double* rnorm_out, double* sums_out, void* stream);
+/* Mixed-precision form of smx_pcg_amg (FP32 inner CG with the FP32 values of
+ * A in mixed->vals32, FP64 refinement), as smx_pcg_jacobi_mixed describes. */
+int smx_pcg_amg_mixed(int64_t n, int64_t nd, int32_t width, int64_t ld, const int32_t* cols,
+                      const double* vals, int32_t mode, const double* b, const double* b_part,
+                      int64_t nb_part, int32_t project, int32_t raw_width, int64_t raw_ld,
+                      const int32_t* raw_cols, const double* raw_vals, const int32_t* bpos,
+                      const double* gb, double* x, int32_t warm, const int32_t* perm,
+                      const smx_amg_level* levels, int32_t nlev, int32_t val_dtype, double rtol,
+                      double atol, int32_t max_it, int32_t check_every, int32_t expected_its,
+                      double* work, int32_t* its_out, double* rnorm_out, double* sums_out,
+                      const smx_cg_mixed* mixed, void* stream);
 
 /* ---------------------------------------------------------------------- *
  * (d) Fused effective field + LLG right-hand side.

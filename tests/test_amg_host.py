@@ -219,7 +219,7 @@ def test_solver_rejects_inconsistent_levels_on_the_host():
     lv[1].s_tpr = 8
     lv[1].b = lv[1].x = fake
     its, rn, sums = C.c_int32(0), C.c_double(0.0), (C.c_double * 1)()
-    args = [100, 7, 128, fake, fake, 0, fake, None, 0, 0, 0, 0, None, None, None, None, fake, 0, None,
+    args = [100, 0, 7, 128, fake, fake, 0, fake, None, 0, 0, 0, 0, None, None, None, None, fake, 0, None,
             C.addressof(lv), 2, _capi.SMX_F64, 1e-6, 0.0, 10, 8, 0, fake, C.byref(its), C.byref(rn), sums, None]
     assert L.smx_pcg_amg(*args) == _capi.SMX_ERR_SHAPE
     assert "level 0" in L.smx_last_error().decode()
@@ -227,8 +227,15 @@ def test_solver_rejects_inconsistent_levels_on_the_host():
     lv[0].r_tpr = 3
     assert L.smx_pcg_amg(*args) == _capi.SMX_ERR_VALUE
     lv[0].r_tpr = 4
-    args[21] = _capi.SMX_BF16
+    args[22] = _capi.SMX_BF16
     assert L.smx_pcg_amg(*args) == _capi.SMX_ERR_TYPE
+    args[22] = _capi.SMX_F64
+    # trailing identity rows: mode 0 only without the mean projection, n >= 1
+    args[1], args[10] = 5, 1
+    assert L.smx_pcg_amg(*args) == _capi.SMX_ERR_VALUE
+    assert "project" in L.smx_last_error().decode()
+    args[0], args[10] = 0, 0
+    assert L.smx_pcg_amg(*args) == _capi.SMX_ERR_SHAPE
     assert L.smx_amg_apply(C.addressof(lv), 2, _capi.SMX_F64, fake, None, fake, None) == _capi.SMX_ERR_VALUE
 
 

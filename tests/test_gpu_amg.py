@@ -116,7 +116,7 @@ def _solve(A, h, b, perm, mode=0, rtol=1e-12, raw=None, bpos=None, gb=None, max_
     pd = None if perm is None else torch.from_numpy(perm.astype(np.int32)).to(DEV)
     raw = raw or A
     its, rn, sums = C.c_int32(0), C.c_double(0.0), (C.c_double * 1)()
-    rc = L.smx_pcg_amg(n, A.width, A.ld, _capi.ptr(A.cols), _capi.ptr(A.vals), mode, _capi.ptr(bd),
+    rc = L.smx_pcg_amg(n, 0, A.width, A.ld, _capi.ptr(A.cols), _capi.ptr(A.vals), mode, _capi.ptr(bd),
                        _capi.ptr(bpart), 0 if b is None else 1, project, raw.width, raw.ld, _capi.ptr(raw.cols),
                        _capi.ptr(raw.vals), _capi.ptr(bpos), _capi.ptr(gb), _capi.ptr(x), 0, _capi.ptr(pd),
                        C.addressof(h.c_levels), h.nlev, h.val_dtype, rtol, 0.0, max_it, 8, 0,

@@ -71,7 +71,7 @@ def _solve(A, dinv, b, perm, rtol=1e-12, project=0):
     bpart = torch.tensor([float(b.sum())], dtype=torch.float64, device=DEV)
     pd = None if perm is None else torch.from_numpy(perm.astype(np.int32)).to(DEV)
     its, rn, sums = C.c_int32(0), C.c_double(0.0), (C.c_double * 1)()
-    rc = L.smx_pcg_jacobi(n, A.width, A.ld, _capi.ptr(A.cols), _capi.ptr(A.vals), _capi.ptr(dinv), 0,
+    rc = L.smx_pcg_jacobi(n, 0, A.width, A.ld, _capi.ptr(A.cols), _capi.ptr(A.vals), _capi.ptr(dinv), 0,
                           _capi.ptr(bd), _capi.ptr(bpart), 1, project, A.width, A.ld, _capi.ptr(A.cols),
                           _capi.ptr(A.vals), None, None, _capi.ptr(x), 0, _capi.ptr(pd), rtol, 0.0, 20000, 32,
                           0, _capi.ptr(ws), C.byref(its), C.byref(rn), sums, _capi.stream_ptr())
