@@ -85,6 +85,7 @@ __device__ __forceinline__ void sot_p(const smx_field_args& a, int64_t k, double
 // compact layouts give the same bits.  0 when the row has no component.
 __device__ __forceinline__ double dmi_row(const smx_field_args& a, int64_t e, int i, double v0, double v1,
                                           double v2) {
+#pragma clang fp contract(off)
   const int nc = a.dmi_ncomp > 0 ? a.dmi_ncomp : 9;
   const double* __restrict__ D = a.dmi_vals + (int64_t)nc * e;
   const double v[3] = {v0, v1, v2};
@@ -101,7 +102,104 @@ __device__ __forceinline__ double dmi_row(const smx_field_args& a, int64_t e, in
   return t;
 }
 
+// Everything after the slot loop: the node's own terms, the outputs and the
+// LLG RHS (shared by both forms of the field kernel).  The field kernels and
+// their helpers compile with contraction off (__dmul_rn and __dadd_rn are a
+// plain * and + in HIP, which the default fp-contract may fuse differently in
+// the two kernels): only the written fma calls fuse, so both forms give the
+// same bits.
+__device__ __forceinline__ double mulr(double x, double y) { return __dmul_rn(x, y); }
+__device__ __forceinline__ double addr(double x, double y) { return __dadd_rn(x, y); }
+// a x b, each component as fma(a_i, b_j, -(a_j b_i))
+__device__ __forceinline__ void cross3r(double ax, double ay, double az, double bx, double by,
+                                        double bz, double& cx, double& cy, double& cz) {
+  cx = fma(ay, bz, -mulr(az, by));
+  cy = fma(az, bx, -mulr(ax, bz));
+  cz = fma(ax, by, -mulr(ay, bx));
+}
+
+__device__ __forceinline__ void field_finish(const smx_field_args& a, int64_t k, bool stt, double hx,
+                                             double hy, double hz, double gx, double gy, double gz,
+                                             double zx, double zy, double zz) {
+#pragma clang fp contract(off)
+  const double mx = a.m[3 * k], my = a.m[3 * k + 1], mz = a.m[3 * k + 2];
+  if (a.ani_pref != 0.0) {
+    double nx, ny, nz;
+    if (a.ani_axis) {
+      nx = a.ani_axis[3 * k];
+      ny = a.ani_axis[3 * k + 1];
+      nz = a.ani_axis[3 * k + 2];
+    } else {
+      nx = a.ani_axis_u[0];
+      ny = a.ani_axis_u[1];
+      nz = a.ani_axis_u[2];
+    }
+    const double cdot = mulr(a.ani_pref, fma(mz, nz, fma(my, ny, mulr(mx, nx))));
+    hx = fma(cdot, nx, hx);
+    hy = fma(cdot, ny, hy);
+    hz = fma(cdot, nz, hz);
+  }
+  if (a.cub_pref != 0.0) cubic_term(a, k, mx, my, mz, nullptr, hx, hy, hz);
+  if (a.grad_vals) {
+    const double iv = a.inv_vol[k];
+    const double dx = mulr(gx, iv), dy = mulr(gy, iv), dz = mulr(gz, iv);
+    if (a.h_demag) {
+      a.h_demag[3 * k] = dx;
+      a.h_demag[3 * k + 1] = dy;
+      a.h_demag[3 * k + 2] = dz;
+    }
+    hx = addr(hx, dx);
+    hy = addr(hy, dy);
+    hz = addr(hz, dz);
+  }
+  if (a.h0) {
+    hx = addr(hx, a.h0[3 * k]);
+    hy = addr(hy, a.h0[3 * k + 1]);
+    hz = addr(hz, a.h0[3 * k + 2]);
+  } else {
+    hx = addr(hx, a.h0_u[0]);
+    hy = addr(hy, a.h0_u[1]);
+    hz = addr(hz, a.h0_u[2]);
+  }
+  if (a.h_eff) {
+    a.h_eff[3 * k] = hx;
+    a.h_eff[3 * k + 1] = hy;
+    a.h_eff[3 * k + 2] = hz;
+  }
+  if (a.rhs) {
+    double mcx, mcy, mcz, mcmx, mcmy, mcmz;
+    cross3r(mx, my, mz, hx, hy, hz, mcx, mcy, mcz);
+    cross3r(mx, my, mz, mcx, mcy, mcz, mcmx, mcmy, mcmz);
+    const double msq = fma(mz, mz, fma(my, my, mulr(mx, mx)));
+    const double stab = mulr(a.stab, __dsub_rn(1.0, msq));
+    double rx = fma(stab, mx, mulr(a.P, fma(a.alpha, mcmx, mulr(a.do_precess, mcx))));
+    double ry = fma(stab, my, mulr(a.P, fma(a.alpha, mcmy, mulr(a.do_precess, mcy))));
+    double rz = fma(stab, mz, mulr(a.P, fma(a.alpha, mcmz, mulr(a.do_precess, mcz))));
+    if (stt) {  // - pref (a m x Z + b m x (m x Z))
+      double c1x, c1y, c1z, c2x, c2y, c2z;
+      cross3(mx, my, mz, zx, zy, zz, c1x, c1y, c1z);
+      cross3(mx, my, mz, c1x, c1y, c1z, c2x, c2y, c2z);
+      rx -= a.stt_pref * (a.stt_a * c1x + a.stt_b * c2x);
+      ry -= a.stt_pref * (a.stt_a * c1y + a.stt_b * c2y);
+      rz -= a.stt_pref * (a.stt_a * c1z + a.stt_b * c2z);
+    }
+    if (a.sot_fl != 0.0 || a.sot_dl != 0.0) {  // c_fl m x p + c_dl m x (m x p)
+      double px, py, pz, c1x, c1y, c1z, c2x, c2y, c2z;
+      sot_p(a, k, px, py, pz);
+      cross3(mx, my, mz, px, py, pz, c1x, c1y, c1z);
+      cross3(mx, my, mz, c1x, c1y, c1z, c2x, c2y, c2z);
+      rx += a.sot_fl * c1x + a.sot_dl * c2x;
+      ry += a.sot_fl * c1y + a.sot_dl * c2y;
+      rz += a.sot_fl * c1z + a.sot_dl * c2z;
+    }
+    a.rhs[3 * k] = rx;
+    a.rhs[3 * k + 1] = ry;
+    a.rhs[3 * k + 2] = rz;
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void field_rhs_kernel(smx_field_args a) {
+#pragma clang fp contract(off)
   const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (k >= a.n) return;
   const int64_t s = k >> 6;
@@ -144,81 +242,157 @@ __global__ __launch_bounds__(kBlock) void field_rhs_kernel(smx_field_args a) {
       gz = fma(a.grad_vals[3 * e + 2], phi, gz);
     }
   }
-  const double mx = a.m[3 * k], my = a.m[3 * k + 1], mz = a.m[3 * k + 2];
-  if (a.ani_pref != 0.0) {
-    double nx, ny, nz;
-    if (a.ani_axis) {
-      nx = a.ani_axis[3 * k];
-      ny = a.ani_axis[3 * k + 1];
-      nz = a.ani_axis[3 * k + 2];
-    } else {
-      nx = a.ani_axis_u[0];
-      ny = a.ani_axis_u[1];
-      nz = a.ani_axis_u[2];
+  field_finish(a, k, stt, hx, hy, hz, gx, gy, gz, zx, zy, zz);
+}
+
+// The field kernel for the common term sets (exchange, DMI in one of the
+// compact layouts below or none, demag gradient or none, uniaxial
+// anisotropy; no cubic anisotropy or spin torques), with the term set and the DMI layout as template parameters.  The
+// general kernel tests the optional terms inside its slot loop, so the
+// compiler waits for each slot's loads at every test; here every load of U
+// consecutive slots (columns, operator values, then the m and phi gathers) is
+// issued before the first of their fmas, which run in the general kernel's
+// order, so both kernels give the same bits.  DM: the stored DMI components
+// as a mask of q = 3 i + j, stored in increasing q (0: no DMI).
+template <int DM>
+__device__ __forceinline__ double dmi_row_m(const double* D, int i, double v0, double v1, double v2) {
+#pragma clang fp contract(off)
+  const double v[3] = {v0, v1, v2};
+  double t = 0.0;
+  bool any = false;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const int q = 3 * i + j;
+    if ((DM >> q) & 1) {
+      const int o = __builtin_popcount(DM & ((1 << q) - 1));
+      t = any ? fma(D[o], v[j], t) : __dmul_rn(D[o], v[j]);
+      any = true;
     }
-    const double cdot = a.ani_pref * (mx * nx + my * ny + mz * nz);
-    hx += cdot * nx;
-    hy += cdot * ny;
-    hz += cdot * nz;
   }
-  if (a.cub_pref != 0.0) cubic_term(a, k, mx, my, mz, nullptr, hx, hy, hz);
+  return t;
+}
+
+// 202 005 nodes fill about three wavefronts per SIMD, so the kernel may use
+// the registers of three (the loads of U slots in flight are its speed).
+template <int DM, bool GRAD, int U>
+__global__ __launch_bounds__(kBlock, 3) void field_rhs_fast_kernel(smx_field_args a) {
+#pragma clang fp contract(off)
+  constexpr int NC = __builtin_popcount(DM);
+  constexpr int NCS = NC > 0 ? NC : 1;
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (k >= a.n) return;
+  const int64_t s = k >> 6;  // one slice per wavefront: w is wave-uniform
+  const int lane = (int)(k & 63);
+  const int64_t start = a.slice_ptr[s];
+  const int64_t w = (a.slice_ptr[s + 1] - start) >> 6;
+  const double shift = (GRAD && a.u1_sum) ? (*a.u1_sum) / (double)a.n : 0.0;
+  double hx = 0.0, hy = 0.0, hz = 0.0;
+  double gx = 0.0, gy = 0.0, gz = 0.0;
+  for (int64_t j0 = 0; j0 < w; j0 += U) {
+    int32_t c[U];
+    int64_t es[U];
+    double ev[U], dv[U][NCS], gv[U][3];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      // slots past the slice width re-read the last one; their sums are skipped
+      es[u] = start + (j0 + u < w ? j0 + u : w - 1) * 64 + lane;
+      c[u] = a.cols[es[u]];
+    }
+    // the columns first, then the operator values, then (once the columns
+    // are in) the gathers: every wait below is for loads issued long before
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t e = es[u];
+      ev[u] = a.ex_vals[e];
+#pragma unroll
+      for (int q = 0; q < NC; ++q) dv[u][q] = a.dmi_vals[(int64_t)NC * e + q];
+      if (GRAD) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) gv[u][q] = a.grad_vals[3 * e + q];
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    double m0[U], m1[U], m2[U], ph[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t cc = c[u];
+      m0[u] = a.m[3 * cc];
+      m1[u] = a.m[3 * cc + 1];
+      m2[u] = a.m[3 * cc + 2];
+      if (GRAD) ph[u] = (a.u1[cc] - shift) + a.u2[cc];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      // a slot past the width keeps the sums by a select, not a branch: a
+      // branch would let the compiler sink that slot's loads behind it
+      const bool ok = u == 0 || j0 + u < w;
+      double tx = fma(ev[u], m0[u], hx);
+      double ty = fma(ev[u], m1[u], hy);
+      double tz = fma(ev[u], m2[u], hz);
+      if (NC > 0) {
+        tx += dmi_row_m<DM>(dv[u], 0, m0[u], m1[u], m2[u]);
+        ty += dmi_row_m<DM>(dv[u], 1, m0[u], m1[u], m2[u]);
+        tz += dmi_row_m<DM>(dv[u], 2, m0[u], m1[u], m2[u]);
+      }
+      hx = ok ? tx : hx;
+      hy = ok ? ty : hy;
+      hz = ok ? tz : hz;
+      if (GRAD) {
+        const double sx = fma(gv[u][0], ph[u], gx);
+        const double sy = fma(gv[u][1], ph[u], gy);
+        const double sz = fma(gv[u][2], ph[u], gz);
+        gx = ok ? sx : gx;
+        gy = ok ? sy : gy;
+        gz = ok ? sz : gz;
+      }
+    }
+  }
+  field_finish(a, k, false, hx, hy, hz, gx, gy, gz, 0.0, 0.0, 0.0);
+}
+
+// The DMI layout as a component mask when it is the compact layout (present
+// components in increasing q) or the full one; -1 otherwise.
+int dmi_mask(const smx_field_args& a) {
+  if (!a.dmi_vals) return 0;
+  if (a.dmi_ncomp == 0) return 0x1FF;
+  int mask = 0, pos = 0;
+  for (int q = 0; q < 9; ++q) {
+    if (a.dmi_off[q] < 0) continue;
+    if (a.dmi_off[q] != pos) return -1;
+    mask |= 1 << q;
+    ++pos;
+  }
+  return pos == a.dmi_ncomp ? mask : -1;
+}
+
+constexpr int kFieldU = 4;
+
+template <int DM>
+bool launch_field_fast_dm(const smx_field_args& a, hipStream_t st) {
+  const dim3 g(smx::grid_for(a.n, kBlock)), b(kBlock);
   if (a.grad_vals) {
-    const double iv = a.inv_vol[k];
-    const double dx = gx * iv, dy = gy * iv, dz = gz * iv;
-    if (a.h_demag) {
-      a.h_demag[3 * k] = dx;
-      a.h_demag[3 * k + 1] = dy;
-      a.h_demag[3 * k + 2] = dz;
-    }
-    hx += dx;
-    hy += dy;
-    hz += dz;
-  }
-  if (a.h0) {
-    hx += a.h0[3 * k];
-    hy += a.h0[3 * k + 1];
-    hz += a.h0[3 * k + 2];
+    hipLaunchKernelGGL((field_rhs_fast_kernel<DM, true, kFieldU>), g, b, 0, st, a);
   } else {
-    hx += a.h0_u[0];
-    hy += a.h0_u[1];
-    hz += a.h0_u[2];
+    hipLaunchKernelGGL((field_rhs_fast_kernel<DM, false, kFieldU>), g, b, 0, st, a);
   }
-  if (a.h_eff) {
-    a.h_eff[3 * k] = hx;
-    a.h_eff[3 * k + 1] = hy;
-    a.h_eff[3 * k + 2] = hz;
-  }
-  if (a.rhs) {
-    const double mcx = my * hz - mz * hy;
-    const double mcy = mz * hx - mx * hz;
-    const double mcz = mx * hy - my * hx;
-    const double mcmx = my * mcz - mz * mcy;
-    const double mcmy = mz * mcx - mx * mcz;
-    const double mcmz = mx * mcy - my * mcx;
-    const double stab = a.stab * (1.0 - (mx * mx + my * my + mz * mz));
-    double rx = a.P * (a.do_precess * mcx + a.alpha * mcmx) + stab * mx;
-    double ry = a.P * (a.do_precess * mcy + a.alpha * mcmy) + stab * my;
-    double rz = a.P * (a.do_precess * mcz + a.alpha * mcmz) + stab * mz;
-    if (stt) {  // - pref (a m x Z + b m x (m x Z))
-      double c1x, c1y, c1z, c2x, c2y, c2z;
-      cross3(mx, my, mz, zx, zy, zz, c1x, c1y, c1z);
-      cross3(mx, my, mz, c1x, c1y, c1z, c2x, c2y, c2z);
-      rx -= a.stt_pref * (a.stt_a * c1x + a.stt_b * c2x);
-      ry -= a.stt_pref * (a.stt_a * c1y + a.stt_b * c2y);
-      rz -= a.stt_pref * (a.stt_a * c1z + a.stt_b * c2z);
-    }
-    if (a.sot_fl != 0.0 || a.sot_dl != 0.0) {  // c_fl m x p + c_dl m x (m x p)
-      double px, py, pz, c1x, c1y, c1z, c2x, c2y, c2z;
-      sot_p(a, k, px, py, pz);
-      cross3(mx, my, mz, px, py, pz, c1x, c1y, c1z);
-      cross3(mx, my, mz, c1x, c1y, c1z, c2x, c2y, c2z);
-      rx += a.sot_fl * c1x + a.sot_dl * c2x;
-      ry += a.sot_fl * c1y + a.sot_dl * c2y;
-      rz += a.sot_fl * c1z + a.sot_dl * c2z;
-    }
-    a.rhs[3 * k] = rx;
-    a.rhs[3 * k + 1] = ry;
-    a.rhs[3 * k + 2] = rz;
+  return true;
+}
+
+// The fast form for exchange + (DMI in a listed layout) + (demag) +
+// (uniaxial) without cubic anisotropy or spin torques; false: the general
+// kernel must run.
+bool launch_field_fast(const smx_field_args& a, hipStream_t st) {
+  const bool stt = a.stt_vals && a.stt_pref != 0.0 && a.rhs;
+  if (stt || !a.ex_vals || a.cub_pref != 0.0 || a.sot_fl != 0.0 || a.sot_dl != 0.0) return false;
+  switch (dmi_mask(a)) {
+    case 0: return launch_field_fast_dm<0>(a, st);
+    case 0xE4: return launch_field_fast_dm<0xE4>(a, st);   // interfacial, n0 along z
+    case 0x4E: return launch_field_fast_dm<0x4E>(a, st);   // interfacial, n0 along x
+    case 0xAA: return launch_field_fast_dm<0xAA>(a, st);   // interfacial, n0 along y
+    case 0xEE: return launch_field_fast_dm<0xEE>(a, st);   // bulk (no diagonal)
+    case 0x1FF: return launch_field_fast_dm<0x1FF>(a, st);  // full blocks
+    default: return false;
   }
 }
 
@@ -495,8 +669,10 @@ extern "C" int smx_field_llg_rhs(const smx_field_args* a, void* stream) {
   int rc = check_field_args(a);
   if (rc) return rc;
   if (a->n == 0) return SMX_OK;
-  hipLaunchKernelGGL(field_rhs_kernel, dim3(smx::grid_for(a->n, kBlock)), dim3(kBlock), 0,
-                     smx::as_stream(stream), *a);
+  if (!launch_field_fast(*a, smx::as_stream(stream))) {
+    hipLaunchKernelGGL(field_rhs_kernel, dim3(smx::grid_for(a->n, kBlock)), dim3(kBlock), 0,
+                       smx::as_stream(stream), *a);
+  }
   SMX_LAUNCH_CHECK();
   return SMX_OK;
 }
