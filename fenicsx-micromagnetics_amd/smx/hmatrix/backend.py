@@ -57,7 +57,30 @@ ZTASK_DTYPE = np.dtype([("v_off", np.int64), ("z_off", np.int64), ("c0", np.int3
                         ("kc", np.int32), ("pad", np.int32)])   # smx_hmat_ztask, 32 bytes
 
 
-def pack_operator(nb, row_start, row_stop, col_start, col_stop, rank, Ufac, Vfac, near_csr, fdt, near_dt):
+def _xcd_slots(work, per_wg):
+    """Work items in position order (``work`` per item) cut into eight
+    groups of equal total work, each largest first, laid out so that
+    workgroup 8 i + x (``per_wg`` consecutive slots) takes group x's i-th
+    items: the dispatcher sends workgroup b to XCD b % 8.  Returns the item
+    of every slot, -1 for the padding of the shorter groups."""
+    work = np.asarray(work, dtype=np.int64)
+    before = np.cumsum(work) - work
+    grp = np.minimum(before * 8 // max(int(work.sum()), 1), 7)
+    lists = []
+    for x in range(8):
+        sel = np.flatnonzero(grp == x)
+        lists.append(sel[np.argsort(-work[sel], kind="stable")])
+    nwg = max((len(l) + per_wg - 1) // per_wg for l in lists)
+    table = np.full((nwg, 8, per_wg), -1, dtype=np.int64)
+    for x, l in enumerate(lists):
+        col = np.full(nwg * per_wg, -1, dtype=np.int64)
+        col[:len(l)] = l
+        table[:, x, :] = col.reshape(nwg, per_wg)
+    return table.reshape(-1)
+
+
+def pack_operator(nb, row_start, row_stop, col_start, col_stop, rank, Ufac, Vfac, near_csr, fdt, near_dt,
+                  xcd_order=True):
     """Host arrays of the packed layout for y = D x + Near x + sum_b U_b V_b x.
 
     Ufac[b] is (m_b x r_b), Vfac[b] is (r_b x n_b).  Returns a dict of numpy
@@ -87,24 +110,37 @@ def pack_operator(nb, row_start, row_stop, col_start, col_stop, rank, Ufac, Vfac
     z_off = np.zeros(nfar, dtype=np.int64)
     if nfar:
         z_off[1:] = np.cumsum(rk)[:-1]
-    def tasks(group):
+    def tasks(group, xcd=False):
         """Tasks of <= group rank rows of one block (rows contiguous in V),
-        largest first."""
+        largest first.  xcd: the tasks are cut by column position into eight
+        groups of equal work, each largest first, and interleaved so that
+        workgroup 8 i + x (of four tasks) runs group x's i-th four on XCD x:
+        each XCD's L2 then holds the x slices of one eighth of the columns
+        instead of all of them.  Short groups are padded with empty tasks
+        (n = kc = 0)."""
         ng = (rk + group - 1) // group
         tb = np.repeat(np.arange(nfar, dtype=np.int64), ng)
         k0 = (np.arange(int(ng.sum()), dtype=np.int64) - np.repeat(np.cumsum(ng) - ng, ng)) * group
         kc = np.minimum(group, rk[tb] - k0)
-        order = np.argsort(-(n[tb] * kc), kind="stable")
-        tb, k0, kc = tb[order], k0[order], kc[order]
-        zt = np.zeros(max(tb.size, 1), dtype=ZTASK_DTYPE)
-        zt["v_off"][:tb.size] = v_off[tb] + k0 * ld[tb]
-        zt["z_off"][:tb.size] = z_off[tb] + k0
-        zt["c0"][:tb.size] = c0[tb]
-        zt["n"][:tb.size] = n[tb]
-        zt["kc"][:tb.size] = kc
-        return zt, int(tb.size)
+        work = n[tb] * kc
+        if xcd and tb.size:
+            oc = np.lexsort((k0, tb, c0[tb]))             # by first column, block, rank row
+            slots = _xcd_slots(work[oc], 4)
+            order = np.where(slots >= 0, oc[np.maximum(slots, 0)], -1)
+        else:
+            order = np.argsort(-work, kind="stable")
+        live = order >= 0
+        ti = order[live]
+        tb, k0, kc = tb[ti], k0[ti], kc[ti]
+        zt = np.zeros(max(order.size, 1), dtype=ZTASK_DTYPE)   # empty tasks: n = kc = 0
+        zt["v_off"][live] = v_off[tb] + k0 * ld[tb]
+        zt["z_off"][live] = z_off[tb] + k0
+        zt["c0"][live] = c0[tb]
+        zt["n"][live] = n[tb]
+        zt["kc"][live] = kc
+        return zt, int(order.size)
 
-    zt, nzt = tasks(_ZG)
+    zt, nzt = tasks(_ZG, xcd=xcd_order)
     # 16-row tasks for the multi-RHS product: one MFMA tile of rank rows each
     zt16, nzt16 = tasks(16)
     P.update(V=V, ztask=zt.view(np.int64), nztask=nzt, ztask16=zt16.view(np.int64), nztask16=nzt16)
@@ -135,13 +171,18 @@ def pack_operator(nb, row_start, row_stop, col_start, col_stop, rank, Ufac, Vfac
     near.sort_indices()
     nptr0 = near.indptr.astype(np.int64)
     work = cl * kw * fb + (nptr0[stops] - nptr0[starts]) * (nv + 4)
-    lo = np.argsort(-work, kind="stable")        # storage slot -> leaf
-    slot = np.empty(nleaf, dtype=np.int64)
-    slot[lo] = np.arange(nleaf)
-    u_off_s = np.zeros(nleaf, dtype=np.int64)
-    if nleaf:
-        u_off_s[1:] = np.cumsum((cl * kw)[lo])[:-1]
-    u_off = u_off_s[slot]                        # per leaf
+    # storage slot -> leaf (-1: an empty padding slot); as the z tasks, the
+    # leaves are cut by row position into eight groups of equal work, each
+    # largest first, interleaved so that workgroup 8 i + x takes group x's
+    # i-th leaf on XCD x (its L2 then holds the z entries of one eighth of
+    # the rows' blocks)
+    lo = _xcd_slots(work, 1) if (xcd_order and nleaf) else np.argsort(-work, kind="stable")
+    live = lo >= 0
+    lol = lo[live]
+    u_off = np.zeros(nleaf, dtype=np.int64)      # per leaf
+    sz = np.zeros(lo.size, dtype=np.int64)
+    sz[live] = (cl * kw)[lol]
+    u_off[lol] = (np.cumsum(sz) - sz)[live]
     U = np.zeros(max(int((cl * kw).sum()), 4), dtype=fdt)
     o_rows = starts[leaf] - r0[blk]
     for e in range(blk.size):
@@ -153,11 +194,14 @@ def pack_operator(nb, row_start, row_stop, col_start, col_stop, rank, Ufac, Vfac
                                             - np.repeat(np.cumsum(rke) - rke, rke))
     gptr = np.zeros(nleaf + 1, dtype=np.int64)
     gptr[1:] = np.cumsum(K)
-    zg = np.concatenate([zg_leaf[gptr[L]:gptr[L + 1]] for L in lo]) if nleaf else np.zeros(0, np.int64)
-    zg_off = np.zeros(nleaf + 1, dtype=np.int64)
-    zg_off[1:] = np.cumsum(K[lo])
-    P.update(leaf_start=starts[lo].astype(np.int32), leaf_stop=stops[lo].astype(np.int32),
-             leaf_u_off=u_off[lo], leaf_kw=kw[lo].astype(np.int32), leaf_zg_off=zg_off,
+    zg = np.concatenate([zg_leaf[gptr[L]:gptr[L + 1]] for L in lol]) if nleaf else np.zeros(0, np.int64)
+    kslot = np.zeros(lo.size, dtype=np.int64)
+    kslot[live] = K[lol]
+    zg_off = np.zeros(lo.size + 1, dtype=np.int64)
+    zg_off[1:] = np.cumsum(kslot)
+    pick = lambda a: np.where(live, a[np.maximum(lo, 0)], 0) if nleaf else a[lo]
+    P.update(leaf_start=pick(starts).astype(np.int32), leaf_stop=pick(stops).astype(np.int32),
+             leaf_u_off=pick(u_off), leaf_kw=pick(kw).astype(np.int32), leaf_zg_off=zg_off,
              leaf_zg=(zg if zg.size else np.zeros(1, np.int64)).astype(np.int32), U=U,
              kw_max=int(kw.max()) if nleaf else 0)
     # ---- near rows in aligned pairs ----
@@ -307,8 +351,14 @@ def near_from_tiles(nb, tiles, row, col, val):
 class HMatrixGPUPackedFused:
     def __init__(self, data: HMatrixCPUData, threads_per_block: int = 128, use_fp32: bool = True,
                  device=None, near_fp32: bool = False, boundary_gather: np.ndarray | None = None,
-                 near_in_z="auto"):
+                 near_in_z="auto", xcd_order: bool | None = None):
         self.data = data
+        # xcd_order: z tasks and leaves in per-XCD position groups
+        # (pack_operator); False: one list each, largest first.  None: with
+        # FP32 factors (film matvec 310 -> 300 us, 64-RHS product 1454 ->
+        # 1424 us); FP64 factors keep the size order (matvec unchanged,
+        # 64-RHS product 2028 -> 2070 us with the groups).
+        self.xcd_order = bool(use_fp32) if xcd_order is None else bool(xcd_order)
         # near_in_z: the near rows are summed by a launch of their own (one
         # wavefront per row) instead of inside the leaf launch.  It pays when
         # the leaves are too few to fill the device (the 150 nm cube has 84);
@@ -335,7 +385,8 @@ class HMatrixGPUPackedFused:
         self.nfar = len(fb)
         host = pack_operator(nb, [b.row_start for b in fb], [b.row_stop for b in fb],
                              [b.col_start for b in fb], [b.col_stop for b in fb], [b.rank for b in fb],
-                             [b.U for b in fb], [b.V for b in fb], data.near_csr, self._fdt, self._ndt)
+                             [b.U for b in fb], [b.V for b in fb], data.near_csr, self._fdt, self._ndt,
+                             xcd_order=self.xcd_order)
         self.far_storage_entries = host["far_storage_entries"]
         self.near_nnz = host["near_nnz"]
         self.fwd = self._upload(host)
@@ -421,7 +472,7 @@ class HMatrixGPUPackedFused:
             host = pack_operator(self.nb, [b.col_start for b in fb], [b.col_stop for b in fb],
                                  [b.row_start for b in fb], [b.row_stop for b in fb], [b.rank for b in fb],
                                  [np.asarray(b.V).T for b in fb], [np.asarray(b.U).T for b in fb],
-                                 self.data.near_csr.T.tocsr(), self._fdt, self._ndt)
+                                 self.data.near_csr.T.tocsr(), self._fdt, self._ndt, xcd_order=self.xcd_order)
             self._tpk = self._upload(host)
         return self._tpk
 

@@ -233,6 +233,9 @@ def _emulate_packed(pk, nb, perm, diag_p, x):
     tasks = pk["ztask"].view(ZTASK_DTYPE)[:pk["nztask"]]
     for t in tasks:
         n, kc, c0, vo, zo = int(t["n"]), int(t["kc"]), int(t["c0"]), int(t["v_off"]), int(t["z_off"])
+        if kc == 0:                      # padding of a short XCD group: no work
+            assert n == 0
+            continue
         assert 1 <= kc <= zG and c0 >= 0 and c0 + n <= nb
         ld = (n + 3) // 4 * 4
         assert vo % 4 == 0 and vo + kc * ld <= V.size and zo + kc <= z.size
@@ -248,6 +251,9 @@ def _emulate_packed(pk, nb, perm, diag_p, x):
     for L in range(pk["leaf_start"].size):
         l0, l1 = int(pk["leaf_start"][L]), int(pk["leaf_stop"][L])
         c, kw, uo = l1 - l0, int(pk["leaf_kw"][L]), int(pk["leaf_u_off"][L])
+        if c == 0:                       # padding of a short XCD group: no rows, no ranks
+            assert kw == 0 and zgo[L + 1] == zgo[L]
+            continue
         assert 0 < c <= 64 and kw % 4 == 0 and uo % 4 == 0 and uo + c * kw <= U.size
         assert kw <= pk["kw_max"]
         covered[l0:l1] += 1
@@ -270,26 +276,43 @@ def _emulate_packed(pk, nb, perm, diag_p, x):
     return y
 
 
-def _pack(data, transpose=False, fdt=np.float64):
+def _pack(data, transpose=False, fdt=np.float64, xcd_order=True):
     from smx.hmatrix.backend import pack_operator
     fb = data.far_blocks
     if transpose:
         return pack_operator(data.size, [b.col_start for b in fb], [b.col_stop for b in fb],
                              [b.row_start for b in fb], [b.row_stop for b in fb], [b.rank for b in fb],
-                             [b.V.T for b in fb], [b.U.T for b in fb], data.near_csr.T.tocsr(), fdt, np.float64)
+                             [b.V.T for b in fb], [b.U.T for b in fb], data.near_csr.T.tocsr(), fdt, np.float64,
+                             xcd_order=xcd_order)
     return pack_operator(data.size, [b.row_start for b in fb], [b.row_stop for b in fb],
                          [b.col_start for b in fb], [b.col_stop for b in fb], [b.rank for b in fb],
-                         [b.U for b in fb], [b.V for b in fb], data.near_csr, fdt, np.float64)
+                         [b.U for b in fb], [b.V for b in fb], data.near_csr, fdt, np.float64, xcd_order=xcd_order)
 
 
-def test_packed_tables_and_matvec(c1):
+@pytest.mark.parametrize("xcd_order", [True, False])
+def test_packed_tables_and_matvec(c1, xcd_order):
+    """Both work orders (per-XCD position groups with padding, or one
+    largest-first list) hold every task and leaf once and give the matvec."""
     _, geo, bd = c1
     data = build_hmatrix_device(geo, HMatrixBuildConfig(leaf_size=32), ops=HostOps(geo, bd))
-    pk = _pack(data)
+    pk = _pack(data, xcd_order=xcd_order)
     perm = np.asarray(data.perm)
     x = np.random.default_rng(0).standard_normal(geo.Nb)
     ref = _matvec(data, x)
     assert np.abs(_emulate_packed(pk, data.size, perm, data.diag_jump[perm], x) - ref).max() < 1e-12
+    if xcd_order:
+        # workgroup b runs on XCD b % 8: its slots must come from group b % 8,
+        # whose first columns (z tasks) / rows (leaves) rise with the group
+        from smx.hmatrix.backend import ZTASK_DTYPE
+        t = pk["ztask"].view(ZTASK_DTYPE)[:pk["nztask"]]
+        wg = np.arange(t.size) // 4
+        live = t["kc"] > 0
+        lo = [t["c0"][live & (wg % 8 == x)].min() for x in range(8) if np.any(live & (wg % 8 == x))]
+        assert lo == sorted(lo)
+        rows = pk["leaf_stop"] > pk["leaf_start"]
+        first = [pk["leaf_start"][rows & (np.arange(rows.size) % 8 == x)].min() for x in range(8)
+                 if np.any(rows & (np.arange(rows.size) % 8 == x))]
+        assert first == sorted(first)
     assert pk["far_storage_entries"] == data.far_storage_entries
     assert pk["near_nnz"] == data.near_csr.nnz
     # FP32 factors: same tables, rounding only in the factors
@@ -450,6 +473,9 @@ def test_sixteen_row_task_table(c1):
         hits = np.zeros(pk["nz"], dtype=int)
         for t in tasks:
             n, kc, vo, zo, c0 = int(t["n"]), int(t["kc"]), int(t["v_off"]), int(t["z_off"]), int(t["c0"])
+            if kc == 0:                  # padding of a short XCD group
+                assert n == 0
+                continue
             ld = (n + 3) // 4 * 4
             assert 1 <= kc <= cap and vo % 4 == 0 and vo + kc * ld <= V.size and zo + kc <= pk["nz"]
             assert 0 <= c0 and c0 + n <= d.size
@@ -460,7 +486,8 @@ def test_sixteen_row_task_table(c1):
 
     z8 = zvec(pk["ztask"].view(ZTASK_DTYPE)[:pk["nztask"]], 8)
     z16 = zvec(pk["ztask16"].view(ZTASK_DTYPE)[:pk["nztask16"]], 16)
-    assert pk["nztask16"] < pk["nztask"]
+    live8 = int(np.count_nonzero(pk["ztask"].view(ZTASK_DTYPE)[:pk["nztask"]]["kc"]))
+    assert pk["nztask16"] < live8
     assert np.abs(z8 - z16).max() <= 1e-12
 
 
