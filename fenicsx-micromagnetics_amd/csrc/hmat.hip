@@ -381,6 +381,37 @@ __global__ __launch_bounds__(256) void hm_near_kernel(smx_hmatrix H) {
   if (lane == 0) H.ynear[p] = s;
 }
 
+// Near rows by task (one workgroup per task): the task's columns of xp are
+// staged in LDS once, then each wavefront sums whole rows as hm_near_kernel
+// does (the same entries per lane, the same fma order, the same wave total),
+// reading a 2-byte slot per entry instead of a 4-byte column and x from LDS
+// instead of the L2.
+template <typename TN>
+__global__ __launch_bounds__(256) void hm_near_lds_kernel(smx_hmatrix H) {
+  extern __shared__ double xs[];
+  const smx_hmat_ntask T = H.near_task[blockIdx.x];
+  const int lane = threadIdx.x & 63;
+  const int wid = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int32_t* __restrict__ stg = H.near_stage + T.s_off;
+  for (int i = threadIdx.x; i < T.w; i += 256) xs[i] = H.xp[stg[i]];
+  __syncthreads();
+  const TN* __restrict__ NV = reinterpret_cast<const TN*>(H.near_val);
+  const uint32_t* __restrict__ LI = reinterpret_cast<const uint32_t*>(H.near_lidx);
+  for (int64_t p = T.p0 + wid; p < T.p1; p += 4) {
+    double s = 0.0;
+    const int64_t n1 = H.near_ptr[p + 1];
+    for (int64_t e = H.near_ptr[p] + 2 * lane; e < n1; e += 128) {
+      double v0, v1;
+      load2(NV + e, v0, v1);
+      const uint32_t c = __builtin_nontemporal_load(LI + (e >> 1));  // slots e, e+1 (e even)
+      s = fma(v0, xs[c & 0xffffu], s);
+      s = fma(v1, xs[c >> 16], s);
+    }
+    s = smx::wave_total(s);
+    if (lane == 0) H.ynear[p] = s;
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256, 8) void hm_far_z_kernel(smx_hmatrix H) {
   constexpr bool kF32 = sizeof(T) == 4;
@@ -602,11 +633,24 @@ int hmat_matvec_impl(const smx_hmatrix* H, const double* src, const double* shif
               "too many leaves or z tasks");
   SMX_REQUIRE(!H->ynear || !H->near_ptr || (H->near_idx && H->near_val), SMX_ERR_VALUE,
               "near arrays missing");
+  const bool near_tasks = H->ynear && H->near_ptr && H->near_task;
+  SMX_REQUIRE(!near_tasks || (H->near_stage && H->near_lidx && H->n_near_task > 0 &&
+                              H->n_near_task <= 0x7fffffffLL && H->near_w_max >= 0 &&
+                              H->near_w_max <= SMX_HMAT_NEAR_W_MAX),
+              SMX_ERR_VALUE, "near task arrays missing or near_w_max beyond %d", SMX_HMAT_NEAR_W_MAX);
   hipStream_t st = smx::as_stream(stream);
   hipLaunchKernelGGL(hm_gather_kernel, dim3(smx::grid_for(H->nb, 256)), dim3(256), 0, st, H->nb,
                      src, H->gidx, shift, shift_scale, H->xp);
   SMX_LAUNCH_CHECK();
-  if (H->ynear && H->near_ptr) {
+  if (near_tasks) {
+    const size_t lds = (size_t)H->near_w_max * sizeof(double);
+    const unsigned gn = (unsigned)H->n_near_task;
+    if (H->near_dtype == SMX_F32)
+      hipLaunchKernelGGL(hm_near_lds_kernel<float>, dim3(gn), dim3(256), lds, st, *H);
+    else
+      hipLaunchKernelGGL(hm_near_lds_kernel<double>, dim3(gn), dim3(256), lds, st, *H);
+    SMX_LAUNCH_CHECK();
+  } else if (H->ynear && H->near_ptr) {
     const unsigned gn = (unsigned)((H->nb + 3) / 4);
     if (H->near_dtype == SMX_F32)
       hipLaunchKernelGGL(hm_near_kernel<float>, dim3(gn), dim3(256), 0, st, *H);

@@ -21,6 +21,7 @@ TRI_FIELDS = 34
 LINDHOLM_TILE_COLS = 1024
 PCG_DONE, PCG_ITERS, PCG_FAIL, PCG_SUMX, PCG_ITERS0 = 0, 1, 5, 6, 7
 HMAT_Z_GROUP = 8
+HMAT_NEAR_W_MAX = 8192
 HMAT_KW_MAX = 16384
 SVD_KMAX = 128
 ACA_TILE_ROWS = 64
@@ -86,6 +87,8 @@ class HMatrixArgs(C.Structure):
         ("nleaf", I64), ("leaf_start", P), ("leaf_stop", P), ("leaf_u_off", P), ("leaf_kw", P),
         ("leaf_zg_off", P), ("leaf_zg", P), ("kw_max", I32), ("leaf_waves", I32),
         ("U", P), ("xp", P), ("z", P), ("ynear", P),
+        ("near_task", P), ("n_near_task", I64), ("near_stage", P), ("near_lidx", P),
+        ("near_w_max", I32), ("pad_near", I32),
     ]
 
 

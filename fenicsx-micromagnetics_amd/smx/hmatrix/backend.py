@@ -218,6 +218,7 @@ def pack_operator(nb, row_start, row_stop, col_start, col_stop, rank, Ufac, Vfac
     if idx.size == 0:
         idx = np.zeros(2, dtype=np.int64)
     P.update(near_ptr=ptr, near_idx=idx.astype(np.int32), near_val=val)
+    P.update(pack_near_tasks(ptr, idx, starts, stops, xcd_order))
     P["far_storage_entries"] = int((m * rk).sum() + (n * rk).sum())
     P["near_nnz"] = int(near.nnz)
     P["nz"] = int(rk.sum())
@@ -225,6 +226,54 @@ def pack_operator(nb, row_start, row_stop, col_start, col_stop, rank, Ufac, Vfac
         raise ValueError(f"a leaf carries {P['kw_max']} far ranks; the matvec stages at most "
                          f"{_capi.HMAT_KW_MAX} in LDS")
     return P
+
+
+NTASK_DTYPE = np.dtype([("s_off", np.int64), ("p0", np.int32), ("p1", np.int32), ("w", np.int32),
+                        ("pad", np.int32)])   # smx_hmat_ntask, 24 bytes
+
+
+def pack_near_tasks(ptr, idx, starts, stops, xcd_order=True, min_tasks=1024):
+    """Tasks of the near launch (hm_near_lds_kernel): row ranges inside one
+    leaf interval (at most rt rows, rt chosen so that there are about
+    ``min_tasks`` tasks, between 4 and 64 rows), the distinct columns their
+    entries touch (staged in LDS, in increasing order) and the slot of every
+    entry's column in its task's list (uint16, stored as int16 bits).  Tasks
+    in per-XCD row groups as the leaves (xcd_order) or largest first.  {} when
+    a task would touch more than SMX_HMAT_NEAR_W_MAX columns."""
+    ptr = np.asarray(ptr, dtype=np.int64)
+    idx = np.asarray(idx, dtype=np.int64)
+    nb = ptr.size - 1
+    if nb == 0 or ptr[-1] == 0:
+        return {}
+    rt = int(min(64, max(4, 4 * -(-nb // (4 * min_tasks)))))
+    starts = np.asarray(starts, dtype=np.int64)
+    stops = np.asarray(stops, dtype=np.int64)
+    nch = -(-(stops - starts) // rt)
+    tl = np.repeat(np.arange(starts.size), nch)
+    j = np.arange(int(nch.sum()), dtype=np.int64) - np.repeat(np.cumsum(nch) - nch, nch)
+    p0 = starts[tl] + j * rt
+    p1 = np.minimum(p0 + rt, stops[tl])
+    ntask = p0.size
+    e0, e1 = ptr[p0], ptr[p1]
+    cnt = e1 - e0
+    tid = np.repeat(np.arange(ntask, dtype=np.int64), cnt)     # tasks cover the entries in order
+    uk, inv = np.unique(tid * nb + idx[:int(ptr[-1])], return_inverse=True)
+    first = np.searchsorted(uk // nb, np.arange(ntask, dtype=np.int64))
+    w = np.diff(np.append(first, uk.size))
+    if int(w.max()) > _capi.HMAT_NEAR_W_MAX:
+        return {}
+    lidx = (inv.reshape(-1) - first[tid]).astype(np.uint16)
+    order = _xcd_slots(cnt, 1) if xcd_order else np.argsort(-cnt, kind="stable")
+    live = order >= 0
+    tk = np.zeros(order.size, dtype=NTASK_DTYPE)              # empty tasks: p0 = p1 = w = 0
+    o = order[live]
+    tk["s_off"][live] = first[o]
+    tk["p0"][live] = p0[o]
+    tk["p1"][live] = p1[o]
+    tk["w"][live] = w[o]
+    return dict(near_task=tk.view(np.int64), n_near_task=int(order.size),
+                near_stage=(uk % nb).astype(np.int32), near_lidx=lidx.view(np.int16),
+                near_w_max=int(w.max()), near_stage_total=int(uk.size))
 
 
 NTILE_DTYPE = np.dtype([("a_off", np.int64), ("col_off", np.int64), ("w", np.int32),
@@ -351,8 +400,11 @@ def near_from_tiles(nb, tiles, row, col, val):
 class HMatrixGPUPackedFused:
     def __init__(self, data: HMatrixCPUData, threads_per_block: int = 128, use_fp32: bool = True,
                  device=None, near_fp32: bool = False, boundary_gather: np.ndarray | None = None,
-                 near_in_z="auto", xcd_order: bool | None = None):
+                 near_in_z="auto", xcd_order: bool | None = None, near_tasks: bool = True):
         self.data = data
+        # near_tasks: the near launch as one workgroup per row range with its
+        # x columns staged in LDS (pack_near_tasks); False: a wavefront per row
+        self.near_tasks = bool(near_tasks)
         # xcd_order: z tasks and leaves in per-XCD position groups
         # (pack_operator); False: one list each, largest first.  None: with
         # FP32 factors (film matvec 310 -> 300 us, 64-RHS product 1454 ->
@@ -462,7 +514,15 @@ class HMatrixGPUPackedFused:
         H.U = p(pk["U"])
         H.xp, H.z = p(self.xp), p(self.z)
         H.ynear = p(self.ynear) if (self.near_in_z and has_near) else None
+        if H.ynear and self.near_tasks and pk.get("near_task") is not None:
+            H.near_task = p(pk["near_task"])
+            H.n_near_task = int(pk["n_near_task"])
+            H.near_stage, H.near_lidx = p(pk["near_stage"]), p(pk["near_lidx"])
+            H.near_w_max = int(pk["near_w_max"])
         return H
+
+    def _near_by_task(self) -> bool:
+        return self.near_in_z and self.near_tasks and self.near_nnz > 0 and "near_task" in self.fwd
 
     def transposed_pack(self):
         """Packed layout of B^T (built on first use): z stage over U_b^T,
@@ -585,14 +645,21 @@ class HMatrixGPUPackedFused:
         row pointers), diagonal, x gather / y scatter (padding excluded)."""
         fb = 4 if self.use_fp32 else 8
         nv = 4 if self.near_dtype == _capi.SMX_F32 else 8
-        return int(self.far_storage_entries * fb + self.near_nnz * (nv + 4) + (self.nb + 1) * 8
-                   + self.nb * 8 + 4 * self.nb * 8)
+        if self._near_by_task():
+            # 2-byte column slots, and every task's staged column list (4 B)
+            # and x entries (8 B)
+            near = self.near_nnz * (nv + 2) + int(self.fwd["near_stage_total"]) * 12
+        else:
+            near = self.near_nnz * (nv + 4)
+        return int(self.far_storage_entries * fb + near + (self.nb + 1) * 8 + self.nb * 8 + 4 * self.nb * 8)
 
     def stored_bytes(self) -> int:
         """Bytes the kernels stream per matvec, alignment padding included."""
         pk = self.fwd
         size = lambda t: t.numel() * t.element_size()
-        return int(size(pk["U"]) + size(pk["V"]) + size(pk["near_val"]) + size(pk["near_idx"])
+        cols = (size(pk["near_lidx"]) + size(pk["near_stage"]) + 8 * int(pk["near_stage_total"])
+                if self._near_by_task() else size(pk["near_idx"]))
+        return int(size(pk["U"]) + size(pk["V"]) + size(pk["near_val"]) + cols
                    + (self.nb + 1) * 8 + self.nb * 8 + 4 * self.nb * 8)
 
     def benchmark(self, x=None, repeats=20, warmup=3):

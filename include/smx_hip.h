@@ -275,6 +275,17 @@ int smx_rsvd_round(const double* A, const smx_svd_task* tasks, int64_t ntasks, c
  * ---------------------------------------------------------------------- */
 #define SMX_HMAT_Z_GROUP 8
 #define SMX_HMAT_KW_MAX 16384
+#define SMX_HMAT_NEAR_W_MAX 8192
+
+/* A near task: rows p0..p1-1 (permuted) and the w columns they touch, listed
+ * at near_stage + s_off. */
+typedef struct smx_hmat_ntask_s {
+  int64_t s_off;
+  int32_t p0;
+  int32_t p1;
+  int32_t w;
+  int32_t pad;
+} smx_hmat_ntask;
 
 typedef struct {
   int64_t v_off;
@@ -315,6 +326,20 @@ typedef struct {
    * of their own (one wavefront per row) into ynear[p], and the leaf launch
    * reads that sum instead of the near row */
   double* ynear;
+  /* optional, with ynear (NULL: one wavefront per row over near_idx): the
+   * near launch as one workgroup per near task (smx_hmat_ntask).  Task t
+   * stages xp[near_stage[s_off + i]] for i < w in LDS and sums its rows
+   * p0..p1-1 with near_lidx, the slot of each entry's column in that list
+   * (uint16, the layout of near_idx, padding entries a valid slot).  Every
+   * row is in one task; tasks with p0 == p1 are empty.  w <= near_w_max <=
+   * SMX_HMAT_NEAR_W_MAX.  The same sums in the same order as the
+   * wavefront-per-row launch, so the same bits. */
+  const struct smx_hmat_ntask_s* near_task;
+  int64_t n_near_task;
+  const int32_t* near_stage;
+  const uint16_t* near_lidx;
+  int32_t near_w_max;
+  int32_t pad_near;
 } smx_hmatrix;
 
 /* The near rows as dense tiles for smx_hmat_matmat: tile t covers rows

@@ -554,3 +554,43 @@ def test_build_config_rejects_bad_values():
     assert HMatrixBuildConfig(compressor="ACA").compressor == "fullaca"
     assert HMatrixBuildConfig(compressor="randomized-svd").compressor == "rsvd"
     assert HMatrixBuildConfig(max_rank=0).max_rank == 0
+
+
+@pytest.mark.parametrize("xcd_order", [True, False])
+@pytest.mark.parametrize("min_tasks", [1024, 8])
+def test_near_task_tables(c1, xcd_order, min_tasks):
+    """Near tasks of the LDS near launch: every row in one task, each task
+    inside one leaf interval, its staged columns sorted and distinct, and
+    every entry's 2-byte slot naming its column; a host replay of the
+    launch (staged x, rows by slot) gives the near product."""
+    from smx.hmatrix.backend import NTASK_DTYPE, leaf_intervals, pack_near_tasks
+    _, geo, bd = c1
+    data = build_hmatrix_device(geo, HMatrixBuildConfig(leaf_size=32), ops=HostOps(geo, bd))
+    pk = _pack(data, xcd_order=xcd_order)
+    fb = data.far_blocks
+    starts, stops = leaf_intervals(data.size, [b.row_start for b in fb], [b.row_stop for b in fb])
+    nt = pack_near_tasks(pk["near_ptr"], pk["near_idx"], starts, stops, xcd_order, min_tasks=min_tasks)
+    ptr, idx, val = pk["near_ptr"], pk["near_idx"].astype(np.int64), pk["near_val"].astype(np.float64)
+    tasks = nt["near_task"].view(NTASK_DTYPE)[:nt["n_near_task"]]
+    stage, lidx = nt["near_stage"], nt["near_lidx"].view(np.uint16).astype(np.int64)
+    x = np.random.default_rng(3).standard_normal(data.size)
+    y = np.full(data.size, np.nan)
+    seen = np.zeros(data.size, dtype=int)
+    for t in tasks:
+        p0, p1, w, so = int(t["p0"]), int(t["p1"]), int(t["w"]), int(t["s_off"])
+        if p1 == p0:
+            assert w == 0
+            continue
+        L = np.searchsorted(starts, p0, side="right") - 1
+        assert starts[L] <= p0 < p1 <= stops[L] and p1 - p0 <= 64
+        cols = stage[so:so + w]
+        assert w <= nt["near_w_max"] and np.all(np.diff(cols) > 0)
+        xs = x[cols]
+        for p in range(p0, p1):
+            e = np.arange(ptr[p], ptr[p + 1])
+            assert np.array_equal(cols[lidx[e]], idx[e])
+            y[p] = float(val[e] @ xs[lidx[e]])
+            seen[p] += 1
+    assert np.all(seen == 1)
+    ref = data.near_csr @ x
+    assert np.abs(y - ref).max() <= 1e-12 * max(np.abs(ref).max(), 1.0)
