@@ -396,47 +396,14 @@ bool launch_field_fast(const smx_field_args& a, hipStream_t st) {
   }
 }
 
-__global__ __launch_bounds__(kBlock) void llg_jvp_kernel(smx_field_args a,
-                                                         const double* __restrict__ hm,
-                                                         const double* __restrict__ v,
-                                                         double* __restrict__ out,
-                                                         double gamma) {
-  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (k >= a.n) return;
-  const int64_t s = k >> 6;
-  const int lane = (int)(k & 63);
-  const int64_t start = a.slice_ptr[s];
-  const int64_t w = (a.slice_ptr[s + 1] - start) >> 6;
-  double hvx = 0.0, hvy = 0.0, hvz = 0.0;
-  const bool stt = a.stt_vals && a.stt_pref != 0.0;
-  double zmx = 0.0, zmy = 0.0, zmz = 0.0, zvx = 0.0, zvy = 0.0, zvz = 0.0;
-  if (a.ex_vals || a.dmi_vals || stt) {
-    for (int64_t j = 0; j < w; ++j) {
-      const int64_t e = start + j * 64 + lane;
-      const int64_t c = a.cols[e];
-      const double v0 = v[3 * c], v1 = v[3 * c + 1], v2 = v[3 * c + 2];
-      if (a.ex_vals) {
-        const double sv = a.ex_vals[e];
-        hvx = fma(sv, v0, hvx);
-        hvy = fma(sv, v1, hvy);
-        hvz = fma(sv, v2, hvz);
-      }
-      if (stt) {
-        const double tv = a.stt_vals[e];
-        zvx = fma(tv, v0, zvx);
-        zvy = fma(tv, v1, zvy);
-        zvz = fma(tv, v2, zvz);
-        zmx = fma(tv, a.m[3 * c], zmx);
-        zmy = fma(tv, a.m[3 * c + 1], zmy);
-        zmz = fma(tv, a.m[3 * c + 2], zmz);
-      }
-      if (a.dmi_vals) {
-        hvx += dmi_row(a, e, 0, v0, v1, v2);
-        hvy += dmi_row(a, e, 1, v0, v1, v2);
-        hvz += dmi_row(a, e, 2, v0, v1, v2);
-      }
-    }
-  }
+// Everything of the JVP after the slot loop (shared by both forms of the JVP
+// kernel; contraction off as in field_finish, so both give the same bits).
+__device__ __forceinline__ void jvp_finish(const smx_field_args& a, int64_t k, const double* __restrict__ hm,
+                                           const double* __restrict__ v, double* __restrict__ out,
+                                           double gamma, bool stt, double hvx, double hvy, double hvz,
+                                           double zmx, double zmy, double zmz, double zvx, double zvy,
+                                           double zvz) {
+#pragma clang fp contract(off)
   const double vx = v[3 * k], vy = v[3 * k + 1], vz = v[3 * k + 2];
   if (a.ani_pref != 0.0) {
     double nx, ny, nz;
@@ -511,6 +478,138 @@ __global__ __launch_bounds__(kBlock) void llg_jvp_kernel(smx_field_args a,
   out[3 * k] = ox;
   out[3 * k + 1] = oy;
   out[3 * k + 2] = oz;
+}
+
+__global__ __launch_bounds__(kBlock) void llg_jvp_kernel(smx_field_args a,
+                                                         const double* __restrict__ hm,
+                                                         const double* __restrict__ v,
+                                                         double* __restrict__ out,
+                                                         double gamma) {
+#pragma clang fp contract(off)
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (k >= a.n) return;
+  const int64_t s = k >> 6;
+  const int lane = (int)(k & 63);
+  const int64_t start = a.slice_ptr[s];
+  const int64_t w = (a.slice_ptr[s + 1] - start) >> 6;
+  double hvx = 0.0, hvy = 0.0, hvz = 0.0;
+  const bool stt = a.stt_vals && a.stt_pref != 0.0;
+  double zmx = 0.0, zmy = 0.0, zmz = 0.0, zvx = 0.0, zvy = 0.0, zvz = 0.0;
+  if (a.ex_vals || a.dmi_vals || stt) {
+    for (int64_t j = 0; j < w; ++j) {
+      const int64_t e = start + j * 64 + lane;
+      const int64_t c = a.cols[e];
+      const double v0 = v[3 * c], v1 = v[3 * c + 1], v2 = v[3 * c + 2];
+      if (a.ex_vals) {
+        const double sv = a.ex_vals[e];
+        hvx = fma(sv, v0, hvx);
+        hvy = fma(sv, v1, hvy);
+        hvz = fma(sv, v2, hvz);
+      }
+      if (stt) {
+        const double tv = a.stt_vals[e];
+        zvx = fma(tv, v0, zvx);
+        zvy = fma(tv, v1, zvy);
+        zvz = fma(tv, v2, zvz);
+        zmx = fma(tv, a.m[3 * c], zmx);
+        zmy = fma(tv, a.m[3 * c + 1], zmy);
+        zmz = fma(tv, a.m[3 * c + 2], zmz);
+      }
+      if (a.dmi_vals) {
+        hvx += dmi_row(a, e, 0, v0, v1, v2);
+        hvy += dmi_row(a, e, 1, v0, v1, v2);
+        hvz += dmi_row(a, e, 2, v0, v1, v2);
+      }
+    }
+  }
+  jvp_finish(a, k, hm, v, out, gamma, stt, hvx, hvy, hvz, zmx, zmy, zmz, zvx, zvy, zvz);
+}
+
+// The JVP for exchange + DMI in a listed layout (or none) + uniaxial, without
+// cubic anisotropy or spin torques: the slot loop of field_rhs_fast_kernel
+// (U slots' column and value loads, then their v gathers, then the fmas in
+// the general kernel's order), the same tail.
+template <int DM, int U>
+__global__ __launch_bounds__(kBlock, 3) void llg_jvp_fast_kernel(smx_field_args a,
+                                                                 const double* __restrict__ hm,
+                                                                 const double* __restrict__ v,
+                                                                 double* __restrict__ out, double gamma) {
+#pragma clang fp contract(off)
+  constexpr int NC = __builtin_popcount(DM);
+  constexpr int NCS = NC > 0 ? NC : 1;
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (k >= a.n) return;
+  const int64_t s = k >> 6;
+  const int lane = (int)(k & 63);
+  const int64_t start = a.slice_ptr[s];
+  const int64_t w = (a.slice_ptr[s + 1] - start) >> 6;
+  double hvx = 0.0, hvy = 0.0, hvz = 0.0;
+  for (int64_t j0 = 0; j0 < w; j0 += U) {
+    int32_t c[U];
+    int64_t es[U];
+    double ev[U], dv[U][NCS];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      es[u] = start + (j0 + u < w ? j0 + u : w - 1) * 64 + lane;
+      c[u] = a.cols[es[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      ev[u] = a.ex_vals[es[u]];
+#pragma unroll
+      for (int q = 0; q < NC; ++q) dv[u][q] = a.dmi_vals[(int64_t)NC * es[u] + q];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    double v0[U], v1[U], v2[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t cc = c[u];
+      v0[u] = v[3 * cc];
+      v1[u] = v[3 * cc + 1];
+      v2[u] = v[3 * cc + 2];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const bool ok = u == 0 || j0 + u < w;
+      double tx = fma(ev[u], v0[u], hvx);
+      double ty = fma(ev[u], v1[u], hvy);
+      double tz = fma(ev[u], v2[u], hvz);
+      if (NC > 0) {
+        tx += dmi_row_m<DM>(dv[u], 0, v0[u], v1[u], v2[u]);
+        ty += dmi_row_m<DM>(dv[u], 1, v0[u], v1[u], v2[u]);
+        tz += dmi_row_m<DM>(dv[u], 2, v0[u], v1[u], v2[u]);
+      }
+      hvx = ok ? tx : hvx;
+      hvy = ok ? ty : hvy;
+      hvz = ok ? tz : hvz;
+    }
+  }
+  jvp_finish(a, k, hm, v, out, gamma, false, hvx, hvy, hvz, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0);
+}
+
+template <int DM>
+bool launch_jvp_fast_dm(const smx_field_args& a, const double* hm, const double* v, double* out,
+                        double gamma, hipStream_t st) {
+  hipLaunchKernelGGL((llg_jvp_fast_kernel<DM, kFieldU>), dim3(smx::grid_for(a.n, kBlock)), dim3(kBlock), 0, st,
+                     a, hm, v, out, gamma);
+  return true;
+}
+
+// The fast JVP for the term sets of launch_field_fast; false: the general kernel.
+bool launch_jvp_fast(const smx_field_args& a, const double* hm, const double* v, double* out, double gamma,
+                     hipStream_t st) {
+  const bool stt = a.stt_vals && a.stt_pref != 0.0;
+  if (stt || !a.ex_vals || a.cub_pref != 0.0 || a.sot_fl != 0.0 || a.sot_dl != 0.0) return false;
+  switch (dmi_mask(a)) {
+    case 0: return launch_jvp_fast_dm<0>(a, hm, v, out, gamma, st);
+    case 0xE4: return launch_jvp_fast_dm<0xE4>(a, hm, v, out, gamma, st);
+    case 0x4E: return launch_jvp_fast_dm<0x4E>(a, hm, v, out, gamma, st);
+    case 0xAA: return launch_jvp_fast_dm<0xAA>(a, hm, v, out, gamma, st);
+    case 0xEE: return launch_jvp_fast_dm<0xEE>(a, hm, v, out, gamma, st);
+    case 0x1FF: return launch_jvp_fast_dm<0x1FF>(a, hm, v, out, gamma, st);
+    default: return false;
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void projdir_kernel(int64_t n, const double* __restrict__ m,
@@ -683,8 +782,10 @@ extern "C" int smx_llg_jvp(const smx_field_args* a, const double* hm, const doub
   if (rc) return rc;
   SMX_REQUIRE(hm && v && out, SMX_ERR_VALUE, "null pointer argument");
   if (a->n == 0) return SMX_OK;
-  hipLaunchKernelGGL(llg_jvp_kernel, dim3(smx::grid_for(a->n, kBlock)), dim3(kBlock), 0,
-                     smx::as_stream(stream), *a, hm, v, out, gamma);
+  if (!launch_jvp_fast(*a, hm, v, out, gamma, smx::as_stream(stream))) {
+    hipLaunchKernelGGL(llg_jvp_kernel, dim3(smx::grid_for(a->n, kBlock)), dim3(kBlock), 0,
+                       smx::as_stream(stream), *a, hm, v, out, gamma);
+  }
   SMX_LAUNCH_CHECK();
   return SMX_OK;
 }
