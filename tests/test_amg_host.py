@@ -50,9 +50,13 @@ def _tentative(agg, na, B):
     return sp.csr_matrix(T), Bc
 
 
-def _textbook_vcycle(A, b, depth, theta=0.0, B=None):
-    """V(1,1) cycle with damped Jacobi, written in its usual form, with
-    `depth` coarsenings and a minimum-norm least-squares coarse solve."""
+def _textbook_vcycle(A, b, depth, theta=0.0, B=None, level=0):
+    """V(nu,nu) cycle with damped Jacobi, written in its usual form: weight
+    SMOOTH_WEIGHT / rho, one sweep before and after the coarse correction on
+    the finest level and COARSE_SWEEPS on the coarser ones, the prolongator
+    smoothed with 4/3 / rho, `depth` coarsenings and a minimum-norm
+    least-squares coarse solve."""
+    from smx.amg import COARSE_SWEEPS, SMOOTH_WEIGHT
     n = A.shape[0]
     if depth == 0:
         return np.linalg.lstsq(A.toarray(), b, rcond=1e-11)[0]
@@ -60,16 +64,21 @@ def _textbook_vcycle(A, b, depth, theta=0.0, B=None):
         B = np.ones(n)
     agg, na = aggregate(strength_graph(A, theta))
     T, Bc = _tentative(agg, na, B)
-    w = 4.0 / (3.0 * spectral_radius_dinv_a(A))
+    rho = spectral_radius_dinv_a(A)
+    w = SMOOTH_WEIGHT / rho
     dinv = 1.0 / A.diagonal()
-    P = T - w * sp.diags(dinv) @ (A @ T)
-    x = w * dinv * b                                  # pre-smoothing from x = 0
+    P = T - (4.0 / (3.0 * rho)) * sp.diags(dinv) @ (A @ T)
+    nu = 1 if level == 0 else COARSE_SWEEPS
+    x = np.zeros(n)
+    for _ in range(nu):                               # pre-smoothing from x = 0
+        x = x + w * dinv * (b - A @ x)
     r = b - A @ x                                     # residual
     Ac = (P.T @ A @ P).tocsr()
     Ac = (Ac + Ac.T) * 0.5
-    e = _textbook_vcycle(Ac, P.T @ r, depth - 1, theta, Bc)
+    e = _textbook_vcycle(Ac, P.T @ r, depth - 1, theta, Bc, level + 1)
     x = x + P @ e                                     # coarse correction
-    x = x + w * dinv * (b - A @ x)                    # post-smoothing
+    for _ in range(nu):                               # post-smoothing
+        x = x + w * dinv * (b - A @ x)
     return x
 
 
