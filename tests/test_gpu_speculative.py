@@ -28,9 +28,11 @@ def test_speculative_equals_waiting(pc, backend):
     from smx.llg import EffectiveField
     m = M.cube_c1()
     N = m.num_nodes
-    # the near-row mode fixed: its set-up timing choice changes the last bits
+    # the near-row mode fixed: its set-up timing choice changes the last bits;
+    # a multi-level AMG (this mesh's 1331 rows would otherwise be one dense
+    # coarse level, solved in one iteration, which no guess can fall short of)
     dk = dict(ksp_pc_type=pc, boundary_backend=backend, ksp_rtol_u1=1e-6, ksp_rtol_u2=1e-6, ksp_atol=1e-12,
-              hmatrix_near_in_z=True)
+              hmatrix_near_in_z=True, amg_max_coarse=256)
     kw = dict(Ms=8e5, Aex=1.3e-11, Ku=5e5, n_ani_vec=(0.0, 0.0, 1.0), alpha=0.5, use_demag=True, device=DEV)
     hs = EffectiveField(m, demag_kwargs=dict(cg_speculative=True, **dk), **kw)
     hw = EffectiveField(m, demag_kwargs=dict(cg_speculative=False, **dk), **kw)
