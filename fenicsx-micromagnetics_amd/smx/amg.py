@@ -157,8 +157,17 @@ class HostLevel:
 SMOOTH_WEIGHT = 1.6
 COARSE_SWEEPS = 1
 
+# Strength threshold of the aggregation and the largest level solved by the
+# dense coarse inverse.  theta 0.02 keeps the film's hierarchy at three
+# levels with a 1174-row coarse level (1600 >= 1174) and takes its u1 from 10
+# to 9 AMG-CG iterations (the 150 nm cube: 10 -> 9); theta 0 with 1024 rows
+# gives 202005 -> 25339 -> 856.  A larger theta adds a level on the film
+# (two more launches per iteration) for the same 9 iterations.
+THETA = 0.02
+MAX_COARSE = 1600
 
-def build_levels(A: sp.csr_matrix, max_coarse: int = 1024, theta: float = 0.0,
+
+def build_levels(A: sp.csr_matrix, max_coarse: int = MAX_COARSE, theta: float = THETA,
                  max_levels: int = 12, smooth_weight: float = SMOOTH_WEIGHT,
                  coarse_sweeps: int = COARSE_SWEEPS) -> list[HostLevel]:
     """Folded V-cycle operators of every level (host, FP64).  With nu damped
@@ -300,8 +309,8 @@ class AmgLevel(C.Structure):
 class AmgHierarchy:
     """Device copy of the folded V-cycle (smx_pcg_amg's ``levels``)."""
 
-    def __init__(self, A: sp.csr_matrix, device, fp32: bool = False, max_coarse: int = 1024,
-                 theta: float = 0.0, host_levels: list[HostLevel] | None = None):
+    def __init__(self, A: sp.csr_matrix, device, fp32: bool = False, max_coarse: int = MAX_COARSE,
+                 theta: float = THETA, host_levels: list[HostLevel] | None = None):
         import torch
         self.device = device
         self.fp32 = bool(fp32)

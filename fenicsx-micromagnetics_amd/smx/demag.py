@@ -123,15 +123,6 @@ class OperatorSet:
         return self._order
 
 
-# AMG hierarchy of the Poisson solves: strength threshold and the largest
-# level solved by the dense coarse inverse.  theta 0.02 keeps the film's
-# hierarchy at three levels with a 1174-row coarse level (1600 >= 1174) and
-# takes its u1 from 10 to 9 AMG-CG iterations (the 150 nm cube: 10 -> 9);
-# theta 0 with 1024 gives 202005 -> 25339 -> 856.
-AMG_THETA = 0.02
-AMG_MAX_COARSE = 1600
-
-
 class DemagFieldLindholmGPU:
     """FEM/BEM demag field on one MI355X (see module docstring)."""
 
@@ -159,7 +150,7 @@ class DemagFieldLindholmGPU:
                  hmatrix_near_fp32=False, hmatrix_near_in_z="auto", cg_reorder=True, cg_speculative=True,
                  ksp_pc_type="gamg",
                  ksp_pc_type_u2=None,
-                 amg_fp32=None, amg_max_coarse=AMG_MAX_COARSE, amg_theta=AMG_THETA, cg_mixed=False, cg_refinements=1,
+                 amg_fp32=None, amg_max_coarse=None, amg_theta=None, cg_mixed=False, cg_refinements=1,
                  cg_rtol_inner=None, verbose=False):
         """Keywords as the reference's constructor (Demag_Lindholm_GPU.py:101-155),
         plus, keyword-only: the solver settings the reference hard-codes
@@ -269,7 +260,9 @@ class DemagFieldLindholmGPU:
                              "jacobi": solver_operator(plain, dinv) if pc in ("jacobi", "auto") else None,
                              "amg": None, "n": n, "nd": self.N - n, "raw": raw, "perm": cperm}
         self._sel = (sel, sel2)
-        self._amg_host = (A1h, A2h, int(amg_max_coarse), float(amg_theta))
+        from .amg import MAX_COARSE, THETA
+        self._amg_host = (A1h, A2h, int(MAX_COARSE if amg_max_coarse is None else amg_max_coarse),
+                          float(THETA if amg_theta is None else amg_theta))
 
         # ---------------- boundary operator ----------------
         self.opB = geometry or build_boundary_geometry(mesh.X, mesh.tets, incident_policy)

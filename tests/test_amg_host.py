@@ -50,13 +50,14 @@ def _tentative(agg, na, B):
     return sp.csr_matrix(T), Bc
 
 
-def _textbook_vcycle(A, b, depth, theta=0.0, B=None, level=0):
+def _textbook_vcycle(A, b, depth, theta=None, B=None, level=0):
     """V(nu,nu) cycle with damped Jacobi, written in its usual form: weight
     SMOOTH_WEIGHT / rho, one sweep before and after the coarse correction on
     the finest level and COARSE_SWEEPS on the coarser ones, the prolongator
     smoothed with 4/3 / rho, `depth` coarsenings and a minimum-norm
     least-squares coarse solve."""
-    from smx.amg import COARSE_SWEEPS, SMOOTH_WEIGHT
+    from smx.amg import COARSE_SWEEPS, SMOOTH_WEIGHT, THETA
+    theta = THETA if theta is None else theta
     n = A.shape[0]
     if depth == 0:
         return np.linalg.lstsq(A.toarray(), b, rcond=1e-11)[0]
@@ -136,6 +137,7 @@ def test_aggregation_properties():
 
 
 def test_coarse_operators_keep_constants_in_the_nullspace():
+    from smx.amg import THETA
     A, _ = _op(16, 14, 10)
     n = A.shape[0]
     assert abs(A @ np.ones(n)).max() <= 1e-12 * abs(A).max()
@@ -143,7 +145,7 @@ def test_coarse_operators_keep_constants_in_the_nullspace():
     B = np.ones(n)
     nlev = 0
     while Al.shape[0] > 20:
-        agg, na = aggregate(strength_graph(Al))
+        agg, na = aggregate(strength_graph(Al, THETA))
         T, c = _tentative(agg, na, B)
         assert np.allclose(T @ c, B)                  # T interpolates the near-nullspace
         w = 4.0 / (3.0 * spectral_radius_dinv_a(Al))
