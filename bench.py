@@ -216,7 +216,8 @@ def matvec_timing(d, flush_mb=512, reps=12):
         nbytes = H.algorithmic_bytes()
         where = "near launch" if H.near_in_z else "leaf launch"
         log(f"near rows in the {where}; set-up times {H.near_mode_seconds}")
-        name = (f"hm_gather_kernel + {'hm_near_kernel + ' if H.near_in_z else ''}hm_far_z_kernel + hm_leaf_kernel "
+        near_k = ("hm_near_lds_kernel + " if H._near_by_task() else "hm_near_kernel + ") if H.near_in_z else ""
+        name = (f"hm_gather_kernel + {near_k}hm_far_z_kernel + hm_leaf_kernel "
                 f"(H-matrix boundary matvec; near rows in the {where}, chosen at set-up)")
     if not reps:
         return None, None, nbytes, name

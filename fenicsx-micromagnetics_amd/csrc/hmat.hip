@@ -386,18 +386,18 @@ __global__ __launch_bounds__(256) void hm_near_kernel(smx_hmatrix H) {
 // does (the same entries per lane, the same fma order, the same wave total),
 // reading a 2-byte slot per entry instead of a 4-byte column and x from LDS
 // instead of the L2.
-template <typename TN>
-__global__ __launch_bounds__(256) void hm_near_lds_kernel(smx_hmatrix H) {
+template <typename TN, int NW>
+__global__ __launch_bounds__(NW * 64) void hm_near_lds_kernel(smx_hmatrix H) {
   extern __shared__ double xs[];
   const smx_hmat_ntask T = H.near_task[blockIdx.x];
   const int lane = threadIdx.x & 63;
   const int wid = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
   const int32_t* __restrict__ stg = H.near_stage + T.s_off;
-  for (int i = threadIdx.x; i < T.w; i += 256) xs[i] = H.xp[stg[i]];
+  for (int i = threadIdx.x; i < T.w; i += NW * 64) xs[i] = H.xp[stg[i]];
   __syncthreads();
   const TN* __restrict__ NV = reinterpret_cast<const TN*>(H.near_val);
   const uint32_t* __restrict__ LI = reinterpret_cast<const uint32_t*>(H.near_lidx);
-  for (int64_t p = T.p0 + wid; p < T.p1; p += 4) {
+  for (int64_t p = T.p0 + wid; p < T.p1; p += NW) {
     double s = 0.0;
     const int64_t n1 = H.near_ptr[p + 1];
     for (int64_t e = H.near_ptr[p] + 2 * lane; e < n1; e += 128) {
@@ -638,6 +638,8 @@ int hmat_matvec_impl(const smx_hmatrix* H, const double* src, const double* shif
                               H->n_near_task <= 0x7fffffffLL && H->near_w_max >= 0 &&
                               H->near_w_max <= SMX_HMAT_NEAR_W_MAX),
               SMX_ERR_VALUE, "near task arrays missing or near_w_max beyond %d", SMX_HMAT_NEAR_W_MAX);
+  SMX_REQUIRE(H->near_waves == 0 || H->near_waves == 4 || H->near_waves == 8 || H->near_waves == 16,
+              SMX_ERR_VALUE, "near_waves must be 0, 4, 8 or 16");
   hipStream_t st = smx::as_stream(stream);
   hipLaunchKernelGGL(hm_gather_kernel, dim3(smx::grid_for(H->nb, 256)), dim3(256), 0, st, H->nb,
                      src, H->gidx, shift, shift_scale, H->xp);
@@ -645,10 +647,15 @@ int hmat_matvec_impl(const smx_hmatrix* H, const double* src, const double* shif
   if (near_tasks) {
     const size_t lds = (size_t)H->near_w_max * sizeof(double);
     const unsigned gn = (unsigned)H->n_near_task;
-    if (H->near_dtype == SMX_F32)
-      hipLaunchKernelGGL(hm_near_lds_kernel<float>, dim3(gn), dim3(256), lds, st, *H);
-    else
-      hipLaunchKernelGGL(hm_near_lds_kernel<double>, dim3(gn), dim3(256), lds, st, *H);
+    const int nw = H->near_waves == 0 ? 8 : H->near_waves;  // 8: fastest on the 83k-node film
+#define SMX_NEAR_LDS(TN, NW) \
+  hipLaunchKernelGGL((hm_near_lds_kernel<TN, NW>), dim3(gn), dim3(NW * 64), lds, st, *H)
+    if (H->near_dtype == SMX_F32) {
+      if (nw == 4) SMX_NEAR_LDS(float, 4); else if (nw == 16) SMX_NEAR_LDS(float, 16); else SMX_NEAR_LDS(float, 8);
+    } else {
+      if (nw == 4) SMX_NEAR_LDS(double, 4); else if (nw == 16) SMX_NEAR_LDS(double, 16); else SMX_NEAR_LDS(double, 8);
+    }
+#undef SMX_NEAR_LDS
     SMX_LAUNCH_CHECK();
   } else if (H->ynear && H->near_ptr) {
     const unsigned gn = (unsigned)((H->nb + 3) / 4);

@@ -519,6 +519,7 @@ class HMatrixGPUPackedFused:
             H.n_near_task = int(pk["n_near_task"])
             H.near_stage, H.near_lidx = p(pk["near_stage"]), p(pk["near_lidx"])
             H.near_w_max = int(pk["near_w_max"])
+            H.near_waves = int(getattr(self, "near_waves", 0))   # 0: the library's default (8)
         return H
 
     def _near_by_task(self) -> bool:

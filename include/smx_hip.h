@@ -339,7 +339,7 @@ typedef struct {
   const int32_t* near_stage;
   const uint16_t* near_lidx;
   int32_t near_w_max;
-  int32_t pad_near;
+  int32_t near_waves; /* wavefronts per near task workgroup: 4, 8 or 16 (0: 8) */
 } smx_hmatrix;
 
 /* The near rows as dense tiles for smx_hmat_matmat: tile t covers rows

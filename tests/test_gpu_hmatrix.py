@@ -151,6 +151,51 @@ def test_near_rows_in_z_launch(c1, fp32):
 
 
 @pytest.mark.parametrize("fp32", [False, True])
+def test_near_task_forms_agree(c1, fp32):
+    """The near launch's forms give the same bits: a wavefront per row, and
+    one workgroup per row task with 4, 8 or 16 wavefronts (plain, boundary
+    gather with a shift, transposed); other task widths are rejected."""
+    import torch
+    from smx.hmatrix import HMatrixBuildConfig, HMatrixGPUPackedFused, build_hmatrix_device
+    m, geo, bd, B = c1
+    data = build_hmatrix_device(geo, HMatrixBuildConfig(epsilon=1e-6, leaf_size=32), torch.device("cuda"))
+    kw = dict(use_fp32=fp32, near_in_z=True, boundary_gather=geo.bverts)
+    forms = {"row": HMatrixGPUPackedFused(data, near_tasks=False, **kw),
+             "task": HMatrixGPUPackedFused(data, **kw)}
+    assert forms["row"]._H.near_task is None and forms["task"]._H.near_task is not None
+    rng = np.random.default_rng(11)
+    x = torch.from_numpy(rng.standard_normal(geo.Nb)).cuda()
+    u = torch.from_numpy(rng.standard_normal(m.num_nodes)).cuda()
+    shift = torch.tensor([0.37], dtype=torch.float64, device="cuda")
+    gather = torch.from_numpy(np.asarray(geo.bverts)).cuda()
+
+    def outputs(H):
+        yb = torch.empty_like(x)
+        H.matvec_from(u, yb, gather=gather, shift_ptr=shift.data_ptr(), shift_scale=1.5)
+        return H.matvec(x).clone(), yb, H.rmatvec(x).clone()
+
+    ref = outputs(forms["row"])
+    print(f"[near forms fp32={fp32}] Nb={geo.Nb} tasks={forms['task'].fwd['n_near_task']}", flush=True)
+    for name, H in forms.items():
+        for nw in ((4, 8, 16, 0) if name != "row" else (0,)):
+            for S in (H._H, H._Hb):
+                S.near_waves = nw
+            if hasattr(H, "_HT"):
+                H._HT.near_waves = nw
+            got = outputs(H)
+            for what, a, b in zip(("matvec", "gather+shift", "rmatvec"), got, ref):
+                assert torch.equal(a, b), (name, nw, what, float((a - b).abs().max()))
+    H = forms["task"]
+    H._H.near_waves = 12
+    with pytest.raises(ValueError):
+        H.matvec(x)
+    H._H.near_waves = 0
+    yb = ref[1].cpu().numpy()
+    xb = u.cpu().numpy()[np.asarray(geo.bverts)] - 0.37 * 1.5
+    assert np.linalg.norm(yb - B @ xb) <= (10 * 1e-6 + (1e-6 if fp32 else 0.0)) * np.linalg.norm(B @ xb)
+
+
+@pytest.mark.parametrize("fp32", [False, True])
 def test_leaf_workgroup_shapes_agree(c1, fp32):
     """4, 8 and 16 wavefronts per leaf workgroup give the same bits (each
     row is one wavefront's dot product either way); other values are
