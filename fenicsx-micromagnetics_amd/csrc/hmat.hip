@@ -479,9 +479,17 @@ __global__ __launch_bounds__(NW * 64) void hm_leaf_kernel(smx_hmatrix H, double*
   const int wid = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
   const int32_t l0 = H.leaf_start[L], l1 = H.leaf_stop[L];
   const int32_t kw = H.leaf_kw[L];
-  {
-    const int64_t g0 = H.leaf_zg_off[L];
-    const int32_t K = (int32_t)(H.leaf_zg_off[L + 1] - g0);
+  const int64_t g0 = H.leaf_zg_off[L];
+  const int32_t K = (int32_t)(H.leaf_zg_off[L + 1] - g0);
+  if (H.leaf_zr) {  // contiguous runs of z (uniform loop, scalar run records)
+    const int64_t q1 = H.leaf_zr_off[L + 1];
+    for (int64_t q = H.leaf_zr_off[L]; q < q1; ++q) {
+      const int32_t zs = H.leaf_zr[2 * q], k0 = H.leaf_zr[2 * q + 1];
+      const int32_t k1 = q + 1 < q1 ? H.leaf_zr[2 * q + 3] : K;
+      for (int k = k0 + (int)threadIdx.x; k < k1; k += NW * 64) zl[k] = H.z[zs + (k - k0)];
+    }
+    for (int k = K + (int)threadIdx.x; k < kw; k += NW * 64) zl[k] = 0.0;
+  } else {
     const int32_t* __restrict__ zg = H.leaf_zg + g0;
     for (int k = threadIdx.x; k < kw; k += NW * 64) zl[k] = k < K ? H.z[zg[k]] : 0.0;
   }
@@ -623,6 +631,7 @@ int hmat_matvec_impl(const smx_hmatrix* H, const double* src, const double* shif
               "factor dtype must be FP32 or FP64");
   SMX_REQUIRE(H->near_dtype == SMX_F32 || H->near_dtype == SMX_F64, SMX_ERR_TYPE,
               "near dtype must be FP32 or FP64");
+  SMX_REQUIRE(!H->leaf_zr == !H->leaf_zr_off, SMX_ERR_VALUE, "leaf_zr and leaf_zr_off go together");
   SMX_REQUIRE(H->nztask == 0 || (H->ztask && H->V && H->z && H->leaf_zg), SMX_ERR_VALUE,
               "far-field arrays missing");
   SMX_REQUIRE(H->leaf_waves == 0 || H->leaf_waves == 4 || H->leaf_waves == 8 || H->leaf_waves == 16,

@@ -88,7 +88,7 @@ class HMatrixArgs(C.Structure):
         ("leaf_zg_off", P), ("leaf_zg", P), ("kw_max", I32), ("leaf_waves", I32),
         ("U", P), ("xp", P), ("z", P), ("ynear", P),
         ("near_task", P), ("n_near_task", I64), ("near_stage", P), ("near_lidx", P),
-        ("near_w_max", I32), ("near_waves", I32),
+        ("near_w_max", I32), ("near_waves", I32), ("leaf_zr_off", P), ("leaf_zr", P),
     ]
 
 

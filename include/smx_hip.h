@@ -340,6 +340,13 @@ typedef struct {
   const uint16_t* near_lidx;
   int32_t near_w_max;
   int32_t near_waves; /* wavefronts per near task workgroup: 4, 8 or 16 (0: 8) */
+  /* optional (NULL: the leaf launch gathers through leaf_zg): leaf L's list
+   * as runs of consecutive z entries, runs leaf_zr_off[L] .. leaf_zr_off[L+1]-1
+   * with run q = (leaf_zr[2q], leaf_zr[2q+1]) = (first z entry, first column
+   * k), running to the next run's first column (K_L for the last): zl_L[k] =
+   * z[leaf_zr[2q] + k - leaf_zr[2q+1]], the same values as leaf_zg gives. */
+  const int64_t* leaf_zr_off; /* nleaf + 1 */
+  const int32_t* leaf_zr;
 } smx_hmatrix;
 
 /* The near rows as dense tiles for smx_hmat_matmat: tile t covers rows

@@ -217,6 +217,34 @@ def test_leaf_workgroup_shapes_agree(c1, fp32):
     H._H.leaf_waves = 0
 
 
+@pytest.mark.parametrize("fp32,near_in_z", [(False, False), (True, False), (True, True)])
+def test_leaf_z_runs_equal_gather(c1, fp32, near_in_z):
+    """The leaf launch's z staging by contiguous runs (leaf_zr) against the
+    per-entry gather (leaf_zg): the same bits, forward and transposed; one of
+    the two run pointers alone is rejected."""
+    import torch
+    from smx.hmatrix import HMatrixBuildConfig, HMatrixGPUPackedFused, build_hmatrix_device
+    m, geo, bd, B = c1
+    data = build_hmatrix_device(geo, HMatrixBuildConfig(epsilon=1e-6, leaf_size=32), torch.device("cuda"))
+    H = HMatrixGPUPackedFused(data, use_fp32=fp32, near_in_z=near_in_z)
+    x = torch.from_numpy(np.random.default_rng(12).standard_normal(geo.Nb)).cuda()
+    assert H._H.leaf_zr is not None
+    runs = H.fwd["leaf_zr"].numel() // 2
+    print(f"[leaf z runs fp32={fp32} near_in_z={near_in_z}] Nb={geo.Nb} runs={runs} "
+          f"entries={H.fwd['leaf_zg'].numel()}", flush=True)
+    ref = (H.matvec(x).clone(), H.rmatvec(x).clone())
+    for S in (H._H, H._HT):
+        S.leaf_zr, S.leaf_zr_off = None, None
+    got = (H.matvec(x).clone(), H.rmatvec(x).clone())
+    for a, b in zip(got, ref):
+        assert torch.equal(a, b)
+    H._H.leaf_zr_off = H.fwd["leaf_zr_off"].data_ptr()
+    with pytest.raises(ValueError):
+        H.matvec(x)
+    assert np.linalg.norm(ref[0].cpu().numpy() - B @ x.cpu().numpy()) <= \
+        (10 * 1e-6 + (1e-6 if fp32 else 0.0)) * np.linalg.norm(B @ x.cpu().numpy())
+
+
 @pytest.mark.parametrize("fp32,k,near_fp32", [(False, 64, False), (False, 7, False), (True, 64, False),
                                                (True, 1, False), (True, 64, True)])
 def test_matmat_against_matvecs(fp32, k, near_fp32):

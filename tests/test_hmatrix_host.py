@@ -261,6 +261,13 @@ def _emulate_packed(pk, nb, perm, diag_p, x):
         assert 0 <= kw - K < 4
         idx = zg[zgo[L]:zgo[L + 1]]
         assert np.all((idx >= 0) & (idx < pk["nz"]))
+        # the run form of the same list: contiguous z ranges, first columns increasing
+        zr, zro = pk["leaf_zr"].reshape(-1, 2), pk["leaf_zr_off"]
+        runs = zr[zro[L]:zro[L + 1]]
+        ends = np.append(runs[1:, 1], K)
+        assert runs.size == 0 or (runs[0, 1] == 0 and np.all(ends > runs[:, 1]))
+        expanded = np.concatenate([np.arange(a, a + e - k) for (a, k), e in zip(runs, ends)] or [np.zeros(0, int)])
+        assert np.array_equal(expanded, idx)
         zl = np.zeros(kw)
         zl[:K] = z[idx]
         UL = U[uo:uo + c * kw].reshape(c, kw)
