@@ -1054,6 +1054,8 @@ def main():
                      "unit": "GB/s", "frac": mv_ach / HBM_PEAK_GBS,
                      "traffic": measured_traffic(args.config, "matvec", mv_variant),
                      "algorithmic_bytes": int(mv_bytes),
+                     "streamed_bytes": (int(hmx.demag_field.B_hmatrix_gpu.stored_bytes())
+                                        if hmx.demag_field.boundary_backend == "hmatrix" else int(mv_bytes)),
                      "avg_us": mv_cold * 1e6, "avg_us_cache_warm": mv_warm * 1e6,
                      "note": "mixed precision boundary operator; timed after a 512 MB read sweep that evicts the "
                              "Infinity Cache",

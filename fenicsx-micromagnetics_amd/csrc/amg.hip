@@ -404,7 +404,9 @@ __global__ __launch_bounds__(BS) void amg_update_kernel(int64_t n, AmgPtrs P, do
     const int nsl = (int)((R0.rows + RPS - 1) / RPS);
     if (sl < nsl) {  // uniform per slice (a workgroup may run past the last one)
       const int32_t o0 = R0.off[sl], o1 = R0.off[sl + 1];
-      f0 = sell_row<TPR, VT>(R0, w, o0, (o1 - o0) / RPS, i, (int)threadIdx.x % TPR);
+      // a launch past convergence (speculative) skips the product: the done
+      // flag was loaded with the slice offsets, so real iterations wait no longer
+      if (!was_done) f0 = sell_row<TPR, VT>(R0, w, o0, (o1 - o0) / RPS, i, (int)threadIdx.x % TPR);
     }
     const int64_t rc = row < R0.rows ? row : R0.rows - 1;
     f1 = P.z1[rc];

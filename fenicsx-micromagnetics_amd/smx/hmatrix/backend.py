@@ -660,25 +660,31 @@ class HMatrixGPUPackedFused:
                                                 y.data_ptr(), _capi.stream_ptr()), "smx_hmat_matvec")
 
     def algorithmic_bytes(self) -> int:
-        """Bytes one matvec must move: factors, near CSR (values + columns +
-        row pointers), diagonal, x gather / y scatter (padding excluded)."""
+        """Bytes one matvec must move, by the H-matrix matvec's per-unit
+        figure: far_storage_entries x factor size + near_nnz x (value +
+        4-byte column) + (nb + 1) x 4 row pointers + nb x 8 diagonal +
+        4 nb x 8 for x and y with the permutation gather and scatter (padding
+        excluded).  The forms differ in what they stream (stored_bytes): the
+        near task form reads 2-byte column slots and per-task column lists
+        instead of 4-byte columns."""
         fb = 4 if self.use_fp32 else 8
         nv = 4 if self.near_dtype == _capi.SMX_F32 else 8
-        if self._near_by_task():
-            # 2-byte column slots, and every task's staged column list (4 B)
-            # and x entries (8 B)
-            near = self.near_nnz * (nv + 2) + int(self.fwd["near_stage_total"]) * 12
-        else:
-            near = self.near_nnz * (nv + 4)
-        return int(self.far_storage_entries * fb + near + (self.nb + 1) * 8 + self.nb * 8 + 4 * self.nb * 8)
+        return int(self.far_storage_entries * fb + self.near_nnz * (nv + 4) + (self.nb + 1) * 4
+                   + self.nb * 8 + 4 * self.nb * 8)
 
     def stored_bytes(self) -> int:
-        """Bytes the kernels stream per matvec, alignment padding included."""
+        """Bytes the kernels stream per matvec in this form, alignment padding
+        included: factors, near values and column data (4-byte columns, or
+        2-byte slots + task column lists + the staged x entries), int64 row
+        pointers, the leaf z run table (or gather lists), task records,
+        vectors."""
         pk = self.fwd
         size = lambda t: t.numel() * t.element_size()
         cols = (size(pk["near_lidx"]) + size(pk["near_stage"]) + 8 * int(pk["near_stage_total"])
-                if self._near_by_task() else size(pk["near_idx"]))
-        return int(size(pk["U"]) + size(pk["V"]) + size(pk["near_val"]) + cols
+                + size(pk["near_task"]) if self._near_by_task() else size(pk["near_idx"]))
+        zlists = (size(pk["leaf_zr"]) + size(pk["leaf_zr_off"]) if self._H.leaf_zr
+                  else size(pk["leaf_zg"]) + size(pk["leaf_zg_off"]))
+        return int(size(pk["U"]) + size(pk["V"]) + size(pk["near_val"]) + cols + zlists + size(pk["ztask"])
                    + (self.nb + 1) * 8 + self.nb * 8 + 4 * self.nb * 8)
 
     def benchmark(self, x=None, repeats=20, warmup=3):

@@ -117,8 +117,11 @@ def _solve(pc, mode, split, jitter, mixed=False, rtol=1e-12):
         rc = L.smx_pcg_jacobi_mixed(*args, C.byref(mx), _capi.stream_ptr()) if mixed else \
             L.smx_pcg_jacobi(*args, _capi.stream_ptr())
     else:
-        h = AmgHierarchy(A_it, DEV, fp32=mixed)
-        assert h.sizes[0] == n
+        # coarse levels up to 256 rows: with the default dense coarse size
+        # the interior block of this box would be one dense level (an exact
+        # solve in one iteration) and the full system not
+        h = AmgHierarchy(A_it, DEV, fp32=mixed, max_coarse=256)
+        assert h.sizes[0] == n and len(h.sizes) >= 2
         mx = None
         if mixed:
             v32 = A.vals.float()
