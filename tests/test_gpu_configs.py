@@ -88,7 +88,7 @@ def test_cube_adjoint_through_500_steps():
         (g,) = torch.autograd.grad(J, Ms)
         torch.cuda.synchronize()
         grads[name] = g
-        print(f"[cube adjoint {name}] J={float(J):.12f}, forward + gradient {time.perf_counter() - t0:.1f}s, "
+        print(f"[cube adjoint {name}] J={float(J.detach()):.12f}, forward + gradient {time.perf_counter() - t0:.1f}s, "
               f"{model.rhs_evals} RHS evaluations, {model.vjp_evals} VJPs", flush=True)
         if name == "fp64":
             rng = np.random.default_rng(11)
