@@ -18,8 +18,9 @@
 // FP64 v_mfma_f64_16x16x4_f64, D[m = g + 4i][n = c]; operand A[m = c][k],
 // B[k][n = c], the k of MFMA j in a step of lane group g being
 // kk + KL g + j for both operands.  A z task fills rows 0..kc-1 of a
-// 16-row tile (rows past kc load zeros); a leaf workgroup's four wavefronts
-// cover its rows in 16-row tiles.  FP32 factors: X, Z in FP32 (one rounding
+// 16-row tile (rows past kc load zeros); with FP32 factors a leaf
+// workgroup's four wavefronts split its columns and each covers all of its
+// 16-row tiles (hmm_leaf_ks_kernel), with FP64 they cover one tile each.  FP32 factors: X, Z in FP32 (one rounding
 // each), FP32 accumulation in the tiles; the near field (over an FP64 copy of
 // X) and all sums in FP64.  Z tasks and leaves run largest first; the
 // near-tile launch maps workgroups to XCDs by contiguous eighths of its
@@ -199,6 +200,104 @@ __global__ __launch_bounds__(256) void hmm_z_kernel(smx_hmatrix H, const T* __re
     }
   }
 }
+
+// FP32 factors, one workgroup per leaf of at most 64 rows: the four
+// wavefronts split the leaf's columns k in KS ranges and its rows in 4 / KS
+// groups of KS row tiles, so every Z row the leaf needs is read by 4 / KS
+// wavefronts (instead of four, one per 16-row tile); the KS partial tiles of
+// a row group are then added through LDS in wavefront order, in FP64.
+template <int KS>
+__global__ __launch_bounds__(256) void hmm_leaf_ks_kernel(smx_hmatrix H, const float* __restrict__ Z,
+                                                          double* __restrict__ YF) {
+  using M = Mt<float>;
+  constexpr int TR = KS;  // row tiles per wavefront
+  __shared__ float red[KS > 1 ? 4 - 4 / KS : 1][TR * 16][64];  // partials of the non-leading wavefronts
+  const int64_t L = blockIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int kp = w % KS, rg = w / KS;  // column range, row group
+  const int g = lane >> 4, c = lane & 15;
+  const int32_t l0 = H.leaf_start[L], rows = H.leaf_stop[L] - l0;
+  const int r0 = 16 * TR * rg;  // first row of this wavefront's group
+  const int nt = rows > r0 ? ((rows - r0 + 15) >> 4 < TR ? (rows - r0 + 15) >> 4 : TR) : 0;
+  const int64_t kw = H.leaf_kw[L];
+  const int64_t z0 = H.leaf_zg_off[L];
+  const int64_t K = H.leaf_zg_off[L + 1] - z0;
+  const int32_t* __restrict__ zg = H.leaf_zg + z0;
+  const int64_t ks = ((kw + KS * M::kStep - 1) / (KS * M::kStep)) * M::kStep;
+  const int64_t kb = kp * ks;
+  const int64_t ke = nt == 0 ? kb : (kb + ks < kw ? kb + ks : kw);
+  const float* __restrict__ Ur[TR];
+  bool row_ok[TR];
+#pragma unroll
+  for (int r = 0; r < TR; ++r) {
+    const int row = r0 + 16 * r + c;
+    row_ok[r] = row < rows;
+    Ur[r] = reinterpret_cast<const float*>(H.U) + H.leaf_u_off[L] + (int64_t)(row_ok[r] ? row : 0) * kw;
+  }
+  M::acc_t acc[TR][4];
+#pragma unroll
+  for (int r = 0; r < TR; ++r)
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) acc[r][q][i] = 0.0f;
+  for (int64_t kk = kb; kk < ke; kk += M::kStep) {
+    const int64_t kc = kk + M::kLane * g;
+    int64_t zi[M::kLane];
+#pragma unroll
+    for (int i = 0; i < M::kLane; ++i) zi[i] = (kc + i < K) ? (int64_t)zg[kc + i] : -1;
+    float a[TR][M::kLane];
+#pragma unroll
+    for (int r = 0; r < TR; ++r) M::load_a(Ur[r] + kc, (row_ok[r] && r < nt) ? kw - kc : 0, a[r]);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      float b[M::kLane];
+#pragma unroll
+      for (int i = 0; i < M::kLane; ++i) b[i] = zi[i] >= 0 ? Z[zi[i] * kR + 16 * q + c] : 0.0f;
+#pragma unroll
+      for (int r = 0; r < TR; ++r) {
+        if (r >= nt) break;
+        M::mma(acc[r][q], a[r], b);
+      }
+    }
+  }
+  if constexpr (KS > 1) {
+    if (kp > 0) {
+      const int slot = rg * (KS - 1) + kp - 1;
+#pragma unroll
+      for (int r = 0; r < TR; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) red[slot][16 * r + 4 * q + i][lane] = acc[r][q][i];
+    }
+    __syncthreads();
+    if (kp > 0) return;
+  }
+#pragma unroll
+  for (int r = 0; r < TR; ++r) {
+    if (r >= nt) break;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int e = 16 * r + 4 * q + i;
+        double v = (double)acc[r][q][i];
+        if constexpr (KS > 1) {
+#pragma unroll
+          for (int j = 1; j < KS; ++j) v += (double)red[rg * (KS - 1) + j - 1][e][lane];
+        }
+        const int rr = r0 + 16 * r + M::row_of(g, i);
+        if (rr < rows) YF[(int64_t)(l0 + rr) * kR + 16 * q + c] = v;
+      }
+  }
+}
+
+// Column ranges of the FP32 leaf launch (film, 64 right-hand sides, cache
+// evicted): 1 (a 16-row tile per wavefront) 627 us, 4 (all rows per
+// wavefront) 505 us; the one-tile hmm_leaf_kernel<float> took 595 us.
+constexpr int kLeafKS = 4;
 
 // One workgroup (four wavefronts) per leaf of at most 64 rows; wavefront w
 // covers rows 16 w + (0..15).  The leaf's Z index list is staged in LDS,
@@ -466,7 +565,10 @@ int run_hmm(const smx_hmatrix* H, const smx_hmat_near_tiles* NT, int64_t nz, con
     SMX_LAUNCH_CHECK();
   }
   if (H->nleaf > 0) {
-    hipLaunchKernelGGL(hmm_leaf_kernel<T>, dim3((unsigned)H->nleaf), dim3(256), 0, st, *H, Z, YF);
+    if constexpr (sizeof(T) == 4)
+      hipLaunchKernelGGL(hmm_leaf_ks_kernel<kLeafKS>, dim3((unsigned)H->nleaf), dim3(256), 0, st, *H, Z, YF);
+    else
+      hipLaunchKernelGGL(hmm_leaf_kernel<T>, dim3((unsigned)H->nleaf), dim3(256), 0, st, *H, Z, YF);
     SMX_LAUNCH_CHECK();
   }
   if (NT) {

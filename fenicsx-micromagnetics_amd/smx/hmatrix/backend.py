@@ -145,9 +145,13 @@ def pack_operator(nb, row_start, row_stop, col_start, col_stop, rank, Ufac, Vfac
         return zt, int(order.size)
 
     zt, nzt = tasks(_ZG, xcd=xcd_order)
-    # 16-row tasks for the multi-RHS product: one MFMA tile of rank rows each
-    zt16, nzt16 = tasks(16)
-    P.update(V=V, ztask=zt.view(np.int64), nztask=nzt, ztask16=zt16.view(np.int64), nztask16=nzt16)
+    # tasks of the multi-RHS product: one 16-row MFMA tile of rank rows each
+    # (smx_hmat_matmat's per-task limit; two or four tiles per task, sharing
+    # the x loads, measured slower on the film: 558 -> 633 us for four)
+    mm_rows = MM_TASK_ROWS
+    ztm, nztm = tasks(mm_rows)
+    P.update(V=V, ztask=zt.view(np.int64), nztask=nzt, ztask_mm=ztm.view(np.int64), nztask_mm=nztm,
+             ztask_mm_rows=mm_rows)
     # ---- leaves ----
     starts, stops = leaf_intervals(nb, r0, r1)
     nleaf = starts.size
@@ -240,6 +244,8 @@ def pack_operator(nb, row_start, row_stop, col_start, col_stop, rank, Ufac, Vfac
                          f"{_capi.HMAT_KW_MAX} in LDS")
     return P
 
+
+MM_TASK_ROWS = 16   # rank rows per multi-RHS z task (SMX_HMAT_MM_ROWS)
 
 NTASK_DTYPE = np.dtype([("s_off", np.int64), ("p0", np.int32), ("p1", np.int32), ("w", np.int32),
                         ("pad", np.int32)])   # smx_hmat_ntask, 24 bytes
@@ -632,10 +638,11 @@ class HMatrixGPUPackedFused:
         _capi.require_device(out, "out", numel=X.numel())
         nz = int(self._nz)
         if getattr(self, "_Hmm", None) is None:
-            # the multi-RHS product walks the 16-row task table (one MFMA tile each)
+            # the multi-RHS product walks its own task table (one MFMA row
+            # tile of rank rows per task)
             Hm = self._struct(self.fwd, self.gidx_plain)
-            Hm.ztask = _capi.ptr(self.fwd["ztask16"])
-            Hm.nztask = int(self.fwd["nztask16"])
+            Hm.ztask = _capi.ptr(self.fwd["ztask_mm"])
+            Hm.nztask = int(self.fwd["nztask_mm"])
             Hm.ynear = None
             self._Hmm = Hm
         need = int(_capi.lib().smx_hmat_matmat_ws_size(C.byref(self._Hmm), nz))

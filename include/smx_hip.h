@@ -385,7 +385,12 @@ typedef struct {
  * tiles over NT's dense near tiles, else one wavefront per CSR row.  No counterpart call in the reference, whose matvec takes one
  * vector (backend_fused.py:496-512): it replaces a loop of matvecs.  nz: the
  * packed operator's z entries (the largest z_off + kc); work: at least
- * smx_hmat_matmat_ws_size(H, nz) bytes. */
+ * smx_hmat_matmat_ws_size(H, nz) bytes.  H's z tasks here hold up to
+ * SMX_HMAT_MM_ROWS rank rows (kc; one MFMA row tile: rows beyond it are not
+ * computed).  With FP32 factors a leaf's four wavefronts split its columns
+ * in four ranges, each covering all of the leaf's rows, and their partial
+ * tiles are added in FP64 in wavefront order. */
+#define SMX_HMAT_MM_ROWS 16
 int64_t smx_hmat_matmat_ws_size(const smx_hmatrix* H, int64_t nz);
 int smx_hmat_matmat(const smx_hmatrix* H, const smx_hmat_near_tiles* NT, int64_t nz, const double* X,
                     int64_t ldx, int32_t nrhs, double* Y, int64_t ldy, void* work, int64_t work_bytes,

@@ -492,10 +492,11 @@ def test_sixteen_row_task_table(c1):
         return z
 
     z8 = zvec(pk["ztask"].view(ZTASK_DTYPE)[:pk["nztask"]], 8)
-    z16 = zvec(pk["ztask16"].view(ZTASK_DTYPE)[:pk["nztask16"]], 16)
+    assert pk["ztask_mm_rows"] == 16                # one MFMA row tile per task
+    zmm = zvec(pk["ztask_mm"].view(ZTASK_DTYPE)[:pk["nztask_mm"]], 16)
     live8 = int(np.count_nonzero(pk["ztask"].view(ZTASK_DTYPE)[:pk["nztask"]]["kc"]))
-    assert pk["nztask16"] < live8
-    assert np.abs(z8 - z16).max() <= 1e-12
+    assert pk["nztask_mm"] < live8
+    assert np.abs(z8 - zmm).max() <= 1e-12
 
 
 @pytest.mark.parametrize("leaf", [16, 32])
