@@ -147,9 +147,12 @@ def pack_operator(nb, row_start, row_stop, col_start, col_stop, rank, Ufac, Vfac
     zt, nzt = tasks(_ZG, xcd=xcd_order)
     # tasks of the multi-RHS product: one 16-row MFMA tile of rank rows each
     # (smx_hmat_matmat's per-task limit; two or four tiles per task, sharing
-    # the x loads, measured slower on the film: 558 -> 633 us for four)
+    # the x loads, measured slower on the film: 558 -> 633 us for four), in
+    # per-XCD column groups for either factor type (each L2 then holds the
+    # staged X rows of one eighth of the columns: film 64-RHS product
+    # 1377 -> 1339 us FP32, 2062 -> 1968-2007 us FP64)
     mm_rows = MM_TASK_ROWS
-    ztm, nztm = tasks(mm_rows)
+    ztm, nztm = tasks(mm_rows, xcd=True)
     P.update(V=V, ztask=zt.view(np.int64), nztask=nzt, ztask_mm=ztm.view(np.int64), nztask_mm=nztm,
              ztask_mm_rows=mm_rows)
     # ---- leaves ----
