@@ -431,11 +431,11 @@ class HMatrixGPUPackedFused:
         # one gather index per entry (leaf_zg); the same values either way
         self.leaf_z_runs = True
         # xcd_order: z tasks and leaves in per-XCD position groups
-        # (pack_operator); False: one list each, largest first.  None: with
-        # FP32 factors (film matvec 310 -> 300 us, 64-RHS product 1454 ->
-        # 1424 us); FP64 factors keep the size order (matvec unchanged,
-        # 64-RHS product 2028 -> 2070 us with the groups).
-        self.xcd_order = bool(use_fp32) if xcd_order is None else bool(xcd_order)
+        # (pack_operator); False: one list each, largest first.  None: on
+        # (film matvec 310 -> 300 us with FP32 factors, 533 -> 530 us with
+        # FP64; the 64-RHS product unchanged with FP64 factors, 1454 -> 1424
+        # us with FP32)
+        self.xcd_order = True if xcd_order is None else bool(xcd_order)
         # near_in_z: the near rows are summed by a launch of their own (one
         # wavefront per row) instead of inside the leaf launch.  It pays when
         # the leaves are too few to fill the device (the 150 nm cube has 84);
