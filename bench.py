@@ -699,7 +699,7 @@ def multi_rhs_timing(d, k=64, flush_mb=512):
             "TFLOPs": 2.0 * n * n * k / t / 1e12,
             "rel_err_vs_fp64": float(torch.linalg.norm(Y - ref) / torch.linalg.norm(ref))}
         del B, work
-    return {"nrhs": k, "kernel": "mm_tile_kernel (v_mfma_f64_16x16x4 / f32_16x16x4 / f32_16x16x32_bf16)",
+    return {"nrhs": k, "kernel": "mm_tile_lds_kernel (X slice staged in LDS; v_mfma_f64_16x16x4 / f32_16x16x4 / f32_16x16x32_bf16)",
             "note": "times include the X staging and split-K reduction launches", **out}
 
 
