@@ -75,7 +75,10 @@ __global__ void gather_shift_kernel(int64_t n, const double* __restrict__ src,
 // contiguous quarters, so each workgroup streams R*n elements with every lane
 // issuing R independent 16-byte loads per step (plus one shared x fragment),
 // and n/R workgroups keep all CUs busy even for a few thousand rows.
-template <typename T, int R>
+// UN: 16-byte vectors per row and lane in flight per step (cube 150 nm,
+// cache evicted: FP32 2 / 3 / 4 -> 27.6 / 33.8 / 28.9 us, FP64 44.5 / 44.1 /
+// 44.5 us, BF16 20.8 / 23.5 / 18.4 us; the same bits)
+template <typename T, int R, int UN>
 __global__ __launch_bounds__(kWavesPerBlock * 64) void gemv_rows_kernel(
     const T* __restrict__ B, int64_t n, int64_t ldb, const double* __restrict__ x,
     double* __restrict__ y) {
@@ -98,15 +101,14 @@ __global__ __launch_bounds__(kWavesPerBlock * 64) void gemv_rows_kernel(
     rowp[q] = B + rr * ldb;
   }
   int64_t v = v0 + lane;
-  for (; v + 64 < v1; v += 128) {
-    Raw16 raw[2][R];
+  for (; v + 64 * (UN - 1) < v1; v += 64 * UN) {
+    Raw16 raw[UN][R];
 #pragma unroll
-    for (int q = 0; q < R; ++q) {
-      raw[0][q] = *reinterpret_cast<const Raw16*>(rowp[q] + v * V);
-      raw[1][q] = *reinterpret_cast<const Raw16*>(rowp[q] + (v + 64) * V);
-    }
+    for (int h = 0; h < UN; ++h)
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+      for (int q = 0; q < R; ++q) raw[h][q] = *reinterpret_cast<const Raw16*>(rowp[q] + (v + 64 * h) * V);
+#pragma unroll
+    for (int h = 0; h < UN; ++h) {
       double xv[V];
       const double* xp = x + (v + 64 * h) * V;
 #pragma unroll
@@ -180,7 +182,7 @@ __global__ void gemv_t_reduce_kernel(int64_t n, int64_t nchunk, const double* __
   y[j] = s;
 }
 
-template <typename T, int R>
+template <typename T, int R, int UN = 2>
 int launch_gemv(const void* B, int64_t n, int64_t ldb, const double* x, double* y,
                 hipStream_t st) {
   constexpr int V = Vec16<T>::N;
@@ -188,7 +190,7 @@ int launch_gemv(const void* B, int64_t n, int64_t ldb, const double* x, double* 
   SMX_REQUIRE((reinterpret_cast<uintptr_t>(B) & 15) == 0, SMX_ERR_VALUE,
               "B must be 16-byte aligned");
   const int grid = smx::grid_for(n, R);
-  hipLaunchKernelGGL((gemv_rows_kernel<T, R>), dim3(grid), dim3(kWavesPerBlock * 64), 0, st,
+  hipLaunchKernelGGL((gemv_rows_kernel<T, R, UN>), dim3(grid), dim3(kWavesPerBlock * 64), 0, st,
                      reinterpret_cast<const T*>(B), n, ldb, x, y);
   SMX_LAUNCH_CHECK();
   return SMX_OK;
@@ -219,7 +221,7 @@ extern "C" int smx_dense_bmatvec(const void* B, int32_t dtype, int64_t n, int64_
       // 4 rows per workgroup: 150 nm cube (5402 rows), cache evicted, 30.2 -> 28.0 us
       return launch_gemv<float, 4>(B, n, ldb, x, y, st);
     case SMX_BF16:
-      return launch_gemv<uint16_t, 4>(B, n, ldb, x, y, st);
+      return launch_gemv<uint16_t, 4, 4>(B, n, ldb, x, y, st);
     default:
       smx::set_error("unsupported dtype %d", dtype);
       return SMX_ERR_TYPE;
