@@ -422,7 +422,8 @@ __global__ __launch_bounds__(256, 8) void hm_far_z_kernel(smx_hmatrix H) {
   const smx_hmat_ztask task = H.ztask[t];
   const int32_t n = task.n;
   const int kc = task.kc;
-  const int64_t ld = ((int64_t)n + 3) & ~(int64_t)3;
+  constexpr int64_t kA = sizeof(T) == 4 ? SMX_HMAT_V_ALIGN_F32 : SMX_HMAT_V_ALIGN_F64;  // V row padding
+  const int64_t ld = ((int64_t)n + kA - 1) & ~(kA - 1);
   const T* __restrict__ Vt = reinterpret_cast<const T*>(H.V) + task.v_off;
   const double* __restrict__ x = H.xp + task.c0;
   double acc[kZG];

@@ -159,7 +159,8 @@ __global__ __launch_bounds__(256) void hmm_z_kernel(smx_hmatrix H, const T* __re
   const smx_hmat_ztask task = H.ztask[t];
   const int64_t n = task.n;
   const int kc = task.kc;
-  const int64_t ld = (n + 3) & ~(int64_t)3;
+  constexpr int64_t kA = sizeof(T) == 4 ? SMX_HMAT_V_ALIGN_F32 : SMX_HMAT_V_ALIGN_F64;  // V row padding
+  const int64_t ld = (n + kA - 1) & ~(kA - 1);
   const bool row_ok = c < kc;
   const T* __restrict__ Vr = reinterpret_cast<const T*>(H.V) + task.v_off + (int64_t)(row_ok ? c : 0) * ld;
   const T* __restrict__ xs = XR + (int64_t)task.c0 * kR + 4 * c;  // lane c's columns of the x segment

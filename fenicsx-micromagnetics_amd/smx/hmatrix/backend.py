@@ -37,6 +37,20 @@ from .storage import HMatrixCPUData
 _ZG = _capi.HMAT_Z_GROUP
 
 
+# V row padding in entries by factor size (SMX_HMAT_V_ALIGN_F32 / _F64): FP32
+# rows start on 128-byte cache lines (film z launch 710.7 -> 697.8 MB
+# fetched, matvec 286.2 -> 284.3 us); FP64 rows keep 4 entries (128-byte rows
+# made the FP64 matvec slower, 532.3 -> 536.4 us)
+V_ROW_ENTRIES = {4: 32, 8: 4}
+# near row padding in entries by value size: rows of values start on 128-byte
+# lines (the pairs the kernels read need an even count)
+NEAR_ROW_ENTRIES = {8: 16, 4: 32}
+
+
+def _round_to(a, m):
+    return (np.asarray(a, dtype=np.int64) + m - 1) // m * m
+
+
 def _round4(a):
     return (np.asarray(a, dtype=np.int64) + 3) // 4 * 4
 
@@ -99,7 +113,7 @@ def pack_operator(nb, row_start, row_stop, col_start, col_stop, rank, Ufac, Vfac
     nv = np.dtype(near_dt).itemsize
     P = {}
     # ---- z stage: V rows padded to 4 entries, tasks of <= _ZG rank rows ----
-    ld = _round4(n)
+    ld = _round_to(n, V_ROW_ENTRIES[fb])
     v_off = np.zeros(nfar, dtype=np.int64)
     if nfar:
         v_off[1:] = np.cumsum(rk * ld)[:-1]
@@ -176,7 +190,7 @@ def pack_operator(nb, row_start, row_stop, col_start, col_stop, rank, Ufac, Vfac
     kpos = cum - base[leaf]                      # first column of each pair in its leaf matrix
     K = np.zeros(nleaf, dtype=np.int64)
     np.add.at(K, leaf, rke)
-    kw = _round4(K)
+    kw = _round_to(K, V_ROW_ENTRIES[fb])        # U rows padded as V rows
     cl = stops - starts
     near = near_csr.tocsr()
     near.sort_indices()
@@ -224,9 +238,9 @@ def pack_operator(nb, row_start, row_stop, col_start, col_stop, rank, Ufac, Vfac
              leaf_zg=(zg if zg.size else np.zeros(1, np.int64)).astype(np.int32), U=U,
              leaf_zr_off=zr_off, leaf_zr=(zr if zr.size else np.zeros(2, np.int32)),
              kw_max=int(kw.max()) if nleaf else 0)
-    # ---- near rows in aligned pairs ----
+    # ---- near rows padded to whole 128-byte lines of values ----
     cnt = np.diff(nptr0)
-    pc = cnt + (cnt & 1)
+    pc = _round_to(cnt, NEAR_ROW_ENTRIES[nv])
     ptr = np.zeros(nb + 1, dtype=np.int64)
     ptr[1:] = np.cumsum(pc)
     idx = np.repeat(np.arange(nb, dtype=np.int64), pc)   # padding: a valid column with value 0

@@ -259,8 +259,9 @@ int smx_rsvd_round(const double* A, const smx_svd_task* tasks, int64_t ntasks, c
  * spreads the near field over the whole device when the leaves are few.
  * Layout (every factor load is 16 bytes and aligned):
  *  - a z task covers kc <= SMX_HMAT_Z_GROUP rank rows of one far block's V:
- *    row g at V + v_off + g*ld, ld = n rounded up to 4 entries (zero padded),
- *    v_off a multiple of 4;
+ *    row g at V + v_off + g*ld, ld = n rounded up to SMX_HMAT_V_ALIGN_F32
+ *    entries for FP32 factors (rows on 128-byte cache lines) or
+ *    SMX_HMAT_V_ALIGN_F64 for FP64 (zero padded), v_off a multiple of it;
  *  - a leaf L (rows leaf_start..leaf_stop, at most 64) owns a dense row-major
  *    matrix of its far rows: row i at U + leaf_u_off[L] + i*leaf_kw[L],
  *    leaf_kw a multiple of 4, zero padded.  Its column k multiplies
@@ -274,6 +275,8 @@ int smx_rsvd_round(const double* A, const smx_svd_task* tasks, int64_t ntasks, c
  * FP64 accumulation.
  * ---------------------------------------------------------------------- */
 #define SMX_HMAT_Z_GROUP 8
+#define SMX_HMAT_V_ALIGN_F32 32
+#define SMX_HMAT_V_ALIGN_F64 4
 #define SMX_HMAT_KW_MAX 16384
 #define SMX_HMAT_NEAR_W_MAX 8192
 

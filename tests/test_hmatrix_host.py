@@ -237,8 +237,9 @@ def _emulate_packed(pk, nb, perm, diag_p, x):
             assert n == 0
             continue
         assert 1 <= kc <= zG and c0 >= 0 and c0 + n <= nb
-        ld = (n + 3) // 4 * 4
-        assert vo % 4 == 0 and vo + kc * ld <= V.size and zo + kc <= z.size
+        al = {4: 32, 8: 4}[pk["V"].dtype.itemsize]  # row padding: 128 bytes FP32, 4 entries FP64
+        ld = (n + al - 1) // al * al
+        assert vo % al == 0 and vo + kc * ld <= V.size and zo + kc <= z.size
         Vt = V[vo:vo + kc * ld].reshape(kc, ld)
         assert np.all(Vt[:, n:] == 0)
         z[zo:zo + kc] = Vt[:, :n] @ xp[c0:c0 + n]
@@ -258,7 +259,7 @@ def _emulate_packed(pk, nb, perm, diag_p, x):
         assert kw <= pk["kw_max"]
         covered[l0:l1] += 1
         K = int(zgo[L + 1] - zgo[L])
-        assert 0 <= kw - K < 4
+        assert 0 <= kw - K < {4: 32, 8: 4}[pk["U"].dtype.itemsize]    # rows padded as V's
         idx = zg[zgo[L]:zgo[L + 1]]
         assert np.all((idx >= 0) & (idx < pk["nz"]))
         # the run form of the same list: contiguous z ranges, first columns increasing
@@ -359,7 +360,9 @@ def test_pack_all_near(c1):
 
 
 def test_pack_fp32_near_values(c1):
-    """FP32 near values: same tables, entries rounded once."""
+    """FP32 near values: every row's entries and columns as with FP64 values,
+    rounded once; rows padded (value 0, the row's own column) to 128-byte
+    lines of values: 32 entries for FP32, 16 for FP64."""
     from smx.hmatrix.backend import pack_operator
     _, geo, bd = c1
     data = build_hmatrix_device(geo, HMatrixBuildConfig(leaf_size=32), ops=HostOps(geo, bd))
@@ -368,10 +371,18 @@ def test_pack_fp32_near_values(c1):
                        [b.col_stop for b in fb], [b.rank for b in fb], [b.U for b in fb], [b.V for b in fb],
                        data.near_csr, np.float64, np.float32)
     assert pk["near_val"].dtype == np.float32
-    nz = pk["near_val"] != 0
     ref = _pack(data)
-    assert np.array_equal(pk["near_idx"], ref["near_idx"])
-    assert np.array_equal(pk["near_val"][nz], ref["near_val"][nz].astype(np.float32))
+    cnt = np.diff(data.near_csr.indptr)
+    for q, (P, R) in enumerate(((pk, 32), (ref, 16))):
+        ptr = P["near_ptr"]
+        assert np.all(ptr % R == 0) and np.all(np.diff(ptr) - cnt >= 0) and np.all(np.diff(ptr) - cnt < R)
+    for row in range(data.size):
+        a, b = int(pk["near_ptr"][row]), int(ref["near_ptr"][row])
+        k = int(cnt[row])
+        assert np.array_equal(pk["near_idx"][a:a + k], ref["near_idx"][b:b + k])
+        assert np.array_equal(pk["near_val"][a:a + k], ref["near_val"][b:b + k].astype(np.float32))
+        pad = slice(a + k, int(pk["near_ptr"][row + 1]))
+        assert np.all(pk["near_val"][pad] == 0) and np.all(pk["near_idx"][pad] == row)
 
 
 @pytest.mark.parametrize("compressor", ["svd", "rsvd"])
@@ -483,8 +494,9 @@ def test_sixteen_row_task_table(c1):
             if kc == 0:                  # padding of a short XCD group
                 assert n == 0
                 continue
-            ld = (n + 3) // 4 * 4
-            assert 1 <= kc <= cap and vo % 4 == 0 and vo + kc * ld <= V.size and zo + kc <= pk["nz"]
+            al = {4: 32, 8: 4}[pk["V"].dtype.itemsize]  # row padding by factor type
+            ld = (n + al - 1) // al * al
+            assert 1 <= kc <= cap and vo % al == 0 and vo + kc * ld <= V.size and zo + kc <= pk["nz"]
             assert 0 <= c0 and c0 + n <= d.size
             z[zo:zo + kc] = V[vo:vo + kc * ld].reshape(kc, ld)[:, :n] @ xp[c0:c0 + n]
             hits[zo:zo + kc] += 1
