@@ -88,7 +88,8 @@ __global__ __launch_bounds__(kWavesPerBlock * 64) void gemv_rows_kernel(
   const int lane = threadIdx.x & 63;
   const int64_t row0 = (int64_t)blockIdx.x * R;
   const int64_t nvec = n / V;  // full 16-byte vectors per row
-  const int64_t per = (nvec + kWavesPerBlock - 1) / kWavesPerBlock;
+  // each wavefront's column range starts on a 128-byte line (8 vectors)
+  const int64_t per = ((nvec + 8 * kWavesPerBlock - 1) / (8 * kWavesPerBlock)) * 8;
   const int64_t v0 = wid * per;
   const int64_t v1 = (v0 + per < nvec) ? v0 + per : nvec;
   double acc[R];
