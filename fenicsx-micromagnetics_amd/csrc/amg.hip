@@ -700,6 +700,10 @@ int run_amg(int64_t n, int64_t nd, int32_t W, int64_t ld, const int32_t* cols, c
         }
 #undef SMX_AMG_UPD
         SMX_LAUNCH_CHECK();
+        // the last update of a speculative batch can only find convergence (if
+        // it does not, the caller solves again): the V-cycle and A u after it
+        // would never be used
+        if (spec && k == nb - 1) continue;
         const V* rn = vec<V>(P.r[1 - (it & 1)]);
         rc = vcycle_tail<VT, V>(lv, nlev, rn, P, st);
         if (rc) return rc;
